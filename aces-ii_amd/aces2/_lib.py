@@ -1,0 +1,69 @@
+"""ctypes binding of the C-ABI library ``libaces2_amd.so`` (include/aces2_amd.h).
+
+The library is the product path; there is no fallback.  Loading fails loudly
+when it has not been built (``__graft_entry__.build()`` or ``make`` in
+``aces-ii_amd/csrc``).
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libaces2_amd.so')
+
+_lib = None
+
+c_int = ctypes.c_int
+c_long = ctypes.c_long
+c_double = ctypes.c_double
+c_vp = ctypes.c_void_p
+c_dp = ctypes.POINTER(ctypes.c_double)
+c_char_p = ctypes.c_char_p
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    'aces_last_error': (c_char_p, []),
+    'aces_gemm': (c_int, [c_int, c_int, c_int, c_int, c_vp, c_vp, c_long, c_int, c_vp, c_vp, c_long, c_int,
+                          c_vp, c_long, c_double, c_double, c_int, c_long, c_long, c_long, c_vp]),
+    'aces_convert_f32': (c_int, [c_int, c_int, c_vp, c_long, c_vp, c_long, c_vp]),
+    'aces_split_bf16': (c_int, [c_int, c_int, c_vp, c_long, c_vp, c_vp, c_long, c_vp]),
+    'aces_ccsd_create': (c_int, [c_int, c_int, c_vp, ctypes.POINTER(c_vp)]),
+    'aces_ccsd_destroy': (c_int, [c_vp]),
+    'aces_ccsd_setup': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    'aces_ccsd_set_amplitudes': (c_int, [c_vp, c_vp, c_vp]),
+    'aces_ccsd_get_amplitudes': (c_int, [c_vp, c_vp, c_vp]),
+    'aces_ccsd_start_mp1': (c_int, [c_vp]),
+    'aces_ccsd_increments': (c_int, [c_vp, c_int, c_vp, c_vp]),
+    'aces_ccsd_cycle': (c_int, [c_vp, c_int, c_int, c_double, c_dp]),
+    'aces_ccsd_set_diis_order': (c_int, [c_vp, c_int]),
+    'aces_ccsd_reset_diis': (c_int, [c_vp]),
+    'aces_ccsd_energy': (c_int, [c_vp, c_vp, c_vp, c_dp]),
+    'aces_ccsd_device_bytes': (c_long, [c_vp]),
+}
+
+
+class AcesError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AcesError('libaces2_amd.so is not built (%s); run __graft_entry__.build()' % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, what=''):
+    if status != 0:
+        msg = lib().aces_last_error()
+        raise AcesError('%s failed (status %d): %s' % (what, status, msg.decode() if msg else ''))
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args), name)

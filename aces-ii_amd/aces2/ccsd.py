@@ -1,0 +1,124 @@
+"""Host driver of the closed-shell CCSD amplitude path (the xvcc CC loop).
+
+Mirrors the reference's loop ``vcc/vcc.F:798-1437`` on top of the C-ABI
+engine (``include/aces2_amd.h``): the MP1 start written by xintprc, one
+``aces_ccsd_cycle`` per iteration (increments, NEWT2, CMPENG, DODIIS0,
+DRMOVE), convergence on |RESMITER| < 10**-CC_CONV, non-convergence raising
+(the reference exits with status 1, ``vcc/vcc.F:1439-1441``).
+
+Device memory is torch-allocated plumbing; no torch op runs on the path.
+"""
+import ctypes
+
+from . import _lib
+
+PREC = {'fp64': 0, 'fp32': 1, 'bf16x3': 2}
+
+
+class NotConverged(RuntimeError):
+    """The Coupled-Cluster equations did not converge (``vcc/vcc.F:1439``)."""
+
+
+class CCSD:
+    def __init__(self, o, v, blocks, stream=None):
+        import torch
+        self.o, self.v = o, v
+        self.torch = torch
+        self.stream = stream
+        h = ctypes.c_void_p()
+        _lib.call('aces_ccsd_create', o, v, stream, ctypes.byref(h))
+        self.h = h
+        dev = {k: torch.as_tensor(x, device='cuda', dtype=torch.float64).contiguous() for k, x in blocks.items()}
+        _lib.call('aces_ccsd_setup', self.h, dev['fock'].data_ptr(), dev['ovov'].data_ptr(), dev['oovv'].data_ptr(),
+                  dev['ovoo'].data_ptr(), dev['oooo'].data_ptr(), dev['ovvv'].data_ptr(), dev['list233'].data_ptr())
+        del dev
+
+    def close(self):
+        if self.h:
+            _lib.lib().aces_ccsd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_bytes(self):
+        return _lib.lib().aces_ccsd_device_bytes(self.h)
+
+    def set_amplitudes(self, t1, t2):
+        torch = self.torch
+        t1 = torch.as_tensor(t1, device='cuda', dtype=torch.float64).contiguous()
+        t2 = torch.as_tensor(t2, device='cuda', dtype=torch.float64).contiguous()
+        _lib.call('aces_ccsd_set_amplitudes', self.h, t1.data_ptr(), t2.data_ptr())
+
+    def get_amplitudes(self):
+        torch = self.torch
+        o, v = self.o, self.v
+        t1 = torch.empty((o, v), device='cuda', dtype=torch.float64)
+        t2 = torch.empty((o, o, v, v), device='cuda', dtype=torch.float64)
+        _lib.call('aces_ccsd_get_amplitudes', self.h, t1.data_ptr(), t2.data_ptr())
+        return t1, t2
+
+    def start_mp1(self):
+        _lib.call('aces_ccsd_start_mp1', self.h)
+
+    def increments(self, prec='fp64'):
+        torch = self.torch
+        o, v = self.o, self.v
+        i1 = torch.empty((o, v), device='cuda', dtype=torch.float64)
+        i2 = torch.empty((o, o, v, v), device='cuda', dtype=torch.float64)
+        _lib.call('aces_ccsd_increments', self.h, PREC[prec], i1.data_ptr(), i2.data_ptr())
+        return i1, i2
+
+    def energy(self, t1, t2):
+        torch = self.torch
+        t1 = torch.as_tensor(t1, device='cuda', dtype=torch.float64).contiguous()
+        t2 = torch.as_tensor(t2, device='cuda', dtype=torch.float64).contiguous()
+        e = ctypes.c_double()
+        _lib.call('aces_ccsd_energy', self.h, t1.data_ptr(), t2.data_ptr(), ctypes.byref(e))
+        return e.value
+
+    def set_diis_order(self, order):
+        _lib.call('aces_ccsd_set_diis_order', self.h, order)
+
+    def reset_diis(self):
+        _lib.call('aces_ccsd_reset_diis', self.h)
+
+    def cycle(self, icycle, prec='fp64', tol=1e-7):
+        st = (ctypes.c_double * 8)()
+        _lib.call('aces_ccsd_cycle', self.h, icycle, PREC[prec], tol, st)
+        return {'energy': st[0], 'resmiter': st[1], 'resmdiis': st[2], 'converged': bool(st[3]),
+                'final_energy': st[4], 'ms_increments': st[5], 'ms_cycle': st[6]}
+
+    def run(self, maxcyc=50, conv=7, order=6, start='mp1', prec='fp64', log=None, raise_on_fail=True):
+        """The reference's CC loop; returns the per-cycle history."""
+        torch = self.torch
+        self.set_diis_order(order)
+        self.reset_diis()
+        if start == 'mp1':
+            self.start_mp1()
+        elif start == 'zero':
+            self.set_amplitudes(torch.zeros(self.o, self.v), torch.zeros(self.o, self.o, self.v, self.v))
+        t1, t2 = self.get_amplitudes()
+        hist = {'energy': [self.energy(t1, t2)], 'resmiter': [], 'resmdiis': [], 'ms_cycle': [],
+                'converged': False}
+        tol = 10.0 ** (-conv)
+        for icycle in range(1, maxcyc + 1):
+            st = self.cycle(icycle, prec, tol)
+            hist['energy'].append(st['energy'])
+            hist['resmiter'].append(st['resmiter'])
+            hist['resmdiis'].append(st['resmdiis'])
+            hist['ms_cycle'].append(st['ms_cycle'])
+            if log:
+                log(icycle, st)
+            if st['converged']:
+                hist['converged'] = True
+                hist['final_energy'] = st['final_energy']
+                hist['cycles'] = icycle
+                break
+        if not hist['converged'] and raise_on_fail:
+            raise NotConverged('@VCC: The Coupled-Cluster equations did not converge!')
+        return hist

@@ -1,0 +1,98 @@
+// C-ABI entry points of the closed-shell CCSD engine.
+#include <new>
+#include "ccsd_engine.h"
+#include "common.h"
+
+using namespace aces;
+
+static CcsdEngine* eng(void* h) { return reinterpret_cast<CcsdEngine*>(h); }
+
+#define REQUIRE_HANDLE(h)                       \
+  do {                                          \
+    if ((h) == nullptr) {                       \
+      aces::set_error("null engine handle");   \
+      return ACES_ERR_ARG;                \
+    }                                           \
+  } while (0)
+
+extern "C" {
+
+int aces_ccsd_create(int o, int v, void* stream, void** handle) {
+  if (handle == nullptr || o < 1 || v < 1) {
+    set_error("aces_ccsd_create: need o >= 1, v >= 1 and a handle pointer");
+    return ACES_ERR_ARG;
+  }
+  CcsdEngine* e = new (std::nothrow) CcsdEngine(o, v, (hipStream_t)stream);
+  if (!e) {
+    set_error("aces_ccsd_create: out of host memory");
+    return ACES_ERR_NOMEM;
+  }
+  *handle = e;
+  return ACES_OK;
+}
+
+int aces_ccsd_destroy(void* handle) {
+  delete eng(handle);
+  return ACES_OK;
+}
+
+int aces_ccsd_setup(void* h, const double* fock, const double* ovov, const double* oovv, const double* ovoo,
+                    const double* oooo, const double* ovvv, const double* list233) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->setup(fock, ovov, oovv, ovoo, oooo, ovvv, list233);
+}
+
+int aces_ccsd_set_amplitudes(void* h, const double* t1, const double* t2) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->set_amplitudes(t1, t2);
+}
+
+int aces_ccsd_get_amplitudes(void* h, double* t1, double* t2) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->get_amplitudes(t1, t2);
+}
+
+int aces_ccsd_start_mp1(void* h) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->start_mp1();
+}
+
+int aces_ccsd_increments(void* h, int prec, double* inc1, double* inc2) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->increments((GemmPrec)prec, inc1, inc2);
+}
+
+int aces_ccsd_cycle(void* h, int icycle, int prec, double tol, double* stats) {
+  REQUIRE_HANDLE(h);
+  CcsdStats st;
+  const int rc = eng(h)->cycle(icycle, (GemmPrec)prec, tol, st);
+  if (rc == ACES_OK && stats) {
+    stats[0] = st.energy_new;
+    stats[1] = st.resmiter;
+    stats[2] = st.resmdiis;
+    stats[3] = st.converged;
+    stats[4] = st.energy_final;
+    stats[5] = st.ms_increments;
+    stats[6] = st.ms_cycle;
+  }
+  return rc;
+}
+
+int aces_ccsd_set_diis_order(void* h, int order) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->set_diis_order(order);
+}
+
+int aces_ccsd_reset_diis(void* h) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->reset_diis();
+}
+
+int aces_ccsd_energy(void* h, const double* t1, const double* t2, double* e) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->energy_of(t1, t2, *e);
+}
+
+long aces_ccsd_device_bytes(void* h) { return h ? eng(h)->device_bytes() : 0; }
+
+}  // extern "C"

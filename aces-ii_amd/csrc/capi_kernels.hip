@@ -1,0 +1,33 @@
+// C-ABI entry points for the raw kernels (GEMM engine and operand conversion).
+#include "common.h"
+#include "gemm.h"
+#include "tensor_ops.h"
+
+using namespace aces;
+
+extern "C" {
+
+const char* aces_last_error(void) { return aces::last_error(); }
+
+int aces_gemm(int prec, int M, int N, int K, const void* A, const void* A_lo, long lda, int a_kc, const void* B,
+              const void* B_lo, long ldb, int b_kc, double* C, long ldc, double alpha, double beta, int batch,
+              long sA, long sB, long sC, void* stream) {
+  GemmDesc d;
+  d.M = M; d.N = N; d.K = K;
+  d.A = A; d.A_lo = A_lo; d.lda = lda; d.a_kc = a_kc;
+  d.B = B; d.B_lo = B_lo; d.ldb = ldb; d.b_kc = b_kc;
+  d.C = C; d.ldc = ldc; d.alpha = alpha; d.beta = beta;
+  d.batch = batch; d.sA = sA; d.sB = sB; d.sC = sC;
+  return gemm(d, (GemmPrec)prec, (hipStream_t)stream);
+}
+
+int aces_convert_f32(int rows, int cols, const double* x, long ldx, float* y, long ldy, void* stream) {
+  return convert_f32(rows, cols, x, ldx, y, ldy, (hipStream_t)stream);
+}
+
+int aces_split_bf16(int rows, int cols, const double* x, long ldx, unsigned short* hi, unsigned short* lo,
+                    long ldy, void* stream) {
+  return split_bf16(rows, cols, x, ldx, hi, lo, ldy, (hipStream_t)stream);
+}
+
+}  // extern "C"

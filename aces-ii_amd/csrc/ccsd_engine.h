@@ -1,0 +1,99 @@
+// Closed-shell CCSD amplitude engine (RHF, C1) on one gfx950 GPU.
+//
+// One Jacobi/DIIS cycle of the reference's xvcc loop (vcc/vcc.F:798-1437):
+// increments (GENINT + FEACONT + FMICONT + FMECONT + T1INT* + DRE3EN + E4S),
+// NEWT2, CMPENG, DODIIS0, DRMOVE.  Integrals and amplitudes stay resident in
+// HBM; the contractions run on the MFMA GEMM engine (gemm.h), the
+// permutations/packings on HBM-bound kernels, the DIIS solve on the host.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <vector>
+#include "gemm.h"
+
+namespace aces {
+
+struct CcsdStats {
+  double energy_new = 0.0;   // CMPENG(60,2): E(T_new) printed each cycle
+  double resmiter = 0.0;     // largest element of the residual (signed)
+  double resmdiis = 0.0;     // largest element of the DIIS residual
+  int converged = 0;         // |resmiter| < tol
+  double energy_final = 0.0; // CMPENG(43,0) on the T of the converged cycle
+  double ms_increments = 0.0;
+  double ms_cycle = 0.0;
+};
+
+class CcsdEngine {
+ public:
+  CcsdEngine(int o, int v, hipStream_t s);
+  ~CcsdEngine();
+
+  // Integral input (device pointers, chemists' notation, row major):
+  //   fock[n][n]; ovov[i][a][j][b]=(ia|jb); oovv[i][j][a][b]=(ij|ab);
+  //   ovoo[i][a][j][k]=(ia|jk); oooo[i][j][k][l]=(ij|kl); ovvv[i][a][b][c]=(ia|bc);
+  //   L233 = reference list 233 (rows A<=B, columns Cd) holding <ab|cd>=(ac|bd).
+  int setup(const double* fock, const double* ovov, const double* oovv, const double* ovoo, const double* oooo,
+            const double* ovvv, const double* L233);
+  // Amplitudes in/out: t1[i][a], t2[i][j][a][b] (device).
+  int set_amplitudes(const double* t1, const double* t2);
+  int get_amplitudes(double* t1, double* t2);
+  int start_mp1();
+  // The residual increments for the current amplitudes (device outputs).
+  int increments(GemmPrec prec, double* inc1_out, double* inc2_out);
+  // One full cycle (increments, Jacobi, energy, DIIS, move).
+  int cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st);
+  int reset_diis();
+  int set_diis_order(int order);
+  int energy_of(const double* t1, const double* t2, double& e);
+  long device_bytes() const { return bytes_; }
+
+  int o, v, n;
+
+ private:
+  int alloc(double** p, long count);
+  int alloc_raw(void** p, long bytes);
+  int compute_increments(GemmPrec prec);
+  int lowp_static(GemmPrec prec);
+  int gemm_lp(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda, const double* B64,
+              int bslot, long ldb, double* C, long ldc, double alpha, double beta);
+
+  hipStream_t s_;
+  long bytes_ = 0;
+  std::vector<void*> allocs_;
+  bool ready_ = false;
+
+  // static
+  double *fock_ = nullptr, *eo_ = nullptr, *ev_ = nullptr;
+  double *ovov_ = nullptr, *oovv_ = nullptr, *ovoo_ = nullptr, *ovvv_ = nullptr, *ovvv_kacd_ = nullptr;
+  double *M3_ = nullptr, *Lmat_ = nullptr, *Kx_ = nullptr, *W0t_ = nullptr, *Wv0_ = nullptr;
+  double *Lo_ = nullptr, *Lv_ = nullptr, *OVkl_ = nullptr, *ooooP_ = nullptr, *ovooQ_ = nullptr;
+  double *M1_ = nullptr, *M5c_ = nullptr, *M5b_ = nullptr, *M7_ = nullptr, *Lijab_ = nullptr;
+  double *Vp_ = nullptr, *Vm_ = nullptr;
+  long nvp_, nvm_, nop_, nom_;
+  // amplitudes and work
+  double *t1_, *t2_, *t1n_, *t2n_, *inc1_, *inc2_, *tau_;
+  double *U_, *X_, *Ut_, *Xt_, *Wt_, *Wv_, *R1_, *R2_;
+  double *Sp_, *Dp_, *Zp_, *Zm_, *Yb_, *L1_, *tauK_, *P1_, *P2_, *Q12_, *Q2_, *Qb_, *T1Q_, *S3_, *G4A_, *G4_;
+  double *Wo_, *Qo_, *ZH_, *tmpL_;
+  double *Fov_, *Foo_, *Fvv_, *Loo_, *Lvv_, *FT_, *FovT_, *Gk_;
+  double *red_ = nullptr;  // reduction scratch
+  double *scal_ = nullptr; // device scalars
+  // low precision operand slots: 0 Vp 1 Vm 2 Lmat 3 Kx (static); 4 Sp 5 Dp 6 Ut 7 Xt 8 Wt 9 Wv 10 U 11 X
+  struct LowP {
+    void* hi = nullptr;
+    void* lo = nullptr;
+    long ld = 0;
+    long rows = 0;
+  };
+  LowP lp_[12];
+  GemmPrec lp_static_prec_ = GP_F64;
+  // DIIS
+  int order_ = 6;
+  std::vector<double*> sol_, err_;
+  std::vector<double> R_;  // order x order (row major)
+  double* err_cur_ = nullptr;
+  double* newvec_ = nullptr;
+  const double** d_ptrs_ = nullptr;  // device pointer table
+  double* d_coef_ = nullptr;
+};
+
+}  // namespace aces

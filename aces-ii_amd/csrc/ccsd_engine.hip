@@ -1,0 +1,625 @@
+// Closed-shell CCSD amplitude engine (see ccsd_engine.h).
+//
+// Spatial-orbital restatement of the increments ACES II accumulates on its
+// alpha-beta lists (vcc/vcc.F:811-1085).  Chemists' notation (pq|rs);
+// t1[i][a]; t2[i][j][a][b] = t_{Ij}^{Ab}; u = 2 t2 - t2(a<->b).
+//
+// Dominant contractions (each a single MFMA GEMM launch):
+//   ladder   Z(+/-) = [V(+/-) ; G(+/-)] . tau(+/-)       symmetric/antisymmetric
+//            pair packing of <ab|cd>, plus the (kb) rows of (kc|bd) that carry
+//            the T1 dressing of W(abef) (DRLAD/LADAB1 vcc/ladab1.F, T1INW1 vcc/t1inw1.f)
+//   W build  Wt = 1/2 (u - 2 t1 t1) . L,  Wv = -1/2 (t^x + 2 t1 t1) . K
+//            (DWMBEJ vcc/dwmbej.F; list 56 = 2 W(MbEj) - W(MbeJ))
+//   ring     R1 = Wt . u,  R2 = Wv . t^x     (DRRNG vcc/drrng.f, RNGPRD vcc/rngprd.f)
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include "ccsd_engine.h"
+#include "ccsd_kernels.h"
+#include "common.h"
+#include "tensor_ops.h"
+
+namespace aces {
+
+namespace {
+struct P4 {
+  Permute4 p;
+  int ns = 0;
+  P4(long a, long b, long c, long d, double* out, double beta = 0.0) {
+    p.d[0] = a; p.d[1] = b; p.d[2] = c; p.d[3] = d;
+    p.out = out;
+    p.beta = beta;
+  }
+  P4& src(const double* x, double al, long s0, long s1, long s2, long s3) {
+    p.in[ns] = x;
+    p.alpha[ns] = al;
+    p.st[ns][0] = s0; p.st[ns][1] = s1; p.st[ns][2] = s2; p.st[ns][3] = s3;
+    ++ns;
+    return *this;
+  }
+  P4& prod(const double* X, long x0, long x1, long x2, long x3, const double* Y, long y0, long y1, long y2, long y3,
+           double g) {
+    p.X = X; p.Y = Y; p.gamma = g;
+    p.xs[0] = x0; p.xs[1] = x1; p.xs[2] = x2; p.xs[3] = x3;
+    p.ys[0] = y0; p.ys[1] = y1; p.ys[2] = y2; p.ys[3] = y3;
+    return *this;
+  }
+  int run(hipStream_t s) { return permute4(p, s); }
+};
+
+GemmDesc gd(int M, int N, int K, const double* A, long lda, int akc, const double* B, long ldb, int bkc, double* C,
+            long ldc, double alpha, double beta) {
+  GemmDesc d;
+  d.M = M; d.N = N; d.K = K;
+  d.A = A; d.lda = lda; d.a_kc = akc;
+  d.B = B; d.ldb = ldb; d.b_kc = bkc;
+  d.C = C; d.ldc = ldc; d.alpha = alpha; d.beta = beta;
+  return d;
+}
+
+inline long pad8(long x) { return (x + 7) / 8 * 8; }
+
+// LU with partial pivoting (dgesv semantics) for the small DIIS system.
+int solve_small(int n, std::vector<double> A, std::vector<double>& b) {
+  std::vector<int> piv(n);
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = std::fabs(A[k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(A[i * n + k]) > best) {
+        best = std::fabs(A[i * n + k]);
+        p = i;
+      }
+    if (best == 0.0) return ACES_ERR_SOLVE;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) std::swap(A[k * n + j], A[p * n + j]);
+      std::swap(b[k], b[p]);
+    }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = A[i * n + k] / A[k * n + k];
+      A[i * n + k] = f;
+      for (int j = k + 1; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < n; ++j) s -= A[i * n + j] * b[j];
+    b[i] = s / A[i * n + i];
+  }
+  return ACES_OK;
+}
+}  // namespace
+
+CcsdEngine::CcsdEngine(int o_, int v_, hipStream_t s) : o(o_), v(v_), n(o_ + v_), s_(s) {
+  nvp_ = (long)v * (v + 1) / 2;
+  nvm_ = (long)v * (v - 1) / 2;
+  nop_ = (long)o * (o + 1) / 2;
+  nom_ = (long)o * (o - 1) / 2;
+}
+
+CcsdEngine::~CcsdEngine() {
+  for (void* p : allocs_) (void)hipFree(p);
+}
+
+int CcsdEngine::alloc_raw(void** p, long bytes) {
+  if (bytes <= 0) bytes = 8;
+  hipError_t e = hipMalloc(p, (size_t)bytes);
+  if (e != hipSuccess) {
+    char b[256];
+    snprintf(b, sizeof(b), "hipMalloc of %ld bytes failed (%s); engine holds %ld bytes", bytes,
+             hipGetErrorString(e), bytes_);
+    set_error(b);
+    return ACES_ERR_NOMEM;
+  }
+  allocs_.push_back(*p);
+  bytes_ += bytes;
+  return ACES_OK;
+}
+
+int CcsdEngine::alloc(double** p, long count) { return alloc_raw((void**)p, count * (long)sizeof(double)); }
+
+int CcsdEngine::setup(const double* fock, const double* ovov, const double* oovv, const double* ovoo,
+                      const double* oooo, const double* ovvv, const double* L233) {
+  ACES_REQUIRE(o >= 1 && v >= 1, "ccsd setup: need o >= 1 and v >= 1");
+  ACES_REQUIRE(!ready_, "ccsd setup: already set up");
+  const long ov = (long)o * v, oo = (long)o * o, vv = (long)v * v;
+  const long ov2 = ov * v, o2v = oo * v, o3 = oo * o, ovvo = ov * ov;
+  hipStream_t s = s_;
+
+  ACES_CHECK(alloc(&fock_, (long)n * n));
+  ACES_HIP(hipMemcpyAsync(fock_, fock, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s));
+  ACES_CHECK(alloc(&eo_, o));
+  ACES_CHECK(alloc(&ev_, v));
+  ACES_CHECK(P4(1, 1, 1, o, eo_).src(fock_, 1, 0, 0, 0, (long)n + 1).run(s));
+  ACES_CHECK(P4(1, 1, 1, v, ev_).src(fock_ + (long)o * n + o, 1, 0, 0, 0, (long)n + 1).run(s));
+
+  ACES_CHECK(alloc(&ovov_, ovvo));
+  ACES_CHECK(alloc(&oovv_, ovvo));
+  ACES_CHECK(alloc(&ovoo_, o3 * v));
+  ACES_CHECK(alloc(&ovvv_, ov * vv));
+  ACES_HIP(hipMemcpyAsync(ovov_, ovov, sizeof(double) * ovvo, hipMemcpyDeviceToDevice, s));
+  ACES_HIP(hipMemcpyAsync(oovv_, oovv, sizeof(double) * ovvo, hipMemcpyDeviceToDevice, s));
+  ACES_HIP(hipMemcpyAsync(ovoo_, ovoo, sizeof(double) * o3 * v, hipMemcpyDeviceToDevice, s));
+  ACES_HIP(hipMemcpyAsync(ovvv_, ovvv, sizeof(double) * ov * vv, hipMemcpyDeviceToDevice, s));
+
+  // strides of the native blocks
+  const long OV[4] = {ov2, ov, v, 1};          // ovov[i][a][j][b]
+  const long OO4[4] = {o3, oo, o, 1};          // oooo
+  (void)OV;
+
+  ACES_CHECK(alloc(&Lmat_, ovvo));  // [(k,c)][(l,d)] = 2(kc|ld) - (kd|lc)
+  ACES_CHECK(P4(o, v, o, v, Lmat_).src(ovov_, 2, ov2, ov, v, 1).src(ovov_, -1, ov2, 1, v, ov).run(s));
+  ACES_CHECK(alloc(&Kx_, ovvo));  // (kd|lc)
+  ACES_CHECK(P4(o, v, o, v, Kx_).src(ovov_, 1, ov2, 1, v, ov).run(s));
+  ACES_CHECK(alloc(&W0t_, ovvo));  // [(i,a)][(k,c)] = 2(ia|kc) - (ik|ac)
+  ACES_CHECK(P4(o, v, o, v, W0t_).src(ovov_, 2, ov2, ov, v, 1).src(oovv_, -1, ov2, v, vv, 1).run(s));
+  ACES_CHECK(alloc(&Wv0_, ovvo));  // (ki|ac)
+  ACES_CHECK(P4(o, v, o, v, Wv0_).src(oovv_, 1, vv, v, ov2, 1).run(s));
+  ACES_CHECK(alloc(&Lo_, ovvo));  // [k][l][c][d] = L_kcld
+  ACES_CHECK(P4(o, o, v, v, Lo_).src(ovov_, 2, ov2, v, ov, 1).src(ovov_, -1, ov2, v, 1, ov).run(s));
+  ACES_CHECK(alloc(&Lv_, ovvo));  // [c][k][l][d] = L_kcld
+  ACES_CHECK(P4(v, o, o, v, Lv_).src(ovov_, 2, ov, ov2, v, 1).src(ovov_, -1, 1, ov2, v, ov).run(s));
+  ACES_CHECK(alloc(&OVkl_, ovvo));  // [k][l][c][d] = (kc|ld)
+  ACES_CHECK(P4(o, o, v, v, OVkl_).src(ovov_, 1, ov2, v, ov, 1).run(s));
+  ACES_CHECK(alloc(&Lijab_, ovvo));  // [i][j][a][b] = 2(ia|jb) - (ib|ja)
+  ACES_CHECK(P4(o, o, v, v, Lijab_).src(ovov_, 2, ov2, v, ov, 1).src(ovov_, -1, ov2, v, 1, ov).run(s));
+  ACES_CHECK(alloc(&ooooP_, oo * oo));  // [k][l][i][j] = (ki|lj)
+  {
+    double* tmp;
+    ACES_CHECK(alloc(&tmp, oo * oo));
+    ACES_HIP(hipMemcpyAsync(tmp, oooo, sizeof(double) * oo * oo, hipMemcpyDeviceToDevice, s));
+    ACES_CHECK(P4(o, o, o, o, ooooP_).src(tmp, 1, OO4[0], OO4[2], OO4[1], OO4[3]).run(s));
+  }
+  const long vo2 = v * oo;
+  ACES_CHECK(alloc(&ovooQ_, o3 * v));  // [k][l][i][c] = (lc|ki) = ovoo[l][c][k][i]
+  ACES_CHECK(P4(o, o, o, v, ovooQ_).src(ovoo_, 1, o, vo2, 1, oo).run(s));
+  ACES_CHECK(alloc(&M1_, o3 * v));  // [(k,i)][(l,c)] = 2(lc|ki) - (kc|li)
+  ACES_CHECK(P4(o, o, o, v, M1_).src(ovoo_, 2, o, 1, vo2, oo).src(ovoo_, -1, vo2, 1, o, oo).run(s));
+  ACES_CHECK(alloc(&M5c_, o3 * v));  // [(k,c,i)][l] = 2(kc|li) - (lc|ki)
+  ACES_CHECK(P4(o, v, o, o, M5c_).src(ovoo_, 2, vo2, oo, 1, o).src(ovoo_, -1, o, oo, 1, vo2).run(s));
+  ACES_CHECK(alloc(&M5b_, o3 * v));  // [(k,c,i)][l] = (lc|ki)
+  ACES_CHECK(P4(o, v, o, o, M5b_).src(ovoo_, 1, o, oo, 1, vo2).run(s));
+  ACES_CHECK(alloc(&M7_, o3 * v));  // [i][k][l][c] = 2(lc|ki) - (kc|li)
+  ACES_CHECK(P4(o, o, o, v, M7_).src(ovoo_, 2, 1, o, vo2, oo).src(ovoo_, -1, 1, vo2, o, oo).run(s));
+
+  const long v3 = vv * v;
+  ACES_CHECK(alloc(&ovvv_kacd_, ov * vv));  // [k][a][c][d] = (kd|ac) = ovvv[k][d][a][c]
+  ACES_CHECK(P4(o, v, v, v, ovvv_kacd_).src(ovvv_, 1, v3, v, 1, vv).run(s));
+  ACES_CHECK(alloc(&M3_, ov * vv));  // [a][k][c][d] = 2(kd|ac) - (kc|ad)
+  ACES_CHECK(P4(v, o, v, v, M3_).src(ovvv_, 2, v, v3, 1, vv).src(ovvv_, -1, v, v3, vv, 1).run(s));
+
+  // ladder operands
+  ACES_CHECK(alloc(&Vp_, (nvp_ + ov) * nvp_));
+  ACES_CHECK(alloc(&Vm_, (nvm_ + ov) * std::max<long>(nvm_, 1)));
+  ACES_CHECK(build_ladder_operands(o, v, L233, ovvv_, Vp_, nvp_, Vm_, std::max<long>(nvm_, 1), s));
+
+  // amplitudes: T = [t1 | t2] contiguous (DIIS vector layout)
+  const long nt = ov + ovvo;
+  ACES_CHECK(alloc(&t1_, nt));
+  t2_ = t1_ + ov;
+  ACES_CHECK(alloc(&t1n_, nt));
+  t2n_ = t1n_ + ov;
+  ACES_CHECK(alloc(&inc1_, nt));
+  inc2_ = inc1_ + ov;
+  ACES_HIP(hipMemsetAsync(t1_, 0, sizeof(double) * nt, s));
+  ACES_CHECK(alloc(&tau_, ovvo));
+  ACES_CHECK(alloc(&U_, ovvo));
+  ACES_CHECK(alloc(&X_, ovvo));
+  ACES_CHECK(alloc(&Ut_, ovvo));
+  ACES_CHECK(alloc(&Xt_, ovvo));
+  ACES_CHECK(alloc(&Wt_, ovvo));
+  ACES_CHECK(alloc(&Wv_, ovvo));
+  ACES_CHECK(alloc(&R1_, ovvo));
+  ACES_CHECK(alloc(&R2_, ovvo));
+  ACES_CHECK(alloc(&Sp_, nop_ * nvp_));
+  ACES_CHECK(alloc(&Dp_, std::max<long>(nom_ * nvm_, 1)));
+  ACES_CHECK(alloc(&Zp_, (nvp_ + ov) * nop_));
+  ACES_CHECK(alloc(&Zm_, std::max<long>((nvm_ + ov) * nom_, 1)));
+  ACES_CHECK(alloc(&Yb_, v * o3));
+  ACES_CHECK(alloc(&L1_, v * v * oo));
+  ACES_CHECK(alloc(&tauK_, ovvo));
+  ACES_CHECK(alloc(&P1_, ovvo));
+  ACES_CHECK(alloc(&P2_, ovvo));
+  ACES_CHECK(alloc(&Q12_, ovvo));
+  ACES_CHECK(alloc(&Q2_, ovvo));
+  ACES_CHECK(alloc(&Qb_, o2v * o));
+  ACES_CHECK(alloc(&T1Q_, o2v * v));
+  ACES_CHECK(alloc(&S3_, o2v * o));
+  ACES_CHECK(alloc(&G4A_, o2v * o));
+  ACES_CHECK(alloc(&G4_, o2v * v));
+  ACES_CHECK(alloc(&Wo_, oo * oo));
+  ACES_CHECK(alloc(&Qo_, oo * oo));
+  ACES_CHECK(alloc(&ZH_, ovvo));
+  ACES_CHECK(alloc(&tmpL_, ovvo));
+  ACES_CHECK(alloc(&Fov_, ov));
+  ACES_CHECK(alloc(&Foo_, oo));
+  ACES_CHECK(alloc(&Fvv_, vv));
+  ACES_CHECK(alloc(&Loo_, oo));
+  ACES_CHECK(alloc(&Lvv_, vv));
+  ACES_CHECK(alloc(&FT_, oo));
+  ACES_CHECK(alloc(&FovT_, oo));
+  ACES_CHECK(alloc(&Gk_, oo));
+  ACES_CHECK(alloc_raw((void**)&red_, reduction_scratch_bytes()));
+  ACES_CHECK(alloc(&scal_, 64));
+  ACES_CHECK(alloc(&err_cur_, nt));
+  ACES_CHECK(alloc(&newvec_, nt));
+  ACES_CHECK(alloc_raw((void**)&d_ptrs_, 64 * sizeof(double*)));
+  ACES_CHECK(alloc(&d_coef_, 64));
+  ACES_CHECK(set_diis_order(order_));
+  ACES_HIP(hipStreamSynchronize(s));
+  ready_ = true;
+  return ACES_OK;
+}
+
+int CcsdEngine::set_diis_order(int order) {
+  ACES_REQUIRE(order >= 1 && order <= 25, "DIIS order must be 1..25 (ld_diis = 25, vcc/dodiis0.F)");
+  const long nt = (long)o * v + (long)o * o * v * v;
+  while ((int)sol_.size() < order) {
+    double *a, *b;
+    ACES_CHECK(alloc(&a, nt));
+    ACES_CHECK(alloc(&b, nt));
+    sol_.push_back(a);
+    err_.push_back(b);
+  }
+  order_ = order;
+  R_.assign((size_t)order * order, 0.0);
+  return ACES_OK;
+}
+
+int CcsdEngine::reset_diis() {
+  R_.assign((size_t)order_ * order_, 0.0);
+  return ACES_OK;
+}
+
+int CcsdEngine::set_amplitudes(const double* t1, const double* t2) {
+  ACES_REQUIRE(ready_, "ccsd: not set up");
+  const long ov = (long)o * v;
+  ACES_HIP(hipMemcpyAsync(t1_, t1, sizeof(double) * ov, hipMemcpyDeviceToDevice, s_));
+  ACES_HIP(hipMemcpyAsync(t2_, t2, sizeof(double) * ov * ov, hipMemcpyDeviceToDevice, s_));
+  ACES_HIP(hipStreamSynchronize(s_));
+  return ACES_OK;
+}
+
+int CcsdEngine::get_amplitudes(double* t1, double* t2) {
+  ACES_REQUIRE(ready_, "ccsd: not set up");
+  const long ov = (long)o * v;
+  if (t1) ACES_HIP(hipMemcpyAsync(t1, t1_, sizeof(double) * ov, hipMemcpyDeviceToDevice, s_));
+  if (t2) ACES_HIP(hipMemcpyAsync(t2, t2_, sizeof(double) * ov * ov, hipMemcpyDeviceToDevice, s_));
+  ACES_HIP(hipStreamSynchronize(s_));
+  return ACES_OK;
+}
+
+// MP1 guess written by xintprc (intprc/dfrmt2.F): T2 = <ij|ab>/D, T1 = 0.
+int CcsdEngine::start_mp1() {
+  ACES_REQUIRE(ready_, "ccsd: not set up");
+  const long ov = (long)o * v, ov2 = ov * v;
+  ACES_HIP(hipMemsetAsync(inc1_, 0, sizeof(double) * ov, s_));
+  ACES_CHECK(P4(o, o, v, v, inc2_).src(ovov_, 1, ov2, v, ov, 1).run(s_));
+  ACES_CHECK(jacobi(o, v, eo_, ev_, inc1_, inc2_, t1_, t2_, s_));
+  ACES_HIP(hipStreamSynchronize(s_));
+  return ACES_OK;
+}
+
+int CcsdEngine::energy_of(const double* t1, const double* t2, double& e) {
+  ACES_CHECK(energy(o, v, fock_ + o, n, Lijab_, t1, t2, red_, scal_, s_));
+  ACES_HIP(hipMemcpyAsync(&e, scal_, sizeof(double), hipMemcpyDeviceToHost, s_));
+  ACES_HIP(hipStreamSynchronize(s_));
+  return ACES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Low-precision operands.  Slots 0..3 static (converted once), 4..11 per call.
+int CcsdEngine::lowp_static(GemmPrec prec) {
+  if (lp_static_prec_ == prec) return ACES_OK;
+  const long ov = (long)o * v;
+  struct S { int slot; const double* src; long rows; long K; long ld; };
+  const S tab[4] = {{0, Vp_, nvp_ + ov, nvp_, nvp_},
+                    {1, Vm_, nvm_ + ov, nvm_, std::max<long>(nvm_, 1)},
+                    {2, Lmat_, ov, ov, ov},
+                    {3, Kx_, ov, ov, ov}};
+  for (const S& t : tab) {
+    LowP& L = lp_[t.slot];
+    const long ld = pad8(std::max<long>(t.K, 1));
+    if (L.hi == nullptr || L.ld != ld || L.rows != t.rows) {
+      void* blk;
+      ACES_CHECK(alloc_raw(&blk, t.rows * ld * 4));
+      L.hi = blk;
+      L.lo = (char*)blk + t.rows * ld * 2;
+      L.ld = ld;
+      L.rows = t.rows;
+    }
+    if (t.K == 0) continue;
+    if (prec == GP_F32)
+      ACES_CHECK(convert_f32((int)t.rows, (int)t.K, t.src, t.ld, (float*)L.hi, ld, s_));
+    else
+      ACES_CHECK(split_bf16((int)t.rows, (int)t.K, t.src, t.ld, (unsigned short*)L.hi, (unsigned short*)L.lo, ld,
+                            s_));
+  }
+  lp_static_prec_ = prec;
+  return ACES_OK;
+}
+
+int CcsdEngine::gemm_lp(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda,
+                        const double* B64, int bslot, long ldb, double* C, long ldc, double alpha, double beta) {
+  if (prec == GP_F64) return gemm(gd(M, N, K, A64, lda, 1, B64, ldb, 1, C, ldc, alpha, beta), GP_F64, s_);
+  ACES_CHECK(lowp_static(prec));
+  const long Kp = pad8(std::max(K, 1));
+  const void* ops[2][2];
+  const int slots[2] = {aslot, bslot};
+  const double* src[2] = {A64, B64};
+  const long rows[2] = {M, N};
+  const long lds[2] = {lda, ldb};
+  for (int q = 0; q < 2; ++q) {
+    LowP& L = lp_[slots[q]];
+    if (slots[q] >= 4) {
+      if (L.hi == nullptr || L.ld != Kp || L.rows < rows[q]) {
+        void* blk;
+        ACES_CHECK(alloc_raw(&blk, rows[q] * Kp * 4));
+        L.hi = blk;
+        L.lo = (char*)blk + rows[q] * Kp * 2;
+        L.ld = Kp;
+        L.rows = rows[q];
+      }
+      if (prec == GP_F32)
+        ACES_CHECK(convert_f32((int)rows[q], K, src[q], lds[q], (float*)L.hi, Kp, s_));
+      else
+        ACES_CHECK(split_bf16((int)rows[q], K, src[q], lds[q], (unsigned short*)L.hi, (unsigned short*)L.lo, Kp,
+                              s_));
+    }
+    ops[q][0] = L.hi;
+    ops[q][1] = L.lo;
+  }
+  GemmDesc d;
+  d.M = M; d.N = N; d.K = (prec == GP_BF16X3) ? (int)Kp : K;
+  d.A = ops[0][0]; d.A_lo = ops[0][1]; d.lda = lp_[aslot].ld; d.a_kc = 1;
+  d.B = ops[1][0]; d.B_lo = ops[1][1]; d.ldb = lp_[bslot].ld; d.b_kc = 1;
+  d.C = C; d.ldc = ldc; d.alpha = alpha; d.beta = beta;
+  return gemm(d, prec, s_);
+}
+
+// ---------------------------------------------------------------------------
+int CcsdEngine::compute_increments(GemmPrec prec) {
+  hipStream_t s = s_;
+  const long ov = (long)o * v, oo = (long)o * o, vv = (long)v * v;
+  const long ov2 = ov * v, o2v = oo * v, vo2 = v * oo, v2o = vv * o;
+  const long ovvo = ov * ov;
+  const double* t1 = t1_;
+  const double* t2 = t2_;
+  const int OV = (int)ov;
+
+  // tau = t2 + t1 t1
+  ACES_CHECK(P4(o, o, v, v, tau_).src(t2, 1, ov2, vv, v, 1).prod(t1, v, 0, 1, 0, t1, 0, v, 0, 1, 1.0).run(s));
+  // U[(i,a)][(j,b)] = 2 t_ijab - t_ijba ; X = t_ijba
+  ACES_CHECK(P4(o, v, o, v, U_).src(t2, 2, ov2, v, vv, 1).src(t2, -1, ov2, 1, vv, v).run(s));
+  ACES_CHECK(P4(o, v, o, v, X_).src(t2, 1, ov2, 1, vv, v).run(s));
+  // Ut = U - 2 t_la t_id ; Xt = X + 2 t_id t_la   (output index order i, a, l, d)
+  ACES_CHECK(P4(o, v, o, v, Ut_).src(U_, 1, ov2, ov, v, 1).prod(t1, 0, 1, v, 0, t1, v, 0, 0, 1, -2.0).run(s));
+  ACES_CHECK(P4(o, v, o, v, Xt_).src(X_, 1, ov2, ov, v, 1).prod(t1, 0, 1, v, 0, t1, v, 0, 0, 1, 2.0).run(s));
+
+  // T1 parts of the W(mbej) intermediates
+  ACES_CHECK(gemm(gd((int)v2o, o, v, ovvv_, v, 1, t1, v, 1, P1_, o, 1.0, 0.0), GP_F64, s));        // [(k,c,a)][i]
+  ACES_CHECK(gemm(gd((int)v2o, o, v, ovvv_kacd_, v, 1, t1, v, 1, P2_, o, 1.0, 0.0), GP_F64, s));   // [(k,a,c)][i]
+  ACES_CHECK(gemm(gd((int)o2v, v, o, M5c_, o, 1, t1, v, 0, Q12_, v, 1.0, 0.0), GP_F64, s));        // [(k,c,i)][a]
+  ACES_CHECK(gemm(gd((int)o2v, v, o, M5b_, o, 1, t1, v, 0, Q2_, v, 1.0, 0.0), GP_F64, s));         // [(k,c,i)][a]
+  // Wt[(i,a)][(k,c)] = W0t + 2 P1[k,c,a,i] - P2[k,a,c,i] - Q12[k,c,i,a]
+  ACES_CHECK(P4(o, v, o, v, Wt_).src(W0t_, 1, ov2, ov, v, 1).src(P1_, 2, 1, o, v2o, v * o)
+                 .src(P2_, -1, 1, v * o, v2o, o).run(s));
+  ACES_CHECK(P4(o, v, o, v, Wt_, 1.0).src(Q12_, -1, v, 1, ov2, ov).run(s));
+  // Wv[(i,a)][(k,c)] = Wv0 + P2[k,a,c,i] - Q2[k,c,i,a]
+  ACES_CHECK(P4(o, v, o, v, Wv_).src(Wv0_, 1, ov2, ov, v, 1).src(P2_, 1, 1, v * o, v2o, o)
+                 .src(Q2_, -1, v, 1, ov2, ov).run(s));
+  // T2 parts (MFMA): Wt += 1/2 Ut L^T ; Wv += -1/2 Xt K^T
+  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Ut_, 6, ov, Lmat_, 2, ov, Wt_, ov, 0.5, 1.0));
+  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Xt_, 7, ov, Kx_, 3, ov, Wv_, ov, -0.5, 1.0));
+  // ring (MFMA): R1 = Wt U^T ; R2 = Wv X^T
+  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Wt_, 8, ov, U_, 10, ov, R1_, ov, 1.0, 0.0));
+  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Wv_, 9, ov, X_, 11, ov, R2_, ov, 1.0, 0.0));
+
+  // inc2 = <ij|ab> = (ia|jb)
+  ACES_CHECK(P4(o, o, v, v, inc2_).src(ovov_, 1, ov2, v, ov, 1).run(s));
+  // ring assembly: RS = (R1 - R2)/2 ; inc2 += RS[iajb] + RS[jbia] - R2[ibja] - R2[jaib]
+  ACES_CHECK(P4(1, 1, 1, ovvo, R1_, 0.5).src(R2_, -0.5, 0, 0, 0, 1).run(s));
+  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(R1_, 1, ov2, v, ov, 1).src(R1_, 1, v, ov2, 1, ov)
+                 .src(R2_, -1, ov2, v, 1, ov).run(s));
+  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(R2_, -1, v, ov2, ov, 1).run(s));
+
+  // particle-particle ladder with the T1-dressing rows (MFMA)
+  ACES_CHECK(pack_tau(o, v, tau_, Sp_, nvp_, Dp_, std::max<long>(nvm_, 1), s));
+  ACES_CHECK(gemm_lp(prec, (int)(nvp_ + ov), (int)nop_, (int)nvp_, Vp_, 0, nvp_, Sp_, 4, nvp_, Zp_, nop_, 1.0, 0.0));
+  if (nom_ > 0 && nvm_ > 0)
+    ACES_CHECK(gemm_lp(prec, (int)(nvm_ + ov), (int)nom_, (int)nvm_, Vm_, 1, nvm_, Dp_, 5, nvm_, Zm_, nom_, 1.0,
+                       0.0));
+  else
+    ACES_HIP(hipMemsetAsync(Zm_, 0, sizeof(double), s));
+  ACES_CHECK(unpack_ladder(o, v, Zp_, nop_, Zm_, std::max<long>(nom_, 1), inc2_, Yb_, s));
+  // L1[a][(b,i,j)] = sum_k t_ka Yb[(b,i,j)][k] ; inc2 -= L1[a,b,i,j] + L1[b,a,j,i]
+  ACES_CHECK(gemm(gd(v, (int)(v * oo), o, t1, v, 0, Yb_, o, 1, L1_, v * oo, 1.0, 0.0), GP_F64, s));
+  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(L1_, -1, o, 1, vo2, oo).src(L1_, -1, 1, o, oo, vo2).run(s));
+
+  // hole-hole ladder: Wo = (ki|lj) + Qo[klij] + Qo[lkji] + (kc|ld) tau_ijcd ; inc2 += tau_klab Wo_klij
+  ACES_CHECK(gemm(gd((int)(oo * o), o, v, ovooQ_, v, 1, t1, v, 1, Qo_, o, 1.0, 0.0), GP_F64, s));
+  ACES_CHECK(P4(o, o, o, o, Wo_).src(ooooP_, 1, oo * o, oo, o, 1).src(Qo_, 1, oo * o, oo, o, 1)
+                 .src(Qo_, 1, oo, oo * o, 1, o).run(s));
+  ACES_CHECK(gemm(gd((int)oo, (int)oo, (int)vv, OVkl_, vv, 1, tau_, vv, 1, Wo_, oo, 1.0, 1.0), GP_F64, s));
+  ACES_CHECK(gemm(gd((int)vv, (int)oo, (int)oo, tau_, vv, 0, Wo_, oo, 0, ZH_, oo, 1.0, 0.0), GP_F64, s));
+  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(ZH_, 1, o, 1, vo2, oo).run(s));
+
+  // F intermediates (QUAD2/QUAD3/MAKFME + T1RING pieces)
+  const double* fov = fock_ + o;
+  ACES_CHECK(P4(1, 1, o, v, Fov_).src(fov, 1, 0, 0, n, 1).run(s));
+  ACES_CHECK(gemv(OV, OV, Lmat_, ov, t1, Fov_, 1.0, 1.0, s));
+  ACES_CHECK(P4(1, 1, o, o, Foo_).src(fock_, 1, 0, 0, n, 1).run(s));
+  ACES_CHECK(gemm(gd(o, o, (int)ov2, Lo_, ov2, 1, tau_, ov2, 1, Foo_, o, 1.0, 1.0), GP_F64, s));
+  ACES_CHECK(P4(v, o, o, v, tauK_).src(tau_, 1, v, ov2, vv, 1).run(s));  // tauK[a][k][l][d] = tau_klad
+  ACES_CHECK(P4(1, 1, v, v, Fvv_).src(fock_ + (long)o * n + o, 1, 0, 0, n, 1).run(s));
+  ACES_CHECK(gemm(gd(v, v, (int)(oo * v), tauK_, oo * v, 1, Lv_, oo * v, 1, Fvv_, v, -1.0, 1.0), GP_F64, s));
+  ACES_CHECK(gemm(gd(o, o, v, fov, n, 1, t1, v, 1, FT_, o, 1.0, 0.0), GP_F64, s));       // FT[k][i] = f_kc t_ic
+  ACES_CHECK(gemm(gd(o, o, v, Fov_, v, 1, t1, v, 1, FovT_, o, 1.0, 0.0), GP_F64, s));    // FovT[k][i]
+  ACES_CHECK(P4(1, 1, o, o, Loo_).src(Foo_, 1, 0, 0, o, 1).src(FT_, 1, 0, 0, o, 1).run(s));
+  ACES_CHECK(gemv((int)oo, OV, M1_, ov, t1, Loo_, 1.0, 1.0, s));
+  ACES_CHECK(P4(1, 1, v, v, Lvv_).src(Fvv_, 1, 0, 0, v, 1).run(s));
+  ACES_CHECK(gemm(gd(v, v, o, t1, v, 0, fov, n, 0, Lvv_, v, -1.0, 1.0), GP_F64, s));
+  ACES_CHECK(gemv_t(OV, (int)vv, ovvv_, vv, t1, Lvv_, 2.0, 1.0, s));
+  ACES_CHECK(contract2(v, v, o, v, ovvv_kacd_, vv, 1, vv * v, v, t1, v, 1, Lvv_, -1.0, 1.0, s));
+  // diagonal (orbital energies) moved into the denominators
+  ACES_CHECK(sub_diag(o, Foo_, o, eo_, s));
+  ACES_CHECK(sub_diag(v, Fvv_, v, ev_, s));
+  ACES_CHECK(sub_diag(o, Loo_, o, eo_, s));
+  ACES_CHECK(sub_diag(v, Lvv_, v, ev_, s));
+
+  // FEACONT/FMICONT: tmpL = Lvv_ac t_ijcb - Loo_ki t_kjab ; inc2 += tmpL + tmpL^T
+  {
+    GemmDesc d = gd(v, v, v, Lvv_, v, 1, t2, v, 0, tmpL_, v, 1.0, 0.0);
+    d.batch = (int)oo;
+    d.sA = 0;
+    d.sB = vv;
+    d.sC = vv;
+    ACES_CHECK(gemm(d, GP_F64, s));
+  }
+  ACES_CHECK(gemm(gd(o, (int)ov2, o, Loo_, o, 0, t2, ov2, 0, tmpL_, ov2, -1.0, 1.0), GP_F64, s));
+  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(tmpL_, 1, ov2, vv, v, 1).src(tmpL_, 1, vv, ov2, 1, v).run(s));
+
+  // T1INT2A/T1INT2B/T12INT2 pieces:
+  //   inc2 += P1[iabj] - T1Q[aibj] + P1[jbai] - T1Q[bjai]
+  ACES_CHECK(gemm(gd((int)o2v, o, v, oovv_, v, 1, t1, v, 1, Qb_, o, 1.0, 0.0), GP_F64, s));         // [(k,i,b)][j]
+  ACES_CHECK(gemm(gd(v, (int)o2v, o, t1, v, 0, Qb_, o2v, 0, T1Q_, o2v, 1.0, 0.0), GP_F64, s));     // [a][(i,b,j)]
+  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(P1_, 1, v2o, 1, v * o, o)
+                 .src(T1Q_, -1, v * o, 1, o2v, o).src(P1_, 1, 1, v2o, o, v * o).run(s));
+  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(T1Q_, -1, 1, v * o, o, o2v).run(s));
+  //   inc2 -= G4[iajb] + G4[jbia],  G4 = (ovoo[iajk] + S3[iakj]) t_kb
+  ACES_CHECK(gemm(gd((int)o2v, o, v, ovov_, v, 1, t1, v, 1, S3_, o, 1.0, 0.0), GP_F64, s));          // [(i,a,k)][j]
+  ACES_CHECK(P4(o, v, o, o, G4A_).src(ovoo_, 1, vo2, oo, o, 1).src(S3_, 1, vo2, oo, 1, o).run(s));
+  ACES_CHECK(gemm(gd((int)o2v, v, o, G4A_, o, 1, t1, v, 0, G4_, v, 1.0, 0.0), GP_F64, s));         // [(i,a,j)][b]
+  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(G4_, -1, ov2, v, ov, 1).src(G4_, -1, v, ov2, 1, ov).run(s));
+
+  // ---- T1 increment ----
+  ACES_CHECK(P4(1, 1, o, v, inc1_).src(fov, 1, 0, 0, n, 1).run(s));
+  ACES_CHECK(P4(1, 1, o, o, Gk_).src(Foo_, 1, 0, 0, o, 1).src(FT_, 2, 0, 0, o, 1).src(FovT_, -1, 0, 0, o, 1).run(s));
+  ACES_CHECK(gemm(gd(o, v, o, Gk_, o, 0, t1, v, 0, inc1_, v, -1.0, 1.0), GP_F64, s));
+  ACES_CHECK(gemm(gd(o, v, v, t1, v, 1, Fvv_, v, 1, inc1_, v, 1.0, 1.0), GP_F64, s));
+  ACES_CHECK(gemv(OV, OV, U_, ov, Fov_, inc1_, 1.0, 1.0, s));
+  ACES_CHECK(gemv(OV, OV, W0t_, ov, t1, inc1_, 1.0, 1.0, s));
+  ACES_CHECK(gemm(gd(o, v, (int)ov2, tau_, ov2, 1, M3_, ov2, 1, inc1_, v, 1.0, 1.0), GP_F64, s));
+  ACES_CHECK(gemm(gd(o, v, (int)o2v, M7_, o2v, 1, tauK_, o2v, 1, inc1_, v, -1.0, 1.0), GP_F64, s));
+  return ACES_OK;
+}
+
+int CcsdEngine::increments(GemmPrec prec, double* inc1_out, double* inc2_out) {
+  ACES_REQUIRE(ready_, "ccsd: not set up");
+  ACES_CHECK(compute_increments(prec));
+  const long ov = (long)o * v;
+  if (inc1_out) ACES_HIP(hipMemcpyAsync(inc1_out, inc1_, sizeof(double) * ov, hipMemcpyDeviceToDevice, s_));
+  if (inc2_out) ACES_HIP(hipMemcpyAsync(inc2_out, inc2_, sizeof(double) * ov * ov, hipMemcpyDeviceToDevice, s_));
+  ACES_HIP(hipStreamSynchronize(s_));
+  return ACES_OK;
+}
+
+// One cycle of the reference loop (vcc/vcc.F:811-1085) with DODIIS0 semantics.
+int CcsdEngine::cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st) {
+  ACES_REQUIRE(ready_, "ccsd: not set up");
+  ACES_REQUIRE(icycle >= 1, "ccsd cycle: icycle starts at 1");
+  hipEvent_t e0, e1, e2;
+  ACES_HIP(hipEventCreate(&e0));
+  ACES_HIP(hipEventCreate(&e1));
+  ACES_HIP(hipEventCreate(&e2));
+  ACES_HIP(hipEventRecord(e0, s_));
+  ACES_CHECK(compute_increments(prec));
+  ACES_HIP(hipEventRecord(e1, s_));
+  const long nt = (long)o * v + (long)o * o * v * v;
+  // NEWT2 / E4S
+  ACES_CHECK(jacobi(o, v, eo_, ev_, inc1_, inc2_, t1n_, t2n_, s_));
+  // CMPENG(60,2): energy of the Jacobi update
+  ACES_CHECK(energy(o, v, fock_ + o, n, Lijab_, t1n_, t2n_, red_, scal_, s_));
+  // DODIIS0
+  ACES_CHECK(symmetrize_t2(o, v, t2_, s_));
+  ACES_CHECK(symmetrize_t2(o, v, t2n_, s_));
+  const int ndimr = std::min(icycle, order_);
+  const int idimr = 1 + (icycle - 1) % order_;  // 1-based slot
+  ACES_CHECK(sub(nt, t1n_, t1_, err_cur_, s_));
+  ACES_CHECK(diis_absmax(o, v, err_cur_, red_, scal_ + 1, s_));
+  ACES_HIP(hipMemcpyAsync(sol_[idimr - 1], t1_, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
+  // dots <err_i, e> for i = 1..ndimr (slot idimr uses e itself)
+  std::vector<const double*> xs(ndimr);
+  for (int i = 1; i <= ndimr; ++i) xs[i - 1] = (i == idimr) ? err_cur_ : err_[i - 1];
+  ACES_HIP(hipMemcpyAsync(d_ptrs_, xs.data(), sizeof(double*) * ndimr, hipMemcpyHostToDevice, s_));
+  ACES_CHECK(diis_dots(o, v, ndimr, d_ptrs_, err_cur_, red_, scal_ + 8, s_));
+  double h[40];
+  ACES_HIP(hipMemcpyAsync(h, scal_, sizeof(double) * 40, hipMemcpyDeviceToHost, s_));
+  ACES_HIP(hipStreamSynchronize(s_));
+  st.energy_new = h[0];
+  st.resmiter = h[1];
+  for (int i = 1; i <= ndimr; ++i) {
+    R_[(size_t)(i - 1) * order_ + (idimr - 1)] = h[8 + i - 1];
+    R_[(size_t)(idimr - 1) * order_ + (i - 1)] = h[8 + i - 1];
+  }
+  ACES_HIP(hipMemcpyAsync(err_[idimr - 1], err_cur_, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
+  // DODIIS: augmented Pulay system
+  const int na = ndimr + 1;
+  std::vector<double> A((size_t)na * na, 0.0), b(na, 0.0);
+  for (int j = 1; j < na; ++j) {
+    A[j] = -1.0;
+    A[(size_t)j * na] = -1.0;
+    for (int i = 1; i < na; ++i) A[(size_t)i * na + j] = R_[(size_t)(i - 1) * order_ + (j - 1)];
+  }
+  b[0] = -1.0;
+  if (solve_small(na, A, b) != ACES_OK) {
+    set_error("DODIIS: linear solver failed (singular DIIS matrix)");
+    return ACES_ERR_SOLVE;
+  }
+  std::vector<double> c(b.begin() + 1, b.end());
+  // DIIS residual in the reference's summation order
+  std::vector<const double*> rv;
+  std::vector<double> rc;
+  rv.push_back(err_[idimr - 1]);
+  rc.push_back(c[idimr - 1]);
+  for (int i = ndimr; i > idimr; --i) {
+    rv.push_back(err_[i - 1]);
+    rc.push_back(c[i - 1]);
+  }
+  for (int i = idimr - 1; i >= 1; --i) {
+    rv.push_back(err_[i - 1]);
+    rc.push_back(c[i - 1]);
+  }
+  ACES_HIP(hipMemcpyAsync(d_ptrs_, rv.data(), sizeof(double*) * rv.size(), hipMemcpyHostToDevice, s_));
+  ACES_HIP(hipMemcpyAsync(d_coef_, rc.data(), sizeof(double) * rc.size(), hipMemcpyHostToDevice, s_));
+  ACES_CHECK(lincomb(nt, (int)rv.size(), d_ptrs_, d_coef_, newvec_, s_));
+  ACES_CHECK(diis_absmax(o, v, newvec_, red_, scal_ + 2, s_));
+  ACES_HIP(hipMemcpyAsync(h, scal_ + 2, sizeof(double), hipMemcpyDeviceToHost, s_));
+  ACES_HIP(hipStreamSynchronize(s_));
+  st.resmdiis = h[0];
+  st.converged = std::fabs(st.resmiter) < tol ? 1 : 0;
+  if (st.converged) {
+    // CMPENG(43,0) on the amplitudes of this cycle; amplitudes are not moved.
+    double e;
+    ACES_CHECK(energy_of(t1_, t2_, e));
+    st.energy_final = e;
+  } else if (ndimr >= 2) {
+    std::vector<const double*> sv;
+    std::vector<double> sc;
+    sv.push_back(newvec_);
+    sc.push_back(1.0);
+    for (int i = 1; i <= ndimr; ++i) {
+      sv.push_back(sol_[i - 1]);
+      sc.push_back(c[i - 1]);
+    }
+    ACES_HIP(hipMemcpyAsync(d_ptrs_, sv.data(), sizeof(double*) * sv.size(), hipMemcpyHostToDevice, s_));
+    ACES_HIP(hipMemcpyAsync(d_coef_, sc.data(), sizeof(double) * sc.size(), hipMemcpyHostToDevice, s_));
+    ACES_CHECK(lincomb(nt, (int)sv.size(), d_ptrs_, d_coef_, t1_, s_));
+  } else {
+    ACES_HIP(hipMemcpyAsync(t1_, t1n_, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
+  }
+  ACES_HIP(hipEventRecord(e2, s_));
+  ACES_HIP(hipEventSynchronize(e2));
+  float ms1 = 0, ms2 = 0;
+  ACES_HIP(hipEventElapsedTime(&ms1, e0, e1));
+  ACES_HIP(hipEventElapsedTime(&ms2, e0, e2));
+  st.ms_increments = ms1;
+  st.ms_cycle = ms2;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  return ACES_OK;
+}
+
+}  // namespace aces

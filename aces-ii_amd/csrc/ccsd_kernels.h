@@ -1,0 +1,25 @@
+// CCSD-specific HBM-bound kernels (see ccsd_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace aces {
+
+int pack_tau(int o, int v, const double* tau, double* S, long lds, double* D, long ldd, hipStream_t s);
+int build_ladder_operands(int o, int v, const double* L233, const double* ovvv, double* Vp, long ldp, double* Vm,
+                          long ldm, hipStream_t s);
+int unpack_ladder(int o, int v, const double* Zp, long ldzp, const double* Zm, long ldzm, double* inc2, double* Yb,
+                  hipStream_t s);
+int jacobi(int o, int v, const double* eo, const double* ev, const double* inc1, const double* inc2, double* t1n,
+           double* t2n, hipStream_t s);
+int symmetrize_t2(int o, int v, double* t2, hipStream_t s);
+int energy(int o, int v, const double* fov, long ldf, const double* Lijab, const double* t1, const double* t2,
+           double* part, double* out, hipStream_t s);
+int diis_dots(int o, int v, int m, const double* const* d_xs, const double* y, double* part, double* out,
+              hipStream_t s);
+int diis_absmax(int o, int v, const double* e, void* part, double* out, hipStream_t s);
+long reduction_scratch_bytes();
+int lincomb(long n, int m, const double* const* d_xs, const double* d_c, double* out, hipStream_t s);
+int sub(long n, const double* x, const double* z, double* y, hipStream_t s);
+int sub_diag(int n, double* X, long ld, const double* d, hipStream_t s);
+
+}  // namespace aces

@@ -1,0 +1,493 @@
+// CCSD-specific HBM-bound kernels: ladder packing/unpacking, Jacobi update,
+// energy, DIIS vector algebra.  All fp64.
+//
+// Pair orderings follow the reference's DPD packing (acescore/aces_com_symloc.F,
+// libr/symexp4.f:36, libr/assym.f):
+//   'PCK2' (p <= q):  index = q*(q+1)/2 + p     (0-based)
+//   'PACK' (p <  q):  index = q*(q-1)/2 + p
+#include "common.h"
+#include "ccsd_kernels.h"
+
+namespace aces {
+
+static inline unsigned grid1(long n, int block = 256, long cap = 1L << 20) {
+  long g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+// Inverse of the packed pair index (q outer, p inner).
+__device__ inline void unpack_le(long idx, int& p, int& q) {  // p <= q
+  int qq = (int)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+  while ((long)qq * (qq + 1) / 2 > idx) --qq;
+  while ((long)(qq + 1) * (qq + 2) / 2 <= idx) ++qq;
+  q = qq;
+  p = (int)(idx - (long)qq * (qq + 1) / 2);
+}
+__device__ inline void unpack_lt(long idx, int& p, int& q) {  // p < q
+  int qq = (int)((sqrt(8.0 * (double)idx + 1.0) + 1.0) * 0.5);
+  while ((long)qq * (qq - 1) / 2 > idx) --qq;
+  while ((long)(qq + 1) * qq / 2 <= idx) ++qq;
+  q = qq;
+  p = (int)(idx - (long)qq * (qq - 1) / 2);
+}
+__device__ inline long pk_le(int p, int q) { return (long)q * (q + 1) / 2 + p; }
+__device__ inline long pk_lt(int p, int q) { return (long)q * (q - 1) / 2 + p; }
+
+// ---------------------------------------------------------------------------
+// Ladder right operand from tau[i][j][c][d]:
+//   Sp[(i<=j)][(c<=d)] = (tau_ijcd + tau_jicd)/2,  Dp[(i<j)][(c<d)] = (tau_ijcd - tau_jicd)/2
+__global__ void pack_tau_kernel(int o, int v, const double* __restrict__ tau, double* __restrict__ S, long lds,
+                                double* __restrict__ D, long ldd) {
+  const long nop = (long)o * (o + 1) / 2, nvp = (long)v * (v + 1) / 2;
+  const long nom = (long)o * (o - 1) / 2, nvm = (long)v * (v - 1) / 2;
+  const long totS = nop * nvp, totD = nom * nvm;
+  const long vv = (long)v * v;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < totS + totD;
+       idx += (long)gridDim.x * blockDim.x) {
+    int i, j, c, d;
+    if (idx < totS) {
+      const long pij = idx / nvp, pcd = idx - pij * nvp;
+      unpack_le(pij, i, j);
+      unpack_le(pcd, c, d);
+      const double x = tau[((long)i * o + j) * vv + (long)c * v + d];
+      const double y = tau[((long)j * o + i) * vv + (long)c * v + d];
+      S[pij * lds + pcd] = 0.5 * (x + y);
+    } else {
+      const long r = idx - totS;
+      const long pij = r / nvm, pcd = r - pij * nvm;
+      unpack_lt(pij, i, j);
+      unpack_lt(pcd, c, d);
+      const double x = tau[((long)i * o + j) * vv + (long)c * v + d];
+      const double y = tau[((long)j * o + i) * vv + (long)c * v + d];
+      D[pij * ldd + pcd] = 0.5 * (x - y);
+    }
+  }
+}
+
+int pack_tau(int o, int v, const double* tau, double* S, long lds, double* D, long ldd, hipStream_t s) {
+  const long n = (long)o * (o + 1) / 2 * ((long)v * (v + 1) / 2) + (long)o * (o - 1) / 2 * ((long)v * (v - 1) / 2);
+  hipLaunchKernelGGL(pack_tau_kernel, dim3(grid1(n)), dim3(256), 0, s, o, v, tau, S, lds, D, ldd);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ladder left operand rows.
+//   Vp[(a<=b)][(c<=d)] = <ab|cd> + <ab|dc>  (c<d),  <ab|cc> (c==d)
+//   Vm[(a<b)][(c<d)]   = <ab|cd> - <ab|dc>
+// from the reference's compressed RHF list 233 (rows A<=B 'PCK2', columns Cd
+// with C fastest, column major): element (ab, cd) at L[(c + v*d)*nvp + pk_le(a,b)],
+// value <ab|cd> = (ac|bd)  (intprc/intproc.F:250-257).
+__global__ void build_vpm_vvvv_kernel(int v, const double* __restrict__ L233, double* __restrict__ Vp, long ldp,
+                                      double* __restrict__ Vm, long ldm) {
+  const long nvp = (long)v * (v + 1) / 2, nvm = (long)v * (v - 1) / 2;
+  const long totP = nvp * nvp, totM = nvm * nvm;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < totP + totM;
+       idx += (long)gridDim.x * blockDim.x) {
+    int a, b, c, d;
+    if (idx < totP) {
+      const long rab = idx / nvp, pcd = idx - rab * nvp;
+      unpack_le(pcd, c, d);
+      const double x = L233[((long)c + (long)v * d) * nvp + rab];
+      const double y = (c == d) ? 0.0 : L233[((long)d + (long)v * c) * nvp + rab];
+      Vp[rab * ldp + pcd] = x + y;
+    } else {
+      const long r = idx - totP;
+      const long rab = r / nvm, pcd = r - rab * nvm;
+      unpack_lt(rab, a, b);
+      unpack_lt(pcd, c, d);
+      const long row = pk_le(a, b);
+      const double x = L233[((long)c + (long)v * d) * nvp + row];
+      const double y = L233[((long)d + (long)v * c) * nvp + row];
+      Vm[rab * ldm + pcd] = x - y;
+    }
+  }
+}
+
+// G rows (k,b) appended to the ladder operand (T1 part of the dressed <ab||cd>):
+//   Gp[(k,b)][(c<=d)] = (kc|bd) + (kd|bc) (c<d), (kc|bc) (c==d);  Gm = (kc|bd) - (kd|bc)
+// from ovvv[k][c][b][d] = (kc|bd).
+__global__ void build_g_ovvv_kernel(int o, int v, const double* __restrict__ ovvv, double* __restrict__ Gp,
+                                    long ldp, double* __restrict__ Gm, long ldm) {
+  const long nvp = (long)v * (v + 1) / 2, nvm = (long)v * (v - 1) / 2;
+  const long rows = (long)o * v;
+  const long totP = rows * nvp, totM = rows * nvm;
+  const long v2 = (long)v * v, v3 = v2 * v;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < totP + totM;
+       idx += (long)gridDim.x * blockDim.x) {
+    int c, d;
+    long row, pcd;
+    if (idx < totP) {
+      row = idx / nvp;
+      pcd = idx - row * nvp;
+      unpack_le(pcd, c, d);
+    } else {
+      const long r = idx - totP;
+      row = r / nvm;
+      pcd = r - row * nvm;
+      unpack_lt(pcd, c, d);
+    }
+    const long k = row / v, b = row - (row / v) * v;
+    const double x = ovvv[k * v3 + (long)c * v2 + b * v + d];  // (kc|bd)
+    const double y = ovvv[k * v3 + (long)d * v2 + b * v + c];  // (kd|bc)
+    if (idx < totP)
+      Gp[row * ldp + pcd] = (c == d) ? x : x + y;
+    else
+      Gm[row * ldm + pcd] = x - y;
+  }
+}
+
+int build_ladder_operands(int o, int v, const double* L233, const double* ovvv, double* Vp, long ldp, double* Vm,
+                          long ldm, hipStream_t s) {
+  const long nvp = (long)v * (v + 1) / 2, nvm = (long)v * (v - 1) / 2;
+  long n = nvp * nvp + nvm * nvm;
+  hipLaunchKernelGGL(build_vpm_vvvv_kernel, dim3(grid1(n)), dim3(256), 0, s, v, L233, Vp, ldp, Vm, ldm);
+  ACES_HIP(hipGetLastError());
+  n = (long)o * v * (nvp + nvm);
+  hipLaunchKernelGGL(build_g_ovvv_kernel, dim3(grid1(n)), dim3(256), 0, s, o, v, ovvv, Vp + nvp * ldp, ldp,
+                     Vm + nvm * ldm, ldm);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ladder result unpack.  Zp rows: [(a<=b) | (k,b)] x cols (i<=j); Zm rows [(a<b) | (k,b)] x (i<j).
+//   inc2[i][j][a][b] += Zp[ab][ij] + s Zm[ab][ij],  s = sign(b-a) * sign(j-i)
+//   Yb[b][i][j][k]   = Zp[kb][ij] + sign(j-i) Zm[kb][ij]
+__global__ void unpack_ladder_kernel(int o, int v, const double* __restrict__ Zp, long ldzp,
+                                     const double* __restrict__ Zm, long ldzm, double* __restrict__ inc2,
+                                     double* __restrict__ Yb) {
+  const long nvp = (long)v * (v + 1) / 2, nvm = (long)v * (v - 1) / 2;
+  const long oo = (long)o * o, vv = (long)v * v;
+  const long tot1 = oo * vv, tot2 = (long)v * oo * o;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < tot1 + tot2;
+       idx += (long)gridDim.x * blockDim.x) {
+    if (idx < tot1) {
+      const long ij = idx / vv, ab = idx - ij * vv;
+      const int i = (int)(ij / o), j = (int)(ij - (ij / o) * o);
+      const int a = (int)(ab / v), b = (int)(ab - (ab / v) * v);
+      const int i0 = min(i, j), j0 = max(i, j), a0 = min(a, b), b0 = max(a, b);
+      double z = Zp[pk_le(a0, b0) * ldzp + pk_le(i0, j0)];
+      if (i != j && a != b) {
+        const double sg = ((i < j) == (a < b)) ? 1.0 : -1.0;
+        z += sg * Zm[pk_lt(a0, b0) * ldzm + pk_lt(i0, j0)];
+      }
+      inc2[idx] += z;
+    } else {
+      const long r = idx - tot1;  // r = ((b*o + i)*o + j)*o + k
+      const int k = (int)(r % o);
+      long q = r / o;
+      const int j = (int)(q % o);
+      q /= o;
+      const int i = (int)(q % o);
+      const int b = (int)(q / o);
+      const long row = (long)k * v + b;
+      const int i0 = min(i, j), j0 = max(i, j);
+      double y = Zp[(nvp + row) * ldzp + pk_le(i0, j0)];
+      if (i != j) y += ((i < j) ? 1.0 : -1.0) * Zm[(nvm + row) * ldzm + pk_lt(i0, j0)];
+      Yb[r] = y;
+    }
+  }
+}
+
+int unpack_ladder(int o, int v, const double* Zp, long ldzp, const double* Zm, long ldzm, double* inc2, double* Yb,
+                  hipStream_t s) {
+  const long n = (long)o * o * v * v + (long)v * o * o * o;
+  hipLaunchKernelGGL(unpack_ladder_kernel, dim3(grid1(n)), dim3(256), 0, s, o, v, Zp, ldzp, Zm, ldzm, inc2, Yb);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi step (NEWT2 / E4S): t1n = inc1 / (e_i - e_a), t2n = inc2 / (e_i + e_j - e_a - e_b).
+__global__ void jacobi_kernel(int o, int v, const double* __restrict__ eo, const double* __restrict__ ev,
+                              const double* __restrict__ inc1, const double* __restrict__ inc2,
+                              double* __restrict__ t1n, double* __restrict__ t2n) {
+  const long n1 = (long)o * v, n2 = n1 * n1;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n1 + n2;
+       idx += (long)gridDim.x * blockDim.x) {
+    if (idx < n1) {
+      const int i = (int)(idx / v), a = (int)(idx % v);
+      t1n[idx] = inc1[idx] / (eo[i] - ev[a]);
+    } else {
+      const long r = idx - n1;
+      const int b = (int)(r % v);
+      const int a = (int)((r / v) % v);
+      const int j = (int)((r / ((long)v * v)) % o);
+      const int i = (int)(r / ((long)v * v * o));
+      t2n[r] = inc2[r] / (eo[i] + eo[j] - ev[a] - ev[b]);
+    }
+  }
+}
+
+int jacobi(int o, int v, const double* eo, const double* ev, const double* inc1, const double* inc2, double* t1n,
+           double* t2n, hipStream_t s) {
+  const long n = (long)o * v * (1 + (long)o * v);
+  hipLaunchKernelGGL(jacobi_kernel, dim3(grid1(n)), dim3(256), 0, s, o, v, eo, ev, inc1, inc2, t1n, t2n);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+// In-place SYMETRIZE_T2 on t2[i][j][a][b]: t <- (t_ijab + t_jiba)/2.
+__global__ void symmetrize_kernel(int o, int v, double* __restrict__ t2) {
+  const long vv = (long)v * v, n = (long)o * o * vv;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n; idx += (long)gridDim.x * blockDim.x) {
+    const long ij = idx / vv, ab = idx - ij * vv;
+    const int i = (int)(ij / o), j = (int)(ij % o), a = (int)(ab / v), b = (int)(ab % v);
+    const long tw = ((long)j * o + i) * vv + (long)b * v + a;
+    if (idx < tw) {
+      const double s = 0.5 * (t2[idx] + t2[tw]);
+      t2[idx] = s;
+      t2[tw] = s;
+    }
+  }
+}
+
+int symmetrize_t2(int o, int v, double* t2, hipStream_t s) {
+  const long n = (long)o * o * v * v;
+  hipLaunchKernelGGL(symmetrize_kernel, dim3(grid1(n)), dim3(256), 0, s, o, v, t2);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Block reductions (fixed grid; partials then a single-block finish).
+static constexpr int kRedBlocks = 1024;
+
+__device__ inline double block_sum(double x, double* sh) {
+  sh[threadIdx.x] = x;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// Energy: E = 2 sum f_ia t_ia + sum_ijab L[i][j][a][b] (t2_ijab + t1_ia t1_jb).
+__global__ void energy_partial_kernel(int o, int v, const double* __restrict__ fov, long ldf,
+                                      const double* __restrict__ Lijab, const double* __restrict__ t1,
+                                      const double* __restrict__ t2, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const long n1 = (long)o * v, n2 = n1 * n1, vv = (long)v * v;
+  double acc = 0.0;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n1 + n2;
+       idx += (long)gridDim.x * blockDim.x) {
+    if (idx < n1) {
+      const int i = (int)(idx / v), a = (int)(idx % v);
+      acc += 2.0 * fov[(long)i * ldf + a] * t1[idx];
+    } else {
+      const long r = idx - n1;
+      const long ij = r / vv, ab = r - ij * vv;
+      const int i = (int)(ij / o), j = (int)(ij % o), a = (int)(ab / v), b = (int)(ab % v);
+      acc += Lijab[r] * (t2[r] + t1[(long)i * v + a] * t1[(long)j * v + b]);
+    }
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void finish_sum_kernel(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += part[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) *out = s;
+}
+
+int energy(int o, int v, const double* fov, long ldf, const double* Lijab, const double* t1, const double* t2,
+           double* part, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(energy_partial_kernel, dim3(kRedBlocks), dim3(256), 0, s, o, v, fov, ldf, Lijab, t1, t2, part);
+  hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(256), 0, s, part, kRedBlocks, out);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+// DIIS inner products of vectors x = [t1 | t2(ijab)] including the implicit
+// same-spin block (t_ijab - t_jiab for i<j, a<b) that DIISLST appends
+// (vcc/diislst.F:20-29).  Computes m dots <x_k, y> at once.
+__global__ void diis_dot_partial_kernel(int o, int v, int m, const double* const* __restrict__ xs,
+                                        const double* __restrict__ y, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const long n1 = (long)o * v, vv = (long)v * v, n2 = n1 * n1;
+  double acc[8];
+  for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n1 + n2;
+       idx += (long)gridDim.x * blockDim.x) {
+    const double yv = y[idx];
+    double yaa = 0.0;
+    long tw = -1;
+    if (idx >= n1) {
+      const long r = idx - n1;
+      const long ij = r / vv, ab = r - ij * vv;
+      const int i = (int)(ij / o), j = (int)(ij % o), a = (int)(ab / v), b = (int)(ab % v);
+      if (i < j && a < b) {
+        tw = n1 + ((long)j * o + i) * vv + ab;
+        yaa = yv - y[tw];
+      }
+    }
+    for (int k = 0; k < m; ++k) {
+      const double* x = xs[k];
+      double c = x[idx] * yv;
+      if (tw >= 0) c += (x[idx] - x[tw]) * yaa;
+      acc[k] += c;
+    }
+  }
+  for (int k = 0; k < m; ++k) {
+    const double s = block_sum(acc[k], sh);
+    if (threadIdx.x == 0) part[k * kRedBlocks + blockIdx.x] = s;
+  }
+}
+
+__global__ void finish_multi_kernel(const double* __restrict__ part, int m, double* __restrict__ out) {
+  __shared__ double sh[256];
+  for (int k = 0; k < m; ++k) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) acc += part[k * kRedBlocks + i];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[k] = s;
+  }
+}
+
+int diis_dots(int o, int v, int m, const double* const* d_xs, const double* y, double* part, double* out,
+              hipStream_t s) {
+  ACES_REQUIRE(m >= 1 && m <= 8, "diis_dots: 1..8 vectors per call");
+  hipLaunchKernelGGL(diis_dot_partial_kernel, dim3(kRedBlocks), dim3(256), 0, s, o, v, m, d_xs, y, part);
+  hipLaunchKernelGGL(finish_multi_kernel, dim3(1), dim3(256), 0, s, part, m, out);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+// Signed value of the first element of largest magnitude of the DIIS vector in
+// the reference's order: [T1 (AI) | T2(Ab,Ij) list order | T2(A<B,I<J) packed]
+// (ISAMAX semantics, vcc/dodiis0.F:96-101).  Partial: (|x|, position, value).
+struct AbsMax {
+  double a;
+  long pos;
+  double val;
+};
+
+__device__ inline void am_update(AbsMax& m, double x, long pos) {
+  const double a = fabs(x);
+  if (a > m.a || (a == m.a && pos < m.pos)) {
+    m.a = a;
+    m.pos = pos;
+    m.val = x;
+  }
+}
+
+__global__ void absmax_partial_kernel(int o, int v, const double* __restrict__ e, AbsMax* __restrict__ part) {
+  __shared__ AbsMax sh[256];
+  const long n1 = (long)o * v, vv = (long)v * v, n2 = n1 * n1;
+  const long nom = (long)o * (o - 1) / 2, nvm = (long)v * (v - 1) / 2;
+  AbsMax m{-1.0, 0x7fffffffffffffffL, 0.0};
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n1 + n2;
+       idx += (long)gridDim.x * blockDim.x) {
+    if (idx < n1) {
+      am_update(m, e[idx], idx);
+    } else {
+      const long r = idx - n1;
+      const long ij = r / vv, ab = r - ij * vv;
+      const int i = (int)(ij / o), j = (int)(ij % o), a = (int)(ab / v), b = (int)(ab % v);
+      // list 46 position of element (A=a,b; I=i,j): a + v*b + v*v*(i + o*j)
+      const long pos46 = n1 + (long)a + (long)v * b + vv * ((long)i + (long)o * j);
+      am_update(m, e[idx], pos46);
+      if (i < j && a < b) {
+        const double x = e[idx] - e[n1 + ((long)j * o + i) * vv + ab];
+        const long pos44 = n1 + n2 + pk_lt(a, b) + nvm * pk_lt(i, j);
+        am_update(m, x, pos44);
+      }
+    }
+  }
+  (void)nom;
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      AbsMax o2 = sh[threadIdx.x + w];
+      AbsMax& mine = sh[threadIdx.x];
+      if (o2.a > mine.a || (o2.a == mine.a && o2.pos < mine.pos)) mine = o2;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void absmax_finish_kernel(const AbsMax* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ AbsMax sh[256];
+  AbsMax m{-1.0, 0x7fffffffffffffffL, 0.0};
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const AbsMax o2 = part[i];
+    if (o2.a > m.a || (o2.a == m.a && o2.pos < m.pos)) m = o2;
+  }
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      AbsMax o2 = sh[threadIdx.x + w];
+      AbsMax& mine = sh[threadIdx.x];
+      if (o2.a > mine.a || (o2.a == mine.a && o2.pos < mine.pos)) mine = o2;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0].val;
+}
+
+int diis_absmax(int o, int v, const double* e, void* part, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(absmax_partial_kernel, dim3(kRedBlocks), dim3(256), 0, s, o, v, e,
+                     reinterpret_cast<AbsMax*>(part));
+  hipLaunchKernelGGL(absmax_finish_kernel, dim3(1), dim3(256), 0, s, reinterpret_cast<const AbsMax*>(part),
+                     kRedBlocks, out);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+long reduction_scratch_bytes() { return (long)kRedBlocks * 8 * (long)sizeof(double) + kRedBlocks * sizeof(AbsMax); }
+
+// out[idx] = sum_k c[k] * x_k[idx] evaluated in the given order (k = 0..m-1).
+__global__ void lincomb_kernel(long n, int m, const double* const* __restrict__ xs, const double* __restrict__ c,
+                               double* __restrict__ out) {
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n; idx += (long)gridDim.x * blockDim.x) {
+    double acc = c[0] * xs[0][idx];
+    for (int k = 1; k < m; ++k) acc += c[k] * xs[k][idx];
+    out[idx] = acc;
+  }
+}
+
+int lincomb(long n, int m, const double* const* d_xs, const double* d_c, double* out, hipStream_t s) {
+  ACES_REQUIRE(m >= 1, "lincomb: empty");
+  hipLaunchKernelGGL(lincomb_kernel, dim3(grid1(n)), dim3(256), 0, s, n, m, d_xs, d_c, out);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+// y = x - z (error vector)
+__global__ void sub_kernel(long n, const double* __restrict__ x, const double* __restrict__ z,
+                           double* __restrict__ y) {
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n; idx += (long)gridDim.x * blockDim.x)
+    y[idx] = x[idx] - z[idx];
+}
+
+int sub(long n, const double* x, const double* z, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(sub_kernel, dim3(grid1(n)), dim3(256), 0, s, n, x, z, y);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+// Remove the diagonal: X[p][p] -= d[p]  (square n x n, leading dimension ld).
+__global__ void sub_diag_kernel(int n, double* __restrict__ X, long ld, const double* __restrict__ d) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) X[(long)p * ld + p] -= d[p];
+}
+
+int sub_diag(int n, double* X, long ld, const double* d, hipStream_t s) {
+  hipLaunchKernelGGL(sub_diag_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, X, ld, d);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+}  // namespace aces

@@ -1,0 +1,42 @@
+// MFMA GEMM engine for gfx950 (CDNA4).
+//
+//   C[m, n] = alpha * sum_k A(m, k) * B(n, k) + beta * C[m, n]      (C is fp64)
+//
+// A(m, k) is A[m*lda + k] when a_kc (k contiguous) else A[k*lda + m];
+// B(n, k) is B[n*ldb + k] when b_kc (k contiguous) else B[k*ldb + n].
+// Batched with element strides sA, sB, sC.
+//
+// Precision modes:
+//   GP_F64    fp64 operands, v_mfma_f64_16x16x4_f64, fp64 accumulation
+//   GP_F32    fp32 operands, v_mfma_f32_16x16x4_f32, fp32 accumulation
+//   GP_BF16X3 operands split a = hi + lo (bf16 each), three products
+//             hi*hi + hi*lo + lo*hi per k on v_mfma_f32_16x16x32_bf16 with
+//             fp32 accumulation (Henry, Tang, Heinecke, ARITH 2019).
+//             k-contiguous operands only, lda/ldb multiples of 8, K multiple of 8.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace aces {
+
+enum GemmPrec : int { GP_F64 = 0, GP_F32 = 1, GP_BF16X3 = 2 };
+
+struct GemmDesc {
+  int M = 0, N = 0, K = 0;
+  const void* A = nullptr;
+  const void* A_lo = nullptr;  // bf16x3 only
+  long lda = 0;
+  int a_kc = 1;
+  const void* B = nullptr;
+  const void* B_lo = nullptr;  // bf16x3 only
+  long ldb = 0;
+  int b_kc = 1;
+  double* C = nullptr;
+  long ldc = 0;
+  double alpha = 1.0, beta = 0.0;
+  int batch = 1;
+  long sA = 0, sB = 0, sC = 0;
+};
+
+int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream);
+
+}  // namespace aces

@@ -1,0 +1,48 @@
+// HBM-bound tensor kernels: conversions, permute/combine, packing, reductions.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace aces {
+
+// y[r*ldy + c] = (float) x[r*ldx + c] for c < cols; columns cols..ldy-1 zeroed.
+int convert_f32(int rows, int cols, const double* x, long ldx, float* y, long ldy, hipStream_t s);
+// hi = bf16(x), lo = bf16(x - hi) (round to nearest even); padding columns zeroed.
+int split_bf16(int rows, int cols, const double* x, long ldx, unsigned short* hi, unsigned short* lo,
+               long ldy, hipStream_t s);
+
+// Generic 4-index permute/combine on a row-major output of dims d[0..3]:
+//   out[i] = beta*out[i] + sum_s alpha[s] * in[s][ sum_k i_k * st[s][k] ]
+//            + gamma * X[ sum_k i_k * xs[k] ] * Y[ sum_k i_k * ys[k] ]
+// Unused sources have in[s] == nullptr; the product term is skipped when X == nullptr.
+struct Permute4 {
+  long d[4] = {1, 1, 1, 1};
+  double* out = nullptr;
+  double beta = 0.0;
+  const double* in[3] = {nullptr, nullptr, nullptr};
+  double alpha[3] = {0.0, 0.0, 0.0};
+  long st[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+  const double* X = nullptr;
+  const double* Y = nullptr;
+  double gamma = 0.0;
+  long xs[4] = {0, 0, 0, 0};
+  long ys[4] = {0, 0, 0, 0};
+};
+int permute4(const Permute4& p, hipStream_t s);
+
+// y[m] = alpha * sum_k A[m*lda + k] x[k] + beta * y[m]
+int gemv(int M, int K, const double* A, long lda, const double* x, double* y, double alpha, double beta,
+         hipStream_t s);
+// y[n] = alpha * sum_k A[k*lda + n] x[k] + beta * y[n]   (column access, coalesced over n)
+int gemv_t(int K, int N, const double* A, long lda, const double* x, double* y, double alpha, double beta,
+           hipStream_t s);
+
+// Two-index strided contraction over a 4-index array:
+//   out[p*P1 + q] = beta*out + alpha * sum_{r<R, s<S} X[p*xp + q*xq + r*xr + s*xs] * Y[r*yr + s*ys]
+int contract2(int P, int Q, int R, int S, const double* X, long xp, long xq, long xr, long xs,
+              const double* Y, long yr, long ys, double* out, double alpha, double beta, hipStream_t s);
+
+// Partial-sum reductions into device scalars (results left in `result`).
+// dot = sum x[i]*y[i]
+int dot(long n, const double* x, const double* y, double* result, hipStream_t s);
+
+}  // namespace aces

@@ -1,0 +1,84 @@
+/*
+ * aces2_amd.h -- C ABI of the MI355X (gfx950) CCSD amplitude library.
+ *
+ * Drop-in boundary for the closed-shell CCSD amplitude path of ACES II
+ * ``xvcc``.  All functions return 0 on success and a nonzero status code
+ * otherwise; the message of the last failure is available from
+ * aces_last_error().  Pointers named d_* are device pointers; ``stream`` is a
+ * hipStream_t passed as void* (NULL = default stream).  No torch types cross
+ * this boundary.
+ */
+#ifndef ACES2_AMD_H
+#define ACES2_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define ACES_OK 0
+#define ACES_ERR_HIP 1
+#define ACES_ERR_ARG 2
+#define ACES_ERR_STATE 3
+#define ACES_ERR_NOMEM 4
+#define ACES_ERR_RANGE 5
+#define ACES_ERR_NOCONV 6
+#define ACES_ERR_SOLVE 7
+
+/* Precision of the dense contractions. */
+#define ACES_PREC_F64 0
+#define ACES_PREC_F32 1
+#define ACES_PREC_BF16X3 2
+
+const char* aces_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Contraction engine.  Replaces the XGEMM funnel every xvcc contraction
+ * goes through (tools/xgemm.F:101 -> dgemm; crust/xgemm.F:30-48).
+ *   C[m*ldc+n] = alpha * sum_k A(m,k) B(n,k) + beta * C[m*ldc+n]
+ *   A(m,k) = a_kc ? A[m*lda+k] : A[k*lda+m];  B(n,k) = b_kc ? B[n*ldb+k] : B[k*ldb+n]
+ * prec ACES_PREC_BF16X3 takes bf16 hi/lo operand pairs (A, A_lo, B, B_lo).
+ * ---------------------------------------------------------------------- */
+int aces_gemm(int prec, int M, int N, int K, const void* d_A, const void* d_A_lo, long lda, int a_kc,
+              const void* d_B, const void* d_B_lo, long ldb, int b_kc, double* d_C, long ldc, double alpha,
+              double beta, int batch, long sA, long sB, long sC, void* stream);
+int aces_convert_f32(int rows, int cols, const double* d_x, long ldx, float* d_y, long ldy, void* stream);
+int aces_split_bf16(int rows, int cols, const double* d_x, long ldx, unsigned short* d_hi, unsigned short* d_lo,
+                    long ldy, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Closed-shell CCSD amplitude engine (RHF, one GPU).  Replaces the body of the
+ * xvcc CC loop (vcc/vcc.F:798-1437) for CCSD: GENINT (vcc/genint.F:1),
+ * FEACONT/FMICONT/FMECONT/T1INT1/T1INT2A/T1INT2B (vcc/vcc.F:885-920), DRE3EN
+ * (vcc/dre3en.F: DRLAD vcc/drlad.f, DRRNG vcc/drrng.f, T12INT2), E4S
+ * (vcc/e4s.F), NEWT2 (libr2/newt2.F:233-244), CMPENG (libr/cmpeng.F),
+ * DODIIS0 (vcc/dodiis0.F, libr/dodiis.F) and DRMOVE (libr/drmove.f).
+ *
+ * Integrals (device, row major, chemists' notation, n = o + v):
+ *   fock[n][n], ovov[i][a][j][b]=(ia|jb), oovv[i][j][a][b]=(ij|ab),
+ *   ovoo[i][a][j][k]=(ia|jk), oooo[i][j][k][l]=(ij|kl), ovvv[i][a][b][c]=(ia|bc),
+ *   list233 = the reference's compressed RHF <Ab|Cd> list (rows A<=B, columns
+ *   Cd, column major; intprc/intproc.F:250-257).
+ * Amplitudes: t1[i][a], t2[i][j][a][b] = T(Ab,Ij) (list 46 storage for C1).
+ * stats (host double[7]) from aces_ccsd_cycle: E(T_new) printed by CMPENG,
+ * RESMITER, RESMDIIS, converged flag, final energy (valid when converged),
+ * ms in the increments, ms in the whole cycle.
+ * ---------------------------------------------------------------------- */
+int aces_ccsd_create(int o, int v, void* stream, void** handle);
+int aces_ccsd_destroy(void* handle);
+int aces_ccsd_setup(void* handle, const double* d_fock, const double* d_ovov, const double* d_oovv,
+                    const double* d_ovoo, const double* d_oooo, const double* d_ovvv, const double* d_list233);
+int aces_ccsd_set_amplitudes(void* handle, const double* d_t1, const double* d_t2);
+int aces_ccsd_get_amplitudes(void* handle, double* d_t1, double* d_t2);
+int aces_ccsd_start_mp1(void* handle);
+int aces_ccsd_increments(void* handle, int prec, double* d_inc1, double* d_inc2);
+int aces_ccsd_cycle(void* handle, int icycle, int prec, double tol, double* stats);
+int aces_ccsd_set_diis_order(void* handle, int order);
+int aces_ccsd_reset_diis(void* handle);
+int aces_ccsd_energy(void* handle, const double* d_t1, const double* d_t2, double* e_out);
+long aces_ccsd_device_bytes(void* handle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ACES2_AMD_H */

@@ -1,0 +1,84 @@
+"""GPU CCSD engine (through the C ABI) against the CPU oracle."""
+import numpy as np
+import pytest
+
+from oracle import rccsd, vcc_loop
+
+torch = pytest.importorskip('torch')
+pytestmark = pytest.mark.gpu
+
+
+def _system(o, v, seed, nonhf=False):
+    from aces2 import synth
+    eps, B = synth.draw(o, v, seed, sigma=synth.scaled_sigma(o, v))
+    pert = None
+    if nonhf:
+        rng = np.random.default_rng(seed + 100)
+        P = rng.normal(0, 0.01, (o + v, o + v))
+        pert = P + P.T
+    blk, eri = synth.blocks_numpy(eps, B, o, perturb=pert)
+    return blk, eri
+
+
+def _random_t(o, v, seed, scale=0.05):
+    rng = np.random.default_rng(seed)
+    t1 = rng.normal(0, scale, (o, v))
+    t2 = rng.normal(0, scale, (o, o, v, v))
+    t2 = 0.5 * (t2 + t2.transpose(1, 0, 3, 2))
+    return t1, t2
+
+
+@pytest.mark.parametrize('o,v,seed,nonhf', [(1, 3, 1, False), (2, 3, 2, True), (5, 19, 3, False),
+                                            (5, 19, 4, True), (7, 40, 5, True), (12, 64, 6, False)])
+def test_increments_fp64(o, v, seed, nonhf):
+    from aces2.ccsd import CCSD
+    print('case increments', o, v, seed, nonhf, flush=True)
+    blk, eri = _system(o, v, seed, nonhf)
+    cc = CCSD(o, v, blk)
+    t1, t2 = _random_t(o, v, seed + 7)
+    cc.set_amplitudes(t1, t2)
+    i1, i2 = cc.increments('fp64')
+    r1, r2 = rccsd.increments(t1, t2, blk['fock'], rccsd.blocks(eri, o), o)
+    s1 = max(np.abs(r1).max(), 1e-30)
+    s2 = np.abs(r2).max()
+    assert np.abs(i1.cpu().numpy() - r1).max() / s1 < 1e-11
+    assert np.abs(i2.cpu().numpy() - r2).max() / s2 < 1e-11
+    e_gpu = cc.energy(t1, t2)
+    e_ref = rccsd.energy(t1, t2, blk['fock'], rccsd.blocks(eri, o), o)
+    assert abs(e_gpu - e_ref) < 1e-12 * max(1.0, abs(e_ref))
+    cc.close()
+
+
+@pytest.mark.parametrize('prec,tol', [('fp32', 2e-6), ('bf16x3', 5e-5)])
+def test_increments_lowp(prec, tol):
+    from aces2.ccsd import CCSD
+    print('case lowp', prec, flush=True)
+    o, v = 8, 48
+    blk, eri = _system(o, v, 11, True)
+    cc = CCSD(o, v, blk)
+    t1, t2 = _random_t(o, v, 12)
+    cc.set_amplitudes(t1, t2)
+    i1, i2 = cc.increments(prec)
+    r1, r2 = rccsd.increments(t1, t2, blk['fock'], rccsd.blocks(eri, o), o)
+    assert np.abs(i2.cpu().numpy() - r2).max() / np.abs(r2).max() < tol
+    assert np.abs(i1.cpu().numpy() - r1).max() / np.abs(r1).max() < tol
+    cc.close()
+
+
+@pytest.mark.parametrize('o,v,seed,nonhf', [(5, 19, 21, False), (6, 30, 22, True)])
+def test_cc_loop_matches_oracle(o, v, seed, nonhf):
+    """Per-cycle energies, RESMITER and RESMDIIS follow the oracle's reference loop."""
+    from aces2.ccsd import CCSD
+    print('case loop', o, v, seed, nonhf, flush=True)
+    blk, eri = _system(o, v, seed, nonhf)
+    g = rccsd.blocks(eri, o)
+    ref = vcc_loop.run(blk['fock'], g, o, maxcyc=60, conv=10, order=6, start='mp1')
+    cc = CCSD(o, v, blk)
+    hist = cc.run(maxcyc=60, conv=10, order=6, start='mp1')
+    assert hist['converged'] and ref['converged']
+    assert hist['cycles'] == ref['cycles']
+    np.testing.assert_allclose(hist['energy'], ref['energy'], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(hist['resmiter'], ref['resmiter'], rtol=1e-6, atol=1e-13)
+    np.testing.assert_allclose(hist['resmdiis'], ref['resmdiis'], rtol=1e-5, atol=1e-12)
+    assert abs(hist['final_energy'] - ref['final_energy']) < 1e-10
+    cc.close()
