@@ -38,6 +38,8 @@ SIGNATURES = {
     'aces_ccsd_reset_diis': (c_int, [c_vp]),
     'aces_ccsd_energy': (c_int, [c_vp, c_vp, c_vp, c_dp]),
     'aces_ccsd_device_bytes': (c_long, [c_vp]),
+    'aces_ccsd_set_profiling': (c_int, [c_vp, c_int]),
+    'aces_ccsd_profile': (c_int, [c_vp, c_dp, c_int]),
 }
 
 

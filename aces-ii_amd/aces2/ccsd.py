@@ -81,6 +81,16 @@ class CCSD:
         _lib.call('aces_ccsd_energy', self.h, t1.data_ptr(), t2.data_ptr(), ctypes.byref(e))
         return e.value
 
+    def set_profiling(self, on=True):
+        _lib.call('aces_ccsd_set_profiling', self.h, 1 if on else 0)
+
+    def profile(self, reset=True):
+        out = (ctypes.c_double * 9)()
+        _lib.call('aces_ccsd_profile', self.h, out, 1 if reset else 0)
+        names = ('ladder', 'wbuild', 'ring')
+        return {nm: {'ms': out[3 * c], 'flops': out[3 * c + 1], 'launches': int(out[3 * c + 2])}
+                for c, nm in enumerate(names)}
+
     def set_diis_order(self, order):
         _lib.call('aces_ccsd_set_diis_order', self.h, order)
 

@@ -95,4 +95,16 @@ int aces_ccsd_energy(void* h, const double* t1, const double* t2, double* e) {
 
 long aces_ccsd_device_bytes(void* h) { return h ? eng(h)->device_bytes() : 0; }
 
+int aces_ccsd_set_profiling(void* h, int on) {
+  REQUIRE_HANDLE(h);
+  eng(h)->set_profiling(on != 0);
+  return ACES_OK;
+}
+
+int aces_ccsd_profile(void* h, double* out, int reset) {
+  REQUIRE_HANDLE(h);
+  eng(h)->profile(out, reset);
+  return ACES_OK;
+}
+
 }  // extern "C"

@@ -44,6 +44,10 @@ class CcsdEngine {
   int reset_diis();
   int set_diis_order(int order);
   int energy_of(const double* t1, const double* t2, double& e);
+  // Per-launch timing of the MFMA contractions (hipEvents on the engine's stream).
+  // out[3*c + 0..2] = {ms, algorithmic flops, launches} for c = 0 ladder, 1 W build, 2 ring.
+  void set_profiling(bool on) { profile_ = on; }
+  void profile(double* out, int reset);
   long device_bytes() const { return bytes_; }
 
   int o, v, n;
@@ -55,9 +59,16 @@ class CcsdEngine {
   int lowp_static(GemmPrec prec);
   int gemm_lp(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda, const double* B64,
               int bslot, long ldb, double* C, long ldc, double alpha, double beta);
+  int timed_gemm(const GemmDesc& d, GemmPrec prec, double flops);
+  int gemm64(GemmDesc d);  // fp64 GEMM with the engine's split-K workspace
+  int gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda, const double* B64,
+                   int bslot, long ldb, double* C, long ldc, double alpha, double beta);
 
   hipStream_t s_;
   long bytes_ = 0;
+  bool profile_ = false;
+  double prof_[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int prof_cat_ = 0;
   std::vector<void*> allocs_;
   bool ready_ = false;
 
@@ -76,6 +87,8 @@ class CcsdEngine {
   double *Wo_, *Qo_, *ZH_, *tmpL_;
   double *Fov_, *Foo_, *Fvv_, *Loo_, *Lvv_, *FT_, *FovT_, *Gk_;
   double *red_ = nullptr;  // reduction scratch
+  double *ws_ = nullptr;   // split-K workspace
+  long ws_elems_ = 0;
   double *scal_ = nullptr; // device scalars
   // low precision operand slots: 0 Vp 1 Vm 2 Lmat 3 Kx (static); 4 Sp 5 Dp 6 Ut 7 Xt 8 Wt 9 Wv 10 U 11 X
   struct LowP {

@@ -241,6 +241,8 @@ int CcsdEngine::setup(const double* fock, const double* ovov, const double* oovv
   ACES_CHECK(alloc(&FovT_, oo));
   ACES_CHECK(alloc(&Gk_, oo));
   ACES_CHECK(alloc_raw((void**)&red_, reduction_scratch_bytes()));
+  ws_elems_ = 1L << 25;  // split-K workspace: 32 Mi doubles
+  ACES_CHECK(alloc(&ws_, ws_elems_));
   ACES_CHECK(alloc(&scal_, 64));
   ACES_CHECK(alloc(&err_cur_, nt));
   ACES_CHECK(alloc(&newvec_, nt));
@@ -340,9 +342,48 @@ int CcsdEngine::lowp_static(GemmPrec prec) {
   return ACES_OK;
 }
 
+int CcsdEngine::gemm64(GemmDesc d) {
+  d.ws = ws_;
+  d.ws_elems = ws_elems_;
+  return gemm(d, GP_F64, s_);
+}
+
+void CcsdEngine::profile(double* out, int reset) {
+  for (int i = 0; i < 9; ++i) out[i] = prof_[i];
+  if (reset)
+    for (int i = 0; i < 9; ++i) prof_[i] = 0.0;
+}
+
 int CcsdEngine::gemm_lp(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda,
                         const double* B64, int bslot, long ldb, double* C, long ldc, double alpha, double beta) {
-  if (prec == GP_F64) return gemm(gd(M, N, K, A64, lda, 1, B64, ldb, 1, C, ldc, alpha, beta), GP_F64, s_);
+  return gemm_lp_impl(prec, M, N, K, A64, aslot, lda, B64, bslot, ldb, C, ldc, alpha, beta);
+}
+
+// Launch one MFMA contraction, timed with hipEvents when profiling is on.
+int CcsdEngine::timed_gemm(const GemmDesc& d, GemmPrec prec, double flops) {
+  if (!profile_) return gemm(d, prec, s_);
+  hipEvent_t ev[2];
+  ACES_HIP(hipEventCreate(&ev[0]));
+  ACES_HIP(hipEventCreate(&ev[1]));
+  ACES_HIP(hipEventRecord(ev[0], s_));
+  ACES_CHECK(gemm(d, prec, s_));
+  ACES_HIP(hipEventRecord(ev[1], s_));
+  ACES_HIP(hipEventSynchronize(ev[1]));
+  float ms = 0.f;
+  ACES_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+  prof_[3 * prof_cat_ + 0] += ms;
+  prof_[3 * prof_cat_ + 1] += flops;
+  prof_[3 * prof_cat_ + 2] += 1.0;
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  return ACES_OK;
+}
+
+int CcsdEngine::gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda,
+                             const double* B64, int bslot, long ldb, double* C, long ldc, double alpha,
+                             double beta) {
+  const double flops = 2.0 * M * (double)N * K;
+  if (prec == GP_F64) return timed_gemm(gd(M, N, K, A64, lda, 1, B64, ldb, 1, C, ldc, alpha, beta), GP_F64, flops);
   ACES_CHECK(lowp_static(prec));
   const long Kp = pad8(std::max(K, 1));
   const void* ops[2][2];
@@ -375,7 +416,7 @@ int CcsdEngine::gemm_lp(GemmPrec prec, int M, int N, int K, const double* A64, i
   d.A = ops[0][0]; d.A_lo = ops[0][1]; d.lda = lp_[aslot].ld; d.a_kc = 1;
   d.B = ops[1][0]; d.B_lo = ops[1][1]; d.ldb = lp_[bslot].ld; d.b_kc = 1;
   d.C = C; d.ldc = ldc; d.alpha = alpha; d.beta = beta;
-  return gemm(d, prec, s_);
+  return timed_gemm(d, prec, flops);
 }
 
 // ---------------------------------------------------------------------------
@@ -398,10 +439,10 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(P4(o, v, o, v, Xt_).src(X_, 1, ov2, ov, v, 1).prod(t1, 0, 1, v, 0, t1, v, 0, 0, 1, 2.0).run(s));
 
   // T1 parts of the W(mbej) intermediates
-  ACES_CHECK(gemm(gd((int)v2o, o, v, ovvv_, v, 1, t1, v, 1, P1_, o, 1.0, 0.0), GP_F64, s));        // [(k,c,a)][i]
-  ACES_CHECK(gemm(gd((int)v2o, o, v, ovvv_kacd_, v, 1, t1, v, 1, P2_, o, 1.0, 0.0), GP_F64, s));   // [(k,a,c)][i]
-  ACES_CHECK(gemm(gd((int)o2v, v, o, M5c_, o, 1, t1, v, 0, Q12_, v, 1.0, 0.0), GP_F64, s));        // [(k,c,i)][a]
-  ACES_CHECK(gemm(gd((int)o2v, v, o, M5b_, o, 1, t1, v, 0, Q2_, v, 1.0, 0.0), GP_F64, s));         // [(k,c,i)][a]
+  ACES_CHECK(gemm64(gd((int)v2o, o, v, ovvv_, v, 1, t1, v, 1, P1_, o, 1.0, 0.0)));        // [(k,c,a)][i]
+  ACES_CHECK(gemm64(gd((int)v2o, o, v, ovvv_kacd_, v, 1, t1, v, 1, P2_, o, 1.0, 0.0)));   // [(k,a,c)][i]
+  ACES_CHECK(gemm64(gd((int)o2v, v, o, M5c_, o, 1, t1, v, 0, Q12_, v, 1.0, 0.0)));        // [(k,c,i)][a]
+  ACES_CHECK(gemm64(gd((int)o2v, v, o, M5b_, o, 1, t1, v, 0, Q2_, v, 1.0, 0.0)));         // [(k,c,i)][a]
   // Wt[(i,a)][(k,c)] = W0t + 2 P1[k,c,a,i] - P2[k,a,c,i] - Q12[k,c,i,a]
   ACES_CHECK(P4(o, v, o, v, Wt_).src(W0t_, 1, ov2, ov, v, 1).src(P1_, 2, 1, o, v2o, v * o)
                  .src(P2_, -1, 1, v * o, v2o, o).run(s));
@@ -410,9 +451,11 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(P4(o, v, o, v, Wv_).src(Wv0_, 1, ov2, ov, v, 1).src(P2_, 1, 1, v * o, v2o, o)
                  .src(Q2_, -1, v, 1, ov2, ov).run(s));
   // T2 parts (MFMA): Wt += 1/2 Ut L^T ; Wv += -1/2 Xt K^T
+  prof_cat_ = 1;
   ACES_CHECK(gemm_lp(prec, OV, OV, OV, Ut_, 6, ov, Lmat_, 2, ov, Wt_, ov, 0.5, 1.0));
   ACES_CHECK(gemm_lp(prec, OV, OV, OV, Xt_, 7, ov, Kx_, 3, ov, Wv_, ov, -0.5, 1.0));
   // ring (MFMA): R1 = Wt U^T ; R2 = Wv X^T
+  prof_cat_ = 2;
   ACES_CHECK(gemm_lp(prec, OV, OV, OV, Wt_, 8, ov, U_, 10, ov, R1_, ov, 1.0, 0.0));
   ACES_CHECK(gemm_lp(prec, OV, OV, OV, Wv_, 9, ov, X_, 11, ov, R2_, ov, 1.0, 0.0));
 
@@ -426,6 +469,7 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
 
   // particle-particle ladder with the T1-dressing rows (MFMA)
   ACES_CHECK(pack_tau(o, v, tau_, Sp_, nvp_, Dp_, std::max<long>(nvm_, 1), s));
+  prof_cat_ = 0;
   ACES_CHECK(gemm_lp(prec, (int)(nvp_ + ov), (int)nop_, (int)nvp_, Vp_, 0, nvp_, Sp_, 4, nvp_, Zp_, nop_, 1.0, 0.0));
   if (nom_ > 0 && nvm_ > 0)
     ACES_CHECK(gemm_lp(prec, (int)(nvm_ + ov), (int)nom_, (int)nvm_, Vm_, 1, nvm_, Dp_, 5, nvm_, Zm_, nom_, 1.0,
@@ -434,15 +478,15 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
     ACES_HIP(hipMemsetAsync(Zm_, 0, sizeof(double), s));
   ACES_CHECK(unpack_ladder(o, v, Zp_, nop_, Zm_, std::max<long>(nom_, 1), inc2_, Yb_, s));
   // L1[a][(b,i,j)] = sum_k t_ka Yb[(b,i,j)][k] ; inc2 -= L1[a,b,i,j] + L1[b,a,j,i]
-  ACES_CHECK(gemm(gd(v, (int)(v * oo), o, t1, v, 0, Yb_, o, 1, L1_, v * oo, 1.0, 0.0), GP_F64, s));
+  ACES_CHECK(gemm64(gd(v, (int)(v * oo), o, t1, v, 0, Yb_, o, 1, L1_, v * oo, 1.0, 0.0)));
   ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(L1_, -1, o, 1, vo2, oo).src(L1_, -1, 1, o, oo, vo2).run(s));
 
   // hole-hole ladder: Wo = (ki|lj) + Qo[klij] + Qo[lkji] + (kc|ld) tau_ijcd ; inc2 += tau_klab Wo_klij
-  ACES_CHECK(gemm(gd((int)(oo * o), o, v, ovooQ_, v, 1, t1, v, 1, Qo_, o, 1.0, 0.0), GP_F64, s));
+  ACES_CHECK(gemm64(gd((int)(oo * o), o, v, ovooQ_, v, 1, t1, v, 1, Qo_, o, 1.0, 0.0)));
   ACES_CHECK(P4(o, o, o, o, Wo_).src(ooooP_, 1, oo * o, oo, o, 1).src(Qo_, 1, oo * o, oo, o, 1)
                  .src(Qo_, 1, oo, oo * o, 1, o).run(s));
-  ACES_CHECK(gemm(gd((int)oo, (int)oo, (int)vv, OVkl_, vv, 1, tau_, vv, 1, Wo_, oo, 1.0, 1.0), GP_F64, s));
-  ACES_CHECK(gemm(gd((int)vv, (int)oo, (int)oo, tau_, vv, 0, Wo_, oo, 0, ZH_, oo, 1.0, 0.0), GP_F64, s));
+  ACES_CHECK(gemm64(gd((int)oo, (int)oo, (int)vv, OVkl_, vv, 1, tau_, vv, 1, Wo_, oo, 1.0, 1.0)));
+  ACES_CHECK(gemm64(gd((int)vv, (int)oo, (int)oo, tau_, vv, 0, Wo_, oo, 0, ZH_, oo, 1.0, 0.0)));
   ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(ZH_, 1, o, 1, vo2, oo).run(s));
 
   // F intermediates (QUAD2/QUAD3/MAKFME + T1RING pieces)
@@ -450,16 +494,16 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(P4(1, 1, o, v, Fov_).src(fov, 1, 0, 0, n, 1).run(s));
   ACES_CHECK(gemv(OV, OV, Lmat_, ov, t1, Fov_, 1.0, 1.0, s));
   ACES_CHECK(P4(1, 1, o, o, Foo_).src(fock_, 1, 0, 0, n, 1).run(s));
-  ACES_CHECK(gemm(gd(o, o, (int)ov2, Lo_, ov2, 1, tau_, ov2, 1, Foo_, o, 1.0, 1.0), GP_F64, s));
+  ACES_CHECK(gemm64(gd(o, o, (int)ov2, Lo_, ov2, 1, tau_, ov2, 1, Foo_, o, 1.0, 1.0)));
   ACES_CHECK(P4(v, o, o, v, tauK_).src(tau_, 1, v, ov2, vv, 1).run(s));  // tauK[a][k][l][d] = tau_klad
   ACES_CHECK(P4(1, 1, v, v, Fvv_).src(fock_ + (long)o * n + o, 1, 0, 0, n, 1).run(s));
-  ACES_CHECK(gemm(gd(v, v, (int)(oo * v), tauK_, oo * v, 1, Lv_, oo * v, 1, Fvv_, v, -1.0, 1.0), GP_F64, s));
-  ACES_CHECK(gemm(gd(o, o, v, fov, n, 1, t1, v, 1, FT_, o, 1.0, 0.0), GP_F64, s));       // FT[k][i] = f_kc t_ic
-  ACES_CHECK(gemm(gd(o, o, v, Fov_, v, 1, t1, v, 1, FovT_, o, 1.0, 0.0), GP_F64, s));    // FovT[k][i]
+  ACES_CHECK(gemm64(gd(v, v, (int)(oo * v), tauK_, oo * v, 1, Lv_, oo * v, 1, Fvv_, v, -1.0, 1.0)));
+  ACES_CHECK(gemm64(gd(o, o, v, fov, n, 1, t1, v, 1, FT_, o, 1.0, 0.0)));       // FT[k][i] = f_kc t_ic
+  ACES_CHECK(gemm64(gd(o, o, v, Fov_, v, 1, t1, v, 1, FovT_, o, 1.0, 0.0)));    // FovT[k][i]
   ACES_CHECK(P4(1, 1, o, o, Loo_).src(Foo_, 1, 0, 0, o, 1).src(FT_, 1, 0, 0, o, 1).run(s));
   ACES_CHECK(gemv((int)oo, OV, M1_, ov, t1, Loo_, 1.0, 1.0, s));
   ACES_CHECK(P4(1, 1, v, v, Lvv_).src(Fvv_, 1, 0, 0, v, 1).run(s));
-  ACES_CHECK(gemm(gd(v, v, o, t1, v, 0, fov, n, 0, Lvv_, v, -1.0, 1.0), GP_F64, s));
+  ACES_CHECK(gemm64(gd(v, v, o, t1, v, 0, fov, n, 0, Lvv_, v, -1.0, 1.0)));
   ACES_CHECK(gemv_t(OV, (int)vv, ovvv_, vv, t1, Lvv_, 2.0, 1.0, s));
   ACES_CHECK(contract2(v, v, o, v, ovvv_kacd_, vv, 1, vv * v, v, t1, v, 1, Lvv_, -1.0, 1.0, s));
   // diagonal (orbital energies) moved into the denominators
@@ -475,33 +519,33 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
     d.sA = 0;
     d.sB = vv;
     d.sC = vv;
-    ACES_CHECK(gemm(d, GP_F64, s));
+    ACES_CHECK(gemm64(d));
   }
-  ACES_CHECK(gemm(gd(o, (int)ov2, o, Loo_, o, 0, t2, ov2, 0, tmpL_, ov2, -1.0, 1.0), GP_F64, s));
+  ACES_CHECK(gemm64(gd(o, (int)ov2, o, Loo_, o, 0, t2, ov2, 0, tmpL_, ov2, -1.0, 1.0)));
   ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(tmpL_, 1, ov2, vv, v, 1).src(tmpL_, 1, vv, ov2, 1, v).run(s));
 
   // T1INT2A/T1INT2B/T12INT2 pieces:
   //   inc2 += P1[iabj] - T1Q[aibj] + P1[jbai] - T1Q[bjai]
-  ACES_CHECK(gemm(gd((int)o2v, o, v, oovv_, v, 1, t1, v, 1, Qb_, o, 1.0, 0.0), GP_F64, s));         // [(k,i,b)][j]
-  ACES_CHECK(gemm(gd(v, (int)o2v, o, t1, v, 0, Qb_, o2v, 0, T1Q_, o2v, 1.0, 0.0), GP_F64, s));     // [a][(i,b,j)]
+  ACES_CHECK(gemm64(gd((int)o2v, o, v, oovv_, v, 1, t1, v, 1, Qb_, o, 1.0, 0.0)));         // [(k,i,b)][j]
+  ACES_CHECK(gemm64(gd(v, (int)o2v, o, t1, v, 0, Qb_, o2v, 0, T1Q_, o2v, 1.0, 0.0)));     // [a][(i,b,j)]
   ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(P1_, 1, v2o, 1, v * o, o)
                  .src(T1Q_, -1, v * o, 1, o2v, o).src(P1_, 1, 1, v2o, o, v * o).run(s));
   ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(T1Q_, -1, 1, v * o, o, o2v).run(s));
   //   inc2 -= G4[iajb] + G4[jbia],  G4 = (ovoo[iajk] + S3[iakj]) t_kb
-  ACES_CHECK(gemm(gd((int)o2v, o, v, ovov_, v, 1, t1, v, 1, S3_, o, 1.0, 0.0), GP_F64, s));          // [(i,a,k)][j]
+  ACES_CHECK(gemm64(gd((int)o2v, o, v, ovov_, v, 1, t1, v, 1, S3_, o, 1.0, 0.0)));          // [(i,a,k)][j]
   ACES_CHECK(P4(o, v, o, o, G4A_).src(ovoo_, 1, vo2, oo, o, 1).src(S3_, 1, vo2, oo, 1, o).run(s));
-  ACES_CHECK(gemm(gd((int)o2v, v, o, G4A_, o, 1, t1, v, 0, G4_, v, 1.0, 0.0), GP_F64, s));         // [(i,a,j)][b]
+  ACES_CHECK(gemm64(gd((int)o2v, v, o, G4A_, o, 1, t1, v, 0, G4_, v, 1.0, 0.0)));         // [(i,a,j)][b]
   ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(G4_, -1, ov2, v, ov, 1).src(G4_, -1, v, ov2, 1, ov).run(s));
 
   // ---- T1 increment ----
   ACES_CHECK(P4(1, 1, o, v, inc1_).src(fov, 1, 0, 0, n, 1).run(s));
   ACES_CHECK(P4(1, 1, o, o, Gk_).src(Foo_, 1, 0, 0, o, 1).src(FT_, 2, 0, 0, o, 1).src(FovT_, -1, 0, 0, o, 1).run(s));
-  ACES_CHECK(gemm(gd(o, v, o, Gk_, o, 0, t1, v, 0, inc1_, v, -1.0, 1.0), GP_F64, s));
-  ACES_CHECK(gemm(gd(o, v, v, t1, v, 1, Fvv_, v, 1, inc1_, v, 1.0, 1.0), GP_F64, s));
+  ACES_CHECK(gemm64(gd(o, v, o, Gk_, o, 0, t1, v, 0, inc1_, v, -1.0, 1.0)));
+  ACES_CHECK(gemm64(gd(o, v, v, t1, v, 1, Fvv_, v, 1, inc1_, v, 1.0, 1.0)));
   ACES_CHECK(gemv(OV, OV, U_, ov, Fov_, inc1_, 1.0, 1.0, s));
   ACES_CHECK(gemv(OV, OV, W0t_, ov, t1, inc1_, 1.0, 1.0, s));
-  ACES_CHECK(gemm(gd(o, v, (int)ov2, tau_, ov2, 1, M3_, ov2, 1, inc1_, v, 1.0, 1.0), GP_F64, s));
-  ACES_CHECK(gemm(gd(o, v, (int)o2v, M7_, o2v, 1, tauK_, o2v, 1, inc1_, v, -1.0, 1.0), GP_F64, s));
+  ACES_CHECK(gemm64(gd(o, v, (int)ov2, tau_, ov2, 1, M3_, ov2, 1, inc1_, v, 1.0, 1.0)));
+  ACES_CHECK(gemm64(gd(o, v, (int)o2v, M7_, o2v, 1, tauK_, o2v, 1, inc1_, v, -1.0, 1.0)));
   return ACES_OK;
 }
 

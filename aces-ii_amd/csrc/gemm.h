@@ -35,6 +35,12 @@ struct GemmDesc {
   double alpha = 1.0, beta = 0.0;
   int batch = 1;
   long sA = 0, sB = 0, sC = 0;
+  // Optional split-K workspace (fp64 elements).  When the output has too few
+  // tiles to fill the chip and K is long, K is split over blockIdx.z; partial
+  // products go to the workspace and a deterministic reduction adds them in
+  // split order.
+  double* ws = nullptr;
+  long ws_elems = 0;
 };
 
 int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream);

@@ -30,6 +30,11 @@ struct GemmArgs {
   double alpha, beta;
   long sA, sB, sC;
   int tiles_m, tiles_n;
+  // split-K: blockIdx.z selects k range [z*kchunk, min(K, (z+1)*kchunk)); the
+  // operand pointers advance by z*kchunk*zkA / zkB and C by z*zC.
+  long kchunk;
+  long zkA, zkB, zC;
+  int vecA, vecB;  // 16-byte aligned operand rows (vector load path allowed)
 };
 
 static constexpr int kBM = 128;
@@ -78,142 +83,177 @@ struct Mfma16<float> {
 };
 
 // ---------------------------------------------------------------------------
-// fp64 / fp32 kernel (T = double | float), BK = 16.
+// fp64 / fp32 kernel.  Block = WM x WN waves, each wave a 64 x 64 tile of
+// 4 x 4 MFMA 16x16 blocks; block tile (64 WM) x (64 WN) x BK.
+// LDS tiles are [m][k] with row pitch LDK = BK + 2 (fp64) or BK + 4 (fp32):
+// LDK = 2*odd (fp64) / 4*odd (fp32) maps the 16 rows x (64/16) k values of one
+// MFMA operand fetch onto 64 distinct banks.  Operands move in 16-byte chunks
+// (2 fp64 / 4 fp32); interior tiles take a branch-free vector path.
 // ---------------------------------------------------------------------------
-template <typename T, bool AKC, bool BKC>
-__global__ __launch_bounds__(256) void gemm16_kernel(GemmArgs p) {
-  constexpr int BK = 16;
-  constexpr int PAD = 16;
-  constexpr int LD = kBM + PAD;
-  __shared__ T As[2][BK][LD];
-  __shared__ T Bs[2][BK][LD];
+template <typename T>
+struct Chunk;
+template <>
+struct Chunk<double> {
+  static constexpr int E = 2;
+  static constexpr int PADK = 2;
+};
+template <>
+struct Chunk<float> {
+  static constexpr int E = 4;
+  static constexpr int PADK = 4;
+};
+
+template <typename T, bool KC, int ROWS, int BK, int NT>
+struct TileLoader {
+  // ROWS x BK tile of a k-contiguous (KC) or row-contiguous (!KC) operand.
+  static constexpr int E = Chunk<T>::E;
+  static constexpr int LDK = BK + Chunk<T>::PADK;
+  static constexpr int NCH = ROWS * BK / E;  // chunks per tile
+  static constexpr int PER = NCH / NT;       // chunks per thread
+  static_assert(NCH % NT == 0, "tile/threads mismatch");
+  uint4 r[PER];
+
+  __device__ inline void load(const T* __restrict__ G, long ld, int row0, int nrows, int k0, int K, bool vec,
+                              int t) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int q = t + i * NT;
+      int row, kk;
+      if (KC) {
+        row = q / (BK / E);
+        kk = (q % (BK / E)) * E;
+      } else {
+        kk = q / (ROWS / E);
+        row = (q % (ROWS / E)) * E;
+      }
+      const int gr = row0 + row, gk = k0 + kk;
+      if (vec) {
+        const T* src = KC ? G + (long)gr * ld + gk : G + (long)gk * ld + gr;
+        r[i] = *reinterpret_cast<const uint4*>(src);
+      } else {
+        T v[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int rr = KC ? gr : gr + e;
+          const int kx = KC ? gk + e : gk;
+          v[e] = (rr < nrows && kx < K) ? (KC ? G[(long)rr * ld + kx] : G[(long)kx * ld + rr]) : T(0);
+        }
+        r[i] = *reinterpret_cast<const uint4*>(v);
+      }
+    }
+  }
+  __device__ inline void store(T* __restrict__ S, int t) const {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int q = t + i * NT;
+      if (KC) {
+        const int row = q / (BK / E);
+        const int kk = (q % (BK / E)) * E;
+        *reinterpret_cast<uint4*>(S + row * LDK + kk) = r[i];
+      } else {
+        const int kk = q / (ROWS / E);
+        const int row = (q % (ROWS / E)) * E;
+        const T* v = reinterpret_cast<const T*>(&r[i]);
+#pragma unroll
+        for (int e = 0; e < E; ++e) S[(row + e) * LDK + kk] = v[e];
+      }
+    }
+  }
+};
+
+template <typename T, bool AKC, bool BKC, int BK, int WM, int WN, int FM, int FN>
+__global__ __launch_bounds__(64 * WM * WN) void gemm16_kernel(GemmArgs p) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int BM = 16 * FM * WM;
+  constexpr int BN = 16 * FN * WN;
+  typedef TileLoader<T, AKC, BM, BK, NT> LA;
+  typedef TileLoader<T, BKC, BN, BK, NT> LB;
+  constexpr int LDK = LA::LDK;
+  __shared__ __attribute__((aligned(16))) T As[2][BM * LDK];
+  __shared__ __attribute__((aligned(16))) T Bs[2][BN * LDK];
   typedef typename Mfma16<T>::acc_t acc_t;
 
   int tm, tn;
   tile_coords(p.tiles_m, p.tiles_n, tm, tn);
-  const int m0 = tm * kBM;
-  const int n0 = tn * kBN;
+  const int m0 = tm * BM;
+  const int n0 = tn * BN;
   const int bz = blockIdx.y;
-  const T* __restrict__ A = reinterpret_cast<const T*>(p.A) + (long)bz * p.sA;
-  const T* __restrict__ B = reinterpret_cast<const T*>(p.B) + (long)bz * p.sB;
-  double* __restrict__ C = p.C + (long)bz * p.sC;
-  const int M = p.M, N = p.N, K = p.K;
+  const long kz = (long)blockIdx.z * p.kchunk;
+  const T* __restrict__ A = reinterpret_cast<const T*>(p.A) + (long)bz * p.sA + kz * p.zkA;
+  const T* __restrict__ B = reinterpret_cast<const T*>(p.B) + (long)bz * p.sB + kz * p.zkB;
+  double* __restrict__ C = p.C + (long)bz * p.sC + (long)blockIdx.z * p.zC;
+  const int M = p.M, N = p.N;
+  const int K = (int)((p.K - kz) < p.kchunk ? (p.K - kz) : p.kchunk);
 
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wave = t >> 6;
-  const int wm = wave >> 1;
-  const int wn = wave & 1;
+  const int wm = wave / WN;
+  const int wn = wave % WN;
 
-  // Loader mapping.  k-contiguous: row = t & 127, 8 consecutive k from
-  // (t >> 7) * 8.  m-contiguous: k row = t >> 4, 8 consecutive m from (t & 15) * 8.
-  T ra[8], rb[8];
-  auto load_a = [&](int k0) {
-    if (AKC) {
-      const int r = t & 127, kb = (t >> 7) * 8;
-      const int m = m0 + r;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int k = k0 + kb + j;
-        ra[j] = (m < M && k < K) ? A[(long)m * p.lda + k] : T(0);
-      }
-    } else {
-      const int kr = t >> 4, mb = (t & 15) * 8;
-      const int k = k0 + kr;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int m = m0 + mb + j;
-        ra[j] = (m < M && k < K) ? A[(long)k * p.lda + m] : T(0);
-      }
-    }
-  };
-  auto load_b = [&](int k0) {
-    if (BKC) {
-      const int r = t & 127, kb = (t >> 7) * 8;
-      const int n = n0 + r;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int k = k0 + kb + j;
-        rb[j] = (n < N && k < K) ? B[(long)n * p.ldb + k] : T(0);
-      }
-    } else {
-      const int kr = t >> 4, nb = (t & 15) * 8;
-      const int k = k0 + kr;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int n = n0 + nb + j;
-        rb[j] = (n < N && k < K) ? B[(long)k * p.ldb + n] : T(0);
-      }
-    }
-  };
-  auto store_tiles = [&](int buf) {
-    if (AKC) {
-      const int r = t & 127, kb = (t >> 7) * 8;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) As[buf][kb + j][r] = ra[j];
-    } else {
-      const int kr = t >> 4, mb = (t & 15) * 8;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) As[buf][kr][mb + j] = ra[j];
-    }
-    if (BKC) {
-      const int r = t & 127, kb = (t >> 7) * 8;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) Bs[buf][kb + j][r] = rb[j];
-    } else {
-      const int kr = t >> 4, nb = (t & 15) * 8;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) Bs[buf][kr][nb + j] = rb[j];
-    }
-  };
+  // vector path: whole tile inside and 16-byte aligned rows
+  const bool alA = p.vecA != 0;
+  const bool alB = p.vecB != 0;
+  const bool inM = (m0 + BM <= M), inN = (n0 + BN <= N);
 
-  acc_t acc[4][4];
+  LA la;
+  LB lb;
+  acc_t acc[FM][FN];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   const int nk = (K + BK - 1) / BK;
-  load_a(0);
-  load_b(0);
-  store_tiles(0);
+  {
+    const bool kin = (BK <= K);
+    la.load(A, p.lda, m0, M, 0, K, alA && inM && kin, t);
+    lb.load(B, p.ldb, n0, N, 0, K, alB && inN && kin, t);
+    la.store(As[0], t);
+    lb.store(Bs[0], t);
+  }
   __syncthreads();
   const int lr = lane & 15;
   const int lk = lane >> 4;
   for (int kt = 0; kt < nk; ++kt) {
     const int buf = kt & 1;
     if (kt + 1 < nk) {
-      load_a((kt + 1) * BK);
-      load_b((kt + 1) * BK);
+      const int k0 = (kt + 1) * BK;
+      const bool kin = (k0 + BK <= K);
+      la.load(A, p.lda, m0, M, k0, K, alA && inM && kin, t);
+      lb.load(B, p.ldb, n0, N, k0, K, alB && inN && kin, t);
     }
+    const T* __restrict__ as = As[buf];
+    const T* __restrict__ bs = Bs[buf];
 #pragma unroll
     for (int kk = 0; kk < BK / 4; ++kk) {
-      T a[4], b[4];
+      T a[FM], b[FN];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[buf][kk * 4 + lk][wm * 64 + i * 16 + lr];
+      for (int i = 0; i < FM; ++i) a[i] = as[(wm * 16 * FM + i * 16 + lr) * LDK + kk * 4 + lk];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[buf][kk * 4 + lk][wn * 64 + j * 16 + lr];
+      for (int j = 0; j < FN; ++j) b[j] = bs[(wn * 16 * FN + j * 16 + lr) * LDK + kk * 4 + lk];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma16<T>::op(a[i], b[j], acc[i][j]);
+        for (int j = 0; j < FN; ++j) acc[i][j] = Mfma16<T>::op(a[i], b[j], acc[i][j]);
     }
     if (kt + 1 < nk) {
-      store_tiles(buf ^ 1);
+      la.store(As[buf ^ 1], t);
+      lb.store(Bs[buf ^ 1], t);
     }
     __syncthreads();
   }
 
-  // Epilogue: acc[i][j][r] -> row wm*64 + i*16 + row(lk, r), col wn*64 + j*16 + lr.
+  // Epilogue: acc[i][j][r] -> row wm*16FM + i*16 + row(lk, r), col wn*16FN + j*16 + lr.
   const double alpha = p.alpha, beta = p.beta;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < FM; ++i) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int col = n0 + wn * 64 + j * 16 + lr;
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + wn * 16 * FN + j * 16 + lr;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * 64 + i * 16 + Mfma16<T>::row(lk, r);
+        const int row = m0 + wm * 16 * FM + i * 16 + Mfma16<T>::row(lk, r);
         if (row < M && col < N) {
           double* c = C + (long)row * p.ldc + col;
           const double v = alpha * (double)acc[i][j][r];
@@ -356,16 +396,57 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmArgs p) {
   }
 }
 
+// C[z-sum] reduction for split-K: C = alpha-scaled partial sums (already in ws) + beta*C.
+__global__ void splitk_reduce_kernel(int M, int N, int splits, int batch, const double* __restrict__ ws, long zC,
+                                     double* __restrict__ C, long ldc, long sC, double beta) {
+  const long per = (long)M * N;
+  const long total = per * batch;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const long bz = idx / per, r = idx - bz * per;
+    const long row = r / N, col = r - row * N;
+    double acc = 0.0;
+    for (int z = 0; z < splits; ++z) acc += ws[z * zC + bz * per + r];
+    double* c = C + bz * sC + row * ldc + col;
+    *c = (beta == 0.0) ? acc : acc + beta * (*c);
+  }
+}
+
+// Tile configuration of the fp64/fp32 kernel: BK, waves along M/N, MFMA
+// 16x16 blocks per wave along M/N.
+// Measured on MI355X (o=48, v=320 contraction shapes): fp64 runs best with
+// 64 x 64 blocks of four 32 x 32 waves, fp32 with 64 x 128 blocks of four
+// 32 x 64 waves.
+template <int BK_, int WM_, int WN_, int FM_, int FN_>
+struct Cfg {
+  static constexpr int BK = BK_, WM = WM_, WN = WN_, FM = FM_, FN = FN_;
+  static constexpr int BM = 16 * FM * WM, BN = 16 * FN * WN, NT = 64 * WM * WN;
+};
+template <typename T>
+struct TileCfg;
+template <>
+struct TileCfg<double> {
+  typedef Cfg<16, 2, 2, 2, 2> type;
+};
+template <>
+struct TileCfg<float> {
+  typedef Cfg<16, 2, 2, 2, 4> type;
+};
+
 template <typename T>
 static void launch16(const GemmArgs& a, bool akc, bool bkc, dim3 grid, hipStream_t s) {
+  typedef typename TileCfg<T>::type C64;
+  const dim3 blk(C64::NT);
+#define L16(AK, BK2) \
+  hipLaunchKernelGGL((gemm16_kernel<T, AK, BK2, C64::BK, C64::WM, C64::WN, C64::FM, C64::FN>), grid, blk, 0, s, a)
   if (akc && bkc)
-    hipLaunchKernelGGL((gemm16_kernel<T, true, true>), grid, dim3(256), 0, s, a);
+    L16(true, true);
   else if (akc && !bkc)
-    hipLaunchKernelGGL((gemm16_kernel<T, true, false>), grid, dim3(256), 0, s, a);
+    L16(true, false);
   else if (!akc && bkc)
-    hipLaunchKernelGGL((gemm16_kernel<T, false, true>), grid, dim3(256), 0, s, a);
+    L16(false, true);
   else
-    hipLaunchKernelGGL((gemm16_kernel<T, false, false>), grid, dim3(256), 0, s, a);
+    L16(false, false);
+#undef L16
 }
 
 int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream) {
@@ -384,11 +465,50 @@ int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream) {
   a.C = d.C; a.ldc = d.ldc;
   a.alpha = d.alpha; a.beta = d.beta;
   a.sA = d.sA; a.sB = d.sB; a.sC = d.sC;
-  a.tiles_m = (d.M + kBM - 1) / kBM;
-  a.tiles_n = (d.N + kBN - 1) / kBN;
+  const int bm = (prec == GP_BF16X3) ? kBM : (prec == GP_F32 ? TileCfg<float>::type::BM : TileCfg<double>::type::BM);
+  const int bn = (prec == GP_BF16X3) ? kBN : (prec == GP_F32 ? TileCfg<float>::type::BN : TileCfg<double>::type::BN);
+  a.tiles_m = (d.M + bm - 1) / bm;
+  a.tiles_n = (d.N + bn - 1) / bn;
+  {
+    const int esz = (prec == GP_F32) ? 4 : 8;
+    auto aligned = [&](const void* ptr, long ld, long sb) {
+      return ((uintptr_t)ptr % 16 == 0) && ((ld * esz) % 16 == 0) && ((sb * esz) % 16 == 0);
+    };
+    a.vecA = aligned(d.A, d.lda, d.sA) ? 1 : 0;
+    a.vecB = aligned(d.B, d.ldb, d.sB) ? 1 : 0;
+  }
+  a.kchunk = d.K > 0 ? d.K : 1;
+  a.zkA = d.a_kc ? 1 : d.lda;
+  a.zkB = d.b_kc ? 1 : d.ldb;
+  a.zC = 0;
   const long nb = (long)a.tiles_m * a.tiles_n;
   ACES_REQUIRE(nb < (1L << 31) && d.batch < 65536, "gemm: grid too large");
-  dim3 grid((unsigned)nb, (unsigned)d.batch);
+  // split-K for outputs that cannot fill 256 CUs (fp64/fp32 kernels only)
+  int splits = 1;
+  if (prec != GP_BF16X3 && d.ws != nullptr && d.K >= 1024) {
+    const long tiles = nb * d.batch;
+    if (tiles < 512) {
+      long sp = 1024 / tiles;
+      const long kmax = d.K / 512;
+      if (sp > kmax) sp = kmax;
+      const long per = (long)d.M * d.N * d.batch;
+      if (sp * per > d.ws_elems) sp = d.ws_elems / per;
+      if (sp > 1024) sp = 1024;
+      if (sp >= 2) splits = (int)sp;
+    }
+  }
+  if (splits > 1) {
+    long kc = (d.K + splits - 1) / splits;
+    kc = (kc + 31) / 32 * 32;
+    splits = (int)((d.K + kc - 1) / kc);
+    a.kchunk = kc;
+    a.C = d.ws;
+    a.ldc = d.N;
+    a.sC = (long)d.M * d.N;
+    a.zC = (long)d.M * d.N * d.batch;
+    a.beta = 0.0;
+  }
+  dim3 grid((unsigned)nb, (unsigned)d.batch, (unsigned)splits);
   if (d.K == 0) {
     // C = beta * C handled by a K = 0 launch of the fp64 kernel (acc stays 0).
     a.K = 0;
@@ -417,6 +537,14 @@ int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream) {
       ACES_REQUIRE(false, "gemm: unknown precision");
   }
   ACES_HIP(hipGetLastError());
+  if (splits > 1) {
+    const long total = (long)d.M * d.N * d.batch;
+    long g = (total + 255) / 256;
+    if (g > (1L << 20)) g = 1L << 20;
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)g), dim3(256), 0, stream, d.M, d.N, splits, d.batch,
+                       d.ws, a.zC, d.C, d.ldc, d.sC, d.beta);
+    ACES_HIP(hipGetLastError());
+  }
   return ACES_OK;
 }
 

@@ -77,6 +77,12 @@ int aces_ccsd_set_diis_order(void* handle, int order);
 int aces_ccsd_reset_diis(void* handle);
 int aces_ccsd_energy(void* handle, const double* d_t1, const double* d_t2, double* e_out);
 long aces_ccsd_device_bytes(void* handle);
+/* Per-launch hipEvent timing of the MFMA contractions (on the engine's stream):
+ * out[3c+0] ms, out[3c+1] algorithmic flops (2mnk), out[3c+2] launches, for
+ * c = 0 particle ladder, 1 W(mbej) build, 2 ring.  No reference counterpart
+ * (the reference has only whole-program timers, acescore/aces_fin.F:29-55). */
+int aces_ccsd_set_profiling(void* handle, int on);
+int aces_ccsd_profile(void* handle, double* out, int reset);
 
 #ifdef __cplusplus
 }
