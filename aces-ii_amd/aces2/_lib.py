@@ -40,6 +40,33 @@ SIGNATURES = {
     'aces_ccsd_device_bytes': (c_long, [c_vp]),
     'aces_ccsd_set_profiling': (c_int, [c_vp, c_int]),
     'aces_ccsd_profile': (c_int, [c_vp, c_dp, c_int]),
+    'aces_init': (c_int, [c_vp]),
+    'aces_fin': (c_int, []),
+    'aces_set_symmetry': (c_int, [c_int, c_vp, c_vp, c_vp, c_vp]),
+    'aces_irpdpd': (c_long, [c_int, c_int]),
+    'aces_isymoff': (c_long, [c_int, c_int, c_int]),
+    'aces_inipck': (c_int, [c_int, c_int, c_int, c_int]),
+    'aces_updmoi': (c_int, [c_long, c_long, c_int, c_int]),
+    'aces_list_shape': (c_int, [c_int, c_int, ctypes.POINTER(c_long), ctypes.POINTER(c_long)]),
+    'aces_list_isytyp': (c_int, [c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]),
+    'aces_getlst': (c_int, [c_vp, c_long, c_long, c_int, c_int, c_int]),
+    'aces_putlst': (c_int, [c_vp, c_long, c_long, c_int, c_int, c_int]),
+    'aces_getall': (c_int, [c_vp, c_long, c_int, c_int, c_int]),
+    'aces_putall': (c_int, [c_vp, c_long, c_int, c_int, c_int]),
+    'aces_sumsym2': (c_int, [c_vp, c_int, c_int, c_int]),
+    'aces_list_device_ptr': (c_vp, [c_int, c_int]),
+    'aces_putrec': (c_int, [c_char_p, c_long, c_vp]),
+    'aces_getrec': (c_int, [c_int, c_char_p, ctypes.POINTER(c_long), c_vp]),
+    'aces_vcc_mp1': (c_int, []),
+    'aces_vcc_set_diis_order': (c_int, [c_int]),
+    'aces_vcc_set_precision': (c_int, [c_int]),
+    'aces_genint': (c_int, []),
+    'aces_dre3en': (c_int, []),
+    'aces_e4s': (c_int, []),
+    'aces_newt2': (c_int, []),
+    'aces_cmpeng': (c_int, [c_int, c_int, c_dp, c_dp]),
+    'aces_dodiis0': (c_int, [c_int, c_double, ctypes.POINTER(c_int), c_dp]),
+    'aces_drmove': (c_int, []),
 }
 
 

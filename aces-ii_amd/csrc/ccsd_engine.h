@@ -41,6 +41,13 @@ class CcsdEngine {
   int increments(GemmPrec prec, double* inc1_out, double* inc2_out);
   // One full cycle (increments, Jacobi, energy, DIIS, move).
   int cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st);
+  // Pieces of one cycle, used by the list-level driver entry points.
+  int jacobi_step();
+  int diis_step(int icycle, double tol, CcsdStats& st);
+  double* T() { return t1_; }        // [t1 | t2] current amplitudes
+  double* Tnew() { return t1n_; }    // [t1 | t2] Jacobi update
+  double* Inc() { return inc1_; }    // [inc1 | inc2]
+  int energy_dev(const double* t1, const double* t2, double* d_out);
   int reset_diis();
   int set_diis_order(int order);
   int energy_of(const double* t1, const double* t2, double& e);

@@ -303,6 +303,12 @@ int CcsdEngine::start_mp1() {
   return ACES_OK;
 }
 
+// Energy into a device scalar; d_out == nullptr targets the slot that
+// diis_step() reports as E(T_new).
+int CcsdEngine::energy_dev(const double* t1, const double* t2, double* d_out) {
+  return energy(o, v, fock_ + o, n, Lijab_, t1, t2, red_, d_out ? d_out : scal_, s_);
+}
+
 int CcsdEngine::energy_of(const double* t1, const double* t2, double& e) {
   ACES_CHECK(energy(o, v, fock_ + o, n, Lijab_, t1, t2, red_, scal_, s_));
   ACES_HIP(hipMemcpyAsync(&e, scal_, sizeof(double), hipMemcpyDeviceToHost, s_));
@@ -552,6 +558,7 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
 int CcsdEngine::increments(GemmPrec prec, double* inc1_out, double* inc2_out) {
   ACES_REQUIRE(ready_, "ccsd: not set up");
   ACES_CHECK(compute_increments(prec));
+  if (inc1_out == nullptr && inc2_out == nullptr) return ACES_OK;
   const long ov = (long)o * v;
   if (inc1_out) ACES_HIP(hipMemcpyAsync(inc1_out, inc1_, sizeof(double) * ov, hipMemcpyDeviceToDevice, s_));
   if (inc2_out) ACES_HIP(hipMemcpyAsync(inc2_out, inc2_, sizeof(double) * ov * ov, hipMemcpyDeviceToDevice, s_));
@@ -570,12 +577,30 @@ int CcsdEngine::cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st) {
   ACES_HIP(hipEventRecord(e0, s_));
   ACES_CHECK(compute_increments(prec));
   ACES_HIP(hipEventRecord(e1, s_));
-  const long nt = (long)o * v + (long)o * o * v * v;
-  // NEWT2 / E4S
-  ACES_CHECK(jacobi(o, v, eo_, ev_, inc1_, inc2_, t1n_, t2n_, s_));
-  // CMPENG(60,2): energy of the Jacobi update
+  ACES_CHECK(jacobi_step());
   ACES_CHECK(energy(o, v, fock_ + o, n, Lijab_, t1n_, t2n_, red_, scal_, s_));
-  // DODIIS0
+  ACES_CHECK(diis_step(icycle, tol, st));
+  ACES_HIP(hipEventRecord(e2, s_));
+  ACES_HIP(hipEventSynchronize(e2));
+  float ms1 = 0, ms2 = 0;
+  ACES_HIP(hipEventElapsedTime(&ms1, e0, e1));
+  ACES_HIP(hipEventElapsedTime(&ms2, e0, e2));
+  st.ms_increments = ms1;
+  st.ms_cycle = ms2;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  return ACES_OK;
+}
+
+// NEWT2 / E4S: T_new = increment / D.
+int CcsdEngine::jacobi_step() { return jacobi(o, v, eo_, ev_, inc1_, inc2_, t1n_, t2n_, s_); }
+
+// DODIIS0 (vcc/dodiis0.F) on T (t1_/t2_) and T_new (t1n_/t2n_); scal_[0] holds
+// the energy of T_new computed by the caller (CMPENG(60,2)).  On return t1_/t2_
+// hold the next amplitudes unless converged.
+int CcsdEngine::diis_step(int icycle, double tol, CcsdStats& st) {
+  const long nt = (long)o * v + (long)o * o * v * v;
   ACES_CHECK(symmetrize_t2(o, v, t2_, s_));
   ACES_CHECK(symmetrize_t2(o, v, t2n_, s_));
   const int ndimr = std::min(icycle, order_);
@@ -653,16 +678,6 @@ int CcsdEngine::cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st) {
   } else {
     ACES_HIP(hipMemcpyAsync(t1_, t1n_, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
   }
-  ACES_HIP(hipEventRecord(e2, s_));
-  ACES_HIP(hipEventSynchronize(e2));
-  float ms1 = 0, ms2 = 0;
-  ACES_HIP(hipEventElapsedTime(&ms1, e0, e1));
-  ACES_HIP(hipEventElapsedTime(&ms2, e0, e2));
-  st.ms_increments = ms1;
-  st.ms_cycle = ms2;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipEventDestroy(e2);
   return ACES_OK;
 }
 

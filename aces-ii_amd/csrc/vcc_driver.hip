@@ -1,0 +1,546 @@
+// xvcc driver entry points over the device list store (the drop-in boundary).
+//
+// ACES II's CC loop (vcc/vcc.F:811-1085) calls, in order, INITIN, GENINT,
+// FEACONT, FMICONT, FMECONT, T1INT2A, T1INT2B, T1INT1, DRE3EN, E4S, NEWT2,
+// CMPENG, DODIIS0, DRMOVE, RNABIJ; each reads lists and accumulates into
+// increment lists.  Here the same entry points (Fortran linkage, INTEGER*8
+// arguments by reference, trailing underscore) act on the device list store:
+//
+//   GENINT   builds the engine from the integral lists on first use, loads T
+//            from lists 46 (T2 AB) and 90 group 1 (T1) and evaluates every
+//            CCSD increment term in one pass (the W/F intermediates of GENINT
+//            and the contributions FEACONT..DRE3EN and E4S add are fused).
+//   FEACONT, FMICONT, FMECONT, T1INT2A, T1INT2B, T1INT1, INITIN, INITSN, RNABIJ
+//            no separate work (their terms are part of the fused increment).
+//   DRE3EN   writes the T2 increment to list 63 (AB) and 61 (AA).
+//   E4S      writes T1_new = T1 increment / D1 to list 90 group 3.
+//   NEWT2    list 63 <- list 63 / D (libr2/newt2.F:233-244), list 61 <- ABTOAA(63).
+//   CMPENG   correlation energy from lists NLIST2+1..+3 and T1 group NLIST1+1.
+//   DODIIS0  DIIS on (lists 44/46/90 g1) vs (61/63/90 g3); extrapolated vector
+//            written to the increment lists as in vcc/dodiis0.F.
+//   DRMOVE   T lists <- increment lists.
+//
+// Integral lists read by GENINT (RHF, C1; types from intprc/clmoio.F):
+//   13 <Mn|Ij> (14,14), 16 <Ab|Ij> (13,14), 10 <Ij|Ka> (14,18),
+//   30 <Ab|Ci> (13,11), 233 <Ab|Cd> (5,13), 23 <AJ||BI> ordered (AI,BJ) (9,9);
+// orbital energies from record SCFEVALA, the MO Fock matrix from record FOCKMOA
+// when present (non-HF), otherwise diag(SCFEVALA).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+#include "ccsd_engine.h"
+#include "ccsd_kernels.h"
+#include "common.h"
+#include "liststore.h"
+#include "tensor_ops.h"
+
+namespace aces {
+
+struct AcesContext {
+  hipStream_t stream = nullptr;
+  std::unique_ptr<ListStore> lists;
+  RecordStore recs;
+  std::unique_ptr<CcsdEngine> eng;
+  int prec = GP_F64;
+  int last_cycle = 0;
+};
+
+static AcesContext* g_ctx = nullptr;
+
+static int ctx(AcesContext** c) {
+  if (!g_ctx) {
+    set_error("aces_init has not been called");
+    return ACES_ERR_STATE;
+  }
+  *c = g_ctx;
+  return ACES_OK;
+}
+
+// AA (A<B, I<J) list 44/61 from the AB amplitudes t2[i][j][a][b]: AA = t_ijab - t_jiab.
+__global__ void abtoaa_kernel(int o, int v, const double* __restrict__ t2, double* __restrict__ aa) {
+  const long nom = (long)o * (o - 1) / 2, nvm = (long)v * (v - 1) / 2;
+  const long vv = (long)v * v;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < nom * nvm;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long cij = idx / nvm, rab = idx - cij * nvm;
+    int j = (int)((sqrt(8.0 * (double)cij + 1.0) + 1.0) * 0.5);
+    while ((long)j * (j - 1) / 2 > cij) --j;
+    while ((long)(j + 1) * j / 2 <= cij) ++j;
+    const int i = (int)(cij - (long)j * (j - 1) / 2);
+    int b = (int)((sqrt(8.0 * (double)rab + 1.0) + 1.0) * 0.5);
+    while ((long)b * (b - 1) / 2 > rab) --b;
+    while ((long)(b + 1) * b / 2 <= rab) ++b;
+    const int a = (int)(rab - (long)b * (b - 1) / 2);
+    aa[idx] = t2[((long)i * o + j) * vv + (long)a * v + b] - t2[((long)j * o + i) * vv + (long)a * v + b];
+  }
+}
+
+static int abtoaa(int o, int v, const double* t2, double* aa, hipStream_t s) {
+  const long n = (long)o * (o - 1) / 2 * ((long)v * (v - 1) / 2);
+  if (n == 0) return ACES_OK;
+  long g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(abtoaa_kernel, dim3((unsigned)g), dim3(256), 0, s, o, v, t2, aa);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+static int require_list(AcesContext* c, int list, long rows, long cols, const char* what) {
+  long r = 0, k = 0;
+  if (c->lists->rows(1, list, &r) != ACES_OK || c->lists->cols(1, list, &k) != ACES_OK || r != rows ||
+      k != cols) {
+    set_error(std::string("list ") + std::to_string(list) + " (" + what + ") missing or of wrong shape: have " +
+              std::to_string(r) + "x" + std::to_string(k) + ", need " + std::to_string(rows) + "x" +
+              std::to_string(cols));
+    return ACES_ERR_RANGE;
+  }
+  return ACES_OK;
+}
+
+// Build the engine from the integral lists (C1 RHF).
+static int setup_engine(AcesContext* c) {
+  const DpdSym& sy = c->lists->sym();
+  ACES_REQUIRE(sy.nirrep == 1, "the list-fed CCSD engine takes C1 lists (nirrep = 1)");
+  const int o = sy.pop[0][0], v = sy.vrt[0][0], n = o + v;
+  ACES_REQUIRE(o >= 1 && v >= 1, "need occupied and virtual orbitals");
+  const long oo = (long)o * o, vv = (long)v * v, ov = (long)o * v;
+  ACES_CHECK(require_list(c, 13, oo, oo, "<Mn|Ij>"));
+  ACES_CHECK(require_list(c, 16, vv, oo, "<Ab|Ij>"));
+  ACES_CHECK(require_list(c, 10, oo, ov, "<Ij|Ka>"));
+  ACES_CHECK(require_list(c, 30, vv, ov, "<Ab|Ci>"));
+  ACES_CHECK(require_list(c, 233, (long)v * (v + 1) / 2, vv, "<Ab|Cd>"));
+  ACES_CHECK(require_list(c, 23, ov, ov, "<AJ||BI>"));
+  hipStream_t s = c->stream;
+  std::vector<double> f((size_t)n * n, 0.0);
+  long len = (long)n * n;
+  if (c->recs.has("FOCKMOA")) {
+    ACES_CHECK(c->recs.getrec(1, "FOCKMOA", &len, f.data()));
+  } else {
+    std::vector<double> e(n);
+    long ne = n;
+    ACES_CHECK(c->recs.getrec(1, "SCFEVALA", &ne, e.data()));
+    for (int p = 0; p < n; ++p) f[(size_t)p * n + p] = e[p];
+  }
+  double *d_f, *ovov, *oovv, *ovoo, *oooo, *ovvv;
+  ACES_HIP(hipMalloc(&d_f, sizeof(double) * n * n));
+  ACES_HIP(hipMalloc(&ovov, sizeof(double) * ov * ov));
+  ACES_HIP(hipMalloc(&oovv, sizeof(double) * ov * ov));
+  ACES_HIP(hipMalloc(&ovoo, sizeof(double) * ov * oo));
+  ACES_HIP(hipMalloc(&oooo, sizeof(double) * oo * oo));
+  ACES_HIP(hipMalloc(&ovvv, sizeof(double) * ov * vv));
+  ACES_HIP(hipMemcpyAsync(d_f, f.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, s));
+  auto perm = [&](double* out, long d0, long d1, long d2, long d3, const double* a, double aa, const long* sa,
+                  const double* b, double ab, const long* sb) {
+    Permute4 q;
+    q.d[0] = d0; q.d[1] = d1; q.d[2] = d2; q.d[3] = d3;
+    q.out = out;
+    q.in[0] = a; q.alpha[0] = aa;
+    for (int k = 0; k < 4; ++k) q.st[0][k] = sa[k];
+    if (b) {
+      q.in[1] = b; q.alpha[1] = ab;
+      for (int k = 0; k < 4; ++k) q.st[1][k] = sb[k];
+    }
+    return permute4(q, s);
+  };
+  const double* L16 = c->lists->device_ptr(1, 16);
+  const double* L13 = c->lists->device_ptr(1, 13);
+  const double* L10 = c->lists->device_ptr(1, 10);
+  const double* L30 = c->lists->device_ptr(1, 30);
+  const double* L23 = c->lists->device_ptr(1, 23);
+  // ovov[i][a][j][b] = <ab|ij>, list 16 element a + v b + v^2 (i + o j)
+  { const long st[4] = {vv, 1, vv * o, v}; ACES_CHECK(perm(ovov, o, v, o, v, L16, 1.0, st, nullptr, 0, nullptr)); }
+  // oooo[m][i][n][j] = <mn|ij>, list 13 element m + o n + o^2 (i + o j)
+  { const long st[4] = {1, oo, o, oo * o}; ACES_CHECK(perm(oooo, o, o, o, o, L13, 1.0, st, nullptr, 0, nullptr)); }
+  // ovoo[j][a][i][k] = <ij|ka>, list 10 element i + o j + o^2 (k + o a)
+  { const long st[4] = {o, oo * o, 1, oo}; ACES_CHECK(perm(ovoo, o, v, o, o, L10, 1.0, st, nullptr, 0, nullptr)); }
+  // ovvv[i][b][a][c] = <ab|ci>, list 30 element a + v b + v^2 (c + v i)
+  { const long st[4] = {vv * v, v, 1, vv}; ACES_CHECK(perm(ovvv, o, v, v, v, L30, 1.0, st, nullptr, 0, nullptr)); }
+  // oovv[i][j][a][b] = (ij|ab) = <AJ||BI>(AI,BJ) + (ai|bj); list 23 element a + v i + ov (b + v j)
+  {
+    const long st[4] = {v, ov * v, 1, ov};
+    const long so[4] = {ov * v, v, ov, 1};  // ovov[i][a][j][b] over output (i, j, a, b)
+    ACES_CHECK(perm(oovv, o, o, v, v, L23, 1.0, st, ovov, 1.0, so));
+  }
+  c->eng.reset(new CcsdEngine(o, v, s));
+  const int rc = c->eng->setup(d_f, ovov, oovv, ovoo, oooo, ovvv, c->lists->device_ptr(1, 233));
+  (void)hipFree(d_f);
+  (void)hipFree(ovov);
+  (void)hipFree(oovv);
+  (void)hipFree(ovoo);
+  (void)hipFree(oooo);
+  (void)hipFree(ovvv);
+  if (rc != ACES_OK) c->eng.reset();
+  return rc;
+}
+
+static int ensure_amp_lists(AcesContext* c) {
+  const int o = c->eng->o, v = c->eng->v;
+  const long nom = (long)o * (o - 1) / 2, nvm = (long)v * (v - 1) / 2;
+  const long vv = (long)v * v, oo = (long)o * o, ov = (long)o * v;
+  long r;
+  // list 46 / 63: (13,14) AB; 44 / 61: (1,3) AA; 90 groups 1..4: T1 and increments
+  if (c->lists->rows(1, 46, &r) != ACES_OK) ACES_CHECK(c->lists->inipck(1, 13, 14, 46));
+  if (c->lists->rows(1, 63, &r) != ACES_OK) ACES_CHECK(c->lists->inipck(1, 13, 14, 63));
+  if (c->lists->rows(1, 44, &r) != ACES_OK) ACES_CHECK(c->lists->inipck(1, 1, 3, 44));
+  if (c->lists->rows(1, 61, &r) != ACES_OK) ACES_CHECK(c->lists->inipck(1, 1, 3, 61));
+  for (int g = 1; g <= 4; ++g)
+    if (c->lists->rows(g, 90, &r) != ACES_OK) ACES_CHECK(c->lists->updmoi(1, ov, g, 90));
+  (void)nom; (void)nvm; (void)vv; (void)oo;
+  return ACES_OK;
+}
+
+}  // namespace aces
+
+using namespace aces;
+
+extern "C" {
+
+// ---- runtime (aces_init / aces_fin, acescore/aces_init.F:26) ----
+int aces_init(void* stream) {
+  if (g_ctx) return ACES_OK;
+  g_ctx = new AcesContext();
+  g_ctx->stream = (hipStream_t)stream;
+  g_ctx->lists.reset(new ListStore(g_ctx->stream));
+  return ACES_OK;
+}
+
+int aces_fin(void) {
+  delete g_ctx;
+  g_ctx = nullptr;
+  return ACES_OK;
+}
+
+int aces_set_symmetry(int nirrep, const int* popa, const int* popb, const int* vrta, const int* vrtb) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  return c->lists->set_symmetry(nirrep, popa, popb, vrta, vrtb);
+}
+
+long aces_irpdpd(int irrep, int type) {
+  if (!g_ctx) return -1;
+  return g_ctx->lists->sym().irpdpd(irrep - 1, type);
+}
+
+long aces_isymoff(int irrep_q, int irrep_pq, int type) {
+  if (!g_ctx) return -1;
+  const long off = g_ctx->lists->sym().isymoff(irrep_q - 1, irrep_pq - 1, type);
+  return off < 0 ? -1 : off + 1;
+}
+
+// ---- list interface (C names) ----
+int aces_inipck(int irrepx, int typel, int typer, int list) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  return c->lists->inipck(irrepx, typel, typer, list);
+}
+int aces_updmoi(long ncols, long nrows, int group, int list) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  return c->lists->updmoi(ncols, nrows, group, list);
+}
+int aces_list_shape(int group, int list, long* nrows, long* ncols) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  ACES_CHECK(c->lists->rows(group, list, nrows));
+  return c->lists->cols(group, list, ncols);
+}
+int aces_list_isytyp(int list, int* typel, int* typer) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  return c->lists->isytyp(list, typel, typer);
+}
+int aces_getlst(double* dest, long ifirst, long ncols, int group, int list, int dest_on_device) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  return c->lists->getlst(dest, ifirst, ncols, group, list, dest_on_device != 0);
+}
+int aces_putlst(const double* src, long ifirst, long ncols, int group, int list, int src_on_device) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  return c->lists->putlst(src, ifirst, ncols, group, list, src_on_device != 0);
+}
+int aces_getall(double* dest, long length, int irrepx, int list, int on_device) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  return c->lists->getall(dest, length, irrepx, list, on_device != 0);
+}
+int aces_putall(const double* src, long length, int irrepx, int list, int on_device) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  return c->lists->putall(src, length, irrepx, list, on_device != 0);
+}
+int aces_sumsym2(const double* add, int group, int list, int on_device) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  return c->lists->sumsym2(add, group, list, on_device != 0);
+}
+double* aces_list_device_ptr(int group, int list) { return g_ctx ? g_ctx->lists->device_ptr(group, list) : nullptr; }
+
+// ---- record interface (C names) ----
+int aces_putrec(const char* name, long nwords, const void* src) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  return c->recs.putrec(name, nwords, src);
+}
+int aces_getrec(int iflag, const char* name, long* nwords, void* dest) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  return c->recs.getrec(iflag, name, nwords, dest);
+}
+
+int aces_vcc_set_precision(int prec) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  ACES_REQUIRE(prec >= 0 && prec <= 2, "precision must be 0 (fp64), 1 (fp32) or 2 (bf16x3)");
+  c->prec = prec;
+  return ACES_OK;
+}
+
+// ---- Fortran linkage (INTEGER*8 by reference, hidden CHARACTER lengths) ----
+typedef int64_t i8;
+static void fatal(int rc, const char* where) {
+  if (rc != ACES_OK) {
+    fprintf(stderr, "@%s: %s\n", where, last_error());
+    fflush(stderr);
+    // The reference's fatal paths end the process through aces_exit(1).
+    exit(1);
+  }
+}
+
+void getlst_(double* dest, i8* ifirst, i8* ncols, i8* /*xcache*/, i8* ileft, i8* iright) {
+  fatal(aces_getlst(dest, *ifirst, *ncols, (int)*ileft, (int)*iright, 0), "GETLST");
+}
+void putlst_(double* src, i8* ifirst, i8* ncols, i8* /*xcache*/, i8* ileft, i8* iright) {
+  fatal(aces_putlst(src, *ifirst, *ncols, (int)*ileft, (int)*iright, 0), "PUTLST");
+}
+void getall_(double* dest, i8* length, i8* irrepx, i8* list) {
+  fatal(aces_getall(dest, *length, (int)*irrepx, (int)*list, 0), "GETALL");
+}
+void putall_(double* src, i8* length, i8* irrepx, i8* list) {
+  fatal(aces_putall(src, *length, (int)*irrepx, (int)*list, 0), "PUTALL");
+}
+void sumsym2_(double* add, double* /*scr*/, i8* /*dimscr*/, i8* /*junk*/, i8* ileft, i8* iright) {
+  fatal(aces_sumsym2(add, (int)*ileft, (int)*iright, 0), "SUMSYM2");
+}
+void inipck_(i8* irrepx, i8* typel, i8* typer, i8* list, i8* /*iarg1*/, i8* /*iarg2*/, i8* /*iset*/) {
+  fatal(aces_inipck((int)*irrepx, (int)*typel, (int)*typer, (int)*list), "INIPCK");
+}
+void updmoi_(i8* ncols, i8* nrows, i8* group, i8* list, i8* /*ienter*/, i8* /*ioff*/) {
+  fatal(aces_updmoi(*ncols, *nrows, (int)*group, (int)*list), "UPDMOI");
+}
+void getrec_(i8* iflag, const char* /*archive*/, const char* name, i8* len, void* dest, size_t /*la*/,
+             size_t ln) {
+  std::string nm(name, ln);
+  long l = *len;
+  fatal(aces_getrec((int)*iflag, nm.c_str(), &l, dest), "GETREC");
+  if (*iflag == 0) *len = l;
+}
+void putrec_(i8* /*iflag*/, const char* /*archive*/, const char* name, i8* len, const void* src, size_t /*la*/,
+             size_t ln) {
+  std::string nm(name, ln);
+  fatal(aces_putrec(nm.c_str(), *len, src), "PUTREC");
+}
+
+// ---- xvcc driver entry points ----
+int aces_genint(void) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  if (!c->eng) ACES_CHECK(setup_engine(c));
+  ACES_CHECK(ensure_amp_lists(c));
+  CcsdEngine& e = *c->eng;
+  const long ov = (long)e.o * e.v;
+  ACES_CHECK(e.set_amplitudes(c->lists->device_ptr(1, 90), c->lists->device_ptr(1, 46)));
+  (void)ov;
+  return e.increments((GemmPrec)c->prec, nullptr, nullptr);
+}
+
+int aces_dre3en(void) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  ACES_REQUIRE(c->eng != nullptr, "DRE3EN before GENINT");
+  CcsdEngine& e = *c->eng;
+  const long ov = (long)e.o * e.v;
+  ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 63), e.Inc() + ov, sizeof(double) * ov * ov,
+                          hipMemcpyDeviceToDevice, c->stream));
+  ACES_CHECK(abtoaa(e.o, e.v, c->lists->device_ptr(1, 63), c->lists->device_ptr(1, 61), c->stream));
+  ACES_HIP(hipStreamSynchronize(c->stream));
+  return ACES_OK;
+}
+
+int aces_e4s(void) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  ACES_REQUIRE(c->eng != nullptr, "E4S before GENINT");
+  CcsdEngine& e = *c->eng;
+  ACES_CHECK(e.jacobi_step());
+  const long ov = (long)e.o * e.v;
+  ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(3, 90), e.Tnew(), sizeof(double) * ov, hipMemcpyDeviceToDevice,
+                          c->stream));
+  ACES_HIP(hipStreamSynchronize(c->stream));
+  return ACES_OK;
+}
+
+int aces_newt2(void) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  ACES_REQUIRE(c->eng != nullptr, "NEWT2 before GENINT");
+  CcsdEngine& e = *c->eng;
+  const long ov = (long)e.o * e.v;
+  // the increment on list 63 (possibly modified by the caller) divided by D
+  ACES_HIP(hipMemcpyAsync(e.Inc() + ov, c->lists->device_ptr(1, 63), sizeof(double) * ov * ov,
+                          hipMemcpyDeviceToDevice, c->stream));
+  ACES_CHECK(e.jacobi_step());
+  ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 63), e.Tnew() + ov, sizeof(double) * ov * ov,
+                          hipMemcpyDeviceToDevice, c->stream));
+  ACES_CHECK(abtoaa(e.o, e.v, c->lists->device_ptr(1, 63), c->lists->device_ptr(1, 61), c->stream));
+  ACES_HIP(hipStreamSynchronize(c->stream));
+  return ACES_OK;
+}
+
+// CMPENG: nlist2 = 43 (T lists 44/46) or 60 (increment lists 61/63); T1 group nlist1 + 1.
+int aces_cmpeng(int nlist2, int nlist1, double* ecorr3, double* etot) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  ACES_REQUIRE(c->eng != nullptr, "CMPENG before GENINT");
+  CcsdEngine& e = *c->eng;
+  double* d_e;
+  ACES_HIP(hipMalloc(&d_e, sizeof(double)));
+  ACES_CHECK(e.energy_dev(c->lists->device_ptr(nlist1 + 1, 90), c->lists->device_ptr(1, nlist2 + 3), d_e));
+  double h = 0.0;
+  ACES_HIP(hipMemcpyAsync(&h, d_e, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  ACES_HIP(hipStreamSynchronize(c->stream));
+  (void)hipFree(d_e);
+  *etot = h;
+  if (ecorr3) {
+    ecorr3[0] = ecorr3[1] = 0.0;
+    ecorr3[2] = h;
+  }
+  return ACES_OK;
+}
+
+// DODIIS0 on T (44/46, 90 g1) and T_new (61/63, 90 g3).  stats as aces_ccsd_cycle.
+int aces_dodiis0(int icycle, double tol, int* iconvg, double* stats) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  ACES_REQUIRE(c->eng != nullptr, "DODIIS0 before GENINT");
+  CcsdEngine& e = *c->eng;
+  const long ov = (long)e.o * e.v;
+  hipStream_t s = c->stream;
+  ACES_HIP(hipMemcpyAsync(e.T(), c->lists->device_ptr(1, 90), sizeof(double) * ov, hipMemcpyDeviceToDevice, s));
+  ACES_HIP(hipMemcpyAsync(e.T() + ov, c->lists->device_ptr(1, 46), sizeof(double) * ov * ov,
+                          hipMemcpyDeviceToDevice, s));
+  ACES_HIP(hipMemcpyAsync(e.Tnew(), c->lists->device_ptr(3, 90), sizeof(double) * ov, hipMemcpyDeviceToDevice,
+                          s));
+  ACES_HIP(hipMemcpyAsync(e.Tnew() + ov, c->lists->device_ptr(1, 63), sizeof(double) * ov * ov,
+                          hipMemcpyDeviceToDevice, s));
+  CcsdStats st;
+  ACES_CHECK(e.energy_dev(e.Tnew(), e.Tnew() + ov, nullptr));
+  ACES_CHECK(e.diis_step(icycle, tol, st));
+  *iconvg = st.converged ? 0 : 1;
+  if (stats) {
+    stats[0] = st.energy_new;
+    stats[1] = st.resmiter;
+    stats[2] = st.resmdiis;
+    stats[3] = st.converged;
+    stats[4] = st.energy_final;
+  }
+  if (!st.converged) {
+    // extrapolated (or Jacobi) vector -> increment lists (vcc/dodiis0.F "WRITE IT OVER CURRENT INCREMENT LISTS")
+    ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(3, 90), e.T(), sizeof(double) * ov, hipMemcpyDeviceToDevice, s));
+    ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 63), e.T() + ov, sizeof(double) * ov * ov,
+                            hipMemcpyDeviceToDevice, s));
+    ACES_CHECK(abtoaa(e.o, e.v, c->lists->device_ptr(1, 63), c->lists->device_ptr(1, 61), s));
+  } else {
+    // DODIIS0 symmetrised list 46 in place (libr2/symetrize_t2.F)
+    ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 46), e.T() + ov, sizeof(double) * ov * ov,
+                            hipMemcpyDeviceToDevice, s));
+  }
+  ACES_HIP(hipStreamSynchronize(s));
+  return ACES_OK;
+}
+
+int aces_drmove(void) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  ACES_REQUIRE(c->eng != nullptr, "DRMOVE before GENINT");
+  CcsdEngine& e = *c->eng;
+  const long ov = (long)e.o * e.v;
+  hipStream_t s = c->stream;
+  ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 90), c->lists->device_ptr(3, 90), sizeof(double) * ov,
+                          hipMemcpyDeviceToDevice, s));
+  ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 46), c->lists->device_ptr(1, 63), sizeof(double) * ov * ov,
+                          hipMemcpyDeviceToDevice, s));
+  ACES_CHECK(abtoaa(e.o, e.v, c->lists->device_ptr(1, 46), c->lists->device_ptr(1, 44), s));
+  ACES_HIP(hipStreamSynchronize(s));
+  return ACES_OK;
+}
+
+// MP1 start written by xintprc (intprc/dfrmt2.F) into lists 46/44 and T1 = 0.
+int aces_vcc_mp1(void) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  if (!c->eng) ACES_CHECK(setup_engine(c));
+  ACES_CHECK(ensure_amp_lists(c));
+  CcsdEngine& e = *c->eng;
+  const long ov = (long)e.o * e.v;
+  ACES_CHECK(e.start_mp1());
+  ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 90), e.T(), sizeof(double) * ov, hipMemcpyDeviceToDevice,
+                          c->stream));
+  ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 46), e.T() + ov, sizeof(double) * ov * ov,
+                          hipMemcpyDeviceToDevice, c->stream));
+  ACES_CHECK(abtoaa(e.o, e.v, c->lists->device_ptr(1, 46), c->lists->device_ptr(1, 44), c->stream));
+  ACES_HIP(hipStreamSynchronize(c->stream));
+  return ACES_OK;
+}
+
+int aces_vcc_set_diis_order(int order) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  ACES_REQUIRE(c->eng != nullptr, "DIISLST before GENINT/MP1");
+  ACES_CHECK(c->eng->set_diis_order(order));
+  return c->eng->reset_diis();
+}
+
+// Fortran spellings of the driver entry points (signatures of vcc/vcc.F's calls).
+void genint_(i8*, i8*, i8*, i8*, i8*) { fatal(aces_genint(), "GENINT"); }
+void feacont_(i8*, i8*, i8*) {}
+void fmicont_(i8*, i8*, i8*) {}
+void fmecont_(i8*, i8*, i8*, i8*) {}
+void t1int2a_(i8*, i8*, i8*) {}
+void t1int2b_(i8*, i8*, i8*) {}
+void t1int1_(i8*, i8*, i8*, i8*) {}
+void initin_(i8*, i8*, i8*) {}
+void initsn_(i8*, i8*, i8*) {}
+void rnabij_(i8*, i8*, i8*, const char*, size_t) {}
+void dre3en_(i8*, i8*, i8*, i8*) { fatal(aces_dre3en(), "DRE3EN"); }
+void e4s_(i8*, i8*, i8*, double* se4) {
+  fatal(aces_e4s(), "E4S");
+  if (se4) *se4 = 0.0;
+}
+void newt2_(i8*, i8*, i8*) { fatal(aces_newt2(), "NEWT2"); }
+void cmpeng_(i8*, i8*, i8* nlist2, i8* nlist1, double* ecorr, double* etot, double* etott2, i8*, i8* iprint) {
+  fatal(aces_cmpeng((int)*nlist2, (int)*nlist1, ecorr, etot), "CMPENG");
+  if (etott2) *etott2 = *etot;
+  // output line of libr/cmpeng.F (format 81)
+  if (iprint && *iprint) printf("   The total correlation energy is %15.12f a.u.\n", *etot);
+  fflush(stdout);
+}
+void dodiis0_(double*, i8*, i8*, i8*, i8* icycle, i8* iconvg, i8* icontl, i8*, i8*, i8*, i8*, i8*, i8*, i8*,
+              i8*, const char*, double*, size_t) {
+  int conv = 1;
+  double st[8];
+  fatal(aces_dodiis0((int)*icycle, std::pow(10.0, -(double)*icontl), &conv, st), "DODIIS0");
+  *iconvg = conv;
+  // output lines of vcc/dodiis0.F (formats 1001-1004)
+  printf("  Convergence information after %5ld iterations: \n", (long)*icycle);
+  printf("  Largest element of residual vector : %15.8E.\n", st[1]);
+  printf("  Largest element of DIIS residual   : %15.8E.\n", st[2]);
+  if (conv == 0) printf("  Amplitude equations converged in %5ld iterations.\n", (long)*icycle);
+  fflush(stdout);
+}
+void drmove_(i8*, i8*, i8*, i8*, i8*) { fatal(aces_drmove(), "DRMOVE"); }
+
+}  // extern "C"

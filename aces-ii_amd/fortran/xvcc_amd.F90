@@ -1,0 +1,102 @@
+! Closed-shell CCSD with the call sequence of ACES II's CC loop (vcc/vcc.F:798-1437),
+! linked against the MI355X library.  Compile with -fdefault-integer-8 like the
+! reference so INTEGER arguments are 8 bytes.
+!
+! Usage: xvcc_amd <input> [cc_conv] [cc_maxcyc] [cc_exporder]
+! Input (stream, little endian): o, v, nlists (int64); per list: list, typel,
+! typer, nrows, ncols (int64) and nrows*ncols doubles (column major); then the
+! n = o + v orbital energies (record SCFEVALA).
+program xvcc_amd
+  use aces2_amd
+  implicit none
+  integer :: icore(1), maxcor, iuhf, inttyp, icycle, ncycle, iconvg, icontl, iorder
+  integer :: o, v, nl, il, lst, tl, tr, nr, nc, n
+  logical :: sing1, do_hbar
+  double precision :: ecorr(3), etot, etott2, edummy, damp
+  double precision, allocatable :: buf(:)
+  character(len=256) :: fname, arg
+  integer(c_int), target :: pop(8), vrt(8)
+
+  call get_command_argument(1, fname)
+  icontl = 7
+  ncycle = 50
+  iorder = 6
+  if (command_argument_count() >= 2) then
+    call get_command_argument(2, arg)
+    read(arg, *) icontl
+  end if
+  if (command_argument_count() >= 3) then
+    call get_command_argument(3, arg)
+    read(arg, *) ncycle
+  end if
+  if (command_argument_count() >= 4) then
+    call get_command_argument(4, arg)
+    read(arg, *) iorder
+  end if
+  maxcor = 1
+  iuhf = 0
+  inttyp = 2
+  sing1 = .true.
+  do_hbar = .false.
+  damp = 1.0d0
+
+  call aces_check(aces_init(c_null_ptr), 'aces_init')
+  open(10, file=trim(fname), access='stream', form='unformatted', status='old')
+  read(10) o, v, nl
+  pop = 0
+  vrt = 0
+  pop(1) = int(o, c_int)
+  vrt(1) = int(v, c_int)
+  call aces_check(aces_set_symmetry(1_c_int, c_loc(pop), c_null_ptr, c_loc(vrt), c_null_ptr), 'symmetry')
+  do il = 1, nl
+    read(10) lst, tl, tr, nr, nc
+    allocate(buf(nr * nc))
+    read(10) buf
+    call inipck(1, tl, tr, lst, 0, 0, 1)
+    call putlst(buf, 1, nc, 1, 1, lst)
+    deallocate(buf)
+  end do
+  n = o + v
+  allocate(buf(n))
+  read(10) buf
+  close(10)
+  call putrec(20, 'JOBARC', 'SCFEVALA', n, buf)
+  deallocate(buf)
+
+  ! MP1 start (xintprc) and DIIS lists (DIISLST)
+  call aces_check(aces_vcc_mp1(), 'MP1')
+  call aces_check(aces_vcc_set_diis_order(int(iorder, c_int)), 'DIISLST')
+  call cmpeng(icore, maxcor, 43, 0, ecorr, etot, etott2, iuhf, 1)
+
+  iconvg = 1
+  do icycle = 1, ncycle
+    call initin(icore, maxcor, iuhf)
+    call initsn(icore, maxcor, iuhf)
+    call genint(icore, maxcor, iuhf, inttyp, do_hbar)
+    call feacont(icore, maxcor, iuhf)
+    call fmicont(icore, maxcor, iuhf)
+    call fmecont(icore, maxcor, iuhf, 1)
+    call t1int2a(icore, maxcor, iuhf)
+    call t1int2b(icore, maxcor, iuhf)
+    call t1int1(icore, maxcor, iuhf, 1)
+    call dre3en(icore, maxcor, iuhf, 0)
+    call e4s(icore, maxcor, iuhf, edummy)
+    call newt2(icore, maxcor, iuhf)
+    call cmpeng(icore, maxcor, 60, 2, ecorr, etot, etott2, iuhf, 1)
+    call dodiis0(icore, maxcor, iuhf, 1, icycle, iconvg, icontl, sing1, 44, 61, 90, 0, 90, 2, 70, &
+                 '     ', damp)
+    if (iconvg /= 0) then
+      call drmove(icore, maxcor, iuhf, 0, sing1)
+      call rnabij(icore, maxcor, iuhf, 'T')
+    else
+      call cmpeng(icore, maxcor, 43, 0, ecorr, etot, etott2, iuhf, 1)
+      write(6, '(a)') '  The CC iterations have converged.'
+      exit
+    end if
+  end do
+  call aces_check(aces_fin(), 'aces_fin')
+  if (iconvg /= 0) then
+    write(6, '(a)') '@VCC: The Coupled-Cluster equations did not converge!'
+    stop 1
+  end if
+end program xvcc_amd

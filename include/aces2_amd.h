@@ -84,6 +84,70 @@ long aces_ccsd_device_bytes(void* handle);
 int aces_ccsd_set_profiling(void* handle, int on);
 int aces_ccsd_profile(void* handle, double* out, int reset);
 
+/* ------------------------------------------------------------------------
+ * Runtime, list store and record store (device-resident MOINTS/JOBARC).
+ *   aces_init              acescore/aces_init.F:26 (lists + records, one stream)
+ *   aces_set_symmetry      POP/VRT per irrep (COMMON /SYM/, records SYMPOPOA/VA)
+ *   aces_irpdpd            IRPDPD(irrep, type) (acescore/aces_com_symloc.F:11-106)
+ *   aces_isymoff           ISYMOFF(irrep_q, irrep_pq, type), 1-based
+ *   aces_inipck            INIPCK (libr/inipck.F:3-33)
+ *   aces_updmoi            UPDMOI (libr/updmoi.F:11)
+ *   aces_getlst/putlst     GETLST/PUTLST (acescore/getlst.F:19, putlst.F:19)
+ *   aces_getall/putall     GETALL/PUTALL (acesutil/getall.F)
+ *   aces_sumsym2           SUMSYM2 (acesutil/sumsym2.F): list += add
+ *   aces_getrec/putrec     GETREC/PUTREC (acescore/getrec.F:29-60, putrec.F:22),
+ *                          lengths in 8-byte words, iflag semantics of GETREC
+ * Fortran hosts link the same functionality under the reference's names and
+ * ILP64 by-reference conventions: getlst_, putlst_, getall_, putall_,
+ * sumsym2_, inipck_, updmoi_, getrec_, putrec_.
+ * ---------------------------------------------------------------------- */
+int aces_init(void* stream);
+int aces_fin(void);
+int aces_set_symmetry(int nirrep, const int* popa, const int* popb, const int* vrta, const int* vrtb);
+long aces_irpdpd(int irrep, int type);
+long aces_isymoff(int irrep_q, int irrep_pq, int type);
+int aces_inipck(int irrepx, int typel, int typer, int list);
+int aces_updmoi(long ncols, long nrows, int group, int list);
+int aces_list_shape(int group, int list, long* nrows, long* ncols);
+int aces_list_isytyp(int list, int* typel, int* typer);
+int aces_getlst(double* dest, long ifirst, long ncols, int group, int list, int dest_on_device);
+int aces_putlst(const double* src, long ifirst, long ncols, int group, int list, int src_on_device);
+int aces_getall(double* dest, long length, int irrepx, int list, int on_device);
+int aces_putall(const double* src, long length, int irrepx, int list, int on_device);
+int aces_sumsym2(const double* add, int group, int list, int on_device);
+double* aces_list_device_ptr(int group, int list);
+int aces_putrec(const char* name, long nwords, const void* src);
+int aces_getrec(int iflag, const char* name, long* nwords, void* dest);
+
+/* ------------------------------------------------------------------------
+ * xvcc driver entry points over the list store (vcc/vcc.F:811-1085 order).
+ *   aces_vcc_mp1           MP1 start (intprc/dfrmt2.F) into lists 46/44, T1 = 0
+ *   aces_vcc_set_diis_order DIISLST (vcc/diislst.F): CC_EXPORDER vectors
+ *   aces_genint            GENINT (vcc/genint.F:1) + the fused increment terms
+ *   aces_dre3en            DRE3EN (vcc/dre3en.F): T2 increment -> lists 63/61
+ *   aces_e4s               E4S (vcc/e4s.F): new T1 -> list 90 group 3
+ *   aces_newt2             NEWT2 (libr2/newt2.F): list 63 <- 63/D, 61 <- ABTOAA
+ *   aces_cmpeng            CMPENG (libr/cmpeng.F) on lists nlist2+1..+3, T1 g nlist1+1
+ *   aces_dodiis0           DODIIS0 (vcc/dodiis0.F); *iconvg = 0 when converged
+ *   aces_drmove            DRMOVE (libr/drmove.f): T lists <- increment lists
+ * Integral input lists (RHF, C1): 13 <Mn|Ij> (14,14), 16 <Ab|Ij> (13,14),
+ * 10 <Ij|Ka> (14,18), 30 <Ab|Ci> (13,11), 233 <Ab|Cd> (5,13) and
+ * 23 <AJ||BI> (9,9); records SCFEVALA (orbital energies) or FOCKMOA (MO Fock).
+ * Fortran spellings with the reference's argument lists: genint_, feacont_,
+ * fmicont_, fmecont_, t1int2a_, t1int2b_, t1int1_, initin_, initsn_, rnabij_,
+ * dre3en_, e4s_, newt2_, cmpeng_, dodiis0_, drmove_.
+ * ---------------------------------------------------------------------- */
+int aces_vcc_mp1(void);
+int aces_vcc_set_diis_order(int order);
+int aces_vcc_set_precision(int prec);
+int aces_genint(void);
+int aces_dre3en(void);
+int aces_e4s(void);
+int aces_newt2(void);
+int aces_cmpeng(int nlist2, int nlist1, double* ecorr3, double* etot);
+int aces_dodiis0(int icycle, double tol, int* iconvg, double* stats);
+int aces_drmove(void);
+
 #ifdef __cplusplus
 }
 #endif

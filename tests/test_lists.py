@@ -1,0 +1,134 @@
+"""List/record store: DPD addressing (CPU) and device round trips + the xvcc driver (GPU)."""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import rccsd, vcc_loop
+
+# Pair types 1..25 as (left space, right space, left spin, right spin, packing) -- the
+# convention of acescore/aces_com_symloc.F:11-34, enumerated independently below.
+TYPES = {1: ('V', 'V', 0, 0, '<'), 2: ('V', 'V', 1, 1, '<'), 3: ('O', 'O', 0, 0, '<'), 4: ('O', 'O', 1, 1, '<'),
+         5: ('V', 'V', 0, 0, '<='), 6: ('V', 'V', 1, 1, '<='), 7: ('O', 'O', 0, 0, '<='), 8: ('O', 'O', 1, 1, '<='),
+         9: ('V', 'O', 0, 0, ''), 10: ('V', 'O', 1, 1, ''), 11: ('V', 'O', 0, 1, ''), 12: ('V', 'O', 1, 0, ''),
+         13: ('V', 'V', 0, 1, ''), 14: ('O', 'O', 0, 1, ''), 15: ('V', 'V', 0, 1, ''), 16: ('O', 'V', 0, 0, ''),
+         17: ('O', 'V', 1, 1, ''), 18: ('O', 'V', 0, 1, ''), 19: ('V', 'V', 0, 0, ''), 20: ('V', 'V', 1, 1, ''),
+         21: ('O', 'O', 0, 0, ''), 22: ('O', 'O', 1, 1, ''), 23: ('V', 'V', 1, 0, ''), 24: ('O', 'O', 1, 0, ''),
+         25: ('O', 'V', 1, 0, '')}
+
+
+def _brute_irpdpd(pop, vrt, irrep, typ):
+    """Count orbital pairs of total irrep `irrep` (0-based) by enumeration."""
+    l, r, sl, sr, pk = TYPES[typ]
+    orbs = lambda sp, spin: [(h, k) for h in range(len(pop[0])) for k in range((pop if sp == 'O' else vrt)[spin][h])]
+    n = 0
+    P, Q = orbs(l, sl), orbs(r, sr)
+    same = (l == r and sl == sr)
+    for (hp, kp), (hq, kq) in itertools.product(P, Q):
+        if hp ^ hq != irrep:
+            continue
+        if pk:
+            ip, iq = P.index((hp, kp)), Q.index((hq, kq))
+            if not same:
+                continue
+            if (pk == '<' and not ip < iq) or (pk == '<=' and not ip <= iq):
+                continue
+        n += 1
+    return n
+
+
+def test_irpdpd_matches_enumeration():
+    from aces2 import lists
+    lists.init()
+    popa = [3, 1, 2, 0, 2, 1, 1, 0]
+    vrta = [5, 2, 3, 4, 2, 3, 1, 2]
+    popb = [2, 1, 2, 1, 2, 1, 0, 0]
+    vrtb = [6, 2, 3, 3, 2, 3, 2, 2]
+    lists.set_symmetry(popa, vrta, popb, vrtb)
+    pop, vrt = [popa, popb], [vrta, vrtb]
+    for typ in range(1, 26):
+        for irrep in range(8):
+            assert lists.irpdpd(irrep + 1, typ) == _brute_irpdpd(pop, vrt, irrep, typ), (typ, irrep)
+    lists.fin()
+
+
+def _pop8(pop):
+    return list(pop) + [0] * (8 - len(pop))
+
+
+@pytest.mark.gpu
+def test_list_round_trip_and_records():
+    from aces2 import lists
+    print('case list round trip', flush=True)
+    lists.init()
+    lists.set_symmetry([3, 2, 1, 2], [4, 3, 5, 2])
+    lists.inipck(1, 13, 14, 46)
+    rng = np.random.default_rng(3)
+    total = 0
+    for g in range(1, 5):
+        nr, nc = lists.shape(g, 46)
+        assert nr == lists.irpdpd(g, 13) and nc == lists.irpdpd(g, 14)
+        x = rng.normal(size=(nc, nr))
+        lists.putlst(x, 1, nc, g, 46)
+        if nc >= 2:
+            y = lists.getlst(2, nc - 1, g, 46)
+            np.testing.assert_array_equal(y, x[1:])
+        lists.sumsym2(x, g, 46)
+        np.testing.assert_allclose(lists.getlst(1, nc, g, 46), 2 * x)
+        total += nr * nc
+    allv = lists.getall(1, 46, total)
+    assert allv.size == total
+    with pytest.raises(Exception):
+        lists.getlst(1, 10 ** 6, 1, 46)  # GETLST range error (acescore/getlst.F:100-108)
+    lists.putrec('SCFEVALA', np.arange(5.0))
+    np.testing.assert_array_equal(lists.getrec('SCFEVALA', 5), np.arange(5.0))
+    assert lists.reclen('NOSUCHRC') == -1
+    np.testing.assert_array_equal(lists.getrec('NOSUCHRC', 3, iflag=-1), np.zeros(3))
+    lists.fin()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('nonhf', [False, True])
+def test_xvcc_driver_over_lists(nonhf):
+    """The vcc.F call sequence over device lists reproduces the oracle loop."""
+    import ctypes
+    from aces2 import _lib, lists, synth
+    print('case xvcc lists nonhf=%s' % nonhf, flush=True)
+    o, v = 5, 19
+    eps, B = synth.draw(o, v, 31, sigma=synth.scaled_sigma(o, v))
+    pert = None
+    if nonhf:
+        rng = np.random.default_rng(5)
+        P = rng.normal(0, 0.01, (o + v, o + v))
+        pert = P + P.T
+    blk, eri = synth.blocks_numpy(eps, B, o, perturb=pert)
+    ref = vcc_loop.run(blk['fock'], rccsd.blocks(eri, o), o, maxcyc=60, conv=9, order=6)
+    lists.init()
+    lists.load_c1_system(blk, o, v, nonhf=nonhf)
+    _lib.call('aces_vcc_mp1')
+    _lib.call('aces_vcc_set_diis_order', 6)
+    e0 = ctypes.c_double()
+    _lib.call('aces_cmpeng', 43, 0, None, ctypes.byref(e0))
+    energies, resm = [e0.value], []
+    conv = ctypes.c_int(1)
+    st = (ctypes.c_double * 8)()
+    for icycle in range(1, 61):
+        _lib.call('aces_genint')
+        _lib.call('aces_dre3en')
+        _lib.call('aces_e4s')
+        _lib.call('aces_newt2')
+        e = ctypes.c_double()
+        _lib.call('aces_cmpeng', 60, 2, None, ctypes.byref(e))
+        energies.append(e.value)
+        _lib.call('aces_dodiis0', icycle, 1e-9, ctypes.byref(conv), st)
+        resm.append(st[1])
+        if conv.value == 0:
+            break
+        _lib.call('aces_drmove')
+    ef = ctypes.c_double()
+    _lib.call('aces_cmpeng', 43, 0, None, ctypes.byref(ef))
+    lists.fin()
+    assert conv.value == 0 and len(resm) == ref['cycles']
+    np.testing.assert_allclose(energies, ref['energy'], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(resm, ref['resmiter'], rtol=1e-6, atol=1e-13)
+    assert abs(ef.value - ref['final_energy']) < 1e-10
