@@ -1,0 +1,110 @@
+"""Golden vectors of the reference's own regression test zmat.015a (O2, STO-3G, RHF-CCSD).
+
+The fixture holds the reference's printed per-cycle CC energies and largest
+residual elements (test/OUTPUT/out.zmat.015a, first geometry, MP1 start,
+CC_EXPORDER = 5, CC_CONV = 7) and the TEST.DAT values of test/zmat.015a.  The MO
+Hamiltonian is rebuilt from the fixture's basis and geometry (tests/golden/molint.py).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import rccsd, vcc_loop
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, 'golden'))
+sys.path.insert(0, HERE)
+import molint  # noqa: E402
+
+FIX = json.load(open(os.path.join(HERE, 'golden', 'o2_sto3g.json')))
+
+
+def _system(z):
+    basis = [(b['l'], b['exps'], np.array(b['coefs'])) for b in FIX['basis_O']]
+    atoms = [(8.0, [0.0, 0.0, z], basis), (8.0, [0.0, 0.0, -z], basis)]
+    counts = {tuple(int(x) for x in k.strip('()').split(',')): v for k, v in FIX['occupation_d2h'].items()}
+    return molint.mo_system(atoms, FIX['nocc'], counts)
+
+
+@pytest.fixture(scope='module')
+def first_geometry():
+    g = FIX['first_geometry']
+    enuc, escf, F, eri, eps = _system(g['z_bohr'])
+    return g, enuc, escf, F, eri
+
+
+def test_scf_matches_reference(first_geometry):
+    g, enuc, escf, F, eri = first_geometry
+    assert abs(enuc - g['nucrep']) < 1e-9
+    assert abs(escf - g['escf']) < 1e-9
+
+
+def test_oracle_reproduces_reference_cc_trace(first_geometry):
+    g, enuc, escf, F, eri = first_geometry
+    o = FIX['nocc']
+    h = vcc_loop.run(F, rccsd.blocks(eri, o), o, maxcyc=50, conv=g['cc_conv'], order=g['cc_exporder'])
+    assert h['cycles'] == g['cycles']
+    np.testing.assert_allclose(h['energy'], g['mp2_and_cycle_energies'], rtol=0, atol=3e-11)
+    np.testing.assert_allclose(h['resmiter'], g['resmiter'], rtol=3e-7, atol=0)
+    assert abs(h['final_energy'] - g['final_energy']) < 1e-11
+
+
+def test_final_geometry_test_dat():
+    """TEST.DAT SCFENEG to 1e-9; TOTENERG to the convergence level of the reference run."""
+    fg = FIX['final_geometry']
+    enuc, escf, F, eri, eps = _system(fg['z_bohr'])
+    assert abs(escf - fg['scfeneg']) < 1e-9
+    o = FIX['nocc']
+    h = vcc_loop.run(F, rccsd.blocks(eri, o), o, maxcyc=80, conv=10, order=5)
+    # the reference's last cycle at this geometry had |RESMITER| = 5.6e-8 (out.zmat.015a:4367)
+    assert abs(escf + h['final_energy'] - fg['totenerg']) < 3e-8
+
+
+@pytest.mark.gpu
+def test_gpu_engine_reproduces_reference_cc_trace(first_geometry):
+    from aces2.ccsd import CCSD
+    print('case golden O2 gpu', flush=True)
+    g, enuc, escf, F, eri = first_geometry
+    o = FIX['nocc']
+    blk = rccsd.blocks(eri, o)
+    from aces2 import synth
+    blk = dict(blk)
+    blk['fock'] = F
+    blk['list233'] = synth.list233_from_eri(eri, o)
+    cc = CCSD(o, F.shape[0] - o, blk)
+    h = cc.run(maxcyc=50, conv=g['cc_conv'], order=g['cc_exporder'])
+    cc.close()
+    assert h['cycles'] == g['cycles']
+    np.testing.assert_allclose(h['energy'], g['mp2_and_cycle_energies'], rtol=0, atol=3e-11)
+    np.testing.assert_allclose(h['resmiter'], g['resmiter'], rtol=3e-7, atol=0)
+    assert abs(h['final_energy'] - g['final_energy']) < 1e-11
+
+
+@pytest.mark.gpu
+def test_fortran_host_reproduces_reference_output(first_geometry, tmp_path):
+    """vcc.F's call sequence (flang host) over the device lists prints the reference's trace."""
+    import re
+    import subprocess
+    from aces2 import synth
+    from test_fortran_gpu import EXE, write_input
+    print('case golden O2 fortran', flush=True)
+    g, enuc, escf, F, eri = first_geometry
+    o = FIX['nocc']
+    v = F.shape[0] - o
+    blk = dict(rccsd.blocks(eri, o))
+    blk['fock'] = F
+    blk['list233'] = synth.list233_from_eri(eri, o)
+    inp = tmp_path / 'o2.bin'
+    write_input(str(inp), blk, o, v)
+    out = subprocess.run([EXE, str(inp), str(g['cc_conv']), '50', str(g['cc_exporder'])], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    eng = [float(x) for x in re.findall(r'The total correlation energy is\s+(\S+) a\.u\.', out.stdout)]
+    res = [float(x.rstrip('.')) for x in re.findall(r'Largest element of residual vector :\s+(\S+)', out.stdout)]
+    np.testing.assert_allclose(eng[:-1], g['mp2_and_cycle_energies'], rtol=0, atol=3e-11)
+    np.testing.assert_allclose(res, g['resmiter'], rtol=3e-7, atol=0)
+    assert abs(eng[-1] - g['final_energy']) < 1e-11
+    assert 'Amplitude equations converged in    12 iterations.' in out.stdout
