@@ -40,6 +40,13 @@ SIGNATURES = {
     'aces_ccsd_device_bytes': (c_long, [c_vp]),
     'aces_ccsd_set_profiling': (c_int, [c_vp, c_int]),
     'aces_ccsd_profile': (c_int, [c_vp, c_dp, c_int]),
+    'aces_ccsd_adjoint': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    'aces_ccsd_lambda_start': (c_int, [c_vp]),
+    'aces_ccsd_lambda_cycle': (c_int, [c_vp, c_int, c_double, c_dp]),
+    'aces_ccsd_set_lambda_diis_order': (c_int, [c_vp, c_int]),
+    'aces_ccsd_get_lambda': (c_int, [c_vp, c_vp, c_vp, c_int]),
+    'aces_ccsd_set_lambda': (c_int, [c_vp, c_vp, c_vp, c_int]),
+    'aces_ccsd_density': (c_int, [c_vp, c_vp]),
     'aces_init': (c_int, [c_vp]),
     'aces_fin': (c_int, []),
     'aces_set_symmetry': (c_int, [c_int, c_vp, c_vp, c_vp, c_vp]),

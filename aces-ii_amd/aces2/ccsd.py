@@ -19,6 +19,10 @@ class NotConverged(RuntimeError):
     """The Coupled-Cluster equations did not converge (``vcc/vcc.F:1439``)."""
 
 
+class LambdaNotConverged(RuntimeError):
+    """The Lambda equations did not converge (``lambda/vlamcc.F``)."""
+
+
 class CCSD:
     def __init__(self, o, v, blocks, stream=None):
         import torch
@@ -131,4 +135,75 @@ class CCSD:
                 break
         if not hist['converged'] and raise_on_fail:
             raise NotConverged('@VCC: The Coupled-Cluster equations did not converge!')
+        return hist
+
+    # ---- differentiation form ----
+    def adjoint(self, l1, l2, density=False):
+        """r = dL/dT for spatial multipliers (l1, l2) at the engine's amplitudes;
+        with ``density`` also dL/dfock (the response one-particle density)."""
+        torch = self.torch
+        o, v = self.o, self.v
+        n = o + v
+        l1 = torch.as_tensor(l1, device='cuda', dtype=torch.float64).contiguous()
+        l2 = torch.as_tensor(l2, device='cuda', dtype=torch.float64).contiguous()
+        r1 = torch.empty((o, v), device='cuda', dtype=torch.float64)
+        r2 = torch.empty((o, o, v, v), device='cuda', dtype=torch.float64)
+        dens = torch.empty((n, n), device='cuda', dtype=torch.float64) if density else None
+        _lib.call('aces_ccsd_adjoint', self.h, l1.data_ptr(), l2.data_ptr(), r1.data_ptr(), r2.data_ptr(),
+                  dens.data_ptr() if density else None)
+        return (r1, r2, dens) if density else (r1, r2)
+
+    def lambda_start(self):
+        _lib.call('aces_ccsd_lambda_start', self.h)
+
+    def set_lambda_diis_order(self, order):
+        _lib.call('aces_ccsd_set_lambda_diis_order', self.h, order)
+
+    def lambda_cycle(self, icycle, tol=1e-7):
+        st = (ctypes.c_double * 8)()
+        _lib.call('aces_ccsd_lambda_cycle', self.h, icycle, tol, st)
+        return {'resmiter': st[1], 'resmdiis': st[2], 'converged': bool(st[3]), 'ms_cycle': st[6]}
+
+    def get_lambda(self, aces_norm=False):
+        torch = self.torch
+        o, v = self.o, self.v
+        l1 = torch.empty((o, v), device='cuda', dtype=torch.float64)
+        l2 = torch.empty((o, o, v, v), device='cuda', dtype=torch.float64)
+        _lib.call('aces_ccsd_get_lambda', self.h, l1.data_ptr(), l2.data_ptr(), 1 if aces_norm else 0)
+        return l1, l2
+
+    def set_lambda(self, l1, l2, aces_norm=False):
+        torch = self.torch
+        l1 = torch.as_tensor(l1, device='cuda', dtype=torch.float64).contiguous()
+        l2 = torch.as_tensor(l2, device='cuda', dtype=torch.float64).contiguous()
+        _lib.call('aces_ccsd_set_lambda', self.h, l1.data_ptr(), l2.data_ptr(), 1 if aces_norm else 0)
+
+    def density(self):
+        """Response one-particle density dE_corr/dfock of the current multipliers."""
+        torch = self.torch
+        n = self.o + self.v
+        d = torch.empty((n, n), device='cuda', dtype=torch.float64)
+        _lib.call('aces_ccsd_density', self.h, d.data_ptr())
+        return d
+
+    def solve_lambda(self, maxcyc=50, conv=7, order=6, log=None, raise_on_fail=True):
+        """xlambda's loop (``lambda/vlamcc.F:599-830``) at the engine's amplitudes."""
+        self.lambda_start()
+        self.set_lambda_diis_order(order)
+        self.lambda_start()
+        hist = {'resmiter': [], 'resmdiis': [], 'ms_cycle': [], 'converged': False}
+        tol = 10.0 ** (-conv)
+        for icycle in range(1, maxcyc + 1):
+            st = self.lambda_cycle(icycle, tol)
+            hist['resmiter'].append(st['resmiter'])
+            hist['resmdiis'].append(st['resmdiis'])
+            hist['ms_cycle'].append(st['ms_cycle'])
+            if log:
+                log(icycle, st)
+            if st['converged']:
+                hist['converged'] = True
+                hist['cycles'] = icycle
+                break
+        if not hist['converged'] and raise_on_fail:
+            raise LambdaNotConverged('The Lambda equations did not converge')
         return hist

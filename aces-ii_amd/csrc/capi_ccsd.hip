@@ -107,4 +107,50 @@ int aces_ccsd_profile(void* h, double* out, int reset) {
   return ACES_OK;
 }
 
+int aces_ccsd_adjoint(void* h, const double* l1, const double* l2, double* r1, double* r2, double* dens) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->adjoint(l1, l2, r1, r2, dens);
+}
+
+int aces_ccsd_lambda_start(void* h) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->lambda_start();
+}
+
+int aces_ccsd_lambda_cycle(void* h, int icycle, double tol, double* stats) {
+  REQUIRE_HANDLE(h);
+  CcsdStats st;
+  const int rc = eng(h)->lambda_cycle(icycle, tol, st);
+  if (rc == ACES_OK && stats) {
+    stats[0] = 0.0;
+    stats[1] = st.resmiter;
+    stats[2] = st.resmdiis;
+    stats[3] = st.converged;
+    stats[4] = 0.0;
+    stats[5] = st.ms_increments;
+    stats[6] = st.ms_cycle;
+  }
+  return rc;
+}
+
+int aces_ccsd_set_lambda_diis_order(void* h, int order) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->set_lambda_diis_order(order);
+}
+
+int aces_ccsd_get_lambda(void* h, double* l1, double* l2, int aces_norm) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->get_lambda(l1, l2, aces_norm);
+}
+
+int aces_ccsd_set_lambda(void* h, const double* l1, const double* l2, int aces_norm) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->set_lambda(l1, l2, aces_norm);
+}
+
+int aces_ccsd_density(void* h, double* dens) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->density(dens);
+}
+
 }  // extern "C"

@@ -44,7 +44,8 @@ class CcsdEngine {
   // Pieces of one cycle, used by the list-level driver entry points.
   int jacobi_step();
   int diis_step(int icycle, double tol, CcsdStats& st);
-  double* T() { return t1_; }        // [t1 | t2] current amplitudes
+  // Writable views; writing through T() invalidates the resident forward intermediates.
+  double* T() { fwd_valid_ = false; return t1_; }  // [t1 | t2] current amplitudes
   double* Tnew() { return t1n_; }    // [t1 | t2] Jacobi update
   double* Inc() { return inc1_; }    // [inc1 | inc2]
   int energy_dev(const double* t1, const double* t2, double* d_out);
@@ -56,6 +57,28 @@ class CcsdEngine {
   void set_profiling(bool on) { profile_ = on; }
   void profile(double* out, int reset);
   long device_bytes() const { return bytes_; }
+
+  // ---- differentiation form (ccsd_adjoint.hip) ----
+  // Lagrangian L = E(T) + sum l1*Omega1 + sum l2*Omega2, Omega = inc - D*T, with
+  // spatial multipliers l1[i][a], l2[i][j][a][b].  adjoint() is the reverse pass
+  // of compute_increments() plus the energy: r = dL/dT at the amplitudes whose
+  // forward intermediates are resident (r2 symmetrised); with dens != nullptr
+  // it also returns dL/dfock, the response one-particle density (n x n, symmetric).
+  int adjoint(const double* l1, const double* l2, double* r1, double* r2, double* dens);
+  // xlambda loop (lambda/vlamcc.F:599-830): lambda_start() forms the forward
+  // intermediates at the current amplitudes and sets Lambda2 = T2, Lambda1 = 0;
+  // each lambda_cycle() is one Jacobi step plus the Lambda DODIIS0.
+  int lambda_start();
+  int lambda_cycle(int icycle, double tol, CcsdStats& st);
+  int set_lambda_diis_order(int order);
+  // Multipliers in/out: spatial l1/l2 (device) or the reference's lists
+  // (L1 = Lambda1 of list 190, L2 = Lambda(Ij,Ab) of list 146).
+  int get_lambda(double* l1, double* l2, int aces_normalisation);
+  int set_lambda(const double* l1, const double* l2, int aces_normalisation);
+  // Response density at the current multipliers (n x n, device).
+  int density(double* dens);
+  double* Lam() { return Lam_; }     // [Lambda1 | Lambda2(AB)] current
+  double* Lamnew() { return Lamn_; }
 
   int o, v, n;
 
@@ -70,6 +93,19 @@ class CcsdEngine {
   int gemm64(GemmDesc d);  // fp64 GEMM with the engine's split-K workspace
   int gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda, const double* B64,
                    int bslot, long ldb, double* C, long ldc, double alpha, double beta);
+  struct DiisState {
+    int order = 6;
+    std::vector<double*> sol, err;
+    std::vector<double> R;  // order x order (row major)
+  };
+  // DODIIS0 core on (x, xn): error, B matrix, dgesv, DIIS residual and
+  // extrapolation into x.  lambda_rule: converged when |RESMITER| or
+  // |RESMDIIS| < tol and the DIIS residual summed in slot order
+  // (lambda/dodiis0.F); otherwise the amplitude rule of vcc/dodiis0.F.
+  int diis_core(DiisState& D, double* x, double* xn, int icycle, double tol, bool lambda_rule, CcsdStats& st);
+  int diis_resize(DiisState& D, int order);
+  int adjoint_alloc();
+  int compute_adjoint(const double* lam, bool density);
 
   hipStream_t s_;
   long bytes_ = 0;
@@ -106,14 +142,22 @@ class CcsdEngine {
   };
   LowP lp_[12];
   GemmPrec lp_static_prec_ = GP_F64;
-  // DIIS
-  int order_ = 6;
-  std::vector<double*> sol_, err_;
-  std::vector<double> R_;  // order x order (row major)
+  // DIIS (amplitudes and multipliers)
+  DiisState tdiis_, ldiis_;
   double* err_cur_ = nullptr;
   double* newvec_ = nullptr;
   const double** d_ptrs_ = nullptr;  // device pointer table
   double* d_coef_ = nullptr;
+
+  // adjoint buffers (allocated on first use)
+  bool adj_ready_ = false;
+  bool fwd_valid_ = false;  // forward intermediates belong to the current t1_/t2_
+  double *lam_ = nullptr, *gT_ = nullptr, *Lam_ = nullptr, *Lamn_ = nullptr;
+  double *gtau_, *gU_, *gX_, *gUt_, *gXt_, *gWt_, *gWv_, *gRS_, *gR2_, *gP1_, *gP2_, *gQ12_, *gQ2_;
+  double *gZp_, *gZm_, *gSp_, *gDp_, *gYb_, *gL1_, *gWo_, *gQo_, *gZH_, *gtmpL_, *gT1Q_, *gQb_, *gG4_, *gG4A_;
+  double *gS3_, *gtauK_;
+  double *gFovI_, *gFooI_, *gFvvI_, *gLoo_, *gLvv_, *gFT_, *gFovT_, *gGk_;
+  double *gfo_, *gfv_, *gfov_, *geo_, *gev_;
 };
 
 }  // namespace aces

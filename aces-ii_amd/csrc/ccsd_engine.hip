@@ -18,44 +18,13 @@
 #include "ccsd_kernels.h"
 #include "common.h"
 #include "tensor_ops.h"
+#include "engine_util.h"
 
 namespace aces {
 
 namespace {
-struct P4 {
-  Permute4 p;
-  int ns = 0;
-  P4(long a, long b, long c, long d, double* out, double beta = 0.0) {
-    p.d[0] = a; p.d[1] = b; p.d[2] = c; p.d[3] = d;
-    p.out = out;
-    p.beta = beta;
-  }
-  P4& src(const double* x, double al, long s0, long s1, long s2, long s3) {
-    p.in[ns] = x;
-    p.alpha[ns] = al;
-    p.st[ns][0] = s0; p.st[ns][1] = s1; p.st[ns][2] = s2; p.st[ns][3] = s3;
-    ++ns;
-    return *this;
-  }
-  P4& prod(const double* X, long x0, long x1, long x2, long x3, const double* Y, long y0, long y1, long y2, long y3,
-           double g) {
-    p.X = X; p.Y = Y; p.gamma = g;
-    p.xs[0] = x0; p.xs[1] = x1; p.xs[2] = x2; p.xs[3] = x3;
-    p.ys[0] = y0; p.ys[1] = y1; p.ys[2] = y2; p.ys[3] = y3;
-    return *this;
-  }
-  int run(hipStream_t s) { return permute4(p, s); }
-};
-
-GemmDesc gd(int M, int N, int K, const double* A, long lda, int akc, const double* B, long ldb, int bkc, double* C,
-            long ldc, double alpha, double beta) {
-  GemmDesc d;
-  d.M = M; d.N = N; d.K = K;
-  d.A = A; d.lda = lda; d.a_kc = akc;
-  d.B = B; d.ldb = ldb; d.b_kc = bkc;
-  d.C = C; d.ldc = ldc; d.alpha = alpha; d.beta = beta;
-  return d;
-}
+using eu::P4;
+using eu::gd;
 
 inline long pad8(long x) { return (x + 7) / 8 * 8; }
 
@@ -248,29 +217,31 @@ int CcsdEngine::setup(const double* fock, const double* ovov, const double* oovv
   ACES_CHECK(alloc(&newvec_, nt));
   ACES_CHECK(alloc_raw((void**)&d_ptrs_, 64 * sizeof(double*)));
   ACES_CHECK(alloc(&d_coef_, 64));
-  ACES_CHECK(set_diis_order(order_));
+  ACES_CHECK(set_diis_order(tdiis_.order));
   ACES_HIP(hipStreamSynchronize(s));
   ready_ = true;
   return ACES_OK;
 }
 
-int CcsdEngine::set_diis_order(int order) {
+int CcsdEngine::diis_resize(DiisState& D, int order) {
   ACES_REQUIRE(order >= 1 && order <= 25, "DIIS order must be 1..25 (ld_diis = 25, vcc/dodiis0.F)");
   const long nt = (long)o * v + (long)o * o * v * v;
-  while ((int)sol_.size() < order) {
+  while ((int)D.sol.size() < order) {
     double *a, *b;
     ACES_CHECK(alloc(&a, nt));
     ACES_CHECK(alloc(&b, nt));
-    sol_.push_back(a);
-    err_.push_back(b);
+    D.sol.push_back(a);
+    D.err.push_back(b);
   }
-  order_ = order;
-  R_.assign((size_t)order * order, 0.0);
+  D.order = order;
+  D.R.assign((size_t)order * order, 0.0);
   return ACES_OK;
 }
 
+int CcsdEngine::set_diis_order(int order) { return diis_resize(tdiis_, order); }
+
 int CcsdEngine::reset_diis() {
-  R_.assign((size_t)order_ * order_, 0.0);
+  tdiis_.R.assign((size_t)tdiis_.order * tdiis_.order, 0.0);
   return ACES_OK;
 }
 
@@ -280,6 +251,7 @@ int CcsdEngine::set_amplitudes(const double* t1, const double* t2) {
   ACES_HIP(hipMemcpyAsync(t1_, t1, sizeof(double) * ov, hipMemcpyDeviceToDevice, s_));
   ACES_HIP(hipMemcpyAsync(t2_, t2, sizeof(double) * ov * ov, hipMemcpyDeviceToDevice, s_));
   ACES_HIP(hipStreamSynchronize(s_));
+  fwd_valid_ = false;
   return ACES_OK;
 }
 
@@ -300,6 +272,7 @@ int CcsdEngine::start_mp1() {
   ACES_CHECK(P4(o, o, v, v, inc2_).src(ovov_, 1, ov2, v, ov, 1).run(s_));
   ACES_CHECK(jacobi(o, v, eo_, ev_, inc1_, inc2_, t1_, t2_, s_));
   ACES_HIP(hipStreamSynchronize(s_));
+  fwd_valid_ = false;
   return ACES_OK;
 }
 
@@ -552,6 +525,8 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(gemv(OV, OV, W0t_, ov, t1, inc1_, 1.0, 1.0, s));
   ACES_CHECK(gemm64(gd(o, v, (int)ov2, tau_, ov2, 1, M3_, ov2, 1, inc1_, v, 1.0, 1.0)));
   ACES_CHECK(gemm64(gd(o, v, (int)o2v, M7_, o2v, 1, tauK_, o2v, 1, inc1_, v, -1.0, 1.0)));
+  // the reverse pass needs fp64 intermediates of these amplitudes
+  fwd_valid_ = (prec == GP_F64);
   return ACES_OK;
 }
 
@@ -600,17 +575,32 @@ int CcsdEngine::jacobi_step() { return jacobi(o, v, eo_, ev_, inc1_, inc2_, t1n_
 // the energy of T_new computed by the caller (CMPENG(60,2)).  On return t1_/t2_
 // hold the next amplitudes unless converged.
 int CcsdEngine::diis_step(int icycle, double tol, CcsdStats& st) {
-  const long nt = (long)o * v + (long)o * o * v * v;
-  ACES_CHECK(symmetrize_t2(o, v, t2_, s_));
-  ACES_CHECK(symmetrize_t2(o, v, t2n_, s_));
-  const int ndimr = std::min(icycle, order_);
-  const int idimr = 1 + (icycle - 1) % order_;  // 1-based slot
-  ACES_CHECK(sub(nt, t1n_, t1_, err_cur_, s_));
+  ACES_CHECK(diis_core(tdiis_, t1_, t1n_, icycle, tol, false, st));
+  if (st.converged) {
+    // CMPENG(43,0) on the amplitudes of this cycle; amplitudes are not moved.
+    double e;
+    ACES_CHECK(energy_of(t1_, t2_, e));
+    st.energy_final = e;
+  }
+  fwd_valid_ = false;
+  return ACES_OK;
+}
+
+int CcsdEngine::diis_core(DiisState& D, double* x, double* xn, int icycle, double tol, bool lambda_rule,
+                          CcsdStats& st) {
+  const long ov = (long)o * v;
+  const long nt = ov + ov * ov;
+  const int order = D.order;
+  ACES_CHECK(symmetrize_t2(o, v, x + ov, s_));
+  ACES_CHECK(symmetrize_t2(o, v, xn + ov, s_));
+  const int ndimr = std::min(icycle, order);
+  const int idimr = 1 + (icycle - 1) % order;  // 1-based slot
+  ACES_CHECK(sub(nt, xn, x, err_cur_, s_));
   ACES_CHECK(diis_absmax(o, v, err_cur_, red_, scal_ + 1, s_));
-  ACES_HIP(hipMemcpyAsync(sol_[idimr - 1], t1_, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
+  ACES_HIP(hipMemcpyAsync(D.sol[idimr - 1], x, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
   // dots <err_i, e> for i = 1..ndimr (slot idimr uses e itself)
   std::vector<const double*> xs(ndimr);
-  for (int i = 1; i <= ndimr; ++i) xs[i - 1] = (i == idimr) ? err_cur_ : err_[i - 1];
+  for (int i = 1; i <= ndimr; ++i) xs[i - 1] = (i == idimr) ? err_cur_ : D.err[i - 1];
   ACES_HIP(hipMemcpyAsync(d_ptrs_, xs.data(), sizeof(double*) * ndimr, hipMemcpyHostToDevice, s_));
   ACES_CHECK(diis_dots(o, v, ndimr, d_ptrs_, err_cur_, red_, scal_ + 8, s_));
   double h[40];
@@ -619,17 +609,17 @@ int CcsdEngine::diis_step(int icycle, double tol, CcsdStats& st) {
   st.energy_new = h[0];
   st.resmiter = h[1];
   for (int i = 1; i <= ndimr; ++i) {
-    R_[(size_t)(i - 1) * order_ + (idimr - 1)] = h[8 + i - 1];
-    R_[(size_t)(idimr - 1) * order_ + (i - 1)] = h[8 + i - 1];
+    D.R[(size_t)(i - 1) * order + (idimr - 1)] = h[8 + i - 1];
+    D.R[(size_t)(idimr - 1) * order + (i - 1)] = h[8 + i - 1];
   }
-  ACES_HIP(hipMemcpyAsync(err_[idimr - 1], err_cur_, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
+  ACES_HIP(hipMemcpyAsync(D.err[idimr - 1], err_cur_, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
   // DODIIS: augmented Pulay system
   const int na = ndimr + 1;
   std::vector<double> A((size_t)na * na, 0.0), b(na, 0.0);
   for (int j = 1; j < na; ++j) {
     A[j] = -1.0;
     A[(size_t)j * na] = -1.0;
-    for (int i = 1; i < na; ++i) A[(size_t)i * na + j] = R_[(size_t)(i - 1) * order_ + (j - 1)];
+    for (int i = 1; i < na; ++i) A[(size_t)i * na + j] = D.R[(size_t)(i - 1) * order + (j - 1)];
   }
   b[0] = -1.0;
   if (solve_small(na, A, b) != ACES_OK) {
@@ -637,18 +627,26 @@ int CcsdEngine::diis_step(int icycle, double tol, CcsdStats& st) {
     return ACES_ERR_SOLVE;
   }
   std::vector<double> c(b.begin() + 1, b.end());
-  // DIIS residual in the reference's summation order
+  // DIIS residual: the amplitude DODIIS0 sums slot idimr first, then the
+  // others downwards; the Lambda DODIIS0 sums slots 1..ndimr in order.
   std::vector<const double*> rv;
   std::vector<double> rc;
-  rv.push_back(err_[idimr - 1]);
-  rc.push_back(c[idimr - 1]);
-  for (int i = ndimr; i > idimr; --i) {
-    rv.push_back(err_[i - 1]);
-    rc.push_back(c[i - 1]);
-  }
-  for (int i = idimr - 1; i >= 1; --i) {
-    rv.push_back(err_[i - 1]);
-    rc.push_back(c[i - 1]);
+  if (lambda_rule) {
+    for (int i = 1; i <= ndimr; ++i) {
+      rv.push_back(D.err[i - 1]);
+      rc.push_back(c[i - 1]);
+    }
+  } else {
+    rv.push_back(D.err[idimr - 1]);
+    rc.push_back(c[idimr - 1]);
+    for (int i = ndimr; i > idimr; --i) {
+      rv.push_back(D.err[i - 1]);
+      rc.push_back(c[i - 1]);
+    }
+    for (int i = idimr - 1; i >= 1; --i) {
+      rv.push_back(D.err[i - 1]);
+      rc.push_back(c[i - 1]);
+    }
   }
   ACES_HIP(hipMemcpyAsync(d_ptrs_, rv.data(), sizeof(double*) * rv.size(), hipMemcpyHostToDevice, s_));
   ACES_HIP(hipMemcpyAsync(d_coef_, rc.data(), sizeof(double) * rc.size(), hipMemcpyHostToDevice, s_));
@@ -657,26 +655,25 @@ int CcsdEngine::diis_step(int icycle, double tol, CcsdStats& st) {
   ACES_HIP(hipMemcpyAsync(h, scal_ + 2, sizeof(double), hipMemcpyDeviceToHost, s_));
   ACES_HIP(hipStreamSynchronize(s_));
   st.resmdiis = h[0];
-  st.converged = std::fabs(st.resmiter) < tol ? 1 : 0;
-  if (st.converged) {
-    // CMPENG(43,0) on the amplitudes of this cycle; amplitudes are not moved.
-    double e;
-    ACES_CHECK(energy_of(t1_, t2_, e));
-    st.energy_final = e;
-  } else if (ndimr >= 2) {
+  if (lambda_rule)
+    st.converged = (std::fabs(st.resmiter) < tol || std::fabs(st.resmdiis) < tol) ? 1 : 0;
+  else
+    st.converged = std::fabs(st.resmiter) < tol ? 1 : 0;
+  if (st.converged) return ACES_OK;
+  if (ndimr >= 2) {
     std::vector<const double*> sv;
     std::vector<double> sc;
     sv.push_back(newvec_);
     sc.push_back(1.0);
     for (int i = 1; i <= ndimr; ++i) {
-      sv.push_back(sol_[i - 1]);
+      sv.push_back(D.sol[i - 1]);
       sc.push_back(c[i - 1]);
     }
     ACES_HIP(hipMemcpyAsync(d_ptrs_, sv.data(), sizeof(double*) * sv.size(), hipMemcpyHostToDevice, s_));
     ACES_HIP(hipMemcpyAsync(d_coef_, sc.data(), sizeof(double) * sc.size(), hipMemcpyHostToDevice, s_));
-    ACES_CHECK(lincomb(nt, (int)sv.size(), d_ptrs_, d_coef_, t1_, s_));
+    ACES_CHECK(lincomb(nt, (int)sv.size(), d_ptrs_, d_coef_, x, s_));
   } else {
-    ACES_HIP(hipMemcpyAsync(t1_, t1n_, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
+    ACES_HIP(hipMemcpyAsync(x, xn, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
   }
   return ACES_OK;
 }

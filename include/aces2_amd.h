@@ -85,6 +85,41 @@ int aces_ccsd_set_profiling(void* handle, int on);
 int aces_ccsd_profile(void* handle, double* out, int reset);
 
 /* ------------------------------------------------------------------------
+ * Differentiation form: Lambda equations and response density (reverse pass
+ * of the engine's increments).  Replaces the body of the xlambda loop
+ * (lambda/vlamcc.F:599-830: INITIN/INITSN, LGENINT lambda/lgenint.F,
+ * F1INL2/F2INL2, G1INL2/G2INL2, L1INL2A/B, L1INL1, GT1INL1, GWINL1,
+ * L2INL2 lambda/l2inl2.F:26-54, L1W1, L2INL1, NEWT2, lambda/dodiis0.F) and
+ * the one-particle density of xdens (DENSOO/DENSVV/DENSVO, dens/vdens.F:362-388).
+ *
+ * Lagrangian L = E(T) + sum l1*Omega1 + sum l2*Omega2 with Omega = inc - D*T and
+ * spatial multipliers l1[i][a], l2[i][j][a][b]; the reference's lists hold
+ * L1 = l1/2 (list 190) and L2(Ij,Ab) = (2 l2 + l2(i<->j))/3 (list 146).
+ *   aces_ccsd_adjoint        r = dL/dT at the engine's amplitudes for given l
+ *                            (r2 symmetrised) and, if d_dens != NULL, the
+ *                            response density dL/dfock (n x n, symmetric).
+ *   aces_ccsd_lambda_start   forward intermediates at the current amplitudes,
+ *                            L2 = T2, L1 = 0 (vcc/vcc.F:320-325, vcc/init2.F:45-53)
+ *   aces_ccsd_lambda_cycle   one Jacobi step + Lambda DODIIS0 (converged when
+ *                            |RESMITER| or |RESMDIIS| < tol); stats as for
+ *                            aces_ccsd_cycle (stats[0], stats[4] unused)
+ *   aces_ccsd_get_lambda / aces_ccsd_set_lambda   aces_norm = 1: lists 190/146
+ *                            normalisation, 0: spatial l1/l2
+ *   aces_ccsd_density        response density of the current multipliers:
+ *                            dE_corr/dfock_pq for a perturbation of the MO Fock
+ *                            matrix at fixed integrals (n x n, symmetric; the
+ *                            reference part 2 delta_ij is not included)
+ * ---------------------------------------------------------------------- */
+int aces_ccsd_adjoint(void* handle, const double* d_l1, const double* d_l2, double* d_r1, double* d_r2,
+                      double* d_dens);
+int aces_ccsd_lambda_start(void* handle);
+int aces_ccsd_lambda_cycle(void* handle, int icycle, double tol, double* stats);
+int aces_ccsd_set_lambda_diis_order(void* handle, int order);
+int aces_ccsd_get_lambda(void* handle, double* d_l1, double* d_l2, int aces_norm);
+int aces_ccsd_set_lambda(void* handle, const double* d_l1, const double* d_l2, int aces_norm);
+int aces_ccsd_density(void* handle, double* d_dens);
+
+/* ------------------------------------------------------------------------
  * Runtime, list store and record store (device-resident MOINTS/JOBARC).
  *   aces_init              acescore/aces_init.F:26 (lists + records, one stream)
  *   aces_set_symmetry      POP/VRT per irrep (COMMON /SYM/, records SYMPOPOA/VA)

@@ -5,7 +5,9 @@ stored is data: the O STO-3G basis of test/KEEP/GENBAS, the geometry implied
 by the printed nuclear repulsion energy, the SCF occupation by D2h irrep, and
 the per-cycle CC energies and largest residual elements printed in
 test/OUTPUT/out.zmat.015a (first geometry, MP1 start, CC_EXPORDER = 5,
-CC_CONV = 7), plus the TEST.DAT values of test/zmat.015a.
+CC_CONV = 7), the xlambda trace that follows (largest residual and DIIS
+residual elements per Lambda cycle, cycle count) and the xdens natural orbital
+occupation numbers of that geometry, plus the TEST.DAT values of test/zmat.015a.
 """
 import json
 import os
@@ -37,6 +39,25 @@ def main():
         if m:
             resid.append(float(m.group(1)))
     cycles = int(re.search(r'converged in\s+(\d+)', lines[i1]).group(1))
+    # xlambda and xdens of the first geometry
+    j0 = next(k for k, s in enumerate(lines) if 'Entering xlambda' in s)
+    j1 = next(k for k, s in enumerate(lines) if 'Lambda equations converged' in s)
+    lres, ldiis = [], []
+    for s in lines[j0:j1]:
+        m = re.search(r'Largest element of residual vector :\s+(\S+)\.$', s)
+        if m:
+            lres.append(float(m.group(1)))
+        m = re.search(r'Largest element of DIIS residual\s+:\s+(\S+)\.$', s)
+        if m:
+            ldiis.append(float(m.group(1)))
+    lcycles = int(re.search(r'converged in\s+(\d+)', lines[j1]).group(1))
+    k0 = next(k for k, s in enumerate(lines) if 'Natural orbital occupation numbers' in s)
+    nocc_vals = []
+    for s in lines[k0 + 2:k0 + 10]:
+        if 'Trace' in s or s.strip().startswith('---'):
+            break
+        nocc_vals += [float(x) for x in s.split()]
+    trace = float(re.search(r'Trace of density matrix :\s+(\S+)\.', '\n'.join(lines[k0:k0 + 10])).group(1))
     with open(os.path.join(REF, 'zmat.015a')) as f:
         zm = f.read()
     totenerg = float(re.search(r'd TOTENERG\s+(\S+)', zm).group(1))
@@ -53,6 +74,11 @@ def main():
             'final_energy': energies[cycles + 1],
             'resmiter': resid[:cycles],
             'cycles': cycles,
+            'lambda_resmiter': lres[:lcycles],
+            'lambda_resmdiis': ldiis[:lcycles],
+            'lambda_cycles': lcycles,
+            'natural_occupations': nocc_vals,
+            'density_trace': trace,
         },
         'final_geometry': {'z_bohr': coords[2], 'totenerg': totenerg, 'scfeneg': scfeneg},
         'occupation_d2h': {'(1,1,1)': 3, '(-1,1,1)': 1, '(1,-1,1)': 1, '(1,1,-1)': 2, '(1,-1,-1)': 1},

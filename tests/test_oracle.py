@@ -189,3 +189,135 @@ def test_c_abi_exports_every_declared_symbol():
     for nm in ('getlst_', 'putlst_', 'getall_', 'putall_', 'sumsym2_', 'inipck_', 'updmoi_', 'getrec_', 'putrec_',
                'genint_', 'dre3en_', 'e4s_', 'newt2_', 'cmpeng_', 'dodiis0_', 'drmove_'):
         assert hasattr(L, nm), nm
+
+
+# ---------------------------------------------------------------------------
+# Lambda / response density oracle (oracle/lambda_.py)
+
+def _converged(o, v, seed, nonhf=False):
+    from aces2 import synth
+    eps, B = synth.draw(o, v, seed, sigma=synth.scaled_sigma(o, v))
+    eri = synth.full_eri(B)
+    f = synth.fock(eps)
+    if nonhf:
+        rng = np.random.default_rng(seed + 100)
+        P = rng.normal(0, 0.01, f.shape)
+        f = f + P + P.T
+    g = rccsd.blocks(eri, o)
+    h = vcc_loop.run(f, g, o, maxcyc=200, conv=12, order=6)
+    assert h['converged']
+    return f, eri, g, h
+
+
+def test_lambda_torch_restatement_matches_numpy():
+    import torch
+    from oracle import lambda_
+    f, eri = _ham(3, 5, 21)
+    g = rccsd.blocks(eri, 3)
+    t1, t2 = _rand_closed_t(3, 5, 22)
+    G = {k: torch.tensor(x) for k, x in g.items()}
+    i1, i2 = lambda_.increments_t(torch.tensor(t1), torch.tensor(t2), torch.tensor(f), G, 3)
+    r1, r2 = rccsd.increments(t1, t2, f, g, 3)
+    np.testing.assert_allclose(i1.numpy(), r1, atol=1e-14)
+    np.testing.assert_allclose(i2.numpy(), r2, atol=1e-14)
+    e = lambda_.energy_t(torch.tensor(t1), torch.tensor(t2), torch.tensor(f), G, 3)
+    assert abs(float(e) - rccsd.energy(t1, t2, f, g, 3)) < 1e-14
+
+
+def test_spatial_lambda_step_equals_spin_orbital_step():
+    """The reference's Jacobi step on its spin-orbital lists equals l <- l + (dL/dT)/D.
+
+    dL/dt(Ij,Ab) of the spin-orbital Lagrangian (AB amplitudes varied alone, by
+    complex step on the brute-force-pinned spin-orbital residual) equals
+    (2 + P(ij)) r2 / 3 of the spatial Lagrangian, and dL/dt(I,A) = r1 / 2.
+    """
+    from oracle import lambda_
+    o, v = 2, 3
+    f, eri = _ham(o, v, 3, nonhf=True)
+    F, G, no = spinorb.spin_orbital_hamiltonian(f, eri, o)
+    F, G = F.astype(complex), G.astype(complex)
+    t1, t2 = _rand_closed_t(o, v, 4)
+    L1, x = _rand_closed_t(o, v, 5)
+    T1, T2 = spinorb.closed_to_so_amplitudes(t1, t2)
+    LL1, LL2 = spinorb.closed_to_so_amplitudes(L1, x)
+
+    def lag(T1c, T2c):
+        om1, om2 = spinorb.omega(T1c, T2c, F, G, no)
+        Fov, Goovv = F[:no, no:], G[:no, :no, no:, no:]
+        E = (np.sum(Fov * T1c) + 0.25 * np.einsum('ijab,ijab->', Goovv, T2c)
+             + 0.5 * np.einsum('ijab,ia,jb->', Goovv, T1c, T1c))
+        return E + np.sum(LL1 * om1) + 0.25 * np.sum(LL2 * om2)
+
+    h = 1e-20
+    RAB = np.zeros((o, o, v, v))
+    for i in range(o):
+        for j in range(o):
+            for a in range(v):
+                for b in range(v):
+                    P = np.zeros(T2.shape, complex)
+                    P[i, o + j, a, v + b] += 1
+                    P[o + j, i, v + b, a] += 1
+                    P[i, o + j, v + b, a] -= 1
+                    P[o + j, i, a, v + b] -= 1
+                    RAB[i, j, a, b] = np.imag(lag(T1.astype(complex), T2 + 1j * h * P)) / h
+    R1 = np.zeros((o, v))
+    for i in range(o):
+        for a in range(v):
+            P = np.zeros(T1.shape, complex)
+            P[i, a] = 1
+            R1[i, a] = np.imag(lag(T1 + 1j * h * P, T2.astype(complex))) / h
+    l1, l2 = lambda_.from_aces(L1, x)
+    r1, r2 = lambda_.lagrangian_grads(t1, t2, l1, l2, f, rccsd.blocks(eri, o), o)[:2]
+    r2 = 0.5 * (r2 + r2.transpose(1, 0, 3, 2))
+    np.testing.assert_allclose(R1, r1 / 2, atol=1e-13)
+    np.testing.assert_allclose(RAB, (2 * r2 + r2.transpose(1, 0, 2, 3)) / 3, atol=1e-13)
+    # the two normalisations are inverse maps
+    b1, b2 = lambda_.to_aces(*lambda_.from_aces(L1, x))
+    np.testing.assert_allclose(b1, L1, atol=1e-15)
+    np.testing.assert_allclose(b2, x, atol=1e-15)
+
+
+@pytest.mark.parametrize('nonhf', [False, True])
+def test_lambda_density_matches_central_fd(nonhf):
+    """dE/dmu from (T, Lambda) vs central differences of the converged energy
+    (h = 1e-4) for F + mu P and for a rank-1 integral perturbation."""
+    from aces2 import synth
+    from oracle import lambda_
+    o, v = 3, 9
+    f, eri, g, h = _converged(o, v, 5, nonhf)
+    l1, l2, res = lambda_.solve_lambda(h['t1'], h['t2'], f, g, o)
+    assert res < 1e-10
+    D = lambda_.response_density(h['t1'], h['t2'], l1, l2, f, g, o)
+    rng = np.random.default_rng(1)
+    P = rng.normal(0, 0.01, f.shape)
+    P = P + P.T
+    n = o + v
+    x = rng.normal(0, 0.1, (n, n))
+    x = x + x.T
+    dg = np.einsum('pq,rs->pqrs', x, x)
+
+    def ecc(fm, em):
+        return vcc_loop.run(fm, rccsd.blocks(em, o), o, maxcyc=200, conv=12, order=6)['final_energy']
+
+    hh = 1e-4
+    fd = (ecc(f + hh * P, eri) - ecc(f - hh * P, eri)) / (2 * hh)
+    assert abs(np.sum(D * P) - fd) < 1e-8
+    gd = rccsd.blocks(dg, o)
+    z = np.zeros_like(f)
+    i1, i2 = rccsd.increments(h['t1'], h['t2'], z, gd, o)
+    an = rccsd.energy(h['t1'], h['t2'], z, gd, o) + np.sum(l1 * i1) + np.sum(l2 * i2)
+    fd2 = (ecc(f, eri + hh * dg) - ecc(f, eri - hh * dg)) / (2 * hh)
+    assert abs(an - fd2) < 1e-8
+    # the trace of the correlation density vanishes (E_corr is invariant to f -> f + c I)
+    assert abs(np.trace(D)) < 1e-12
+
+
+def test_lambda_reference_iteration_reaches_the_linear_solution():
+    from oracle import lambda_
+    o, v = 3, 9
+    f, eri, g, h = _converged(o, v, 7)
+    l1, l2, _ = lambda_.solve_lambda(h['t1'], h['t2'], f, g, o)
+    lh = lambda_.run_reference(f, g, o, h['t1'], h['t2'], maxcyc=100, conv=11, order=6)
+    assert lh['converged']
+    np.testing.assert_allclose(lh['l1'], l1, atol=1e-9)
+    np.testing.assert_allclose(lh['l2'], l2, atol=1e-9)
