@@ -89,11 +89,16 @@ class CCSD:
         _lib.call('aces_ccsd_set_profiling', self.h, 1 if on else 0)
 
     def profile(self, reset=True):
-        out = (ctypes.c_double * 9)()
+        out = (ctypes.c_double * 15)()
         _lib.call('aces_ccsd_profile', self.h, out, 1 if reset else 0)
-        names = ('ladder', 'wbuild', 'ring')
+        names = ('ladder', 'wbuild', 'ring', 'other', 'adjoint')
         return {nm: {'ms': out[3 * c], 'flops': out[3 * c + 1], 'launches': int(out[3 * c + 2])}
                 for c, nm in enumerate(names)}
+
+    def set_fock(self, fock):
+        torch = self.torch
+        f = torch.as_tensor(fock, device='cuda', dtype=torch.float64).contiguous()
+        _lib.call('aces_ccsd_set_fock', self.h, f.data_ptr())
 
     def set_diis_order(self, order):
         _lib.call('aces_ccsd_set_diis_order', self.h, order)
@@ -114,6 +119,8 @@ class CCSD:
         self.reset_diis()
         if start == 'mp1':
             self.start_mp1()
+        elif start == 'current':
+            pass
         elif start == 'zero':
             self.set_amplitudes(torch.zeros(self.o, self.v), torch.zeros(self.o, self.o, self.v, self.v))
         t1, t2 = self.get_amplitudes()

@@ -87,31 +87,38 @@ def blocks_numpy(eps, B, o, perturb=None):
     }, eri
 
 
-def blocks_device(eps, B, o, perturb=None, device='cuda'):
+def blocks_device(eps, B, o, perturb=None, device='cuda', weights=None, fock_matrix=None):
     """Integral blocks and list 233 built on the GPU from the factors B (any size).
 
-    (pq|rs) = sum_P B[P,p,q] B[P,r,s] evaluated with fp64 matrix products; the
-    vvvv list is assembled one row index a at a time so that only the A<=B
-    compressed list (v^3 (v+1)/2 doubles) is ever resident.
+    (pq|rs) = sum_P w_P B[P,p,q] B[P,r,s] (w = 1 unless ``weights`` is given)
+    evaluated with fp64 matrix products; the vvvv list is assembled one row
+    index a at a time so that only the A<=B compressed list (v^3 (v+1)/2
+    doubles) is ever resident.  ``fock_matrix`` replaces diag(eps) + perturb.
     """
     import torch
     naux, n, _ = B.shape
     v = n - o
     Bd = torch.as_tensor(B, device=device, dtype=torch.float64)
+    if weights is None:
+        Wd = Bd
+    else:
+        Wd = Bd * torch.as_tensor(np.asarray(weights, dtype=np.float64), device=device)[:, None, None]
     Boo = Bd[:, :o, :o].reshape(naux, o * o)
     Bov = Bd[:, :o, o:].reshape(naux, o * v)
     Bvv = Bd[:, o:, o:].reshape(naux, v * v)
+    Woo = Wd[:, :o, :o].reshape(naux, o * o)
+    Wov = Wd[:, :o, o:].reshape(naux, o * v)
     out = {}
-    f = fock(eps, perturb)
+    f = fock(eps, perturb) if fock_matrix is None else fock_matrix
     out['fock'] = torch.as_tensor(f, device=device, dtype=torch.float64).contiguous()
-    out['ovov'] = (Bov.T @ Bov).reshape(o, v, o, v).contiguous()
-    out['oovv'] = (Boo.T @ Bvv).reshape(o, o, v, v).contiguous()
-    out['ovoo'] = (Bov.T @ Boo).reshape(o, v, o, o).contiguous()
-    out['oooo'] = (Boo.T @ Boo).reshape(o, o, o, o).contiguous()
-    out['ovvv'] = (Bov.T @ Bvv).reshape(o, v, v, v).contiguous()
+    out['ovov'] = (Wov.T @ Bov).reshape(o, v, o, v).contiguous()
+    out['oovv'] = (Woo.T @ Bvv).reshape(o, o, v, v).contiguous()
+    out['ovoo'] = (Wov.T @ Boo).reshape(o, v, o, o).contiguous()
+    out['oooo'] = (Woo.T @ Boo).reshape(o, o, o, o).contiguous()
+    out['ovvv'] = (Wov.T @ Bvv).reshape(o, v, v, v).contiguous()
     nvp = v * (v + 1) // 2
     L = torch.empty((v, v, nvp), device=device, dtype=torch.float64)   # [d][c][pair]
-    B3 = Bd[:, o:, o:]                                                    # [P][a][c]
+    B3 = Wd[:, o:, o:]                                                    # [P][a][c]
     for a in range(v):
         # T[c][b][d] = (ac|bd) for b >= a
         Ta = (B3[:, a, :].T @ Bvv).reshape(v, v, v)                      # [c][b][d]
@@ -120,5 +127,5 @@ def blocks_device(eps, B, o, perturb=None, device='cuda'):
         L[:, :, cols] = Ta[:, a:, :].permute(2, 0, 1)                     # [d][c][b]
         del Ta
     out['list233'] = L
-    del Bd, Boo, Bov, Bvv, B3
+    del Bd, Wd, Boo, Bov, Bvv, Woo, Wov, B3
     return out

@@ -107,6 +107,11 @@ int aces_ccsd_profile(void* h, double* out, int reset) {
   return ACES_OK;
 }
 
+int aces_ccsd_set_fock(void* h, const double* fock) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->set_fock(fock);
+}
+
 int aces_ccsd_adjoint(void* h, const double* l1, const double* l2, double* r1, double* r2, double* dens) {
   REQUIRE_HANDLE(h);
   return eng(h)->adjoint(l1, l2, r1, r2, dens);

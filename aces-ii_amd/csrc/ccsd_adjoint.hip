@@ -90,6 +90,7 @@ int CcsdEngine::compute_adjoint(const double* lam, bool density) {
   double* gt2 = gT_ + ov;
   const double* fov = fock_ + o;
 
+  prof_cat_ = 4;
   // seeds: energy (2 f_ia t_ia + L_ijab tau_ijab) and -D*T
   ACES_CHECK(adjoint_seed(o, v, eo_, ev_, fov, n, l1, l2, gt1, gt2, s));
   ACES_HIP(hipMemcpyAsync(gtau_, Lijab_, sizeof(double) * ovvo, hipMemcpyDeviceToDevice, s));

@@ -52,8 +52,11 @@ class CcsdEngine {
   int reset_diis();
   int set_diis_order(int order);
   int energy_of(const double* t1, const double* t2, double& e);
+  // Replace the MO Fock matrix (n x n, device) keeping the integrals.
+  int set_fock(const double* fock);
   // Per-launch timing of the MFMA contractions (hipEvents on the engine's stream).
-  // out[3*c + 0..2] = {ms, algorithmic flops, launches} for c = 0 ladder, 1 W build, 2 ring.
+  // out[3*c + 0..2] = {ms, algorithmic flops, launches} for c = 0 ladder, 1 W build,
+  // 2 ring, 3 the other fp64 contractions of the increments, 4 the reverse pass.
   void set_profiling(bool on) { profile_ = on; }
   void profile(double* out, int reset);
   long device_bytes() const { return bytes_; }
@@ -90,7 +93,7 @@ class CcsdEngine {
   int gemm_lp(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda, const double* B64,
               int bslot, long ldb, double* C, long ldc, double alpha, double beta);
   int timed_gemm(const GemmDesc& d, GemmPrec prec, double flops);
-  int gemm64(GemmDesc d);  // fp64 GEMM with the engine's split-K workspace
+  int gemm64(GemmDesc d);  // fp64 GEMM with the engine's split-K workspace (timed as prof_cat_)
   int gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda, const double* B64,
                    int bslot, long ldb, double* C, long ldc, double alpha, double beta);
   struct DiisState {
@@ -110,7 +113,8 @@ class CcsdEngine {
   hipStream_t s_;
   long bytes_ = 0;
   bool profile_ = false;
-  double prof_[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  static constexpr int kProfCats = 5;
+  double prof_[3 * kProfCats] = {};
   int prof_cat_ = 0;
   std::vector<void*> allocs_;
   bool ready_ = false;

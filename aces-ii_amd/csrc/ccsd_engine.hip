@@ -324,13 +324,23 @@ int CcsdEngine::lowp_static(GemmPrec prec) {
 int CcsdEngine::gemm64(GemmDesc d) {
   d.ws = ws_;
   d.ws_elems = ws_elems_;
-  return gemm(d, GP_F64, s_);
+  return timed_gemm(d, GP_F64, 2.0 * d.M * (double)d.N * d.K * d.batch);
+}
+
+int CcsdEngine::set_fock(const double* fock) {
+  ACES_REQUIRE(ready_, "ccsd: not set up");
+  ACES_HIP(hipMemcpyAsync(fock_, fock, sizeof(double) * n * n, hipMemcpyDeviceToDevice, s_));
+  ACES_CHECK(P4(1, 1, 1, o, eo_).src(fock_, 1, 0, 0, 0, (long)n + 1).run(s_));
+  ACES_CHECK(P4(1, 1, 1, v, ev_).src(fock_ + (long)o * n + o, 1, 0, 0, 0, (long)n + 1).run(s_));
+  ACES_HIP(hipStreamSynchronize(s_));
+  fwd_valid_ = false;
+  return ACES_OK;
 }
 
 void CcsdEngine::profile(double* out, int reset) {
-  for (int i = 0; i < 9; ++i) out[i] = prof_[i];
+  for (int i = 0; i < 3 * kProfCats; ++i) out[i] = prof_[i];
   if (reset)
-    for (int i = 0; i < 9; ++i) prof_[i] = 0.0;
+    for (int i = 0; i < 3 * kProfCats; ++i) prof_[i] = 0.0;
 }
 
 int CcsdEngine::gemm_lp(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda,
@@ -418,6 +428,7 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(P4(o, v, o, v, Xt_).src(X_, 1, ov2, ov, v, 1).prod(t1, 0, 1, v, 0, t1, v, 0, 0, 1, 2.0).run(s));
 
   // T1 parts of the W(mbej) intermediates
+  prof_cat_ = 3;
   ACES_CHECK(gemm64(gd((int)v2o, o, v, ovvv_, v, 1, t1, v, 1, P1_, o, 1.0, 0.0)));        // [(k,c,a)][i]
   ACES_CHECK(gemm64(gd((int)v2o, o, v, ovvv_kacd_, v, 1, t1, v, 1, P2_, o, 1.0, 0.0)));   // [(k,a,c)][i]
   ACES_CHECK(gemm64(gd((int)o2v, v, o, M5c_, o, 1, t1, v, 0, Q12_, v, 1.0, 0.0)));        // [(k,c,i)][a]
@@ -456,6 +467,7 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   else
     ACES_HIP(hipMemsetAsync(Zm_, 0, sizeof(double), s));
   ACES_CHECK(unpack_ladder(o, v, Zp_, nop_, Zm_, std::max<long>(nom_, 1), inc2_, Yb_, s));
+  prof_cat_ = 3;
   // L1[a][(b,i,j)] = sum_k t_ka Yb[(b,i,j)][k] ; inc2 -= L1[a,b,i,j] + L1[b,a,j,i]
   ACES_CHECK(gemm64(gd(v, (int)(v * oo), o, t1, v, 0, Yb_, o, 1, L1_, v * oo, 1.0, 0.0)));
   ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(L1_, -1, o, 1, vo2, oo).src(L1_, -1, 1, o, oo, vo2).run(s));

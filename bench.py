@@ -13,7 +13,15 @@ study runs both forms from the MP1 start: fp64 to |RESMITER| < 1e-9, mixed to
 1e-6 followed by fp64 polish to 1e-9, and reports |dE|, the fp64 seconds per
 cycle and the speedup.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+With ``--workload lambda`` the step is one cycle of the Lambda equations (the
+xlambda loop body: the engine's reverse pass, Jacobi step, Lambda DIIS) at the
+converged fp64 amplitudes of o=32, v=256; the same run checks dE/dmu from the
+response density and from the Lambda-weighted increments against central
+finite differences (h = 1e-4) for a Fock perturbation F + mu P and a rank-1
+integral perturbation (pq|rs) + mu x_pq x_rs, at fp64 amplitudes and at the
+amplitudes of the mixed form after its fp64 polish.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ccsd|lambda]
 """
 import argparse
 import json
@@ -83,8 +91,9 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--o', type=int, default=48)
-    ap.add_argument('--v', type=int, default=320)
+    ap.add_argument('--workload', default='ccsd', choices=['ccsd', 'lambda'])
+    ap.add_argument('--o', type=int, default=None)
+    ap.add_argument('--v', type=int, default=None)
     ap.add_argument('--seed', type=int, default=20260)
     ap.add_argument('--prec', default='bf16x3', choices=['bf16x3', 'fp32', 'fp64'])
     ap.add_argument('--no-convergence', action='store_true', help='skip the convergence study')
@@ -97,12 +106,15 @@ def main():
               'MI355X); --gpus %d is not supported' % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == 'lambda':
+        return bench_lambda(args)
     import numpy as np
     import torch
     from aces2 import synth
     from aces2.ccsd import CCSD
 
-    o, v = args.o, args.v
+    o = args.o or 48
+    v = args.v or 320
     log = lambda *a: print('[bench]', *a, file=sys.stderr, flush=True)
     t_setup = time.perf_counter()
     eps, B = synth.draw(o, v, args.seed, sigma=synth.scaled_sigma(o, v))
@@ -136,8 +148,9 @@ def main():
     nprof = 2
     for icycle in range(args.warmup + args.steps + 1, args.warmup + args.steps + 1 + nprof):
         cc.cycle(icycle, args.prec, 1e-30)
-    prof = cc.profile(reset=True)
+    prof_all = cc.profile(reset=True)
     cc.set_profiling(False)
+    prof = {k: prof_all[k] for k in ('ladder', 'wbuild', 'ring')}
     ms = sum(p['ms'] for p in prof.values())
     fl = sum(p['flops'] for p in prof.values())
     nl = sum(p['launches'] for p in prof.values())
@@ -151,7 +164,10 @@ def main():
                 'avg_launch_ms': ms / nl, 'launches_per_cycle': nl / nprof,
                 'gemm_share_of_cycle': (ms / nprof) / (s_per_cycle * 1e3),
                 'per_class': {k: {'TFLOP/s': execf * p['flops'] / max(p['ms'], 1e-9) / 1e9, 'ms': p['ms'] / nprof}
-                              for k, p in prof.items()}}
+                              for k, p in prof.items()},
+                'fp64_other': {'TFLOP/s': prof_all['other']['flops'] / max(prof_all['other']['ms'], 1e-9) / 1e9,
+                               'ms': prof_all['other']['ms'] / nprof,
+                               'launches': prof_all['other']['launches'] / nprof}}
     log('roofline', json.dumps(roofline))
 
     result = {
@@ -221,6 +237,222 @@ def main():
     if not args.no_cpu:
         log('cpu baseline ...')
         result['cpu_baseline'] = cpu_baseline(o, v, args.seed)
+    else:
+        result['cpu_baseline'] = None
+    print(json.dumps(result), flush=True)
+
+
+def cpu_baseline_lambda(o, v, seed, budget_s=20.0):
+    """Oracle reverse pass (torch autograd of the restated increments, CPU) on a
+    bounded sample, scaled to (o, v) by reference-equivalent flops."""
+    import numpy as np
+    import torch
+    from oracle import lambda_, rccsd
+    from aces2 import synth
+    threads = torch.get_num_threads()
+    best = None
+    for (os_, vs_) in [(8, 48), (12, 72), (16, 96)]:
+        eps, B = synth.draw(os_, vs_, seed, sigma=synth.scaled_sigma(os_, vs_))
+        blk, eri = synth.blocks_numpy(eps, B, os_)
+        g = rccsd.blocks(eri, os_)
+        f = blk['fock']
+        rng = np.random.default_rng(0)
+        t1 = rng.normal(0, 0.01, (os_, vs_))
+        t2 = rng.normal(0, 0.01, (os_, os_, vs_, vs_))
+        lambda_.lagrangian_grads(t1, t2, t1, t2, f, g, os_)
+        t0 = time.perf_counter()
+        lambda_.lagrangian_grads(t1, t2, t1, t2, f, g, os_)
+        dt = time.perf_counter() - t0
+        best = (os_, vs_, dt)
+        if dt * (f_ref(os_ + 4, vs_ + 24) / f_ref(os_, vs_)) > budget_s:
+            break
+    os_, vs_, dt = best
+    scale = f_ref(o, v) / f_ref(os_, vs_)
+    return {'value': dt * scale, 'unit': 's/iter', 'cores': int(threads), 'kind': 'port',
+            'sample': 'oracle.lambda_.lagrangian_grads (torch CPU autograd, %d threads; forward + reverse of the '
+                      'increments) at o=%d, v=%d: %.3f s, scaled x%.1f by reference-equivalent flops to o=%d, v=%d'
+                      % (threads, os_, vs_, dt, scale, o, v)}
+
+
+def bench_lambda(args):
+    """Lambda equations and response density at o=32, v=256 (differentiation form)."""
+    import numpy as np
+    import torch
+    from aces2 import synth
+    from aces2.ccsd import CCSD
+
+    o = args.o or 32
+    v = args.v or 256
+    n = o + v
+    log = lambda *a: print('[bench]', *a, file=sys.stderr, flush=True)
+    conv = 10
+    t_setup = time.perf_counter()
+    sigma = synth.scaled_sigma(o, v)
+    eps, B = synth.draw(o, v, args.seed, sigma=sigma)
+    blocks = synth.blocks_device(eps, B, o)
+    F0 = blocks['fock'].clone()
+    cc = CCSD(o, v, blocks)
+    del blocks
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    log('engine set up: %.1f GB device, %.1f s' % (cc.device_bytes / 1e9, time.perf_counter() - t_setup))
+
+    def converge(c, prec='fp64', start='mp1', polish=False):
+        c.set_diis_order(6)
+        c.reset_diis()
+        if start == 'mp1':
+            c.start_mp1()
+        tol = 10.0 ** (-(6 if prec != 'fp64' else conv))
+        phase, icycle, ncyc = prec, 0, {'lowp': 0, 'fp64': 0}
+        while icycle < args.maxcyc:
+            icycle += 1
+            st = c.cycle(icycle, phase, tol)
+            ncyc['fp64' if phase == 'fp64' else 'lowp'] += 1
+            if st['converged']:
+                if phase != 'fp64':
+                    phase, tol = 'fp64', 10.0 ** (-conv)
+                    continue
+                return st['final_energy'], ncyc
+        raise RuntimeError('CCSD did not converge in %d cycles' % args.maxcyc)
+
+    ta = time.perf_counter()
+    E0, ncyc0 = converge(cc)
+    t_ccsd = time.perf_counter() - ta
+    T0 = cc.get_amplitudes()
+    log('fp64 CCSD: E=%r, %d cycles, %.1f s' % (E0, ncyc0['fp64'], t_ccsd))
+
+    # ---- timed region: K Lambda cycles after W warmup cycles ----
+    tol = 10.0 ** (-conv)
+    cc.set_lambda_diis_order(6)
+    cc.lambda_start()
+    icycle = 0
+    for _ in range(args.warmup):
+        icycle += 1
+        cc.lambda_cycle(icycle, tol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        icycle += 1
+        cc.lambda_cycle(icycle, tol)
+    torch.cuda.synchronize()
+    s_per_cycle = (time.perf_counter() - t0) / args.steps
+    log('lambda: %.4f s/cycle' % s_per_cycle)
+    cc.set_profiling(True)
+    cc.profile(reset=True)
+    nprof = 2
+    converged = False
+    for _ in range(nprof):
+        icycle += 1
+        converged = cc.lambda_cycle(icycle, tol)['converged'] or converged
+    prof = cc.profile(reset=True)
+    cc.set_profiling(False)
+    while not converged and icycle < args.maxcyc:
+        icycle += 1
+        converged = cc.lambda_cycle(icycle, tol)['converged']
+    if not converged:
+        raise RuntimeError('Lambda equations did not converge')
+    lam_cycles = icycle
+    adj = prof['adjoint']
+    achieved = adj['flops'] / (adj['ms'] * 1e-3)
+    roofline = {'bound': 'mfma', 'kernel': 'gemm16_kernel<double> (reverse-pass contractions)',
+                'achieved': achieved / 1e12, 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
+                'frac': achieved / FP64_PEAK, 'traffic': None, 'avg_launch_ms': adj['ms'] / adj['launches'],
+                'launches_per_cycle': adj['launches'] / nprof,
+                'gemm_share_of_cycle': (adj['ms'] / nprof) / (s_per_cycle * 1e3)}
+    log('roofline', json.dumps(roofline))
+
+    # ---- dE/dmu checks ----
+    rng = np.random.default_rng(args.seed + 1)
+    P = rng.normal(0.0, 0.01, (n, n))
+    P = torch.tensor(P + P.T, dtype=torch.float64, device='cuda')
+    x = rng.normal(0.0, sigma, (n, n))
+    x = np.triu(x) + np.triu(x, 1).T
+    hh = 1e-4
+
+    def analytic(c, t1, t2):
+        D = c.density()
+        l1, l2 = c.get_lambda()
+        an_f = float(torch.sum(D * P))
+        bd = synth.blocks_device(np.zeros(n), x[None], o, fock_matrix=np.zeros((n, n)))
+        cd = CCSD(o, v, bd)
+        del bd
+        cd.set_amplitudes(t1, t2)
+        i1, i2 = cd.increments('fp64')
+        an_g = cd.energy(t1, t2) + float(torch.sum(l1 * i1) + torch.sum(l2 * i2))
+        cd.close()
+        del i1, i2, l1, l2
+        torch.cuda.empty_cache()
+        return an_f, an_g
+
+    an_f, an_g = analytic(cc, *T0)
+    log('analytic fp64: fock %.12e integral %.12e' % (an_f, an_g))
+    e_f = []
+    for mu in (hh, -hh, 2 * hh, -2 * hh):
+        cc.set_fock(F0 + mu * P)
+        cc.set_amplitudes(*T0)
+        e_f.append(converge(cc, start='current')[0])
+    cc.set_fock(F0)
+    fd_f = (e_f[0] - e_f[1]) / (2 * hh)
+    fd_f2 = (e_f[2] - e_f[3]) / (4 * hh)
+    # mixed form: bf16x3 to 1e-6, fp64 polish to 1e-10, Lambda in fp64 at those amplitudes
+    Em, ncm = converge(cc, prec=args.prec if args.prec != 'fp64' else 'bf16x3')
+    Tm = cc.get_amplitudes()
+    cc.set_lambda_diis_order(6)
+    lm = cc.solve_lambda(maxcyc=args.maxcyc, conv=conv, order=6)
+    an_fm, an_gm = analytic(cc, *Tm)
+    cc.close()
+    del cc
+    torch.cuda.empty_cache()
+    e_g = []
+    Baug = np.concatenate([B, x[None]], axis=0)
+    for mu in (hh, -hh, 2 * hh, -2 * hh):
+        w = np.ones(Baug.shape[0])
+        w[-1] = mu
+        bp = synth.blocks_device(eps, Baug, o, weights=w)
+        c2 = CCSD(o, v, bp)
+        del bp
+        c2.set_amplitudes(*T0)
+        e_g.append(converge(c2, start='current')[0])
+        c2.close()
+        torch.cuda.empty_cache()
+    fd_g = (e_g[0] - e_g[1]) / (2 * hh)
+    fd_g2 = (e_g[2] - e_g[3]) / (4 * hh)
+    # Richardson: the O(h^2) truncation of the central difference cancels in (4 FD(h) - FD(2h)) / 3
+    rf, rg = (4 * fd_f - fd_f2) / 3, (4 * fd_g - fd_g2) / 3
+    fd = {'h': hh, 'conv': conv,
+          'fock': {'analytic_fp64': an_f, 'analytic_mixed_polished': an_fm, 'central_fd': fd_f,
+                   'central_fd_2h': fd_f2, 'richardson': rf,
+                   'abs_err_fp64': abs(an_f - fd_f), 'abs_err_mixed': abs(an_fm - fd_f),
+                   'abs_err_fp64_richardson': abs(an_f - rf), 'abs_err_mixed_richardson': abs(an_fm - rf)},
+          'integral_rank1': {'analytic_fp64': an_g, 'analytic_mixed_polished': an_gm, 'central_fd': fd_g,
+                             'central_fd_2h': fd_g2, 'richardson': rg,
+                             'abs_err_fp64': abs(an_g - fd_g), 'abs_err_mixed': abs(an_gm - fd_g),
+                             'abs_err_fp64_richardson': abs(an_g - rg), 'abs_err_mixed_richardson': abs(an_gm - rg)},
+          'ccsd_energy_fp64': E0, 'ccsd_energy_mixed': Em, 'abs_dE_mixed': abs(Em - E0),
+          'ccsd_cycles_fp64': ncyc0['fp64'], 'ccsd_cycles_mixed': ncm,
+          'lambda_cycles': lam_cycles, 'lambda_cycles_mixed': lm.get('cycles')}
+    log('fd', json.dumps(fd))
+    result = {
+        'metric': 'Lambda (xlambda) s/iter at o=%d,v=%d on GPU; dE/dmu vs central FD' % (o, v),
+        'value': s_per_cycle,
+        'unit': 's/iter',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': s_per_cycle * 1e3,
+        'higher_is_better': False,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic (DF integrals, seed %d, sigma=0.05*sqrt(95/(o*v)))' % args.seed,
+        'config': {'workload': 'closed-shell CCSD Lambda cycle (reverse pass + Jacobi + DIIS), RHF C1, o=%d v=%d '
+                               'n_aux=%d' % (o, v, 4 * n), 'o': o, 'v': v, 'diis': 6},
+        'roofline': roofline,
+        'fd_checks': fd,
+    }
+    if not args.no_cpu:
+        log('cpu baseline ...')
+        result['cpu_baseline'] = cpu_baseline_lambda(o, v, args.seed)
     else:
         result['cpu_baseline'] = None
     print(json.dumps(result), flush=True)

@@ -77,9 +77,14 @@ int aces_ccsd_set_diis_order(void* handle, int order);
 int aces_ccsd_reset_diis(void* handle);
 int aces_ccsd_energy(void* handle, const double* d_t1, const double* d_t2, double* e_out);
 long aces_ccsd_device_bytes(void* handle);
+/* Replace the MO Fock matrix (device n x n) of a set-up engine, keeping the
+ * integrals (a one-electron perturbation of the same system). */
+int aces_ccsd_set_fock(void* handle, const double* d_fock);
 /* Per-launch hipEvent timing of the MFMA contractions (on the engine's stream):
  * out[3c+0] ms, out[3c+1] algorithmic flops (2mnk), out[3c+2] launches, for
- * c = 0 particle ladder, 1 W(mbej) build, 2 ring.  No reference counterpart
+ * c = 0 particle ladder, 1 W(mbej) build, 2 ring, 3 the other fp64
+ * contractions of the increments, 4 the reverse pass (out holds 15 values).
+ * No reference counterpart
  * (the reference has only whole-program timers, acescore/aces_fin.F:29-55). */
 int aces_ccsd_set_profiling(void* handle, int on);
 int aces_ccsd_profile(void* handle, double* out, int reset);
