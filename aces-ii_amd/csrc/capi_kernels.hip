@@ -21,6 +21,25 @@ int aces_gemm(int prec, int M, int N, int K, const void* A, const void* A_lo, lo
   return gemm(d, (GemmPrec)prec, (hipStream_t)stream);
 }
 
+int aces_gemm_grouped(int prec, int ngroups, const int* M, const int* N, const int* K, const void* const* A,
+                      const void* const* A_lo, const long* lda, int a_kc, const void* const* B,
+                      const void* const* B_lo, const long* ldb, int b_kc, double* const* C, const long* ldc,
+                      double alpha, double beta, double* ws, long ws_elems, void* stream) {
+  if (ngroups < 1 || ngroups > 8) {
+    set_error("aces_gemm_grouped: 1..8 groups");
+    return ACES_ERR_ARG;
+  }
+  GemmDesc d[8];
+  for (int g = 0; g < ngroups; ++g) {
+    d[g].M = M[g]; d[g].N = N[g]; d[g].K = K[g];
+    d[g].A = A[g]; d[g].A_lo = A_lo ? A_lo[g] : nullptr; d[g].lda = lda[g]; d[g].a_kc = a_kc;
+    d[g].B = B[g]; d[g].B_lo = B_lo ? B_lo[g] : nullptr; d[g].ldb = ldb[g]; d[g].b_kc = b_kc;
+    d[g].C = C[g]; d[g].ldc = ldc[g]; d[g].alpha = alpha; d[g].beta = beta;
+    d[g].ws = ws; d[g].ws_elems = ws_elems;
+  }
+  return gemm_grouped(d, ngroups, (GemmPrec)prec, (hipStream_t)stream);
+}
+
 int aces_convert_f32(int rows, int cols, const double* x, long ldx, float* y, long ldy, void* stream) {
   return convert_f32(rows, cols, x, ldx, y, ldy, (hipStream_t)stream);
 }

@@ -1,0 +1,343 @@
+// Irrep-blocked tensor layouts, box plans and the HBM-bound kernels that walk them.
+#include <algorithm>
+#include <cstring>
+#include "common.h"
+#include "dpd.h"
+
+namespace aces {
+namespace dpd {
+
+Space Space::make(int nirr, const int* counts) {
+  Space s;
+  s.nirr = nirr;
+  int o = 0;
+  for (int g = 0; g < nirr; ++g) {
+    s.cnt[g] = counts[g];
+    s.off[g] = o;
+    o += counts[g];
+  }
+  s.n = o;
+  return s;
+}
+
+static long compound_count(const Space* sp, int k, int G, int nirr, long* sub) {
+  // compound of k indices (spaces sp[0..k-1]) with total irrep G; sub[t] for
+  // the tuple t of the leading k-1 irreps (lexicographic).
+  long tot = 0;
+  if (k == 1) {
+    sub[0] = 0;
+    return sp[0].cnt[G];
+  }
+  if (k == 2) {
+    for (int g0 = 0; g0 < nirr; ++g0) {
+      sub[g0] = tot;
+      tot += (long)sp[0].cnt[g0] * sp[1].cnt[G ^ g0];
+    }
+    return tot;
+  }
+  for (int g0 = 0; g0 < nirr; ++g0)
+    for (int g1 = 0; g1 < nirr; ++g1) {
+      sub[g0 * nirr + g1] = tot;
+      tot += (long)sp[0].cnt[g0] * sp[1].cnt[g1] * sp[2].cnt[G ^ g0 ^ g1];
+    }
+  return tot;
+}
+
+Layout make_layout(int rank, int split, const Space* s0, const Space* s1, const Space* s2, const Space* s3) {
+  Layout L;
+  L.rank = rank;
+  L.split = split;
+  const Space* ss[4] = {s0, s1, s2, s3};
+  for (int k = 0; k < rank; ++k) L.sp[k] = *ss[k];
+  L.nirr = s0->nirr;
+  long off = 0;
+  for (int G = 0; G < L.nirr; ++G) {
+    L.nrow[G] = compound_count(L.sp, split, G, L.nirr, L.subr[G]);
+    L.ncol[G] = compound_count(L.sp + split, rank - split, G, L.nirr, L.subc[G]);
+    L.blk_off[G] = off;
+    off += L.nrow[G] * L.ncol[G];
+  }
+  L.blk_off[L.nirr] = off;
+  return L;
+}
+
+bool Layout::same(const Layout& o) const {
+  if (rank != o.rank || split != o.split || nirr != o.nirr) return false;
+  for (int k = 0; k < rank; ++k)
+    for (int g = 0; g < nirr; ++g)
+      if (sp[k].cnt[g] != o.sp[k].cnt[g]) return false;
+  return true;
+}
+
+PairSpace make_pairs(const Space& s, bool strict) {
+  PairSpace P;
+  P.strict = strict;
+  P.nirr = s.nirr;
+  for (int G = 0; G < s.nirr; ++G) {
+    long tot = 0;
+    for (int gp = 0; gp < s.nirr; ++gp) {
+      const int gq = G ^ gp;
+      P.sub[G][gp] = tot;
+      if (G == 0) {
+        const long c = s.cnt[gp];
+        tot += strict ? c * (c - 1) / 2 : c * (c + 1) / 2;
+      } else if (gp < gq) {
+        tot += (long)s.cnt[gp] * s.cnt[gq];
+      }
+    }
+    P.n[G] = tot;
+  }
+  return P;
+}
+
+// ---------------------------------------------------------------------------
+static constexpr long kChunk = 4096;
+
+int build_plan(Plan& p, const Layout& out, const std::vector<Src>& srcs, const Src* X, const Src* Y) {
+  ACES_REQUIRE(out.rank == 2 || out.rank == 4, "dpd plan: output rank must be 2 or 4");
+  ACES_REQUIRE(srcs.size() <= 3, "dpd plan: at most three sources");
+  const int nirr = out.nirr;
+  p.host.clear();
+  p.nsrc = (int)srcs.size();
+  p.prod = (X != nullptr);
+  for (const Src& s : srcs) ACES_REQUIRE(s.L->rank == out.rank, "dpd plan: source rank differs from output");
+  long first = 0;
+  auto add_box = [&](const int* g) {
+    Box b;
+    std::memset(&b, 0, sizeof(b));
+    long elems = 1;
+    for (int k = 0; k < 4; ++k) {
+      b.d[k] = (k < out.rank) ? out.sp[k].cnt[g[k]] : 1;
+      b.g[k] = (k < out.rank) ? g[k] : 0;
+      elems *= b.d[k];
+    }
+    if (elems == 0) return;
+    long st[4] = {0, 0, 0, 0};
+    out.affine(g, b.ob, st);
+    for (int k = 0; k < 4; ++k) b.os[k] = st[k];
+    for (size_t s = 0; s < srcs.size(); ++s) {
+      const Src& sr = srcs[s];
+      int gs[4];
+      for (int m = 0; m < sr.L->rank; ++m) gs[m] = g[sr.map[m]];
+      long sst[4] = {0, 0, 0, 0};
+      sr.L->affine(gs, b.sb[s], sst);
+      for (int m = 0; m < sr.L->rank; ++m) b.ss[s][sr.map[m]] += sst[m];
+    }
+    if (X) {
+      const int gx0 = g[X->map[0]], gx1 = g[X->map[1]];
+      const int gy0 = g[Y->map[0]], gy1 = g[Y->map[1]];
+      if (gx0 != gx1 || gy0 != gy1) {
+        b.xb = -1;
+      } else {
+        int gx[2] = {gx0, gx1}, gy[2] = {gy0, gy1};
+        long xs[4] = {0, 0, 0, 0}, ys[4] = {0, 0, 0, 0};
+        X->L->affine(gx, b.xb, xs);
+        Y->L->affine(gy, b.yb, ys);
+        for (int m = 0; m < 2; ++m) {
+          b.xs[X->map[m]] += xs[m];
+          b.ys[Y->map[m]] += ys[m];
+        }
+      }
+    }
+    b.first = first;
+    first += (elems + kChunk - 1) / kChunk;
+    p.host.push_back(b);
+  };
+  if (out.rank == 4) {
+    for (int g0 = 0; g0 < nirr; ++g0)
+      for (int g1 = 0; g1 < nirr; ++g1)
+        for (int g2 = 0; g2 < nirr; ++g2) {
+          int g[4] = {g0, g1, g2, g0 ^ g1 ^ g2};
+          add_box(g);
+        }
+  } else {
+    for (int g0 = 0; g0 < nirr; ++g0) {
+      int g[4] = {g0, g0, 0, 0};
+      add_box(g);
+    }
+  }
+  p.nbox = (int)p.host.size();
+  p.total = first;  // chunks
+  free_plan(p);
+  if (p.nbox > 0) {
+    ACES_HIP(hipMalloc((void**)&p.dev, sizeof(Box) * p.nbox));
+    ACES_HIP(hipMemcpy(p.dev, p.host.data(), sizeof(Box) * p.nbox, hipMemcpyHostToDevice));
+  }
+  return ACES_OK;
+}
+
+void free_plan(Plan& p) {
+  if (p.dev) (void)hipFree(p.dev);
+  p.dev = nullptr;
+}
+
+struct ApplyArgs {
+  const Box* boxes;
+  int nbox;
+  double* out;
+  double beta;
+  const double* src[3];
+  double alpha[3];
+  int nsrc;
+  const double* X;
+  const double* Y;
+  double gamma;
+  Denom den;
+};
+
+__global__ __launch_bounds__(256) void apply_plan_kernel(ApplyArgs a) {
+  __shared__ int sb;
+  if (threadIdx.x == 0) {
+    int lo = 0, hi = a.nbox - 1;
+    const long c = blockIdx.x;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.boxes[mid].first <= c)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    sb = lo;
+  }
+  __syncthreads();
+  const Box& b = a.boxes[sb];
+  const long elems = (long)b.d[0] * b.d[1] * b.d[2] * b.d[3];
+  const long e0 = (blockIdx.x - b.first) * kChunk;
+  const long e1 = min(elems, e0 + kChunk);
+  const int d1 = b.d[1], d2 = b.d[2], d3 = b.d[3];
+  for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int l3 = (int)(e % d3);
+    const long r2 = e / d3;
+    const int l2 = (int)(r2 % d2);
+    const long r1 = r2 / d2;
+    const int l1 = (int)(r1 % d1);
+    const int l0 = (int)(r1 / d1);
+    double acc = 0.0;
+    for (int s = 0; s < a.nsrc; ++s)
+      acc += a.alpha[s] * a.src[s][b.sb[s] + l0 * b.ss[s][0] + l1 * b.ss[s][1] + l2 * b.ss[s][2] + l3 * b.ss[s][3]];
+    if (a.den.pow != 0) {
+      const int l[4] = {l0, l1, l2, l3};
+      double D = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (a.den.sign[k] != 0) D += a.den.sign[k] * a.den.e[k][a.den.off[k][b.g[k]] + l[k]];
+      acc = (a.den.pow > 0) ? acc * D : acc / D;
+    }
+    if (a.X != nullptr && b.xb >= 0)
+      acc += a.gamma * a.X[b.xb + l0 * b.xs[0] + l1 * b.xs[1] + l2 * b.xs[2] + l3 * b.xs[3]] *
+             a.Y[b.yb + l0 * b.ys[0] + l1 * b.ys[1] + l2 * b.ys[2] + l3 * b.ys[3]];
+    double* o = a.out + b.ob + l0 * b.os[0] + l1 * b.os[1] + l2 * b.os[2] + l3 * b.os[3];
+    *o = (a.beta == 0.0) ? acc : a.beta * (*o) + acc;
+  }
+}
+
+int apply_plan(const Plan& p, double* out, double beta, const double* const* src, const double* alpha,
+               const double* X, const double* Y, double gamma, const Denom* den, hipStream_t s) {
+  if (p.nbox == 0 || p.total == 0) return ACES_OK;
+  ApplyArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.boxes = p.dev;
+  a.nbox = p.nbox;
+  a.out = out;
+  a.beta = beta;
+  a.nsrc = p.nsrc;
+  for (int k = 0; k < p.nsrc; ++k) {
+    a.src[k] = src[k];
+    a.alpha[k] = alpha[k];
+  }
+  a.X = p.prod ? X : nullptr;
+  a.Y = p.prod ? Y : nullptr;
+  a.gamma = gamma;
+  if (den) a.den = *den;
+  ACES_REQUIRE(p.total < (1L << 31), "dpd apply: too many chunks");
+  hipLaunchKernelGGL(apply_plan_kernel, dim3((unsigned)p.total), dim3(256), 0, s, a);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+// ---------------------------------------------------------------------------
+struct BlockInfo {
+  long blk_off[kMaxIrr];
+  long nrow[kMaxIrr], ncol[kMaxIrr];
+  long rstart[kMaxIrr + 1];
+  int nirr;
+};
+
+static BlockInfo block_info(const Layout& L) {
+  BlockInfo b;
+  b.nirr = L.nirr;
+  long r = 0;
+  for (int g = 0; g < L.nirr; ++g) {
+    b.blk_off[g] = L.blk_off[g];
+    b.nrow[g] = L.nrow[g];
+    b.ncol[g] = L.ncol[g];
+    b.rstart[g] = r;
+    r += L.nrow[g];
+  }
+  b.rstart[L.nirr] = r;
+  return b;
+}
+
+__global__ void block_rowsum_kernel(BlockInfo b, const double* __restrict__ x, double* __restrict__ out,
+                                    double alpha, double beta) {
+  const int lane = threadIdx.x & 63;
+  const long wid = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long nw = (long)gridDim.x * (blockDim.x >> 6);
+  for (long r = wid; r < b.rstart[b.nirr]; r += nw) {
+    int g = 0;
+    while (r >= b.rstart[g + 1]) ++g;
+    const long lr = r - b.rstart[g];
+    const double* row = x + b.blk_off[g] + lr * b.ncol[g];
+    double acc = 0.0;
+    for (long c = lane; c < b.ncol[g]; c += 64) acc += row[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) out[r] = alpha * acc + (beta == 0.0 ? 0.0 : beta * out[r]);
+  }
+}
+
+int block_rowsum(const Layout& L, const double* x, double* out, double alpha, double beta, hipStream_t s) {
+  BlockInfo b = block_info(L);
+  const long rows = b.rstart[b.nirr];
+  if (rows == 0) return ACES_OK;
+  const unsigned grid = (unsigned)std::min<long>((rows + 3) / 4, 65536);
+  hipLaunchKernelGGL(block_rowsum_kernel, dim3(grid), dim3(256), 0, s, b, x, out, alpha, beta);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+// mode 0: x_ll += alpha e_l ; mode 1: e_l += alpha x_ll
+__global__ void diag_kernel(BlockInfo b, double* __restrict__ x, double* __restrict__ e, double alpha, int mode) {
+  const long n = b.rstart[b.nirr];
+  for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
+    int g = 0;
+    while (r >= b.rstart[g + 1]) ++g;
+    const long l = r - b.rstart[g];
+    double* xd = x + b.blk_off[g] + l * b.ncol[g] + l;
+    if (mode == 0)
+      *xd += alpha * e[r];
+    else
+      e[r] += alpha * (*xd);
+  }
+}
+
+static int diag_launch(const Layout& L, double* x, double* e, double alpha, int mode, hipStream_t s) {
+  ACES_REQUIRE(L.rank == 2, "dpd diag: rank-2 layout required");
+  BlockInfo b = block_info(L);
+  const long n = b.rstart[b.nirr];
+  if (n == 0) return ACES_OK;
+  hipLaunchKernelGGL(diag_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, x, e, alpha, mode);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+int sub_diag_blocks(const Layout& L, double* x, const double* e, double alpha, hipStream_t s) {
+  return diag_launch(L, x, const_cast<double*>(e), alpha, 0, s);
+}
+
+int diag_to_vec(const Layout& L, const double* x, double* e, double alpha, hipStream_t s) {
+  return diag_launch(L, const_cast<double*>(x), e, alpha, 1, s);
+}
+
+}  // namespace dpd
+}  // namespace aces

@@ -44,5 +44,9 @@ struct GemmDesc {
 };
 
 int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream);
+// Grouped launch: up to 8 independent problems (same operand orders and
+// precision, batch 1) in one kernel launch, blockIdx.y selecting the problem.
+// Split-K partials of all groups share d[0].ws.
+int gemm_grouped(const GemmDesc* d, int ngroups, GemmPrec prec, hipStream_t stream);
 
 }  // namespace aces

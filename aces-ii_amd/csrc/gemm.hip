@@ -7,6 +7,7 @@
 // the 8 XCDs each receive a contiguous range of logical tiles, and tiles are
 // walked in groups of 8 row tiles so that co-resident blocks share A and B
 // tiles in the XCD's L2.
+#include <algorithm>
 #include "common.h"
 #include "gemm.h"
 
@@ -35,7 +36,27 @@ struct GemmArgs {
   long kchunk;
   long zkA, zkB, zC;
   int vecA, vecB;  // 16-byte aligned operand rows (vector load path allowed)
+  int splits;      // number of k splits of this problem (grid.z may be larger)
 };
+
+// One launch computes up to kMaxGroups independent problems (the irrep blocks
+// of one term); blockIdx.y selects the group when n > 1, the batch entry when
+// n == 1.  Passed by value in the kernel arguments.
+static constexpr int kMaxGroups = 8;
+struct GemmGroups {
+  int n;
+  GemmArgs g[kMaxGroups];
+};
+
+// Uniform selection with constant indices (keeps the argument block in the
+// kernarg segment).
+__device__ inline GemmArgs select_group(const GemmGroups& gg, int gi) {
+  GemmArgs p = gg.g[0];
+#pragma unroll
+  for (int k = 1; k < kMaxGroups; ++k)
+    if (k == gi) p = gg.g[k];
+  return p;
+}
 
 static constexpr int kBM = 128;
 static constexpr int kBN = 128;
@@ -162,7 +183,7 @@ struct TileLoader {
 };
 
 template <typename T, bool AKC, bool BKC, int BK, int WM, int WN, int FM, int FN>
-__global__ __launch_bounds__(64 * WM * WN) void gemm16_kernel(GemmArgs p) {
+__global__ __launch_bounds__(64 * WM * WN) void gemm16_kernel(GemmGroups gg) {
   constexpr int NT = 64 * WM * WN;
   constexpr int BM = 16 * FM * WM;
   constexpr int BN = 16 * FN * WN;
@@ -173,11 +194,14 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm16_kernel(GemmArgs p) {
   __shared__ __attribute__((aligned(16))) T Bs[2][BN * LDK];
   typedef typename Mfma16<T>::acc_t acc_t;
 
+  const int gi = (gg.n > 1) ? (int)blockIdx.y : 0;
+  const GemmArgs p = select_group(gg, gi);
+  if ((int)blockIdx.x >= p.tiles_m * p.tiles_n || (int)blockIdx.z >= p.splits) return;
   int tm, tn;
   tile_coords(p.tiles_m, p.tiles_n, tm, tn);
   const int m0 = tm * BM;
   const int n0 = tn * BN;
-  const int bz = blockIdx.y;
+  const int bz = (gg.n > 1) ? 0 : (int)blockIdx.y;
   const long kz = (long)blockIdx.z * p.kchunk;
   const T* __restrict__ A = reinterpret_cast<const T*>(p.A) + (long)bz * p.sA + kz * p.zkA;
   const T* __restrict__ B = reinterpret_cast<const T*>(p.B) + (long)bz * p.sB + kz * p.zkB;
@@ -268,7 +292,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm16_kernel(GemmArgs p) {
 // bf16x3 kernel: A = Ahi + Alo, B = Bhi + Blo (bf16, k contiguous), BK = 32.
 // acc += Ahi*Bhi + Ahi*Blo + Alo*Bhi with v_mfma_f32_16x16x32_bf16.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmArgs p) {
+__global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmGroups gg) {
   constexpr int BK = 32;
   constexpr int LDK = BK + 8;  // bf16 elements per LDS row (80 bytes)
   __shared__ __attribute__((aligned(16))) unsigned short Ah[2][kBM * LDK];
@@ -276,11 +300,14 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmArgs p) {
   __shared__ __attribute__((aligned(16))) unsigned short Bh[2][kBN * LDK];
   __shared__ __attribute__((aligned(16))) unsigned short Bl[2][kBN * LDK];
 
+  const int gi = (gg.n > 1) ? (int)blockIdx.y : 0;
+  const GemmArgs p = select_group(gg, gi);
+  if ((int)blockIdx.x >= p.tiles_m * p.tiles_n) return;
   int tm, tn;
   tile_coords(p.tiles_m, p.tiles_n, tm, tn);
   const int m0 = tm * kBM;
   const int n0 = tn * kBN;
-  const int bz = blockIdx.y;
+  const int bz = (gg.n > 1) ? 0 : (int)blockIdx.y;
   const unsigned short* __restrict__ Ahg = reinterpret_cast<const unsigned short*>(p.A) + (long)bz * p.sA;
   const unsigned short* __restrict__ Alg = reinterpret_cast<const unsigned short*>(p.Alo) + (long)bz * p.sA;
   const unsigned short* __restrict__ Bhg = reinterpret_cast<const unsigned short*>(p.B) + (long)bz * p.sB;
@@ -433,7 +460,7 @@ struct TileCfg<float> {
 };
 
 template <typename T>
-static void launch16(const GemmArgs& a, bool akc, bool bkc, dim3 grid, hipStream_t s) {
+static void launch16(const GemmGroups& a, bool akc, bool bkc, dim3 grid, hipStream_t s) {
   typedef typename TileCfg<T>::type C64;
   const dim3 blk(C64::NT);
 #define L16(AK, BK2) \
@@ -449,16 +476,11 @@ static void launch16(const GemmArgs& a, bool akc, bool bkc, dim3 grid, hipStream
 #undef L16
 }
 
-int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream) {
-  ACES_REQUIRE(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.batch >= 1, "gemm: bad dimensions");
-  if (d.M == 0 || d.N == 0) return ACES_OK;
-  ACES_REQUIRE(d.C != nullptr && d.ldc >= d.N, "gemm: bad C");
-  if (d.K > 0) {
-    ACES_REQUIRE(d.A != nullptr && d.B != nullptr, "gemm: null operand");
-    ACES_REQUIRE(d.lda >= (d.a_kc ? d.K : d.M), "gemm: lda too small");
-    ACES_REQUIRE(d.ldb >= (d.b_kc ? d.K : d.N), "gemm: ldb too small");
-  }
-  GemmArgs a;
+// Fill the kernel arguments of one problem; split-K (fp64/fp32 kernels only)
+// when the output cannot fill the chip and K is long, with partial products
+// in ws[ws_off ...].  Returns the number of splits.
+static int prepare(const GemmDesc& d, GemmPrec prec, GemmArgs& a, double* ws, long ws_avail, long ws_off,
+                   long tiles_total) {
   a.M = d.M; a.N = d.N; a.K = d.K;
   a.A = d.A; a.Alo = d.A_lo; a.lda = d.lda;
   a.B = d.B; a.Blo = d.B_lo; a.ldb = d.ldb;
@@ -481,70 +503,143 @@ int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream) {
   a.zkA = d.a_kc ? 1 : d.lda;
   a.zkB = d.b_kc ? 1 : d.ldb;
   a.zC = 0;
-  const long nb = (long)a.tiles_m * a.tiles_n;
-  ACES_REQUIRE(nb < (1L << 31) && d.batch < 65536, "gemm: grid too large");
-  // split-K for outputs that cannot fill 256 CUs (fp64/fp32 kernels only)
+  a.splits = 1;
   int splits = 1;
-  if (prec != GP_BF16X3 && d.ws != nullptr && d.K >= 1024) {
-    const long tiles = nb * d.batch;
-    if (tiles < 512) {
-      long sp = 1024 / tiles;
-      const long kmax = d.K / 512;
-      if (sp > kmax) sp = kmax;
-      const long per = (long)d.M * d.N * d.batch;
-      if (sp * per > d.ws_elems) sp = d.ws_elems / per;
-      if (sp > 1024) sp = 1024;
-      if (sp >= 2) splits = (int)sp;
-    }
+  if (prec != GP_BF16X3 && ws != nullptr && d.K >= 1024 && tiles_total < 512) {
+    long sp = 1024 / tiles_total;
+    const long kmax = d.K / 512;
+    if (sp > kmax) sp = kmax;
+    const long per = (long)d.M * d.N * d.batch;
+    if (sp * per > ws_avail) sp = ws_avail / per;
+    if (sp > 1024) sp = 1024;
+    if (sp >= 2) splits = (int)sp;
   }
   if (splits > 1) {
     long kc = (d.K + splits - 1) / splits;
     kc = (kc + 31) / 32 * 32;
     splits = (int)((d.K + kc - 1) / kc);
     a.kchunk = kc;
-    a.C = d.ws;
+    a.C = ws + ws_off;
     a.ldc = d.N;
     a.sC = (long)d.M * d.N;
     a.zC = (long)d.M * d.N * d.batch;
     a.beta = 0.0;
+    a.splits = splits;
   }
-  dim3 grid((unsigned)nb, (unsigned)d.batch, (unsigned)splits);
-  if (d.K == 0) {
-    // C = beta * C handled by a K = 0 launch of the fp64 kernel (acc stays 0).
-    a.K = 0;
-    launch16<double>(a, true, true, grid, stream);
-    ACES_HIP(hipGetLastError());
-    return ACES_OK;
+  return splits;
+}
+
+static int check_desc(const GemmDesc& d, GemmPrec prec) {
+  ACES_REQUIRE(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.batch >= 1, "gemm: bad dimensions");
+  if (d.M == 0 || d.N == 0) return ACES_OK;
+  ACES_REQUIRE(d.C != nullptr && d.ldc >= d.N, "gemm: bad C");
+  if (d.K > 0) {
+    ACES_REQUIRE(d.A != nullptr && d.B != nullptr, "gemm: null operand");
+    ACES_REQUIRE(d.lda >= (d.a_kc ? d.K : d.M), "gemm: lda too small");
+    ACES_REQUIRE(d.ldb >= (d.b_kc ? d.K : d.N), "gemm: ldb too small");
   }
-  switch (prec) {
-    case GP_F64:
-      launch16<double>(a, d.a_kc != 0, d.b_kc != 0, grid, stream);
-      break;
-    case GP_F32:
-      launch16<float>(a, d.a_kc != 0, d.b_kc != 0, grid, stream);
-      break;
-    case GP_BF16X3:
-      ACES_REQUIRE(d.a_kc && d.b_kc, "gemm bf16x3: operands must be k-contiguous");
-      ACES_REQUIRE(d.A_lo != nullptr && d.B_lo != nullptr, "gemm bf16x3: missing lo parts");
-      ACES_REQUIRE(d.lda % 8 == 0 && d.ldb % 8 == 0 && d.K % 8 == 0 && d.sA % 8 == 0 && d.sB % 8 == 0,
-                   "gemm bf16x3: lda, ldb, K and strides must be multiples of 8");
-      ACES_REQUIRE(((uintptr_t)d.A % 16) == 0 && ((uintptr_t)d.B % 16) == 0 &&
-                       ((uintptr_t)d.A_lo % 16) == 0 && ((uintptr_t)d.B_lo % 16) == 0,
-                   "gemm bf16x3: operands must be 16-byte aligned");
-      hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, dim3(256), 0, stream, a);
-      break;
-    default:
-      ACES_REQUIRE(false, "gemm: unknown precision");
+  if (prec == GP_BF16X3 && d.K > 0) {
+    ACES_REQUIRE(d.a_kc && d.b_kc, "gemm bf16x3: operands must be k-contiguous");
+    ACES_REQUIRE(d.A_lo != nullptr && d.B_lo != nullptr, "gemm bf16x3: missing lo parts");
+    ACES_REQUIRE(d.lda % 8 == 0 && d.ldb % 8 == 0 && d.K % 8 == 0 && d.sA % 8 == 0 && d.sB % 8 == 0,
+                 "gemm bf16x3: lda, ldb, K and strides must be multiples of 8");
+    ACES_REQUIRE(((uintptr_t)d.A % 16) == 0 && ((uintptr_t)d.B % 16) == 0 && ((uintptr_t)d.A_lo % 16) == 0 &&
+                     ((uintptr_t)d.B_lo % 16) == 0,
+                 "gemm bf16x3: operands must be 16-byte aligned");
+  }
+  return ACES_OK;
+}
+
+static int launch_groups(const GemmGroups& gg, GemmPrec prec, bool akc, bool bkc, bool kzero, dim3 grid,
+                         hipStream_t stream) {
+  if (kzero) {
+    // C = beta * C via a K = 0 launch of the fp64 kernel (acc stays 0).
+    launch16<double>(gg, true, true, grid, stream);
+  } else {
+    switch (prec) {
+      case GP_F64:
+        launch16<double>(gg, akc, bkc, grid, stream);
+        break;
+      case GP_F32:
+        launch16<float>(gg, akc, bkc, grid, stream);
+        break;
+      case GP_BF16X3:
+        hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, dim3(256), 0, stream, gg);
+        break;
+      default:
+        ACES_REQUIRE(false, "gemm: unknown precision");
+    }
   }
   ACES_HIP(hipGetLastError());
-  if (splits > 1) {
-    const long total = (long)d.M * d.N * d.batch;
-    long g = (total + 255) / 256;
-    if (g > (1L << 20)) g = 1L << 20;
-    hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)g), dim3(256), 0, stream, d.M, d.N, splits, d.batch,
-                       d.ws, a.zC, d.C, d.ldc, d.sC, d.beta);
-    ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+static int reduce_splits(const GemmDesc& d, const GemmArgs& a, hipStream_t stream) {
+  if (a.splits <= 1) return ACES_OK;
+  const long total = (long)d.M * d.N * d.batch;
+  long g = (total + 255) / 256;
+  if (g > (1L << 20)) g = 1L << 20;
+  hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)g), dim3(256), 0, stream, d.M, d.N, a.splits, d.batch,
+                     a.C, a.zC, d.C, d.ldc, d.sC, d.beta);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream) {
+  ACES_CHECK(check_desc(d, prec));
+  if (d.M == 0 || d.N == 0) return ACES_OK;
+  GemmGroups gg;
+  gg.n = 1;
+  GemmArgs& a = gg.g[0];
+  // tiles for the split decision use this precision's block size
+  GemmArgs probe;
+  prepare(d, prec, probe, nullptr, 0, 0, 1L << 30);
+  const long nb = (long)probe.tiles_m * probe.tiles_n;
+  ACES_REQUIRE(nb < (1L << 31) && d.batch < 65536, "gemm: grid too large");
+  const int splits = prepare(d, prec, a, d.ws, d.ws_elems, 0, nb * d.batch);
+  for (int k = 1; k < kMaxGroups; ++k) gg.g[k] = a;
+  dim3 grid((unsigned)nb, (unsigned)d.batch, (unsigned)splits);
+  if (d.K == 0) a.K = 0;
+  ACES_CHECK(launch_groups(gg, prec, d.a_kc != 0, d.b_kc != 0, d.K == 0, grid, stream));
+  return reduce_splits(d, a, stream);
+}
+
+int gemm_grouped(const GemmDesc* d, int ng, GemmPrec prec, hipStream_t stream) {
+  ACES_REQUIRE(ng >= 1 && ng <= kMaxGroups, "gemm_grouped: 1..8 groups");
+  if (ng == 1) return gemm(d[0], prec, stream);
+  GemmGroups gg;
+  gg.n = ng;
+  long maxnb = 0, ws_off = 0;
+  int maxsplit = 1;
+  bool any = false;
+  for (int g = 0; g < ng; ++g) {
+    ACES_CHECK(check_desc(d[g], prec));
+    ACES_REQUIRE(d[g].batch == 1, "gemm_grouped: batched groups are not supported");
+    ACES_REQUIRE(d[g].a_kc == d[0].a_kc && d[g].b_kc == d[0].b_kc, "gemm_grouped: operand orders must agree");
+    ACES_REQUIRE((d[g].K == 0) == (d[0].K == 0) || d[g].M == 0 || d[g].N == 0,
+                 "gemm_grouped: mixed K = 0 and K > 0 groups");
+    GemmArgs probe;
+    prepare(d[g], prec, probe, nullptr, 0, 0, 1L << 30);
+    const long nb = (d[g].M == 0 || d[g].N == 0) ? 0 : (long)probe.tiles_m * probe.tiles_n;
+    const long avail = d[0].ws ? d[0].ws_elems - ws_off : 0;
+    const int sp = prepare(d[g], prec, gg.g[g], d[0].ws, avail, ws_off, nb);
+    if (sp > 1) ws_off += (long)sp * d[g].M * d[g].N;
+    if (nb == 0) gg.g[g].tiles_m = 0;  // empty block: every workgroup exits
+    if (d[g].K == 0) gg.g[g].K = 0;
+    maxnb = std::max(maxnb, nb);
+    maxsplit = std::max(maxsplit, sp);
+    any = any || nb > 0;
   }
+  for (int g = ng; g < kMaxGroups; ++g) gg.g[g] = gg.g[0];
+  if (!any) return ACES_OK;
+  ACES_REQUIRE(maxnb < (1L << 31), "gemm_grouped: grid too large");
+  int kz = 0;
+  for (int g = 0; g < ng; ++g)
+    if (d[g].M > 0 && d[g].N > 0) kz = (d[g].K == 0);
+  dim3 grid((unsigned)maxnb, (unsigned)ng, (unsigned)maxsplit);
+  ACES_CHECK(launch_groups(gg, prec, d[0].a_kc != 0, d[0].b_kc != 0, kz != 0, grid, stream));
+  for (int g = 0; g < ng; ++g)
+    if (d[g].M > 0 && d[g].N > 0) ACES_CHECK(reduce_splits(d[g], gg.g[g], stream));
   return ACES_OK;
 }
 

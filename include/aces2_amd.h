@@ -42,6 +42,16 @@ const char* aces_last_error(void);
 int aces_gemm(int prec, int M, int N, int K, const void* d_A, const void* d_A_lo, long lda, int a_kc,
               const void* d_B, const void* d_B_lo, long ldb, int b_kc, double* d_C, long ldc, double alpha,
               double beta, int batch, long sA, long sB, long sC, void* stream);
+/* Grouped launch: ngroups (<= 8) independent problems C_g = alpha A_g B_g^T +
+ * beta C_g (per-group sizes, leading dimensions and device pointers; shared
+ * operand orders) in one kernel launch, blockIdx.y selecting the problem -- the
+ * per-irrep blocks of one contraction term.  ws (optional, ws_elems doubles)
+ * holds split-K partials.  No reference counterpart (the reference loops over
+ * irreps with one XGEMM per block, e.g. vcc/ladab1.F, vcc/rngprd.f). */
+int aces_gemm_grouped(int prec, int ngroups, const int* M, const int* N, const int* K, const void* const* d_A,
+                      const void* const* d_A_lo, const long* lda, int a_kc, const void* const* d_B,
+                      const void* const* d_B_lo, const long* ldb, int b_kc, double* const* d_C, const long* ldc,
+                      double alpha, double beta, double* d_ws, long ws_elems, void* stream);
 int aces_convert_f32(int rows, int cols, const double* d_x, long ldx, float* d_y, long ldy, void* stream);
 int aces_split_bf16(int rows, int cols, const double* d_x, long ldx, unsigned short* d_hi, unsigned short* d_lo,
                     long ldy, void* stream);

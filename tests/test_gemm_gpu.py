@@ -76,3 +76,82 @@ def test_f32_and_bf16x3(M, N, K):
     torch.cuda.synchronize()
     err16 = ((C - ref).abs() / scale).max().item()
     assert err16 < 3e-5, err16
+
+
+def _grouped(prec, groups, a_kc, b_kc, alpha, beta, ws=None):
+    import ctypes
+    from aces2 import _lib
+    ng = len(groups)
+    I = ctypes.c_int * ng
+    Lg = ctypes.c_long * ng
+    P = ctypes.c_void_p * ng
+    M = I(*[g['M'] for g in groups])
+    N = I(*[g['N'] for g in groups])
+    K = I(*[g['K'] for g in groups])
+    A = P(*[g['A'].data_ptr() for g in groups])
+    B = P(*[g['B'].data_ptr() for g in groups])
+    Alo = P(*[g['Alo'].data_ptr() for g in groups]) if prec == 2 else None
+    Blo = P(*[g['Blo'].data_ptr() for g in groups]) if prec == 2 else None
+    C = P(*[g['C'].data_ptr() for g in groups])
+    lda = Lg(*[g['A'].shape[-1] for g in groups])
+    ldb = Lg(*[g['B'].shape[-1] for g in groups])
+    ldc = Lg(*[g['C'].shape[-1] for g in groups])
+    _lib.call('aces_gemm_grouped', prec, ng, M, N, K, A, Alo, lda, a_kc, B, Blo, ldb, b_kc, C, ldc, alpha, beta,
+              ws.data_ptr() if ws is not None else None, ws.numel() if ws is not None else 0, None)
+
+
+@pytest.mark.parametrize('a_kc,b_kc', [(1, 1), (1, 0), (0, 1), (0, 0)])
+@pytest.mark.parametrize('split', [False, True])
+def test_f64_grouped(a_kc, b_kc, split):
+    """Uneven irrep-block shapes in one grouped launch (incl. an empty block and
+    skinny long-K blocks that take the split-K path)."""
+    print('case f64 grouped', a_kc, b_kc, split, flush=True)
+    g = torch.Generator(device='cuda').manual_seed(11 + 2 * a_kc + b_kc)
+    shapes = [(70, 50, 90), (0, 10, 10), (5, 7, 3000), (129, 3, 1), (16, 16, 5000), (200, 180, 64), (1, 1, 1)]
+    groups, refs = [], []
+    for (M, N, K) in shapes:
+        Am = torch.randn(M, K, dtype=torch.float64, device='cuda', generator=g)
+        Bm = torch.randn(N, K, dtype=torch.float64, device='cuda', generator=g)
+        C0 = torch.randn(M, N, dtype=torch.float64, device='cuda', generator=g)
+        A = Am.contiguous() if a_kc else Am.t().contiguous()
+        B = Bm.contiguous() if b_kc else Bm.t().contiguous()
+        if M == 0:
+            A = torch.zeros(1, max(K, 1), dtype=torch.float64, device='cuda')
+            C0 = torch.zeros(0, N, dtype=torch.float64, device='cuda')
+        groups.append({'M': M, 'N': N, 'K': K, 'A': A, 'B': B, 'C': C0.clone()})
+        refs.append(0.5 * (Am @ Bm.t()) + 0.25 * C0)
+    ws = torch.empty(1 << 22, dtype=torch.float64, device='cuda') if split else None
+    _grouped(0, groups, a_kc, b_kc, 0.5, 0.25, ws)
+    torch.cuda.synchronize()
+    for gr, ref in zip(groups, refs):
+        assert torch.allclose(gr['C'], ref, rtol=1e-12, atol=1e-12 * (gr['K'] ** 0.5 + 1))
+
+
+@pytest.mark.parametrize('prec', [1, 2])
+def test_lowp_grouped(prec):
+    from aces2 import _lib
+    print('case lowp grouped', prec, flush=True)
+    g = torch.Generator(device='cuda').manual_seed(31 + prec)
+    shapes = [(130, 70, 64), (33, 200, 128), (256, 256, 256)]
+    groups, refs = [], []
+    for (M, N, K) in shapes:
+        Am = torch.randn(M, K, dtype=torch.float64, device='cuda', generator=g)
+        Bm = torch.randn(N, K, dtype=torch.float64, device='cuda', generator=g)
+        C0 = torch.zeros(M, N, dtype=torch.float64, device='cuda')
+        gr = {'M': M, 'N': N, 'K': K, 'C': C0}
+        if prec == 1:
+            gr['A'], gr['B'] = Am.float().contiguous(), Bm.float().contiguous()
+        else:
+            for nm, X in (('A', Am), ('B', Bm)):
+                hi = torch.empty(X.shape, dtype=torch.int16, device='cuda')
+                lo = torch.empty(X.shape, dtype=torch.int16, device='cuda')
+                _lib.call('aces_split_bf16', X.shape[0], X.shape[1], X.data_ptr(), X.shape[1], hi.data_ptr(),
+                          lo.data_ptr(), X.shape[1], None)
+                gr[nm], gr[nm + 'lo'] = hi, lo
+        groups.append(gr)
+        refs.append((Am @ Bm.t(), Am.abs() @ Bm.abs().t()))
+    _grouped(prec, groups, 1, 1, 1.0, 0.0)
+    torch.cuda.synchronize()
+    for gr, (ref, scale) in zip(groups, refs):
+        err = ((gr['C'] - ref).abs() / scale).max().item()
+        assert err < (4e-7 if prec == 1 else 3e-5), err
