@@ -50,6 +50,27 @@ SIGNATURES = {
     'aces_ccsd_get_lambda': (c_int, [c_vp, c_vp, c_vp, c_int]),
     'aces_ccsd_set_lambda': (c_int, [c_vp, c_vp, c_vp, c_int]),
     'aces_ccsd_density': (c_int, [c_vp, c_vp]),
+    'aces_sym_create': (c_int, [c_int, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
+    'aces_sym_destroy': (c_int, [c_vp]),
+    'aces_sym_setup': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    'aces_sym_set_amplitudes': (c_int, [c_vp, c_vp, c_vp]),
+    'aces_sym_get_amplitudes': (c_int, [c_vp, c_vp, c_vp]),
+    'aces_sym_start_mp1': (c_int, [c_vp]),
+    'aces_sym_increments': (c_int, [c_vp, c_int, c_vp, c_vp]),
+    'aces_sym_cycle': (c_int, [c_vp, c_int, c_int, c_double, c_dp]),
+    'aces_sym_set_diis_order': (c_int, [c_vp, c_int]),
+    'aces_sym_reset_diis': (c_int, [c_vp]),
+    'aces_sym_energy': (c_int, [c_vp, c_dp]),
+    'aces_sym_adjoint': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    'aces_sym_lambda_start': (c_int, [c_vp]),
+    'aces_sym_lambda_cycle': (c_int, [c_vp, c_int, c_double, c_dp]),
+    'aces_sym_set_lambda_diis_order': (c_int, [c_vp, c_int]),
+    'aces_sym_get_lambda': (c_int, [c_vp, c_vp, c_vp, c_int]),
+    'aces_sym_density': (c_int, [c_vp, c_vp]),
+    'aces_sym_set_profiling': (c_int, [c_vp, c_int]),
+    'aces_sym_profile': (c_int, [c_vp, c_dp, c_int]),
+    'aces_sym_device_bytes': (c_long, [c_vp]),
+    'aces_sym_packed_t2': (c_long, [c_vp]),
     'aces_init': (c_int, [c_vp]),
     'aces_fin': (c_int, []),
     'aces_set_symmetry': (c_int, [c_int, c_vp, c_vp, c_vp, c_vp]),

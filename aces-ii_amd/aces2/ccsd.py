@@ -24,22 +24,24 @@ class LambdaNotConverged(RuntimeError):
 
 
 class CCSD:
+    _p = 'aces_ccsd_'
+
     def __init__(self, o, v, blocks, stream=None):
         import torch
         self.o, self.v = o, v
         self.torch = torch
         self.stream = stream
         h = ctypes.c_void_p()
-        _lib.call('aces_ccsd_create', o, v, stream, ctypes.byref(h))
+        _lib.call(self._p + 'create', o, v, stream, ctypes.byref(h))
         self.h = h
         dev = {k: torch.as_tensor(x, device='cuda', dtype=torch.float64).contiguous() for k, x in blocks.items()}
-        _lib.call('aces_ccsd_setup', self.h, dev['fock'].data_ptr(), dev['ovov'].data_ptr(), dev['oovv'].data_ptr(),
+        _lib.call(self._p + 'setup', self.h, dev['fock'].data_ptr(), dev['ovov'].data_ptr(), dev['oovv'].data_ptr(),
                   dev['ovoo'].data_ptr(), dev['oooo'].data_ptr(), dev['ovvv'].data_ptr(), dev['list233'].data_ptr())
         del dev
 
     def close(self):
         if self.h:
-            _lib.lib().aces_ccsd_destroy(self.h)
+            getattr(_lib.lib(), self._p + 'destroy')(self.h)
             self.h = None
 
     def __del__(self):
@@ -50,31 +52,31 @@ class CCSD:
 
     @property
     def device_bytes(self):
-        return _lib.lib().aces_ccsd_device_bytes(self.h)
+        return getattr(_lib.lib(), self._p + 'device_bytes')(self.h)
 
     def set_amplitudes(self, t1, t2):
         torch = self.torch
         t1 = torch.as_tensor(t1, device='cuda', dtype=torch.float64).contiguous()
         t2 = torch.as_tensor(t2, device='cuda', dtype=torch.float64).contiguous()
-        _lib.call('aces_ccsd_set_amplitudes', self.h, t1.data_ptr(), t2.data_ptr())
+        _lib.call(self._p + 'set_amplitudes', self.h, t1.data_ptr(), t2.data_ptr())
 
     def get_amplitudes(self):
         torch = self.torch
         o, v = self.o, self.v
         t1 = torch.empty((o, v), device='cuda', dtype=torch.float64)
         t2 = torch.empty((o, o, v, v), device='cuda', dtype=torch.float64)
-        _lib.call('aces_ccsd_get_amplitudes', self.h, t1.data_ptr(), t2.data_ptr())
+        _lib.call(self._p + 'get_amplitudes', self.h, t1.data_ptr(), t2.data_ptr())
         return t1, t2
 
     def start_mp1(self):
-        _lib.call('aces_ccsd_start_mp1', self.h)
+        _lib.call(self._p + 'start_mp1', self.h)
 
     def increments(self, prec='fp64'):
         torch = self.torch
         o, v = self.o, self.v
         i1 = torch.empty((o, v), device='cuda', dtype=torch.float64)
         i2 = torch.empty((o, o, v, v), device='cuda', dtype=torch.float64)
-        _lib.call('aces_ccsd_increments', self.h, PREC[prec], i1.data_ptr(), i2.data_ptr())
+        _lib.call(self._p + 'increments', self.h, PREC[prec], i1.data_ptr(), i2.data_ptr())
         return i1, i2
 
     def energy(self, t1, t2):
@@ -82,15 +84,15 @@ class CCSD:
         t1 = torch.as_tensor(t1, device='cuda', dtype=torch.float64).contiguous()
         t2 = torch.as_tensor(t2, device='cuda', dtype=torch.float64).contiguous()
         e = ctypes.c_double()
-        _lib.call('aces_ccsd_energy', self.h, t1.data_ptr(), t2.data_ptr(), ctypes.byref(e))
+        _lib.call(self._p + 'energy', self.h, t1.data_ptr(), t2.data_ptr(), ctypes.byref(e))
         return e.value
 
     def set_profiling(self, on=True):
-        _lib.call('aces_ccsd_set_profiling', self.h, 1 if on else 0)
+        _lib.call(self._p + 'set_profiling', self.h, 1 if on else 0)
 
     def profile(self, reset=True):
         out = (ctypes.c_double * 15)()
-        _lib.call('aces_ccsd_profile', self.h, out, 1 if reset else 0)
+        _lib.call(self._p + 'profile', self.h, out, 1 if reset else 0)
         names = ('ladder', 'wbuild', 'ring', 'other', 'adjoint')
         return {nm: {'ms': out[3 * c], 'flops': out[3 * c + 1], 'launches': int(out[3 * c + 2])}
                 for c, nm in enumerate(names)}
@@ -98,17 +100,17 @@ class CCSD:
     def set_fock(self, fock):
         torch = self.torch
         f = torch.as_tensor(fock, device='cuda', dtype=torch.float64).contiguous()
-        _lib.call('aces_ccsd_set_fock', self.h, f.data_ptr())
+        _lib.call(self._p + 'set_fock', self.h, f.data_ptr())
 
     def set_diis_order(self, order):
-        _lib.call('aces_ccsd_set_diis_order', self.h, order)
+        _lib.call(self._p + 'set_diis_order', self.h, order)
 
     def reset_diis(self):
-        _lib.call('aces_ccsd_reset_diis', self.h)
+        _lib.call(self._p + 'reset_diis', self.h)
 
     def cycle(self, icycle, prec='fp64', tol=1e-7):
         st = (ctypes.c_double * 8)()
-        _lib.call('aces_ccsd_cycle', self.h, icycle, PREC[prec], tol, st)
+        _lib.call(self._p + 'cycle', self.h, icycle, PREC[prec], tol, st)
         return {'energy': st[0], 'resmiter': st[1], 'resmdiis': st[2], 'converged': bool(st[3]),
                 'final_energy': st[4], 'ms_increments': st[5], 'ms_cycle': st[6]}
 
@@ -156,19 +158,19 @@ class CCSD:
         r1 = torch.empty((o, v), device='cuda', dtype=torch.float64)
         r2 = torch.empty((o, o, v, v), device='cuda', dtype=torch.float64)
         dens = torch.empty((n, n), device='cuda', dtype=torch.float64) if density else None
-        _lib.call('aces_ccsd_adjoint', self.h, l1.data_ptr(), l2.data_ptr(), r1.data_ptr(), r2.data_ptr(),
+        _lib.call(self._p + 'adjoint', self.h, l1.data_ptr(), l2.data_ptr(), r1.data_ptr(), r2.data_ptr(),
                   dens.data_ptr() if density else None)
         return (r1, r2, dens) if density else (r1, r2)
 
     def lambda_start(self):
-        _lib.call('aces_ccsd_lambda_start', self.h)
+        _lib.call(self._p + 'lambda_start', self.h)
 
     def set_lambda_diis_order(self, order):
-        _lib.call('aces_ccsd_set_lambda_diis_order', self.h, order)
+        _lib.call(self._p + 'set_lambda_diis_order', self.h, order)
 
     def lambda_cycle(self, icycle, tol=1e-7):
         st = (ctypes.c_double * 8)()
-        _lib.call('aces_ccsd_lambda_cycle', self.h, icycle, tol, st)
+        _lib.call(self._p + 'lambda_cycle', self.h, icycle, tol, st)
         return {'resmiter': st[1], 'resmdiis': st[2], 'converged': bool(st[3]), 'ms_cycle': st[6]}
 
     def get_lambda(self, aces_norm=False):
@@ -176,21 +178,21 @@ class CCSD:
         o, v = self.o, self.v
         l1 = torch.empty((o, v), device='cuda', dtype=torch.float64)
         l2 = torch.empty((o, o, v, v), device='cuda', dtype=torch.float64)
-        _lib.call('aces_ccsd_get_lambda', self.h, l1.data_ptr(), l2.data_ptr(), 1 if aces_norm else 0)
+        _lib.call(self._p + 'get_lambda', self.h, l1.data_ptr(), l2.data_ptr(), 1 if aces_norm else 0)
         return l1, l2
 
     def set_lambda(self, l1, l2, aces_norm=False):
         torch = self.torch
         l1 = torch.as_tensor(l1, device='cuda', dtype=torch.float64).contiguous()
         l2 = torch.as_tensor(l2, device='cuda', dtype=torch.float64).contiguous()
-        _lib.call('aces_ccsd_set_lambda', self.h, l1.data_ptr(), l2.data_ptr(), 1 if aces_norm else 0)
+        _lib.call(self._p + 'set_lambda', self.h, l1.data_ptr(), l2.data_ptr(), 1 if aces_norm else 0)
 
     def density(self):
         """Response one-particle density dE_corr/dfock of the current multipliers."""
         torch = self.torch
         n = self.o + self.v
         d = torch.empty((n, n), device='cuda', dtype=torch.float64)
-        _lib.call('aces_ccsd_density', self.h, d.data_ptr())
+        _lib.call(self._p + 'density', self.h, d.data_ptr())
         return d
 
     def solve_lambda(self, maxcyc=50, conv=7, order=6, log=None, raise_on_fail=True):
@@ -213,4 +215,71 @@ class CCSD:
                 break
         if not hist['converged'] and raise_on_fail:
             raise LambdaNotConverged('The Lambda equations did not converge')
+        return hist
+
+
+class SymCCSD(CCSD):
+    """The same driver over the irrep-blocked engine (``aces_sym_*``).
+
+    ``occ``/``vir`` give the orbitals per irrep (nirr = 1, 2, 4 or 8, irrep
+    product = XOR of 0-based labels); ``blocks`` are dense C1 arrays with the
+    orbitals irrep-sorted inside the occupied and the virtual space.
+    """
+    _p = 'aces_sym_'
+
+    def __init__(self, occ, vir, blocks, stream=None):
+        import torch
+        self.occ, self.vir = list(occ), list(vir)
+        self.o, self.v = sum(occ), sum(vir)
+        self.torch = torch
+        self.stream = stream
+        nirr = len(occ)
+        h = ctypes.c_void_p()
+        O = (ctypes.c_int * nirr)(*occ)
+        V = (ctypes.c_int * nirr)(*vir)
+        _lib.call('aces_sym_create', nirr, O, V, stream, ctypes.byref(h))
+        self.h = h
+        dev = {k: torch.as_tensor(x, device='cuda', dtype=torch.float64).contiguous() for k, x in blocks.items()
+               if k in ('fock', 'ovov', 'oovv', 'ovoo', 'oooo', 'ovvv', 'list233')}
+        _lib.call('aces_sym_setup', self.h, dev['fock'].data_ptr(), dev['ovov'].data_ptr(), dev['oovv'].data_ptr(),
+                  dev['ovoo'].data_ptr(), dev['oooo'].data_ptr(), dev['ovvv'].data_ptr(), dev['list233'].data_ptr())
+        del dev
+
+    def energy(self, t1=None, t2=None):
+        """Energy of the engine's current amplitudes (after set_amplitudes when given)."""
+        if t1 is not None:
+            self.set_amplitudes(t1, t2)
+        e = ctypes.c_double()
+        _lib.call('aces_sym_energy', self.h, ctypes.byref(e))
+        return e.value
+
+    @property
+    def packed_t2(self):
+        return _lib.lib().aces_sym_packed_t2(self.h)
+
+    def run(self, maxcyc=50, conv=7, order=6, start='mp1', prec='fp64', log=None, raise_on_fail=True):
+        torch = self.torch
+        self.set_diis_order(order)
+        self.reset_diis()
+        if start == 'mp1':
+            self.start_mp1()
+        elif start == 'zero':
+            self.set_amplitudes(torch.zeros(self.o, self.v), torch.zeros(self.o, self.o, self.v, self.v))
+        hist = {'energy': [self.energy()], 'resmiter': [], 'resmdiis': [], 'ms_cycle': [], 'converged': False}
+        tol = 10.0 ** (-conv)
+        for icycle in range(1, maxcyc + 1):
+            st = self.cycle(icycle, prec, tol)
+            hist['energy'].append(st['energy'])
+            hist['resmiter'].append(st['resmiter'])
+            hist['resmdiis'].append(st['resmdiis'])
+            hist['ms_cycle'].append(st['ms_cycle'])
+            if log:
+                log(icycle, st)
+            if st['converged']:
+                hist['converged'] = True
+                hist['final_energy'] = st['final_energy']
+                hist['cycles'] = icycle
+                break
+        if not hist['converged'] and raise_on_fail:
+            raise NotConverged('@VCC: The Coupled-Cluster equations did not converge!')
         return hist

@@ -129,3 +129,50 @@ def blocks_device(eps, B, o, perturb=None, device='cuda', weights=None, fock_mat
     out['list233'] = L
     del Bd, Wd, Boo, Bov, Bvv, Woo, Wov, B3
     return out
+
+
+# ---------------------------------------------------------------------------
+# Symmetry-blocked inputs (abelian groups, irrep product = XOR of labels)
+
+def irrep_split(total, nirr, minimum, rng):
+    """Per-irrep counts: `minimum` each plus a multinomial share of the rest."""
+    extra = total - minimum * nirr
+    if extra < 0:
+        raise ValueError('too few orbitals for %d irreps with at least %d each' % (nirr, minimum))
+    return [minimum + int(x) for x in rng.multinomial(extra, [1.0 / nirr] * nirr)]
+
+
+def sym_draw(occ, vir, seed, naux=None, sigma=None):
+    """Orbital energies and DF factors of a symmetric system.
+
+    The per-irrep generator: occupied eps ~ U(-2.0, -0.4) and virtual eps ~
+    U(0.1, 4.0), sorted inside each irrep; B[P] symmetric with entries
+    N(0, sigma^2), auxiliary function P of irrep P mod nirr, and B[P][p][q]
+    zeroed unless irrep(p) ^ irrep(q) = irrep(P), so (pq|rs) vanishes unless
+    irrep(p)^irrep(q)^irrep(r)^irrep(s) = 0.  Orbitals are ordered occupied
+    (irrep 0..nirr-1) then virtual (irrep 0..nirr-1).  Returns (eps, B, irreps).
+    """
+    nirr = len(occ)
+    o, v = sum(occ), sum(vir)
+    n = o + v
+    naux = 4 * n if naux is None else naux
+    sigma = scaled_sigma(o, v) if sigma is None else sigma
+    rng = np.random.default_rng(seed)
+    eo = np.concatenate([np.sort(rng.uniform(-2.0, -0.4, c)) for c in occ])
+    ev = np.concatenate([np.sort(rng.uniform(0.1, 4.0, c)) for c in vir])
+    irr = np.concatenate([np.repeat(np.arange(nirr), occ), np.repeat(np.arange(nirr), vir)]).astype(np.int64)
+    X = rng.normal(0.0, sigma, (naux, n, n))
+    iu = np.triu_indices(n, 1)
+    X[:, iu[1], iu[0]] = X[:, iu[0], iu[1]]
+    pair_irr = irr[:, None] ^ irr[None, :]
+    aux_irr = np.arange(naux) % nirr
+    X *= (pair_irr[None, :, :] == aux_irr[:, None, None])
+    return np.concatenate([eo, ev]), X, irr
+
+
+def sym_masks(irr, o):
+    """Boolean masks of the symmetry-allowed t1 (o, v) and t2 (o, o, v, v) elements."""
+    io, iv = irr[:o], irr[o:]
+    m1 = (io[:, None] ^ iv[None, :]) == 0
+    m2 = (io[:, None, None, None] ^ io[None, :, None, None] ^ iv[None, None, :, None] ^ iv[None, None, None, :]) == 0
+    return m1, m2

@@ -28,36 +28,6 @@ using eu::gd;
 
 inline long pad8(long x) { return (x + 7) / 8 * 8; }
 
-// LU with partial pivoting (dgesv semantics) for the small DIIS system.
-int solve_small(int n, std::vector<double> A, std::vector<double>& b) {
-  std::vector<int> piv(n);
-  for (int k = 0; k < n; ++k) {
-    int p = k;
-    double best = std::fabs(A[k * n + k]);
-    for (int i = k + 1; i < n; ++i)
-      if (std::fabs(A[i * n + k]) > best) {
-        best = std::fabs(A[i * n + k]);
-        p = i;
-      }
-    if (best == 0.0) return ACES_ERR_SOLVE;
-    if (p != k) {
-      for (int j = 0; j < n; ++j) std::swap(A[k * n + j], A[p * n + j]);
-      std::swap(b[k], b[p]);
-    }
-    for (int i = k + 1; i < n; ++i) {
-      const double f = A[i * n + k] / A[k * n + k];
-      A[i * n + k] = f;
-      for (int j = k + 1; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
-      b[i] -= f * b[k];
-    }
-  }
-  for (int i = n - 1; i >= 0; --i) {
-    double s = b[i];
-    for (int j = i + 1; j < n; ++j) s -= A[i * n + j] * b[j];
-    b[i] = s / A[i * n + i];
-  }
-  return ACES_OK;
-}
 }  // namespace
 
 CcsdEngine::CcsdEngine(int o_, int v_, hipStream_t s) : o(o_), v(v_), n(o_ + v_), s_(s) {
@@ -634,7 +604,7 @@ int CcsdEngine::diis_core(DiisState& D, double* x, double* xn, int icycle, doubl
     for (int i = 1; i < na; ++i) A[(size_t)i * na + j] = D.R[(size_t)(i - 1) * order + (j - 1)];
   }
   b[0] = -1.0;
-  if (solve_small(na, A, b) != ACES_OK) {
+  if (eu::solve_small_lu(na, A, b) != ACES_OK) {
     set_error("DODIIS: linear solver failed (singular DIIS matrix)");
     return ACES_ERR_SOLVE;
   }

@@ -34,6 +34,10 @@ struct Space {
 struct Layout {
   int rank = 0;   // 2 or 4
   int split = 1;  // number of row indices
+  // dense: a plain row-major array over the full index ranges (orbitals in
+  // irrep-sorted order) with element strides dstride (no blocks)
+  int dense = 0;
+  long dstride[4] = {0, 0, 0, 0};
   Space sp[4];
   int nirr = 1;
   long blk_off[kMaxIrr + 1] = {0};
@@ -46,6 +50,14 @@ struct Layout {
   // Affine map for fixed index irreps g[0..rank-1] (total irrep 0):
   // offset = base + sum_k loc_k * st[k].
   __host__ __device__ void affine(const int* g, long& base, long* st) const {
+    if (dense) {
+      base = 0;
+      for (int k = 0; k < rank; ++k) {
+        base += (long)sp[k].off[g[k]] * dstride[k];
+        st[k] = dstride[k];
+      }
+      return;
+    }
     int G = 0;
     for (int k = 0; k < split; ++k) G ^= g[k];
     const long nc = ncol[G];
@@ -91,6 +103,8 @@ struct Layout {
 
 Layout make_layout(int rank, int split, const Space* s0, const Space* s1, const Space* s2 = nullptr,
                    const Space* s3 = nullptr);
+// Dense row-major array (leading dimensions given as element strides).
+Layout make_dense(int rank, const Space* const* sp, const long* strides);
 
 // Packed pair spaces for the particle-particle ladder: (p<=q) or (p<q) over
 // one Space, blocked by pair irrep.  G = 0: per irrep g a triangle of pairs
@@ -166,6 +180,21 @@ struct Box {
   long xb, xs[4], yb, ys[4];  // xb < 0: product term vanishes in this box
 };
 
+constexpr long kChunk = 4096;  // elements per workgroup in box walks
+
+// Box of workgroup `chunk` (binary search over Box::first).
+__device__ inline int box_of_chunk(const Box* boxes, int nbox, long chunk) {
+  int lo = 0, hi = nbox - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (boxes[mid].first <= chunk)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 struct Plan {
   std::vector<Box> host;
   Box* dev = nullptr;
@@ -188,6 +217,8 @@ struct Src {
 // symmetric operands that map one to one).
 int build_plan(Plan& p, const Layout& out, const std::vector<Src>& srcs, const Src* X, const Src* Y);
 void free_plan(Plan& p);
+// Host part of build_plan (boxes only, nothing uploaded).
+int build_boxes(Plan& p, const Layout& out, const std::vector<Src>& srcs, const Src* X, const Src* Y);
 
 // Denominator scaling: D = sum_k dsign[k] * e(space of dim k)[global index],
 // e = eo for occupied dims, ev for virtual dims.

@@ -61,6 +61,22 @@ Layout make_layout(int rank, int split, const Space* s0, const Space* s1, const 
   return L;
 }
 
+Layout make_dense(int rank, const Space* const* sp, const long* strides) {
+  Layout L;
+  L.rank = rank;
+  L.split = rank / 2;
+  L.dense = 1;
+  L.nirr = sp[0]->nirr;
+  long n = 1;
+  for (int k = 0; k < rank; ++k) {
+    L.sp[k] = *sp[k];
+    L.dstride[k] = strides[k];
+    n *= sp[k]->n;
+  }
+  L.blk_off[L.nirr] = n;
+  return L;
+}
+
 bool Layout::same(const Layout& o) const {
   if (rank != o.rank || split != o.split || nirr != o.nirr) return false;
   for (int k = 0; k < rank; ++k)
@@ -91,9 +107,8 @@ PairSpace make_pairs(const Space& s, bool strict) {
 }
 
 // ---------------------------------------------------------------------------
-static constexpr long kChunk = 4096;
 
-int build_plan(Plan& p, const Layout& out, const std::vector<Src>& srcs, const Src* X, const Src* Y) {
+int build_boxes(Plan& p, const Layout& out, const std::vector<Src>& srcs, const Src* X, const Src* Y) {
   ACES_REQUIRE(out.rank == 2 || out.rank == 4, "dpd plan: output rank must be 2 or 4");
   ACES_REQUIRE(srcs.size() <= 3, "dpd plan: at most three sources");
   const int nirr = out.nirr;
@@ -158,6 +173,11 @@ int build_plan(Plan& p, const Layout& out, const std::vector<Src>& srcs, const S
   }
   p.nbox = (int)p.host.size();
   p.total = first;  // chunks
+  return ACES_OK;
+}
+
+int build_plan(Plan& p, const Layout& out, const std::vector<Src>& srcs, const Src* X, const Src* Y) {
+  ACES_CHECK(build_boxes(p, out, srcs, X, Y));
   free_plan(p);
   if (p.nbox > 0) {
     ACES_HIP(hipMalloc((void**)&p.dev, sizeof(Box) * p.nbox));
@@ -187,18 +207,7 @@ struct ApplyArgs {
 
 __global__ __launch_bounds__(256) void apply_plan_kernel(ApplyArgs a) {
   __shared__ int sb;
-  if (threadIdx.x == 0) {
-    int lo = 0, hi = a.nbox - 1;
-    const long c = blockIdx.x;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (a.boxes[mid].first <= c)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    sb = lo;
-  }
+  if (threadIdx.x == 0) sb = box_of_chunk(a.boxes, a.nbox, blockIdx.x);
   __syncthreads();
   const Box& b = a.boxes[sb];
   const long elems = (long)b.d[0] * b.d[1] * b.d[2] * b.d[3];

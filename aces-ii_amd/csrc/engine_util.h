@@ -2,6 +2,10 @@
 #pragma once
 #include "gemm.h"
 #include "tensor_ops.h"
+#include <cmath>
+#include <utility>
+#include <vector>
+#include "common.h"
 
 namespace aces {
 namespace eu {
@@ -43,5 +47,35 @@ inline GemmDesc gd(int M, int N, int K, const double* A, long lda, int akc, cons
   return d;
 }
 
+// LU with partial pivoting (dgesv semantics) for the small DIIS system.
+inline int solve_small_lu(int n, std::vector<double> A, std::vector<double>& b) {
+  std::vector<int> piv(n);
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = std::fabs(A[k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(A[i * n + k]) > best) {
+        best = std::fabs(A[i * n + k]);
+        p = i;
+      }
+    if (best == 0.0) return ACES_ERR_SOLVE;
+    if (p != k) {
+      for (int j = 0; j < n; ++j) std::swap(A[k * n + j], A[p * n + j]);
+      std::swap(b[k], b[p]);
+    }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = A[i * n + k] / A[k * n + k];
+      A[i * n + k] = f;
+      for (int j = k + 1; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < n; ++j) s -= A[i * n + j] * b[j];
+    b[i] = s / A[i * n + i];
+  }
+  return ACES_OK;
+}
 }  // namespace eu
 }  // namespace aces

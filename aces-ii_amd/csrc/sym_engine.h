@@ -1,0 +1,165 @@
+// Closed-shell CCSD on irrep-blocked lists (abelian point groups up to D2h),
+// one gfx950 GPU: amplitude equations, Lambda equations and response density.
+//
+// Same equations as CcsdEngine (C1), restated on irrep-blocked storage
+// (dpd.h): every contraction term is one grouped MFMA launch over the irrep
+// blocks (gemm_grouped), every index permutation one box-plan launch.  The
+// increments are recorded once as a tape of operations; running the tape
+// forwards gives the CC increments, running it backwards with the generic
+// adjoint of each operation gives the Lambda residual and dL/dfock.
+//
+// Orbitals are irrep-sorted inside the occupied and the virtual space; the
+// dense interfaces (setup, amplitudes in/out, densities) use C1 arrays in
+// that order, with symmetry-forbidden elements zero.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <vector>
+#include "ccsd_engine.h"
+#include "dpd.h"
+#include "gemm.h"
+#include "sym_ladder.h"
+
+namespace aces {
+
+class SymEngine {
+ public:
+  SymEngine(int nirr, const int* occ, const int* vir, hipStream_t s);
+  ~SymEngine();
+
+  // Dense C1 integral blocks (device, chemists' notation, row major, the
+  // CcsdEngine::setup conventions); only their symmetry-allowed elements are read.
+  int setup(const double* fock, const double* ovov, const double* oovv, const double* ovoo, const double* oooo,
+            const double* ovvv, const double* L233);
+  int set_amplitudes(const double* t1, const double* t2);  // dense
+  int get_amplitudes(double* t1, double* t2);              // dense
+  int start_mp1();
+  int increments(GemmPrec prec, double* inc1, double* inc2);  // dense outputs (may be null)
+  int cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st);
+  int energy_current(double& e);
+  int set_diis_order(int order);
+  int reset_diis();
+  // differentiation form (dense multipliers/densities, spatial normalisation
+  // of CcsdEngine::adjoint)
+  int adjoint(const double* l1, const double* l2, double* r1, double* r2, double* dens);
+  int lambda_start();
+  int lambda_cycle(int icycle, double tol, CcsdStats& st);
+  int set_lambda_diis_order(int order);
+  int get_lambda(double* l1, double* l2, int aces_normalisation);
+  int density(double* dens);
+  void set_profiling(bool on) { profile_ = on; }
+  void profile(double* out, int reset);
+  long device_bytes() const { return bytes_; }
+  long packed_t2() const;
+
+  int o, v, n, nirr;
+
+ private:
+  struct Tensor {
+    dpd::Layout L;
+    bool has_layout = false;
+    long nrow[dpd::kMaxIrr] = {0}, ncol[dpd::kMaxIrr] = {0}, boff[dpd::kMaxIrr + 1] = {0};
+    double* d = nullptr;
+    double* g = nullptr;
+    bool grad = false;
+  };
+  enum Kind { OP_PERM, OP_GEMM, OP_GEMV, OP_SUBDIAG, OP_PACK, OP_UNPACK };
+  struct Op {
+    int kind = OP_PERM;
+    int out = -1;
+    double beta = 0.0;
+    // permutation
+    int nsrc = 0;
+    int src[3] = {-1, -1, -1};
+    double alpha[3] = {0, 0, 0};
+    int map[3][4] = {{0, 1, 2, 3}, {0, 1, 2, 3}, {0, 1, 2, 3}};
+    int X = -1, Y = -1;
+    int xmap[2] = {0, 1}, ymap[2] = {0, 1};
+    double gamma = 0.0;
+    dpd::Plan fwd, bwd[3], hplan;
+    int H = -1;
+    // gemm / gemv: out = alpha op(A) op(B)^T + beta out ; gemv: out = alpha A0 x (trans: A0^T x)
+    int A = -1, B = -1, a_kc = 1, b_kc = 1, trans = 0;
+    double al = 1.0;
+    int cat = 3;
+    bool lowp = false;
+    // subdiag: out += al * diag(e), e = 0: eo, 1: ev
+    int evec = 0;
+  };
+  struct LowBuf {
+    void* hi = nullptr;
+    void* lo = nullptr;
+    long off[dpd::kMaxIrr + 1] = {0};
+    long ld[dpd::kMaxIrr] = {0};
+    GemmPrec prec = GP_F64;
+    bool valid = false;
+  };
+
+  int alloc(double** p, long count);
+  int tensor(const dpd::Layout& L, bool grad, double* data = nullptr, double* gdata = nullptr);
+  int block_tensor(const long* nrow, const long* ncol, bool grad);
+  int perm(int out, double beta, std::vector<std::pair<int, std::vector<int>>> srcs, std::vector<double> alpha,
+           int X = -1, std::vector<int> xmap = {}, int Y = -1, std::vector<int> ymap = {}, double gamma = 0.0,
+           bool record = true);
+  int gemm_op(int C, int A, int a_kc, int B, int b_kc, double alpha, double beta, int cat, bool lowp);
+  int gemv_op(int y, int A, int x, int trans, double alpha, double beta);
+  int ggemm(int C, const double* Cd, int A, const double* Ad, int a_kc, int B, const double* Bd, int b_kc,
+            double alpha, double beta, GemmPrec prec, int cat, const LowBuf* la, const LowBuf* lb);
+  int gemv_run(int y, double* yd, int A, const double* Ad, const double* xd, int trans, double alpha, double beta);
+  int outer0(int A, double* Ad, const double* u, const double* w, double alpha);  // A0 += alpha u w^T
+  int lowp_convert(int t, GemmPrec prec, LowBuf& lb);
+  int run_perm(Op& op, double* out, const double* const* src, const double* X, const double* Y,
+               const dpd::Denom* den = nullptr);
+  int forward(GemmPrec prec);
+  int backward(const double* lam1, const double* lam2, bool density);
+  int pack_dense(int t, const double* dense, const long* strides, long base_off);
+  int unpack_dense(int t, const double* packed, double* dense, const long* strides, long base_off);
+  int denom_apply(int t, double* out, double beta, const double* in, double alpha, const double* in2,
+                  double alpha2, int pmap, int pow);
+  int pair_symmetrize(double* t2);
+  int energy_dev(const double* t1, const double* t2, double& e);
+  int diis_core(std::vector<double*>& sol, std::vector<double*>& err, std::vector<double>& R, int order, double* x,
+                double* xn, int icycle, double tol, bool lambda_rule, CcsdStats& st);
+  int diis_grow(std::vector<double*>& sol, std::vector<double*>& err, int order);
+  int absmax_vec(const double* e, double& val);
+  int assemble_dense_density(double* dens);
+  int lam_from_aces(const double* L, double* lam);
+  int timed(int cat, double flops, double ms);
+
+  hipStream_t s_;
+  long bytes_ = 0;
+  std::vector<void*> allocs_;
+  bool ready_ = false, profile_ = false;
+  double prof_[15] = {0};
+  dpd::Space O_, V_;
+  std::vector<Tensor> T_;
+  std::vector<Op> ops_;
+  std::vector<dpd::Plan> plans_;  // auxiliary plans owned by the engine
+  std::vector<LowBuf> low_;
+  dpd::LadderDesc lad_;
+  dpd::LadderDesc* d_lad_ = nullptr;
+  dpd::Layout* d_layouts_ = nullptr;  // device copies: 0 tau/t2 (O,O|V,V), 1 Yb
+  double *eo_ = nullptr, *ev_ = nullptr, *geo_ = nullptr, *gev_ = nullptr;
+  double *scal_ = nullptr, *ws_ = nullptr, *scr_ = nullptr, *scr2_ = nullptr;
+  long ws_elems_ = 0, scr_elems_ = 0;
+  void* red_ = nullptr;
+  const double** dptr_ = nullptr;
+  double* coef_ = nullptr;
+  double* dscr_ = nullptr;
+  double* absp_ = nullptr;
+  // tensor ids
+  int t1_, t2_, fov_, foo_, fvv_, tau_, inc1_, inc2_, Lijab_, Yb_, Sp_, Dp_, Zp_, Zm_, Vp_, Vm_;
+  int pack_op_ = -1, unpack_op_ = -1, tau_op_ = -1, last_op_ = -1;
+  dpd::Plan jac1_, jac2_, sym2_, perm2_, absplan_;
+  dpd::Layout aux_o_, aux_v_, aux_vo_;
+  // amplitude state (packed [t1 | t2]) and DIIS
+  double *T_cur_ = nullptr, *T_new_ = nullptr, *err_cur_ = nullptr, *newvec_ = nullptr, *eperm_ = nullptr;
+  long nt_ = 0, n1_ = 0;
+  int order_ = 6, lorder_ = 6;
+  std::vector<double*> sol_, errv_, lsol_, lerr_;
+  std::vector<double> R_, LR_;
+  // Lambda state
+  double *Lam_ = nullptr, *Lamn_ = nullptr, *lam_ = nullptr, *gT_ = nullptr;
+  bool fwd_valid_ = false, lam_ready_ = false;
+};
+
+}  // namespace aces

@@ -1,0 +1,1571 @@
+// Closed-shell CCSD, Lambda equations and response density on irrep-blocked
+// storage (see sym_engine.h).  The equations are those of ccsd_engine.hip /
+// ccsd_adjoint.hip (same terms, same grouping); each C1 contraction becomes a
+// grouped MFMA launch over the irrep blocks, each permutation a box plan.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include "adjoint_kernels.h"
+#include "ccsd_kernels.h"
+#include "common.h"
+#include "engine_util.h"
+#include "sym_engine.h"
+#include "tensor_ops.h"
+
+namespace aces {
+
+using dpd::Layout;
+using dpd::Plan;
+using dpd::Space;
+using dpd::Src;
+
+namespace {
+
+inline long pad8(long x) { return (x + 7) / 8 * 8; }
+
+constexpr int kDotBlocks = 512;
+constexpr int kMaxDots = 32;
+
+// part[k*kDotBlocks + b] = block b's share of sum_i xs[k][i] * y[i]
+__global__ __launch_bounds__(256) void mdot_kernel(long n, int m, const double* const* __restrict__ xs,
+                                                   const double* __restrict__ y, double* __restrict__ part) {
+  __shared__ double sh[256];
+  double acc[kMaxDots];
+#pragma unroll
+  for (int k = 0; k < kMaxDots; ++k) acc[k] = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double yi = y[i];
+#pragma unroll
+    for (int k = 0; k < kMaxDots; ++k)
+      if (k < m) acc[k] += xs[k][i] * yi;
+  }
+  for (int k = 0; k < m; ++k) {
+    sh[threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(long)k * kDotBlocks + blockIdx.x] = sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ void mdot_finish_kernel(int nb, const double* __restrict__ part, double* __restrict__ out, double scale,
+                                   int accumulate) {
+  __shared__ double sh[256];
+  const int k = blockIdx.x;
+  double a = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) a += part[(long)k * kDotBlocks + b];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[k] = scale * sh[0] + (accumulate ? out[k] : 0.0);
+}
+
+__global__ void hadamard_kernel(long n, const double* __restrict__ x, const double* __restrict__ y,
+                                double* __restrict__ z) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    z[i] = x[i] * y[i];
+}
+
+// x <- (x + partner(x)) / 2 for the element pairs of a plan whose source 0 is
+// the partner map (SYMETRIZE_T2 on irrep-blocked storage).
+__global__ __launch_bounds__(256) void pair_avg_kernel(const dpd::Box* boxes, int nbox, double* __restrict__ x) {
+  __shared__ int sb;
+  if (threadIdx.x == 0) sb = dpd::box_of_chunk(boxes, nbox, blockIdx.x);
+  __syncthreads();
+  const dpd::Box& b = boxes[sb];
+  const long elems = (long)b.d[0] * b.d[1] * b.d[2] * b.d[3];
+  const long e0 = (blockIdx.x - b.first) * dpd::kChunk;
+  const long e1 = min(elems, e0 + dpd::kChunk);
+  for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int l3 = (int)(e % b.d[3]);
+    const long r2 = e / b.d[3];
+    const int l2 = (int)(r2 % b.d[2]);
+    const long r1 = r2 / b.d[2];
+    const int l1 = (int)(r1 % b.d[1]);
+    const int l0 = (int)(r1 / b.d[1]);
+    const long o1 = b.ob + l0 * b.os[0] + l1 * b.os[1] + l2 * b.os[2] + l3 * b.os[3];
+    const long o2 = b.sb[0] + l0 * b.ss[0][0] + l1 * b.ss[0][1] + l2 * b.ss[0][2] + l3 * b.ss[0][3];
+    if (o1 < o2) {
+      const double s = 0.5 * (x[o1] + x[o2]);
+      x[o1] = s;
+      x[o2] = s;
+    }
+  }
+}
+
+struct AbsM {
+  double val;
+  double a;
+  long key;
+};
+
+__device__ inline void absm_take(AbsM& best, double val, long key) {
+  const double a = fabs(val);
+  if (a > best.a || (a == best.a && key < best.key)) {
+    best.a = a;
+    best.val = val;
+    best.key = key;
+  }
+}
+
+__device__ inline AbsM absm_block(AbsM m, AbsM* sh) {
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      AbsM o = sh[threadIdx.x + w];
+      AbsM& s = sh[threadIdx.x];
+      if (o.a > s.a || (o.a == s.a && o.key < s.key)) s = o;
+    }
+    __syncthreads();
+  }
+  AbsM r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// Largest |element| of the DIIS vector [t1 | t2(AB) | t2(AA)], AA = x(ij,ab) -
+// x(ji,ab) over I<J, A<B; ties resolved by position in that order.
+__global__ __launch_bounds__(256) void absmax_t2_kernel(const dpd::Box* boxes, int nbox, Space O, Space V,
+                                                        const double* __restrict__ e2, long n1, long n2,
+                                                        AbsM* __restrict__ part) {
+  __shared__ int sb;
+  __shared__ AbsM sh[256];
+  if (threadIdx.x == 0) sb = dpd::box_of_chunk(boxes, nbox, blockIdx.x);
+  __syncthreads();
+  const dpd::Box& b = boxes[sb];
+  const long elems = (long)b.d[0] * b.d[1] * b.d[2] * b.d[3];
+  const long e0 = (blockIdx.x - b.first) * dpd::kChunk;
+  const long e1 = min(elems, e0 + dpd::kChunk);
+  AbsM best{0.0, -1.0, 1L << 62};
+  for (long e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int l3 = (int)(e % b.d[3]);
+    const long r2 = e / b.d[3];
+    const int l2 = (int)(r2 % b.d[2]);
+    const long r1 = r2 / b.d[2];
+    const int l1 = (int)(r1 % b.d[1]);
+    const int l0 = (int)(r1 / b.d[1]);
+    const long o1 = b.ob + l0 * b.os[0] + l1 * b.os[1] + l2 * b.os[2] + l3 * b.os[3];
+    const double x = e2[o1];
+    absm_take(best, x, n1 + o1);
+    const int I = O.off[b.g[0]] + l0, J = O.off[b.g[1]] + l1, A = V.off[b.g[2]] + l2, B = V.off[b.g[3]] + l3;
+    if (I < J && A < B) {
+      const long o2 = b.sb[0] + l0 * b.ss[0][0] + l1 * b.ss[0][1] + l2 * b.ss[0][2] + l3 * b.ss[0][3];
+      absm_take(best, x - e2[o2], n1 + n2 + o1);
+    }
+  }
+  AbsM r = absm_block(best, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(256) void absmax_flat_kernel(long n, const double* __restrict__ x, AbsM* __restrict__ part) {
+  __shared__ AbsM sh[256];
+  AbsM best{0.0, -1.0, 1L << 62};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    absm_take(best, x[i], i);
+  AbsM r = absm_block(best, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(256) void absmax_finish_kernel(int nb, const AbsM* __restrict__ part, double* out) {
+  __shared__ AbsM sh[256];
+  AbsM best{0.0, -1.0, 1L << 62};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    const AbsM& p = part[b];
+    if (p.a > best.a || (p.a == best.a && p.key < best.key)) best = p;
+  }
+  AbsM r = absm_block(best, sh);
+  if (threadIdx.x == 0) out[0] = r.val;
+}
+
+}  // namespace
+
+// ===========================================================================
+SymEngine::SymEngine(int nirr_, const int* occ, const int* vir, hipStream_t s) : nirr(nirr_), s_(s) {
+  O_ = Space::make(nirr, occ);
+  V_ = Space::make(nirr, vir);
+  o = O_.n;
+  v = V_.n;
+  n = o + v;
+}
+
+SymEngine::~SymEngine() {
+  for (Op& op : ops_) {
+    dpd::free_plan(op.fwd);
+    for (auto& p : op.bwd) dpd::free_plan(p);
+    dpd::free_plan(op.hplan);
+  }
+  for (Plan* p : {&jac1_, &jac2_, &sym2_, &perm2_, &absplan_}) dpd::free_plan(*p);
+  for (Plan& p : plans_) dpd::free_plan(p);
+  for (void* p : allocs_) (void)hipFree(p);
+}
+
+int SymEngine::alloc(double** p, long count) {
+  if (count < 1) count = 1;
+  hipError_t e = hipMalloc((void**)p, (size_t)count * sizeof(double));
+  if (e != hipSuccess) {
+    char b[256];
+    snprintf(b, sizeof(b), "hipMalloc of %ld doubles failed (%s); engine holds %ld bytes", count,
+             hipGetErrorString(e), bytes_);
+    set_error(b);
+    return ACES_ERR_NOMEM;
+  }
+  allocs_.push_back(*p);
+  bytes_ += count * (long)sizeof(double);
+  return ACES_OK;
+}
+
+int SymEngine::tensor(const Layout& L, bool grad, double* data, double* gdata) {
+  Tensor t;
+  t.L = L;
+  t.has_layout = true;
+  for (int g = 0; g < nirr; ++g) {
+    t.nrow[g] = L.nrow[g];
+    t.ncol[g] = L.ncol[g];
+    t.boff[g] = L.blk_off[g];
+  }
+  t.boff[nirr] = L.size();
+  t.grad = grad;
+  if (data)
+    t.d = data;
+  else if (alloc(&t.d, L.size()) != ACES_OK)
+    return -1;
+  if (grad) {
+    if (gdata)
+      t.g = gdata;
+    else if (alloc(&t.g, L.size()) != ACES_OK)
+      return -1;
+  }
+  T_.push_back(t);
+  return (int)T_.size() - 1;
+}
+
+int SymEngine::block_tensor(const long* nrow, const long* ncol, bool grad) {
+  Tensor t;
+  long off = 0;
+  for (int g = 0; g < nirr; ++g) {
+    t.nrow[g] = nrow[g];
+    t.ncol[g] = ncol[g];
+    t.boff[g] = off;
+    off += nrow[g] * ncol[g];
+  }
+  t.boff[nirr] = off;
+  t.grad = grad;
+  if (alloc(&t.d, off) != ACES_OK) return -1;
+  if (grad && alloc(&t.g, off) != ACES_OK) return -1;
+  T_.push_back(t);
+  return (int)T_.size() - 1;
+}
+
+long SymEngine::packed_t2() const { return nt_ - n1_; }
+
+// ---------------------------------------------------------------------------
+// Operation recording
+int SymEngine::perm(int out, double beta, std::vector<std::pair<int, std::vector<int>>> srcs,
+                    std::vector<double> alpha, int X, std::vector<int> xmap, int Y, std::vector<int> ymap,
+                    double gamma, bool record) {
+  ACES_REQUIRE(out >= 0 && srcs.size() <= 3 && alpha.size() == srcs.size(), "sym perm: bad arguments");
+  Op op;
+  op.kind = OP_PERM;
+  op.out = out;
+  op.beta = beta;
+  op.nsrc = (int)srcs.size();
+  std::vector<Src> ss;
+  for (int s = 0; s < op.nsrc; ++s) {
+    ACES_REQUIRE(srcs[s].first >= 0 && (int)srcs[s].second.size() == T_[srcs[s].first].L.rank,
+                 "sym perm: source map rank");
+    op.src[s] = srcs[s].first;
+    op.alpha[s] = alpha[s];
+    Src sr;
+    sr.L = &T_[op.src[s]].L;
+    for (int m = 0; m < sr.L->rank; ++m) sr.map[m] = op.map[s][m] = srcs[s].second[m];
+    ss.push_back(sr);
+  }
+  Src sx, sy;
+  if (X >= 0) {
+    op.X = X;
+    op.Y = Y;
+    op.gamma = gamma;
+    sx.L = &T_[X].L;
+    sy.L = &T_[Y].L;
+    for (int m = 0; m < 2; ++m) {
+      sx.map[m] = op.xmap[m] = xmap[m];
+      sy.map[m] = op.ymap[m] = ymap[m];
+    }
+  }
+  ACES_CHECK(dpd::build_plan(op.fwd, T_[out].L, ss, X >= 0 ? &sx : nullptr, X >= 0 ? &sy : nullptr));
+  if (!record) {
+    const double* sp[3] = {nullptr, nullptr, nullptr};
+    for (int s = 0; s < op.nsrc; ++s) sp[s] = T_[op.src[s]].d;
+    ACES_CHECK(run_perm(op, T_[out].d, sp, X >= 0 ? T_[X].d : nullptr, X >= 0 ? T_[Y].d : nullptr));
+    ACES_HIP(hipStreamSynchronize(s_));
+    dpd::free_plan(op.fwd);
+    return ACES_OK;
+  }
+  ACES_REQUIRE(beta == 0.0 || beta == 1.0, "sym perm: recorded beta must be 0 or 1");
+  bool g = false;
+  const int rank = T_[out].L.rank;
+  for (int s = 0; s < op.nsrc; ++s) {
+    if (!T_[op.src[s]].grad) continue;
+    g = true;
+    // g_src += alpha * g_out at the inverse index map
+    Src inv;
+    inv.L = &T_[out].L;
+    for (int k = 0; k < rank; ++k) {
+      int mm = -1;
+      for (int m = 0; m < rank; ++m)
+        if (op.map[s][m] == k) mm = m;
+      ACES_REQUIRE(mm >= 0, "sym perm: recorded source map must be a permutation");
+      inv.map[k] = mm;
+    }
+    ACES_CHECK(dpd::build_plan(op.bwd[s], T_[op.src[s]].L, {inv}, nullptr, nullptr));
+  }
+  if (X >= 0 && (T_[X].grad || T_[Y].grad)) {
+    g = true;
+    const Layout& Lo = T_[out].L;
+    Layout H = dpd::make_layout(4, 2, &Lo.sp[op.xmap[0]], &Lo.sp[op.xmap[1]], &Lo.sp[op.ymap[0]],
+                                &Lo.sp[op.ymap[1]]);
+    op.H = tensor(H, false, scr_);
+    ACES_REQUIRE(op.H >= 0, "sym perm: scratch tensor");
+    ACES_REQUIRE(H.size() <= scr_elems_, "sym perm: scratch too small");
+    Src so;
+    so.L = &T_[out].L;
+    const int pos[4] = {op.xmap[0], op.xmap[1], op.ymap[0], op.ymap[1]};
+    for (int k = 0; k < 4; ++k)
+      for (int m = 0; m < 4; ++m)
+        if (pos[m] == k) so.map[k] = m;
+    ACES_CHECK(dpd::build_plan(op.hplan, T_[op.H].L, {so}, nullptr, nullptr));
+  }
+  T_[out].grad = T_[out].grad || g;
+  if (T_[out].grad && T_[out].g == nullptr) ACES_CHECK(alloc(&T_[out].g, T_[out].boff[nirr]));
+  ops_.push_back(op);
+  last_op_ = (int)ops_.size() - 1;
+  return ACES_OK;
+}
+
+static bool same_spaces(const Space* a, const Space* b, int k, int nirr) {
+  for (int i = 0; i < k; ++i)
+    for (int g = 0; g < nirr; ++g)
+      if (a[i].cnt[g] != b[i].cnt[g]) return false;
+  return true;
+}
+
+int SymEngine::gemm_op(int C, int A, int a_kc, int B, int b_kc, double alpha, double beta, int cat, bool lowp) {
+  const Tensor &tc = T_[C], &ta = T_[A], &tb = T_[B];
+  for (int g = 0; g < nirr; ++g) {
+    const long M = tc.nrow[g], N = tc.ncol[g];
+    const long Ma = a_kc ? ta.nrow[g] : ta.ncol[g], K = a_kc ? ta.ncol[g] : ta.nrow[g];
+    const long Nb = b_kc ? tb.nrow[g] : tb.ncol[g], Kb = b_kc ? tb.ncol[g] : tb.nrow[g];
+    if (M != Ma || N != Nb || K != Kb) {
+      char b[200];
+      snprintf(b, sizeof(b), "sym gemm %d: block %d shapes C %ldx%ld A %ldx%ld B %ldx%ld", (int)ops_.size(), g, M,
+               N, Ma, K, Nb, Kb);
+      set_error(b);
+      return ACES_ERR_ARG;
+    }
+  }
+  if (tc.has_layout && ta.has_layout && tb.has_layout) {
+    // compound orders must agree: rows of C = m-indices of A, columns of C =
+    // n-indices of B, contraction = k-indices of A and B
+    const Layout &Lc = tc.L, &La = ta.L, &Lb = tb.L;
+    const int cs = (Lc.rank == 2) ? 1 : Lc.split, cr = Lc.rank;
+    const int as = (La.rank == 2) ? 1 : La.split, bs = (Lb.rank == 2) ? 1 : Lb.split;
+    const Space* am = a_kc ? La.sp : La.sp + as;
+    const Space* ak = a_kc ? La.sp + as : La.sp;
+    const Space* bn = b_kc ? Lb.sp : Lb.sp + bs;
+    const Space* bk = b_kc ? Lb.sp + bs : Lb.sp;
+    const int nak = a_kc ? La.rank - as : as;
+    const int nbk = b_kc ? Lb.rank - bs : bs;
+    bool ok = same_spaces(Lc.sp, am, cs, nirr) && same_spaces(Lc.sp + cs, bn, cr - cs, nirr) && nak == nbk &&
+              same_spaces(ak, bk, nak, nirr);
+    ACES_REQUIRE(ok, "sym gemm: compound index orders of C, A and B disagree");
+  }
+  Op op;
+  op.kind = OP_GEMM;
+  op.out = C;
+  op.A = A;
+  op.B = B;
+  op.a_kc = a_kc;
+  op.b_kc = b_kc;
+  op.al = alpha;
+  op.beta = beta;
+  op.cat = cat;
+  op.lowp = lowp;
+  ACES_REQUIRE(beta == 0.0 || beta == 1.0, "sym gemm: recorded beta must be 0 or 1");
+  T_[C].grad = T_[C].grad || ta.grad || tb.grad;
+  if (T_[C].grad && T_[C].g == nullptr) ACES_CHECK(alloc(&T_[C].g, T_[C].boff[nirr]));
+  ops_.push_back(op);
+  last_op_ = (int)ops_.size() - 1;
+  return ACES_OK;
+}
+
+int SymEngine::gemv_op(int y, int A, int x, int trans, double alpha, double beta) {
+  const long M0 = T_[A].nrow[0], K0 = T_[A].ncol[0];
+  const long ny = T_[y].boff[nirr], nx = T_[x].boff[nirr];
+  ACES_REQUIRE(trans ? (ny == K0 && nx == M0) : (ny == M0 && nx == K0), "sym gemv: vector lengths");
+  Op op;
+  op.kind = OP_GEMV;
+  op.out = y;
+  op.A = A;
+  op.B = x;
+  op.trans = trans;
+  op.al = alpha;
+  op.beta = beta;
+  T_[y].grad = T_[y].grad || T_[A].grad || T_[x].grad;
+  if (T_[y].grad && T_[y].g == nullptr) ACES_CHECK(alloc(&T_[y].g, T_[y].boff[nirr]));
+  ops_.push_back(op);
+  last_op_ = (int)ops_.size() - 1;
+  return ACES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Execution helpers
+int SymEngine::run_perm(Op& op, double* out, const double* const* src, const double* X, const double* Y,
+                        const dpd::Denom* den) {
+  return dpd::apply_plan(op.fwd, out, op.beta, src, op.alpha, X, Y, op.gamma, den, s_);
+}
+
+int SymEngine::timed(int cat, double flops, double ms) {
+  prof_[3 * cat + 0] += ms;
+  prof_[3 * cat + 1] += flops;
+  prof_[3 * cat + 2] += 1.0;
+  return ACES_OK;
+}
+
+void SymEngine::profile(double* out, int reset) {
+  for (int i = 0; i < 15; ++i) out[i] = prof_[i];
+  if (reset)
+    for (int i = 0; i < 15; ++i) prof_[i] = 0.0;
+}
+
+int SymEngine::lowp_convert(int t, GemmPrec prec, LowBuf& lb) {
+  const Tensor& tt = T_[t];
+  if (lb.hi == nullptr) {
+    long off = 0;
+    for (int g = 0; g < nirr; ++g) {
+      lb.ld[g] = pad8(std::max<long>(tt.ncol[g], 1));
+      lb.off[g] = off;
+      off += tt.nrow[g] * lb.ld[g];
+    }
+    lb.off[nirr] = off;
+    double* p;
+    ACES_CHECK(alloc(&p, (off * 4 + 7) / 8 + 8));  // 4 bytes per element (fp32, or bf16 hi + lo)
+    lb.hi = p;
+    lb.lo = (char*)p + off * 2;
+  }
+  for (int g = 0; g < nirr; ++g) {
+    if (tt.nrow[g] == 0 || tt.ncol[g] == 0) continue;
+    if (prec == GP_F32)
+      ACES_CHECK(convert_f32((int)tt.nrow[g], (int)tt.ncol[g], tt.d + tt.boff[g], tt.ncol[g],
+                             (float*)lb.hi + lb.off[g], lb.ld[g], s_));
+    else
+      ACES_CHECK(split_bf16((int)tt.nrow[g], (int)tt.ncol[g], tt.d + tt.boff[g], tt.ncol[g],
+                            (unsigned short*)lb.hi + lb.off[g], (unsigned short*)lb.lo + lb.off[g], lb.ld[g], s_));
+  }
+  lb.prec = prec;
+  lb.valid = true;
+  return ACES_OK;
+}
+
+int SymEngine::ggemm(int C, const double* Cd, int A, const double* Ad, int a_kc, int B, const double* Bd, int b_kc,
+                     double alpha, double beta, GemmPrec prec, int cat, const LowBuf* la, const LowBuf* lb) {
+  const Tensor &tc = T_[C], &ta = T_[A], &tb = T_[B];
+  GemmDesc d[dpd::kMaxIrr];
+  int ng = 0;
+  double flops = 0.0;
+  std::vector<GemmDesc> kzero;
+  for (int g = 0; g < nirr; ++g) {
+    GemmDesc& e = d[ng];
+    e = GemmDesc();
+    const long M = tc.nrow[g], N = tc.ncol[g];
+    const long K = a_kc ? ta.ncol[g] : ta.nrow[g];
+    e.M = (int)M;
+    e.N = (int)N;
+    e.C = const_cast<double*>(Cd) + tc.boff[g];
+    e.ldc = std::max<long>(N, 1);
+    e.alpha = alpha;
+    e.beta = beta;
+    e.a_kc = a_kc;
+    e.b_kc = b_kc;
+    e.ws = ws_;
+    e.ws_elems = ws_elems_;
+    if (M == 0 || N == 0) {
+      e.M = 0;
+      e.K = (int)K;
+      e.A = Ad;
+      e.B = Bd;
+      e.lda = std::max<long>(a_kc ? K : M, 1);
+      e.ldb = std::max<long>(b_kc ? K : N, 1);
+      ++ng;
+      continue;
+    }
+    if (K == 0) {
+      // C = beta C for this block (separate launch)
+      e.K = 0;
+      e.A = Ad;
+      e.B = Bd;
+      e.lda = std::max<long>(a_kc ? 1 : M, 1);
+      e.ldb = std::max<long>(b_kc ? 1 : N, 1);
+      if (beta != 1.0) kzero.push_back(e);
+      e.M = 0;
+      ++ng;
+      continue;
+    }
+    flops += 2.0 * M * N * K;
+    if (prec == GP_F64 || la == nullptr) {
+      e.K = (int)K;
+      e.A = Ad + ta.boff[g];
+      e.lda = ta.ncol[g];
+      e.B = Bd + tb.boff[g];
+      e.ldb = tb.ncol[g];
+    } else {
+      const int esz = (prec == GP_F32) ? 4 : 2;
+      e.K = (prec == GP_BF16X3) ? (int)la->ld[g] : (int)K;
+      e.A = (const char*)la->hi + la->off[g] * esz;
+      e.A_lo = (prec == GP_BF16X3) ? (const char*)la->lo + la->off[g] * 2 : nullptr;
+      e.lda = la->ld[g];
+      e.B = (const char*)lb->hi + lb->off[g] * esz;
+      e.B_lo = (prec == GP_BF16X3) ? (const char*)lb->lo + lb->off[g] * 2 : nullptr;
+      e.ldb = lb->ld[g];
+    }
+    ++ng;
+  }
+  hipEvent_t ev[2];
+  if (profile_) {
+    ACES_HIP(hipEventCreate(&ev[0]));
+    ACES_HIP(hipEventCreate(&ev[1]));
+    ACES_HIP(hipEventRecord(ev[0], s_));
+  }
+  // a grouped launch needs every group's K > 0 (empty groups carry M = 0)
+  for (int g = 0; g < ng; ++g)
+    if (d[g].M == 0) d[g].K = std::max(d[g].K, 1);
+  ACES_CHECK(gemm_grouped(d, ng, (la && prec != GP_F64) ? prec : GP_F64, s_));
+  for (GemmDesc& e : kzero) ACES_CHECK(gemm(e, GP_F64, s_));
+  if (profile_) {
+    ACES_HIP(hipEventRecord(ev[1], s_));
+    ACES_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    ACES_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    timed(cat, flops, ms);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+  }
+  return ACES_OK;
+}
+
+int SymEngine::gemv_run(int y, double* yd, int A, const double* Ad, const double* xd, int trans, double alpha,
+                        double beta) {
+  const long M0 = T_[A].nrow[0], K0 = T_[A].ncol[0];
+  if (M0 == 0 || K0 == 0) {
+    if (beta != 1.0) {
+      const long ny = T_[y].boff[nirr];
+      if (ny) ACES_CHECK(eu::P4(1, 1, 1, ny, yd, beta).run(s_));
+    }
+    return ACES_OK;
+  }
+  GemmDesc e;
+  e.N = 1;
+  e.C = yd;
+  e.ldc = 1;
+  e.alpha = alpha;
+  e.beta = beta;
+  e.ws = ws_;
+  e.ws_elems = ws_elems_;
+  e.B = xd;
+  e.b_kc = 1;
+  if (!trans) {
+    e.M = (int)M0;
+    e.K = (int)K0;
+    e.A = Ad + T_[A].boff[0];
+    e.lda = K0;
+    e.a_kc = 1;
+    e.ldb = K0;
+  } else {
+    e.M = (int)K0;
+    e.K = (int)M0;
+    e.A = Ad + T_[A].boff[0];
+    e.lda = K0;
+    e.a_kc = 0;
+    e.ldb = M0;
+  }
+  return gemm(e, GP_F64, s_);
+}
+
+int SymEngine::outer0(int A, double* Ad, const double* u, const double* w, double alpha) {
+  const long M0 = T_[A].nrow[0], K0 = T_[A].ncol[0];
+  if (M0 == 0 || K0 == 0) return ACES_OK;
+  GemmDesc e;
+  e.M = (int)M0;
+  e.N = (int)K0;
+  e.K = 1;
+  e.A = u;
+  e.lda = 1;
+  e.a_kc = 1;
+  e.B = w;
+  e.ldb = 1;
+  e.b_kc = 1;
+  e.C = Ad + T_[A].boff[0];
+  e.ldc = K0;
+  e.alpha = alpha;
+  e.beta = 1.0;
+  return gemm(e, GP_F64, s_);
+}
+
+// ---------------------------------------------------------------------------
+int SymEngine::forward(GemmPrec prec) {
+  for (Op& op : ops_) {
+    switch (op.kind) {
+      case OP_PERM: {
+        const double* sp[3] = {nullptr, nullptr, nullptr};
+        for (int s = 0; s < op.nsrc; ++s) sp[s] = T_[op.src[s]].d;
+        ACES_CHECK(run_perm(op, T_[op.out].d, sp, op.X >= 0 ? T_[op.X].d : nullptr,
+                            op.X >= 0 ? T_[op.Y].d : nullptr));
+        break;
+      }
+      case OP_GEMM: {
+        if (op.lowp && prec != GP_F64) {
+          for (int t : {op.A, op.B}) {
+            LowBuf& lb = low_[t];
+            if (T_[t].grad || !lb.valid || lb.prec != prec) ACES_CHECK(lowp_convert(t, prec, lb));
+          }
+          ACES_CHECK(ggemm(op.out, T_[op.out].d, op.A, T_[op.A].d, op.a_kc, op.B, T_[op.B].d, op.b_kc, op.al,
+                           op.beta, prec, op.cat, &low_[op.A], &low_[op.B]));
+        } else {
+          ACES_CHECK(ggemm(op.out, T_[op.out].d, op.A, T_[op.A].d, op.a_kc, op.B, T_[op.B].d, op.b_kc, op.al,
+                           op.beta, GP_F64, op.cat, nullptr, nullptr));
+        }
+        break;
+      }
+      case OP_GEMV:
+        ACES_CHECK(gemv_run(op.out, T_[op.out].d, op.A, T_[op.A].d, T_[op.B].d, op.trans, op.al, op.beta));
+        break;
+      case OP_SUBDIAG:
+        ACES_CHECK(dpd::sub_diag_blocks(T_[op.out].L, T_[op.out].d, op.evec ? ev_ : eo_, op.al, s_));
+        break;
+      case OP_PACK:
+        ACES_CHECK(dpd::pack_tau_sym(d_lad_, lad_, d_layouts_, T_[tau_].d, T_[Sp_].d, T_[Dp_].d, s_));
+        break;
+      case OP_UNPACK:
+        ACES_CHECK(dpd::unpack_ladder_sym(d_lad_, op.fwd, op.hplan, T_[Zp_].d, T_[Zm_].d, T_[inc2_].d,
+                                          T_[Yb_].d, s_));
+        break;
+    }
+  }
+  fwd_valid_ = (prec == GP_F64);
+  return ACES_OK;
+}
+
+// Reverse pass: seeds of L = E + lam.(inc - D*T), then every recorded
+// operation's adjoint in reverse order.  Gradients accumulate in Tensor::g.
+int SymEngine::backward(const double* lam1, const double* lam2, bool density) {
+  hipStream_t s = s_;
+  for (Tensor& t : T_)
+    if (t.g && t.boff[nirr] > 0) ACES_HIP(hipMemsetAsync(t.g, 0, sizeof(double) * t.boff[nirr], s));
+  ACES_HIP(hipMemsetAsync(geo_, 0, sizeof(double) * o, s));
+  ACES_HIP(hipMemsetAsync(gev_, 0, sizeof(double) * v, s));
+  const long n1 = T_[t1_].boff[nirr], n2 = T_[t2_].boff[nirr];
+  // Omega = inc - D*T:  g_inc = lam ; g_T -= D*lam
+  ACES_HIP(hipMemcpyAsync(T_[inc1_].g, lam1, sizeof(double) * n1, hipMemcpyDeviceToDevice, s));
+  ACES_HIP(hipMemcpyAsync(T_[inc2_].g, lam2, sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
+  ACES_CHECK(denom_apply(t1_, T_[t1_].g, 1.0, lam1, -1.0, nullptr, 0.0, 0, 1));
+  ACES_CHECK(denom_apply(t2_, T_[t2_].g, 1.0, lam2, -1.0, nullptr, 0.0, 0, 1));
+  // E = 2 f_ia t_ia + L_ijab tau_ijab
+  ACES_CHECK(eu::P4(1, 1, 1, n1, T_[t1_].g, 1.0).src(T_[fov_].d, 2.0, 0, 0, 0, 1).run(s));
+  ACES_CHECK(eu::P4(1, 1, 1, n1, T_[fov_].g, 1.0).src(T_[t1_].d, 2.0, 0, 0, 0, 1).run(s));
+  ACES_CHECK(eu::P4(1, 1, 1, n2, T_[tau_].g, 1.0).src(T_[Lijab_].d, 1.0, 0, 0, 0, 1).run(s));
+  if (density) {
+    // orbital-energy derivatives of -lam.(D*T): rows sums of lam o T over each index
+    double* h = scr2_;
+    hipLaunchKernelGGL(hadamard_kernel, dim3(1024), dim3(256), 0, s, n2, lam2, T_[t2_].d, h);
+    hipLaunchKernelGGL(hadamard_kernel, dim3(64), dim3(256), 0, s, n1, lam1, T_[t1_].d, h + n2);
+    ACES_HIP(hipGetLastError());
+    // plans_[0..3] gather lam2 o t2 with the differentiated index first:
+    // (O|O,V,V) by i, by j; (V|O,O,V) by a, by b.  plans_[4]: (V|O) from lam1 o t1.
+    const double* src[1] = {h};
+    const double one[1] = {1.0};
+    double* q = scr_;
+    for (int k = 0; k < 4; ++k) {
+      ACES_CHECK(dpd::apply_plan(plans_[k], q, 0.0, src, one, nullptr, nullptr, 0.0, nullptr, s));
+      ACES_CHECK(dpd::block_rowsum(k < 2 ? aux_o_ : aux_v_, q, k < 2 ? geo_ : gev_, k < 2 ? -1.0 : 1.0, 1.0, s));
+    }
+    const double* src1[1] = {h + n2};
+    ACES_CHECK(dpd::block_rowsum(T_[t1_].L, h + n2, geo_, -1.0, 1.0, s));
+    ACES_CHECK(dpd::apply_plan(plans_[4], q, 0.0, src1, one, nullptr, nullptr, 0.0, nullptr, s));
+    ACES_CHECK(dpd::block_rowsum(aux_vo_, q, gev_, 1.0, 1.0, s));
+  }
+  for (int i = (int)ops_.size() - 1; i >= 0; --i) {
+    Op& op = ops_[i];
+    Tensor& to = T_[op.out];
+    if (!to.grad) continue;
+    switch (op.kind) {
+      case OP_PERM: {
+        const double* go[1] = {to.g};
+        for (int k = 0; k < op.nsrc; ++k) {
+          if (!T_[op.src[k]].grad) continue;
+          const double al[1] = {op.alpha[k]};
+          ACES_CHECK(dpd::apply_plan(op.bwd[k], T_[op.src[k]].g, 1.0, go, al, nullptr, nullptr, 0.0, nullptr, s));
+        }
+        if (op.H >= 0) {
+          const double one[1] = {1.0};
+          ACES_CHECK(dpd::apply_plan(op.hplan, scr_, 0.0, go, one, nullptr, nullptr, 0.0, nullptr, s));
+          if (T_[op.X].grad) ACES_CHECK(gemv_run(op.X, T_[op.X].g, op.H, scr_, T_[op.Y].d, 0, op.gamma, 1.0));
+          if (T_[op.Y].grad) ACES_CHECK(gemv_run(op.Y, T_[op.Y].g, op.H, scr_, T_[op.X].d, 1, op.gamma, 1.0));
+        }
+        break;
+      }
+      case OP_GEMM: {
+        const int A = op.A, B = op.B, C = op.out;
+        if (T_[A].grad) {
+          if (op.a_kc)
+            ACES_CHECK(ggemm(A, T_[A].g, C, to.g, 1, B, T_[B].d, !op.b_kc, op.al, 1.0, GP_F64, 4, nullptr, nullptr));
+          else
+            ACES_CHECK(ggemm(A, T_[A].g, B, T_[B].d, !op.b_kc, C, to.g, 1, op.al, 1.0, GP_F64, 4, nullptr, nullptr));
+        }
+        if (T_[B].grad) {
+          if (op.b_kc)
+            ACES_CHECK(ggemm(B, T_[B].g, C, to.g, 0, A, T_[A].d, !op.a_kc, op.al, 1.0, GP_F64, 4, nullptr, nullptr));
+          else
+            ACES_CHECK(ggemm(B, T_[B].g, A, T_[A].d, !op.a_kc, C, to.g, 0, op.al, 1.0, GP_F64, 4, nullptr, nullptr));
+        }
+        break;
+      }
+      case OP_GEMV: {
+        const int A = op.A, x = op.B;
+        if (!op.trans) {
+          if (T_[A].grad) ACES_CHECK(outer0(A, T_[A].g, to.g, T_[x].d, op.al));
+          if (T_[x].grad) ACES_CHECK(gemv_run(x, T_[x].g, A, T_[A].d, to.g, 1, op.al, 1.0));
+        } else {
+          if (T_[A].grad) ACES_CHECK(outer0(A, T_[A].g, T_[x].d, to.g, op.al));
+          if (T_[x].grad) ACES_CHECK(gemv_run(x, T_[x].g, A, T_[A].d, to.g, 0, op.al, 1.0));
+        }
+        break;
+      }
+      case OP_SUBDIAG:
+        if (density) ACES_CHECK(dpd::diag_to_vec(to.L, to.g, op.evec ? gev_ : geo_, op.al, s));
+        break;
+      case OP_PACK:
+        ACES_CHECK(dpd::pack_tau_sym_adjoint(d_lad_, op.fwd, T_[Sp_].g, T_[Dp_].g, T_[tau_].g, s));
+        break;
+      case OP_UNPACK:
+        ACES_CHECK(dpd::unpack_ladder_sym_adjoint(d_lad_, lad_, d_layouts_, d_layouts_ + 1, T_[inc2_].g,
+                                                  T_[Yb_].g, T_[Zp_].g, T_[Zm_].g, s));
+        break;
+    }
+  }
+  return ACES_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Dense <-> irrep-blocked transfers and element-wise helpers
+int SymEngine::pack_dense(int t, const double* dense, const long* strides, long base_off) {
+  const Layout& L = T_[t].L;
+  const Space* sp[4] = {&L.sp[0], &L.sp[1], L.rank > 2 ? &L.sp[2] : nullptr, L.rank > 2 ? &L.sp[3] : nullptr};
+  Layout D = dpd::make_dense(L.rank, sp, strides);
+  Plan p;
+  Src sr;
+  sr.L = &D;
+  ACES_CHECK(dpd::build_plan(p, L, {sr}, nullptr, nullptr));
+  const double* src[1] = {dense + base_off};
+  const double one[1] = {1.0};
+  const int rc = dpd::apply_plan(p, T_[t].d, 0.0, src, one, nullptr, nullptr, 0.0, nullptr, s_);
+  ACES_HIP(hipStreamSynchronize(s_));
+  dpd::free_plan(p);
+  return rc;
+}
+
+int SymEngine::unpack_dense(int t, const double* packed, double* dense, const long* strides, long base_off) {
+  const Layout& L = T_[t].L;
+  const Space* sp[4] = {&L.sp[0], &L.sp[1], L.rank > 2 ? &L.sp[2] : nullptr, L.rank > 2 ? &L.sp[3] : nullptr};
+  Layout D = dpd::make_dense(L.rank, sp, strides);
+  Plan p;
+  Src sr;
+  sr.L = &L;
+  ACES_CHECK(dpd::build_plan(p, D, {sr}, nullptr, nullptr));
+  const double* src[1] = {packed};
+  const double one[1] = {1.0};
+  const int rc = dpd::apply_plan(p, dense + base_off, 0.0, src, one, nullptr, nullptr, 0.0, nullptr, s_);
+  ACES_HIP(hipStreamSynchronize(s_));
+  dpd::free_plan(p);
+  return rc;
+}
+
+// out = beta*out + D^pow * (alpha*in + alpha2*in2(i<->j)) on the layout of t
+// (t1: D = e_i - e_a; t2: D = e_i + e_j - e_a - e_b).
+int SymEngine::denom_apply(int t, double* out, double beta, const double* in, double alpha, const double* in2,
+                           double alpha2, int pmap, int pow) {
+  const bool two = (t == t2_);
+  dpd::Denom den;
+  den.pow = pow;
+  if (two) {
+    const int sg[4] = {1, 1, -1, -1};
+    const double* e[4] = {eo_, eo_, ev_, ev_};
+    const Space* sp[4] = {&O_, &O_, &V_, &V_};
+    for (int k = 0; k < 4; ++k) {
+      den.sign[k] = sg[k];
+      den.e[k] = e[k];
+      for (int g = 0; g < nirr; ++g) den.off[k][g] = sp[k]->off[g];
+    }
+  } else {
+    den.sign[0] = 1;
+    den.sign[1] = -1;
+    den.e[0] = eo_;
+    den.e[1] = ev_;
+    for (int g = 0; g < nirr; ++g) {
+      den.off[0][g] = O_.off[g];
+      den.off[1][g] = V_.off[g];
+    }
+  }
+  const double* src[2] = {in, in2};
+  const double al[2] = {alpha, alpha2};
+  const Plan& p = two ? (pmap ? perm2_ : jac2_) : jac1_;
+  return dpd::apply_plan(p, out, beta, src, al, nullptr, nullptr, 0.0, pow ? &den : nullptr, s_);
+}
+
+int SymEngine::pair_symmetrize(double* t2) {
+  if (sym2_.total == 0) return ACES_OK;
+  hipLaunchKernelGGL(pair_avg_kernel, dim3((unsigned)sym2_.total), dim3(256), 0, s_, sym2_.dev, sym2_.nbox, t2);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+int SymEngine::absmax_vec(const double* e, double& val) {
+  AbsM* part = (AbsM*)absp_;
+  const int nb1 = 64;
+  hipLaunchKernelGGL(absmax_flat_kernel, dim3(nb1), dim3(256), 0, s_, n1_, e, part);
+  const int nb2 = (int)absplan_.total;
+  if (nb2 > 0)
+    hipLaunchKernelGGL(absmax_t2_kernel, dim3(nb2), dim3(256), 0, s_, absplan_.dev, absplan_.nbox, O_, V_, e + n1_,
+                       n1_, nt_ - n1_, part + nb1);
+  hipLaunchKernelGGL(absmax_finish_kernel, dim3(1), dim3(256), 0, s_, nb1 + nb2, part, scal_ + 1);
+  ACES_HIP(hipGetLastError());
+  ACES_HIP(hipMemcpyAsync(&val, scal_ + 1, sizeof(double), hipMemcpyDeviceToHost, s_));
+  ACES_HIP(hipStreamSynchronize(s_));
+  return ACES_OK;
+}
+
+// E = 2 sum f_ia t_ia + sum L_ijab (t_ijab + t_ia t_jb)
+int SymEngine::energy_dev(const double* t1, const double* t2, double& e) {
+  Op& op = ops_[tau_op_];
+  const double* src[1] = {t2};
+  double* tau = scr2_;
+  ACES_CHECK(run_perm(op, tau, src, t1, t1));
+  const double* xs[1] = {T_[Lijab_].d};
+  ACES_HIP(hipMemcpyAsync(dptr_, xs, sizeof(double*), hipMemcpyHostToDevice, s_));
+  const long n2 = nt_ - n1_;
+  hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks), dim3(256), 0, s_, n2, 1, (const double* const*)dptr_, tau,
+                     (double*)red_);
+  hipLaunchKernelGGL(mdot_finish_kernel, dim3(1), dim3(256), 0, s_, kDotBlocks, (double*)red_, scal_ + 2, 1.0, 0);
+  const double* xf[1] = {T_[fov_].d};
+  ACES_HIP(hipMemcpyAsync(dptr_ + 1, xf, sizeof(double*), hipMemcpyHostToDevice, s_));
+  hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks), dim3(256), 0, s_, n1_, 1, (const double* const*)(dptr_ + 1), t1,
+                     (double*)red_);
+  hipLaunchKernelGGL(mdot_finish_kernel, dim3(1), dim3(256), 0, s_, kDotBlocks, (double*)red_, scal_ + 2, 2.0, 1);
+  ACES_HIP(hipGetLastError());
+  ACES_HIP(hipMemcpyAsync(&e, scal_ + 2, sizeof(double), hipMemcpyDeviceToHost, s_));
+  ACES_HIP(hipStreamSynchronize(s_));
+  return ACES_OK;
+}
+
+// ---------------------------------------------------------------------------
+int SymEngine::setup(const double* fock, const double* ovov_dense, const double* oovv_dense, const double* ovoo,
+                     const double* oooo, const double* ovvv, const double* L233) {
+  ACES_REQUIRE(!ready_, "sym setup: already set up");
+  ACES_REQUIRE(nirr == 1 || nirr == 2 || nirr == 4 || nirr == 8, "sym setup: nirr must be 1, 2, 4 or 8");
+  for (int g = 0; g < nirr; ++g) ACES_REQUIRE(O_.cnt[g] >= 0 && V_.cnt[g] >= 0, "sym setup: negative counts");
+  ACES_REQUIRE(o >= 1 && v >= 1, "sym setup: need o >= 1 and v >= 1");
+  hipStream_t s = s_;
+  const Space *O = &O_, *V = &V_;
+  auto L4 = [&](const Space* a, const Space* b, const Space* c, const Space* d, int split) {
+    return dpd::make_layout(4, split, a, b, c, d);
+  };
+  auto L2 = [&](const Space* a, const Space* b) { return dpd::make_layout(2, 1, a, b); };
+  const Layout LOVOV = L4(O, V, O, V, 2), LOOVV = L4(O, O, V, V, 2), LOVOO = L4(O, V, O, O, 2);
+  const Layout LOOOO = L4(O, O, O, O, 2), LOVVV3 = L4(O, V, V, V, 3), LOVVV2 = L4(O, V, V, V, 2);
+  const Layout LOO = L2(O, O), LVV = L2(V, V), LOV = L2(O, V), LVO = L2(V, O);
+  const Layout LOOVV1 = L4(O, O, V, V, 1), LVOOV1 = L4(V, O, O, V, 1), LOOOV3 = L4(O, O, O, V, 3);
+  const Layout LOOOV2 = L4(O, O, O, V, 2), LOVOO3 = L4(O, V, O, O, 3), LOOOV1 = L4(O, O, O, V, 1);
+  const Layout LVOVV1 = L4(V, O, V, V, 1), LOVVO3 = L4(O, V, V, O, 3), LOVOV3 = L4(O, V, O, V, 3);
+  const Layout LOOVV3 = L4(O, O, V, V, 3), LOOVO3 = L4(O, O, V, O, 3), LOOVO1 = L4(O, O, V, O, 1);
+  const Layout LVOVO1 = L4(V, O, V, O, 1), LOOOO3 = L4(O, O, O, O, 3), LVVOO2 = L4(V, V, O, O, 2);
+  const Layout LVVOO1 = L4(V, V, O, O, 1), LVOOO3 = L4(V, O, O, O, 3);
+  aux_o_ = LOOVV1;
+  aux_v_ = LVOOV1;
+  aux_vo_ = LVO;
+
+  // scratch, workspace, reductions
+  scr_elems_ = std::max({LOVOV.size(), LOOVV.size(), LOVVV2.size(), 1L});
+  ACES_CHECK(alloc(&scr_, scr_elems_));
+  ACES_CHECK(alloc(&scr2_, LOOVV.size() + LOV.size() + 8));
+  ws_elems_ = 1L << 25;
+  ACES_CHECK(alloc(&ws_, ws_elems_));
+  double* rp;
+  ACES_CHECK(alloc(&rp, (long)kDotBlocks * kMaxDots + 4096));
+  red_ = rp;
+  ACES_CHECK(alloc(&scal_, 64));
+  {
+    double* pp;
+    ACES_CHECK(alloc(&pp, 2 * kMaxDots + 8));
+    dptr_ = (const double**)pp;
+    coef_ = pp + kMaxDots + 4;
+  }
+
+  // orbital energies from the Fock diagonal (irrep-sorted orbital order)
+  ACES_CHECK(alloc(&eo_, o));
+  ACES_CHECK(alloc(&ev_, v));
+  ACES_CHECK(alloc(&geo_, o));
+  ACES_CHECK(alloc(&gev_, v));
+  ACES_CHECK(eu::P4(1, 1, 1, o, eo_).src(fock, 1, 0, 0, 0, (long)n + 1).run(s));
+  ACES_CHECK(eu::P4(1, 1, 1, v, ev_).src(fock + (long)o * n + o, 1, 0, 0, 0, (long)n + 1).run(s));
+
+  // packed amplitude state [t1 | t2] and its gradient
+  n1_ = LOV.size();
+  nt_ = n1_ + LOOVV.size();
+  ACES_CHECK(alloc(&T_cur_, nt_));
+  ACES_CHECK(alloc(&T_new_, nt_));
+  ACES_CHECK(alloc(&gT_, nt_));
+  ACES_HIP(hipMemsetAsync(T_cur_, 0, sizeof(double) * nt_, s));
+
+  // ---- inputs ----
+  const long sOVOV[4] = {(long)v * o * v, (long)o * v, v, 1};
+  const long sOOVV[4] = {(long)o * v * v, (long)v * v, v, 1};
+  const long sOVOO[4] = {(long)v * o * o, (long)o * o, o, 1};
+  const long sOOOO[4] = {(long)o * o * o, (long)o * o, o, 1};
+  const long sOVVV[4] = {(long)v * v * v, (long)v * v, v, 1};
+  const long sF[2] = {n, 1};
+  t1_ = tensor(LOV, true, T_cur_, gT_);
+  t2_ = tensor(LOOVV, true, T_cur_ + n1_, gT_ + n1_);
+  fov_ = tensor(LOV, true);
+  foo_ = tensor(LOO, true);
+  fvv_ = tensor(LVV, true);
+  const int ovov = tensor(LOVOV, false), oovv = tensor(LOOVV, false), ovoo_t = tensor(LOVOO, false);
+  const int oooo_t = tensor(LOOOO, false), ovvv_t = tensor(LOVVV3, false);
+  for (int t : {t1_, t2_, fov_, foo_, fvv_, ovov, oovv, ovoo_t, oooo_t, ovvv_t}) ACES_REQUIRE(t >= 0, "sym setup: alloc");
+  ACES_CHECK(pack_dense(fov_, fock, sF, o));
+  ACES_CHECK(pack_dense(foo_, fock, sF, 0));
+  ACES_CHECK(pack_dense(fvv_, fock, sF, (long)o * n + o));
+  ACES_CHECK(pack_dense(ovov, ovov_dense, sOVOV, 0));
+  ACES_CHECK(pack_dense(oovv, oovv_dense, sOOVV, 0));
+  ACES_CHECK(pack_dense(ovoo_t, ovoo, sOVOO, 0));
+  ACES_CHECK(pack_dense(oooo_t, oooo, sOOOO, 0));
+  ACES_CHECK(pack_dense(ovvv_t, ovvv, sOVVV, 0));
+
+  // ---- static intermediates (same definitions as CcsdEngine::setup) ----
+  const std::vector<int> I4 = {0, 1, 2, 3};
+  auto st = [&](const Layout& L) {
+    const int t = tensor(L, false);
+    return t;
+  };
+  const int Lmat = st(LOVOV), Kx = st(LOVOV), W0t = st(LOVOV), Wv0 = st(LOVOV), Lo = st(LOOVV1), Lv = st(LVOOV1);
+  const int OVkl = st(LOOVV), ooooP = st(LOOOO), ovooQ = st(LOOOV3), M1 = st(LOOOV2), M5c = st(LOVOO3);
+  const int M5b = st(LOVOO3), M7 = st(LOOOV1), kacd = st(LOVVV3), M3 = st(LVOVV1), LvT = st(LOVVV2);
+  const int ovov3 = st(LOVOV3), oovv3 = st(LOOVV3);
+  Lijab_ = st(LOOVV);
+  ACES_CHECK(perm(Lmat, 0, {{ovov, I4}, {ovov, {0, 3, 2, 1}}}, {2, -1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(Kx, 0, {{ovov, {0, 3, 2, 1}}}, {1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(W0t, 0, {{ovov, I4}, {oovv, {0, 2, 1, 3}}}, {2, -1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(Wv0, 0, {{oovv, {2, 0, 1, 3}}}, {1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(Lo, 0, {{ovov, {0, 2, 1, 3}}, {ovov, {0, 3, 1, 2}}}, {2, -1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(Lv, 0, {{ovov, {1, 0, 2, 3}}, {ovov, {1, 3, 2, 0}}}, {2, -1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(OVkl, 0, {{ovov, {0, 2, 1, 3}}}, {1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(Lijab_, 0, {{ovov, {0, 2, 1, 3}}, {ovov, {0, 3, 1, 2}}}, {2, -1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(ooooP, 0, {{oooo_t, {0, 2, 1, 3}}}, {1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(ovooQ, 0, {{ovoo_t, {1, 3, 0, 2}}}, {1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(M1, 0, {{ovoo_t, {2, 3, 0, 1}}, {ovoo_t, {0, 3, 2, 1}}}, {2, -1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(M5c, 0, {{ovoo_t, {0, 1, 3, 2}}, {ovoo_t, {3, 1, 0, 2}}}, {2, -1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(M5b, 0, {{ovoo_t, {3, 1, 0, 2}}}, {1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(M7, 0, {{ovoo_t, {2, 3, 1, 0}}, {ovoo_t, {1, 3, 2, 0}}}, {2, -1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(kacd, 0, {{ovvv_t, {0, 3, 1, 2}}}, {1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(M3, 0, {{ovvv_t, {1, 3, 0, 2}}, {ovvv_t, {1, 2, 0, 3}}}, {2, -1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(LvT, 0, {{ovvv_t, I4}, {ovvv_t, {0, 3, 2, 1}}}, {2, -1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(ovov3, 0, {{ovov, I4}}, {1}, -1, {}, -1, {}, 0, false));
+  ACES_CHECK(perm(oovv3, 0, {{oovv, I4}}, {1}, -1, {}, -1, {}, 0, false));
+
+  // ---- ladder operands ----
+  lad_ = dpd::make_ladder(O_, V_);
+  {
+    double* p;
+    ACES_CHECK(alloc(&p, (sizeof(dpd::LadderDesc) + 7) / 8));
+    d_lad_ = (dpd::LadderDesc*)p;
+    ACES_HIP(hipMemcpy(d_lad_, &lad_, sizeof(dpd::LadderDesc), hipMemcpyHostToDevice));
+    ACES_CHECK(alloc(&p, 2 * ((sizeof(Layout) + 7) / 8)));
+    d_layouts_ = (Layout*)p;
+    const Layout two[2] = {LOOVV, LVOOO3};
+    ACES_HIP(hipMemcpy(d_layouts_, two, 2 * sizeof(Layout), hipMemcpyHostToDevice));
+  }
+  long r[dpd::kMaxIrr], c[dpd::kMaxIrr];
+  for (int g = 0; g < nirr; ++g) { r[g] = lad_.VP.n[g] + lad_.nkb[g]; c[g] = lad_.VP.n[g]; }
+  Vp_ = block_tensor(r, c, false);
+  for (int g = 0; g < nirr; ++g) { r[g] = lad_.VM.n[g] + lad_.nkb[g]; c[g] = lad_.VM.n[g]; }
+  Vm_ = block_tensor(r, c, false);
+  ACES_REQUIRE(Vp_ >= 0 && Vm_ >= 0, "sym setup: ladder alloc");
+  ACES_CHECK(dpd::build_ladder_sym(d_lad_, lad_, L233, ovvv, T_[Vp_].d, T_[Vm_].d, s));
+  for (int g = 0; g < nirr; ++g) { r[g] = lad_.OP.n[g]; c[g] = lad_.VP.n[g]; }
+  Sp_ = block_tensor(r, c, true);
+  for (int g = 0; g < nirr; ++g) { r[g] = lad_.OM.n[g]; c[g] = lad_.VM.n[g]; }
+  Dp_ = block_tensor(r, c, true);
+  for (int g = 0; g < nirr; ++g) { r[g] = lad_.VP.n[g] + lad_.nkb[g]; c[g] = lad_.OP.n[g]; }
+  Zp_ = block_tensor(r, c, true);
+  for (int g = 0; g < nirr; ++g) { r[g] = lad_.VM.n[g] + lad_.nkb[g]; c[g] = lad_.OM.n[g]; }
+  Zm_ = block_tensor(r, c, true);
+  ACES_REQUIRE(Sp_ >= 0 && Dp_ >= 0 && Zp_ >= 0 && Zm_ >= 0, "sym setup: ladder alloc");
+
+  // ---- the increments, recorded in the order of CcsdEngine::compute_increments ----
+  auto dy = [&](const Layout& L) { return tensor(L, false); };  // grad flag set by the recording ops
+  tau_ = dy(LOOVV);
+  const int U = dy(LOVOV), X = dy(LOVOV), Ut = dy(LOVOV), Xt = dy(LOVOV);
+  ACES_CHECK(perm(tau_, 0, {{t2_, I4}}, {1}, t1_, {0, 2}, t1_, {1, 3}, 1.0));
+  tau_op_ = last_op_;
+  // U = 2 t_ijab - t_ijba, X = t_ijba  (index order i a j b)
+  ACES_CHECK(perm(U, 0, {{t2_, {0, 2, 1, 3}}, {t2_, {0, 2, 3, 1}}}, {2, -1}));
+  ACES_CHECK(perm(X, 0, {{t2_, {0, 2, 3, 1}}}, {1}));
+  ACES_CHECK(perm(Ut, 0, {{U, I4}}, {1}, t1_, {2, 1}, t1_, {0, 3}, -2.0));
+  ACES_CHECK(perm(Xt, 0, {{X, I4}}, {1}, t1_, {0, 3}, t1_, {2, 1}, 2.0));
+  // T1 parts of W(mbej)
+  const int P1 = dy(LOVVO3), P2 = dy(LOVVO3), Q12 = dy(LOVOV3), Q2 = dy(LOVOV3);
+  ACES_CHECK(gemm_op(P1, ovvv_t, 1, t1_, 1, 1, 0, 3, false));
+  ACES_CHECK(gemm_op(P2, kacd, 1, t1_, 1, 1, 0, 3, false));
+  ACES_CHECK(gemm_op(Q12, M5c, 1, t1_, 0, 1, 0, 3, false));
+  ACES_CHECK(gemm_op(Q2, M5b, 1, t1_, 0, 1, 0, 3, false));
+  const int Wt = dy(LOVOV), Wv = dy(LOVOV);
+  ACES_CHECK(perm(Wt, 0, {{W0t, I4}, {P1, {2, 3, 1, 0}}, {P2, {2, 1, 3, 0}}}, {1, 2, -1}));
+  ACES_CHECK(perm(Wt, 1, {{Q12, {2, 3, 0, 1}}}, {-1}));
+  ACES_CHECK(perm(Wv, 0, {{Wv0, I4}, {P2, {2, 1, 3, 0}}, {Q2, {2, 3, 0, 1}}}, {1, 1, -1}));
+  ACES_CHECK(gemm_op(Wt, Ut, 1, Lmat, 1, 0.5, 1, 1, true));
+  ACES_CHECK(gemm_op(Wv, Xt, 1, Kx, 1, -0.5, 1, 1, true));
+  const int R1 = dy(LOVOV), R2 = dy(LOVOV);
+  ACES_CHECK(gemm_op(R1, Wt, 1, U, 1, 1, 0, 2, true));
+  ACES_CHECK(gemm_op(R2, Wv, 1, X, 1, 1, 0, 2, true));
+  inc2_ = dy(LOOVV);
+  ACES_CHECK(perm(inc2_, 0, {{ovov, {0, 2, 1, 3}}, {R1, {0, 2, 1, 3}}, {R1, {1, 3, 0, 2}}}, {1, 0.5, 0.5}));
+  ACES_CHECK(perm(inc2_, 1, {{R2, {0, 2, 1, 3}}, {R2, {1, 3, 0, 2}}, {R2, {0, 3, 1, 2}}}, {-0.5, -0.5, -1}));
+  ACES_CHECK(perm(inc2_, 1, {{R2, {1, 2, 0, 3}}}, {-1}));
+  // particle-particle ladder
+  {
+    Op op;
+    op.kind = OP_PACK;
+    op.out = Sp_;
+    ACES_CHECK(dpd::build_plan(op.fwd, LOOVV, {}, nullptr, nullptr));  // tau walk for the adjoint
+    ops_.push_back(op);
+    pack_op_ = (int)ops_.size() - 1;
+    T_[Dp_].grad = true;
+  }
+  ACES_CHECK(gemm_op(Zp_, Vp_, 1, Sp_, 1, 1, 0, 0, true));
+  ACES_CHECK(gemm_op(Zm_, Vm_, 1, Dp_, 1, 1, 0, 0, true));
+  Yb_ = dy(LVOOO3);
+  {
+    Op op;
+    op.kind = OP_UNPACK;
+    op.out = inc2_;
+    ACES_CHECK(dpd::build_plan(op.fwd, LOOVV, {}, nullptr, nullptr));
+    ACES_CHECK(dpd::build_plan(op.hplan, LVOOO3, {}, nullptr, nullptr));
+    ops_.push_back(op);
+    unpack_op_ = (int)ops_.size() - 1;
+    T_[Yb_].grad = true;
+    ACES_CHECK(alloc(&T_[Yb_].g, T_[Yb_].boff[nirr]));
+  }
+  const int L1 = dy(LVVOO1);
+  ACES_CHECK(gemm_op(L1, t1_, 0, Yb_, 1, 1, 0, 3, false));
+  ACES_CHECK(perm(inc2_, 1, {{L1, {2, 3, 0, 1}}, {L1, {3, 2, 1, 0}}}, {-1, -1}));
+  // hole-hole ladder
+  const int Qo = dy(LOOOO3), Wo = dy(LOOOO), ZH = dy(LVVOO2);
+  ACES_CHECK(gemm_op(Qo, ovooQ, 1, t1_, 1, 1, 0, 3, false));
+  ACES_CHECK(perm(Wo, 0, {{ooooP, I4}, {Qo, I4}, {Qo, {1, 0, 3, 2}}}, {1, 1, 1}));
+  ACES_CHECK(gemm_op(Wo, OVkl, 1, tau_, 1, 1, 1, 3, false));
+  ACES_CHECK(gemm_op(ZH, tau_, 0, Wo, 0, 1, 0, 3, false));
+  ACES_CHECK(perm(inc2_, 1, {{ZH, {2, 3, 0, 1}}}, {1}));
+  // F intermediates
+  const int Fov = dy(LOV), tau1 = dy(LOOVV1), Foo = dy(LOO), tauK = dy(LVOOV1), Fvv = dy(LVV);
+  const int FT = dy(LOO), FovT = dy(LOO), Loo = dy(LOO), Lvv = dy(LVV);
+  ACES_CHECK(perm(Fov, 0, {{fov_, {0, 1}}}, {1}));
+  ACES_CHECK(gemv_op(Fov, Lmat, t1_, 0, 1, 1));
+  ACES_CHECK(perm(tau1, 0, {{tau_, I4}}, {1}));
+  ACES_CHECK(perm(Foo, 0, {{foo_, {0, 1}}}, {1}));
+  ACES_CHECK(gemm_op(Foo, Lo, 1, tau1, 1, 1, 1, 3, false));
+  ACES_CHECK(perm(tauK, 0, {{tau_, {1, 2, 0, 3}}}, {1}));
+  ACES_CHECK(perm(Fvv, 0, {{fvv_, {0, 1}}}, {1}));
+  ACES_CHECK(gemm_op(Fvv, tauK, 1, Lv, 1, -1, 1, 3, false));
+  ACES_CHECK(gemm_op(FT, fov_, 1, t1_, 1, 1, 0, 3, false));
+  ACES_CHECK(gemm_op(FovT, Fov, 1, t1_, 1, 1, 0, 3, false));
+  ACES_CHECK(perm(Loo, 0, {{Foo, {0, 1}}, {FT, {0, 1}}}, {1, 1}));
+  ACES_CHECK(gemv_op(Loo, M1, t1_, 0, 1, 1));
+  ACES_CHECK(perm(Lvv, 0, {{Fvv, {0, 1}}}, {1}));
+  ACES_CHECK(gemm_op(Lvv, t1_, 0, fov_, 0, -1, 1, 3, false));
+  ACES_CHECK(gemv_op(Lvv, LvT, t1_, 1, 1, 1));
+  auto subdiag = [&](int t, int evec) {
+    T_[t].grad = true;
+    Op op;
+    op.kind = OP_SUBDIAG;
+    op.out = t;
+    op.al = -1.0;
+    op.evec = evec;
+    ops_.push_back(op);
+  };
+  subdiag(Foo, 0);
+  subdiag(Fvv, 1);
+  subdiag(Loo, 0);
+  subdiag(Lvv, 1);
+  // FEACONT / FMICONT
+  const int t2c = dy(LVOOV1), tA = dy(LVOOV1), t2s1 = dy(LOOVV1), tB = dy(LOOVV1);
+  ACES_CHECK(perm(t2c, 0, {{t2_, {1, 2, 0, 3}}}, {1}));
+  ACES_CHECK(gemm_op(tA, Lvv, 1, t2c, 0, 1, 0, 3, false));
+  ACES_CHECK(perm(t2s1, 0, {{t2_, I4}}, {1}));
+  ACES_CHECK(gemm_op(tB, Loo, 0, t2s1, 0, -1, 0, 3, false));
+  ACES_CHECK(perm(inc2_, 1, {{tA, {2, 0, 1, 3}}, {tA, {3, 1, 0, 2}}, {tB, I4}}, {1, 1, 1}));
+  ACES_CHECK(perm(inc2_, 1, {{tB, {1, 0, 3, 2}}}, {1}));
+  // T1INT2A / T1INT2B
+  const int Qb = dy(LOOVO3), Qb1 = dy(LOOVO1), T1Q = dy(LVOVO1);
+  ACES_CHECK(gemm_op(Qb, oovv3, 1, t1_, 1, 1, 0, 3, false));
+  ACES_CHECK(perm(Qb1, 0, {{Qb, I4}}, {1}));
+  ACES_CHECK(gemm_op(T1Q, t1_, 0, Qb1, 0, 1, 0, 3, false));
+  ACES_CHECK(perm(inc2_, 1, {{P1, {0, 2, 3, 1}}, {T1Q, {2, 0, 3, 1}}, {P1, {1, 3, 2, 0}}}, {1, -1, 1}));
+  ACES_CHECK(perm(inc2_, 1, {{T1Q, {3, 1, 2, 0}}}, {-1}));
+  const int S3 = dy(LOVOO3), G4A = dy(LOVOO3), G4 = dy(LOVOV3);
+  ACES_CHECK(gemm_op(S3, ovov3, 1, t1_, 1, 1, 0, 3, false));
+  ACES_CHECK(perm(G4A, 0, {{ovoo_t, I4}, {S3, {0, 1, 3, 2}}}, {1, 1}));
+  ACES_CHECK(gemm_op(G4, G4A, 1, t1_, 0, 1, 0, 3, false));
+  ACES_CHECK(perm(inc2_, 1, {{G4, {0, 2, 1, 3}}, {G4, {1, 3, 0, 2}}}, {-1, -1}));
+  // T1 increment
+  inc1_ = dy(LOV);
+  const int Gk = dy(LOO);
+  ACES_CHECK(perm(inc1_, 0, {{fov_, {0, 1}}}, {1}));
+  ACES_CHECK(perm(Gk, 0, {{Foo, {0, 1}}, {FT, {0, 1}}, {FovT, {0, 1}}}, {1, 2, -1}));
+  ACES_CHECK(gemm_op(inc1_, Gk, 0, t1_, 0, -1, 1, 3, false));
+  ACES_CHECK(gemm_op(inc1_, t1_, 1, Fvv, 1, 1, 1, 3, false));
+  ACES_CHECK(gemv_op(inc1_, U, Fov, 0, 1, 1));
+  ACES_CHECK(gemv_op(inc1_, W0t, t1_, 0, 1, 1));
+  ACES_CHECK(gemm_op(inc1_, tau1, 1, M3, 1, 1, 1, 3, false));
+  ACES_CHECK(gemm_op(inc1_, M7, 1, tauK, 1, -1, 1, 3, false));
+  for (Tensor& t : T_)
+    if (t.grad && t.g == nullptr) ACES_CHECK(alloc(&t.g, std::max<long>(t.boff[nirr], 1)));
+  low_.resize(T_.size());
+
+  // ---- auxiliary plans ----
+  {
+    Src id2, id4, sw, swp;
+    id2.L = &T_[t1_].L;
+    id4.L = &T_[t2_].L;
+    sw.L = &T_[t2_].L;
+    sw.map[0] = 1; sw.map[1] = 0; sw.map[2] = 2; sw.map[3] = 3;
+    swp.L = &T_[t2_].L;
+    swp.map[0] = 1; swp.map[1] = 0; swp.map[2] = 3; swp.map[3] = 2;
+    ACES_CHECK(dpd::build_plan(jac1_, LOV, {id2}, nullptr, nullptr));
+    ACES_CHECK(dpd::build_plan(jac2_, LOOVV, {id4}, nullptr, nullptr));
+    ACES_CHECK(dpd::build_plan(perm2_, LOOVV, {id4, sw}, nullptr, nullptr));
+    ACES_CHECK(dpd::build_plan(sym2_, LOOVV, {swp}, nullptr, nullptr));
+    ACES_CHECK(dpd::build_plan(absplan_, LOOVV, {sw}, nullptr, nullptr));
+    plans_.resize(5);
+    Src a;
+    a.L = &T_[t2_].L;
+    const int maps[4][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {1, 2, 0, 3}, {1, 2, 3, 0}};
+    for (int k = 0; k < 4; ++k) {
+      for (int m = 0; m < 4; ++m) a.map[m] = maps[k][m];
+      ACES_CHECK(dpd::build_plan(plans_[k], k < 2 ? aux_o_ : aux_v_, {a}, nullptr, nullptr));
+    }
+    Src b;
+    b.L = &T_[t1_].L;
+    b.map[0] = 1;
+    b.map[1] = 0;
+    ACES_CHECK(dpd::build_plan(plans_[4], aux_vo_, {b}, nullptr, nullptr));
+  }
+  // absmax partials: 64 blocks over t1 plus one per chunk of the t2 walk
+  ACES_CHECK(alloc(&absp_, (long)((64 + absplan_.total + 1) * sizeof(AbsM) + 7) / 8));
+  ACES_CHECK(diis_grow(sol_, errv_, order_));
+  R_.assign((size_t)order_ * order_, 0.0);
+  ACES_CHECK(alloc(&err_cur_, nt_));
+  ACES_CHECK(alloc(&newvec_, nt_));
+  ACES_CHECK(alloc(&eperm_, nt_));
+  ACES_HIP(hipStreamSynchronize(s));
+  ready_ = true;
+  return ACES_OK;
+}
+
+int SymEngine::diis_grow(std::vector<double*>& sol, std::vector<double*>& err, int order) {
+  ACES_REQUIRE(order >= 1 && order <= 25, "DIIS order must be 1..25 (ld_diis = 25, vcc/dodiis0.F)");
+  while ((int)sol.size() < order) {
+    double *a, *b;
+    ACES_CHECK(alloc(&a, nt_));
+    ACES_CHECK(alloc(&b, nt_));
+    sol.push_back(a);
+    err.push_back(b);
+  }
+  return ACES_OK;
+}
+
+int SymEngine::set_diis_order(int order) {
+  ACES_CHECK(diis_grow(sol_, errv_, order));
+  order_ = order;
+  R_.assign((size_t)order * order, 0.0);
+  return ACES_OK;
+}
+
+int SymEngine::reset_diis() {
+  R_.assign((size_t)order_ * order_, 0.0);
+  return ACES_OK;
+}
+
+int SymEngine::set_amplitudes(const double* t1, const double* t2) {
+  ACES_REQUIRE(ready_, "sym: not set up");
+  const long s1[2] = {v, 1};
+  const long s2[4] = {(long)o * v * v, (long)v * v, v, 1};
+  ACES_CHECK(pack_dense(t1_, t1, s1, 0));
+  ACES_CHECK(pack_dense(t2_, t2, s2, 0));
+  fwd_valid_ = false;
+  return ACES_OK;
+}
+
+int SymEngine::get_amplitudes(double* t1, double* t2) {
+  ACES_REQUIRE(ready_, "sym: not set up");
+  const long s1[2] = {v, 1};
+  const long s2[4] = {(long)o * v * v, (long)v * v, v, 1};
+  if (t1) {
+    ACES_HIP(hipMemsetAsync(t1, 0, sizeof(double) * o * v, s_));
+    ACES_CHECK(unpack_dense(t1_, T_[t1_].d, t1, s1, 0));
+  }
+  if (t2) {
+    ACES_HIP(hipMemsetAsync(t2, 0, sizeof(double) * (long)o * o * v * v, s_));
+    ACES_CHECK(unpack_dense(t2_, T_[t2_].d, t2, s2, 0));
+  }
+  return ACES_OK;
+}
+
+// MP1 (intprc/dfrmt2.F): T2 = <ij|ab>/D, T1 = 0.
+int SymEngine::start_mp1() {
+  ACES_REQUIRE(ready_, "sym: not set up");
+  ACES_HIP(hipMemsetAsync(T_cur_, 0, sizeof(double) * n1_, s_));
+  // <ij|ab> = (ia|jb) = (O,O|V,V) gather of ovov; Lijab's source is ovov, so use
+  // the first recorded inc2 permutation's source 0 (ovov, map 0 2 1 3).
+  Op* first = nullptr;
+  for (Op& op : ops_)
+    if (op.kind == OP_PERM && op.out == inc2_ && op.beta == 0.0) first = &op;
+  ACES_REQUIRE(first != nullptr, "sym mp1: no inc2 gather");
+  double* tmp = scr2_;
+  const double* src[3] = {T_[first->src[0]].d, nullptr, nullptr};
+  Src sr;
+  sr.L = &T_[first->src[0]].L;
+  for (int m = 0; m < 4; ++m) sr.map[m] = first->map[0][m];
+  Plan p;
+  ACES_CHECK(dpd::build_plan(p, T_[t2_].L, {sr}, nullptr, nullptr));
+  const double one[1] = {1.0};
+  ACES_CHECK(dpd::apply_plan(p, tmp, 0.0, src, one, nullptr, nullptr, 0.0, nullptr, s_));
+  dpd::free_plan(p);
+  ACES_CHECK(denom_apply(t2_, T_[t2_].d, 0.0, tmp, 1.0, nullptr, 0.0, 0, -1));
+  ACES_HIP(hipStreamSynchronize(s_));
+  fwd_valid_ = false;
+  return ACES_OK;
+}
+
+int SymEngine::increments(GemmPrec prec, double* inc1, double* inc2) {
+  ACES_REQUIRE(ready_, "sym: not set up");
+  ACES_CHECK(forward(prec));
+  const long s1[2] = {v, 1};
+  const long s2[4] = {(long)o * v * v, (long)v * v, v, 1};
+  if (inc1) {
+    ACES_HIP(hipMemsetAsync(inc1, 0, sizeof(double) * o * v, s_));
+    ACES_CHECK(unpack_dense(inc1_, T_[inc1_].d, inc1, s1, 0));
+  }
+  if (inc2) {
+    ACES_HIP(hipMemsetAsync(inc2, 0, sizeof(double) * (long)o * o * v * v, s_));
+    ACES_CHECK(unpack_dense(inc2_, T_[inc2_].d, inc2, s2, 0));
+  }
+  ACES_HIP(hipStreamSynchronize(s_));
+  return ACES_OK;
+}
+
+int SymEngine::energy_current(double& e) { return energy_dev(T_[t1_].d, T_[t2_].d, e); }
+
+// DODIIS0 on irrep-blocked vectors [t1 | t2]; the AA part of the reference's
+// DIIS vector enters the dot products as 1/2 <e2, f2> - 1/2 <e2(i<->j), f2>
+// (sum over I<J, A<B of (e-e^P)(f-f^P) for pair-symmetric vectors).
+int SymEngine::diis_core(std::vector<double*>& sol, std::vector<double*>& err, std::vector<double>& R, int order,
+                         double* x, double* xn, int icycle, double tol, bool lambda_rule, CcsdStats& st) {
+  hipStream_t s = s_;
+  ACES_CHECK(pair_symmetrize(x + n1_));
+  ACES_CHECK(pair_symmetrize(xn + n1_));
+  const int ndimr = std::min(icycle, order);
+  const int idimr = 1 + (icycle - 1) % order;
+  ACES_CHECK(sub(nt_, xn, x, err_cur_, s));
+  double resmiter;
+  ACES_CHECK(absmax_vec(err_cur_, resmiter));
+  ACES_HIP(hipMemcpyAsync(sol[idimr - 1], x, sizeof(double) * nt_, hipMemcpyDeviceToDevice, s));
+  // e(i<->j) for the AA part
+  {
+    const double* src[1] = {err_cur_ + n1_};
+    const double one[1] = {1.0};
+    Plan& p = absplan_;
+    ACES_CHECK(dpd::apply_plan(p, eperm_ + n1_, 0.0, src, one, nullptr, nullptr, 0.0, nullptr, s));
+  }
+  std::vector<const double*> xs(ndimr);
+  for (int i = 1; i <= ndimr; ++i) xs[i - 1] = (i == idimr) ? err_cur_ : err[i - 1];
+  ACES_HIP(hipMemcpyAsync(dptr_, xs.data(), sizeof(double*) * ndimr, hipMemcpyHostToDevice, s));
+  double* out = scal_ + 8;
+  const long n2 = nt_ - n1_;
+  std::vector<const double*> xs2(ndimr);
+  for (int i = 0; i < ndimr; ++i) xs2[i] = xs[i] + n1_;
+  // t1 part
+  hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks), dim3(256), 0, s, n1_, ndimr, (const double* const*)dptr_,
+                     err_cur_, (double*)red_);
+  hipLaunchKernelGGL(mdot_finish_kernel, dim3(ndimr), dim3(256), 0, s, kDotBlocks, (double*)red_, out, 1.0, 0);
+  ACES_HIP(hipMemcpyAsync(dptr_, xs2.data(), sizeof(double*) * ndimr, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks), dim3(256), 0, s, n2, ndimr, (const double* const*)dptr_,
+                     err_cur_ + n1_, (double*)red_);
+  hipLaunchKernelGGL(mdot_finish_kernel, dim3(ndimr), dim3(256), 0, s, kDotBlocks, (double*)red_, out, 1.5, 1);
+  hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks), dim3(256), 0, s, n2, ndimr, (const double* const*)dptr_,
+                     eperm_ + n1_, (double*)red_);
+  hipLaunchKernelGGL(mdot_finish_kernel, dim3(ndimr), dim3(256), 0, s, kDotBlocks, (double*)red_, out, -0.5, 1);
+  ACES_HIP(hipGetLastError());
+  double h[40];
+  ACES_HIP(hipMemcpyAsync(h, scal_, sizeof(double) * 40, hipMemcpyDeviceToHost, s));
+  ACES_HIP(hipStreamSynchronize(s));
+  st.resmiter = resmiter;
+  for (int i = 1; i <= ndimr; ++i) {
+    R[(size_t)(i - 1) * order + (idimr - 1)] = h[8 + i - 1];
+    R[(size_t)(idimr - 1) * order + (i - 1)] = h[8 + i - 1];
+  }
+  ACES_HIP(hipMemcpyAsync(err[idimr - 1], err_cur_, sizeof(double) * nt_, hipMemcpyDeviceToDevice, s));
+  const int na = ndimr + 1;
+  std::vector<double> A((size_t)na * na, 0.0), b(na, 0.0);
+  for (int j = 1; j < na; ++j) {
+    A[j] = -1.0;
+    A[(size_t)j * na] = -1.0;
+    for (int i = 1; i < na; ++i) A[(size_t)i * na + j] = R[(size_t)(i - 1) * order + (j - 1)];
+  }
+  b[0] = -1.0;
+  if (eu::solve_small_lu(na, A, b) != ACES_OK) {
+    set_error("DODIIS: linear solver failed (singular DIIS matrix)");
+    return ACES_ERR_SOLVE;
+  }
+  std::vector<double> c(b.begin() + 1, b.end());
+  std::vector<const double*> rv;
+  std::vector<double> rc;
+  if (lambda_rule) {
+    for (int i = 1; i <= ndimr; ++i) {
+      rv.push_back(err[i - 1]);
+      rc.push_back(c[i - 1]);
+    }
+  } else {
+    rv.push_back(err[idimr - 1]);
+    rc.push_back(c[idimr - 1]);
+    for (int i = ndimr; i > idimr; --i) {
+      rv.push_back(err[i - 1]);
+      rc.push_back(c[i - 1]);
+    }
+    for (int i = idimr - 1; i >= 1; --i) {
+      rv.push_back(err[i - 1]);
+      rc.push_back(c[i - 1]);
+    }
+  }
+  ACES_HIP(hipMemcpyAsync(dptr_, rv.data(), sizeof(double*) * rv.size(), hipMemcpyHostToDevice, s));
+  ACES_HIP(hipMemcpyAsync(coef_, rc.data(), sizeof(double) * rc.size(), hipMemcpyHostToDevice, s));
+  ACES_CHECK(lincomb(nt_, (int)rv.size(), dptr_, coef_, newvec_, s));
+  double resmdiis;
+  ACES_CHECK(absmax_vec(newvec_, resmdiis));
+  st.resmdiis = resmdiis;
+  if (lambda_rule)
+    st.converged = (std::fabs(st.resmiter) < tol || std::fabs(st.resmdiis) < tol) ? 1 : 0;
+  else
+    st.converged = std::fabs(st.resmiter) < tol ? 1 : 0;
+  if (st.converged) return ACES_OK;
+  if (ndimr >= 2) {
+    std::vector<const double*> sv;
+    std::vector<double> sc;
+    sv.push_back(newvec_);
+    sc.push_back(1.0);
+    for (int i = 1; i <= ndimr; ++i) {
+      sv.push_back(sol[i - 1]);
+      sc.push_back(c[i - 1]);
+    }
+    ACES_HIP(hipMemcpyAsync(dptr_, sv.data(), sizeof(double*) * sv.size(), hipMemcpyHostToDevice, s));
+    ACES_HIP(hipMemcpyAsync(coef_, sc.data(), sizeof(double) * sc.size(), hipMemcpyHostToDevice, s));
+    ACES_CHECK(lincomb(nt_, (int)sv.size(), dptr_, coef_, x, s));
+  } else {
+    ACES_HIP(hipMemcpyAsync(x, xn, sizeof(double) * nt_, hipMemcpyDeviceToDevice, s));
+  }
+  return ACES_OK;
+}
+
+int SymEngine::cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st) {
+  ACES_REQUIRE(ready_, "sym: not set up");
+  ACES_REQUIRE(icycle >= 1, "sym cycle: icycle starts at 1");
+  hipEvent_t e0, e1, e2;
+  ACES_HIP(hipEventCreate(&e0));
+  ACES_HIP(hipEventCreate(&e1));
+  ACES_HIP(hipEventCreate(&e2));
+  ACES_HIP(hipEventRecord(e0, s_));
+  ACES_CHECK(forward(prec));
+  ACES_HIP(hipEventRecord(e1, s_));
+  // NEWT2 / E4S
+  ACES_CHECK(denom_apply(t1_, T_new_, 0.0, T_[inc1_].d, 1.0, nullptr, 0.0, 0, -1));
+  ACES_CHECK(denom_apply(t2_, T_new_ + n1_, 0.0, T_[inc2_].d, 1.0, nullptr, 0.0, 0, -1));
+  double enew;
+  ACES_CHECK(energy_dev(T_new_, T_new_ + n1_, enew));
+  st.energy_new = enew;
+  ACES_CHECK(diis_core(sol_, errv_, R_, order_, T_cur_, T_new_, icycle, tol, false, st));
+  if (st.converged) {
+    double e;
+    ACES_CHECK(energy_dev(T_cur_, T_cur_ + n1_, e));
+    st.energy_final = e;
+  } else {
+    fwd_valid_ = false;
+  }
+  ACES_HIP(hipEventRecord(e2, s_));
+  ACES_HIP(hipEventSynchronize(e2));
+  float ms1 = 0, ms2 = 0;
+  ACES_HIP(hipEventElapsedTime(&ms1, e0, e1));
+  ACES_HIP(hipEventElapsedTime(&ms2, e0, e2));
+  st.ms_increments = ms1;
+  st.ms_cycle = ms2;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  return ACES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Differentiation form
+int SymEngine::adjoint(const double* l1, const double* l2, double* r1, double* r2, double* dens) {
+  ACES_REQUIRE(ready_, "sym: not set up");
+  if (!lam_) {
+    ACES_CHECK(alloc(&lam_, nt_));
+  }
+  if (!fwd_valid_) ACES_CHECK(forward(GP_F64));
+  const long s1[2] = {v, 1};
+  const long s2[4] = {(long)o * v * v, (long)v * v, v, 1};
+  // pack the multipliers through the t1/t2 tensors' layouts into lam_
+  {
+    double* keep1 = T_[t1_].d;
+    double* keep2 = T_[t2_].d;
+    T_[t1_].d = lam_;
+    T_[t2_].d = lam_ + n1_;
+    int rc = pack_dense(t1_, l1, s1, 0);
+    if (rc == ACES_OK) rc = pack_dense(t2_, l2, s2, 0);
+    T_[t1_].d = keep1;
+    T_[t2_].d = keep2;
+    ACES_CHECK(rc);
+  }
+  ACES_CHECK(backward(lam_, lam_ + n1_, dens != nullptr));
+  ACES_CHECK(pair_symmetrize(gT_ + n1_));
+  if (r1) {
+    ACES_HIP(hipMemsetAsync(r1, 0, sizeof(double) * o * v, s_));
+    ACES_CHECK(unpack_dense(t1_, gT_, r1, s1, 0));
+  }
+  if (r2) {
+    ACES_HIP(hipMemsetAsync(r2, 0, sizeof(double) * (long)o * o * v * v, s_));
+    ACES_CHECK(unpack_dense(t2_, gT_ + n1_, r2, s2, 0));
+  }
+  if (dens) ACES_CHECK(assemble_dense_density(dens));
+  ACES_HIP(hipStreamSynchronize(s_));
+  return ACES_OK;
+}
+
+int SymEngine::assemble_dense_density(double* dens) {
+  // dense f_oo, f_vv, f_ov gradients, then the symmetrised n x n matrix
+  double *go, *gv, *gov;
+  if (!dscr_) ACES_CHECK(alloc(&dscr_, (long)o * o + (long)v * v + (long)o * v));
+  go = dscr_;
+  gv = go + (long)o * o;
+  gov = gv + (long)v * v;
+  const long so[2] = {o, 1}, sv[2] = {v, 1}, sov[2] = {v, 1};
+  ACES_HIP(hipMemsetAsync(dscr_, 0, sizeof(double) * ((long)o * o + (long)v * v + (long)o * v), s_));
+  ACES_CHECK(unpack_dense(foo_, T_[foo_].g, go, so, 0));
+  ACES_CHECK(unpack_dense(fvv_, T_[fvv_].g, gv, sv, 0));
+  ACES_CHECK(unpack_dense(fov_, T_[fov_].g, gov, sov, 0));
+  return assemble_density(o, v, go, gv, gov, geo_, gev_, dens, s_);
+}
+
+int SymEngine::lambda_start() {
+  ACES_REQUIRE(ready_, "sym: not set up");
+  if (!Lam_) {
+    ACES_CHECK(alloc(&Lam_, nt_));
+    ACES_CHECK(alloc(&Lamn_, nt_));
+    if (!lam_) ACES_CHECK(alloc(&lam_, nt_));
+  }
+  ACES_CHECK(diis_grow(lsol_, lerr_, lorder_));
+  LR_.assign((size_t)lorder_ * lorder_, 0.0);
+  ACES_CHECK(forward(GP_F64));
+  // Lambda1 = 0 (vcc/init2.F:45-53), Lambda2 = T2 (vcc/vcc.F:320-325)
+  ACES_HIP(hipMemsetAsync(Lam_, 0, sizeof(double) * n1_, s_));
+  ACES_HIP(hipMemcpyAsync(Lam_ + n1_, T_cur_ + n1_, sizeof(double) * (nt_ - n1_), hipMemcpyDeviceToDevice, s_));
+  ACES_HIP(hipStreamSynchronize(s_));
+  lam_ready_ = true;
+  return ACES_OK;
+}
+
+int SymEngine::set_lambda_diis_order(int order) {
+  ACES_CHECK(diis_grow(lsol_, lerr_, order));
+  lorder_ = order;
+  LR_.assign((size_t)order * order, 0.0);
+  return ACES_OK;
+}
+
+// l1 = 2 L1, l2 = 2 L2 - L2(i<->j)
+int SymEngine::lam_from_aces(const double* L, double* lam) {
+  ACES_CHECK(eu::P4(1, 1, 1, n1_, lam).src(L, 2.0, 0, 0, 0, 1).run(s_));
+  const double* src[2] = {L + n1_, L + n1_};
+  const double al[2] = {2.0, -1.0};
+  return dpd::apply_plan(perm2_, lam + n1_, 0.0, src, al, nullptr, nullptr, 0.0, nullptr, s_);
+}
+
+int SymEngine::lambda_cycle(int icycle, double tol, CcsdStats& st) {
+  ACES_REQUIRE(lam_ready_, "sym lambda: call lambda_start first");
+  ACES_REQUIRE(fwd_valid_, "sym lambda: amplitudes changed since lambda_start");
+  hipEvent_t e0, e1;
+  ACES_HIP(hipEventCreate(&e0));
+  ACES_HIP(hipEventCreate(&e1));
+  ACES_HIP(hipEventRecord(e0, s_));
+  ACES_CHECK(lam_from_aces(Lam_, lam_));
+  ACES_CHECK(backward(lam_, lam_ + n1_, false));
+  ACES_CHECK(pair_symmetrize(gT_ + n1_));
+  // L1n = L1 + g1/(2 D1) ; L2n = L2 + (2 g2 + g2(i<->j)) / (3 D2)
+  ACES_HIP(hipMemcpyAsync(Lamn_, Lam_, sizeof(double) * nt_, hipMemcpyDeviceToDevice, s_));
+  ACES_CHECK(denom_apply(t1_, Lamn_, 1.0, gT_, 0.5, nullptr, 0.0, 0, -1));
+  ACES_CHECK(denom_apply(t2_, Lamn_ + n1_, 1.0, gT_ + n1_, 2.0 / 3.0, gT_ + n1_, 1.0 / 3.0, 1, -1));
+  ACES_HIP(hipEventRecord(e1, s_));
+  ACES_CHECK(diis_core(lsol_, lerr_, LR_, lorder_, Lam_, Lamn_, icycle, tol, true, st));
+  float ms = 0.f;
+  ACES_HIP(hipEventSynchronize(e1));
+  ACES_HIP(hipEventElapsedTime(&ms, e0, e1));
+  st.ms_increments = ms;
+  st.ms_cycle = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ACES_OK;
+}
+
+int SymEngine::get_lambda(double* l1, double* l2, int aces_normalisation) {
+  ACES_REQUIRE(lam_ready_, "sym lambda: no multipliers");
+  const double* src = Lam_;
+  if (!aces_normalisation) {
+    ACES_CHECK(lam_from_aces(Lam_, lam_));
+    src = lam_;
+  }
+  const long s1[2] = {v, 1};
+  const long s2[4] = {(long)o * v * v, (long)v * v, v, 1};
+  if (l1) {
+    ACES_HIP(hipMemsetAsync(l1, 0, sizeof(double) * o * v, s_));
+    ACES_CHECK(unpack_dense(t1_, src, l1, s1, 0));
+  }
+  if (l2) {
+    ACES_HIP(hipMemsetAsync(l2, 0, sizeof(double) * (long)o * o * v * v, s_));
+    ACES_CHECK(unpack_dense(t2_, src + n1_, l2, s2, 0));
+  }
+  return ACES_OK;
+}
+
+int SymEngine::density(double* dens) {
+  ACES_REQUIRE(lam_ready_, "sym density: no multipliers (call lambda_start)");
+  if (!fwd_valid_) ACES_CHECK(forward(GP_F64));
+  ACES_CHECK(lam_from_aces(Lam_, lam_));
+  ACES_CHECK(backward(lam_, lam_ + n1_, true));
+  ACES_CHECK(assemble_dense_density(dens));
+  ACES_HIP(hipStreamSynchronize(s_));
+  return ACES_OK;
+}
+
+}  // namespace aces

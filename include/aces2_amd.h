@@ -135,6 +135,41 @@ int aces_ccsd_set_lambda(void* handle, const double* d_l1, const double* d_l2, i
 int aces_ccsd_density(void* handle, double* d_dens);
 
 /* ------------------------------------------------------------------------
+ * Irrep-blocked CCSD (abelian point groups up to D2h, nirr = 1, 2, 4, 8).
+ * Replaces the same xvcc/xlambda/xdens loop bodies as the engine above on
+ * symmetry-blocked lists (the reference's DPD lists, acescore/aces_com_symloc.F,
+ * libr/inipck.F, contracted per irrep pair, e.g. vcc/ladab1.F, vcc/rngprd.f):
+ * every contraction term is one grouped MFMA launch over the irrep blocks.
+ * occ[g], vir[g]: orbitals per irrep; orbitals are irrep-sorted inside the
+ * occupied and the virtual space (irrep product = XOR of 0-based labels).
+ * Inputs and outputs are dense C1 arrays in that order, with the conventions
+ * of aces_ccsd_* (symmetry-forbidden elements ignored on input, zero on output).
+ * ---------------------------------------------------------------------- */
+int aces_sym_create(int nirr, const int* occ, const int* vir, void* stream, void** handle);
+int aces_sym_destroy(void* handle);
+int aces_sym_setup(void* handle, const double* d_fock, const double* d_ovov, const double* d_oovv,
+                   const double* d_ovoo, const double* d_oooo, const double* d_ovvv, const double* d_list233);
+int aces_sym_set_amplitudes(void* handle, const double* d_t1, const double* d_t2);
+int aces_sym_get_amplitudes(void* handle, double* d_t1, double* d_t2);
+int aces_sym_start_mp1(void* handle);
+int aces_sym_increments(void* handle, int prec, double* d_inc1, double* d_inc2);
+int aces_sym_cycle(void* handle, int icycle, int prec, double tol, double* stats);
+int aces_sym_set_diis_order(void* handle, int order);
+int aces_sym_reset_diis(void* handle);
+int aces_sym_energy(void* handle, double* e_out);
+int aces_sym_adjoint(void* handle, const double* d_l1, const double* d_l2, double* d_r1, double* d_r2,
+                     double* d_dens);
+int aces_sym_lambda_start(void* handle);
+int aces_sym_lambda_cycle(void* handle, int icycle, double tol, double* stats);
+int aces_sym_set_lambda_diis_order(void* handle, int order);
+int aces_sym_get_lambda(void* handle, double* d_l1, double* d_l2, int aces_norm);
+int aces_sym_density(void* handle, double* d_dens);
+int aces_sym_set_profiling(void* handle, int on);
+int aces_sym_profile(void* handle, double* out, int reset);
+long aces_sym_device_bytes(void* handle);
+long aces_sym_packed_t2(void* handle);
+
+/* ------------------------------------------------------------------------
  * Runtime, list store and record store (device-resident MOINTS/JOBARC).
  *   aces_init              acescore/aces_init.F:26 (lists + records, one stream)
  *   aces_set_symmetry      POP/VRT per irrep (COMMON /SYM/, records SYMPOPOA/VA)
