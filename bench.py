@@ -21,7 +21,15 @@ finite differences (h = 1e-4) for a Fock perturbation F + mu P and a rank-1
 integral perturbation (pq|rs) + mu x_pq x_rs, at fp64 amplitudes and at the
 amplitudes of the mixed form after its fp64 polish.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ccsd|lambda]
+With ``--workload sym`` the system is the symmetry-blocked form: D2h (8
+irreps), o=64, v=384 split over the irreps by a seeded multinomial (at least
+2 occupied and 16 virtual orbitals per irrep), integrals from the same DF
+generator with symmetry-forbidden elements zeroed; the step is one CC cycle of
+the irrep-blocked engine (one grouped MFMA launch per contraction term) in the
+mixed form, followed outside the timed region by the fp64 polish to 1e-9 and
+the Lambda equations on the same lists.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ccsd|lambda|sym]
 """
 import argparse
 import json
@@ -91,7 +99,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--workload', default='ccsd', choices=['ccsd', 'lambda'])
+    ap.add_argument('--workload', default='ccsd', choices=['ccsd', 'lambda', 'sym'])
     ap.add_argument('--o', type=int, default=None)
     ap.add_argument('--v', type=int, default=None)
     ap.add_argument('--seed', type=int, default=20260)
@@ -108,6 +116,8 @@ def main():
 
     if args.workload == 'lambda':
         return bench_lambda(args)
+    if args.workload == 'sym':
+        return bench_sym(args)
     import numpy as np
     import torch
     from aces2 import synth
@@ -455,6 +465,138 @@ def bench_lambda(args):
         result['cpu_baseline'] = cpu_baseline_lambda(o, v, args.seed)
     else:
         result['cpu_baseline'] = None
+    print(json.dumps(result), flush=True)
+
+
+def bench_sym(args):
+    """Symmetry-blocked grouped form (D2h, o=64, v=384)."""
+    import numpy as np
+    import torch
+    from aces2 import synth
+    from aces2.ccsd import SymCCSD
+
+    o = args.o or 64
+    v = args.v or 384
+    nirr = 8
+    log = lambda *a: print('[bench]', *a, file=sys.stderr, flush=True)
+    rng = np.random.default_rng(args.seed + 5)
+    occ = synth.irrep_split(o, nirr, 2, rng)
+    vir = synth.irrep_split(v, nirr, 16, rng)
+    t0 = time.perf_counter()
+    eps, B, irr = synth.sym_draw(occ, vir, args.seed)
+    log('irreps occ %s vir %s; factors drawn (%.1f s)' % (occ, vir, time.perf_counter() - t0))
+    blocks = synth.blocks_device(eps, B, o)
+    del B
+    torch.cuda.synchronize()
+    log('dense integral blocks on the GPU (%.1f s)' % (time.perf_counter() - t0))
+    cc = SymCCSD(occ, vir, blocks)
+    del blocks
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    log('irrep-blocked engine set up: %.1f GB, packed T2 %d of %d dense elements (%.1f s)' %
+        (cc.device_bytes / 1e9, cc.packed_t2, o * o * v * v, time.perf_counter() - t0))
+    prec = args.prec if args.prec != 'fp64' else 'bf16x3'
+
+    def timed_cycles(form):
+        cc.set_diis_order(6)
+        cc.reset_diis()
+        cc.start_mp1()
+        for ic in range(1, args.warmup + 1):
+            cc.cycle(ic, form, 1e-30)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        for ic in range(args.warmup + 1, args.warmup + args.steps + 1):
+            cc.cycle(ic, form, 1e-30)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - ta) / args.steps
+
+    s_mixed = timed_cycles(prec)
+    log('%s: %.4f s/cycle' % (prec, s_mixed))
+    s64 = timed_cycles('fp64')
+    log('fp64: %.4f s/cycle' % s64)
+    cc.set_profiling(True)
+    cc.profile(reset=True)
+    cc.start_mp1()
+    cc.reset_diis()
+    for ic in (1, 2):
+        cc.cycle(ic, prec, 1e-30)
+    prof = cc.profile(reset=True)
+    cc.set_profiling(False)
+    heavy = {k: prof[k] for k in ('ladder', 'wbuild', 'ring')}
+    ms = sum(p['ms'] for p in heavy.values())
+    fl = sum(p['flops'] for p in heavy.values())
+    nl = sum(p['launches'] for p in heavy.values())
+    execf = 3.0 if prec == 'bf16x3' else 1.0
+    peak = BF16_PEAK if prec == 'bf16x3' else FP32_PEAK
+    roofline = {'bound': 'mfma', 'kernel': 'grouped %s GEMM (8 irrep blocks per launch)' % prec,
+                'achieved': execf * fl / (ms * 1e-3) / 1e12, 'peak': peak / 1e12, 'unit': 'TFLOP/s',
+                'frac': execf * fl / (ms * 1e-3) / peak, 'traffic': None, 'avg_launch_ms': ms / max(nl, 1),
+                'launches_per_cycle': nl / 2, 'gemm_share_of_cycle': (ms / 2) / (s_mixed * 1e3),
+                'per_class': {k: {'TFLOP/s': execf * p['flops'] / max(p['ms'], 1e-9) / 1e9, 'ms': p['ms'] / 2}
+                              for k, p in heavy.items()},
+                'fp64_other': {'TFLOP/s': prof['other']['flops'] / max(prof['other']['ms'], 1e-9) / 1e9,
+                               'ms': prof['other']['ms'] / 2}}
+    log('roofline', json.dumps(roofline))
+
+    # convergence: mixed to 1e-6, fp64 polish to 1e-9; fp64 reference run
+    def converge(form, conv_lo, conv_hi):
+        cc.set_diis_order(6)
+        cc.reset_diis()
+        cc.start_mp1()
+        phase, tol, ic, n = form, 10.0 ** (-conv_lo), 0, {'lowp': 0, 'fp64': 0}
+        while ic < args.maxcyc:
+            ic += 1
+            st = cc.cycle(ic, phase, tol)
+            n['fp64' if phase == 'fp64' else 'lowp'] += 1
+            if st['converged']:
+                if phase != 'fp64':
+                    phase, tol = 'fp64', 10.0 ** (-conv_hi)
+                    continue
+                return st['final_energy'], n
+        raise RuntimeError('CCSD did not converge')
+
+    ta = time.perf_counter()
+    e64, n64 = converge('fp64', 9, 9)
+    t64 = time.perf_counter() - ta
+    ta = time.perf_counter()
+    emx, nmx = converge(prec, 6, 9)
+    tmx = time.perf_counter() - ta
+    log('fp64 %r (%s) %.1f s; mixed+polish %r (%s) %.1f s' % (e64, n64, t64, emx, nmx, tmx))
+    # Lambda on the same lists at the polished amplitudes
+    ta = time.perf_counter()
+    lh = cc.solve_lambda(maxcyc=args.maxcyc, conv=9, order=6)
+    tl = time.perf_counter() - ta
+    s_lam = float(np.median(lh['ms_cycle'])) / 1e3
+    D = cc.density()
+    trace = float(torch.trace(D))
+    log('lambda: %d cycles, %.1f s, median %.4f s/cycle; trace(D_corr) = %.2e' %
+        (lh['cycles'], tl, s_lam, trace))
+    cc.close()
+    result = {
+        'metric': 'CCSD s/iter, D2h irrep-blocked o=%d,v=%d (grouped MFMA), mixed vs FP64; Lambda s/iter' % (o, v),
+        'value': s_mixed,
+        'unit': 's/iter',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': s_mixed * 1e3,
+        'higher_is_better': False,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'bf16x3/fp32-acc + f64' if prec == 'bf16x3' else 'f32 + f64',
+        'data': 'synthetic (DF integrals with D2h symmetry, seed %d, sigma=0.05*sqrt(95/(o*v)))' % args.seed,
+        'config': {'workload': 'closed-shell CCSD cycle, RHF D2h (8 irreps), o=%d v=%d' % (o, v), 'o': o, 'v': v,
+                   'occ_per_irrep': occ, 'vir_per_irrep': vir, 'form': prec, 'diis': 6},
+        'roofline': roofline,
+        'fp64_s_per_iter': s64,
+        'speedup_mixed_vs_fp64': s64 / s_mixed,
+        'convergence': {'fp64': {'energy': e64, 'cycles': n64, 'total_s': t64},
+                        'mixed_polish': {'energy': emx, 'cycles': nmx, 'total_s': tmx},
+                        'abs_dE': abs(emx - e64)},
+        'lambda': {'cycles': lh['cycles'], 's_per_cycle_median': s_lam, 'total_s': tl,
+                   'density_trace_corr': trace},
+        'cpu_baseline': None,
+    }
     print(json.dumps(result), flush=True)
 
 
