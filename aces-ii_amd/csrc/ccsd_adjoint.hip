@@ -42,7 +42,7 @@ int CcsdEngine::adjoint_alloc() {
   ACES_CHECK(alloc(&Lam_, nt));
   ACES_CHECK(alloc(&Lamn_, nt));
   double** big[] = {&gtau_, &gU_, &gX_, &gUt_, &gXt_, &gWt_, &gWv_, &gRS_, &gR2_, &gP1_, &gP2_, &gQ12_, &gQ2_,
-                    &gZH_, &gtmpL_, &gtauK_, &gL1_, &gT1Q_, &gG4_};
+                    &gtmpL_, &gtauK_, &gL1_, &gT1Q_, &gG4_};
   for (double** p : big) ACES_CHECK(alloc(p, ovvo));
   ACES_CHECK(alloc(&gZp_, (nvp_ + ov) * nop_));
   ACES_CHECK(alloc(&gZm_, std::max<long>((nvm_ + ov) * nom_, 1)));
@@ -111,8 +111,8 @@ int CcsdEngine::compute_adjoint(const double* lam, bool density) {
   ACES_CHECK(gemm64(gd(o, v, v, l1, v, 1, Fvv_, v, 0, gt1, v, 1.0, 1.0)));
   ACES_CHECK(gemm64(gd(v, v, o, l1, v, 0, t1, v, 0, gFvvI_, v, 1.0, 0.0)));
   ACES_CHECK(P4(o, v, o, v, gU_).prod(l1, v, 1, 0, 0, Fov_, 0, 0, v, 1, 1.0).run(s));
-  ACES_CHECK(gemv_t(OV, OV, U_, ov, l1, gFovI_, 1.0, 0.0, s));
-  ACES_CHECK(gemv_t(OV, OV, W0t_, ov, l1, gt1, 1.0, 1.0, s));
+  ACES_CHECK(gemv_t(OV, OV, U_, ov, l1, gFovI_, 1.0, 0.0, s, ws_, ws_elems_));
+  ACES_CHECK(gemv_t(OV, OV, W0t_, ov, l1, gt1, 1.0, 1.0, s, ws_, ws_elems_));
   ACES_CHECK(gemm64(gd(o, (int)ov2, v, l1, v, 1, M3_, ov2, 0, gtau_, ov2, 1.0, 1.0)));
   ACES_CHECK(gemm64(gd(v, (int)o2v, o, l1, v, 0, M7_, o2v, 0, gtauK_, o2v, -1.0, 0.0)));
   // Gk = Foo + 2 FT - FovT
@@ -161,12 +161,11 @@ int CcsdEngine::compute_adjoint(const double* lam, bool density) {
   ACES_CHECK(P4(1, 1, 1, vv, gFvvI_, 1.0).src(gLvv_, 1.0, 0, 0, 0, 1).run(s));
   ACES_CHECK(gemm64(gd(o, v, v, fov, n, 1, gLvv_, v, 1, gt1, v, -1.0, 1.0)));
   if (density) ACES_CHECK(gemm64(gd(o, v, v, t1, v, 1, gLvv_, v, 0, gfov_, v, -1.0, 1.0)));
-  ACES_CHECK(gemv(OV, (int)vv, ovvv_, vv, gLvv_, gt1, 2.0, 1.0, s));
-  ACES_CHECK(contract2(o, v, v, v, ovvv_kacd_, vv * v, v, vv, 1, gLvv_, v, 1, gt1, -1.0, 1.0, s));
+  ACES_CHECK(gemv(OV, (int)vv, ovvvL_, vv, gLvv_, gt1, 1.0, 1.0, s));
   // Loo = Foo + FT + M1 t1
   ACES_CHECK(P4(1, 1, 1, oo, gFooI_, 1.0).src(gLoo_, 1.0, 0, 0, 0, 1).run(s));
   ACES_CHECK(P4(1, 1, 1, oo, gFT_, 1.0).src(gLoo_, 1.0, 0, 0, 0, 1).run(s));
-  ACES_CHECK(gemv_t((int)oo, OV, M1_, ov, gLoo_, gt1, 1.0, 1.0, s));
+  ACES_CHECK(gemv_t((int)oo, OV, M1_, ov, gLoo_, gt1, 1.0, 1.0, s, ws_, ws_elems_));
   // FovT = Fov t1^T ; FT = fov t1^T
   ACES_CHECK(gemm64(gd(o, v, o, gFovT_, o, 1, t1, v, 0, gFovI_, v, 1.0, 1.0)));
   ACES_CHECK(gemm64(gd(o, v, o, gFovT_, o, 0, Fov_, v, 0, gt1, v, 1.0, 1.0)));
@@ -184,9 +183,8 @@ int CcsdEngine::compute_adjoint(const double* lam, bool density) {
   ACES_CHECK(gemv(OV, OV, Lmat_, ov, gFovI_, gt1, 1.0, 1.0, s));
 
   // ---- (12) hole-hole ladder ----
-  ACES_CHECK(P4(v, v, o, o, gZH_).src(l2, 1.0, v, 1, ov2, vv).run(s));
-  ACES_CHECK(gemm64(gd((int)oo, (int)vv, (int)oo, Wo_, oo, 1, gZH_, oo, 1, gtau_, vv, 1.0, 1.0)));
-  ACES_CHECK(gemm64(gd((int)oo, (int)oo, (int)vv, tau_, vv, 1, gZH_, oo, 0, gWo_, oo, 1.0, 0.0)));
+  ACES_CHECK(gemm64(gd((int)oo, (int)vv, (int)oo, Wo_, oo, 1, l2, vv, 0, gtau_, vv, 1.0, 1.0)));
+  ACES_CHECK(gemm64(gd((int)oo, (int)oo, (int)vv, tau_, vv, 1, l2, vv, 1, gWo_, oo, 1.0, 0.0)));
   ACES_CHECK(gemm64(gd((int)oo, (int)vv, (int)oo, gWo_, oo, 0, OVkl_, vv, 0, gtau_, vv, 1.0, 1.0)));
   ACES_CHECK(P4(o, o, o, o, gQo_).src(gWo_, 1.0, o3, oo, o, 1).src(gWo_, 1.0, oo, o3, 1, o).run(s));
   ACES_CHECK(gemm64(gd(o, v, (int)o3, gQo_, o, 0, ovooQ_, v, 0, gt1, v, 1.0, 1.0)));
@@ -233,7 +231,7 @@ int CcsdEngine::compute_adjoint(const double* lam, bool density) {
   // H[(l,a)][(i,d)] = -2 gUt[i,a,l,d] + 2 gXt[i,a,l,d]
   ACES_CHECK(P4(o, v, o, v, gRS_).src(gUt_, -2.0, v, ov, ov2, 1).src(gXt_, 2.0, v, ov, ov2, 1).run(s));
   ACES_CHECK(gemv(OV, OV, gRS_, ov, t1, gt1, 1.0, 1.0, s));
-  ACES_CHECK(gemv_t(OV, OV, gRS_, ov, t1, gt1, 1.0, 1.0, s));
+  ACES_CHECK(gemv_t(OV, OV, gRS_, ov, t1, gt1, 1.0, 1.0, s, ws_, ws_elems_));
   // ---- (2) U = 2 t_ijab - t_ijba, X = t_ijba (with gU += gUt, gX += gXt) ----
   ACES_CHECK(P4(1, 1, 1, ovvo, gU_, 1.0).src(gUt_, 1.0, 0, 0, 0, 1).run(s));
   ACES_CHECK(P4(1, 1, 1, ovvo, gX_, 1.0).src(gXt_, 1.0, 0, 0, 0, 1).run(s));
@@ -245,7 +243,7 @@ int CcsdEngine::compute_adjoint(const double* lam, bool density) {
   // ---- (1) tau = t2 + t1 t1 ----
   ACES_CHECK(P4(o, v, o, v, gR2_).src(gtau_, 1.0, ov2, v, vv, 1).run(s));
   ACES_CHECK(gemv(OV, OV, gR2_, ov, t1, gt1, 1.0, 1.0, s));
-  ACES_CHECK(gemv_t(OV, OV, gR2_, ov, t1, gt1, 1.0, 1.0, s));
+  ACES_CHECK(gemv_t(OV, OV, gR2_, ov, t1, gt1, 1.0, 1.0, s, ws_, ws_elems_));
   ACES_CHECK(P4(1, 1, 1, ovvo, gt2, 1.0).src(gtau_, 1.0, 0, 0, 0, 1).run(s));
   return ACES_OK;
 }

@@ -90,12 +90,15 @@ class CcsdEngine {
   int alloc_raw(void** p, long bytes);
   int compute_increments(GemmPrec prec);
   int lowp_static(GemmPrec prec);
+  // Contraction with operands of the given precision; A64/B64 are the fp64
+  // sources (a_kc/b_kc as in GemmDesc), converted into their operand slot
+  // unless the slot is static.
   int gemm_lp(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda, const double* B64,
-              int bslot, long ldb, double* C, long ldc, double alpha, double beta);
+              int bslot, long ldb, double* C, long ldc, double alpha, double beta, int a_kc = 1, int b_kc = 1);
   int timed_gemm(const GemmDesc& d, GemmPrec prec, double flops);
   int gemm64(GemmDesc d);  // fp64 GEMM with the engine's split-K workspace (timed as prof_cat_)
   int gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda, const double* B64,
-                   int bslot, long ldb, double* C, long ldc, double alpha, double beta);
+                   int bslot, long ldb, double* C, long ldc, double alpha, double beta, int a_kc, int b_kc);
   struct DiisState {
     int order = 6;
     std::vector<double*> sol, err;
@@ -126,25 +129,29 @@ class CcsdEngine {
   double *Lo_ = nullptr, *Lv_ = nullptr, *OVkl_ = nullptr, *ooooP_ = nullptr, *ovooQ_ = nullptr;
   double *M1_ = nullptr, *M5c_ = nullptr, *M5b_ = nullptr, *M7_ = nullptr, *Lijab_ = nullptr;
   double *Vp_ = nullptr, *Vm_ = nullptr;
+  double *ovvvL_ = nullptr;  // [k][c][a][d] = 2(kc|ad) - (kd|ac): the t1 term of Lvv
   long nvp_, nvm_, nop_, nom_;
   // amplitudes and work
   double *t1_, *t2_, *t1n_, *t2n_, *inc1_, *inc2_, *tau_;
   double *U_, *X_, *Ut_, *Xt_, *Wt_, *Wv_, *R1_, *R2_;
   double *Sp_, *Dp_, *Zp_, *Zm_, *Yb_, *L1_, *tauK_, *P1_, *P2_, *Q12_, *Q2_, *Qb_, *T1Q_, *S3_, *G4A_, *G4_;
-  double *Wo_, *Qo_, *ZH_, *tmpL_;
+  double *Wo_, *Qo_, *tmpL_;
   double *Fov_, *Foo_, *Fvv_, *Loo_, *Lvv_, *FT_, *FovT_, *Gk_;
   double *red_ = nullptr;  // reduction scratch
   double *ws_ = nullptr;   // split-K workspace
   long ws_elems_ = 0;
   double *scal_ = nullptr; // device scalars
-  // low precision operand slots: 0 Vp 1 Vm 2 Lmat 3 Kx (static); 4 Sp 5 Dp 6 Ut 7 Xt 8 Wt 9 Wv 10 U 11 X
+  // low precision operand slots: static (converted once per precision)
+  // 0 Vp 1 Vm 2 Lmat 3 Kx 4 OVkl; per call 5 Sp 6 Dp 7 Ut 8 Xt 9 Wt 10 Wv
+  // 11 U 12 X 13 tau 14 Wo^T 15 tau^T
   struct LowP {
     void* hi = nullptr;
     void* lo = nullptr;
     long ld = 0;
     long rows = 0;
   };
-  LowP lp_[12];
+  static constexpr int kLpStatic = 5, kLpSlots = 16;
+  LowP lp_[kLpSlots];
   GemmPrec lp_static_prec_ = GP_F64;
   // DIIS (amplitudes and multipliers)
   DiisState tdiis_, ldiis_;
@@ -158,7 +165,7 @@ class CcsdEngine {
   bool fwd_valid_ = false;  // forward intermediates belong to the current t1_/t2_
   double *lam_ = nullptr, *gT_ = nullptr, *Lam_ = nullptr, *Lamn_ = nullptr;
   double *gtau_, *gU_, *gX_, *gUt_, *gXt_, *gWt_, *gWv_, *gRS_, *gR2_, *gP1_, *gP2_, *gQ12_, *gQ2_;
-  double *gZp_, *gZm_, *gSp_, *gDp_, *gYb_, *gL1_, *gWo_, *gQo_, *gZH_, *gtmpL_, *gT1Q_, *gQb_, *gG4_, *gG4A_;
+  double *gZp_, *gZm_, *gSp_, *gDp_, *gYb_, *gL1_, *gWo_, *gQo_, *gtmpL_, *gT1Q_, *gQb_, *gG4_, *gG4A_;
   double *gS3_, *gtauK_;
   double *gFovI_, *gFooI_, *gFvvI_, *gLoo_, *gLvv_, *gFT_, *gFovT_, *gGk_;
   double *gfo_, *gfv_, *gfov_, *geo_, *gev_;

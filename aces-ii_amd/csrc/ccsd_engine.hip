@@ -127,6 +127,8 @@ int CcsdEngine::setup(const double* fock, const double* ovov, const double* oovv
   ACES_CHECK(P4(o, v, v, v, ovvv_kacd_).src(ovvv_, 1, v3, v, 1, vv).run(s));
   ACES_CHECK(alloc(&M3_, ov * vv));  // [a][k][c][d] = 2(kd|ac) - (kc|ad)
   ACES_CHECK(P4(v, o, v, v, M3_).src(ovvv_, 2, v, v3, 1, vv).src(ovvv_, -1, v, v3, vv, 1).run(s));
+  ACES_CHECK(alloc(&ovvvL_, ov * vv));  // [k][c][a][d] = 2(kc|ad) - (kd|ac)
+  ACES_CHECK(P4(o, v, v, v, ovvvL_).src(ovvv_, 2, v3, vv, v, 1).src(ovvv_, -1, v3, 1, v, vv).run(s));
 
   // ladder operands
   ACES_CHECK(alloc(&Vp_, (nvp_ + ov) * nvp_));
@@ -169,7 +171,6 @@ int CcsdEngine::setup(const double* fock, const double* ovov, const double* oovv
   ACES_CHECK(alloc(&G4_, o2v * v));
   ACES_CHECK(alloc(&Wo_, oo * oo));
   ACES_CHECK(alloc(&Qo_, oo * oo));
-  ACES_CHECK(alloc(&ZH_, ovvo));
   ACES_CHECK(alloc(&tmpL_, ovvo));
   ACES_CHECK(alloc(&Fov_, ov));
   ACES_CHECK(alloc(&Foo_, oo));
@@ -265,10 +266,12 @@ int CcsdEngine::lowp_static(GemmPrec prec) {
   if (lp_static_prec_ == prec) return ACES_OK;
   const long ov = (long)o * v;
   struct S { int slot; const double* src; long rows; long K; long ld; };
-  const S tab[4] = {{0, Vp_, nvp_ + ov, nvp_, nvp_},
-                    {1, Vm_, nvm_ + ov, nvm_, std::max<long>(nvm_, 1)},
-                    {2, Lmat_, ov, ov, ov},
-                    {3, Kx_, ov, ov, ov}};
+  const long oo = (long)o * o, vv = (long)v * v;
+  const S tab[kLpStatic] = {{0, Vp_, nvp_ + ov, nvp_, nvp_},
+                            {1, Vm_, nvm_ + ov, nvm_, std::max<long>(nvm_, 1)},
+                            {2, Lmat_, ov, ov, ov},
+                            {3, Kx_, ov, ov, ov},
+                            {4, OVkl_, oo, vv, vv}};
   for (const S& t : tab) {
     LowP& L = lp_[t.slot];
     const long ld = pad8(std::max<long>(t.K, 1));
@@ -314,8 +317,9 @@ void CcsdEngine::profile(double* out, int reset) {
 }
 
 int CcsdEngine::gemm_lp(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda,
-                        const double* B64, int bslot, long ldb, double* C, long ldc, double alpha, double beta) {
-  return gemm_lp_impl(prec, M, N, K, A64, aslot, lda, B64, bslot, ldb, C, ldc, alpha, beta);
+                        const double* B64, int bslot, long ldb, double* C, long ldc, double alpha, double beta,
+                        int a_kc, int b_kc) {
+  return gemm_lp_impl(prec, M, N, K, A64, aslot, lda, B64, bslot, ldb, C, ldc, alpha, beta, a_kc, b_kc);
 }
 
 // Launch one MFMA contraction, timed with hipEvents when profiling is on.
@@ -340,9 +344,17 @@ int CcsdEngine::timed_gemm(const GemmDesc& d, GemmPrec prec, double flops) {
 
 int CcsdEngine::gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda,
                              const double* B64, int bslot, long ldb, double* C, long ldc, double alpha,
-                             double beta) {
+                             double beta, int a_kc, int b_kc) {
   const double flops = 2.0 * M * (double)N * K;
-  if (prec == GP_F64) return timed_gemm(gd(M, N, K, A64, lda, 1, B64, ldb, 1, C, ldc, alpha, beta), GP_F64, flops);
+  if (prec == GP_F64) {
+    GemmDesc d = gd(M, N, K, A64, lda, a_kc, B64, ldb, b_kc, C, ldc, alpha, beta);
+    d.ws = ws_;
+    d.ws_elems = ws_elems_;
+    return timed_gemm(d, GP_F64, flops);
+  }
+  ACES_REQUIRE(aslot >= 0 && aslot < kLpSlots && bslot >= 0 && bslot < kLpSlots, "gemm_lp: bad operand slot");
+  ACES_REQUIRE((aslot >= kLpStatic || a_kc) && (bslot >= kLpStatic || b_kc),
+               "gemm_lp: static operands are k-contiguous");
   ACES_CHECK(lowp_static(prec));
   const long Kp = pad8(std::max(K, 1));
   const void* ops[2][2];
@@ -350,9 +362,10 @@ int CcsdEngine::gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A
   const double* src[2] = {A64, B64};
   const long rows[2] = {M, N};
   const long lds[2] = {lda, ldb};
+  const int kcs[2] = {a_kc, b_kc};
   for (int q = 0; q < 2; ++q) {
     LowP& L = lp_[slots[q]];
-    if (slots[q] >= 4) {
+    if (slots[q] >= kLpStatic) {
       if (L.hi == nullptr || L.ld != Kp || L.rows < rows[q]) {
         void* blk;
         ACES_CHECK(alloc_raw(&blk, rows[q] * Kp * 4));
@@ -361,11 +374,16 @@ int CcsdEngine::gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A
         L.ld = Kp;
         L.rows = rows[q];
       }
-      if (prec == GP_F32)
+      if (prec == GP_F32 && kcs[q])
         ACES_CHECK(convert_f32((int)rows[q], K, src[q], lds[q], (float*)L.hi, Kp, s_));
-      else
+      else if (prec == GP_F32)
+        ACES_CHECK(convert_f32_t((int)rows[q], K, src[q], lds[q], (float*)L.hi, Kp, s_));
+      else if (kcs[q])
         ACES_CHECK(split_bf16((int)rows[q], K, src[q], lds[q], (unsigned short*)L.hi, (unsigned short*)L.lo, Kp,
                               s_));
+      else
+        ACES_CHECK(split_bf16_t((int)rows[q], K, src[q], lds[q], (unsigned short*)L.hi, (unsigned short*)L.lo, Kp,
+                                s_));
     }
     ops[q][0] = L.hi;
     ops[q][1] = L.lo;
@@ -375,6 +393,8 @@ int CcsdEngine::gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A
   d.A = ops[0][0]; d.A_lo = ops[0][1]; d.lda = lp_[aslot].ld; d.a_kc = 1;
   d.B = ops[1][0]; d.B_lo = ops[1][1]; d.ldb = lp_[bslot].ld; d.b_kc = 1;
   d.C = C; d.ldc = ldc; d.alpha = alpha; d.beta = beta;
+  d.ws = ws_;
+  d.ws_elems = ws_elems_;
   return timed_gemm(d, prec, flops);
 }
 
@@ -412,12 +432,12 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
                  .src(Q2_, -1, v, 1, ov2, ov).run(s));
   // T2 parts (MFMA): Wt += 1/2 Ut L^T ; Wv += -1/2 Xt K^T
   prof_cat_ = 1;
-  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Ut_, 6, ov, Lmat_, 2, ov, Wt_, ov, 0.5, 1.0));
-  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Xt_, 7, ov, Kx_, 3, ov, Wv_, ov, -0.5, 1.0));
+  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Ut_, 7, ov, Lmat_, 2, ov, Wt_, ov, 0.5, 1.0));
+  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Xt_, 8, ov, Kx_, 3, ov, Wv_, ov, -0.5, 1.0));
   // ring (MFMA): R1 = Wt U^T ; R2 = Wv X^T
   prof_cat_ = 2;
-  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Wt_, 8, ov, U_, 10, ov, R1_, ov, 1.0, 0.0));
-  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Wv_, 9, ov, X_, 11, ov, R2_, ov, 1.0, 0.0));
+  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Wt_, 9, ov, U_, 11, ov, R1_, ov, 1.0, 0.0));
+  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Wv_, 10, ov, X_, 12, ov, R2_, ov, 1.0, 0.0));
 
   // inc2 = <ij|ab> = (ia|jb)
   ACES_CHECK(P4(o, o, v, v, inc2_).src(ovov_, 1, ov2, v, ov, 1).run(s));
@@ -430,9 +450,9 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   // particle-particle ladder with the T1-dressing rows (MFMA)
   ACES_CHECK(pack_tau(o, v, tau_, Sp_, nvp_, Dp_, std::max<long>(nvm_, 1), s));
   prof_cat_ = 0;
-  ACES_CHECK(gemm_lp(prec, (int)(nvp_ + ov), (int)nop_, (int)nvp_, Vp_, 0, nvp_, Sp_, 4, nvp_, Zp_, nop_, 1.0, 0.0));
+  ACES_CHECK(gemm_lp(prec, (int)(nvp_ + ov), (int)nop_, (int)nvp_, Vp_, 0, nvp_, Sp_, 5, nvp_, Zp_, nop_, 1.0, 0.0));
   if (nom_ > 0 && nvm_ > 0)
-    ACES_CHECK(gemm_lp(prec, (int)(nvm_ + ov), (int)nom_, (int)nvm_, Vm_, 1, nvm_, Dp_, 5, nvm_, Zm_, nom_, 1.0,
+    ACES_CHECK(gemm_lp(prec, (int)(nvm_ + ov), (int)nom_, (int)nvm_, Vm_, 1, nvm_, Dp_, 6, nvm_, Zm_, nom_, 1.0,
                        0.0));
   else
     ACES_HIP(hipMemsetAsync(Zm_, 0, sizeof(double), s));
@@ -442,13 +462,15 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(gemm64(gd(v, (int)(v * oo), o, t1, v, 0, Yb_, o, 1, L1_, v * oo, 1.0, 0.0)));
   ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(L1_, -1, o, 1, vo2, oo).src(L1_, -1, 1, o, oo, vo2).run(s));
 
-  // hole-hole ladder: Wo = (ki|lj) + Qo[klij] + Qo[lkji] + (kc|ld) tau_ijcd ; inc2 += tau_klab Wo_klij
+  // hole-hole ladder: Wo = (ki|lj) + Qo[klij] + Qo[lkji] + (kc|ld) tau_ijcd ;
+  // inc2[ij][ab] += sum_kl Wo[kl][ij] tau[kl][ab]  (both products in the ladder precision)
   ACES_CHECK(gemm64(gd((int)(oo * o), o, v, ovooQ_, v, 1, t1, v, 1, Qo_, o, 1.0, 0.0)));
   ACES_CHECK(P4(o, o, o, o, Wo_).src(ooooP_, 1, oo * o, oo, o, 1).src(Qo_, 1, oo * o, oo, o, 1)
                  .src(Qo_, 1, oo, oo * o, 1, o).run(s));
-  ACES_CHECK(gemm64(gd((int)oo, (int)oo, (int)vv, OVkl_, vv, 1, tau_, vv, 1, Wo_, oo, 1.0, 1.0)));
-  ACES_CHECK(gemm64(gd((int)vv, (int)oo, (int)oo, tau_, vv, 0, Wo_, oo, 0, ZH_, oo, 1.0, 0.0)));
-  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(ZH_, 1, o, 1, vo2, oo).run(s));
+  prof_cat_ = 0;
+  ACES_CHECK(gemm_lp(prec, (int)oo, (int)oo, (int)vv, OVkl_, 4, vv, tau_, 13, vv, Wo_, oo, 1.0, 1.0));
+  ACES_CHECK(gemm_lp(prec, (int)oo, (int)vv, (int)oo, Wo_, 14, oo, tau_, 15, vv, inc2_, vv, 1.0, 1.0, 0, 0));
+  prof_cat_ = 3;
 
   // F intermediates (QUAD2/QUAD3/MAKFME + T1RING pieces)
   const double* fov = fock_ + o;
@@ -465,8 +487,7 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(gemv((int)oo, OV, M1_, ov, t1, Loo_, 1.0, 1.0, s));
   ACES_CHECK(P4(1, 1, v, v, Lvv_).src(Fvv_, 1, 0, 0, v, 1).run(s));
   ACES_CHECK(gemm64(gd(v, v, o, t1, v, 0, fov, n, 0, Lvv_, v, -1.0, 1.0)));
-  ACES_CHECK(gemv_t(OV, (int)vv, ovvv_, vv, t1, Lvv_, 2.0, 1.0, s));
-  ACES_CHECK(contract2(v, v, o, v, ovvv_kacd_, vv, 1, vv * v, v, t1, v, 1, Lvv_, -1.0, 1.0, s));
+  ACES_CHECK(gemv_t(OV, (int)vv, ovvvL_, vv, t1, Lvv_, 1.0, 1.0, s, ws_, ws_elems_));
   // diagonal (orbital energies) moved into the denominators
   ACES_CHECK(sub_diag(o, Foo_, o, eo_, s));
   ACES_CHECK(sub_diag(v, Fvv_, v, ev_, s));

@@ -207,23 +207,64 @@ int jacobi(int o, int v, const double* eo, const double* ev, const double* inc1,
 }
 
 // In-place SYMETRIZE_T2 on t2[i][j][a][b]: t <- (t_ijab + t_jiba)/2.
-__global__ void symmetrize_kernel(int o, int v, double* __restrict__ t2) {
-  const long vv = (long)v * v, n = (long)o * o * vv;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n; idx += (long)gridDim.x * blockDim.x) {
-    const long ij = idx / vv, ab = idx - ij * vv;
-    const int i = (int)(ij / o), j = (int)(ij % o), a = (int)(ab / v), b = (int)(ab % v);
-    const long tw = ((long)j * o + i) * vv + (long)b * v + a;
-    if (idx < tw) {
-      const double s = 0.5 * (t2[idx] + t2[tw]);
-      t2[idx] = s;
-      t2[tw] = s;
-    }
+// One block per (ij, a-tile, b-tile) with i < j, or i == j and a-tile <= b-tile:
+// it owns the 64x64 tile of block ij and its transposed partner in block ji,
+// reads both coalesced and writes both back through an LDS transpose.
+constexpr int kSymT = 64;
+
+__global__ __launch_bounds__(256) void symmetrize_kernel(int o, int v, double* __restrict__ t2) {
+  __shared__ double tp[kSymT][kSymT + 1];
+  const int nt = (v + kSymT - 1) / kSymT;
+  long b = blockIdx.x;
+  const int tb = (int)(b % nt);
+  b /= nt;
+  const int ta = (int)(b % nt);
+  b /= nt;
+  const int j = (int)(b % o);
+  const int i = (int)(b / o);
+  if (i > j || (i == j && ta > tb)) return;  // whole block: the partner block owns it
+  const long vv = (long)v * v;
+  double* A = t2 + ((long)i * o + j) * vv;  // A[a][b]
+  double* B = t2 + ((long)j * o + i) * vv;  // B[b][a]
+  const int a0 = ta * kSymT, b0 = tb * kSymT;
+  const int c = threadIdx.x & 63, r = threadIdx.x >> 6;
+  // B tile: rows b0.., columns a0.. (coalesced along a) into tp[b][a]
+  double areg[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int bb = b0 + r + 4 * k, aa = a0 + c;
+    tp[r + 4 * k][c] = (bb < v && aa < v) ? B[(long)bb * v + aa] : 0.0;
+  }
+  // A tile: rows a0.., columns b0.. (coalesced along b) into registers
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int aa = a0 + r + 4 * k, bb = b0 + c;
+    areg[k] = (aa < v && bb < v) ? A[(long)aa * v + bb] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) areg[k] = 0.5 * (areg[k] + tp[c][r + 4 * k]);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int aa = a0 + r + 4 * k, bb = b0 + c;
+    tp[c][r + 4 * k] = areg[k];
+    if (aa < v && bb < v) A[(long)aa * v + bb] = areg[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int bb = b0 + r + 4 * k, aa = a0 + c;
+    if (bb < v && aa < v) B[(long)bb * v + aa] = tp[r + 4 * k][c];
   }
 }
 
 int symmetrize_t2(int o, int v, double* t2, hipStream_t s) {
-  const long n = (long)o * o * v * v;
-  hipLaunchKernelGGL(symmetrize_kernel, dim3(grid1(n)), dim3(256), 0, s, o, v, t2);
+  const long nt = (v + kSymT - 1) / kSymT;
+  const long nblk = (long)o * o * nt * nt;
+  if (nblk == 0) return ACES_OK;
+  ACES_REQUIRE(nblk < (1L << 31), "symmetrize_t2: too many tiles");
+  hipLaunchKernelGGL(symmetrize_kernel, dim3((unsigned)nblk), dim3(256), 0, s, o, v, t2);
   ACES_HIP(hipGetLastError());
   return ACES_OK;
 }

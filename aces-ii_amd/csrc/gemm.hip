@@ -58,8 +58,6 @@ __device__ inline GemmArgs select_group(const GemmGroups& gg, int gi) {
   return p;
 }
 
-static constexpr int kBM = 128;
-static constexpr int kBN = 128;
 static constexpr int kGroupM = 8;
 static constexpr int kXcd = 8;
 
@@ -290,30 +288,43 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm16_kernel(GemmGroups gg) {
 
 // ---------------------------------------------------------------------------
 // bf16x3 kernel: A = Ahi + Alo, B = Bhi + Blo (bf16, k contiguous), BK = 32.
-// acc += Ahi*Bhi + Ahi*Blo + Alo*Bhi with v_mfma_f32_16x16x32_bf16.
+// acc += Alo*Bhi + Ahi*Blo + Ahi*Bhi with v_mfma_f32_16x16x32_bf16.
+// Block tile (64 WM) x 128 on 2 WM waves (each 64 x 64); WM = 4 halves the
+// operand bytes per flop that the block pulls through L2 (the bound of this
+// kernel) and is used when M is large.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmGroups gg) {
+template <int WM>
+__global__ __launch_bounds__(128 * WM) void gemm_bf16x3_kernel(GemmGroups gg) {
   constexpr int BK = 32;
-  constexpr int LDK = BK + 8;  // bf16 elements per LDS row (80 bytes)
-  __shared__ __attribute__((aligned(16))) unsigned short Ah[2][kBM * LDK];
-  __shared__ __attribute__((aligned(16))) unsigned short Al[2][kBM * LDK];
-  __shared__ __attribute__((aligned(16))) unsigned short Bh[2][kBN * LDK];
-  __shared__ __attribute__((aligned(16))) unsigned short Bl[2][kBN * LDK];
+  constexpr int BM = 64 * WM, BN = 128, NT = 128 * WM;
+  // LDS rows of 32 bf16 (64 bytes, no padding); the 16-byte chunk q of row r
+  // sits at position q ^ ((r >> 2) & 3), which keeps both the row-wise
+  // stores and the MFMA fragment reads free of bank conflicts.
+  constexpr int LDK = BK;
+  constexpr int LA = BM * 4 / NT, LB = BN * 4 / NT;  // row loads per thread and array
+  __shared__ __attribute__((aligned(16))) unsigned short Ah[2][BM * LDK];
+  __shared__ __attribute__((aligned(16))) unsigned short Al[2][BM * LDK];
+  __shared__ __attribute__((aligned(16))) unsigned short Bh[2][BN * LDK];
+  __shared__ __attribute__((aligned(16))) unsigned short Bl[2][BN * LDK];
 
   const int gi = (gg.n > 1) ? (int)blockIdx.y : 0;
   const GemmArgs p = select_group(gg, gi);
-  if ((int)blockIdx.x >= p.tiles_m * p.tiles_n) return;
+  if ((int)blockIdx.x >= p.tiles_m * p.tiles_n || (int)blockIdx.z >= p.splits) return;
   int tm, tn;
   tile_coords(p.tiles_m, p.tiles_n, tm, tn);
-  const int m0 = tm * kBM;
-  const int n0 = tn * kBN;
+  const int m0 = tm * BM;
+  const int n0 = tn * BN;
   const int bz = (gg.n > 1) ? 0 : (int)blockIdx.y;
-  const unsigned short* __restrict__ Ahg = reinterpret_cast<const unsigned short*>(p.A) + (long)bz * p.sA;
-  const unsigned short* __restrict__ Alg = reinterpret_cast<const unsigned short*>(p.Alo) + (long)bz * p.sA;
-  const unsigned short* __restrict__ Bhg = reinterpret_cast<const unsigned short*>(p.B) + (long)bz * p.sB;
-  const unsigned short* __restrict__ Blg = reinterpret_cast<const unsigned short*>(p.Blo) + (long)bz * p.sB;
-  double* __restrict__ C = p.C + (long)bz * p.sC;
-  const int M = p.M, N = p.N, K = p.K;
+  // split-K: k range [kz, kz + kchunk), kchunk a multiple of 32 (keeps the
+  // 16-byte row loads aligned)
+  const long kz = (long)blockIdx.z * p.kchunk;
+  const unsigned short* __restrict__ Ahg = reinterpret_cast<const unsigned short*>(p.A) + (long)bz * p.sA + kz;
+  const unsigned short* __restrict__ Alg = reinterpret_cast<const unsigned short*>(p.Alo) + (long)bz * p.sA + kz;
+  const unsigned short* __restrict__ Bhg = reinterpret_cast<const unsigned short*>(p.B) + (long)bz * p.sB + kz;
+  const unsigned short* __restrict__ Blg = reinterpret_cast<const unsigned short*>(p.Blo) + (long)bz * p.sB + kz;
+  double* __restrict__ C = p.C + (long)bz * p.sC + (long)blockIdx.z * p.zC;
+  const int M = p.M, N = p.N;
+  const int K = (int)((p.K - kz) < p.kchunk ? (p.K - kz) : p.kchunk);
 
   const int t = threadIdx.x;
   const int lane = t & 63;
@@ -321,18 +332,17 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmGroups gg) {
   const int wm = wave >> 1;
   const int wn = wave & 1;
 
-  // Loader: row = t >> 1 (0..127), 16 consecutive k from (t & 1) * 16, as two
-  // 16-byte chunks.
-  const int lrow = t >> 1;
-  const int lkb = (t & 1) * 16;
-  uint4 rah[2], ral[2], rbh[2], rbl[2];
+  // Loader: four consecutive lanes read one row's 32 k values (64 contiguous
+  // bytes, two whole 32-byte sectors) per array; rows (t >> 2) + c NT/4.
+  const int lrow = t >> 2;
+  const int lkb = (t & 3) * 8;
+  uint4 rah[LA], ral[LA], rbh[LB], rbl[LB];
   auto load = [&](int k0) {
-    const int m = m0 + lrow;
-    const int n = n0 + lrow;
+    const int k = k0 + lkb;
+    const bool ka = (k < K);
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int k = k0 + lkb + c * 8;
-      const bool ka = (k < K);
+    for (int c = 0; c < LA; ++c) {
+      const int m = m0 + lrow + (NT / 4) * c;
       if (m < M && ka) {
         rah[c] = *reinterpret_cast<const uint4*>(Ahg + (long)m * p.lda + k);
         ral[c] = *reinterpret_cast<const uint4*>(Alg + (long)m * p.lda + k);
@@ -340,6 +350,10 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmGroups gg) {
         rah[c] = make_uint4(0, 0, 0, 0);
         ral[c] = make_uint4(0, 0, 0, 0);
       }
+    }
+#pragma unroll
+    for (int c = 0; c < LB; ++c) {
+      const int n = n0 + lrow + (NT / 4) * c;
       if (n < N && ka) {
         rbh[c] = *reinterpret_cast<const uint4*>(Bhg + (long)n * p.ldb + k);
         rbl[c] = *reinterpret_cast<const uint4*>(Blg + (long)n * p.ldb + k);
@@ -349,12 +363,18 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmGroups gg) {
       }
     }
   };
+  // (NT / 4) * c is a multiple of 16, so the swizzle of row lrow + (NT/4) c is that of lrow
+  const int soff = lrow * LDK + (((t & 3) ^ ((lrow >> 2) & 3)) << 3);
   auto store = [&](int buf) {
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int off = lrow * LDK + lkb + c * 8;
+    for (int c = 0; c < LA; ++c) {
+      const int off = soff + (NT / 4) * c * LDK;
       *reinterpret_cast<uint4*>(&Ah[buf][off]) = rah[c];
       *reinterpret_cast<uint4*>(&Al[buf][off]) = ral[c];
+    }
+#pragma unroll
+    for (int c = 0; c < LB; ++c) {
+      const int off = soff + (NT / 4) * c * LDK;
       *reinterpret_cast<uint4*>(&Bh[buf][off]) = rbh[c];
       *reinterpret_cast<uint4*>(&Bl[buf][off]) = rbl[c];
     }
@@ -372,19 +392,20 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GemmGroups gg) {
   __syncthreads();
   const int lr = lane & 15;
   const int lk = lane >> 4;
+  const int rsw = (lk ^ ((lr >> 2) & 3)) << 3;  // swizzled chunk of k values lk*8..lk*8+7
   for (int kt = 0; kt < nk; ++kt) {
     const int buf = kt & 1;
     if (kt + 1 < nk) load((kt + 1) * BK);
     s8_t ah[4], al[4], bh[4], bl[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int off = (wm * 64 + i * 16 + lr) * LDK + lk * 8;
+      const int off = (wm * 64 + i * 16 + lr) * LDK + rsw;
       ah[i] = *reinterpret_cast<const s8_t*>(&Ah[buf][off]);
       al[i] = *reinterpret_cast<const s8_t*>(&Al[buf][off]);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int off = (wn * 64 + j * 16 + lr) * LDK + lk * 8;
+      const int off = (wn * 64 + j * 16 + lr) * LDK + rsw;
       bh[j] = *reinterpret_cast<const s8_t*>(&Bh[buf][off]);
       bl[j] = *reinterpret_cast<const s8_t*>(&Bl[buf][off]);
     }
@@ -476,19 +497,20 @@ static void launch16(const GemmGroups& a, bool akc, bool bkc, dim3 grid, hipStre
 #undef L16
 }
 
-// Fill the kernel arguments of one problem; split-K (fp64/fp32 kernels only)
-// when the output cannot fill the chip and K is long, with partial products
+// Fill the kernel arguments of one problem; split-K when the output cannot
+// fill the chip and K is long, with partial products
 // in ws[ws_off ...].  Returns the number of splits.
 static int prepare(const GemmDesc& d, GemmPrec prec, GemmArgs& a, double* ws, long ws_avail, long ws_off,
-                   long tiles_total) {
+                   long tiles_total, int bf_wm) {
   a.M = d.M; a.N = d.N; a.K = d.K;
   a.A = d.A; a.Alo = d.A_lo; a.lda = d.lda;
   a.B = d.B; a.Blo = d.B_lo; a.ldb = d.ldb;
   a.C = d.C; a.ldc = d.ldc;
   a.alpha = d.alpha; a.beta = d.beta;
   a.sA = d.sA; a.sB = d.sB; a.sC = d.sC;
-  const int bm = (prec == GP_BF16X3) ? kBM : (prec == GP_F32 ? TileCfg<float>::type::BM : TileCfg<double>::type::BM);
-  const int bn = (prec == GP_BF16X3) ? kBN : (prec == GP_F32 ? TileCfg<float>::type::BN : TileCfg<double>::type::BN);
+  const int bm =
+      (prec == GP_BF16X3) ? 64 * bf_wm : (prec == GP_F32 ? TileCfg<float>::type::BM : TileCfg<double>::type::BM);
+  const int bn = (prec == GP_BF16X3) ? 128 : (prec == GP_F32 ? TileCfg<float>::type::BN : TileCfg<double>::type::BN);
   a.tiles_m = (d.M + bm - 1) / bm;
   a.tiles_n = (d.N + bn - 1) / bn;
   {
@@ -505,7 +527,7 @@ static int prepare(const GemmDesc& d, GemmPrec prec, GemmArgs& a, double* ws, lo
   a.zC = 0;
   a.splits = 1;
   int splits = 1;
-  if (prec != GP_BF16X3 && ws != nullptr && d.K >= 1024 && tiles_total < 512) {
+  if (ws != nullptr && d.K >= 1024 && tiles_total < 512) {
     long sp = 1024 / tiles_total;
     const long kmax = d.K / 512;
     if (sp > kmax) sp = kmax;
@@ -550,7 +572,20 @@ static int check_desc(const GemmDesc& d, GemmPrec prec) {
   return ACES_OK;
 }
 
-static int launch_groups(const GemmGroups& gg, GemmPrec prec, bool akc, bool bkc, bool kzero, dim3 grid,
+// bf16x3 block rows / 64: 4 (256 x 128 tiles) for tall, narrow problems
+// (every non-empty one with M >= 1024 and N < 4096: the particle ladder
+// shapes, where fewer operand bytes per flop pay), else 2 (128 x 128).
+static int bf16_wm(const GemmDesc* d, int ng) {
+  bool any = false;
+  for (int g = 0; g < ng; ++g)
+    if (d[g].M > 0 && d[g].N > 0) {
+      if (d[g].M < 1024 || d[g].N >= 4096) return 2;
+      any = true;
+    }
+  return any ? 4 : 2;
+}
+
+static int launch_groups(const GemmGroups& gg, GemmPrec prec, bool akc, bool bkc, bool kzero, dim3 grid, int bf_wm,
                          hipStream_t stream) {
   if (kzero) {
     // C = beta * C via a K = 0 launch of the fp64 kernel (acc stays 0).
@@ -564,7 +599,10 @@ static int launch_groups(const GemmGroups& gg, GemmPrec prec, bool akc, bool bkc
         launch16<float>(gg, akc, bkc, grid, stream);
         break;
       case GP_BF16X3:
-        hipLaunchKernelGGL(gemm_bf16x3_kernel, grid, dim3(256), 0, stream, gg);
+        if (bf_wm == 4)
+          hipLaunchKernelGGL(gemm_bf16x3_kernel<4>, grid, dim3(512), 0, stream, gg);
+        else
+          hipLaunchKernelGGL(gemm_bf16x3_kernel<2>, grid, dim3(256), 0, stream, gg);
         break;
       default:
         ACES_REQUIRE(false, "gemm: unknown precision");
@@ -592,15 +630,16 @@ int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream) {
   gg.n = 1;
   GemmArgs& a = gg.g[0];
   // tiles for the split decision use this precision's block size
+  const int wm = bf16_wm(&d, 1);
   GemmArgs probe;
-  prepare(d, prec, probe, nullptr, 0, 0, 1L << 30);
+  prepare(d, prec, probe, nullptr, 0, 0, 1L << 30, wm);
   const long nb = (long)probe.tiles_m * probe.tiles_n;
   ACES_REQUIRE(nb < (1L << 31) && d.batch < 65536, "gemm: grid too large");
-  const int splits = prepare(d, prec, a, d.ws, d.ws_elems, 0, nb * d.batch);
+  const int splits = prepare(d, prec, a, d.ws, d.ws_elems, 0, nb * d.batch, wm);
   for (int k = 1; k < kMaxGroups; ++k) gg.g[k] = a;
   dim3 grid((unsigned)nb, (unsigned)d.batch, (unsigned)splits);
   if (d.K == 0) a.K = 0;
-  ACES_CHECK(launch_groups(gg, prec, d.a_kc != 0, d.b_kc != 0, d.K == 0, grid, stream));
+  ACES_CHECK(launch_groups(gg, prec, d.a_kc != 0, d.b_kc != 0, d.K == 0, grid, wm, stream));
   return reduce_splits(d, a, stream);
 }
 
@@ -612,6 +651,7 @@ int gemm_grouped(const GemmDesc* d, int ng, GemmPrec prec, hipStream_t stream) {
   long maxnb = 0, ws_off = 0;
   int maxsplit = 1;
   bool any = false;
+  const int wm = bf16_wm(d, ng);
   for (int g = 0; g < ng; ++g) {
     ACES_CHECK(check_desc(d[g], prec));
     ACES_REQUIRE(d[g].batch == 1, "gemm_grouped: batched groups are not supported");
@@ -619,10 +659,10 @@ int gemm_grouped(const GemmDesc* d, int ng, GemmPrec prec, hipStream_t stream) {
     ACES_REQUIRE((d[g].K == 0) == (d[0].K == 0) || d[g].M == 0 || d[g].N == 0,
                  "gemm_grouped: mixed K = 0 and K > 0 groups");
     GemmArgs probe;
-    prepare(d[g], prec, probe, nullptr, 0, 0, 1L << 30);
+    prepare(d[g], prec, probe, nullptr, 0, 0, 1L << 30, wm);
     const long nb = (d[g].M == 0 || d[g].N == 0) ? 0 : (long)probe.tiles_m * probe.tiles_n;
     const long avail = d[0].ws ? d[0].ws_elems - ws_off : 0;
-    const int sp = prepare(d[g], prec, gg.g[g], d[0].ws, avail, ws_off, nb);
+    const int sp = prepare(d[g], prec, gg.g[g], d[0].ws, avail, ws_off, nb, wm);
     if (sp > 1) ws_off += (long)sp * d[g].M * d[g].N;
     if (nb == 0) gg.g[g].tiles_m = 0;  // empty block: every workgroup exits
     if (d[g].K == 0) gg.g[g].K = 0;
@@ -637,7 +677,7 @@ int gemm_grouped(const GemmDesc* d, int ng, GemmPrec prec, hipStream_t stream) {
   for (int g = 0; g < ng; ++g)
     if (d[g].M > 0 && d[g].N > 0) kz = (d[g].K == 0);
   dim3 grid((unsigned)maxnb, (unsigned)ng, (unsigned)maxsplit);
-  ACES_CHECK(launch_groups(gg, prec, d[0].a_kc != 0, d[0].b_kc != 0, kz != 0, grid, stream));
+  ACES_CHECK(launch_groups(gg, prec, d[0].a_kc != 0, d[0].b_kc != 0, kz != 0, grid, wm, stream));
   for (int g = 0; g < ng; ++g)
     if (d[g].M > 0 && d[g].N > 0) ACES_CHECK(reduce_splits(d[g], gg.g[g], stream));
   return ACES_OK;

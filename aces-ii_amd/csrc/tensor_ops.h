@@ -9,6 +9,11 @@ int convert_f32(int rows, int cols, const double* x, long ldx, float* y, long ld
 // hi = bf16(x), lo = bf16(x - hi) (round to nearest even); padding columns zeroed.
 int split_bf16(int rows, int cols, const double* x, long ldx, unsigned short* hi, unsigned short* lo,
                long ldy, hipStream_t s);
+// The same conversions for an operand stored k-major, x[c*ldx + r] = element
+// (r, c): the output is the transposed, k-contiguous y[r*ldy + c].
+int convert_f32_t(int rows, int cols, const double* x, long ldx, float* y, long ldy, hipStream_t s);
+int split_bf16_t(int rows, int cols, const double* x, long ldx, unsigned short* hi, unsigned short* lo, long ldy,
+                 hipStream_t s);
 
 // Generic 4-index permute/combine on a row-major output of dims d[0..3]:
 //   out[i] = beta*out[i] + sum_s alpha[s] * in[s][ sum_k i_k * st[s][k] ]
@@ -32,9 +37,11 @@ int permute4(const Permute4& p, hipStream_t s);
 // y[m] = alpha * sum_k A[m*lda + k] x[k] + beta * y[m]
 int gemv(int M, int K, const double* A, long lda, const double* x, double* y, double alpha, double beta,
          hipStream_t s);
-// y[n] = alpha * sum_k A[k*lda + n] x[k] + beta * y[n]   (column access, coalesced over n)
+// y[n] = alpha * sum_k A[k*lda + n] x[k] + beta * y[n]   (column access, coalesced over n).
+// With a workspace (ws, ws_elems doubles) k is split over blocks and the slice
+// sums are added in a second, deterministic pass.
 int gemv_t(int K, int N, const double* A, long lda, const double* x, double* y, double alpha, double beta,
-           hipStream_t s);
+           hipStream_t s, double* ws = nullptr, long ws_elems = 0);
 
 // Two-index strided contraction over a 4-index array:
 //   out[p*P1 + q] = beta*out + alpha * sum_{r<R, s<S} X[p*xp + q*xq + r*xr + s*xs] * Y[r*yr + s*ys]

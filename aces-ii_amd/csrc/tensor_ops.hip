@@ -1,4 +1,5 @@
 // HBM-bound tensor kernels for the CCSD amplitude path (gfx950).
+#include <algorithm>
 #include "common.h"
 #include "tensor_ops.h"
 
@@ -65,6 +66,64 @@ int split_bf16(int rows, int cols, const double* x, long ldx, unsigned short* hi
   return ACES_OK;
 }
 
+// Transposing conversions for operands stored k-major: x[c*ldx + r] holds
+// element (r, c); y[r*ldy + c] is written for c < ldy (columns cols..ldy-1
+// zero).  64 x 64 tiles pass through LDS so that both sides are coalesced.
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void convert_t_kernel(int rows, int cols, const double* __restrict__ x, long ldx,
+                                                        void* __restrict__ y0, void* __restrict__ y1, long ldy) {
+  __shared__ double tile[64][65];
+  const long ntr = (rows + 63) / 64;
+  const long r0 = (blockIdx.x % ntr) * 64;
+  const long c0 = (blockIdx.x / ntr) * 64;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const long c = c0 + w + 4 * k, r = r0 + l;
+    tile[w + 4 * k][l] = (c < cols && r < rows) ? x[c * ldx + r] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const long r = r0 + w + 4 * k, c = c0 + l;
+    if (r < rows && c < ldy) {
+      const double v = tile[l][w + 4 * k];
+      if (SPLIT) {
+        const unsigned short h = bf16_rn((float)v);
+        reinterpret_cast<unsigned short*>(y0)[r * ldy + c] = h;
+        reinterpret_cast<unsigned short*>(y1)[r * ldy + c] = bf16_rn((float)(v - (double)bf16_to_f(h)));
+      } else {
+        reinterpret_cast<float*>(y0)[r * ldy + c] = (float)v;
+      }
+    }
+  }
+}
+
+static int convert_t_launch(bool split, int rows, int cols, const double* x, long ldx, void* y0, void* y1, long ldy,
+                            hipStream_t s) {
+  ACES_REQUIRE(ldy >= cols && ldx >= rows, "transposing conversion: bad leading dimension");
+  if (rows == 0 || ldy == 0) return ACES_OK;
+  const long nblk = ((rows + 63) / 64) * ((ldy + 63) / 64);
+  ACES_REQUIRE(nblk < (1L << 31), "transposing conversion: too many tiles");
+  if (split)
+    hipLaunchKernelGGL(convert_t_kernel<true>, dim3((unsigned)nblk), dim3(256), 0, s, rows, cols, x, ldx, y0, y1,
+                       ldy);
+  else
+    hipLaunchKernelGGL(convert_t_kernel<false>, dim3((unsigned)nblk), dim3(256), 0, s, rows, cols, x, ldx, y0, y1,
+                       ldy);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+int convert_f32_t(int rows, int cols, const double* x, long ldx, float* y, long ldy, hipStream_t s) {
+  return convert_t_launch(false, rows, cols, x, ldx, y, nullptr, ldy, s);
+}
+
+int split_bf16_t(int rows, int cols, const double* x, long ldx, unsigned short* hi, unsigned short* lo, long ldy,
+                 hipStream_t s) {
+  return convert_t_launch(true, rows, cols, x, ldx, hi, lo, ldy, s);
+}
+
 // ---------------------------------------------------------------------------
 struct Permute4Args {
   long d1, d2, d3;
@@ -104,15 +163,98 @@ __global__ void permute4_kernel(Permute4Args p) {
   }
 }
 
-int permute4(const Permute4& q, hipStream_t s) {
+// Tiled form, used when a source is read across the output's fastest index:
+// the output is walked in tiles of 64 (index 3) x 32 (index q, one of 0..2).
+// A source whose unit stride lies on index q is read along q (coalesced) and
+// transposed through LDS; every other source and the product term are read
+// in the output's order.
+constexpr int kPT3 = 64, kPTQ = 32;
+
+struct Permute4Tile {
+  long d[4];
+  long os[4];  // output strides (row major)
+  int q, ra, rb;
+  long n3, nq, ntiles;
+  double* out;
+  double beta;
+  int nd, nt;  // sources 0..nd-1 read directly, nd..nd+nt-1 through LDS
+  const double* in[3];
+  double alpha[3];
+  long st[3][4];
+  const double* X;
+  const double* Y;
+  double gamma;
+  long xs[4], ys[4];
+};
+
+__global__ __launch_bounds__(256) void permute4_tiled_kernel(Permute4Tile p) {
+  __shared__ double tile[kPT3][kPTQ + 1];
+  const int tid = threadIdx.x;
+  const int c3 = tid & 63, r0 = tid >> 6;  // output walk: rows r0 + 4k, column c3
+  const int cq = tid & 31, s0 = tid >> 5;  // transposed read: rows s0 + 8k, column cq
+  const long dq = p.d[p.q], d3 = p.d[3];
+  for (long t = blockIdx.x; t < p.ntiles; t += gridDim.x) {
+    long b = t;
+    const long t3 = b % p.n3;
+    b /= p.n3;
+    const long tq = b % p.nq;
+    b /= p.nq;
+    const long ib = b % p.d[p.rb];
+    const long ia = b / p.d[p.rb];
+    const long i3b = t3 * kPT3, iqb = tq * kPTQ;
+    const bool ok3 = i3b + c3 < d3;
+    double acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+    for (int s = 0; s < p.nd; ++s) {
+      const long* st = p.st[s];
+      const double* src = p.in[s] + ia * st[p.ra] + ib * st[p.rb] + (iqb + r0) * st[p.q] + (i3b + c3) * st[3];
+      const long step = 4 * st[p.q];
+      const double al = p.alpha[s];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (ok3 && iqb + r0 + 4 * k < dq) acc[k] += al * src[k * step];
+    }
+    if (p.X) {
+      const double* xp = p.X + ia * p.xs[p.ra] + ib * p.xs[p.rb] + (iqb + r0) * p.xs[p.q] + (i3b + c3) * p.xs[3];
+      const double* yp = p.Y + ia * p.ys[p.ra] + ib * p.ys[p.rb] + (iqb + r0) * p.ys[p.q] + (i3b + c3) * p.ys[3];
+      const long xstep = 4 * p.xs[p.q], ystep = 4 * p.ys[p.q];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (ok3 && iqb + r0 + 4 * k < dq) acc[k] += p.gamma * xp[k * xstep] * yp[k * ystep];
+    }
+    const bool okq = iqb + cq < dq;
+    for (int s = p.nd; s < p.nd + p.nt; ++s) {
+      const long* st = p.st[s];
+      const double* src = p.in[s] + ia * st[p.ra] + ib * st[p.rb] + (iqb + cq) + (i3b + s0) * st[3];
+      const long step = 8 * st[3];
+      __syncthreads();  // the previous reader of the tile is done
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        tile[s0 + 8 * k][cq] = (okq && i3b + s0 + 8 * k < d3) ? src[k * step] : 0.0;
+      __syncthreads();
+      const double al = p.alpha[s];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] += al * tile[c3][r0 + 4 * k];
+    }
+    double* o = p.out + ia * p.os[p.ra] + ib * p.os[p.rb] + (iqb + r0) * p.os[p.q] + i3b + c3;
+    const long ostep = 4 * p.os[p.q];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (ok3 && iqb + r0 + 4 * k < dq) {
+        double v = acc[k];
+        if (p.beta != 0.0) v += p.beta * o[k * ostep];
+        o[k * ostep] = v;
+      }
+  }
+}
+
+static int permute4_simple(const Permute4& q, hipStream_t s) {
   Permute4Args a;
   a.d1 = q.d[1];
   a.d2 = q.d[2];
   a.d3 = q.d[3];
   a.total = q.d[0] * q.d[1] * q.d[2] * q.d[3];
-  if (a.total == 0) return ACES_OK;
-  ACES_REQUIRE(q.out != nullptr, "permute4: null output");
-  ACES_REQUIRE(q.X == nullptr || q.Y != nullptr, "permute4: product term needs X and Y");
   a.out = q.out;
   a.beta = q.beta;
   a.in0 = q.in[0];
@@ -136,6 +278,93 @@ int permute4(const Permute4& q, hipStream_t s) {
   return ACES_OK;
 }
 
+// Index (0..2) on which source s has unit stride and that a tile can span, or
+// 3 when the source is read well enough in the output's order.
+static int permute4_fast_index(const Permute4& q, int s) {
+  const long s3 = q.st[s][3] < 0 ? -q.st[s][3] : q.st[s][3];
+  if (s3 < 8) return 3;
+  for (int k = 2; k >= 0; --k)
+    if (q.st[s][k] == 1 && q.d[k] >= 8) return k;
+  return 3;
+}
+
+int permute4(const Permute4& q, hipStream_t s) {
+  const long total = q.d[0] * q.d[1] * q.d[2] * q.d[3];
+  if (total == 0) return ACES_OK;
+  ACES_REQUIRE(q.out != nullptr, "permute4: null output");
+  ACES_REQUIRE(q.X == nullptr || q.Y != nullptr, "permute4: product term needs X and Y");
+  int fast[3] = {3, 3, 3};
+  bool any = false, alias = (q.X == q.out || q.Y == q.out);
+  for (int k = 0; k < 3; ++k)
+    if (q.in[k]) {
+      fast[k] = permute4_fast_index(q, k);
+      any |= fast[k] < 3;
+      alias |= q.in[k] == q.out;
+    }
+  if (!any || q.d[3] < 16 || total < 65536) return permute4_simple(q, s);
+  // one launch per distinct tile index; the first also takes the direct
+  // sources, the product term and beta, the later ones accumulate
+  int qs[3], nqs = 0;
+  for (int k = 0; k < 3; ++k)
+    if (q.in[k] && fast[k] < 3) {
+      bool seen = false;
+      for (int j = 0; j < nqs; ++j) seen |= qs[j] == fast[k];
+      if (!seen) qs[nqs++] = fast[k];
+    }
+  if (nqs > 1 && alias) return permute4_simple(q, s);
+  for (int L = 0; L < nqs; ++L) {
+    Permute4Tile t;
+    for (int k = 0; k < 4; ++k) t.d[k] = q.d[k];
+    t.os[3] = 1;
+    t.os[2] = q.d[3];
+    t.os[1] = q.d[2] * q.d[3];
+    t.os[0] = q.d[1] * q.d[2] * q.d[3];
+    t.q = qs[L];
+    t.ra = (t.q == 0) ? 1 : 0;
+    t.rb = (t.q == 2) ? 1 : 2;
+    t.n3 = (q.d[3] + kPT3 - 1) / kPT3;
+    t.nq = (q.d[t.q] + kPTQ - 1) / kPTQ;
+    t.ntiles = t.n3 * t.nq * q.d[t.ra] * q.d[t.rb];
+    t.out = q.out;
+    t.beta = (L == 0) ? q.beta : 1.0;
+    t.nd = 0;
+    t.nt = 0;
+    int slot = 0;
+    if (L == 0)
+      for (int k = 0; k < 3; ++k)
+        if (q.in[k] && fast[k] == 3) {
+          t.in[slot] = q.in[k];
+          t.alpha[slot] = q.alpha[k];
+          for (int j = 0; j < 4; ++j) t.st[slot][j] = q.st[k][j];
+          ++slot;
+          ++t.nd;
+        }
+    for (int k = 0; k < 3; ++k)
+      if (q.in[k] && fast[k] == t.q) {
+        t.in[slot] = q.in[k];
+        t.alpha[slot] = q.alpha[k];
+        for (int j = 0; j < 4; ++j) t.st[slot][j] = q.st[k][j];
+        ++slot;
+        ++t.nt;
+      }
+    for (int k = slot; k < 3; ++k) {
+      t.in[k] = nullptr;
+      t.alpha[k] = 0.0;
+      for (int j = 0; j < 4; ++j) t.st[k][j] = 0;
+    }
+    t.X = (L == 0) ? q.X : nullptr;
+    t.Y = (L == 0) ? q.Y : nullptr;
+    t.gamma = q.gamma;
+    for (int j = 0; j < 4; ++j) {
+      t.xs[j] = q.xs[j];
+      t.ys[j] = q.ys[j];
+    }
+    hipLaunchKernelGGL(permute4_tiled_kernel, dim3(grid_for(t.ntiles, 1, 1L << 20)), dim3(256), 0, s, t);
+    ACES_HIP(hipGetLastError());
+  }
+  return ACES_OK;
+}
+
 // ---------------------------------------------------------------------------
 // GEMV: one wave per row, lanes stride over k.
 __global__ void gemv_kernel(int M, int K, const double* __restrict__ A, long lda, const double* __restrict__ x,
@@ -153,21 +382,37 @@ __global__ void gemv_kernel(int M, int K, const double* __restrict__ A, long lda
 }
 
 __global__ void gemv_t_kernel(int K, int N, const double* __restrict__ A, long lda, const double* __restrict__ x,
-                              double* __restrict__ y, double alpha, double beta) {
-  // Block handles 256 columns and a slice of k; slices are summed with a
-  // second pass through shared memory (blockDim.y partial rows).
-  __shared__ double part[4][256];
+                              double* __restrict__ y, double alpha, double beta, int kchunk,
+                              double* __restrict__ part) {
+  // Block (256 x 4) handles 256 columns and the k slice [by*kchunk, (by+1)*kchunk);
+  // its four rows of threads stride that slice and are summed through LDS.
+  // With part != nullptr the block writes its slice sum to part[by][n].
+  __shared__ double red[4][256];
   const int n = blockIdx.x * 256 + threadIdx.x;
+  const long k0 = (long)blockIdx.y * kchunk;
+  const long k1 = min((long)K, k0 + kchunk);
   double acc = 0.0;
   if (n < N)
-    for (long k = threadIdx.y; k < K; k += blockDim.y) acc += A[k * lda + n] * x[k];
-  part[threadIdx.y][threadIdx.x] = acc;
+    for (long k = k0 + threadIdx.y; k < k1; k += blockDim.y) acc += A[k * lda + n] * x[k];
+  red[threadIdx.y][threadIdx.x] = acc;
   __syncthreads();
   if (threadIdx.y == 0 && n < N) {
     double s = 0.0;
-    for (int j = 0; j < (int)blockDim.y; ++j) s += part[j][threadIdx.x];
-    y[n] = alpha * s + (beta == 0.0 ? 0.0 : beta * y[n]);
+    for (int j = 0; j < (int)blockDim.y; ++j) s += red[j][threadIdx.x];
+    if (part)
+      part[(long)blockIdx.y * N + n] = s;
+    else
+      y[n] = alpha * s + (beta == 0.0 ? 0.0 : beta * y[n]);
   }
+}
+
+__global__ void gemv_t_finish_kernel(int N, int nsplit, const double* __restrict__ part, double* __restrict__ y,
+                                     double alpha, double beta) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  double s = 0.0;
+  for (int j = 0; j < nsplit; ++j) s += part[(long)j * N + n];
+  y[n] = alpha * s + (beta == 0.0 ? 0.0 : beta * y[n]);
 }
 
 int gemv(int M, int K, const double* A, long lda, const double* x, double* y, double alpha, double beta,
@@ -179,10 +424,25 @@ int gemv(int M, int K, const double* A, long lda, const double* x, double* y, do
 }
 
 int gemv_t(int K, int N, const double* A, long lda, const double* x, double* y, double alpha, double beta,
-           hipStream_t s) {
+           hipStream_t s, double* ws, long ws_elems) {
   if (N == 0) return ACES_OK;
-  hipLaunchKernelGGL(gemv_t_kernel, dim3((N + 255) / 256), dim3(256, 4), 0, s, K, N, A, lda, x, y, alpha, beta);
+  const long nbx = (N + 255) / 256;
+  // split k so that the launch holds about 2048 blocks (deterministic two-pass sum)
+  long nsplit = 1;
+  if (ws != nullptr && K > 0) {
+    nsplit = std::max<long>(1, std::min<long>((2048 + nbx - 1) / nbx, (K + 255) / 256));
+    nsplit = std::min<long>(nsplit, std::max<long>(1, ws_elems / N));
+    nsplit = std::min<long>(nsplit, 65535);
+  }
+  const long kchunk = (K + nsplit - 1) / std::max<long>(nsplit, 1);
+  hipLaunchKernelGGL(gemv_t_kernel, dim3((unsigned)nbx, (unsigned)nsplit), dim3(256, 4), 0, s, K, N, A, lda, x, y,
+                     alpha, beta, (int)std::max<long>(kchunk, 1), nsplit > 1 ? ws : nullptr);
   ACES_HIP(hipGetLastError());
+  if (nsplit > 1) {
+    hipLaunchKernelGGL(gemv_t_finish_kernel, dim3((unsigned)nbx), dim3(256), 0, s, N, (int)nsplit, ws, y, alpha,
+                       beta);
+    ACES_HIP(hipGetLastError());
+  }
   return ACES_OK;
 }
 
