@@ -59,9 +59,11 @@ __device__ inline GemmArgs select_group(const GemmGroups& gg, int gi) {
 }
 
 static constexpr int kGroupM = 8;
+static constexpr int kNarrowN = 24;
 static constexpr int kXcd = 8;
 
-// Logical tile for this block: XCD-contiguous ranges, then grouped order.
+// Logical tile for this block: XCD-contiguous ranges, then row-major order
+// (narrow outputs) or groups of kGroupM row tiles walked column by column.
 __device__ inline void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
   const int nb = tiles_m * tiles_n;
   const int bid = blockIdx.x;
@@ -70,6 +72,13 @@ __device__ inline void tile_coords(int tiles_m, int tiles_n, int& tm, int& tn) {
   const int per = nb / kXcd;
   const int rem = nb % kXcd;
   const int logical = x * per + (x < rem ? x : rem) + local;
+  if (tiles_n <= kNarrowN) {
+    // narrow output: all column tiles of a row tile run together, so each A
+    // row panel is pulled through the XCD's L2 once
+    tm = logical / tiles_n;
+    tn = logical % tiles_n;
+    return;
+  }
   const int group = kGroupM * tiles_n;
   const int gid = logical / group;
   const int first_m = gid * kGroupM;

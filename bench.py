@@ -107,6 +107,9 @@ def main():
     ap.add_argument('--no-convergence', action='store_true', help='skip the convergence study')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--maxcyc', type=int, default=60)
+    ap.add_argument('--calib-copy', action='store_true',
+                    help='copy 1 GiB device to device first (the reference dispatch that scales '
+                         'FETCH_SIZE/WRITE_SIZE counters to bytes)')
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
@@ -136,6 +139,14 @@ def main():
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     log('engine set up: %.1f GB device, %.1f s' % (cc.device_bytes / 1e9, time.perf_counter() - t_setup))
+    if args.calib_copy:
+        src = torch.ones(1 << 27, dtype=torch.float64, device='cuda')
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log('calibration copy: %d bytes read, %d bytes written' % (src.numel() * 8, dst.numel() * 8))
+        del src, dst
+        torch.cuda.empty_cache()
 
     # ---- timed region: K cycles of the headline form after W warmup cycles ----
     cc.set_diis_order(6)
@@ -170,7 +181,7 @@ def main():
     roofline = {'bound': 'mfma', 'kernel': {'bf16x3': 'gemm_bf16x3_kernel', 'fp32': 'gemm16_kernel<float>',
                                              'fp64': 'gemm16_kernel<double>'}[args.prec],
                 'achieved': achieved / 1e12, 'peak': peak / 1e12, 'unit': 'TFLOP/s',
-                'frac': achieved / peak, 'traffic': None,
+                'frac': achieved / peak, 'traffic': measured_traffic('ccsd_cycle_o%d_v%d_%s' % (o, v, args.prec)),
                 'avg_launch_ms': ms / nl, 'launches_per_cycle': nl / nprof,
                 'gemm_share_of_cycle': (ms / nprof) / (s_per_cycle * 1e3),
                 'per_class': {k: {'TFLOP/s': execf * p['flops'] / max(p['ms'], 1e-9) / 1e9, 'ms': p['ms'] / nprof}
@@ -179,6 +190,26 @@ def main():
                                'ms': prof_all['other']['ms'] / nprof,
                                'launches': prof_all['other']['launches'] / nprof}}
     log('roofline', json.dumps(roofline))
+
+    # ---- the plain form's MFMA rate: one fp64 cycle with the same per-launch timing ----
+    cc.set_profiling(True)
+    cc.profile(reset=True)
+    cc.cycle(args.warmup + args.steps + 1 + nprof, 'fp64', 1e-30)
+    p64 = cc.profile(reset=True)
+    cc.set_profiling(False)
+    big = {k: p64[k] for k in ('ladder', 'wbuild', 'ring')}
+    ms64 = sum(p['ms'] for p in big.values())
+    fl64 = sum(p['flops'] for p in big.values())
+    ms64all = ms64 + p64['other']['ms']
+    fl64all = fl64 + p64['other']['flops']
+    fp64_roofline = {'bound': 'mfma', 'kernel': 'gemm16_kernel<double>', 'achieved': fl64 / (ms64 * 1e-3) / 1e12,
+                     'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s', 'frac': fl64 / (ms64 * 1e-3) / FP64_PEAK,
+                     'scope': 'ladder + W build + ring contractions of one fp64 cycle',
+                     'all_fp64_gemms': {'TFLOP/s': fl64all / (ms64all * 1e-3) / 1e12,
+                                        'frac': fl64all / (ms64all * 1e-3) / FP64_PEAK, 'ms': ms64all},
+                     'per_class': {k: {'TFLOP/s': p['flops'] / max(p['ms'], 1e-9) / 1e9, 'ms': p['ms']}
+                                   for k, p in p64.items() if k != 'adjoint' and p['launches'] > 0}}
+    log('fp64 roofline', json.dumps(fp64_roofline))
 
     result = {
         'metric': 'CCSD s/iter at o=48,v=320: mixed vs FP64 on GPU at |dE|<=1e-7 Eh; MFMA % peak',
@@ -197,6 +228,7 @@ def main():
                    'o': o, 'v': v, 'form': args.prec, 'diis': 6},
         'effective_TFLOPs_ref': f_ref(o, v) / s_per_cycle / 1e12,
         'roofline': roofline,
+        'fp64_roofline': fp64_roofline,
     }
 
     # ---- convergence study (untimed for `value`) ----
@@ -250,6 +282,21 @@ def main():
     else:
         result['cpu_baseline'] = None
     print(json.dumps(result), flush=True)
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary profiles/hbm_traffic_<name>.md (the line
+    'traffic-json: {...}'), or None when that workload has no such summary."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles', 'hbm_traffic_%s.md' % name)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith('traffic-json:'):
+                    return json.loads(line.split(':', 1)[1])
+    except OSError:
+        return None
+    return None
 
 
 def cpu_baseline_lambda(o, v, seed, budget_s=20.0):
