@@ -132,46 +132,96 @@ int build_ladder_operands(int o, int v, const double* L233, const double* ovvv, 
 // Ladder result unpack.  Zp rows: [(a<=b) | (k,b)] x cols (i<=j); Zm rows [(a<b) | (k,b)] x (i<j).
 //   inc2[i][j][a][b] += Zp[ab][ij] + s Zm[ab][ij],  s = sign(b-a) * sign(j-i)
 //   Yb[b][i][j][k]   = Zp[kb][ij] + sign(j-i) Zm[kb][ij]
-__global__ void unpack_ladder_kernel(int o, int v, const double* __restrict__ Zp, long ldzp,
-                                     const double* __restrict__ Zm, long ldzm, double* __restrict__ inc2,
-                                     double* __restrict__ Yb) {
-  const long nvp = (long)v * (v + 1) / 2, nvm = (long)v * (v - 1) / 2;
-  const long oo = (long)o * o, vv = (long)v * v;
-  const long tot1 = oo * vv, tot2 = (long)v * oo * o;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < tot1 + tot2;
-       idx += (long)gridDim.x * blockDim.x) {
-    if (idx < tot1) {
-      const long ij = idx / vv, ab = idx - ij * vv;
-      const int i = (int)(ij / o), j = (int)(ij - (ij / o) * o);
-      const int a = (int)(ab / v), b = (int)(ab - (ab / v) * v);
-      const int i0 = min(i, j), j0 = max(i, j), a0 = min(a, b), b0 = max(a, b);
-      double z = Zp[pk_le(a0, b0) * ldzp + pk_le(i0, j0)];
-      if (i != j && a != b) {
-        const double sg = ((i < j) == (a < b)) ? 1.0 : -1.0;
-        z += sg * Zm[pk_lt(a0, b0) * ldzm + pk_lt(i0, j0)];
+//
+// inc2 part: one block per (32 consecutive (i<=j) columns, a, 64 values of b).
+// The Z rows of the pairs (a, b) are read along the pair columns (coalesced),
+// turned round in LDS, and added into inc2[i][j][a][b0..] and its mirror
+// inc2[j][i][a][b0..] along b.
+constexpr int kUpC = 32, kUpB = 64;
+
+__global__ __launch_bounds__(256) void unpack_ladder_t2_kernel(int o, int v, const double* __restrict__ Zp,
+                                                               long ldzp, const double* __restrict__ Zm, long ldzm,
+                                                               double* __restrict__ inc2) {
+  __shared__ double zp[kUpB][kUpC + 1];
+  __shared__ double zm[kUpB][kUpC + 1];
+  const long nop = (long)o * (o + 1) / 2;
+  const int nbt = (v + kUpB - 1) / kUpB;
+  long blk = blockIdx.x;
+  const int bt = (int)(blk % nbt);
+  blk /= nbt;
+  const int a = (int)(blk % v);
+  const long c0 = (blk / v) * kUpC;
+  const int b0 = bt * kUpB;
+  const int t = threadIdx.x;
+  // read: column c0 + cj, rows b0 + rb + 8k
+  {
+    const int cj = t & 31, rb = t >> 5;
+    const long p = c0 + cj;
+    int i = 0, j = 0;
+    if (p < nop) unpack_le(p, i, j);
+#pragma unroll
+    for (int k = 0; k < kUpB / 8; ++k) {
+      const int b = b0 + rb + 8 * k;
+      double xp = 0.0, xm = 0.0;
+      if (p < nop && b < v) {
+        const int lo = min(a, b), hi = max(a, b);
+        xp = Zp[pk_le(lo, hi) * ldzp + p];
+        if (i < j && a != b) xm = Zm[pk_lt(lo, hi) * ldzm + pk_lt(i, j)];
       }
-      inc2[idx] += z;
-    } else {
-      const long r = idx - tot1;  // r = ((b*o + i)*o + j)*o + k
-      const int k = (int)(r % o);
-      long q = r / o;
-      const int j = (int)(q % o);
-      q /= o;
-      const int i = (int)(q % o);
-      const int b = (int)(q / o);
-      const long row = (long)k * v + b;
-      const int i0 = min(i, j), j0 = max(i, j);
-      double y = Zp[(nvp + row) * ldzp + pk_le(i0, j0)];
-      if (i != j) y += ((i < j) ? 1.0 : -1.0) * Zm[(nvm + row) * ldzm + pk_lt(i0, j0)];
-      Yb[r] = y;
+      zp[rb + 8 * k][cj] = xp;
+      zm[rb + 8 * k][cj] = xm;
     }
+  }
+  __syncthreads();
+  // write: lanes along b, waves over the pair columns
+  const int lb = t & 63, w = t >> 6;
+  const int b = b0 + lb;
+  if (b >= v) return;
+  const double sab = (b > a) ? 1.0 : (b < a ? -1.0 : 0.0);
+  const long vv = (long)v * v;
+  for (int cj = w; cj < kUpC; cj += 4) {
+    const long p = c0 + cj;
+    if (p >= nop) break;
+    int i, j;
+    unpack_le(p, i, j);
+    const double xp = zp[lb][cj], xm = zm[lb][cj];
+    inc2[((long)i * o + j) * vv + (long)a * v + b] += xp + sab * xm;
+    if (i < j) inc2[((long)j * o + i) * vv + (long)a * v + b] += xp - sab * xm;
+  }
+}
+
+//   Yb[b][i][j][k] = Zp[kb][ij] + sign(j-i) Zm[kb][ij]   (v o^3 elements)
+__global__ void unpack_ladder_yb_kernel(int o, int v, const double* __restrict__ Zp, long ldzp,
+                                        const double* __restrict__ Zm, long ldzm, double* __restrict__ Yb) {
+  const long nvp = (long)v * (v + 1) / 2, nvm = (long)v * (v - 1) / 2;
+  const long tot = (long)v * o * o * o;
+  for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < tot; r += (long)gridDim.x * blockDim.x) {
+    // r = ((b*o + i)*o + j)*o + k
+    const int k = (int)(r % o);
+    long q = r / o;
+    const int j = (int)(q % o);
+    q /= o;
+    const int i = (int)(q % o);
+    const int b = (int)(q / o);
+    const long row = (long)k * v + b;
+    const int i0 = min(i, j), j0 = max(i, j);
+    double y = Zp[(nvp + row) * ldzp + pk_le(i0, j0)];
+    if (i != j) y += ((i < j) ? 1.0 : -1.0) * Zm[(nvm + row) * ldzm + pk_lt(i0, j0)];
+    Yb[r] = y;
   }
 }
 
 int unpack_ladder(int o, int v, const double* Zp, long ldzp, const double* Zm, long ldzm, double* inc2, double* Yb,
                   hipStream_t s) {
-  const long n = (long)o * o * v * v + (long)v * o * o * o;
-  hipLaunchKernelGGL(unpack_ladder_kernel, dim3(grid1(n)), dim3(256), 0, s, o, v, Zp, ldzp, Zm, ldzm, inc2, Yb);
+  const long nop = (long)o * (o + 1) / 2;
+  const long nblk = (nop + kUpC - 1) / kUpC * v * ((v + kUpB - 1) / kUpB);
+  ACES_REQUIRE(nblk < (1L << 31), "unpack_ladder: too many tiles");
+  if (nblk > 0)
+    hipLaunchKernelGGL(unpack_ladder_t2_kernel, dim3((unsigned)nblk), dim3(256), 0, s, o, v, Zp, ldzp, Zm, ldzm,
+                       inc2);
+  ACES_HIP(hipGetLastError());
+  const long n = (long)v * o * o * o;
+  hipLaunchKernelGGL(unpack_ladder_yb_kernel, dim3(grid1(n)), dim3(256), 0, s, o, v, Zp, ldzp, Zm, ldzm, Yb);
   ACES_HIP(hipGetLastError());
   return ACES_OK;
 }
