@@ -28,8 +28,10 @@ def _random_t(o, v, seed, scale=0.05):
     return t1, t2
 
 
+# (9, 80): ragged LDS tiles of the permute, ladder-unpack and symmetrize kernels
 @pytest.mark.parametrize('o,v,seed,nonhf', [(1, 3, 1, False), (2, 3, 2, True), (5, 19, 3, False),
-                                            (5, 19, 4, True), (7, 40, 5, True), (12, 64, 6, False)])
+                                            (5, 19, 4, True), (7, 40, 5, True), (12, 64, 6, False),
+                                            (9, 80, 7, True)])
 def test_increments_fp64(o, v, seed, nonhf):
     from aces2.ccsd import CCSD
     print('case increments', o, v, seed, nonhf, flush=True)
@@ -65,7 +67,7 @@ def test_increments_lowp(prec, tol):
     cc.close()
 
 
-@pytest.mark.parametrize('o,v,seed,nonhf', [(5, 19, 21, False), (6, 30, 22, True)])
+@pytest.mark.parametrize('o,v,seed,nonhf', [(5, 19, 21, False), (6, 30, 22, True), (4, 70, 23, False)])
 def test_cc_loop_matches_oracle(o, v, seed, nonhf):
     """Per-cycle energies, RESMITER and RESMDIIS follow the oracle's reference loop."""
     from aces2.ccsd import CCSD
