@@ -26,7 +26,8 @@ namespace {
 using eu::P4;
 using eu::gd;
 
-inline long pad8(long x) { return (x + 7) / 8 * 8; }
+// low-precision operand rows are padded (with zeros) to whole 32-wide k tiles
+inline long pad32(long x) { return (x + 31) / 32 * 32; }
 
 }  // namespace
 
@@ -274,7 +275,7 @@ int CcsdEngine::lowp_static(GemmPrec prec) {
                             {4, OVkl_, oo, vv, vv}};
   for (const S& t : tab) {
     LowP& L = lp_[t.slot];
-    const long ld = pad8(std::max<long>(t.K, 1));
+    const long ld = pad32(std::max<long>(t.K, 1));
     if (L.hi == nullptr || L.ld != ld || L.rows != t.rows) {
       void* blk;
       ACES_CHECK(alloc_raw(&blk, t.rows * ld * 4));
@@ -356,7 +357,7 @@ int CcsdEngine::gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A
   ACES_REQUIRE((aslot >= kLpStatic || a_kc) && (bslot >= kLpStatic || b_kc),
                "gemm_lp: static operands are k-contiguous");
   ACES_CHECK(lowp_static(prec));
-  const long Kp = pad8(std::max(K, 1));
+  const long Kp = pad32(std::max(K, 1));
   const void* ops[2][2];
   const int slots[2] = {aslot, bslot};
   const double* src[2] = {A64, B64};

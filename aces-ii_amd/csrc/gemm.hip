@@ -453,6 +453,167 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_kernel(GemmGroups gg) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// bf16x3 kernel with direct-to-LDS operand loads (global_load_lds_dwordx4)
+// and an S-stage LDS ring: the loads of k tile kt + S - 1 are in flight while
+// tile kt is multiplied, and no VGPR holds operand data on its way to LDS.
+// Same block tile, wave tile, MFMA sequence and LDS row swizzle as
+// gemm_bf16x3_kernel.  Requires K % 32 == 0 (no k tail: every loaded chunk
+// lies inside the operand rows).
+//
+// Loads: one wave instruction moves 16 rows x 64 bytes of one array into 1 KB
+// of LDS (lane L -> row L / 4, LDS chunk position L % 4); the lane fetches the
+// global chunk (L % 4) ^ ((row >> 2) & 3), which puts chunk q of row r at
+// position q ^ ((r >> 2) & 3).  Rows past M (N) are clamped to the last row:
+// they only feed accumulator rows (columns) that are never stored.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int vmcnt_imm(int n) {
+  return (n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14);
+}
+
+template <int WM, int S>
+__global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg) {
+  constexpr int BK = 32;
+  constexpr int BM = 64 * WM, BN = 128, NT = 128 * WM, NW = NT / 64;
+  constexpr int GA = BM / 16, GB = BN / 16, NG = 2 * GA + 2 * GB;  // 16-row groups per stage
+  constexpr int IPW = NG / NW;                                      // load instructions per wave and stage
+  constexpr int SA = BM * BK, SB = BN * BK;                         // elements per array and stage
+  constexpr int STAGE = 2 * SA + 2 * SB;
+  static_assert(NG % NW == 0, "groups per wave");
+  __shared__ __attribute__((aligned(16))) unsigned short lds[S * STAGE];
+
+  const int gi = (gg.n > 1) ? (int)blockIdx.y : 0;
+  const GemmArgs p = select_group(gg, gi);
+  if ((int)blockIdx.x >= p.tiles_m * p.tiles_n || (int)blockIdx.z >= p.splits) return;
+  int tm, tn;
+  tile_coords(p.tiles_m, p.tiles_n, tm, tn);
+  const int m0 = tm * BM;
+  const int n0 = tn * BN;
+  const int bz = (gg.n > 1) ? 0 : (int)blockIdx.y;
+  const long kz = (long)blockIdx.z * p.kchunk;
+  const unsigned short* __restrict__ Ahg = reinterpret_cast<const unsigned short*>(p.A) + (long)bz * p.sA + kz;
+  const unsigned short* __restrict__ Alg = reinterpret_cast<const unsigned short*>(p.Alo) + (long)bz * p.sA + kz;
+  const unsigned short* __restrict__ Bhg = reinterpret_cast<const unsigned short*>(p.B) + (long)bz * p.sB + kz;
+  const unsigned short* __restrict__ Blg = reinterpret_cast<const unsigned short*>(p.Blo) + (long)bz * p.sB + kz;
+  double* __restrict__ C = p.C + (long)bz * p.sC + (long)blockIdx.z * p.zC;
+  const int M = p.M, N = p.N;
+  const int K = (int)((p.K - kz) < p.kchunk ? (p.K - kz) : p.kchunk);
+
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wm = wave >> 1;
+  const int wn = wave & 1;
+
+  // per-instruction global row pointer (this lane) and LDS group offset (wave-uniform)
+  const int lrow = lane >> 2;
+  const int lchunk = (lane & 3) ^ ((lrow >> 2) & 3);
+  const unsigned short* gp[IPW];
+  int goff[IPW];
+#pragma unroll
+  for (int j = 0; j < IPW; ++j) {
+    const int G = wave * IPW + j;
+    const unsigned short* base;
+    long ld;
+    int row, nrows, off;
+    if (G < GA) {
+      base = Ahg; ld = p.lda; row = m0 + G * 16 + lrow; nrows = M; off = G * 16 * BK;
+    } else if (G < 2 * GA) {
+      base = Alg; ld = p.lda; row = m0 + (G - GA) * 16 + lrow; nrows = M; off = SA + (G - GA) * 16 * BK;
+    } else if (G < 2 * GA + GB) {
+      base = Bhg; ld = p.ldb; row = n0 + (G - 2 * GA) * 16 + lrow; nrows = N; off = 2 * SA + (G - 2 * GA) * 16 * BK;
+    } else {
+      base = Blg; ld = p.ldb; row = n0 + (G - 2 * GA - GB) * 16 + lrow; nrows = N;
+      off = 2 * SA + SB + (G - 2 * GA - GB) * 16 * BK;
+    }
+    row = row < nrows ? row : nrows - 1;
+    gp[j] = base + (long)row * ld + lchunk * 8;
+    goff[j] = off;
+  }
+  auto issue = [&](int kt) {
+    unsigned short* st = lds + (kt % S) * STAGE;
+#if __HIP_DEVICE_COMPILE__
+    // (the host pass parses kernel bodies too but has no gfx950 target features)
+#pragma unroll
+    for (int j = 0; j < IPW; ++j)
+      __builtin_amdgcn_global_load_lds(gp[j] + kt * BK, (__attribute__((address_space(3))) void*)(st + goff[j]),
+                                       16, 0, 0);
+#endif
+  };
+
+  f4_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0, 0, 0, 0};
+
+  const int nk = K / BK;
+  const int lr = lane & 15;
+  const int lk = lane >> 4;
+  const int rsw = (lk ^ ((lr >> 2) & 3)) << 3;
+#pragma unroll
+  for (int s = 0; s < S - 1; ++s)
+    if (s < nk) issue(s);
+  for (int kt = 0; kt < nk; ++kt) {
+    // this wave's loads of tile kt are done when at most min(S - 2, nk - 1 - kt) later tiles remain
+    const int later = (nk - 1 - kt) < (S - 2) ? (nk - 1 - kt) : (S - 2);
+    if (S >= 4 && later >= 2)
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm(2 * IPW));
+    else if (S >= 3 && later >= 1)
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm(IPW));
+    else
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+    // plain s_barrier: a __syncthreads() fence would wait for every load in
+    // flight (the later stages too); the wait above covers this tile
+    __builtin_amdgcn_s_barrier();
+    if (kt + S - 1 < nk) issue(kt + S - 1);
+    const unsigned short* st = lds + (kt % S) * STAGE;
+    s8_t ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int off = (wm * 64 + i * 16 + lr) * BK + rsw;
+      ah[i] = *reinterpret_cast<const s8_t*>(st + off);
+      al[i] = *reinterpret_cast<const s8_t*>(st + SA + off);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int off = (wn * 64 + j * 16 + lr) * BK + rsw;
+      bh[j] = *reinterpret_cast<const s8_t*>(st + 2 * SA + off);
+      bl[j] = *reinterpret_cast<const s8_t*>(st + 2 * SA + SB + off);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            __builtin_bit_cast(bf8_t, al[i]), __builtin_bit_cast(bf8_t, bh[j]), acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            __builtin_bit_cast(bf8_t, ah[i]), __builtin_bit_cast(bf8_t, bl[j]), acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+            __builtin_bit_cast(bf8_t, ah[i]), __builtin_bit_cast(bf8_t, bh[j]), acc[i][j], 0, 0, 0);
+      }
+    }
+  }
+
+  const double alpha = p.alpha, beta = p.beta;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = n0 + wn * 64 + j * 16 + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 64 + i * 16 + 4 * lk + r;
+        if (row < M && col < N) {
+          double* c = C + (long)row * p.ldc + col;
+          const double v = alpha * (double)acc[i][j][r];
+          *c = (beta == 0.0) ? v : v + beta * (*c);
+        }
+      }
+    }
+  }
+}
+
 // C[z-sum] reduction for split-K: C = alpha-scaled partial sums (already in ws) + beta*C.
 __global__ void splitk_reduce_kernel(int M, int N, int splits, int batch, const double* __restrict__ ws, long zC,
                                      double* __restrict__ C, long ldc, long sC, double beta) {
@@ -588,14 +749,24 @@ static int bf16_wm(const GemmDesc* d, int ng) {
   bool any = false;
   for (int g = 0; g < ng; ++g)
     if (d[g].M > 0 && d[g].N > 0) {
-      if (d[g].M < 1024 || d[g].N >= 4096) return 2;
+      if (d[g].M < 1024) return 2;
       any = true;
     }
   return any ? 4 : 2;
 }
 
+// LDS ring depth of the direct-to-LDS bf16x3 kernel per block shape
+static constexpr int kDlStages2 = 2, kDlStages4 = 3;
+
+// The direct-to-LDS bf16x3 kernel needs whole k tiles in every problem.
+static bool bf16_direct(const GemmDesc* d, int ng) {
+  for (int g = 0; g < ng; ++g)
+    if (d[g].M > 0 && d[g].N > 0 && d[g].K % 32 != 0) return false;
+  return true;
+}
+
 static int launch_groups(const GemmGroups& gg, GemmPrec prec, bool akc, bool bkc, bool kzero, dim3 grid, int bf_wm,
-                         hipStream_t stream) {
+                         bool bf_dl, hipStream_t stream) {
   if (kzero) {
     // C = beta * C via a K = 0 launch of the fp64 kernel (acc stays 0).
     launch16<double>(gg, true, true, grid, stream);
@@ -608,7 +779,11 @@ static int launch_groups(const GemmGroups& gg, GemmPrec prec, bool akc, bool bkc
         launch16<float>(gg, akc, bkc, grid, stream);
         break;
       case GP_BF16X3:
-        if (bf_wm == 4)
+        if (bf_dl && bf_wm == 4)
+          hipLaunchKernelGGL((gemm_bf16x3_dl_kernel<4, kDlStages4>), grid, dim3(512), 0, stream, gg);
+        else if (bf_dl)
+          hipLaunchKernelGGL((gemm_bf16x3_dl_kernel<2, kDlStages2>), grid, dim3(256), 0, stream, gg);
+        else if (bf_wm == 4)
           hipLaunchKernelGGL(gemm_bf16x3_kernel<4>, grid, dim3(512), 0, stream, gg);
         else
           hipLaunchKernelGGL(gemm_bf16x3_kernel<2>, grid, dim3(256), 0, stream, gg);
@@ -648,7 +823,7 @@ int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream) {
   for (int k = 1; k < kMaxGroups; ++k) gg.g[k] = a;
   dim3 grid((unsigned)nb, (unsigned)d.batch, (unsigned)splits);
   if (d.K == 0) a.K = 0;
-  ACES_CHECK(launch_groups(gg, prec, d.a_kc != 0, d.b_kc != 0, d.K == 0, grid, wm, stream));
+  ACES_CHECK(launch_groups(gg, prec, d.a_kc != 0, d.b_kc != 0, d.K == 0, grid, wm, bf16_direct(&d, 1), stream));
   return reduce_splits(d, a, stream);
 }
 
@@ -686,7 +861,7 @@ int gemm_grouped(const GemmDesc* d, int ng, GemmPrec prec, hipStream_t stream) {
   for (int g = 0; g < ng; ++g)
     if (d[g].M > 0 && d[g].N > 0) kz = (d[g].K == 0);
   dim3 grid((unsigned)maxnb, (unsigned)ng, (unsigned)maxsplit);
-  ACES_CHECK(launch_groups(gg, prec, d[0].a_kc != 0, d[0].b_kc != 0, kz != 0, grid, wm, stream));
+  ACES_CHECK(launch_groups(gg, prec, d[0].a_kc != 0, d[0].b_kc != 0, kz != 0, grid, wm, bf16_direct(d, ng), stream));
   for (int g = 0; g < ng; ++g)
     if (d[g].M > 0 && d[g].N > 0) ACES_CHECK(reduce_splits(d[g], gg.g[g], stream));
   return ACES_OK;

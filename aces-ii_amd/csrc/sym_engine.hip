@@ -21,7 +21,8 @@ using dpd::Src;
 
 namespace {
 
-inline long pad8(long x) { return (x + 7) / 8 * 8; }
+// low-precision operand rows are padded (with zeros) to whole 32-wide k tiles
+inline long pad32(long x) { return (x + 31) / 32 * 32; }
 
 constexpr int kDotBlocks = 512;
 constexpr int kMaxDots = 32;
@@ -449,7 +450,7 @@ int SymEngine::lowp_convert(int t, GemmPrec prec, LowBuf& lb) {
   if (lb.hi == nullptr) {
     long off = 0;
     for (int g = 0; g < nirr; ++g) {
-      lb.ld[g] = pad8(std::max<long>(tt.ncol[g], 1));
+      lb.ld[g] = pad32(std::max<long>(tt.ncol[g], 1));
       lb.off[g] = off;
       off += tt.nrow[g] * lb.ld[g];
     }
