@@ -457,9 +457,11 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_kernel(GemmGroups gg) {
 // bf16x3 kernel with direct-to-LDS operand loads (global_load_lds_dwordx4)
 // and an S-stage LDS ring: the loads of k tile kt + S - 1 are in flight while
 // tile kt is multiplied, and no VGPR holds operand data on its way to LDS.
-// Same block tile, wave tile, MFMA sequence and LDS row swizzle as
-// gemm_bf16x3_kernel.  Requires K % 32 == 0 (no k tail: every loaded chunk
-// lies inside the operand rows).
+// Block (64 WM) x (32 FN) on 2 WM waves, each a 64 x 16 FN tile; same MFMA
+// sequence and LDS row swizzle as gemm_bf16x3_kernel.  FN = 8 (256 x 256
+// blocks) halves the operand bytes per flop of the 256 x 128 shape.
+// Requires K % 32 == 0 (no k tail: every loaded chunk lies inside the
+// operand rows).
 //
 // Loads: one wave instruction moves 16 rows x 64 bytes of one array into 1 KB
 // of LDS (lane L -> row L / 4, LDS chunk position L % 4); the lane fetches the
@@ -471,10 +473,10 @@ __host__ __device__ constexpr int vmcnt_imm(int n) {
   return (n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14);
 }
 
-template <int WM, int S>
+template <int WM, int FN, int S>
 __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg) {
   constexpr int BK = 32;
-  constexpr int BM = 64 * WM, BN = 128, NT = 128 * WM, NW = NT / 64;
+  constexpr int BM = 64 * WM, BN = 32 * FN, NT = 128 * WM, NW = NT / 64;  // wave tile 64 x 16 FN
   constexpr int GA = BM / 16, GB = BN / 16, NG = 2 * GA + 2 * GB;  // 16-row groups per stage
   constexpr int IPW = NG / NW;                                      // load instructions per wave and stage
   constexpr int SA = BM * BK, SB = BN * BK;                         // elements per array and stage
@@ -541,11 +543,11 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
 #endif
   };
 
-  f4_t acc[4][4];
+  f4_t acc[4][FN];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0, 0, 0, 0};
+    for (int j = 0; j < FN; ++j) acc[i][j] = f4_t{0, 0, 0, 0};
 
   const int nk = K / BK;
   const int lr = lane & 15;
@@ -568,7 +570,7 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
     __builtin_amdgcn_s_barrier();
     if (kt + S - 1 < nk) issue(kt + S - 1);
     const unsigned short* st = lds + (kt % S) * STAGE;
-    s8_t ah[4], al[4], bh[4], bl[4];
+    s8_t ah[4], al[4], bh[FN], bl[FN];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int off = (wm * 64 + i * 16 + lr) * BK + rsw;
@@ -576,15 +578,15 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
       al[i] = *reinterpret_cast<const s8_t*>(st + SA + off);
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int off = (wn * 64 + j * 16 + lr) * BK + rsw;
+    for (int j = 0; j < FN; ++j) {
+      const int off = (wn * 16 * FN + j * 16 + lr) * BK + rsw;
       bh[j] = *reinterpret_cast<const s8_t*>(st + 2 * SA + off);
       bl[j] = *reinterpret_cast<const s8_t*>(st + 2 * SA + SB + off);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < FN; ++j) {
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
             __builtin_bit_cast(bf8_t, al[i]), __builtin_bit_cast(bf8_t, bh[j]), acc[i][j], 0, 0, 0);
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
@@ -599,8 +601,8 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int col = n0 + wn * 64 + j * 16 + lr;
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + wn * 16 * FN + j * 16 + lr;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm * 64 + i * 16 + 4 * lk + r;
@@ -667,6 +669,33 @@ static void launch16(const GemmGroups& a, bool akc, bool bkc, dim3 grid, hipStre
 #undef L16
 }
 
+// LDS ring depth of the direct-to-LDS bf16x3 kernel per block shape
+static constexpr int kDlStages2 = 2, kDlStages4 = 3, kDlStagesWide = 2;
+
+// The direct-to-LDS bf16x3 kernel needs whole k tiles in every problem.
+static bool bf16_direct(const GemmDesc* d, int ng) {
+  for (int g = 0; g < ng; ++g)
+    if (d[g].M > 0 && d[g].N > 0 && d[g].K % 32 != 0) return false;
+  return true;
+}
+
+// bf16x3 block shape: 2 = 128 x 128 (some problem has M < 1024); 8 = 256 x 256
+// (direct-to-LDS kernel; every problem at least 1024 x 1024 and with at least
+// 256 such tiles or a split-K workspace to fill the chip); else 4 = 256 x 128.
+static int bf16_shape(const GemmDesc* d, int ng) {
+  bool any = false, wide = bf16_direct(d, ng);
+  for (int g = 0; g < ng; ++g)
+    if (d[g].M > 0 && d[g].N > 0) {
+      if (d[g].M < 1024) return 2;
+      const long tiles = (long)((d[g].M + 255) / 256) * ((d[g].N + 255) / 256);
+      wide = wide && d[g].N >= 1024 && (tiles >= 256 || (d[0].ws != nullptr && d[g].K >= 1024));
+      any = true;
+    }
+  return !any ? 2 : (wide ? 8 : 4);
+}
+static int bf16_bm(int shape) { return shape == 2 ? 128 : 256; }
+static int bf16_bn(int shape) { return shape == 8 ? 256 : 128; }
+
 // Fill the kernel arguments of one problem; split-K when the output cannot
 // fill the chip and K is long, with partial products
 // in ws[ws_off ...].  Returns the number of splits.
@@ -679,8 +708,9 @@ static int prepare(const GemmDesc& d, GemmPrec prec, GemmArgs& a, double* ws, lo
   a.alpha = d.alpha; a.beta = d.beta;
   a.sA = d.sA; a.sB = d.sB; a.sC = d.sC;
   const int bm =
-      (prec == GP_BF16X3) ? 64 * bf_wm : (prec == GP_F32 ? TileCfg<float>::type::BM : TileCfg<double>::type::BM);
-  const int bn = (prec == GP_BF16X3) ? 128 : (prec == GP_F32 ? TileCfg<float>::type::BN : TileCfg<double>::type::BN);
+      (prec == GP_BF16X3) ? bf16_bm(bf_wm) : (prec == GP_F32 ? TileCfg<float>::type::BM : TileCfg<double>::type::BM);
+  const int bn =
+      (prec == GP_BF16X3) ? bf16_bn(bf_wm) : (prec == GP_F32 ? TileCfg<float>::type::BN : TileCfg<double>::type::BN);
   a.tiles_m = (d.M + bm - 1) / bm;
   a.tiles_n = (d.N + bn - 1) / bn;
   {
@@ -742,29 +772,6 @@ static int check_desc(const GemmDesc& d, GemmPrec prec) {
   return ACES_OK;
 }
 
-// bf16x3 block rows / 64: 4 (256 x 128 tiles) for tall, narrow problems
-// (every non-empty one with M >= 1024 and N < 4096: the particle ladder
-// shapes, where fewer operand bytes per flop pay), else 2 (128 x 128).
-static int bf16_wm(const GemmDesc* d, int ng) {
-  bool any = false;
-  for (int g = 0; g < ng; ++g)
-    if (d[g].M > 0 && d[g].N > 0) {
-      if (d[g].M < 1024) return 2;
-      any = true;
-    }
-  return any ? 4 : 2;
-}
-
-// LDS ring depth of the direct-to-LDS bf16x3 kernel per block shape
-static constexpr int kDlStages2 = 2, kDlStages4 = 3;
-
-// The direct-to-LDS bf16x3 kernel needs whole k tiles in every problem.
-static bool bf16_direct(const GemmDesc* d, int ng) {
-  for (int g = 0; g < ng; ++g)
-    if (d[g].M > 0 && d[g].N > 0 && d[g].K % 32 != 0) return false;
-  return true;
-}
-
 static int launch_groups(const GemmGroups& gg, GemmPrec prec, bool akc, bool bkc, bool kzero, dim3 grid, int bf_wm,
                          bool bf_dl, hipStream_t stream) {
   if (kzero) {
@@ -779,10 +786,12 @@ static int launch_groups(const GemmGroups& gg, GemmPrec prec, bool akc, bool bkc
         launch16<float>(gg, akc, bkc, grid, stream);
         break;
       case GP_BF16X3:
-        if (bf_dl && bf_wm == 4)
-          hipLaunchKernelGGL((gemm_bf16x3_dl_kernel<4, kDlStages4>), grid, dim3(512), 0, stream, gg);
+        if (bf_dl && bf_wm == 8)
+          hipLaunchKernelGGL((gemm_bf16x3_dl_kernel<4, 8, kDlStagesWide>), grid, dim3(512), 0, stream, gg);
+        else if (bf_dl && bf_wm == 4)
+          hipLaunchKernelGGL((gemm_bf16x3_dl_kernel<4, 4, kDlStages4>), grid, dim3(512), 0, stream, gg);
         else if (bf_dl)
-          hipLaunchKernelGGL((gemm_bf16x3_dl_kernel<2, kDlStages2>), grid, dim3(256), 0, stream, gg);
+          hipLaunchKernelGGL((gemm_bf16x3_dl_kernel<2, 4, kDlStages2>), grid, dim3(256), 0, stream, gg);
         else if (bf_wm == 4)
           hipLaunchKernelGGL(gemm_bf16x3_kernel<4>, grid, dim3(512), 0, stream, gg);
         else
@@ -814,7 +823,7 @@ int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream) {
   gg.n = 1;
   GemmArgs& a = gg.g[0];
   // tiles for the split decision use this precision's block size
-  const int wm = bf16_wm(&d, 1);
+  const int wm = bf16_shape(&d, 1);
   GemmArgs probe;
   prepare(d, prec, probe, nullptr, 0, 0, 1L << 30, wm);
   const long nb = (long)probe.tiles_m * probe.tiles_n;
@@ -835,7 +844,7 @@ int gemm_grouped(const GemmDesc* d, int ng, GemmPrec prec, hipStream_t stream) {
   long maxnb = 0, ws_off = 0;
   int maxsplit = 1;
   bool any = false;
-  const int wm = bf16_wm(d, ng);
+  const int wm = bf16_shape(d, ng);
   for (int g = 0; g < ng; ++g) {
     ACES_CHECK(check_desc(d[g], prec));
     ACES_REQUIRE(d[g].batch == 1, "gemm_grouped: batched groups are not supported");
