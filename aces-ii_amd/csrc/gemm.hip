@@ -295,6 +295,13 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm16_kernel(GemmGroups gg) {
   }
 }
 
+// Position of 16-byte chunk 0 of a 64-byte LDS row r (chunk q sits at q ^ sw):
+// the 16 rows of one MFMA fragment read (one chunk each) then cover all banks
+// both when 8 lanes (32 banks) and when 16 lanes (64 banks) share a cycle:
+// rows with equal 16r mod 128 bytes (r ^ 2, 4, 6) and with equal 16r mod 256
+// bytes (r ^ 4, 8, 12) get four different positions.
+__device__ inline int lds_swizzle(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
+
 // ---------------------------------------------------------------------------
 // bf16x3 kernel: A = Ahi + Alo, B = Bhi + Blo (bf16, k contiguous), BK = 32.
 // acc += Alo*Bhi + Ahi*Blo + Ahi*Bhi with v_mfma_f32_16x16x32_bf16.
@@ -307,7 +314,7 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_kernel(GemmGroups gg) {
   constexpr int BK = 32;
   constexpr int BM = 64 * WM, BN = 128, NT = 128 * WM;
   // LDS rows of 32 bf16 (64 bytes, no padding); the 16-byte chunk q of row r
-  // sits at position q ^ ((r >> 2) & 3), which keeps both the row-wise
+  // sits at position q ^ sw(r), sw(r) = ((r >> 1) ^ (r >> 2)) & 3, which keeps both the row-wise
   // stores and the MFMA fragment reads free of bank conflicts.
   constexpr int LDK = BK;
   constexpr int LA = BM * 4 / NT, LB = BN * 4 / NT;  // row loads per thread and array
@@ -373,7 +380,7 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_kernel(GemmGroups gg) {
     }
   };
   // (NT / 4) * c is a multiple of 16, so the swizzle of row lrow + (NT/4) c is that of lrow
-  const int soff = lrow * LDK + (((t & 3) ^ ((lrow >> 2) & 3)) << 3);
+  const int soff = lrow * LDK + (((t & 3) ^ lds_swizzle(lrow)) << 3);
   auto store = [&](int buf) {
 #pragma unroll
     for (int c = 0; c < LA; ++c) {
@@ -401,7 +408,7 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_kernel(GemmGroups gg) {
   __syncthreads();
   const int lr = lane & 15;
   const int lk = lane >> 4;
-  const int rsw = (lk ^ ((lr >> 2) & 3)) << 3;  // swizzled chunk of k values lk*8..lk*8+7
+  const int rsw = (lk ^ lds_swizzle(lr)) << 3;  // swizzled chunk of k values lk*8..lk*8+7
   for (int kt = 0; kt < nk; ++kt) {
     const int buf = kt & 1;
     if (kt + 1 < nk) load((kt + 1) * BK);
@@ -465,8 +472,8 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_kernel(GemmGroups gg) {
 //
 // Loads: one wave instruction moves 16 rows x 64 bytes of one array into 1 KB
 // of LDS (lane L -> row L / 4, LDS chunk position L % 4); the lane fetches the
-// global chunk (L % 4) ^ ((row >> 2) & 3), which puts chunk q of row r at
-// position q ^ ((r >> 2) & 3).  Rows past M (N) are clamped to the last row:
+// global chunk (L % 4) ^ sw(row), which puts chunk q of row r at position
+// q ^ sw(r).  Rows past M (N) are clamped to the last row:
 // they only feed accumulator rows (columns) that are never stored.
 // ---------------------------------------------------------------------------
 __host__ __device__ constexpr int vmcnt_imm(int n) {
@@ -509,7 +516,7 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
 
   // per-instruction global row pointer (this lane) and LDS group offset (wave-uniform)
   const int lrow = lane >> 2;
-  const int lchunk = (lane & 3) ^ ((lrow >> 2) & 3);
+  const int lchunk = (lane & 3) ^ lds_swizzle(lrow);
   const unsigned short* gp[IPW];
   int goff[IPW];
 #pragma unroll
@@ -552,7 +559,7 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
   const int nk = K / BK;
   const int lr = lane & 15;
   const int lk = lane >> 4;
-  const int rsw = (lk ^ ((lr >> 2) & 3)) << 3;
+  const int rsw = (lk ^ lds_swizzle(lr)) << 3;
 #pragma unroll
   for (int s = 0; s < S - 1; ++s)
     if (s < nk) issue(s);
