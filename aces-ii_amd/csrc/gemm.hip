@@ -623,6 +623,137 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
   }
 }
 
+// ---------------------------------------------------------------------------
+// fp64 kernel with direct-to-LDS operand loads (global_load_lds_dwordx4) into
+// an S-stage LDS ring; both operands k-contiguous, K % 16 == 0.  Block
+// (32 FM) x (32 FN) on 2 x 2 waves, each a (16 FM) x (16 FN) tile of
+// v_mfma_f64_16x16x4_f64.  LDS rows hold 16 doubles (128 bytes, eight 16-byte
+// chunks); chunk q of row r sits at position q ^ (r & 7), so the 16 rows of
+// one fragment read fall on at most two rows per bank group.  One wave
+// instruction moves 8 rows x 128 bytes (lane L: row L / 8, position L % 8).
+// ---------------------------------------------------------------------------
+template <int FM, int FN, int S>
+__global__ __launch_bounds__(256) void gemm_f64_dl_kernel(GemmGroups gg) {
+  constexpr int BK = 16, NW = 4;
+  constexpr int BM = 32 * FM, BN = 32 * FN;
+  constexpr int GA = BM / 8, GB = BN / 8, NG = GA + GB, IPW = NG / NW;
+  constexpr int SA = BM * BK, STAGE = (BM + BN) * BK;  // doubles
+  static_assert(NG % NW == 0, "groups per wave");
+  __shared__ __attribute__((aligned(16))) double lds[S * STAGE];
+
+  const int gi = (gg.n > 1) ? (int)blockIdx.y : 0;
+  const GemmArgs p = select_group(gg, gi);
+  if ((int)blockIdx.x >= p.tiles_m * p.tiles_n || (int)blockIdx.z >= p.splits) return;
+  int tm, tn;
+  tile_coords(p.tiles_m, p.tiles_n, tm, tn);
+  const int m0 = tm * BM;
+  const int n0 = tn * BN;
+  const int bz = (gg.n > 1) ? 0 : (int)blockIdx.y;
+  const long kz = (long)blockIdx.z * p.kchunk;
+  const double* __restrict__ Ag = reinterpret_cast<const double*>(p.A) + (long)bz * p.sA + kz;
+  const double* __restrict__ Bg = reinterpret_cast<const double*>(p.B) + (long)bz * p.sB + kz;
+  double* __restrict__ C = p.C + (long)bz * p.sC + (long)blockIdx.z * p.zC;
+  const int M = p.M, N = p.N;
+  const int K = (int)((p.K - kz) < p.kchunk ? (p.K - kz) : p.kchunk);
+
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wm = wave >> 1;
+  const int wn = wave & 1;
+
+  const int drow = lane >> 3;
+  const int dchunk = (lane & 7) ^ drow;  // rows of a group start at multiples of 8: r & 7 = drow
+  const double* gp[IPW];
+  int goff[IPW];
+#pragma unroll
+  for (int j = 0; j < IPW; ++j) {
+    const int G = wave * IPW + j;
+    const double* base;
+    long ld;
+    int row, nrows, off;
+    if (G < GA) {
+      base = Ag; ld = p.lda; row = m0 + G * 8 + drow; nrows = M; off = G * 8 * BK;
+    } else {
+      base = Bg; ld = p.ldb; row = n0 + (G - GA) * 8 + drow; nrows = N; off = SA + (G - GA) * 8 * BK;
+    }
+    row = row < nrows ? row : nrows - 1;  // clamped rows feed accumulators that are never stored
+    gp[j] = base + (long)row * ld + dchunk * 2;
+    goff[j] = off;
+  }
+  auto issue = [&](int kt) {
+#if __HIP_DEVICE_COMPILE__
+    double* st = lds + (kt % S) * STAGE;
+#pragma unroll
+    for (int j = 0; j < IPW; ++j)
+      __builtin_amdgcn_global_load_lds(gp[j] + kt * BK, (__attribute__((address_space(3))) void*)(st + goff[j]),
+                                       16, 0, 0);
+#endif
+  };
+
+  d4_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = d4_t{0, 0, 0, 0};
+
+  const int nk = K / BK;
+  const int lr = lane & 15;
+  const int lk = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < S - 1; ++s)
+    if (s < nk) issue(s);
+  for (int kt = 0; kt < nk; ++kt) {
+    const int later = (nk - 1 - kt) < (S - 2) ? (nk - 1 - kt) : (S - 2);
+    if (S >= 4 && later >= 2)
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm(2 * IPW));
+    else if (S >= 3 && later >= 1)
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm(IPW));
+    else
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+    __builtin_amdgcn_s_barrier();
+    if (kt + S - 1 < nk) issue(kt + S - 1);
+    const double* st = lds + (kt % S) * STAGE;
+#pragma unroll
+    for (int kk = 0; kk < BK / 4; ++kk) {
+      const int k = kk * 4 + lk;
+      double a[FM], b[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int r = wm * 16 * FM + i * 16 + lr;
+        a[i] = st[r * BK + (((k >> 1) ^ (r & 7)) << 1) + (k & 1)];
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int r = wn * 16 * FN + j * 16 + lr;
+        b[j] = st[SA + r * BK + (((k >> 1) ^ (r & 7)) << 1) + (k & 1)];
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = Mfma16<double>::op(a[i], b[j], acc[i][j]);
+    }
+  }
+
+  const double alpha = p.alpha, beta = p.beta;
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + wn * 16 * FN + j * 16 + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 16 * FM + i * 16 + Mfma16<double>::row(lk, r);
+        if (row < M && col < N) {
+          double* c = C + (long)row * p.ldc + col;
+          const double v = alpha * acc[i][j][r];
+          *c = (beta == 0.0) ? v : v + beta * (*c);
+        }
+      }
+    }
+  }
+}
+
 // C[z-sum] reduction for split-K: C = alpha-scaled partial sums (already in ws) + beta*C.
 __global__ void splitk_reduce_kernel(int M, int N, int splits, int batch, const double* __restrict__ ws, long zC,
                                      double* __restrict__ C, long ldc, long sC, double beta) {
@@ -700,6 +831,32 @@ static int bf16_shape(const GemmDesc* d, int ng) {
     }
   return !any ? 2 : (wide ? 8 : 4);
 }
+// fp64 problems with both operands k-contiguous, 16-byte aligned rows,
+// K % 16 == 0, K >= 256 and a tall narrow output (M >= 1024, N < 4096: the
+// particle ladder, the t1 contractions) run on the direct-to-LDS kernel
+// (128 x 64 blocks);
+// square (ov)^3 products keep the 64 x 64 register-staged kernel, which is as
+// fast there.
+static constexpr int kShapeF64Dl = 16;
+static constexpr int kF64DlFM = 4, kF64DlFN = 2, kF64DlStages = 3;
+static bool f64_direct(const GemmDesc* d, int ng) {
+  bool any = false;
+  for (int g = 0; g < ng; ++g)
+    if (d[g].M > 0 && d[g].N > 0) {
+      if (!d[g].a_kc || !d[g].b_kc || d[g].K < 256 || d[g].K % 16 != 0 || d[g].N >= 4096 || d[g].M < 1024)
+        return false;
+      if ((uintptr_t)d[g].A % 16 || (uintptr_t)d[g].B % 16 || d[g].lda % 2 || d[g].ldb % 2 || d[g].sA % 2 ||
+          d[g].sB % 2)
+        return false;
+      any = true;
+    }
+  return any;
+}
+static int gemm_shape(const GemmDesc* d, int ng, GemmPrec prec) {
+  if (prec == GP_BF16X3) return bf16_shape(d, ng);
+  if (prec == GP_F64 && f64_direct(d, ng)) return kShapeF64Dl;
+  return 0;
+}
 static int bf16_bm(int shape) { return shape == 2 ? 128 : 256; }
 static int bf16_bn(int shape) { return shape == 8 ? 256 : 128; }
 
@@ -714,10 +871,20 @@ static int prepare(const GemmDesc& d, GemmPrec prec, GemmArgs& a, double* ws, lo
   a.C = d.C; a.ldc = d.ldc;
   a.alpha = d.alpha; a.beta = d.beta;
   a.sA = d.sA; a.sB = d.sB; a.sC = d.sC;
-  const int bm =
-      (prec == GP_BF16X3) ? bf16_bm(bf_wm) : (prec == GP_F32 ? TileCfg<float>::type::BM : TileCfg<double>::type::BM);
-  const int bn =
-      (prec == GP_BF16X3) ? bf16_bn(bf_wm) : (prec == GP_F32 ? TileCfg<float>::type::BN : TileCfg<double>::type::BN);
+  int bm, bn;
+  if (prec == GP_BF16X3) {
+    bm = bf16_bm(bf_wm);
+    bn = bf16_bn(bf_wm);
+  } else if (prec == GP_F64 && bf_wm == kShapeF64Dl) {
+    bm = 32 * kF64DlFM;
+    bn = 32 * kF64DlFN;
+  } else if (prec == GP_F32) {
+    bm = TileCfg<float>::type::BM;
+    bn = TileCfg<float>::type::BN;
+  } else {
+    bm = TileCfg<double>::type::BM;
+    bn = TileCfg<double>::type::BN;
+  }
   a.tiles_m = (d.M + bm - 1) / bm;
   a.tiles_n = (d.N + bn - 1) / bn;
   {
@@ -787,7 +954,10 @@ static int launch_groups(const GemmGroups& gg, GemmPrec prec, bool akc, bool bkc
   } else {
     switch (prec) {
       case GP_F64:
-        launch16<double>(gg, akc, bkc, grid, stream);
+        if (bf_wm == kShapeF64Dl)
+          hipLaunchKernelGGL((gemm_f64_dl_kernel<kF64DlFM, kF64DlFN, kF64DlStages>), grid, dim3(256), 0, stream, gg);
+        else
+          launch16<double>(gg, akc, bkc, grid, stream);
         break;
       case GP_F32:
         launch16<float>(gg, akc, bkc, grid, stream);
@@ -830,7 +1000,7 @@ int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream) {
   gg.n = 1;
   GemmArgs& a = gg.g[0];
   // tiles for the split decision use this precision's block size
-  const int wm = bf16_shape(&d, 1);
+  const int wm = gemm_shape(&d, 1, prec);
   GemmArgs probe;
   prepare(d, prec, probe, nullptr, 0, 0, 1L << 30, wm);
   const long nb = (long)probe.tiles_m * probe.tiles_n;
@@ -851,7 +1021,7 @@ int gemm_grouped(const GemmDesc* d, int ng, GemmPrec prec, hipStream_t stream) {
   long maxnb = 0, ws_off = 0;
   int maxsplit = 1;
   bool any = false;
-  const int wm = bf16_shape(d, ng);
+  const int wm = gemm_shape(d, ng, prec);
   for (int g = 0; g < ng; ++g) {
     ACES_CHECK(check_desc(d[g], prec));
     ACES_REQUIRE(d[g].batch == 1, "gemm_grouped: batched groups are not supported");

@@ -155,3 +155,49 @@ def test_lowp_grouped(prec):
     for gr, (ref, scale) in zip(groups, refs):
         err = ((gr['C'] - ref).abs() / scale).max().item()
         assert err < (4e-7 if prec == 1 else 3e-5), err
+
+
+@pytest.mark.parametrize('M,N,K,ws', [(1500, 300, 512, False), (4100, 4200, 96, False), (1100, 2100, 1024, True),
+                                      (1030, 1040, 2080, True)])
+def test_bf16x3_block_shapes(M, N, K, ws):
+    """The bf16x3 block shapes (128x128, 256x128, 256x256; direct-to-LDS ring when
+    K % 32 == 0) on ragged M/N, with and without split-K."""
+    from aces2 import _lib
+    print('case bf16x3 shape', M, N, K, ws, flush=True)
+    g = torch.Generator(device='cuda').manual_seed(M + N + K)
+    A = torch.randn(M, K, dtype=torch.float64, device='cuda', generator=g)
+    B = torch.randn(N, K, dtype=torch.float64, device='cuda', generator=g)
+    C0 = torch.randn(M, N, dtype=torch.float64, device='cuda', generator=g)
+    ref = 0.75 * (A @ B.t()) + 0.5 * C0
+    scale = 0.75 * (A.abs() @ B.abs().t()) + 0.5 * C0.abs()
+    ops = []
+    for X in (A, B):
+        hi = torch.empty(X.shape, dtype=torch.int16, device='cuda')
+        lo = torch.empty(X.shape, dtype=torch.int16, device='cuda')
+        _lib.call('aces_split_bf16', X.shape[0], K, X.data_ptr(), K, hi.data_ptr(), lo.data_ptr(), K, None)
+        ops.append((hi, lo))
+    C = C0.clone()
+    groups = [{'M': M, 'N': N, 'K': K, 'A': ops[0][0], 'Alo': ops[0][1], 'B': ops[1][0], 'Blo': ops[1][1], 'C': C}]
+    wsb = torch.empty(1 << 24, dtype=torch.float64, device='cuda') if ws else None
+    _grouped(2, groups, 1, 1, 0.75, 0.5, wsb)
+    torch.cuda.synchronize()
+    err = ((C - ref).abs() / scale).max().item()
+    assert err < 3e-5, err
+
+
+@pytest.mark.parametrize('M,N,K,ws', [(1300, 200, 512, False), (3000, 1176, 1024, False), (1100, 90, 4096, True)])
+def test_f64_direct_lds(M, N, K, ws):
+    """fp64 problems on the direct-to-LDS kernel (k-contiguous operands, K % 16 == 0,
+    narrow outputs), ragged M/N, with and without split-K."""
+    print('case f64 dl', M, N, K, ws, flush=True)
+    g = torch.Generator(device='cuda').manual_seed(M + 7 * N + K)
+    A = torch.randn(M, K, dtype=torch.float64, device='cuda', generator=g)
+    B = torch.randn(N, K, dtype=torch.float64, device='cuda', generator=g)
+    C0 = torch.randn(M, N, dtype=torch.float64, device='cuda', generator=g)
+    C = C0.clone()
+    wsb = torch.empty(1 << 22, dtype=torch.float64, device='cuda') if ws else None
+    groups = [{'M': M, 'N': N, 'K': K, 'A': A, 'B': B, 'C': C}]
+    _grouped(0, groups, 1, 1, -0.5, 1.5, wsb)
+    torch.cuda.synchronize()
+    ref = -0.5 * (A @ B.t()) + 1.5 * C0
+    assert torch.allclose(C, ref, rtol=1e-12, atol=1e-12 * K ** 0.5)
