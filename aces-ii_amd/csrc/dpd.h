@@ -174,13 +174,15 @@ __host__ __device__ inline void pair_decode(const PairSpace& P, const Space& S, 
 struct Box {
   int d[4];
   int g[4];
-  long first;  // first element (prefix over boxes)
+  long first;   // first element chunk (prefix over boxes)
+  long tfirst;  // first LDS tile of the tiled walk (prefix over boxes)
   long ob, os[4];
   long sb[3], ss[3][4];
   long xb, xs[4], yb, ys[4];  // xb < 0: product term vanishes in this box
 };
 
 constexpr long kChunk = 4096;  // elements per workgroup in box walks
+constexpr int kTile3 = 64, kTileQ = 32;  // tiled walk: output tile (index 3) x (index tq)
 
 // Box of workgroup `chunk` (binary search over Box::first).
 __device__ inline int box_of_chunk(const Box* boxes, int nbox, long chunk) {
@@ -202,6 +204,12 @@ struct Plan {
   long total = 0;
   int nsrc = 0;
   bool prod = false;
+  // Tiled walk (rank 4): tq = the output index (0..2) along which the sources
+  // in the mask tr have unit stride; they are read along it and turned round
+  // in LDS.  tq < 0: element walk.  ttotal: tiles over all boxes.
+  int tq = -1;
+  int tr = 0;
+  long ttotal = 0;
 };
 
 // Source description for plan building: layout and, for each source

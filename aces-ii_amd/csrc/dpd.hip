@@ -108,6 +108,53 @@ PairSpace make_pairs(const Space& s, bool strict) {
 
 // ---------------------------------------------------------------------------
 
+// Choose the tiled walk for a rank-4 plan from its largest box: a source is
+// read through LDS when its stride along output index 3 is large and it has
+// unit stride along an index q of 0..2; all such sources must agree on q.
+// (The tiled kernel uses every box's own strides, so this choice only decides
+// speed.)  Sets p.tq, p.tr, p.ttotal and Box::tfirst.
+static void plan_tiles(Plan& p, int rank) {
+  p.tq = -1;
+  p.tr = 0;
+  p.ttotal = 0;
+  if (rank != 4 || p.nbox == 0) return;
+  const Box* big = &p.host[0];
+  long best = 0;
+  for (const Box& b : p.host) {
+    const long e = (long)b.d[0] * b.d[1] * b.d[2] * b.d[3];
+    if (e > best) {
+      best = e;
+      big = &b;
+    }
+  }
+  if (big->d[3] < 16) return;
+  int q = -1, mask = 0;
+  for (int s = 0; s < p.nsrc; ++s) {
+    const long s3 = big->ss[s][3] < 0 ? -big->ss[s][3] : big->ss[s][3];
+    if (s3 < 8) continue;
+    int k = -1;
+    for (int kk = 2; kk >= 0; --kk)
+      if (big->d[kk] >= 8 && big->ss[s][kk] == 1) {
+        k = kk;
+        break;
+      }
+    if (k < 0) continue;
+    if (q >= 0 && q != k) return;
+    q = k;
+    mask |= 1 << s;
+  }
+  if (q < 0) return;
+  const int ra = (q == 0) ? 1 : 0, rb = (q == 2) ? 1 : 2;
+  long t = 0;
+  for (Box& b : p.host) {
+    b.tfirst = t;
+    t += (long)((b.d[3] + kTile3 - 1) / kTile3) * ((b.d[q] + kTileQ - 1) / kTileQ) * b.d[ra] * b.d[rb];
+  }
+  p.tq = q;
+  p.tr = mask;
+  p.ttotal = t;
+}
+
 int build_boxes(Plan& p, const Layout& out, const std::vector<Src>& srcs, const Src* X, const Src* Y) {
   ACES_REQUIRE(out.rank == 2 || out.rank == 4, "dpd plan: output rank must be 2 or 4");
   ACES_REQUIRE(srcs.size() <= 3, "dpd plan: at most three sources");
@@ -173,6 +220,7 @@ int build_boxes(Plan& p, const Layout& out, const std::vector<Src>& srcs, const 
   }
   p.nbox = (int)p.host.size();
   p.total = first;  // chunks
+  plan_tiles(p, out.rank);
   return ACES_OK;
 }
 
@@ -240,6 +288,92 @@ __global__ __launch_bounds__(256) void apply_plan_kernel(ApplyArgs a) {
   }
 }
 
+// Tiled walk: one workgroup per (box, 64 x 32 tile of output indices 3 and tq,
+// one value of each remaining index).  Sources in the mask tr are read along
+// tq (their unit-stride index) and turned round through LDS; the others, the
+// denominator and the product term follow the output order.
+__device__ inline int box_of_tile(const Box* boxes, int nbox, long tile) {
+  int lo = 0, hi = nbox - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (boxes[mid].tfirst <= tile)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void apply_plan_tiled_kernel(ApplyArgs a, int tq, int tr) {
+  __shared__ double tile[kTile3][kTileQ + 1];
+  __shared__ int sb;
+  if (threadIdx.x == 0) sb = box_of_tile(a.boxes, a.nbox, blockIdx.x);
+  __syncthreads();
+  const Box& b = a.boxes[sb];
+  const int ra = (tq == 0) ? 1 : 0, rb = (tq == 2) ? 1 : 2;
+  const int d3 = b.d[3], dq = b.d[tq];
+  const long n3 = (d3 + kTile3 - 1) / kTile3, nq = (dq + kTileQ - 1) / kTileQ;
+  long t = blockIdx.x - b.tfirst;
+  const int t3 = (int)(t % n3);
+  t /= n3;
+  const int tqi = (int)(t % nq);
+  t /= nq;
+  const int ib = (int)(t % b.d[rb]);
+  const int ia = (int)(t / b.d[rb]);
+  const int i3b = t3 * kTile3, iqb = tqi * kTileQ;
+  const int tid = threadIdx.x;
+  const int c3 = tid & 63, r0 = tid >> 6;  // output walk: rows iqb + r0 + 4k, column i3b + c3
+  const int cq = tid & 31, s0 = tid >> 5;  // LDS reads: column iqb + cq, rows i3b + s0 + 8k
+  const bool ok3 = i3b + c3 < d3;
+  double acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+  for (int s = 0; s < a.nsrc; ++s) {
+    const long* st = b.ss[s];
+    const double al = a.alpha[s];
+    const double* base = a.src[s] + b.sb[s] + ia * st[ra] + ib * st[rb];
+    if ((tr >> s) & 1) {
+      const bool okq = iqb + cq < dq;
+      const double* src = base + (long)(iqb + cq) * st[tq] + (long)(i3b + s0) * st[3];
+      __syncthreads();  // the previous reader of the tile is done
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        tile[s0 + 8 * k][cq] = (okq && i3b + s0 + 8 * k < d3) ? src[(long)(8 * k) * st[3]] : 0.0;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] += al * tile[c3][r0 + 4 * k];
+    } else {
+      const double* src = base + (long)(iqb + r0) * st[tq] + (long)(i3b + c3) * st[3];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (ok3 && iqb + r0 + 4 * k < dq) acc[k] += al * src[(long)(4 * k) * st[tq]];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int iq = iqb + r0 + 4 * k;
+    if (!ok3 || iq >= dq) continue;
+    int l[4];
+    l[ra] = ia;
+    l[rb] = ib;
+    l[tq] = iq;
+    l[3] = i3b + c3;
+    double v = acc[k];
+    if (a.den.pow != 0) {
+      double D = 0.0;
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        if (a.den.sign[m] != 0) D += a.den.sign[m] * a.den.e[m][a.den.off[m][b.g[m]] + l[m]];
+      v = (a.den.pow > 0) ? v * D : v / D;
+    }
+    if (a.X != nullptr && b.xb >= 0)
+      v += a.gamma * a.X[b.xb + l[0] * b.xs[0] + l[1] * b.xs[1] + l[2] * b.xs[2] + l[3] * b.xs[3]] *
+           a.Y[b.yb + l[0] * b.ys[0] + l[1] * b.ys[1] + l[2] * b.ys[2] + l[3] * b.ys[3]];
+    double* o = a.out + b.ob + l[0] * b.os[0] + l[1] * b.os[1] + l[2] * b.os[2] + l[3] * b.os[3];
+    *o = (a.beta == 0.0) ? v : a.beta * (*o) + v;
+  }
+}
+
 int apply_plan(const Plan& p, double* out, double beta, const double* const* src, const double* alpha,
                const double* X, const double* Y, double gamma, const Denom* den, hipStream_t s) {
   if (p.nbox == 0 || p.total == 0) return ACES_OK;
@@ -258,6 +392,12 @@ int apply_plan(const Plan& p, double* out, double beta, const double* const* src
   a.Y = p.prod ? Y : nullptr;
   a.gamma = gamma;
   if (den) a.den = *den;
+  if (p.tq >= 0) {
+    ACES_REQUIRE(p.ttotal < (1L << 31), "dpd apply: too many tiles");
+    hipLaunchKernelGGL(apply_plan_tiled_kernel, dim3((unsigned)p.ttotal), dim3(256), 0, s, a, p.tq, p.tr);
+    ACES_HIP(hipGetLastError());
+    return ACES_OK;
+  }
   ACES_REQUIRE(p.total < (1L << 31), "dpd apply: too many chunks");
   hipLaunchKernelGGL(apply_plan_kernel, dim3((unsigned)p.total), dim3(256), 0, s, a);
   ACES_HIP(hipGetLastError());
