@@ -110,7 +110,8 @@ PairSpace make_pairs(const Space& s, bool strict) {
 
 // Choose the tiled walk for a rank-4 plan from its largest box: a source is
 // read through LDS when its stride along output index 3 is large and it has
-// unit stride along an index q of 0..2; all such sources must agree on q.
+// unit stride along an index q of 0..2; all such sources must agree on q
+// (with none, q is the largest of indices 0..2).
 // (The tiled kernel uses every box's own strides, so this choice only decides
 // speed.)  Sets p.tq, p.tr, p.ttotal and Box::tfirst.
 static void plan_tiles(Plan& p, int rank) {
@@ -143,7 +144,13 @@ static void plan_tiles(Plan& p, int rank) {
     q = k;
     mask |= 1 << s;
   }
-  if (q < 0) return;
+  if (q < 0) {
+    // every source reads in the output's order: tile over the largest of
+    // indices 0..2 (no LDS turn), which drops the per-element index division
+    q = 2;
+    for (int kk = 1; kk >= 0; --kk)
+      if (big->d[kk] > big->d[q]) q = kk;
+  }
   const int ra = (q == 0) ? 1 : 0, rb = (q == 2) ? 1 : 2;
   long t = 0;
   for (Box& b : p.host) {

@@ -769,6 +769,36 @@ __global__ void splitk_reduce_kernel(int M, int N, int splits, int batch, const 
   }
 }
 
+// The split-K reductions of every group of one grouped launch in one launch
+// (blockIdx.y = group; groups without splits have splits = 0).
+struct ReduceGroups {
+  int n;
+  struct G {
+    int M, N, splits;
+    const double* ws;
+    long zC;
+    double* C;
+    long ldc;
+    double beta;
+  } g[kMaxGroups];
+};
+
+__global__ void splitk_reduce_groups_kernel(ReduceGroups rg) {
+  ReduceGroups::G q = rg.g[0];
+#pragma unroll
+  for (int k = 1; k < kMaxGroups; ++k)
+    if (k == (int)blockIdx.y) q = rg.g[k];
+  if (q.splits <= 1) return;
+  const long total = (long)q.M * q.N;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const long row = idx / q.N, col = idx - row * q.N;
+    double acc = 0.0;
+    for (int z = 0; z < q.splits; ++z) acc += q.ws[z * q.zC + idx];
+    double* c = q.C + row * q.ldc + col;
+    *c = (q.beta == 0.0) ? acc : acc + q.beta * (*c);
+  }
+}
+
 // Tile configuration of the fp64/fp32 kernel: BK, waves along M/N, MFMA
 // 16x16 blocks per wave along M/N.
 // Measured on MI355X (o=48, v=320 contraction shapes): fp64 runs best with
@@ -1048,8 +1078,32 @@ int gemm_grouped(const GemmDesc* d, int ng, GemmPrec prec, hipStream_t stream) {
     if (d[g].M > 0 && d[g].N > 0) kz = (d[g].K == 0);
   dim3 grid((unsigned)maxnb, (unsigned)ng, (unsigned)maxsplit);
   ACES_CHECK(launch_groups(gg, prec, d[0].a_kc != 0, d[0].b_kc != 0, kz != 0, grid, wm, bf16_direct(d, ng), stream));
-  for (int g = 0; g < ng; ++g)
-    if (d[g].M > 0 && d[g].N > 0) ACES_CHECK(reduce_splits(d[g], gg.g[g], stream));
+  // one launch for the split-K sums of all groups (same order of additions
+  // as the single-problem reduction)
+  ReduceGroups rg;
+  rg.n = ng;
+  long most = 0;
+  for (int g = 0; g < kMaxGroups; ++g) {
+    ReduceGroups::G& q = rg.g[g];
+    q.splits = 0;
+    if (g >= ng || d[g].M == 0 || d[g].N == 0 || gg.g[g].splits <= 1) continue;
+    q.M = d[g].M;
+    q.N = d[g].N;
+    q.splits = gg.g[g].splits;
+    q.ws = gg.g[g].C;
+    q.zC = gg.g[g].zC;
+    q.C = d[g].C;
+    q.ldc = d[g].ldc;
+    q.beta = d[g].beta;
+    most = std::max(most, (long)q.M * q.N);
+  }
+  if (most > 0) {
+    long gx = (most + 255) / 256;
+    if (gx > 65536) gx = 65536;
+    hipLaunchKernelGGL(splitk_reduce_groups_kernel, dim3((unsigned)gx, (unsigned)kMaxGroups), dim3(256), 0, stream,
+                       rg);
+    ACES_HIP(hipGetLastError());
+  }
   return ACES_OK;
 }
 
