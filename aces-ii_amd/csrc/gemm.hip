@@ -625,18 +625,52 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
 
 // ---------------------------------------------------------------------------
 // fp64 kernel with direct-to-LDS operand loads (global_load_lds_dwordx4) into
-// an S-stage LDS ring; both operands k-contiguous, K % 16 == 0.  Block
-// (32 FM) x (32 FN) on 2 x 2 waves, each a (16 FM) x (16 FN) tile of
-// v_mfma_f64_16x16x4_f64.  LDS rows hold 16 doubles (128 bytes, eight 16-byte
-// chunks); chunk q of row r sits at position q ^ (r & 7), so the 16 rows of
-// one fragment read fall on at most two rows per bank group.  One wave
-// instruction moves 8 rows x 128 bytes (lane L: row L / 8, position L % 8).
+// an S-stage LDS ring; K % 16 == 0.  Block (32 FM) x (32 FN) on 2 x 2 waves,
+// each a (16 FM) x (16 FN) tile of v_mfma_f64_16x16x4_f64.
+// LDS layout per operand: k-contiguous (KC) rows hold 16 doubles (128 bytes,
+// eight 16-byte chunks) with chunk q of row r at position q ^ (r & 7); a
+// row-contiguous operand keeps its own [k][row] order (one k row of the tile
+// is 8 ROWS bytes).  Either way one wave instruction moves 1 KB (64 lanes x
+// 16 bytes) and the 16 rows of one fragment read hit distinct banks (KC: at
+// most two rows per bank group).  Rows past M (N) are clamped to the last row
+// (KC) or the last row pair (row-contiguous, even row counts): they only feed
+// accumulators that are never stored.
 // ---------------------------------------------------------------------------
-template <int FM, int FN, int S>
+template <bool KC, int ROWS>
+struct DlOperand {
+  static constexpr int BK = 16;
+  static constexpr int NI = ROWS / 8;  // 1 KB wave instructions per stage
+  // global pointer (this lane) and LDS offset (wave-uniform) of instruction G
+  __device__ static inline void map(const double* base, long ld, int row0, int nrows, int G, int lane,
+                                    const double*& gp, int& off) {
+    if (KC) {
+      const int r = lane >> 3;
+      int row = row0 + G * 8 + r;
+      row = row < nrows ? row : nrows - 1;
+      gp = base + (long)row * ld + (((lane & 7) ^ r) << 1);
+      off = G * 8 * BK;
+    } else {
+      constexpr int HALF = ROWS / 2;  // lanes per k row
+      const int kr = G * (128 / ROWS) + lane / HALF;
+      int m = row0 + 2 * (lane % HALF);
+      m = m <= nrows - 2 ? m : nrows - 2;
+      gp = base + (long)kr * ld + m;
+      off = G * 128;
+    }
+  }
+  __device__ static inline long kstep(long ld) { return KC ? BK : BK * ld; }
+  __device__ static inline double read(const double* st, int r, int k) {
+    return KC ? st[r * BK + (((k >> 1) ^ (r & 7)) << 1) + (k & 1)] : st[k * ROWS + r];
+  }
+};
+
+template <bool AKC, bool BKC, int FM, int FN, int S>
 __global__ __launch_bounds__(256) void gemm_f64_dl_kernel(GemmGroups gg) {
   constexpr int BK = 16, NW = 4;
   constexpr int BM = 32 * FM, BN = 32 * FN;
-  constexpr int GA = BM / 8, GB = BN / 8, NG = GA + GB, IPW = NG / NW;
+  typedef DlOperand<AKC, BM> OA;
+  typedef DlOperand<BKC, BN> OB;
+  constexpr int NG = OA::NI + OB::NI, IPW = NG / NW;
   constexpr int SA = BM * BK, STAGE = (BM + BN) * BK;  // doubles
   static_assert(NG % NW == 0, "groups per wave");
   __shared__ __attribute__((aligned(16))) double lds[S * STAGE];
@@ -650,8 +684,8 @@ __global__ __launch_bounds__(256) void gemm_f64_dl_kernel(GemmGroups gg) {
   const int n0 = tn * BN;
   const int bz = (gg.n > 1) ? 0 : (int)blockIdx.y;
   const long kz = (long)blockIdx.z * p.kchunk;
-  const double* __restrict__ Ag = reinterpret_cast<const double*>(p.A) + (long)bz * p.sA + kz;
-  const double* __restrict__ Bg = reinterpret_cast<const double*>(p.B) + (long)bz * p.sB + kz;
+  const double* __restrict__ Ag = reinterpret_cast<const double*>(p.A) + (long)bz * p.sA + kz * p.zkA;
+  const double* __restrict__ Bg = reinterpret_cast<const double*>(p.B) + (long)bz * p.sB + kz * p.zkB;
   double* __restrict__ C = p.C + (long)bz * p.sC + (long)blockIdx.z * p.zC;
   const int M = p.M, N = p.N;
   const int K = (int)((p.K - kz) < p.kchunk ? (p.K - kz) : p.kchunk);
@@ -662,31 +696,27 @@ __global__ __launch_bounds__(256) void gemm_f64_dl_kernel(GemmGroups gg) {
   const int wm = wave >> 1;
   const int wn = wave & 1;
 
-  const int drow = lane >> 3;
-  const int dchunk = (lane & 7) ^ drow;  // rows of a group start at multiples of 8: r & 7 = drow
   const double* gp[IPW];
   int goff[IPW];
+  long gstep[IPW];
 #pragma unroll
   for (int j = 0; j < IPW; ++j) {
     const int G = wave * IPW + j;
-    const double* base;
-    long ld;
-    int row, nrows, off;
-    if (G < GA) {
-      base = Ag; ld = p.lda; row = m0 + G * 8 + drow; nrows = M; off = G * 8 * BK;
+    if (G < OA::NI) {
+      OA::map(Ag, p.lda, m0, M, G, lane, gp[j], goff[j]);
+      gstep[j] = OA::kstep(p.lda);
     } else {
-      base = Bg; ld = p.ldb; row = n0 + (G - GA) * 8 + drow; nrows = N; off = SA + (G - GA) * 8 * BK;
+      OB::map(Bg, p.ldb, n0, N, G - OA::NI, lane, gp[j], goff[j]);
+      goff[j] += SA;
+      gstep[j] = OB::kstep(p.ldb);
     }
-    row = row < nrows ? row : nrows - 1;  // clamped rows feed accumulators that are never stored
-    gp[j] = base + (long)row * ld + dchunk * 2;
-    goff[j] = off;
   }
   auto issue = [&](int kt) {
 #if __HIP_DEVICE_COMPILE__
     double* st = lds + (kt % S) * STAGE;
 #pragma unroll
     for (int j = 0; j < IPW; ++j)
-      __builtin_amdgcn_global_load_lds(gp[j] + kt * BK, (__attribute__((address_space(3))) void*)(st + goff[j]),
+      __builtin_amdgcn_global_load_lds(gp[j] + kt * gstep[j], (__attribute__((address_space(3))) void*)(st + goff[j]),
                                        16, 0, 0);
 #endif
   };
@@ -719,15 +749,9 @@ __global__ __launch_bounds__(256) void gemm_f64_dl_kernel(GemmGroups gg) {
       const int k = kk * 4 + lk;
       double a[FM], b[FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i) {
-        const int r = wm * 16 * FM + i * 16 + lr;
-        a[i] = st[r * BK + (((k >> 1) ^ (r & 7)) << 1) + (k & 1)];
-      }
+      for (int i = 0; i < FM; ++i) a[i] = OA::read(st, wm * 16 * FM + i * 16 + lr, k);
 #pragma unroll
-      for (int j = 0; j < FN; ++j) {
-        const int r = wn * 16 * FN + j * 16 + lr;
-        b[j] = st[SA + r * BK + (((k >> 1) ^ (r & 7)) << 1) + (k & 1)];
-      }
+      for (int j = 0; j < FN; ++j) b[j] = OB::read(st + SA, wn * 16 * FN + j * 16 + lr, k);
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -861,23 +885,24 @@ static int bf16_shape(const GemmDesc* d, int ng) {
     }
   return !any ? 2 : (wide ? 8 : 4);
 }
-// fp64 problems with both operands k-contiguous, 16-byte aligned rows,
-// K % 16 == 0, K >= 256 and a tall narrow output (M >= 1024, N < 4096: the
-// particle ladder, the t1 contractions) run on the direct-to-LDS kernel
-// (128 x 64 blocks);
-// square (ov)^3 products keep the 64 x 64 register-staged kernel, which is as
-// fast there.
+// fp64 problems with 16-byte aligned rows, K % 16 == 0, K >= 256 and
+// M >= 256 run on the direct-to-LDS kernel (128 x 64 blocks), except products
+// of two k-contiguous operands that are not tall and narrow (M >= 1024,
+// N < 4096), which run as fast on the 64 x 64 register-staged kernel.
 static constexpr int kShapeF64Dl = 16;
 static constexpr int kF64DlFM = 4, kF64DlFN = 2, kF64DlStages = 3;
 static bool f64_direct(const GemmDesc* d, int ng) {
   bool any = false;
   for (int g = 0; g < ng; ++g)
     if (d[g].M > 0 && d[g].N > 0) {
-      if (!d[g].a_kc || !d[g].b_kc || d[g].K < 256 || d[g].K % 16 != 0 || d[g].N >= 4096 || d[g].M < 1024)
-        return false;
-      if ((uintptr_t)d[g].A % 16 || (uintptr_t)d[g].B % 16 || d[g].lda % 2 || d[g].ldb % 2 || d[g].sA % 2 ||
-          d[g].sB % 2)
-        return false;
+      const GemmDesc& e = d[g];
+      if (e.K < 256 || e.K % 16 != 0 || e.M < 256) return false;
+      // k-contiguous products: tall narrow outputs only (the others run as fast
+      // on the register-staged kernel)
+      if (e.a_kc && e.b_kc && (e.N >= 4096 || e.M < 1024)) return false;
+      if ((uintptr_t)e.A % 16 || (uintptr_t)e.B % 16 || e.lda % 2 || e.ldb % 2 || e.sA % 2 || e.sB % 2) return false;
+      // row-contiguous operands are loaded in row pairs
+      if ((!e.a_kc && e.M % 2) || (!e.b_kc && e.N % 2)) return false;
       any = true;
     }
   return any;
@@ -984,10 +1009,22 @@ static int launch_groups(const GemmGroups& gg, GemmPrec prec, bool akc, bool bkc
   } else {
     switch (prec) {
       case GP_F64:
-        if (bf_wm == kShapeF64Dl)
-          hipLaunchKernelGGL((gemm_f64_dl_kernel<kF64DlFM, kF64DlFN, kF64DlStages>), grid, dim3(256), 0, stream, gg);
-        else
+        if (bf_wm == kShapeF64Dl) {
+#define F64DL(AK, BK2)                                                                                        \
+  hipLaunchKernelGGL((gemm_f64_dl_kernel<AK, BK2, kF64DlFM, kF64DlFN, kF64DlStages>), grid, dim3(256), 0, stream, \
+                     gg)
+          if (akc && bkc)
+            F64DL(true, true);
+          else if (akc)
+            F64DL(true, false);
+          else if (bkc)
+            F64DL(false, true);
+          else
+            F64DL(false, false);
+#undef F64DL
+        } else {
           launch16<double>(gg, akc, bkc, grid, stream);
+        }
         break;
       case GP_F32:
         launch16<float>(gg, akc, bkc, grid, stream);

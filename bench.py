@@ -416,7 +416,7 @@ def bench_lambda(args):
     lam_cycles = icycle
     adj = prof['adjoint']
     achieved = adj['flops'] / (adj['ms'] * 1e-3)
-    roofline = {'bound': 'mfma', 'kernel': 'gemm16_kernel<double> (reverse-pass contractions)',
+    roofline = {'bound': 'mfma', 'kernel': 'gemm16_kernel<double> / gemm_f64_dl_kernel (reverse-pass contractions)',
                 'achieved': achieved / 1e12, 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
                 'frac': achieved / FP64_PEAK, 'traffic': None, 'avg_launch_ms': adj['ms'] / adj['launches'],
                 'launches_per_cycle': adj['launches'] / nprof,

@@ -201,3 +201,24 @@ def test_f64_direct_lds(M, N, K, ws):
     torch.cuda.synchronize()
     ref = -0.5 * (A @ B.t()) + 1.5 * C0
     assert torch.allclose(C, ref, rtol=1e-12, atol=1e-12 * K ** 0.5)
+
+
+@pytest.mark.parametrize('a_kc,b_kc', [(1, 0), (0, 1), (0, 0)])
+@pytest.mark.parametrize('M,N,K,ws', [(1030, 500, 528, False), (300, 90, 4096, True), (1030, 501, 528, False)])
+def test_f64_direct_lds_layouts(a_kc, b_kc, M, N, K, ws):
+    """Row-contiguous operands on the direct-to-LDS fp64 kernel ([k][row] LDS
+    tiles, row pairs clamped at the edge); an odd row count takes the
+    register-staged kernel."""
+    print('case f64 dl layout', a_kc, b_kc, M, N, K, ws, flush=True)
+    g = torch.Generator(device='cuda').manual_seed(M * 3 + N + K + a_kc + 2 * b_kc)
+    Am = torch.randn(M, K, dtype=torch.float64, device='cuda', generator=g)
+    Bm = torch.randn(N, K, dtype=torch.float64, device='cuda', generator=g)
+    C0 = torch.randn(M, N, dtype=torch.float64, device='cuda', generator=g)
+    A = Am.contiguous() if a_kc else Am.t().contiguous()
+    B = Bm.contiguous() if b_kc else Bm.t().contiguous()
+    C = C0.clone()
+    wsb = torch.empty(1 << 22, dtype=torch.float64, device='cuda') if ws else None
+    _grouped(0, [{'M': M, 'N': N, 'K': K, 'A': A, 'B': B, 'C': C}], a_kc, b_kc, 0.25, -1.0, wsb)
+    torch.cuda.synchronize()
+    ref = 0.25 * (Am @ Bm.t()) - C0
+    assert torch.allclose(C, ref, rtol=1e-12, atol=1e-12 * K ** 0.5)
