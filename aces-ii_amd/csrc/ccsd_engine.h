@@ -152,6 +152,8 @@ class CcsdEngine {
   };
   static constexpr int kLpStatic = 5, kLpSlots = 16;
   LowP lp_[kLpSlots];
+  bool lp_fresh_[kLpSlots] = {};  // slot already holds this cycle's operand (bf16x3)
+  int lowp_slot(int slot, long rows, long K);
   GemmPrec lp_static_prec_ = GP_F64;
   // DIIS (amplitudes and multipliers)
   DiisState tdiis_, ldiis_;

@@ -295,6 +295,23 @@ int CcsdEngine::lowp_static(GemmPrec prec) {
   return ACES_OK;
 }
 
+// Make operand slot `slot` hold rows x pad32(K) low-precision elements
+// (allocated zero-filled, so padding columns stay zero).
+int CcsdEngine::lowp_slot(int slot, long rows, long K) {
+  LowP& L = lp_[slot];
+  const long Kp = pad32(std::max<long>(K, 1));
+  if (L.hi == nullptr || L.ld != Kp || L.rows < rows) {
+    void* blk;
+    ACES_CHECK(alloc_raw(&blk, rows * Kp * 4));
+    ACES_HIP(hipMemsetAsync(blk, 0, rows * Kp * 4, s_));
+    L.hi = blk;
+    L.lo = (char*)blk + rows * Kp * 2;
+    L.ld = Kp;
+    L.rows = rows;
+  }
+  return ACES_OK;
+}
+
 int CcsdEngine::gemm64(GemmDesc d) {
   d.ws = ws_;
   d.ws_elems = ws_elems_;
@@ -366,15 +383,10 @@ int CcsdEngine::gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A
   const int kcs[2] = {a_kc, b_kc};
   for (int q = 0; q < 2; ++q) {
     LowP& L = lp_[slots[q]];
-    if (slots[q] >= kLpStatic) {
-      if (L.hi == nullptr || L.ld != Kp || L.rows < rows[q]) {
-        void* blk;
-        ACES_CHECK(alloc_raw(&blk, rows[q] * Kp * 4));
-        L.hi = blk;
-        L.lo = (char*)blk + rows[q] * Kp * 2;
-        L.ld = Kp;
-        L.rows = rows[q];
-      }
+    if (slots[q] >= kLpStatic && lp_fresh_[slots[q]] && prec == GP_BF16X3) {
+      lp_fresh_[slots[q]] = false;  // written in this precision by the fused operand kernel
+    } else if (slots[q] >= kLpStatic) {
+      ACES_CHECK(lowp_slot(slots[q], rows[q], K));
       if (prec == GP_F32 && kcs[q])
         ACES_CHECK(convert_f32((int)rows[q], K, src[q], lds[q], (float*)L.hi, Kp, s_));
       else if (prec == GP_F32)
@@ -411,12 +423,30 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
 
   // tau = t2 + t1 t1
   ACES_CHECK(P4(o, o, v, v, tau_).src(t2, 1, ov2, vv, v, 1).prod(t1, v, 0, 1, 0, t1, 0, v, 0, 1, 1.0).run(s));
-  // U[(i,a)][(j,b)] = 2 t_ijab - t_ijba ; X = t_ijba
-  ACES_CHECK(P4(o, v, o, v, U_).src(t2, 2, ov2, v, vv, 1).src(t2, -1, ov2, 1, vv, v).run(s));
-  ACES_CHECK(P4(o, v, o, v, X_).src(t2, 1, ov2, 1, vv, v).run(s));
-  // Ut = U - 2 t_la t_id ; Xt = X + 2 t_id t_la   (output index order i, a, l, d)
-  ACES_CHECK(P4(o, v, o, v, Ut_).src(U_, 1, ov2, ov, v, 1).prod(t1, 0, 1, v, 0, t1, v, 0, 0, 1, -2.0).run(s));
-  ACES_CHECK(P4(o, v, o, v, Xt_).src(X_, 1, ov2, ov, v, 1).prod(t1, 0, 1, v, 0, t1, v, 0, 0, 1, 2.0).run(s));
+  // U[(i,a)][(j,b)] = 2 t_ijab - t_ijba ; X = t_ijba ; Ut = U - 2 t_ja t_ib ; Xt = X + 2 t_ja t_ib,
+  // one pass; the bf16 form writes the split operands of the W-build and ring
+  // products directly (slots 11 U, 12 X, 7 Ut, 8 Xt) and keeps only U in fp64
+  for (bool& f : lp_fresh_) f = false;
+  {
+    RingOperands r;
+    r.f64[0] = U_;
+    if (prec == GP_BF16X3) {
+      const int slot[4] = {11, 12, 7, 8};
+      for (int q = 0; q < 4; ++q) {
+        ACES_CHECK(lowp_slot(slot[q], ov, ov));
+        r.hi[q] = (unsigned short*)lp_[slot[q]].hi;
+        r.lo[q] = (unsigned short*)lp_[slot[q]].lo;
+      }
+      r.ldl = lp_[slot[0]].ld;
+      ACES_CHECK(ring_operands(o, v, t1, t2, r, s));
+      for (int q = 0; q < 4; ++q) lp_fresh_[slot[q]] = true;
+    } else {
+      r.f64[1] = X_;
+      r.f64[2] = Ut_;
+      r.f64[3] = Xt_;
+      ACES_CHECK(ring_operands(o, v, t1, t2, r, s));
+    }
+  }
 
   // T1 parts of the W(mbej) intermediates
   prof_cat_ = 3;
@@ -440,13 +470,17 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(gemm_lp(prec, OV, OV, OV, Wt_, 9, ov, U_, 11, ov, R1_, ov, 1.0, 0.0));
   ACES_CHECK(gemm_lp(prec, OV, OV, OV, Wv_, 10, ov, X_, 12, ov, R2_, ov, 1.0, 0.0));
 
-  // inc2 = <ij|ab> = (ia|jb)
-  ACES_CHECK(P4(o, o, v, v, inc2_).src(ovov_, 1, ov2, v, ov, 1).run(s));
-  // ring assembly: RS = (R1 - R2)/2 ; inc2 += RS[iajb] + RS[jbia] - R2[ibja] - R2[jaib]
-  ACES_CHECK(P4(1, 1, 1, ovvo, R1_, 0.5).src(R2_, -0.5, 0, 0, 0, 1).run(s));
-  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(R1_, 1, ov2, v, ov, 1).src(R1_, 1, v, ov2, 1, ov)
-                 .src(R2_, -1, ov2, v, 1, ov).run(s));
-  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(R2_, -1, v, ov2, ov, 1).run(s));
+  // inc2 = <ij|ab> + ring assembly, one pass:
+  //   inc2 = (ia|jb) + RS[iajb] + RS[jbia] - R2[ibja] - R2[jaib],  RS = (R1 - R2)/2
+  ACES_CHECK(P4(o, o, v, v, inc2_)
+                 .src(ovov_, 1, ov2, v, ov, 1)
+                 .src(R1_, 0.5, ov2, v, ov, 1)
+                 .src(R2_, -0.5, ov2, v, ov, 1)
+                 .src(R1_, 0.5, v, ov2, 1, ov)
+                 .src(R2_, -0.5, v, ov2, 1, ov)
+                 .src(R2_, -1, ov2, v, 1, ov)
+                 .src(R2_, -1, v, ov2, ov, 1)
+                 .run(s));
 
   // particle-particle ladder with the T1-dressing rows (MFMA)
   ACES_CHECK(pack_tau(o, v, tau_, Sp_, nvp_, Dp_, std::max<long>(nvm_, 1), s));

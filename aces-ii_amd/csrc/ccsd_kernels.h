@@ -9,6 +9,15 @@ int build_ladder_operands(int o, int v, const double* L233, const double* ovvv, 
                           long ldm, hipStream_t s);
 int unpack_ladder(int o, int v, const double* Zp, long ldzp, const double* Zm, long ldzm, double* inc2, double* Yb,
                   hipStream_t s);
+// Ring operands U, X, Ut, Xt ([(i,a)][(j,b)]) from t1 and t2 in one pass
+// (ccsd_kernels.hip); index q = 0 U, 1 X, 2 Ut, 3 Xt; null pointers skip.
+struct RingOperands {
+  double* f64[4] = {nullptr, nullptr, nullptr, nullptr};  // ld = o v
+  unsigned short* hi[4] = {nullptr, nullptr, nullptr, nullptr};
+  unsigned short* lo[4] = {nullptr, nullptr, nullptr, nullptr};
+  long ldl = 0;  // bf16 row length (>= o v, padding untouched)
+};
+int ring_operands(int o, int v, const double* t1, const double* t2, const RingOperands& r, hipStream_t s);
 int jacobi(int o, int v, const double* eo, const double* ev, const double* inc1, const double* inc2, double* t1n,
            double* t2n, hipStream_t s);
 int symmetrize_t2(int o, int v, double* t2, hipStream_t s);

@@ -7,6 +7,7 @@
 //   'PACK' (p <  q):  index = q*(q-1)/2 + p
 #include "common.h"
 #include "ccsd_kernels.h"
+#include "bf16_util.h"
 #include "pairs.h"
 
 namespace aces {
@@ -222,6 +223,76 @@ int unpack_ladder(int o, int v, const double* Zp, long ldzp, const double* Zm, l
   ACES_HIP(hipGetLastError());
   const long n = (long)v * o * o * o;
   hipLaunchKernelGGL(unpack_ladder_yb_kernel, dim3(grid1(n)), dim3(256), 0, s, o, v, Zp, ldzp, Zm, ldzm, Yb);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ring operands from t2[i][j][a][b] and t1[i][a], rows (i,a), columns (j,b):
+//   U = 2 t_ijab - t_ijba,  X = t_ijba,  Ut = U - 2 t1_ja t1_ib,  Xt = X + 2 t1_ja t1_ib
+// One block per (i, j, 32 values of a, 64 values of b): t_ijab is read along b,
+// t_ijba along a and turned round through LDS.  Each output goes to fp64
+// (ld = ov) and/or to bf16 hi/lo rows (ld = ldl) where the pointer is set.
+
+__global__ __launch_bounds__(256) void ring_operands_kernel(int o, int v, const double* __restrict__ t1,
+                                                            const double* __restrict__ t2, RingOperands r) {
+  __shared__ double tr[64][33];
+  const int nbt = (v + 63) / 64, nat = (v + 31) / 32;
+  long blk = blockIdx.x;
+  const int bt = (int)(blk % nbt);
+  blk /= nbt;
+  const int at = (int)(blk % nat);
+  blk /= nat;
+  const int j = (int)(blk % o);
+  const int i = (int)(blk / o);
+  const int a0 = at * 32, b0 = bt * 64;
+  const long vv = (long)v * v, ov = (long)o * v;
+  const double* T = t2 + ((long)i * o + j) * vv;  // T[a][b] = t_ijab
+  const int tid = threadIdx.x;
+  {
+    // t_ijba for a in [a0, a0+32), b in [b0, b0+64): read along a (T[b][a])
+    const int ca = tid & 31, rb = tid >> 5;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int b = b0 + rb + 8 * k, a = a0 + ca;
+      tr[rb + 8 * k][ca] = (a < v && b < v) ? T[(long)b * v + a] : 0.0;
+    }
+  }
+  __syncthreads();
+  const int cb = tid & 63, ra = tid >> 6;
+  const int b = b0 + cb;
+  if (b >= v) return;
+  const double tib = t1[(long)i * v + b];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int a = a0 + ra + 4 * k;
+    if (a >= v) break;
+    const double d = T[(long)a * v + b];
+    const double x = tr[cb][ra + 4 * k];
+    const double u = 2.0 * d - x;
+    const double p = 2.0 * t1[(long)j * v + a] * tib;
+    const double val[4] = {u, x, u - p, x + p};
+    const long row = (long)i * v + a, col = (long)j * v + b;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (r.f64[q]) r.f64[q][row * ov + col] = val[q];
+      if (r.hi[q]) {
+        const unsigned short h = bf16_rn((float)val[q]);
+        r.hi[q][row * r.ldl + col] = h;
+        r.lo[q][row * r.ldl + col] = bf16_rn((float)(val[q] - (double)bf16_to_f(h)));
+      }
+    }
+  }
+}
+
+int ring_operands(int o, int v, const double* t1, const double* t2, const RingOperands& r, hipStream_t s) {
+  const long nblk = (long)o * o * ((v + 31) / 32) * ((v + 63) / 64);
+  if (nblk == 0) return ACES_OK;
+  ACES_REQUIRE(nblk < (1L << 31), "ring_operands: too many tiles");
+  for (int q = 0; q < 4; ++q)
+    ACES_REQUIRE((r.hi[q] == nullptr) == (r.lo[q] == nullptr) && (r.hi[q] == nullptr || r.ldl >= (long)o * v),
+                 "ring_operands: bad bf16 output");
+  hipLaunchKernelGGL(ring_operands_kernel, dim3((unsigned)nblk), dim3(256), 0, s, o, v, t1, t2, r);
   ACES_HIP(hipGetLastError());
   return ACES_OK;
 }

@@ -10,7 +10,7 @@
 namespace aces {
 namespace eu {
 
-// Builder for one Permute4 launch (tensor_ops.h): up to three strided sources
+// Builder for one Permute4 launch (tensor_ops.h): up to kP4Src strided sources
 // and one product term, combined into a row-major 4-index output.
 struct P4 {
   Permute4 p;
@@ -21,6 +21,10 @@ struct P4 {
     p.beta = beta;
   }
   P4& src(const double* x, double al, long s0, long s1, long s2, long s3) {
+    if (ns >= kP4Src) {  // reported by run()
+      ++ns;
+      return *this;
+    }
     p.in[ns] = x;
     p.alpha[ns] = al;
     p.st[ns][0] = s0; p.st[ns][1] = s1; p.st[ns][2] = s2; p.st[ns][3] = s3;
@@ -34,7 +38,10 @@ struct P4 {
     p.ys[0] = y0; p.ys[1] = y1; p.ys[2] = y2; p.ys[3] = y3;
     return *this;
   }
-  int run(hipStream_t s) { return permute4(p, s); }
+  int run(hipStream_t s) {
+    ACES_REQUIRE(ns <= kP4Src, "permute4: too many sources");
+    return permute4(p, s);
+  }
 };
 
 inline GemmDesc gd(int M, int N, int K, const double* A, long lda, int akc, const double* B, long ldb, int bkc, double* C,

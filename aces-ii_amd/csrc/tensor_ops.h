@@ -15,17 +15,19 @@ int convert_f32_t(int rows, int cols, const double* x, long ldx, float* y, long 
 int split_bf16_t(int rows, int cols, const double* x, long ldx, unsigned short* hi, unsigned short* lo, long ldy,
                  hipStream_t s);
 
-// Generic 4-index permute/combine on a row-major output of dims d[0..3]:
+// Generic 4-index permute/combine on a row-major output of dims d[0..3]
+// (up to kP4Src sources):
 //   out[i] = beta*out[i] + sum_s alpha[s] * in[s][ sum_k i_k * st[s][k] ]
 //            + gamma * X[ sum_k i_k * xs[k] ] * Y[ sum_k i_k * ys[k] ]
 // Unused sources have in[s] == nullptr; the product term is skipped when X == nullptr.
+constexpr int kP4Src = 8;  // sources per permute
 struct Permute4 {
   long d[4] = {1, 1, 1, 1};
   double* out = nullptr;
   double beta = 0.0;
-  const double* in[3] = {nullptr, nullptr, nullptr};
-  double alpha[3] = {0.0, 0.0, 0.0};
-  long st[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+  const double* in[kP4Src] = {};
+  double alpha[kP4Src] = {};
+  long st[kP4Src][4] = {};
   const double* X = nullptr;
   const double* Y = nullptr;
   double gamma = 0.0;

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include "common.h"
 #include "tensor_ops.h"
+#include "bf16_util.h"
 
 namespace aces {
 
@@ -12,13 +13,6 @@ static inline unsigned grid_for(long n, int block, long cap = 1L << 20) {
   return (unsigned)g;
 }
 
-__device__ inline unsigned short bf16_rn(float f) {
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16);  // inf / nan
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
-__device__ inline float bf16_to_f(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
 
 __global__ void convert_f32_kernel(int rows, int cols, const double* __restrict__ x, long ldx,
                                    float* __restrict__ y, long ldy) {
@@ -178,9 +172,9 @@ struct Permute4Tile {
   double* out;
   double beta;
   int nd, nt;  // sources 0..nd-1 read directly, nd..nd+nt-1 through LDS
-  const double* in[3];
-  double alpha[3];
-  long st[3][4];
+  const double* in[kP4Src];
+  double alpha[kP4Src];
+  long st[kP4Src][4];
   const double* X;
   const double* Y;
   double gamma;
@@ -250,31 +244,37 @@ __global__ __launch_bounds__(256) void permute4_tiled_kernel(Permute4Tile p) {
 }
 
 static int permute4_simple(const Permute4& q, hipStream_t s) {
-  Permute4Args a;
-  a.d1 = q.d[1];
-  a.d2 = q.d[2];
-  a.d3 = q.d[3];
-  a.total = q.d[0] * q.d[1] * q.d[2] * q.d[3];
-  a.out = q.out;
-  a.beta = q.beta;
-  a.in0 = q.in[0];
-  a.in1 = q.in[1];
-  a.in2 = q.in[2];
-  a.a0 = q.alpha[0];
-  a.a1 = q.alpha[1];
-  a.a2 = q.alpha[2];
-  for (int k = 0; k < 4; ++k) {
-    a.s0[k] = q.st[0][k];
-    a.s1[k] = q.st[1][k];
-    a.s2[k] = q.st[2][k];
-    a.xs[k] = q.xs[k];
-    a.ys[k] = q.ys[k];
+  // three sources per launch; later launches accumulate (beta = 1)
+  int idx[kP4Src], n = 0;
+  for (int k = 0; k < kP4Src; ++k)
+    if (q.in[k]) idx[n++] = k;
+  for (int first = 0; first == 0 || first < n; first += 3) {
+    Permute4Args a;
+    a.d1 = q.d[1];
+    a.d2 = q.d[2];
+    a.d3 = q.d[3];
+    a.total = q.d[0] * q.d[1] * q.d[2] * q.d[3];
+    a.out = q.out;
+    a.beta = (first == 0) ? q.beta : 1.0;
+    const double** ins[3] = {&a.in0, &a.in1, &a.in2};
+    double* als[3] = {&a.a0, &a.a1, &a.a2};
+    long* sts[3] = {a.s0, a.s1, a.s2};
+    for (int j = 0; j < 3; ++j) {
+      const int k = first + j < n ? idx[first + j] : -1;
+      *ins[j] = k >= 0 ? q.in[k] : nullptr;
+      *als[j] = k >= 0 ? q.alpha[k] : 0.0;
+      for (int m = 0; m < 4; ++m) sts[j][m] = k >= 0 ? q.st[k][m] : 0;
+    }
+    for (int m = 0; m < 4; ++m) {
+      a.xs[m] = q.xs[m];
+      a.ys[m] = q.ys[m];
+    }
+    a.X = (first == 0) ? q.X : nullptr;
+    a.Y = (first == 0) ? q.Y : nullptr;
+    a.gamma = q.gamma;
+    hipLaunchKernelGGL(permute4_kernel, dim3(grid_for(a.total, 256)), dim3(256), 0, s, a);
+    ACES_HIP(hipGetLastError());
   }
-  a.X = q.X;
-  a.Y = q.Y;
-  a.gamma = q.gamma;
-  hipLaunchKernelGGL(permute4_kernel, dim3(grid_for(a.total, 256)), dim3(256), 0, s, a);
-  ACES_HIP(hipGetLastError());
   return ACES_OK;
 }
 
@@ -293,9 +293,10 @@ int permute4(const Permute4& q, hipStream_t s) {
   if (total == 0) return ACES_OK;
   ACES_REQUIRE(q.out != nullptr, "permute4: null output");
   ACES_REQUIRE(q.X == nullptr || q.Y != nullptr, "permute4: product term needs X and Y");
-  int fast[3] = {3, 3, 3};
+  int fast[kP4Src];
+  for (int k = 0; k < kP4Src; ++k) fast[k] = 3;
   bool any = false, alias = (q.X == q.out || q.Y == q.out);
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < kP4Src; ++k)
     if (q.in[k]) {
       fast[k] = permute4_fast_index(q, k);
       any |= fast[k] < 3;
@@ -305,7 +306,7 @@ int permute4(const Permute4& q, hipStream_t s) {
   // one launch per distinct tile index; the first also takes the direct
   // sources, the product term and beta, the later ones accumulate
   int qs[3], nqs = 0;
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < kP4Src; ++k)
     if (q.in[k] && fast[k] < 3) {
       bool seen = false;
       for (int j = 0; j < nqs; ++j) seen |= qs[j] == fast[k];
@@ -331,7 +332,7 @@ int permute4(const Permute4& q, hipStream_t s) {
     t.nt = 0;
     int slot = 0;
     if (L == 0)
-      for (int k = 0; k < 3; ++k)
+      for (int k = 0; k < kP4Src; ++k)
         if (q.in[k] && fast[k] == 3) {
           t.in[slot] = q.in[k];
           t.alpha[slot] = q.alpha[k];
@@ -339,7 +340,7 @@ int permute4(const Permute4& q, hipStream_t s) {
           ++slot;
           ++t.nd;
         }
-    for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < kP4Src; ++k)
       if (q.in[k] && fast[k] == t.q) {
         t.in[slot] = q.in[k];
         t.alpha[slot] = q.alpha[k];
@@ -347,7 +348,7 @@ int permute4(const Permute4& q, hipStream_t s) {
         ++slot;
         ++t.nt;
       }
-    for (int k = slot; k < 3; ++k) {
+    for (int k = slot; k < kP4Src; ++k) {
       t.in[k] = nullptr;
       t.alpha[k] = 0.0;
       for (int j = 0; j < 4; ++j) t.st[k][j] = 0;
