@@ -262,17 +262,22 @@ int CcsdEngine::energy_of(const double* t1, const double* t2, double& e) {
 }
 
 // ---------------------------------------------------------------------------
-// Low-precision operands.  Slots 0..3 static (converted once), 4..11 per call.
+// Low-precision operands.  Static slots (integral matrices) are converted
+// once per precision; the other slots per call.
 int CcsdEngine::lowp_static(GemmPrec prec) {
   if (lp_static_prec_ == prec) return ACES_OK;
   const long ov = (long)o * v;
   struct S { int slot; const double* src; long rows; long K; long ld; };
   const long oo = (long)o * o, vv = (long)v * v;
-  const S tab[kLpStatic] = {{0, Vp_, nvp_ + ov, nvp_, nvp_},
-                            {1, Vm_, nvm_ + ov, nvm_, std::max<long>(nvm_, 1)},
-                            {2, Lmat_, ov, ov, ov},
-                            {3, Kx_, ov, ov, ov},
-                            {4, OVkl_, oo, vv, vv}};
+  const S tab[] = {{0, Vp_, nvp_ + ov, nvp_, nvp_},
+                   {1, Vm_, nvm_ + ov, nvm_, std::max<long>(nvm_, 1)},
+                   {2, Lmat_, ov, ov, ov},
+                   {3, Kx_, ov, ov, ov},
+                   {4, OVkl_, oo, vv, vv},
+                   {16, ovvv_, vv * o, v, v},             // [(k,c,a)][d]
+                   {17, ovvv_kacd_, vv * o, v, v},        // [(k,a,c)][d]
+                   {18, M3_, v, ov * v, ov * v},          // [a][(k,c,d)]
+                   {19, Lv_, v, oo * v, oo * v}};         // [c][(k,l,d)]
   for (const S& t : tab) {
     LowP& L = lp_[t.slot];
     const long ld = pad32(std::max<long>(t.K, 1));
@@ -371,7 +376,7 @@ int CcsdEngine::gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A
     return timed_gemm(d, GP_F64, flops);
   }
   ACES_REQUIRE(aslot >= 0 && aslot < kLpSlots && bslot >= 0 && bslot < kLpSlots, "gemm_lp: bad operand slot");
-  ACES_REQUIRE((aslot >= kLpStatic || a_kc) && (bslot >= kLpStatic || b_kc),
+  ACES_REQUIRE((!lp_static_slot(aslot) || a_kc) && (!lp_static_slot(bslot) || b_kc),
                "gemm_lp: static operands are k-contiguous");
   ACES_CHECK(lowp_static(prec));
   const long Kp = pad32(std::max(K, 1));
@@ -383,9 +388,9 @@ int CcsdEngine::gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A
   const int kcs[2] = {a_kc, b_kc};
   for (int q = 0; q < 2; ++q) {
     LowP& L = lp_[slots[q]];
-    if (slots[q] >= kLpStatic && lp_fresh_[slots[q]] && prec == GP_BF16X3) {
+    if (!lp_static_slot(slots[q]) && lp_fresh_[slots[q]] && prec == GP_BF16X3) {
       lp_fresh_[slots[q]] = false;  // written in this precision by the fused operand kernel
-    } else if (slots[q] >= kLpStatic) {
+    } else if (!lp_static_slot(slots[q])) {
       ACES_CHECK(lowp_slot(slots[q], rows[q], K));
       if (prec == GP_F32 && kcs[q])
         ACES_CHECK(convert_f32((int)rows[q], K, src[q], lds[q], (float*)L.hi, Kp, s_));
@@ -448,10 +453,10 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
     }
   }
 
-  // T1 parts of the W(mbej) intermediates
+  // T1 parts of the W(mbej) intermediates (the two o v^3 reads in the ladder precision)
   prof_cat_ = 3;
-  ACES_CHECK(gemm64(gd((int)v2o, o, v, ovvv_, v, 1, t1, v, 1, P1_, o, 1.0, 0.0)));        // [(k,c,a)][i]
-  ACES_CHECK(gemm64(gd((int)v2o, o, v, ovvv_kacd_, v, 1, t1, v, 1, P2_, o, 1.0, 0.0)));   // [(k,a,c)][i]
+  ACES_CHECK(gemm_lp(prec, (int)v2o, o, v, ovvv_, 16, v, t1, 20, v, P1_, o, 1.0, 0.0));       // [(k,c,a)][i]
+  ACES_CHECK(gemm_lp(prec, (int)v2o, o, v, ovvv_kacd_, 17, v, t1, 21, v, P2_, o, 1.0, 0.0));  // [(k,a,c)][i]
   ACES_CHECK(gemm64(gd((int)o2v, v, o, M5c_, o, 1, t1, v, 0, Q12_, v, 1.0, 0.0)));        // [(k,c,i)][a]
   ACES_CHECK(gemm64(gd((int)o2v, v, o, M5b_, o, 1, t1, v, 0, Q2_, v, 1.0, 0.0)));         // [(k,c,i)][a]
   // Wt[(i,a)][(k,c)] = W0t + 2 P1[k,c,a,i] - P2[k,a,c,i] - Q12[k,c,i,a]
@@ -515,7 +520,7 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(gemm64(gd(o, o, (int)ov2, Lo_, ov2, 1, tau_, ov2, 1, Foo_, o, 1.0, 1.0)));
   ACES_CHECK(P4(v, o, o, v, tauK_).src(tau_, 1, v, ov2, vv, 1).run(s));  // tauK[a][k][l][d] = tau_klad
   ACES_CHECK(P4(1, 1, v, v, Fvv_).src(fock_ + (long)o * n + o, 1, 0, 0, n, 1).run(s));
-  ACES_CHECK(gemm64(gd(v, v, (int)(oo * v), tauK_, oo * v, 1, Lv_, oo * v, 1, Fvv_, v, -1.0, 1.0)));
+  ACES_CHECK(gemm_lp(prec, v, v, (int)(oo * v), tauK_, 23, oo * v, Lv_, 19, oo * v, Fvv_, v, -1.0, 1.0));
   ACES_CHECK(gemm64(gd(o, o, v, fov, n, 1, t1, v, 1, FT_, o, 1.0, 0.0)));       // FT[k][i] = f_kc t_ic
   ACES_CHECK(gemm64(gd(o, o, v, Fov_, v, 1, t1, v, 1, FovT_, o, 1.0, 0.0)));    // FovT[k][i]
   ACES_CHECK(P4(1, 1, o, o, Loo_).src(Foo_, 1, 0, 0, o, 1).src(FT_, 1, 0, 0, o, 1).run(s));
@@ -561,7 +566,7 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(gemm64(gd(o, v, v, t1, v, 1, Fvv_, v, 1, inc1_, v, 1.0, 1.0)));
   ACES_CHECK(gemv(OV, OV, U_, ov, Fov_, inc1_, 1.0, 1.0, s));
   ACES_CHECK(gemv(OV, OV, W0t_, ov, t1, inc1_, 1.0, 1.0, s));
-  ACES_CHECK(gemm64(gd(o, v, (int)ov2, tau_, ov2, 1, M3_, ov2, 1, inc1_, v, 1.0, 1.0)));
+  ACES_CHECK(gemm_lp(prec, o, v, (int)ov2, tau_, 22, ov2, M3_, 18, ov2, inc1_, v, 1.0, 1.0));
   ACES_CHECK(gemm64(gd(o, v, (int)o2v, M7_, o2v, 1, tauK_, o2v, 1, inc1_, v, -1.0, 1.0)));
   // the reverse pass needs fp64 intermediates of these amplitudes
   fwd_valid_ = (prec == GP_F64);
