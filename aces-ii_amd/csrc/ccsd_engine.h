@@ -152,7 +152,7 @@ class CcsdEngine {
   };
   // slots 0..4 and 16..19 hold static operands (converted once per precision),
   // the others this cycle's operands
-  static constexpr int kLpStatic = 5, kLpStatic2 = 16, kLpSlots = 24;
+  static constexpr int kLpStatic = 5, kLpStatic2 = 16, kLpSlots = 26;
   static constexpr bool lp_static_slot(int slot) { return slot < kLpStatic || (slot >= kLpStatic2 && slot < 20); }
   LowP lp_[kLpSlots];
   bool lp_fresh_[kLpSlots] = {};  // slot already holds this cycle's operand (bf16x3)

@@ -426,15 +426,16 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   const double* t2 = t2_;
   const int OV = (int)ov;
 
-  // tau = t2 + t1 t1
-  ACES_CHECK(P4(o, o, v, v, tau_).src(t2, 1, ov2, vv, v, 1).prod(t1, v, 0, 1, 0, t1, 0, v, 0, 1, 1.0).run(s));
-  // U[(i,a)][(j,b)] = 2 t_ijab - t_ijba ; X = t_ijba ; Ut = U - 2 t_ja t_ib ; Xt = X + 2 t_ja t_ib,
-  // one pass; the bf16 form writes the split operands of the W-build and ring
-  // products directly (slots 11 U, 12 X, 7 Ut, 8 Xt) and keeps only U in fp64
+  // One pass over t2: tau = t2 + t1 t1 ; U[(i,a)][(j,b)] = 2 t_ijab - t_ijba ;
+  // X = t_ijba ; Ut = U - 2 t_ja t_ib ; Xt = X + 2 t_ja t_ib.  The bf16 form
+  // writes the split operands of the W-build and ring products directly
+  // (slots 11 U, 12 X, 7 Ut, 8 Xt) and the split of t2 (slot 24), and keeps
+  // only U in fp64
   for (bool& f : lp_fresh_) f = false;
   {
     RingOperands r;
     r.f64[0] = U_;
+    r.tau = tau_;
     if (prec == GP_BF16X3) {
       const int slot[4] = {11, 12, 7, 8};
       for (int q = 0; q < 4; ++q) {
@@ -443,8 +444,13 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
         r.lo[q] = (unsigned short*)lp_[slot[q]].lo;
       }
       r.ldl = lp_[slot[0]].ld;
+      ACES_CHECK(lowp_slot(24, o2v, v));
+      r.t2hi = (unsigned short*)lp_[24].hi;
+      r.t2lo = (unsigned short*)lp_[24].lo;
+      r.ldt = lp_[24].ld;
       ACES_CHECK(ring_operands(o, v, t1, t2, r, s));
       for (int q = 0; q < 4; ++q) lp_fresh_[slot[q]] = true;
+      lp_fresh_[24] = true;
     } else {
       r.f64[1] = X_;
       r.f64[2] = Ut_;
@@ -488,7 +494,29 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
                  .run(s));
 
   // particle-particle ladder with the T1-dressing rows (MFMA)
-  ACES_CHECK(pack_tau(o, v, tau_, Sp_, nvp_, Dp_, std::max<long>(nvm_, 1), s));
+  {
+    PackTauOut po;
+    if (prec == GP_BF16X3) {  // the split operands of the two ladder products (slots 5, 6)
+      ACES_CHECK(lowp_slot(5, nop_, nvp_));
+      po.Shi = (unsigned short*)lp_[5].hi;
+      po.Slo = (unsigned short*)lp_[5].lo;
+      po.ldsl = lp_[5].ld;
+      lp_fresh_[5] = true;
+      if (nom_ > 0 && nvm_ > 0) {
+        ACES_CHECK(lowp_slot(6, nom_, nvm_));
+        po.Dhi = (unsigned short*)lp_[6].hi;
+        po.Dlo = (unsigned short*)lp_[6].lo;
+        po.lddl = lp_[6].ld;
+        lp_fresh_[6] = true;
+      }
+    } else {
+      po.S = Sp_;
+      po.lds = nvp_;
+      po.D = Dp_;
+      po.ldd = std::max<long>(nvm_, 1);
+    }
+    ACES_CHECK(pack_tau(o, v, tau_, po, s));
+  }
   prof_cat_ = 0;
   ACES_CHECK(gemm_lp(prec, (int)(nvp_ + ov), (int)nop_, (int)nvp_, Vp_, 0, nvp_, Sp_, 5, nvp_, Zp_, nop_, 1.0, 0.0));
   if (nom_ > 0 && nvm_ > 0)
@@ -535,15 +563,28 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(sub_diag(v, Lvv_, v, ev_, s));
 
   // FEACONT/FMICONT: tmpL = Lvv_ac t_ijcb - Loo_ki t_kjab ; inc2 += tmpL + tmpL^T
-  {
+  if (prec == GP_F64) {
     GemmDesc d = gd(v, v, v, Lvv_, v, 1, t2, v, 0, tmpL_, v, 1.0, 0.0);
     d.batch = (int)oo;
     d.sA = 0;
     d.sB = vv;
     d.sC = vv;
     ACES_CHECK(gemm64(d));
+    ACES_CHECK(gemm64(gd(o, (int)ov2, o, Loo_, o, 0, t2, ov2, 0, tmpL_, ov2, -1.0, 1.0)));
+  } else {
+    // With t_ijcb = t_jibc (closed-shell amplitudes; the engine's iterates are
+    // symmetrized) the transpose F[j][i][b][a] = tmpL[i][j][a][b] is one product
+    // with t2's own rows (j,i,b) as the left operand:
+    //   F[(j,i,b)][a] = sum_c t_jibc Lvv_ac,   F_j[i][(b,a)] -= sum_k Loo_ki t_jkba,
+    // and tmpL + tmpL^T = F + F^T.
+    ACES_CHECK(gemm_lp(prec, (int)o2v, v, v, t2, 24, v, Lvv_, 25, v, tmpL_, v, 1.0, 0.0));
+    GemmDesc d = gd(o, (int)vv, o, Loo_, o, 0, t2, vv, 0, tmpL_, vv, -1.0, 1.0);
+    d.batch = o;
+    d.sA = 0;
+    d.sB = ov2;
+    d.sC = ov2;
+    ACES_CHECK(gemm64(d));
   }
-  ACES_CHECK(gemm64(gd(o, (int)ov2, o, Loo_, o, 0, t2, ov2, 0, tmpL_, ov2, -1.0, 1.0)));
   ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(tmpL_, 1, ov2, vv, v, 1).src(tmpL_, 1, vv, ov2, 1, v).run(s));
 
   // T1INT2A/T1INT2B/T12INT2 pieces:

@@ -4,18 +4,38 @@
 
 namespace aces {
 
-int pack_tau(int o, int v, const double* tau, double* S, long lds, double* D, long ldd, hipStream_t s);
+// Ladder operands Sp / Dp from tau (ccsd_kernels.hip): fp64 rows and/or the
+// bf16 hi/lo split rows, wherever the pointers are set.
+struct PackTauOut {
+  double* S = nullptr;
+  long lds = 0;
+  double* D = nullptr;
+  long ldd = 0;
+  unsigned short* Shi = nullptr;
+  unsigned short* Slo = nullptr;
+  long ldsl = 0;
+  unsigned short* Dhi = nullptr;
+  unsigned short* Dlo = nullptr;
+  long lddl = 0;
+};
+int pack_tau(int o, int v, const double* tau, const PackTauOut& out, hipStream_t s);
 int build_ladder_operands(int o, int v, const double* L233, const double* ovvv, double* Vp, long ldp, double* Vm,
                           long ldm, hipStream_t s);
 int unpack_ladder(int o, int v, const double* Zp, long ldzp, const double* Zm, long ldzm, double* inc2, double* Yb,
                   hipStream_t s);
 // Ring operands U, X, Ut, Xt ([(i,a)][(j,b)]) from t1 and t2 in one pass
 // (ccsd_kernels.hip); index q = 0 U, 1 X, 2 Ut, 3 Xt; null pointers skip.
+// The same pass writes tau = t2 + t1 t1 ([i][j][a][b]) and, when t2hi is set,
+// the bf16 split of t2 itself (rows (i,j,a), ld = ldt).
 struct RingOperands {
   double* f64[4] = {nullptr, nullptr, nullptr, nullptr};  // ld = o v
   unsigned short* hi[4] = {nullptr, nullptr, nullptr, nullptr};
   unsigned short* lo[4] = {nullptr, nullptr, nullptr, nullptr};
   long ldl = 0;  // bf16 row length (>= o v, padding untouched)
+  double* tau = nullptr;
+  unsigned short* t2hi = nullptr;
+  unsigned short* t2lo = nullptr;
+  long ldt = 0;  // >= v
 };
 int ring_operands(int o, int v, const double* t1, const double* t2, const RingOperands& r, hipStream_t s);
 int jacobi(int o, int v, const double* eo, const double* ev, const double* inc1, const double* inc2, double* t1n,

@@ -15,37 +15,77 @@ namespace aces {
 // ---------------------------------------------------------------------------
 // Ladder right operand from tau[i][j][c][d]:
 //   Sp[(i<=j)][(c<=d)] = (tau_ijcd + tau_jicd)/2,  Dp[(i<j)][(c<d)] = (tau_ijcd - tau_jicd)/2
-__global__ void pack_tau_kernel(int o, int v, const double* __restrict__ tau, double* __restrict__ S, long lds,
-                                double* __restrict__ D, long ldd) {
-  const long nop = (long)o * (o + 1) / 2, nvp = (long)v * (v + 1) / 2;
-  const long nom = (long)o * (o - 1) / 2, nvm = (long)v * (v - 1) / 2;
-  const long totS = nop * nvp, totD = nom * nvm;
+// One block per (pair i<=j, 64 values of d).  The (c, d) pair index runs over
+// c inside d, so a row segment of Sp/Dp is a column strip of the tau_ij and
+// tau_ji blocks: strips of 32 c x 64 d are read along d (coalesced) into LDS
+// and written along c.  Outputs go to fp64 rows and/or bf16 hi/lo rows.
+constexpr int kPtC = 32, kPtD = 64;
+
+__global__ __launch_bounds__(256) void pack_tau_kernel(int o, int v, const double* __restrict__ tau, PackTauOut out) {
+  __shared__ double xs[kPtC][kPtD + 1], ys[kPtC][kPtD + 1];
+  const int ndt = (v + kPtD - 1) / kPtD;
+  const long p = blockIdx.x / ndt;
+  const int d0 = (int)(blockIdx.x % ndt) * kPtD;
+  int i, j;
+  unpack_le(p, i, j);
   const long vv = (long)v * v;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < totS + totD;
-       idx += (long)gridDim.x * blockDim.x) {
-    int i, j, c, d;
-    if (idx < totS) {
-      const long pij = idx / nvp, pcd = idx - pij * nvp;
-      unpack_le(pij, i, j);
-      unpack_le(pcd, c, d);
-      const double x = tau[((long)i * o + j) * vv + (long)c * v + d];
-      const double y = tau[((long)j * o + i) * vv + (long)c * v + d];
-      S[pij * lds + pcd] = 0.5 * (x + y);
-    } else {
-      const long r = idx - totS;
-      const long pij = r / nvm, pcd = r - pij * nvm;
-      unpack_lt(pij, i, j);
-      unpack_lt(pcd, c, d);
-      const double x = tau[((long)i * o + j) * vv + (long)c * v + d];
-      const double y = tau[((long)j * o + i) * vv + (long)c * v + d];
-      D[pij * ldd + pcd] = 0.5 * (x - y);
+  const double* X = tau + ((long)i * o + j) * vv;
+  const double* Y = tau + ((long)j * o + i) * vv;
+  const long pm = (i < j) ? pk_lt(i, j) : -1;
+  const int t = threadIdx.x;
+  const int dmax = (d0 + kPtD < v) ? d0 + kPtD : v;  // d in [d0, dmax)
+  for (int c0 = 0; c0 < dmax; c0 += kPtC) {
+    {
+      const int dl = t & 63, cl = t >> 6;
+#pragma unroll
+      for (int k = 0; k < kPtC / 4; ++k) {
+        const int c = c0 + cl + 4 * k, d = d0 + dl;
+        const bool in = c < v && d < v;
+        xs[cl + 4 * k][dl] = in ? X[(long)c * v + d] : 0.0;
+        ys[cl + 4 * k][dl] = in ? Y[(long)c * v + d] : 0.0;
+      }
     }
+    __syncthreads();
+    {
+      const int cl = t & 31, dl0 = t >> 5;
+      const int c = c0 + cl;
+#pragma unroll
+      for (int k = 0; k < kPtD / 8; ++k) {
+        const int dl = dl0 + 8 * k, d = d0 + dl;
+        if (d >= dmax || c > d) continue;
+        const double x = xs[cl][dl], y = ys[cl][dl];
+        const double sv = 0.5 * (x + y);
+        const long ps = (long)d * (d + 1) / 2 + c;
+        if (out.S) out.S[p * out.lds + ps] = sv;
+        if (out.Shi) {
+          const unsigned short h = bf16_rn((float)sv);
+          out.Shi[p * out.ldsl + ps] = h;
+          out.Slo[p * out.ldsl + ps] = bf16_rn((float)(sv - (double)bf16_to_f(h)));
+        }
+        if (pm >= 0 && c < d) {
+          const double dv = 0.5 * (x - y);
+          const long pd = (long)d * (d - 1) / 2 + c;
+          if (out.D) out.D[pm * out.ldd + pd] = dv;
+          if (out.Dhi) {
+            const unsigned short h = bf16_rn((float)dv);
+            out.Dhi[pm * out.lddl + pd] = h;
+            out.Dlo[pm * out.lddl + pd] = bf16_rn((float)(dv - (double)bf16_to_f(h)));
+          }
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
-int pack_tau(int o, int v, const double* tau, double* S, long lds, double* D, long ldd, hipStream_t s) {
-  const long n = (long)o * (o + 1) / 2 * ((long)v * (v + 1) / 2) + (long)o * (o - 1) / 2 * ((long)v * (v - 1) / 2);
-  hipLaunchKernelGGL(pack_tau_kernel, dim3(grid1(n)), dim3(256), 0, s, o, v, tau, S, lds, D, ldd);
+int pack_tau(int o, int v, const double* tau, const PackTauOut& out, hipStream_t s) {
+  const long nop = (long)o * (o + 1) / 2;
+  const long nblk = nop * ((v + kPtD - 1) / kPtD);
+  if (nblk == 0) return ACES_OK;
+  ACES_REQUIRE(nblk < (1L << 31), "pack_tau: too many tiles");
+  ACES_REQUIRE((out.Shi == nullptr) == (out.Slo == nullptr) && (out.Dhi == nullptr) == (out.Dlo == nullptr),
+               "pack_tau: bf16 output needs hi and lo");
+  hipLaunchKernelGGL(pack_tau_kernel, dim3((unsigned)nblk), dim3(256), 0, s, o, v, tau, out);
   ACES_HIP(hipGetLastError());
   return ACES_OK;
 }
@@ -232,7 +272,8 @@ int unpack_ladder(int o, int v, const double* Zp, long ldzp, const double* Zm, l
 //   U = 2 t_ijab - t_ijba,  X = t_ijba,  Ut = U - 2 t1_ja t1_ib,  Xt = X + 2 t1_ja t1_ib
 // One block per (i, j, 32 values of a, 64 values of b): t_ijab is read along b,
 // t_ijba along a and turned round through LDS.  Each output goes to fp64
-// (ld = ov) and/or to bf16 hi/lo rows (ld = ldl) where the pointer is set.
+// (ld = ov) and/or to bf16 hi/lo rows (ld = ldl) where the pointer is set;
+// tau_ijab = t_ijab + t1_ia t1_jb and the bf16 split of t_ijab keep t2's order.
 
 __global__ __launch_bounds__(256) void ring_operands_kernel(int o, int v, const double* __restrict__ t1,
                                                             const double* __restrict__ t2, RingOperands r) {
@@ -263,11 +304,19 @@ __global__ __launch_bounds__(256) void ring_operands_kernel(int o, int v, const 
   const int b = b0 + cb;
   if (b >= v) return;
   const double tib = t1[(long)i * v + b];
+  const double tjb = t1[(long)j * v + b];
+  const long trow = ((long)i * o + j) * v;  // row of t2 element (i, j, a = 0)
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int a = a0 + ra + 4 * k;
     if (a >= v) break;
     const double d = T[(long)a * v + b];
+    if (r.tau) r.tau[(trow + a) * v + b] = d + t1[(long)i * v + a] * tjb;
+    if (r.t2hi) {
+      const unsigned short h = bf16_rn((float)d);
+      r.t2hi[(trow + a) * r.ldt + b] = h;
+      r.t2lo[(trow + a) * r.ldt + b] = bf16_rn((float)(d - (double)bf16_to_f(h)));
+    }
     const double x = tr[cb][ra + 4 * k];
     const double u = 2.0 * d - x;
     const double p = 2.0 * t1[(long)j * v + a] * tib;
@@ -292,6 +341,8 @@ int ring_operands(int o, int v, const double* t1, const double* t2, const RingOp
   for (int q = 0; q < 4; ++q)
     ACES_REQUIRE((r.hi[q] == nullptr) == (r.lo[q] == nullptr) && (r.hi[q] == nullptr || r.ldl >= (long)o * v),
                  "ring_operands: bad bf16 output");
+  ACES_REQUIRE((r.t2hi == nullptr) == (r.t2lo == nullptr) && (r.t2hi == nullptr || r.ldt >= v),
+               "ring_operands: bad t2 split output");
   hipLaunchKernelGGL(ring_operands_kernel, dim3((unsigned)nblk), dim3(256), 0, s, o, v, t1, t2, r);
   ACES_HIP(hipGetLastError());
   return ACES_OK;
