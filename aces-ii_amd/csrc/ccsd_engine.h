@@ -93,12 +93,16 @@ class CcsdEngine {
   // Contraction with operands of the given precision; A64/B64 are the fp64
   // sources (a_kc/b_kc as in GemmDesc), converted into their operand slot
   // unless the slot is static.
+  // cslot >= 0 (bf16x3 only): the result goes to that operand slot as its
+  // bf16 split for a later product, and C is not written.
   int gemm_lp(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda, const double* B64,
-              int bslot, long ldb, double* C, long ldc, double alpha, double beta, int a_kc = 1, int b_kc = 1);
+              int bslot, long ldb, double* C, long ldc, double alpha, double beta, int a_kc = 1, int b_kc = 1,
+              int cslot = -1);
   int timed_gemm(const GemmDesc& d, GemmPrec prec, double flops);
   int gemm64(GemmDesc d);  // fp64 GEMM with the engine's split-K workspace (timed as prof_cat_)
   int gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda, const double* B64,
-                   int bslot, long ldb, double* C, long ldc, double alpha, double beta, int a_kc, int b_kc);
+                   int bslot, long ldb, double* C, long ldc, double alpha, double beta, int a_kc, int b_kc,
+                   int cslot);
   struct DiisState {
     int order = 6;
     std::vector<double*> sol, err;

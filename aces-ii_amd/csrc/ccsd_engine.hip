@@ -341,8 +341,8 @@ void CcsdEngine::profile(double* out, int reset) {
 
 int CcsdEngine::gemm_lp(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda,
                         const double* B64, int bslot, long ldb, double* C, long ldc, double alpha, double beta,
-                        int a_kc, int b_kc) {
-  return gemm_lp_impl(prec, M, N, K, A64, aslot, lda, B64, bslot, ldb, C, ldc, alpha, beta, a_kc, b_kc);
+                        int a_kc, int b_kc, int cslot) {
+  return gemm_lp_impl(prec, M, N, K, A64, aslot, lda, B64, bslot, ldb, C, ldc, alpha, beta, a_kc, b_kc, cslot);
 }
 
 // Launch one MFMA contraction, timed with hipEvents when profiling is on.
@@ -367,7 +367,7 @@ int CcsdEngine::timed_gemm(const GemmDesc& d, GemmPrec prec, double flops) {
 
 int CcsdEngine::gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda,
                              const double* B64, int bslot, long ldb, double* C, long ldc, double alpha,
-                             double beta, int a_kc, int b_kc) {
+                             double beta, int a_kc, int b_kc, int cslot) {
   const double flops = 2.0 * M * (double)N * K;
   if (prec == GP_F64) {
     GemmDesc d = gd(M, N, K, A64, lda, a_kc, B64, ldb, b_kc, C, ldc, alpha, beta);
@@ -413,6 +413,16 @@ int CcsdEngine::gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A
   d.C = C; d.ldc = ldc; d.alpha = alpha; d.beta = beta;
   d.ws = ws_;
   d.ws_elems = ws_elems_;
+  if (cslot >= 0 && prec == GP_BF16X3) {
+    ACES_REQUIRE(!lp_static_slot(cslot) && cslot != aslot && cslot != bslot, "gemm_lp: bad output slot");
+    ACES_CHECK(lowp_slot(cslot, M, N));
+    d.C_hi = (unsigned short*)lp_[cslot].hi;
+    d.C_lo = (unsigned short*)lp_[cslot].lo;
+    d.ldc_lp = lp_[cslot].ld;
+    ACES_CHECK(timed_gemm(d, prec, flops));
+    lp_fresh_[cslot] = true;
+    return ACES_OK;
+  }
   return timed_gemm(d, prec, flops);
 }
 
@@ -474,8 +484,10 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
                  .src(Q2_, -1, v, 1, ov2, ov).run(s));
   // T2 parts (MFMA): Wt += 1/2 Ut L^T ; Wv += -1/2 Xt K^T
   prof_cat_ = 1;
-  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Ut_, 7, ov, Lmat_, 2, ov, Wt_, ov, 0.5, 1.0));
-  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Xt_, 8, ov, Kx_, 3, ov, Wv_, ov, -0.5, 1.0));
+  // (in the bf16 form the results go straight to the ring products' operand
+  // slots 9 and 10)
+  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Ut_, 7, ov, Lmat_, 2, ov, Wt_, ov, 0.5, 1.0, 1, 1, 9));
+  ACES_CHECK(gemm_lp(prec, OV, OV, OV, Xt_, 8, ov, Kx_, 3, ov, Wv_, ov, -0.5, 1.0, 1, 1, 10));
   // ring (MFMA): R1 = Wt U^T ; R2 = Wv X^T
   prof_cat_ = 2;
   ACES_CHECK(gemm_lp(prec, OV, OV, OV, Wt_, 9, ov, U_, 11, ov, R1_, ov, 1.0, 0.0));

@@ -41,6 +41,13 @@ struct GemmDesc {
   // split order.
   double* ws = nullptr;
   long ws_elems = 0;
+  // Optional bf16 split output: when C_hi is set, the result goes to the hi/lo
+  // rows C_hi/C_lo (row r at r * ldc_lp, columns N.. left as they are) and C
+  // is only read (beta != 0) or used as scratch: its contents afterwards are
+  // unspecified.  batch 1.
+  unsigned short* C_hi = nullptr;
+  unsigned short* C_lo = nullptr;
+  long ldc_lp = 0;
 };
 
 int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream);

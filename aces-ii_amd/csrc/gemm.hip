@@ -10,6 +10,8 @@
 #include <algorithm>
 #include "common.h"
 #include "gemm.h"
+#include "bf16_util.h"
+#include "tensor_ops.h"
 
 namespace aces {
 
@@ -37,6 +39,9 @@ struct GemmArgs {
   long zkA, zkB, zC;
   int vecA, vecB;  // 16-byte aligned operand rows (vector load path allowed)
   int splits;      // number of k splits of this problem (grid.z may be larger)
+  unsigned short* Chi;  // bf16 split output instead of C (direct-to-LDS bf16x3 kernel, no split-K)
+  unsigned short* Clo;
+  long ldcl;
 };
 
 // One launch computes up to kMaxGroups independent problems (the irrep blocks
@@ -605,6 +610,8 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
   }
 
   const double alpha = p.alpha, beta = p.beta;
+  unsigned short* __restrict__ Chi = p.Chi;
+  unsigned short* __restrict__ Clo = p.Clo;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -615,8 +622,15 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
         const int row = m0 + wm * 64 + i * 16 + 4 * lk + r;
         if (row < M && col < N) {
           double* c = C + (long)row * p.ldc + col;
-          const double v = alpha * (double)acc[i][j][r];
-          *c = (beta == 0.0) ? v : v + beta * (*c);
+          double v = alpha * (double)acc[i][j][r];
+          if (beta != 0.0) v += beta * (*c);
+          if (Chi) {  // the result as the bf16 hi/lo operand of a later product
+            const unsigned short h = bf16_rn((float)v);
+            Chi[(long)row * p.ldcl + col] = h;
+            Clo[(long)row * p.ldcl + col] = bf16_rn((float)(v - (double)bf16_to_f(h)));
+          } else {
+            *c = v;
+          }
         }
       }
     }
@@ -926,6 +940,7 @@ static int prepare(const GemmDesc& d, GemmPrec prec, GemmArgs& a, double* ws, lo
   a.C = d.C; a.ldc = d.ldc;
   a.alpha = d.alpha; a.beta = d.beta;
   a.sA = d.sA; a.sB = d.sB; a.sC = d.sC;
+  a.Chi = nullptr; a.Clo = nullptr; a.ldcl = 0;  // set by gemm() where the kernel writes the split
   int bm, bn;
   if (prec == GP_BF16X3) {
     bm = bf16_bm(bf_wm);
@@ -984,6 +999,8 @@ static int check_desc(const GemmDesc& d, GemmPrec prec) {
   ACES_REQUIRE(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.batch >= 1, "gemm: bad dimensions");
   if (d.M == 0 || d.N == 0) return ACES_OK;
   ACES_REQUIRE(d.C != nullptr && d.ldc >= d.N, "gemm: bad C");
+  ACES_REQUIRE(d.C_hi == nullptr || (d.C_lo != nullptr && d.ldc_lp >= d.N && d.batch == 1),
+               "gemm: bad split output");
   if (d.K > 0) {
     ACES_REQUIRE(d.A != nullptr && d.B != nullptr, "gemm: null operand");
     ACES_REQUIRE(d.lda >= (d.a_kc ? d.K : d.M), "gemm: lda too small");
@@ -1073,11 +1090,22 @@ int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream) {
   const long nb = (long)probe.tiles_m * probe.tiles_n;
   ACES_REQUIRE(nb < (1L << 31) && d.batch < 65536, "gemm: grid too large");
   const int splits = prepare(d, prec, a, d.ws, d.ws_elems, 0, nb * d.batch, wm);
+  // split output straight from the direct-to-LDS bf16x3 epilogue; any other
+  // path writes C and splits it afterwards
+  const bool dl = prec == GP_BF16X3 && bf16_direct(&d, 1);
+  if (d.C_hi && dl && splits == 1) {
+    a.Chi = d.C_hi;
+    a.Clo = d.C_lo;
+    a.ldcl = d.ldc_lp;
+  }
   for (int k = 1; k < kMaxGroups; ++k) gg.g[k] = a;
   dim3 grid((unsigned)nb, (unsigned)d.batch, (unsigned)splits);
   if (d.K == 0) a.K = 0;
-  ACES_CHECK(launch_groups(gg, prec, d.a_kc != 0, d.b_kc != 0, d.K == 0, grid, wm, bf16_direct(&d, 1), stream));
-  return reduce_splits(d, a, stream);
+  ACES_CHECK(launch_groups(gg, prec, d.a_kc != 0, d.b_kc != 0, d.K == 0, grid, wm, dl, stream));
+  ACES_CHECK(reduce_splits(d, a, stream));
+  if (d.C_hi && a.Chi == nullptr)
+    ACES_CHECK(split_bf16(d.M, d.N, d.C, d.ldc, d.C_hi, d.C_lo, d.ldc_lp, stream));
+  return ACES_OK;
 }
 
 int gemm_grouped(const GemmDesc* d, int ng, GemmPrec prec, hipStream_t stream) {
@@ -1092,6 +1120,7 @@ int gemm_grouped(const GemmDesc* d, int ng, GemmPrec prec, hipStream_t stream) {
   for (int g = 0; g < ng; ++g) {
     ACES_CHECK(check_desc(d[g], prec));
     ACES_REQUIRE(d[g].batch == 1, "gemm_grouped: batched groups are not supported");
+    ACES_REQUIRE(d[g].C_hi == nullptr, "gemm_grouped: split output is not supported");
     ACES_REQUIRE(d[g].a_kc == d[0].a_kc && d[g].b_kc == d[0].b_kc, "gemm_grouped: operand orders must agree");
     ACES_REQUIRE((d[g].K == 0) == (d[0].K == 0) || d[g].M == 0 || d[g].N == 0,
                  "gemm_grouped: mixed K = 0 and K > 0 groups");
