@@ -540,7 +540,6 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   prof_cat_ = 3;
   // L1[a][(b,i,j)] = sum_k t_ka Yb[(b,i,j)][k] ; inc2 -= L1[a,b,i,j] + L1[b,a,j,i]
   ACES_CHECK(gemm64(gd(v, (int)(v * oo), o, t1, v, 0, Yb_, o, 1, L1_, v * oo, 1.0, 0.0)));
-  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(L1_, -1, o, 1, vo2, oo).src(L1_, -1, 1, o, oo, vo2).run(s));
 
   // hole-hole ladder: Wo = (ki|lj) + Qo[klij] + Qo[lkji] + (kc|ld) tau_ijcd ;
   // inc2[ij][ab] += sum_kl Wo[kl][ij] tau[kl][ab]  (both products in the ladder precision)
@@ -597,20 +596,34 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
     d.sC = ov2;
     ACES_CHECK(gemm64(d));
   }
-  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(tmpL_, 1, ov2, vv, v, 1).src(tmpL_, 1, vv, ov2, 1, v).run(s));
 
   // T1INT2A/T1INT2B/T12INT2 pieces:
   //   inc2 += P1[iabj] - T1Q[aibj] + P1[jbai] - T1Q[bjai]
   ACES_CHECK(gemm64(gd((int)o2v, o, v, oovv_, v, 1, t1, v, 1, Qb_, o, 1.0, 0.0)));         // [(k,i,b)][j]
   ACES_CHECK(gemm64(gd(v, (int)o2v, o, t1, v, 0, Qb_, o2v, 0, T1Q_, o2v, 1.0, 0.0)));     // [a][(i,b,j)]
-  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(P1_, 1, v2o, 1, v * o, o)
-                 .src(T1Q_, -1, v * o, 1, o2v, o).src(P1_, 1, 1, v2o, o, v * o).run(s));
-  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(T1Q_, -1, 1, v * o, o, o2v).run(s));
   //   inc2 -= G4[iajb] + G4[jbia],  G4 = (ovoo[iajk] + S3[iakj]) t_kb
   ACES_CHECK(gemm64(gd((int)o2v, o, v, ovov_, v, 1, t1, v, 1, S3_, o, 1.0, 0.0)));          // [(i,a,k)][j]
   ACES_CHECK(P4(o, v, o, o, G4A_).src(ovoo_, 1, vo2, oo, o, 1).src(S3_, 1, vo2, oo, 1, o).run(s));
   ACES_CHECK(gemm64(gd((int)o2v, v, o, G4A_, o, 1, t1, v, 0, G4_, v, 1.0, 0.0)));         // [(i,a,j)][b]
-  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0).src(G4_, -1, ov2, v, ov, 1).src(G4_, -1, v, ov2, 1, ov).run(s));
+
+  // the pair-symmetric accumulations into inc2, in one permute (one pass of
+  // inc2 per distinct transposed source index):
+  //   inc2 -= L1[a,b,i,j] + L1[b,a,j,i]              (ladder T1 rows)
+  //   inc2 += tmpL + tmpL^T                           (FEACONT / FMICONT)
+  //   inc2 += P1[iabj] - T1Q[aibj] + P1[jbai] - T1Q[bjai]
+  //   inc2 -= G4[iajb] + G4[jbia]
+  ACES_CHECK(P4(o, o, v, v, inc2_, 1.0)
+                 .src(L1_, -1, o, 1, vo2, oo)
+                 .src(L1_, -1, 1, o, oo, vo2)
+                 .src(tmpL_, 1, ov2, vv, v, 1)
+                 .src(tmpL_, 1, vv, ov2, 1, v)
+                 .src(P1_, 1, v2o, 1, v * o, o)
+                 .src(T1Q_, -1, v * o, 1, o2v, o)
+                 .src(P1_, 1, 1, v2o, o, v * o)
+                 .src(T1Q_, -1, 1, v * o, o, o2v)
+                 .src(G4_, -1, ov2, v, ov, 1)
+                 .src(G4_, -1, v, ov2, 1, ov)
+                 .run(s));
 
   // ---- T1 increment ----
   ACES_CHECK(P4(1, 1, o, v, inc1_).src(fov, 1, 0, 0, n, 1).run(s));

@@ -20,7 +20,7 @@ int split_bf16_t(int rows, int cols, const double* x, long ldx, unsigned short* 
 //   out[i] = beta*out[i] + sum_s alpha[s] * in[s][ sum_k i_k * st[s][k] ]
 //            + gamma * X[ sum_k i_k * xs[k] ] * Y[ sum_k i_k * ys[k] ]
 // Unused sources have in[s] == nullptr; the product term is skipped when X == nullptr.
-constexpr int kP4Src = 8;  // sources per permute
+constexpr int kP4Src = 10;  // sources per permute
 struct Permute4 {
   long d[4] = {1, 1, 1, 1};
   double* out = nullptr;
