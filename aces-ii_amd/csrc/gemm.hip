@@ -580,7 +580,6 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
     // plain s_barrier: a __syncthreads() fence would wait for every load in
     // flight (the later stages too); the wait above covers this tile
     __builtin_amdgcn_s_barrier();
-    if (kt + S - 1 < nk) issue(kt + S - 1);
     const unsigned short* st = lds + (kt % S) * STAGE;
     s8_t ah[4], al[4], bh[FN], bl[FN];
 #pragma unroll
@@ -595,6 +594,9 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
       bh[j] = *reinterpret_cast<const s8_t*>(st + 2 * SA + off);
       bl[j] = *reinterpret_cast<const s8_t*>(st + 2 * SA + SB + off);
     }
+    // the next tile's loads go out after this tile's fragment reads, so that
+    // their LDS writes do not compete with the reads that start the products
+    if (kt + S - 1 < nk) issue(kt + S - 1);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -932,6 +934,18 @@ static int bf16_bn(int shape) { return shape == 8 ? 256 : 128; }
 // Fill the kernel arguments of one problem; split-K when the output cannot
 // fill the chip and K is long, with partial products
 // in ws[ws_off ...].  Returns the number of splits.
+// Output block of each kernel: rows x columns.
+static int block_rows(GemmPrec prec, int shape) {
+  if (prec == GP_BF16X3) return bf16_bm(shape);
+  if (prec == GP_F64 && shape == kShapeF64Dl) return 32 * kF64DlFM;
+  return (prec == GP_F32) ? TileCfg<float>::type::BM : TileCfg<double>::type::BM;
+}
+static int block_cols(GemmPrec prec, int shape) {
+  if (prec == GP_BF16X3) return bf16_bn(shape);
+  if (prec == GP_F64 && shape == kShapeF64Dl) return 32 * kF64DlFN;
+  return (prec == GP_F32) ? TileCfg<float>::type::BN : TileCfg<double>::type::BN;
+}
+
 static int prepare(const GemmDesc& d, GemmPrec prec, GemmArgs& a, double* ws, long ws_avail, long ws_off,
                    long tiles_total, int bf_wm) {
   a.M = d.M; a.N = d.N; a.K = d.K;
@@ -941,20 +955,7 @@ static int prepare(const GemmDesc& d, GemmPrec prec, GemmArgs& a, double* ws, lo
   a.alpha = d.alpha; a.beta = d.beta;
   a.sA = d.sA; a.sB = d.sB; a.sC = d.sC;
   a.Chi = nullptr; a.Clo = nullptr; a.ldcl = 0;  // set by gemm() where the kernel writes the split
-  int bm, bn;
-  if (prec == GP_BF16X3) {
-    bm = bf16_bm(bf_wm);
-    bn = bf16_bn(bf_wm);
-  } else if (prec == GP_F64 && bf_wm == kShapeF64Dl) {
-    bm = 32 * kF64DlFM;
-    bn = 32 * kF64DlFN;
-  } else if (prec == GP_F32) {
-    bm = TileCfg<float>::type::BM;
-    bn = TileCfg<float>::type::BN;
-  } else {
-    bm = TileCfg<double>::type::BM;
-    bn = TileCfg<double>::type::BN;
-  }
+  const int bm = block_rows(prec, bf_wm), bn = block_cols(prec, bf_wm);
   a.tiles_m = (d.M + bm - 1) / bm;
   a.tiles_n = (d.N + bn - 1) / bn;
   {
@@ -1077,9 +1078,76 @@ static int reduce_splits(const GemmDesc& d, const GemmArgs& a, hipStream_t strea
   return ACES_OK;
 }
 
+static int gemm_one(const GemmDesc& d, GemmPrec prec, hipStream_t stream);
+
+// Compute units of the current device (one value per process: one GPU).
+static int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      cus = n;
+    else
+      cus = 256;
+  }
+  return cus;
+}
+
+// Wave quantization: an output of nb equal tiles runs in ceil(nb / CUs) rounds
+// of one tile time each, so a last round that is under half full leaves most
+// of the chip idle for a whole (long-K) tile.  Such a product runs as two:
+// the rows that fill the full rounds, then the remaining rows, whose few
+// tiles split K over the workspace (deterministic split-K) and finish in a
+// fraction of a tile time.
 int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream) {
   ACES_CHECK(check_desc(d, prec));
   if (d.M == 0 || d.N == 0) return ACES_OK;
+  if (d.batch == 1 && d.ws != nullptr && d.K >= 1024) {
+    const int wm = gemm_shape(&d, 1, prec);
+    GemmArgs probe;
+    prepare(d, prec, probe, nullptr, 0, 0, 1L << 30, wm);
+    const long nb = (long)probe.tiles_m * probe.tiles_n;
+    const long cus = device_cus();
+    const long full = nb / cus, tail = nb - full * cus;
+    if (full >= 1 && full <= 32 && tail > 0 && 2 * tail <= cus) {
+      const long main_tm = full * cus / probe.tiles_n;
+      const long rows = main_tm * block_rows(prec, wm);
+      if (main_tm >= 1 && rows < d.M) {
+        const long esz = (prec == GP_BF16X3) ? 2 : (prec == GP_F32) ? 4 : 8;
+        const long aoff = (d.a_kc ? rows * d.lda : rows) * esz;
+        GemmDesc top = d, bot = d;
+        top.M = (int)rows;
+        bot.M = d.M - (int)rows;
+        bot.A = (const char*)d.A + aoff;
+        if (d.A_lo) bot.A_lo = (const char*)d.A_lo + aoff;
+        bot.C = d.C + rows * d.ldc;
+        if (d.C_hi) {
+          bot.C_hi = d.C_hi + rows * d.ldc_lp;
+          bot.C_lo = d.C_lo + rows * d.ldc_lp;
+        }
+        // worth it only where the tail's split-K fits in much less than a round
+        const int wb = gemm_shape(&bot, 1, prec);
+        GemmArgs pb;
+        prepare(bot, prec, pb, nullptr, 0, 0, 1L << 30, wb);
+        const long nbot = (long)pb.tiles_m * pb.tiles_n;
+        const int sp = prepare(bot, prec, pb, d.ws, d.ws_elems, 0, nbot, wb);
+        const long ntop = main_tm * probe.tiles_n;
+        // rounds in units of one full-K tile of the original block size
+        const double unit = (double)block_rows(prec, wm) * block_cols(prec, wm) * d.K;
+        const double bot_round = (double)block_rows(prec, wb) * block_cols(prec, wb) * pb.kchunk / unit;
+        const double rounds = (double)((ntop + cus - 1) / cus) + (double)((nbot * sp + cus - 1) / cus) * bot_round;
+        if (rounds <= (double)full + 0.75) {
+          ACES_CHECK(gemm_one(top, prec, stream));
+          return gemm_one(bot, prec, stream);
+        }
+      }
+    }
+  }
+  return gemm_one(d, prec, stream);
+}
+
+static int gemm_one(const GemmDesc& d, GemmPrec prec, hipStream_t stream) {
   GemmGroups gg;
   gg.n = 1;
   GemmArgs& a = gg.g[0];

@@ -222,3 +222,42 @@ def test_f64_direct_lds_layouts(a_kc, b_kc, M, N, K, ws):
     torch.cuda.synchronize()
     ref = 0.25 * (Am @ Bm.t()) - C0
     assert torch.allclose(C, ref, rtol=1e-12, atol=1e-12 * K ** 0.5)
+
+
+@pytest.mark.parametrize('prec,a_kc,b_kc,M,N,K', [(2, 1, 1, 4352, 4096, 2048), (2, 1, 1, 4300, 4000, 1024),
+                                                  (0, 0, 1, 2150, 1024, 2048), (0, 0, 0, 2150, 1000, 1024),
+                                                  (0, 1, 1, 2200, 1024, 1024)])
+def test_sparse_last_round(prec, a_kc, b_kc, M, N, K):
+    """Products whose tile count leaves the last round of the chip's CUs under
+    half full (about 272 tiles on 256 CUs) run as the rows that fill the full
+    rounds plus the remaining rows with split-K: row offsets of k-contiguous and
+    row-contiguous operands, alpha/beta on both parts."""
+    from aces2 import _lib
+    print('case sparse last round', prec, a_kc, b_kc, M, N, K, flush=True)
+    g = torch.Generator(device='cuda').manual_seed(M + 3 * N + K + a_kc)
+    Am = torch.randn(M, K, dtype=torch.float64, device='cuda', generator=g)
+    Bm = torch.randn(N, K, dtype=torch.float64, device='cuda', generator=g)
+    C0 = torch.randn(M, N, dtype=torch.float64, device='cuda', generator=g)
+    ref = 0.75 * (Am @ Bm.t()) - 0.5 * C0
+    C = C0.clone()
+    wsb = torch.empty(1 << 24, dtype=torch.float64, device='cuda')
+    if prec == 2:
+        ops = []
+        for X in (Am, Bm):
+            hi = torch.empty(X.shape, dtype=torch.int16, device='cuda')
+            lo = torch.empty(X.shape, dtype=torch.int16, device='cuda')
+            _lib.call('aces_split_bf16', X.shape[0], K, X.data_ptr(), K, hi.data_ptr(), lo.data_ptr(), K, None)
+            ops.append((hi, lo))
+        groups = [{'M': M, 'N': N, 'K': K, 'A': ops[0][0], 'Alo': ops[0][1], 'B': ops[1][0], 'Blo': ops[1][1],
+                   'C': C}]
+        _grouped(2, groups, 1, 1, 0.75, -0.5, wsb)
+        torch.cuda.synchronize()
+        scale = 0.75 * (Am.abs() @ Bm.abs().t()) + 0.5 * C0.abs()
+        err = ((C - ref).abs() / scale).max().item()
+        assert err < 3e-5, err
+    else:
+        A = Am.contiguous() if a_kc else Am.t().contiguous()
+        B = Bm.contiguous() if b_kc else Bm.t().contiguous()
+        _grouped(0, [{'M': M, 'N': N, 'K': K, 'A': A, 'B': B, 'C': C}], a_kc, b_kc, 0.75, -0.5, wsb)
+        torch.cuda.synchronize()
+        assert torch.allclose(C, ref, rtol=1e-12, atol=1e-12 * K ** 0.5)
