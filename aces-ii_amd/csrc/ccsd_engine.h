@@ -154,12 +154,15 @@ class CcsdEngine {
     long ld = 0;
     long rows = 0;
   };
-  // slots 0..4 and 16..19 hold static operands (converted once per precision),
-  // the others this cycle's operands
-  static constexpr int kLpStatic = 5, kLpStatic2 = 16, kLpSlots = 26;
-  static constexpr bool lp_static_slot(int slot) { return slot < kLpStatic || (slot >= kLpStatic2 && slot < 20); }
+  // slots 0..4, 16..19 and 26..29 hold static operands (converted once per
+  // precision), the others this cycle's operands
+  static constexpr int kLpStatic = 5, kLpStatic2 = 16, kLpSlots = 30;
+  static constexpr bool lp_static_slot(int slot) {
+    return slot < kLpStatic || (slot >= kLpStatic2 && slot < 20) || slot >= 26;
+  }
   LowP lp_[kLpSlots];
   bool lp_fresh_[kLpSlots] = {};  // slot already holds this cycle's operand (bf16x3)
+  bool lp_keep_[kLpSlots] = {};   // operand fixed for the whole cycle: convert once, use many times
   int lowp_slot(int slot, long rows, long K);
   GemmPrec lp_static_prec_ = GP_F64;
   // DIIS (amplitudes and multipliers)

@@ -277,7 +277,11 @@ int CcsdEngine::lowp_static(GemmPrec prec) {
                    {16, ovvv_, vv * o, v, v},             // [(k,c,a)][d]
                    {17, ovvv_kacd_, vv * o, v, v},        // [(k,a,c)][d]
                    {18, M3_, v, ov * v, ov * v},          // [a][(k,c,d)]
-                   {19, Lv_, v, oo * v, oo * v}};         // [c][(k,l,d)]
+                   {19, Lv_, v, oo * v, oo * v},          // [c][(k,l,d)]
+                   {26, oovv_, oo * v, v, v},             // [(k,i,b)][c]
+                   {27, ovov_, oo * v, v, v},             // [(i,a,k)][c]
+                   {28, Lo_, o, ov * v, ov * v},          // [k][(l,c,d)]
+                   {29, M7_, o, oo * v, oo * v}};         // [i][(k,l,c)]
   for (const S& t : tab) {
     LowP& L = lp_[t.slot];
     const long ld = pad32(std::max<long>(t.K, 1));
@@ -389,8 +393,10 @@ int CcsdEngine::gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A
   for (int q = 0; q < 2; ++q) {
     LowP& L = lp_[slots[q]];
     if (!lp_static_slot(slots[q]) && lp_fresh_[slots[q]] && prec == GP_BF16X3) {
-      lp_fresh_[slots[q]] = false;  // written in this precision by the fused operand kernel
+      // written in this precision by a fused kernel or an earlier product of this cycle
+      if (!lp_keep_[slots[q]]) lp_fresh_[slots[q]] = false;
     } else if (!lp_static_slot(slots[q])) {
+      if (prec == GP_BF16X3 && lp_keep_[slots[q]]) lp_fresh_[slots[q]] = true;
       ACES_CHECK(lowp_slot(slots[q], rows[q], K));
       if (prec == GP_F32 && kcs[q])
         ACES_CHECK(convert_f32((int)rows[q], K, src[q], lds[q], (float*)L.hi, Kp, s_));
@@ -442,6 +448,10 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   // (slots 11 U, 12 X, 7 Ut, 8 Xt) and the split of t2 (slot 24), and keeps
   // only U in fp64
   for (bool& f : lp_fresh_) f = false;
+  // operands that stay fixed through the cycle once split: t1 (slot 20), tau
+  // as [i][(k,c,d)] (22), tauK (23)
+  for (bool& f : lp_keep_) f = false;
+  lp_keep_[20] = lp_keep_[22] = lp_keep_[23] = true;
   {
     RingOperands r;
     r.f64[0] = U_;
@@ -472,7 +482,7 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   // T1 parts of the W(mbej) intermediates (the two o v^3 reads in the ladder precision)
   prof_cat_ = 3;
   ACES_CHECK(gemm_lp(prec, (int)v2o, o, v, ovvv_, 16, v, t1, 20, v, P1_, o, 1.0, 0.0));       // [(k,c,a)][i]
-  ACES_CHECK(gemm_lp(prec, (int)v2o, o, v, ovvv_kacd_, 17, v, t1, 21, v, P2_, o, 1.0, 0.0));  // [(k,a,c)][i]
+  ACES_CHECK(gemm_lp(prec, (int)v2o, o, v, ovvv_kacd_, 17, v, t1, 20, v, P2_, o, 1.0, 0.0));  // [(k,a,c)][i]
   ACES_CHECK(gemm64(gd((int)o2v, v, o, M5c_, o, 1, t1, v, 0, Q12_, v, 1.0, 0.0)));        // [(k,c,i)][a]
   ACES_CHECK(gemm64(gd((int)o2v, v, o, M5b_, o, 1, t1, v, 0, Q2_, v, 1.0, 0.0)));         // [(k,c,i)][a]
   // Wt[(i,a)][(k,c)] = W0t + 2 P1[k,c,a,i] - P2[k,a,c,i] - Q12[k,c,i,a]
@@ -556,7 +566,7 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(P4(1, 1, o, v, Fov_).src(fov, 1, 0, 0, n, 1).run(s));
   ACES_CHECK(gemv(OV, OV, Lmat_, ov, t1, Fov_, 1.0, 1.0, s));
   ACES_CHECK(P4(1, 1, o, o, Foo_).src(fock_, 1, 0, 0, n, 1).run(s));
-  ACES_CHECK(gemm64(gd(o, o, (int)ov2, Lo_, ov2, 1, tau_, ov2, 1, Foo_, o, 1.0, 1.0)));
+  ACES_CHECK(gemm_lp(prec, o, o, (int)ov2, Lo_, 28, ov2, tau_, 22, ov2, Foo_, o, 1.0, 1.0));
   ACES_CHECK(P4(v, o, o, v, tauK_).src(tau_, 1, v, ov2, vv, 1).run(s));  // tauK[a][k][l][d] = tau_klad
   ACES_CHECK(P4(1, 1, v, v, Fvv_).src(fock_ + (long)o * n + o, 1, 0, 0, n, 1).run(s));
   ACES_CHECK(gemm_lp(prec, v, v, (int)(oo * v), tauK_, 23, oo * v, Lv_, 19, oo * v, Fvv_, v, -1.0, 1.0));
@@ -599,10 +609,10 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
 
   // T1INT2A/T1INT2B/T12INT2 pieces:
   //   inc2 += P1[iabj] - T1Q[aibj] + P1[jbai] - T1Q[bjai]
-  ACES_CHECK(gemm64(gd((int)o2v, o, v, oovv_, v, 1, t1, v, 1, Qb_, o, 1.0, 0.0)));         // [(k,i,b)][j]
+  ACES_CHECK(gemm_lp(prec, (int)o2v, o, v, oovv_, 26, v, t1, 20, v, Qb_, o, 1.0, 0.0));     // [(k,i,b)][j]
   ACES_CHECK(gemm64(gd(v, (int)o2v, o, t1, v, 0, Qb_, o2v, 0, T1Q_, o2v, 1.0, 0.0)));     // [a][(i,b,j)]
   //   inc2 -= G4[iajb] + G4[jbia],  G4 = (ovoo[iajk] + S3[iakj]) t_kb
-  ACES_CHECK(gemm64(gd((int)o2v, o, v, ovov_, v, 1, t1, v, 1, S3_, o, 1.0, 0.0)));          // [(i,a,k)][j]
+  ACES_CHECK(gemm_lp(prec, (int)o2v, o, v, ovov_, 27, v, t1, 20, v, S3_, o, 1.0, 0.0));      // [(i,a,k)][j]
   ACES_CHECK(P4(o, v, o, o, G4A_).src(ovoo_, 1, vo2, oo, o, 1).src(S3_, 1, vo2, oo, 1, o).run(s));
   ACES_CHECK(gemm64(gd((int)o2v, v, o, G4A_, o, 1, t1, v, 0, G4_, v, 1.0, 0.0)));         // [(i,a,j)][b]
 
@@ -633,7 +643,7 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(gemv(OV, OV, U_, ov, Fov_, inc1_, 1.0, 1.0, s));
   ACES_CHECK(gemv(OV, OV, W0t_, ov, t1, inc1_, 1.0, 1.0, s));
   ACES_CHECK(gemm_lp(prec, o, v, (int)ov2, tau_, 22, ov2, M3_, 18, ov2, inc1_, v, 1.0, 1.0));
-  ACES_CHECK(gemm64(gd(o, v, (int)o2v, M7_, o2v, 1, tauK_, o2v, 1, inc1_, v, -1.0, 1.0)));
+  ACES_CHECK(gemm_lp(prec, o, v, (int)o2v, M7_, 29, o2v, tauK_, 23, o2v, inc1_, v, -1.0, 1.0));
   // the reverse pass needs fp64 intermediates of these amplitudes
   fwd_valid_ = (prec == GP_F64);
   return ACES_OK;
