@@ -43,7 +43,7 @@ class CcsdEngine {
   int cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st);
   // Pieces of one cycle, used by the list-level driver entry points.
   int jacobi_step();
-  int diis_step(int icycle, double tol, CcsdStats& st);
+  int diis_step(int icycle, double tol, CcsdStats& st, bool xn_sym = false);
   // Writable views; writing through T() invalidates the resident forward intermediates.
   double* T() { fwd_valid_ = false; return t1_; }  // [t1 | t2] current amplitudes
   double* Tnew() { return t1n_; }    // [t1 | t2] Jacobi update
@@ -112,7 +112,10 @@ class CcsdEngine {
   // extrapolation into x.  lambda_rule: converged when |RESMITER| or
   // |RESMDIIS| < tol and the DIIS residual summed in slot order
   // (lambda/dodiis0.F); otherwise the amplitude rule of vcc/dodiis0.F.
-  int diis_core(DiisState& D, double* x, double* xn, int icycle, double tol, bool lambda_rule, CcsdStats& st);
+  // x_sym / xn_sym: the T2 part of x / xn is already exactly pair-symmetric
+  // (SYMETRIZE_T2 would leave it unchanged and is skipped)
+  int diis_core(DiisState& D, double* x, double* xn, int icycle, double tol, bool lambda_rule, CcsdStats& st,
+                bool x_sym = false, bool xn_sym = false);
   int diis_resize(DiisState& D, int order);
   int adjoint_alloc();
   int compute_adjoint(const double* lam, bool density);
@@ -142,6 +145,8 @@ class CcsdEngine {
   double *Wo_, *Qo_, *tmpL_;
   double *Fov_, *Foo_, *Fvv_, *Loo_, *Lvv_, *FT_, *FovT_, *Gk_;
   double *red_ = nullptr;  // reduction scratch
+  double* jpart_ = nullptr;  // energy partials of the fused Jacobi pass
+  bool t_sym_ = false;       // t2_ is known to be exactly pair-symmetric (set by DIIS)
   double *ws_ = nullptr;   // split-K workspace
   long ws_elems_ = 0;
   double *scal_ = nullptr; // device scalars

@@ -182,6 +182,7 @@ int CcsdEngine::setup(const double* fock, const double* ovov, const double* oovv
   ACES_CHECK(alloc(&FovT_, oo));
   ACES_CHECK(alloc(&Gk_, oo));
   ACES_CHECK(alloc_raw((void**)&red_, reduction_scratch_bytes()));
+  ACES_CHECK(alloc_raw((void**)&jpart_, sizeof(double) * jacobi_energy_scratch(o, v)));
   ws_elems_ = 1L << 25;  // split-K workspace: 32 Mi doubles
   ACES_CHECK(alloc(&ws_, ws_elems_));
   ACES_CHECK(alloc(&scal_, 64));
@@ -220,6 +221,7 @@ int CcsdEngine::reset_diis() {
 int CcsdEngine::set_amplitudes(const double* t1, const double* t2) {
   ACES_REQUIRE(ready_, "ccsd: not set up");
   const long ov = (long)o * v;
+  t_sym_ = false;
   ACES_HIP(hipMemcpyAsync(t1_, t1, sizeof(double) * ov, hipMemcpyDeviceToDevice, s_));
   ACES_HIP(hipMemcpyAsync(t2_, t2, sizeof(double) * ov * ov, hipMemcpyDeviceToDevice, s_));
   ACES_HIP(hipStreamSynchronize(s_));
@@ -242,6 +244,7 @@ int CcsdEngine::start_mp1() {
   const long ov = (long)o * v, ov2 = ov * v;
   ACES_HIP(hipMemsetAsync(inc1_, 0, sizeof(double) * ov, s_));
   ACES_CHECK(P4(o, o, v, v, inc2_).src(ovov_, 1, ov2, v, ov, 1).run(s_));
+  t_sym_ = false;
   ACES_CHECK(jacobi(o, v, eo_, ev_, inc1_, inc2_, t1_, t2_, s_));
   ACES_HIP(hipStreamSynchronize(s_));
   fwd_valid_ = false;
@@ -437,7 +440,6 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   hipStream_t s = s_;
   const long ov = (long)o * v, oo = (long)o * o, vv = (long)v * v;
   const long ov2 = ov * v, o2v = oo * v, vo2 = v * oo, v2o = vv * o;
-  const long ovvo = ov * ov;
   const double* t1 = t1_;
   const double* t2 = t2_;
   const int OV = (int)ov;
@@ -671,9 +673,10 @@ int CcsdEngine::cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st) {
   ACES_HIP(hipEventRecord(e0, s_));
   ACES_CHECK(compute_increments(prec));
   ACES_HIP(hipEventRecord(e1, s_));
-  ACES_CHECK(jacobi_step());
-  ACES_CHECK(energy(o, v, fock_ + o, n, Lijab_, t1n_, t2n_, red_, scal_, s_));
-  ACES_CHECK(diis_step(icycle, tol, st));
+  // Jacobi update, its energy (CMPENG before SYMETRIZE) and SYMETRIZE_T2 of
+  // the update in one pass
+  ACES_CHECK(jacobi_energy_sym(o, v, eo_, ev_, inc1_, inc2_, fock_ + o, n, Lijab_, t1n_, t2n_, jpart_, scal_, s_));
+  ACES_CHECK(diis_step(icycle, tol, st, true));
   ACES_HIP(hipEventRecord(e2, s_));
   ACES_HIP(hipEventSynchronize(e2));
   float ms1 = 0, ms2 = 0;
@@ -693,8 +696,11 @@ int CcsdEngine::jacobi_step() { return jacobi(o, v, eo_, ev_, inc1_, inc2_, t1n_
 // DODIIS0 (vcc/dodiis0.F) on T (t1_/t2_) and T_new (t1n_/t2n_); scal_[0] holds
 // the energy of T_new computed by the caller (CMPENG(60,2)).  On return t1_/t2_
 // hold the next amplitudes unless converged.
-int CcsdEngine::diis_step(int icycle, double tol, CcsdStats& st) {
-  ACES_CHECK(diis_core(tdiis_, t1_, t1n_, icycle, tol, false, st));
+int CcsdEngine::diis_step(int icycle, double tol, CcsdStats& st, bool xn_sym) {
+  ACES_CHECK(diis_core(tdiis_, t1_, t1n_, icycle, tol, false, st, t_sym_, xn_sym));
+  // a DIIS combination of symmetrized vectors (or the symmetrized update
+  // itself) is exactly pair-symmetric
+  if (!st.converged) t_sym_ = true;
   if (st.converged) {
     // CMPENG(43,0) on the amplitudes of this cycle; amplitudes are not moved.
     double e;
@@ -706,12 +712,12 @@ int CcsdEngine::diis_step(int icycle, double tol, CcsdStats& st) {
 }
 
 int CcsdEngine::diis_core(DiisState& D, double* x, double* xn, int icycle, double tol, bool lambda_rule,
-                          CcsdStats& st) {
+                          CcsdStats& st, bool x_sym, bool xn_sym) {
   const long ov = (long)o * v;
   const long nt = ov + ov * ov;
   const int order = D.order;
-  ACES_CHECK(symmetrize_t2(o, v, x + ov, s_));
-  ACES_CHECK(symmetrize_t2(o, v, xn + ov, s_));
+  if (!x_sym) ACES_CHECK(symmetrize_t2(o, v, x + ov, s_));
+  if (!xn_sym) ACES_CHECK(symmetrize_t2(o, v, xn + ov, s_));
   const int ndimr = std::min(icycle, order);
   const int idimr = 1 + (icycle - 1) % order;  // 1-based slot
   ACES_CHECK(sub(nt, xn, x, err_cur_, s_));

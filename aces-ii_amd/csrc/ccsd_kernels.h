@@ -41,6 +41,13 @@ int ring_operands(int o, int v, const double* t1, const double* t2, const RingOp
 int jacobi(int o, int v, const double* eo, const double* ev, const double* inc1, const double* inc2, double* t1n,
            double* t2n, hipStream_t s);
 int symmetrize_t2(int o, int v, double* t2, hipStream_t s);
+// Jacobi update, energy of the update (before symmetrization) and
+// SYMETRIZE_T2 of t2n in one pass (ccsd_kernels.hip); part holds
+// jacobi_energy_scratch(o, v) doubles, *out receives the energy.
+long jacobi_energy_scratch(int o, int v);
+int jacobi_energy_sym(int o, int v, const double* eo, const double* ev, const double* inc1, const double* inc2,
+                      const double* fov, long ldf, const double* Lijab, double* t1n, double* t2n, double* part,
+                      double* out, hipStream_t s);
 int energy(int o, int v, const double* fov, long ldf, const double* Lijab, const double* t1, const double* t2,
            double* part, double* out, hipStream_t s);
 int diis_dots(int o, int v, int m, const double* const* d_xs, const double* y, double* part, double* out,

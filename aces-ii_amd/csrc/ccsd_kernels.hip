@@ -457,6 +457,129 @@ __device__ inline double block_sum(double x, double* sh) {
   return r;
 }
 
+__global__ void finish_sum_kernel(const double* __restrict__ part, int n, double* __restrict__ out);
+
+// ---------------------------------------------------------------------------
+// The end of a CC cycle in one pass over T2: the Jacobi update
+// t2n = inc2 / (e_i + e_j - e_a - e_b), the energy of the update
+// (E = 2 sum f_ia t1n_ia + sum L_ijab (t2n_ijab + t1n_ia t1n_jb), taken before
+// symmetrization as the reference does) and SYMETRIZE_T2 of t2n.  Blocks
+// pair the tile (ij, a-tile, b-tile) with its transposed partner in block ji,
+// as symmetrize_kernel does; each block leaves its energy partial in
+// part[blockIdx.x] (0 for blocks the partner owns).  t1n comes from
+// jacobi_t1_energy_kernel, launched first.
+__global__ __launch_bounds__(256) void jacobi_t2_energy_sym_kernel(int o, int v, const double* __restrict__ eo,
+                                                                   const double* __restrict__ ev,
+                                                                   const double* __restrict__ t1n,
+                                                                   const double* __restrict__ inc2,
+                                                                   const double* __restrict__ L,
+                                                                   double* __restrict__ t2n,
+                                                                   double* __restrict__ part) {
+  __shared__ double tp[kSymT][kSymT + 1];
+  __shared__ double sh[256];
+  const int nt = (v + kSymT - 1) / kSymT;
+  long blk = blockIdx.x;
+  const int tb = (int)(blk % nt);
+  blk /= nt;
+  const int ta = (int)(blk % nt);
+  blk /= nt;
+  const int j = (int)(blk % o);
+  const int i = (int)(blk / o);
+  if (i > j || (i == j && ta > tb)) {  // whole block: the partner block owns it
+    if (threadIdx.x == 0) part[blockIdx.x] = 0.0;
+    return;
+  }
+  const bool self = (i == j && ta == tb);  // the tile is its own partner
+  const long vv = (long)v * v;
+  const long oA = ((long)i * o + j) * vv, oB = ((long)j * o + i) * vv;  // A[a][b], B[b][a]
+  const int a0 = ta * kSymT, b0 = tb * kSymT;
+  const int c = threadIdx.x & 63, r = threadIdx.x >> 6;
+  double e = 0.0;
+  // B tile (block ji, rows b, columns a): update, energy, into tp[b][a]
+#pragma unroll 4
+  for (int k = 0; k < 16; ++k) {
+    const int bb = b0 + r + 4 * k, aa = a0 + c;
+    double t = 0.0;
+    if (bb < v && aa < v) {
+      const long q = oB + (long)bb * v + aa;
+      t = inc2[q] / (eo[j] + eo[i] - ev[bb] - ev[aa]);
+      if (!self) e += L[q] * (t + t1n[(long)j * v + bb] * t1n[(long)i * v + aa]);
+    }
+    tp[r + 4 * k][c] = t;
+  }
+  // A tile (block ij, rows a, columns b) in registers
+  double areg[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int aa = a0 + r + 4 * k, bb = b0 + c;
+    double t = 0.0;
+    if (aa < v && bb < v) {
+      const long q = oA + (long)aa * v + bb;
+      t = inc2[q] / (eo[i] + eo[j] - ev[aa] - ev[bb]);
+      e += L[q] * (t + t1n[(long)i * v + aa] * t1n[(long)j * v + bb]);
+    }
+    areg[k] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) areg[k] = 0.5 * (areg[k] + tp[c][r + 4 * k]);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int aa = a0 + r + 4 * k, bb = b0 + c;
+    tp[c][r + 4 * k] = areg[k];
+    if (aa < v && bb < v) t2n[oA + (long)aa * v + bb] = areg[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int bb = b0 + r + 4 * k, aa = a0 + c;
+    if (bb < v && aa < v) t2n[oB + (long)bb * v + aa] = tp[r + 4 * k][c];
+  }
+  const double sum = block_sum(e, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = sum;
+}
+
+// t1n = inc1 / (e_i - e_a) and 2 sum f_ia t1n_ia (one block, deterministic).
+__global__ __launch_bounds__(256) void jacobi_t1_energy_kernel(int o, int v, const double* __restrict__ eo,
+                                                               const double* __restrict__ ev,
+                                                               const double* __restrict__ inc1,
+                                                               const double* __restrict__ fov, long ldf,
+                                                               double* __restrict__ t1n, double* __restrict__ out) {
+  __shared__ double sh[256];
+  const long n1 = (long)o * v;
+  double e = 0.0;
+  for (long idx = threadIdx.x; idx < n1; idx += blockDim.x) {
+    const int i = (int)(idx / v), a = (int)(idx % v);
+    const double t = inc1[idx] / (eo[i] - ev[a]);
+    t1n[idx] = t;
+    e += 2.0 * fov[(long)i * ldf + a] * t;
+  }
+  const double sum = block_sum(e, sh);
+  if (threadIdx.x == 0) *out = sum;
+}
+
+long jacobi_energy_scratch(int o, int v) {
+  const long nt = (v + kSymT - 1) / kSymT;
+  return (long)o * o * nt * nt + 1;
+}
+
+int jacobi_energy_sym(int o, int v, const double* eo, const double* ev, const double* inc1, const double* inc2,
+                      const double* fov, long ldf, const double* Lijab, double* t1n, double* t2n, double* part,
+                      double* out, hipStream_t s) {
+  const long nt = (v + kSymT - 1) / kSymT;
+  const long nblk = (long)o * o * nt * nt;
+  ACES_REQUIRE(nblk < (1L << 31), "jacobi_energy_sym: too many tiles");
+  hipLaunchKernelGGL(jacobi_t1_energy_kernel, dim3(1), dim3(256), 0, s, o, v, eo, ev, inc1, fov, ldf, t1n,
+                     part + nblk);
+  if (nblk > 0)
+    hipLaunchKernelGGL(jacobi_t2_energy_sym_kernel, dim3((unsigned)nblk), dim3(256), 0, s, o, v, eo, ev, t1n,
+                       inc2, Lijab, t2n, part);
+  hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(256), 0, s, part, (int)(nblk + 1), out);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
 // Energy: E = 2 sum f_ia t_ia + sum_ijab L[i][j][a][b] (t2_ijab + t1_ia t1_jb).
 __global__ void energy_partial_kernel(int o, int v, const double* __restrict__ fov, long ldf,
                                       const double* __restrict__ Lijab, const double* __restrict__ t1,

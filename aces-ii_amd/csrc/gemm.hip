@@ -545,9 +545,9 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
     goff[j] = off;
   }
   auto issue = [&](int kt) {
-    unsigned short* st = lds + (kt % S) * STAGE;
 #if __HIP_DEVICE_COMPILE__
     // (the host pass parses kernel bodies too but has no gfx950 target features)
+    unsigned short* st = lds + (kt % S) * STAGE;
 #pragma unroll
     for (int j = 0; j < IPW; ++j)
       __builtin_amdgcn_global_load_lds(gp[j] + kt * BK, (__attribute__((address_space(3))) void*)(st + goff[j]),
