@@ -483,12 +483,12 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
 
   // T1 parts of the W(mbej) intermediates (the two o v^3 reads in the ladder precision)
   prof_cat_ = 3;
-  ACES_CHECK(gemm_lp(prec, (int)v2o, o, v, ovvv_, 16, v, t1, 20, v, P1_, o, 1.0, 0.0));       // [(k,c,a)][i]
+  ACES_CHECK(gemm_lp(prec, o, (int)v2o, v, t1, 20, v, ovvv_, 16, v, P1_, v2o, 1.0, 0.0));     // [i][(k,c,a)]
   ACES_CHECK(gemm_lp(prec, (int)v2o, o, v, ovvv_kacd_, 17, v, t1, 20, v, P2_, o, 1.0, 0.0));  // [(k,a,c)][i]
   ACES_CHECK(gemm64(gd((int)o2v, v, o, M5c_, o, 1, t1, v, 0, Q12_, v, 1.0, 0.0)));        // [(k,c,i)][a]
   ACES_CHECK(gemm64(gd((int)o2v, v, o, M5b_, o, 1, t1, v, 0, Q2_, v, 1.0, 0.0)));         // [(k,c,i)][a]
   // Wt[(i,a)][(k,c)] = W0t + 2 P1[k,c,a,i] - P2[k,a,c,i] - Q12[k,c,i,a]
-  ACES_CHECK(P4(o, v, o, v, Wt_).src(W0t_, 1, ov2, ov, v, 1).src(P1_, 2, 1, o, v2o, v * o)
+  ACES_CHECK(P4(o, v, o, v, Wt_).src(W0t_, 1, ov2, ov, v, 1).src(P1_, 2, v2o, 1, vv, v)
                  .src(P2_, -1, 1, v * o, v2o, o).run(s));
   ACES_CHECK(P4(o, v, o, v, Wt_, 1.0).src(Q12_, -1, v, 1, ov2, ov).run(s));
   // Wv[(i,a)][(k,c)] = Wv0 + P2[k,a,c,i] - Q2[k,c,i,a]
@@ -550,8 +550,8 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
     ACES_HIP(hipMemsetAsync(Zm_, 0, sizeof(double), s));
   ACES_CHECK(unpack_ladder(o, v, Zp_, nop_, Zm_, std::max<long>(nom_, 1), inc2_, Yb_, s));
   prof_cat_ = 3;
-  // L1[a][(b,i,j)] = sum_k t_ka Yb[(b,i,j)][k] ; inc2 -= L1[a,b,i,j] + L1[b,a,j,i]
-  ACES_CHECK(gemm64(gd(v, (int)(v * oo), o, t1, v, 0, Yb_, o, 1, L1_, v * oo, 1.0, 0.0)));
+  // L1 = sum_k Yb[(b,i,j)][k] t_ka stored [(b,i,j)][a] ; inc2 -= L1[a,b,i,j] + L1[b,a,j,i]
+  ACES_CHECK(gemm64(gd((int)(v * oo), v, o, Yb_, o, 1, t1, v, 0, L1_, v, 1.0, 0.0)));
 
   // hole-hole ladder: Wo = (ki|lj) + Qo[klij] + Qo[lkji] + (kc|ld) tau_ijcd ;
   // inc2[ij][ab] += sum_kl Wo[kl][ij] tau[kl][ab]  (both products in the ladder precision)
@@ -612,27 +612,28 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   // T1INT2A/T1INT2B/T12INT2 pieces:
   //   inc2 += P1[iabj] - T1Q[aibj] + P1[jbai] - T1Q[bjai]
   ACES_CHECK(gemm_lp(prec, (int)o2v, o, v, oovv_, 26, v, t1, 20, v, Qb_, o, 1.0, 0.0));     // [(k,i,b)][j]
-  ACES_CHECK(gemm64(gd(v, (int)o2v, o, t1, v, 0, Qb_, o2v, 0, T1Q_, o2v, 1.0, 0.0)));     // [a][(i,b,j)]
+  ACES_CHECK(gemm64(gd((int)o2v, v, o, Qb_, o2v, 0, t1, v, 0, T1Q_, v, 1.0, 0.0)));       // [(i,b,j)][a]
   //   inc2 -= G4[iajb] + G4[jbia],  G4 = (ovoo[iajk] + S3[iakj]) t_kb
   ACES_CHECK(gemm_lp(prec, (int)o2v, o, v, ovov_, 27, v, t1, 20, v, S3_, o, 1.0, 0.0));      // [(i,a,k)][j]
   ACES_CHECK(P4(o, v, o, o, G4A_).src(ovoo_, 1, vo2, oo, o, 1).src(S3_, 1, vo2, oo, 1, o).run(s));
   ACES_CHECK(gemm64(gd((int)o2v, v, o, G4A_, o, 1, t1, v, 0, G4_, v, 1.0, 0.0)));         // [(i,a,j)][b]
 
-  // the pair-symmetric accumulations into inc2, in one permute (one pass of
-  // inc2 per distinct transposed source index):
+  // the pair-symmetric accumulations into inc2, in one permute: P1, L1 and
+  // T1Q are stored so that each term reads either along b (direct) or along
+  // a (one transposed index), so the permute is a single pass over inc2
   //   inc2 -= L1[a,b,i,j] + L1[b,a,j,i]              (ladder T1 rows)
   //   inc2 += tmpL + tmpL^T                           (FEACONT / FMICONT)
   //   inc2 += P1[iabj] - T1Q[aibj] + P1[jbai] - T1Q[bjai]
   //   inc2 -= G4[iajb] + G4[jbia]
   ACES_CHECK(P4(o, o, v, v, inc2_, 1.0)
-                 .src(L1_, -1, o, 1, vo2, oo)
-                 .src(L1_, -1, 1, o, oo, vo2)
+                 .src(L1_, -1, ov, v, 1, o2v)
+                 .src(L1_, -1, v, ov, o2v, 1)
                  .src(tmpL_, 1, ov2, vv, v, 1)
                  .src(tmpL_, 1, vv, ov2, 1, v)
-                 .src(P1_, 1, v2o, 1, v * o, o)
-                 .src(T1Q_, -1, v * o, 1, o2v, o)
-                 .src(P1_, 1, 1, v2o, o, v * o)
-                 .src(T1Q_, -1, 1, v * o, o, o2v)
+                 .src(P1_, 1, vv, v2o, v, 1)
+                 .src(T1Q_, -1, ov2, v, 1, ov)
+                 .src(P1_, 1, v2o, vv, 1, v)
+                 .src(T1Q_, -1, v, ov2, ov, 1)
                  .src(G4_, -1, ov2, v, ov, 1)
                  .src(G4_, -1, v, ov2, 1, ov)
                  .run(s));
