@@ -484,15 +484,19 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   // T1 parts of the W(mbej) intermediates (the two o v^3 reads in the ladder precision)
   prof_cat_ = 3;
   ACES_CHECK(gemm_lp(prec, o, (int)v2o, v, t1, 20, v, ovvv_, 16, v, P1_, v2o, 1.0, 0.0));     // [i][(k,c,a)]
-  ACES_CHECK(gemm_lp(prec, (int)v2o, o, v, ovvv_kacd_, 17, v, t1, 20, v, P2_, o, 1.0, 0.0));  // [(k,a,c)][i]
+  ACES_CHECK(gemm_lp(prec, o, (int)v2o, v, t1, 20, v, ovvv_kacd_, 17, v, P2_, v2o, 1.0, 0.0));  // [i][(k,a,c)]
   ACES_CHECK(gemm64(gd((int)o2v, v, o, M5c_, o, 1, t1, v, 0, Q12_, v, 1.0, 0.0)));        // [(k,c,i)][a]
   ACES_CHECK(gemm64(gd((int)o2v, v, o, M5b_, o, 1, t1, v, 0, Q2_, v, 1.0, 0.0)));         // [(k,c,i)][a]
-  // Wt[(i,a)][(k,c)] = W0t + 2 P1[k,c,a,i] - P2[k,a,c,i] - Q12[k,c,i,a]
-  ACES_CHECK(P4(o, v, o, v, Wt_).src(W0t_, 1, ov2, ov, v, 1).src(P1_, 2, v2o, 1, vv, v)
-                 .src(P2_, -1, 1, v * o, v2o, o).run(s));
-  ACES_CHECK(P4(o, v, o, v, Wt_, 1.0).src(Q12_, -1, v, 1, ov2, ov).run(s));
-  // Wv[(i,a)][(k,c)] = Wv0 + P2[k,a,c,i] - Q2[k,c,i,a]
-  ACES_CHECK(P4(o, v, o, v, Wv_).src(Wv0_, 1, ov2, ov, v, 1).src(P2_, 1, 1, v * o, v2o, o)
+  // Wt[(i,a)][(k,c)] = W0t + 2 P1[k,c,a,i] - P2[k,a,c,i] - Q12[k,c,i,a] and
+  // Wv[(i,a)][(k,c)] = Wv0 + P2[k,a,c,i] - Q2[k,c,i,a]: each source reads
+  // along c (direct) or along a, one pass each
+  ACES_CHECK(P4(o, v, o, v, Wt_)
+                 .src(W0t_, 1, ov2, ov, v, 1)
+                 .src(P1_, 2, v2o, 1, vv, v)
+                 .src(P2_, -1, v2o, v, vv, 1)
+                 .src(Q12_, -1, v, 1, ov2, ov)
+                 .run(s));
+  ACES_CHECK(P4(o, v, o, v, Wv_).src(Wv0_, 1, ov2, ov, v, 1).src(P2_, 1, v2o, v, vv, 1)
                  .src(Q2_, -1, v, 1, ov2, ov).run(s));
   // T2 parts (MFMA): Wt += 1/2 Ut L^T ; Wv += -1/2 Xt K^T
   prof_cat_ = 1;
