@@ -172,7 +172,6 @@ class CcsdEngine {
   GemmPrec lp_static_prec_ = GP_F64;
   // DIIS (amplitudes and multipliers)
   DiisState tdiis_, ldiis_;
-  double* err_cur_ = nullptr;
   double* newvec_ = nullptr;
   const double** d_ptrs_ = nullptr;  // device pointer table
   double* d_coef_ = nullptr;

@@ -186,7 +186,6 @@ int CcsdEngine::setup(const double* fock, const double* ovov, const double* oovv
   ws_elems_ = 1L << 25;  // split-K workspace: 32 Mi doubles
   ACES_CHECK(alloc(&ws_, ws_elems_));
   ACES_CHECK(alloc(&scal_, 64));
-  ACES_CHECK(alloc(&err_cur_, nt));
   ACES_CHECK(alloc(&newvec_, nt));
   ACES_CHECK(alloc_raw((void**)&d_ptrs_, 64 * sizeof(double*)));
   ACES_CHECK(alloc(&d_coef_, 64));
@@ -725,14 +724,14 @@ int CcsdEngine::diis_core(DiisState& D, double* x, double* xn, int icycle, doubl
   if (!xn_sym) ACES_CHECK(symmetrize_t2(o, v, xn + ov, s_));
   const int ndimr = std::min(icycle, order);
   const int idimr = 1 + (icycle - 1) % order;  // 1-based slot
-  ACES_CHECK(sub(nt, xn, x, err_cur_, s_));
-  ACES_CHECK(diis_absmax(o, v, err_cur_, red_, scal_ + 1, s_));
-  ACES_HIP(hipMemcpyAsync(D.sol[idimr - 1], x, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
-  // dots <err_i, e> for i = 1..ndimr (slot idimr uses e itself)
+  // one pass: e = xn - x into the error slot idimr, x into the solution slot,
+  // RESMAX = ISAMAX(e) and the dots <err_i, e> for i = 1..ndimr (slot idimr
+  // is e itself)
   std::vector<const double*> xs(ndimr);
-  for (int i = 1; i <= ndimr; ++i) xs[i - 1] = (i == idimr) ? err_cur_ : D.err[i - 1];
+  for (int i = 1; i <= ndimr; ++i) xs[i - 1] = (i == idimr) ? nullptr : D.err[i - 1];
   ACES_HIP(hipMemcpyAsync(d_ptrs_, xs.data(), sizeof(double*) * ndimr, hipMemcpyHostToDevice, s_));
-  ACES_CHECK(diis_dots(o, v, ndimr, d_ptrs_, err_cur_, red_, scal_ + 8, s_));
+  ACES_CHECK(diis_error(o, v, x, xn, ndimr, d_ptrs_, idimr - 1, D.err[idimr - 1], D.sol[idimr - 1], red_, scal_ + 8,
+                        scal_ + 1, s_));
   double h[40];
   ACES_HIP(hipMemcpyAsync(h, scal_, sizeof(double) * 40, hipMemcpyDeviceToHost, s_));
   ACES_HIP(hipStreamSynchronize(s_));
@@ -742,7 +741,6 @@ int CcsdEngine::diis_core(DiisState& D, double* x, double* xn, int icycle, doubl
     D.R[(size_t)(i - 1) * order + (idimr - 1)] = h[8 + i - 1];
     D.R[(size_t)(idimr - 1) * order + (i - 1)] = h[8 + i - 1];
   }
-  ACES_HIP(hipMemcpyAsync(D.err[idimr - 1], err_cur_, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
   // DODIIS: augmented Pulay system
   const int na = ndimr + 1;
   std::vector<double> A((size_t)na * na, 0.0), b(na, 0.0);

@@ -53,6 +53,11 @@ int energy(int o, int v, const double* fov, long ldf, const double* Lijab, const
 int diis_dots(int o, int v, int m, const double* const* d_xs, const double* y, double* part, double* out,
               hipStream_t s);
 int diis_absmax(int o, int v, const double* e, void* part, double* out, hipStream_t s);
+// sub + diis_absmax + the solution copy + diis_dots of one DODIIS0 step in
+// one pass (ccsd_kernels.hip): e_out = xn - x, sol_out = x, out_dots[k] =
+// <x_k, e> (d_xs[self] unused: e itself), *out_absmax = ISAMAX value of e.
+int diis_error(int o, int v, const double* x, const double* xn, int m, const double* const* d_xs, int self,
+               double* e_out, double* sol_out, void* part, double* out_dots, double* out_absmax, hipStream_t s);
 long reduction_scratch_bytes();
 int lincomb(long n, int m, const double* const* d_xs, const double* d_c, double* out, hipStream_t s);
 int sub(long n, const double* x, const double* z, double* y, hipStream_t s);

@@ -749,6 +749,80 @@ __global__ void absmax_finish_kernel(const AbsMax* __restrict__ part, int n, dou
   if (threadIdx.x == 0) *out = sh[0].val;
 }
 
+// The first half of a DODIIS0 step in one pass: e = xn - x is written to
+// the error slot and x to the solution slot, with the ISAMAX of e (as
+// absmax_partial_kernel) and the m dots <x_k, e> (as diis_dot_partial_kernel;
+// x_self is e itself).  The same-spin twin e[tw] is recomputed from xn and x.
+__global__ void diis_error_partial_kernel(int o, int v, int m, int self, const double* const* __restrict__ xs,
+                                          const double* __restrict__ x, const double* __restrict__ xn,
+                                          double* __restrict__ e_out, double* __restrict__ sol_out,
+                                          double* __restrict__ dpart, AbsMax* __restrict__ apart) {
+  __shared__ double shd[256];
+  __shared__ AbsMax sh[256];
+  const long n1 = (long)o * v, vv = (long)v * v, n2 = n1 * n1;
+  const long nvm = (long)v * (v - 1) / 2;
+  AbsMax am{-1.0, 0x7fffffffffffffffL, 0.0};
+  double acc[8];
+  for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n1 + n2;
+       idx += (long)gridDim.x * blockDim.x) {
+    const double xv = x[idx];
+    const double ev = xn[idx] - xv;
+    e_out[idx] = ev;
+    sol_out[idx] = xv;
+    long tw = -1;
+    double etw = 0.0, eaa = 0.0;
+    if (idx < n1) {
+      am_update(am, ev, idx);
+    } else {
+      const long r = idx - n1;
+      const long ij = r / vv, ab = r - ij * vv;
+      const int i = (int)(ij / o), j = (int)(ij % o), a = (int)(ab / v), b = (int)(ab % v);
+      am_update(am, ev, n1 + (long)a + (long)v * b + vv * ((long)i + (long)o * j));
+      if (i < j && a < b) {
+        tw = n1 + ((long)j * o + i) * vv + ab;
+        etw = xn[tw] - x[tw];
+        eaa = ev - etw;
+        am_update(am, eaa, n1 + n2 + pk_lt(a, b) + nvm * pk_lt(i, j));
+      }
+    }
+    for (int k = 0; k < m; ++k) {
+      const double xk = (k == self) ? ev : xs[k][idx];
+      double c = xk * ev;
+      if (tw >= 0) c += (xk - ((k == self) ? etw : xs[k][tw])) * eaa;
+      acc[k] += c;
+    }
+  }
+  for (int k = 0; k < m; ++k) {
+    const double sum = block_sum(acc[k], shd);
+    if (threadIdx.x == 0) dpart[k * kRedBlocks + blockIdx.x] = sum;
+  }
+  sh[threadIdx.x] = am;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      AbsMax o2 = sh[threadIdx.x + w];
+      AbsMax& mine = sh[threadIdx.x];
+      if (o2.a > mine.a || (o2.a == mine.a && o2.pos < mine.pos)) mine = o2;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) apart[blockIdx.x] = sh[0];
+}
+
+int diis_error(int o, int v, const double* x, const double* xn, int m, const double* const* d_xs, int self,
+               double* e_out, double* sol_out, void* part, double* out_dots, double* out_absmax, hipStream_t s) {
+  ACES_REQUIRE(m >= 1 && m <= 8 && self >= 0 && self < m, "diis_error: 1..8 vectors, self among them");
+  double* dpart = reinterpret_cast<double*>(part);
+  AbsMax* apart = reinterpret_cast<AbsMax*>(dpart + 8L * kRedBlocks);
+  hipLaunchKernelGGL(diis_error_partial_kernel, dim3(kRedBlocks), dim3(256), 0, s, o, v, m, self, d_xs, x, xn, e_out,
+                     sol_out, dpart, apart);
+  hipLaunchKernelGGL(finish_multi_kernel, dim3(1), dim3(256), 0, s, dpart, m, out_dots);
+  hipLaunchKernelGGL(absmax_finish_kernel, dim3(1), dim3(256), 0, s, apart, kRedBlocks, out_absmax);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
 int diis_absmax(int o, int v, const double* e, void* part, double* out, hipStream_t s) {
   hipLaunchKernelGGL(absmax_partial_kernel, dim3(kRedBlocks), dim3(256), 0, s, o, v, e,
                      reinterpret_cast<AbsMax*>(part));
