@@ -159,9 +159,9 @@ class CcsdEngine {
     long ld = 0;
     long rows = 0;
   };
-  // slots 0..4, 16..19 and 26..29 hold static operands (converted once per
+  // slots 0..4, 16..19 and 26..30 hold static operands (converted once per
   // precision), the others this cycle's operands
-  static constexpr int kLpStatic = 5, kLpStatic2 = 16, kLpSlots = 30;
+  static constexpr int kLpStatic = 5, kLpStatic2 = 16, kLpSlots = 31;
   static constexpr bool lp_static_slot(int slot) {
     return slot < kLpStatic || (slot >= kLpStatic2 && slot < 20) || slot >= 26;
   }

@@ -283,7 +283,8 @@ int CcsdEngine::lowp_static(GemmPrec prec) {
                    {26, oovv_, oo * v, v, v},             // [(k,i,b)][c]
                    {27, ovov_, oo * v, v, v},             // [(i,a,k)][c]
                    {28, Lo_, o, ov * v, ov * v},          // [k][(l,c,d)]
-                   {29, M7_, o, oo * v, oo * v}};         // [i][(k,l,c)]
+                   {29, M7_, o, oo * v, oo * v},          // [i][(k,l,c)]
+                   {30, ovvvL_, ov, vv, vv}};             // [(k,d)][(c,a)] (the Lvv gemv)
   for (const S& t : tab) {
     LowP& L = lp_[t.slot];
     const long ld = pad32(std::max<long>(t.K, 1));
@@ -581,7 +582,13 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(gemv((int)oo, OV, M1_, ov, t1, Loo_, 1.0, 1.0, s));
   ACES_CHECK(P4(1, 1, v, v, Lvv_).src(Fvv_, 1, 0, 0, v, 1).run(s));
   ACES_CHECK(gemm64(gd(v, v, o, t1, v, 0, fov, n, 0, Lvv_, v, -1.0, 1.0)));
-  ACES_CHECK(gemv_t(OV, (int)vv, ovvvL_, vv, t1, Lvv_, 1.0, 1.0, s, ws_, ws_elems_));
+  if (prec == GP_BF16X3) {  // the 12.6 GB integral matrix read as its bf16 split (half the bytes)
+    ACES_CHECK(lowp_static(prec));
+    ACES_CHECK(gemv_t_split(OV, (int)vv, (const unsigned short*)lp_[30].hi, (const unsigned short*)lp_[30].lo,
+                            lp_[30].ld, t1, Lvv_, 1.0, 1.0, s, ws_, ws_elems_));
+  } else {
+    ACES_CHECK(gemv_t(OV, (int)vv, ovvvL_, vv, t1, Lvv_, 1.0, 1.0, s, ws_, ws_elems_));
+  }
   // diagonal (orbital energies) moved into the denominators
   ACES_CHECK(sub_diag(o, Foo_, o, eo_, s));
   ACES_CHECK(sub_diag(v, Fvv_, v, ev_, s));

@@ -44,6 +44,9 @@ int gemv(int M, int K, const double* A, long lda, const double* x, double* y, do
 // sums are added in a second, deterministic pass.
 int gemv_t(int K, int N, const double* A, long lda, const double* x, double* y, double alpha, double beta,
            hipStream_t s, double* ws = nullptr, long ws_elems = 0);
+// gemv_t on a bf16 hi/lo split matrix (element = hi + lo, fp64 accumulation).
+int gemv_t_split(int K, int N, const unsigned short* hi, const unsigned short* lo, long lda, const double* x,
+                 double* y, double alpha, double beta, hipStream_t s, double* ws, long ws_elems);
 
 // Two-index strided contraction over a 4-index array:
 //   out[p*P1 + q] = beta*out + alpha * sum_{r<R, s<S} X[p*xp + q*xq + r*xr + s*xs] * Y[r*yr + s*ys]

@@ -382,9 +382,20 @@ __global__ void gemv_kernel(int M, int K, const double* __restrict__ A, long lda
   }
 }
 
-__global__ void gemv_t_kernel(int K, int N, const double* __restrict__ A, long lda, const double* __restrict__ x,
-                              double* __restrict__ y, double alpha, double beta, int kchunk,
-                              double* __restrict__ part) {
+// Matrix element (k, n) of gemv_t: fp64, or a bf16 hi/lo split (hi + lo).
+struct GemvF64 {
+  const double* a;
+  __device__ double operator()(long q) const { return a[q]; }
+};
+struct GemvSplit {
+  const unsigned short* hi;
+  const unsigned short* lo;
+  __device__ double operator()(long q) const { return (double)bf16_to_f(hi[q]) + (double)bf16_to_f(lo[q]); }
+};
+
+template <class Mat>
+__global__ void gemv_t_kernel(int K, int N, Mat A, long lda, const double* __restrict__ x, double* __restrict__ y,
+                              double alpha, double beta, int kchunk, double* __restrict__ part) {
   // Block (256 x 4) handles 256 columns and the k slice [by*kchunk, (by+1)*kchunk);
   // its four rows of threads stride that slice and are summed through LDS.
   // With part != nullptr the block writes its slice sum to part[by][n].
@@ -394,7 +405,7 @@ __global__ void gemv_t_kernel(int K, int N, const double* __restrict__ A, long l
   const long k1 = min((long)K, k0 + kchunk);
   double acc = 0.0;
   if (n < N)
-    for (long k = k0 + threadIdx.y; k < k1; k += blockDim.y) acc += A[k * lda + n] * x[k];
+    for (long k = k0 + threadIdx.y; k < k1; k += blockDim.y) acc += A(k * lda + n) * x[k];
   red[threadIdx.y][threadIdx.x] = acc;
   __syncthreads();
   if (threadIdx.y == 0 && n < N) {
@@ -424,8 +435,9 @@ int gemv(int M, int K, const double* A, long lda, const double* x, double* y, do
   return ACES_OK;
 }
 
-int gemv_t(int K, int N, const double* A, long lda, const double* x, double* y, double alpha, double beta,
-           hipStream_t s, double* ws, long ws_elems) {
+template <class Mat>
+static int gemv_t_launch(int K, int N, Mat A, long lda, const double* x, double* y, double alpha, double beta,
+                         hipStream_t s, double* ws, long ws_elems) {
   if (N == 0) return ACES_OK;
   const long nbx = (N + 255) / 256;
   // split k so that the launch holds about 2048 blocks (deterministic two-pass sum)
@@ -436,8 +448,8 @@ int gemv_t(int K, int N, const double* A, long lda, const double* x, double* y, 
     nsplit = std::min<long>(nsplit, 65535);
   }
   const long kchunk = (K + nsplit - 1) / std::max<long>(nsplit, 1);
-  hipLaunchKernelGGL(gemv_t_kernel, dim3((unsigned)nbx, (unsigned)nsplit), dim3(256, 4), 0, s, K, N, A, lda, x, y,
-                     alpha, beta, (int)std::max<long>(kchunk, 1), nsplit > 1 ? ws : nullptr);
+  hipLaunchKernelGGL(gemv_t_kernel<Mat>, dim3((unsigned)nbx, (unsigned)nsplit), dim3(256, 4), 0, s, K, N, A, lda,
+                     x, y, alpha, beta, (int)std::max<long>(kchunk, 1), nsplit > 1 ? ws : nullptr);
   ACES_HIP(hipGetLastError());
   if (nsplit > 1) {
     hipLaunchKernelGGL(gemv_t_finish_kernel, dim3((unsigned)nbx), dim3(256), 0, s, N, (int)nsplit, ws, y, alpha,
@@ -445,6 +457,16 @@ int gemv_t(int K, int N, const double* A, long lda, const double* x, double* y, 
     ACES_HIP(hipGetLastError());
   }
   return ACES_OK;
+}
+
+int gemv_t(int K, int N, const double* A, long lda, const double* x, double* y, double alpha, double beta,
+           hipStream_t s, double* ws, long ws_elems) {
+  return gemv_t_launch(K, N, GemvF64{A}, lda, x, y, alpha, beta, s, ws, ws_elems);
+}
+
+int gemv_t_split(int K, int N, const unsigned short* hi, const unsigned short* lo, long lda, const double* x,
+                 double* y, double alpha, double beta, hipStream_t s, double* ws, long ws_elems) {
+  return gemv_t_launch(K, N, GemvSplit{hi, lo}, lda, x, y, alpha, beta, s, ws, ws_elems);
 }
 
 // ---------------------------------------------------------------------------
