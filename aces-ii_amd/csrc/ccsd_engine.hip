@@ -470,9 +470,23 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
       r.t2hi = (unsigned short*)lp_[24].hi;
       r.t2lo = (unsigned short*)lp_[24].lo;
       r.ldt = lp_[24].ld;
+      // tau's splits: slot 13 [(i,j)][(a,b)] (hole ladder), 22 [i][(j,a,b)]
+      // (Foo, tau M3), 23 [a][(i,j,b)] (tauK: Fvv, M7 tauK)
+      ACES_CHECK(lowp_slot(13, oo, vv));
+      ACES_CHECK(lowp_slot(22, o, ov2));
+      ACES_CHECK(lowp_slot(23, v, o2v));
+      const long ts[3][3] = {{o * lp_[13].ld, lp_[13].ld, v}, {lp_[22].ld, vv, v}, {(long)o * v, v, lp_[23].ld}};
+      const int tslot[3] = {13, 22, 23};
+      for (int q = 0; q < 3; ++q) {
+        r.tauhi[q] = (unsigned short*)lp_[tslot[q]].hi;
+        r.taulo[q] = (unsigned short*)lp_[tslot[q]].lo;
+        r.tsi[q] = ts[q][0];
+        r.tsj[q] = ts[q][1];
+        r.tsa[q] = ts[q][2];
+      }
       ACES_CHECK(ring_operands(o, v, t1, t2, r, s));
       for (int q = 0; q < 4; ++q) lp_fresh_[slot[q]] = true;
-      lp_fresh_[24] = true;
+      lp_fresh_[24] = lp_fresh_[13] = lp_fresh_[22] = lp_fresh_[23] = true;
     } else {
       r.f64[1] = X_;
       r.f64[2] = Ut_;
@@ -560,11 +574,21 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   // hole-hole ladder: Wo = (ki|lj) + Qo[klij] + Qo[lkji] + (kc|ld) tau_ijcd ;
   // inc2[ij][ab] += sum_kl Wo[kl][ij] tau[kl][ab]  (both products in the ladder precision)
   ACES_CHECK(gemm64(gd((int)(oo * o), o, v, ovooQ_, v, 1, t1, v, 1, Qo_, o, 1.0, 0.0)));
-  ACES_CHECK(P4(o, o, o, o, Wo_).src(ooooP_, 1, oo * o, oo, o, 1).src(Qo_, 1, oo * o, oo, o, 1)
-                 .src(Qo_, 1, oo, oo * o, 1, o).run(s));
-  prof_cat_ = 0;
-  ACES_CHECK(gemm_lp(prec, (int)oo, (int)oo, (int)vv, OVkl_, 4, vv, tau_, 13, vv, Wo_, oo, 1.0, 1.0));
-  ACES_CHECK(gemm_lp(prec, (int)oo, (int)vv, (int)oo, Wo_, 14, oo, tau_, 15, vv, inc2_, vv, 1.0, 1.0, 0, 0));
+  if (prec == GP_BF16X3) {
+    // transposed: Wo^T[(i,j)][(k,l)] += tau[ij][cd] OVkl[kl][cd], written as the
+    // split left operand (slot 14) of inc2 += Wo^T tau
+    ACES_CHECK(P4(o, o, o, o, Wo_).src(ooooP_, 1, o, 1, oo * o, oo).src(Qo_, 1, o, 1, oo * o, oo)
+                   .src(Qo_, 1, 1, o, oo, oo * o).run(s));
+    prof_cat_ = 0;
+    ACES_CHECK(gemm_lp(prec, (int)oo, (int)oo, (int)vv, tau_, 13, vv, OVkl_, 4, vv, Wo_, oo, 1.0, 1.0, 1, 1, 14));
+    ACES_CHECK(gemm_lp(prec, (int)oo, (int)vv, (int)oo, Wo_, 14, oo, tau_, 15, vv, inc2_, vv, 1.0, 1.0, 1, 0));
+  } else {
+    ACES_CHECK(P4(o, o, o, o, Wo_).src(ooooP_, 1, oo * o, oo, o, 1).src(Qo_, 1, oo * o, oo, o, 1)
+                   .src(Qo_, 1, oo, oo * o, 1, o).run(s));
+    prof_cat_ = 0;
+    ACES_CHECK(gemm_lp(prec, (int)oo, (int)oo, (int)vv, OVkl_, 4, vv, tau_, 13, vv, Wo_, oo, 1.0, 1.0));
+    ACES_CHECK(gemm_lp(prec, (int)oo, (int)vv, (int)oo, Wo_, 14, oo, tau_, 15, vv, inc2_, vv, 1.0, 1.0, 0, 0));
+  }
   prof_cat_ = 3;
 
   // F intermediates (QUAD2/QUAD3/MAKFME + T1RING pieces)
@@ -573,7 +597,8 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
   ACES_CHECK(gemv(OV, OV, Lmat_, ov, t1, Fov_, 1.0, 1.0, s));
   ACES_CHECK(P4(1, 1, o, o, Foo_).src(fock_, 1, 0, 0, n, 1).run(s));
   ACES_CHECK(gemm_lp(prec, o, o, (int)ov2, Lo_, 28, ov2, tau_, 22, ov2, Foo_, o, 1.0, 1.0));
-  ACES_CHECK(P4(v, o, o, v, tauK_).src(tau_, 1, v, ov2, vv, 1).run(s));  // tauK[a][k][l][d] = tau_klad
+  if (prec != GP_BF16X3)  // (the bf16 form has tauK's split from the ring-operand pass)
+    ACES_CHECK(P4(v, o, o, v, tauK_).src(tau_, 1, v, ov2, vv, 1).run(s));  // tauK[a][k][l][d] = tau_klad
   ACES_CHECK(P4(1, 1, v, v, Fvv_).src(fock_ + (long)o * n + o, 1, 0, 0, n, 1).run(s));
   ACES_CHECK(gemm_lp(prec, v, v, (int)(oo * v), tauK_, 23, oo * v, Lv_, 19, oo * v, Fvv_, v, -1.0, 1.0));
   ACES_CHECK(gemm64(gd(o, o, v, fov, n, 1, t1, v, 1, FT_, o, 1.0, 0.0)));       // FT[k][i] = f_kc t_ic

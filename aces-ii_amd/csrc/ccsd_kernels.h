@@ -36,6 +36,10 @@ struct RingOperands {
   unsigned short* t2hi = nullptr;
   unsigned short* t2lo = nullptr;
   long ldt = 0;  // >= v
+  // up to three bf16 splits of tau, element (i,j,a,b) at i*tsi + j*tsj + a*tsa + b
+  unsigned short* tauhi[3] = {nullptr, nullptr, nullptr};
+  unsigned short* taulo[3] = {nullptr, nullptr, nullptr};
+  long tsi[3] = {0, 0, 0}, tsj[3] = {0, 0, 0}, tsa[3] = {0, 0, 0};
 };
 int ring_operands(int o, int v, const double* t1, const double* t2, const RingOperands& r, hipStream_t s);
 int jacobi(int o, int v, const double* eo, const double* ev, const double* inc1, const double* inc2, double* t1n,

@@ -273,7 +273,8 @@ int unpack_ladder(int o, int v, const double* Zp, long ldzp, const double* Zm, l
 // One block per (i, j, 32 values of a, 64 values of b): t_ijab is read along b,
 // t_ijba along a and turned round through LDS.  Each output goes to fp64
 // (ld = ov) and/or to bf16 hi/lo rows (ld = ldl) where the pointer is set;
-// tau_ijab = t_ijab + t1_ia t1_jb and the bf16 split of t_ijab keep t2's order.
+// tau_ijab = t_ijab + t1_ia t1_jb and the bf16 split of t_ijab keep t2's order;
+// the bf16 splits of tau go wherever their strides put them.
 
 __global__ __launch_bounds__(256) void ring_operands_kernel(int o, int v, const double* __restrict__ t1,
                                                             const double* __restrict__ t2, RingOperands r) {
@@ -311,7 +312,16 @@ __global__ __launch_bounds__(256) void ring_operands_kernel(int o, int v, const 
     const int a = a0 + ra + 4 * k;
     if (a >= v) break;
     const double d = T[(long)a * v + b];
-    if (r.tau) r.tau[(trow + a) * v + b] = d + t1[(long)i * v + a] * tjb;
+    const double tv = d + t1[(long)i * v + a] * tjb;
+    if (r.tau) r.tau[(trow + a) * v + b] = tv;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      if (r.tauhi[q]) {
+        const long off = i * r.tsi[q] + j * r.tsj[q] + a * r.tsa[q] + b;
+        const unsigned short h = bf16_rn((float)tv);
+        r.tauhi[q][off] = h;
+        r.taulo[q][off] = bf16_rn((float)(tv - (double)bf16_to_f(h)));
+      }
     if (r.t2hi) {
       const unsigned short h = bf16_rn((float)d);
       r.t2hi[(trow + a) * r.ldt + b] = h;
@@ -343,6 +353,8 @@ int ring_operands(int o, int v, const double* t1, const double* t2, const RingOp
                  "ring_operands: bad bf16 output");
   ACES_REQUIRE((r.t2hi == nullptr) == (r.t2lo == nullptr) && (r.t2hi == nullptr || r.ldt >= v),
                "ring_operands: bad t2 split output");
+  for (int q = 0; q < 3; ++q)
+    ACES_REQUIRE((r.tauhi[q] == nullptr) == (r.taulo[q] == nullptr), "ring_operands: bad tau split output");
   hipLaunchKernelGGL(ring_operands_kernel, dim3((unsigned)nblk), dim3(256), 0, s, o, v, t1, t2, r);
   ACES_HIP(hipGetLastError());
   return ACES_OK;
