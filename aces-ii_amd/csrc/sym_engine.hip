@@ -1030,8 +1030,8 @@ int SymEngine::setup(const double* fock, const double* ovov_dense, const double*
   ACES_CHECK(perm(Xt, 0, {{X, I4}}, {1}, t1_, {0, 3}, t1_, {2, 1}, 2.0));
   // T1 parts of W(mbej)
   const int P1 = dy(LOVVO3), P2 = dy(LOVVO3), Q12 = dy(LOVOV3), Q2 = dy(LOVOV3);
-  ACES_CHECK(gemm_op(P1, ovvv_t, 1, t1_, 1, 1, 0, 3, false));
-  ACES_CHECK(gemm_op(P2, kacd, 1, t1_, 1, 1, 0, 3, false));
+  ACES_CHECK(gemm_op(P1, ovvv_t, 1, t1_, 1, 1, 0, 3, true));
+  ACES_CHECK(gemm_op(P2, kacd, 1, t1_, 1, 1, 0, 3, true));
   ACES_CHECK(gemm_op(Q12, M5c, 1, t1_, 0, 1, 0, 3, false));
   ACES_CHECK(gemm_op(Q2, M5b, 1, t1_, 0, 1, 0, 3, false));
   const int Wt = dy(LOVOV), Wv = dy(LOVOV);
@@ -1078,7 +1078,7 @@ int SymEngine::setup(const double* fock, const double* ovov_dense, const double*
   const int Qo = dy(LOOOO3), Wo = dy(LOOOO), ZH = dy(LVVOO2);
   ACES_CHECK(gemm_op(Qo, ovooQ, 1, t1_, 1, 1, 0, 3, false));
   ACES_CHECK(perm(Wo, 0, {{ooooP, I4}, {Qo, I4}, {Qo, {1, 0, 3, 2}}}, {1, 1, 1}));
-  ACES_CHECK(gemm_op(Wo, OVkl, 1, tau_, 1, 1, 1, 3, false));
+  ACES_CHECK(gemm_op(Wo, OVkl, 1, tau_, 1, 1, 1, 3, true));
   ACES_CHECK(gemm_op(ZH, tau_, 0, Wo, 0, 1, 0, 3, false));
   ACES_CHECK(perm(inc2_, 1, {{ZH, {2, 3, 0, 1}}}, {1}));
   // F intermediates
@@ -1088,10 +1088,10 @@ int SymEngine::setup(const double* fock, const double* ovov_dense, const double*
   ACES_CHECK(gemv_op(Fov, Lmat, t1_, 0, 1, 1));
   ACES_CHECK(perm(tau1, 0, {{tau_, I4}}, {1}));
   ACES_CHECK(perm(Foo, 0, {{foo_, {0, 1}}}, {1}));
-  ACES_CHECK(gemm_op(Foo, Lo, 1, tau1, 1, 1, 1, 3, false));
+  ACES_CHECK(gemm_op(Foo, Lo, 1, tau1, 1, 1, 1, 3, true));
   ACES_CHECK(perm(tauK, 0, {{tau_, {1, 2, 0, 3}}}, {1}));
   ACES_CHECK(perm(Fvv, 0, {{fvv_, {0, 1}}}, {1}));
-  ACES_CHECK(gemm_op(Fvv, tauK, 1, Lv, 1, -1, 1, 3, false));
+  ACES_CHECK(gemm_op(Fvv, tauK, 1, Lv, 1, -1, 1, 3, true));
   ACES_CHECK(gemm_op(FT, fov_, 1, t1_, 1, 1, 0, 3, false));
   ACES_CHECK(gemm_op(FovT, Fov, 1, t1_, 1, 1, 0, 3, false));
   ACES_CHECK(perm(Loo, 0, {{Foo, {0, 1}}, {FT, {0, 1}}}, {1, 1}));
@@ -1122,13 +1122,13 @@ int SymEngine::setup(const double* fock, const double* ovov_dense, const double*
   ACES_CHECK(perm(inc2_, 1, {{tB, {1, 0, 3, 2}}}, {1}));
   // T1INT2A / T1INT2B
   const int Qb = dy(LOOVO3), Qb1 = dy(LOOVO1), T1Q = dy(LVOVO1);
-  ACES_CHECK(gemm_op(Qb, oovv3, 1, t1_, 1, 1, 0, 3, false));
+  ACES_CHECK(gemm_op(Qb, oovv3, 1, t1_, 1, 1, 0, 3, true));
   ACES_CHECK(perm(Qb1, 0, {{Qb, I4}}, {1}));
   ACES_CHECK(gemm_op(T1Q, t1_, 0, Qb1, 0, 1, 0, 3, false));
   ACES_CHECK(perm(inc2_, 1, {{P1, {0, 2, 3, 1}}, {T1Q, {2, 0, 3, 1}}, {P1, {1, 3, 2, 0}}}, {1, -1, 1}));
   ACES_CHECK(perm(inc2_, 1, {{T1Q, {3, 1, 2, 0}}}, {-1}));
   const int S3 = dy(LOVOO3), G4A = dy(LOVOO3), G4 = dy(LOVOV3);
-  ACES_CHECK(gemm_op(S3, ovov3, 1, t1_, 1, 1, 0, 3, false));
+  ACES_CHECK(gemm_op(S3, ovov3, 1, t1_, 1, 1, 0, 3, true));
   ACES_CHECK(perm(G4A, 0, {{ovoo_t, I4}, {S3, {0, 1, 3, 2}}}, {1, 1}));
   ACES_CHECK(gemm_op(G4, G4A, 1, t1_, 0, 1, 0, 3, false));
   ACES_CHECK(perm(inc2_, 1, {{G4, {0, 2, 1, 3}}, {G4, {1, 3, 0, 2}}}, {-1, -1}));
@@ -1141,8 +1141,8 @@ int SymEngine::setup(const double* fock, const double* ovov_dense, const double*
   ACES_CHECK(gemm_op(inc1_, t1_, 1, Fvv, 1, 1, 1, 3, false));
   ACES_CHECK(gemv_op(inc1_, U, Fov, 0, 1, 1));
   ACES_CHECK(gemv_op(inc1_, W0t, t1_, 0, 1, 1));
-  ACES_CHECK(gemm_op(inc1_, tau1, 1, M3, 1, 1, 1, 3, false));
-  ACES_CHECK(gemm_op(inc1_, M7, 1, tauK, 1, -1, 1, 3, false));
+  ACES_CHECK(gemm_op(inc1_, tau1, 1, M3, 1, 1, 1, 3, true));
+  ACES_CHECK(gemm_op(inc1_, M7, 1, tauK, 1, -1, 1, 3, true));
   for (Tensor& t : T_)
     if (t.grad && t.g == nullptr) ACES_CHECK(alloc(&t.g, std::max<long>(t.boff[nirr], 1)));
   low_.resize(T_.size());
