@@ -227,6 +227,11 @@ int build_plan(Plan& p, const Layout& out, const std::vector<Src>& srcs, const S
 void free_plan(Plan& p);
 // Host part of build_plan (boxes only, nothing uploaded).
 int build_boxes(Plan& p, const Layout& out, const std::vector<Src>& srcs, const Src* X, const Src* Y);
+// For a rank-4 plan whose largest box spans at least 16 along the output's
+// fastest index: the index (0..2) along which source s has unit stride and
+// that a tile can span, 3 when the source reads well in the output's order,
+// -1 when the plan cannot be tiled at all.
+int source_fast_index(const Plan& p, int rank, int s);
 
 // Denominator scaling: D = sum_k dsign[k] * e(space of dim k)[global index],
 // e = eo for occupied dims, ev for virtual dims.

@@ -114,6 +114,25 @@ PairSpace make_pairs(const Space& s, bool strict) {
 // (with none, q is the largest of indices 0..2).
 // (The tiled kernel uses every box's own strides, so this choice only decides
 // speed.)  Sets p.tq, p.tr, p.ttotal and Box::tfirst.
+int source_fast_index(const Plan& p, int rank, int s) {
+  if (rank != 4 || p.nbox == 0 || s < 0 || s >= p.nsrc) return -1;
+  const Box* big = &p.host[0];
+  long best = 0;
+  for (const Box& b : p.host) {
+    const long e = (long)b.d[0] * b.d[1] * b.d[2] * b.d[3];
+    if (e > best) {
+      best = e;
+      big = &b;
+    }
+  }
+  if (big->d[3] < 16) return -1;
+  const long s3 = big->ss[s][3] < 0 ? -big->ss[s][3] : big->ss[s][3];
+  if (s3 < 8) return 3;
+  for (int kk = 2; kk >= 0; --kk)
+    if (big->d[kk] >= 8 && big->ss[s][kk] == 1) return kk;
+  return 3;
+}
+
 static void plan_tiles(Plan& p, int rank) {
   p.tq = -1;
   p.tr = 0;

@@ -301,6 +301,38 @@ int SymEngine::perm(int out, double beta, std::vector<std::pair<int, std::vector
     }
   }
   ACES_CHECK(dpd::build_plan(op.fwd, T_[out].L, ss, X >= 0 ? &sx : nullptr, X >= 0 ? &sy : nullptr));
+  if (record && op.fwd.tq < 0 && op.nsrc >= 2) {
+    // sources read across the output's fastest index along different
+    // indices: one tiled pass per such index (the first also takes the direct
+    // sources, the product term and beta; the others accumulate)
+    int q[3], qs[3], nq = 0;
+    for (int k = 0; k < op.nsrc; ++k) {
+      q[k] = dpd::source_fast_index(op.fwd, T_[out].L.rank, k);
+      if (q[k] < 0) nq = -8;
+      if (q[k] >= 0 && q[k] < 3) {
+        bool seen = false;
+        for (int j = 0; j < nq; ++j) seen |= qs[j] == q[k];
+        if (!seen && nq >= 0) qs[nq++] = q[k];
+      }
+    }
+    if (nq >= 2) {
+      dpd::free_plan(op.fwd);
+      for (int g = 0; g < nq; ++g) {
+        std::vector<std::pair<int, std::vector<int>>> sub;
+        std::vector<double> al;
+        for (int k = 0; k < op.nsrc; ++k)
+          if (q[k] == qs[g] || (g == 0 && q[k] == 3)) {
+            sub.push_back(srcs[k]);
+            al.push_back(alpha[k]);
+          }
+        if (g == 0)
+          ACES_CHECK(perm(out, beta, sub, al, X, xmap, Y, ymap, gamma, record));
+        else
+          ACES_CHECK(perm(out, 1.0, sub, al, -1, {}, -1, {}, 0.0, record));
+      }
+      return ACES_OK;
+    }
+  }
   if (!record) {
     const double* sp[3] = {nullptr, nullptr, nullptr};
     for (int s = 0; s < op.nsrc; ++s) sp[s] = T_[op.src[s]].d;
