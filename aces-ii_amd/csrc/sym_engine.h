@@ -61,6 +61,7 @@ class SymEngine {
     double* d = nullptr;
     double* g = nullptr;
     bool grad = false;
+    long ver = 0;  // bumped whenever the forward pass (re)writes the tensor
   };
   enum Kind { OP_PERM, OP_GEMM, OP_GEMV, OP_SUBDIAG, OP_PACK, OP_UNPACK };
   struct Op {
@@ -92,6 +93,7 @@ class SymEngine {
     long ld[dpd::kMaxIrr] = {0};
     GemmPrec prec = GP_F64;
     bool valid = false;
+    long ver = -1;  // tensor version the split was made from
   };
 
   int alloc(double** p, long count);

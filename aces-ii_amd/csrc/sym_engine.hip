@@ -652,6 +652,10 @@ int SymEngine::outer0(int A, double* Ad, const double* u, const double* w, doubl
 
 // ---------------------------------------------------------------------------
 int SymEngine::forward(GemmPrec prec) {
+  // amplitude-dependent tensors change between passes; within a pass a split
+  // is reused until the tensor is written again
+  for (Tensor& t : T_)
+    if (t.grad) ++t.ver;
   for (Op& op : ops_) {
     switch (op.kind) {
       case OP_PERM: {
@@ -665,7 +669,10 @@ int SymEngine::forward(GemmPrec prec) {
         if (op.lowp && prec != GP_F64) {
           for (int t : {op.A, op.B}) {
             LowBuf& lb = low_[t];
-            if (T_[t].grad || !lb.valid || lb.prec != prec) ACES_CHECK(lowp_convert(t, prec, lb));
+            if (!lb.valid || lb.prec != prec || lb.ver != T_[t].ver) {
+              ACES_CHECK(lowp_convert(t, prec, lb));
+              lb.ver = T_[t].ver;
+            }
           }
           ACES_CHECK(ggemm(op.out, T_[op.out].d, op.A, T_[op.A].d, op.a_kc, op.B, T_[op.B].d, op.b_kc, op.al,
                            op.beta, prec, op.cat, &low_[op.A], &low_[op.B]));
@@ -687,7 +694,13 @@ int SymEngine::forward(GemmPrec prec) {
       case OP_UNPACK:
         ACES_CHECK(dpd::unpack_ladder_sym(d_lad_, op.fwd, op.hplan, T_[Zp_].d, T_[Zm_].d, T_[inc2_].d,
                                           T_[Yb_].d, s_));
+        ++T_[Yb_].ver;
         break;
+    }
+    if (op.out >= 0) ++T_[op.out].ver;
+    if (op.kind == OP_PACK) {
+      ++T_[Sp_].ver;
+      ++T_[Dp_].ver;
     }
   }
   fwd_valid_ = (prec == GP_F64);
