@@ -51,11 +51,13 @@ def test_increments_fp64(o, v, seed, nonhf):
     cc.close()
 
 
-@pytest.mark.parametrize('prec,tol', [('fp32', 2e-6), ('bf16x3', 5e-5)])
-def test_increments_lowp(prec, tol):
+# (7, 45): no operand length is a multiple of 32 (v, v^2, ov, ov^2, o^2 v), so
+# every split written straight into an operand slot has padding columns, and
+# odd o gives the antisymmetric ladder pairs a ragged row count
+@pytest.mark.parametrize('prec,tol,o,v', [('fp32', 2e-6, 8, 48), ('bf16x3', 5e-5, 8, 48), ('bf16x3', 5e-5, 7, 45)])
+def test_increments_lowp(prec, tol, o, v):
     from aces2.ccsd import CCSD
-    print('case lowp', prec, flush=True)
-    o, v = 8, 48
+    print('case lowp', prec, o, v, flush=True)
     blk, eri = _system(o, v, 11, True)
     cc = CCSD(o, v, blk)
     t1, t2 = _random_t(o, v, 12)
