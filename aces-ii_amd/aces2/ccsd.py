@@ -114,8 +114,13 @@ class CCSD:
         return {'energy': st[0], 'resmiter': st[1], 'resmdiis': st[2], 'converged': bool(st[3]),
                 'final_energy': st[4], 'ms_increments': st[5], 'ms_cycle': st[6]}
 
-    def run(self, maxcyc=50, conv=7, order=6, start='mp1', prec='fp64', log=None, raise_on_fail=True):
-        """The reference's CC loop; returns the per-cycle history."""
+    def run(self, maxcyc=50, conv=7, order=6, start='mp1', prec='fp64', log=None, raise_on_fail=True,
+            polish_conv=None):
+        """The reference's CC loop; returns the per-cycle history.  With a
+        low-precision ``prec`` and ``polish_conv``, the loop switches to fp64
+        cycles once |RESMITER| < 10^-conv (DIIS restarted, as its low-precision
+        error vectors would hold the extrapolation at their accuracy) and runs to
+        10^-polish_conv; ``hist['precision']`` records each cycle's form."""
         torch = self.torch
         self.set_diis_order(order)
         self.reset_diis()
@@ -129,14 +134,21 @@ class CCSD:
         hist = {'energy': [self.energy(t1, t2)], 'resmiter': [], 'resmdiis': [], 'ms_cycle': [],
                 'converged': False}
         tol = 10.0 ** (-conv)
+        phase, ic0 = prec, 0
+        hist['precision'] = []
         for icycle in range(1, maxcyc + 1):
-            st = self.cycle(icycle, prec, tol)
+            st = self.cycle(icycle - ic0, phase, tol)
             hist['energy'].append(st['energy'])
             hist['resmiter'].append(st['resmiter'])
             hist['resmdiis'].append(st['resmdiis'])
             hist['ms_cycle'].append(st['ms_cycle'])
+            hist['precision'].append(phase)
             if log:
                 log(icycle, st)
+            if st['converged'] and phase != 'fp64' and polish_conv is not None:
+                phase, tol, ic0 = 'fp64', 10.0 ** (-polish_conv), icycle
+                self.reset_diis()
+                continue
             if st['converged']:
                 hist['converged'] = True
                 hist['final_energy'] = st['final_energy']
@@ -257,7 +269,8 @@ class SymCCSD(CCSD):
     def packed_t2(self):
         return _lib.lib().aces_sym_packed_t2(self.h)
 
-    def run(self, maxcyc=50, conv=7, order=6, start='mp1', prec='fp64', log=None, raise_on_fail=True):
+    def run(self, maxcyc=50, conv=7, order=6, start='mp1', prec='fp64', log=None, raise_on_fail=True,
+            polish_conv=None):
         torch = self.torch
         self.set_diis_order(order)
         self.reset_diis()
@@ -267,14 +280,21 @@ class SymCCSD(CCSD):
             self.set_amplitudes(torch.zeros(self.o, self.v), torch.zeros(self.o, self.o, self.v, self.v))
         hist = {'energy': [self.energy()], 'resmiter': [], 'resmdiis': [], 'ms_cycle': [], 'converged': False}
         tol = 10.0 ** (-conv)
+        phase, ic0 = prec, 0
+        hist['precision'] = []
         for icycle in range(1, maxcyc + 1):
-            st = self.cycle(icycle, prec, tol)
+            st = self.cycle(icycle - ic0, phase, tol)
             hist['energy'].append(st['energy'])
             hist['resmiter'].append(st['resmiter'])
             hist['resmdiis'].append(st['resmdiis'])
             hist['ms_cycle'].append(st['ms_cycle'])
+            hist['precision'].append(phase)
             if log:
                 log(icycle, st)
+            if st['converged'] and phase != 'fp64' and polish_conv is not None:
+                phase, tol, ic0 = 'fp64', 10.0 ** (-polish_conv), icycle
+                self.reset_diis()
+                continue
             if st['converged']:
                 hist['converged'] = True
                 hist['final_energy'] = st['final_energy']

@@ -86,3 +86,21 @@ def test_cc_loop_matches_oracle(o, v, seed, nonhf):
     np.testing.assert_allclose(hist['resmdiis'], ref['resmdiis'], rtol=1e-5, atol=1e-12)
     assert abs(hist['final_energy'] - ref['final_energy']) < 1e-10
     cc.close()
+
+
+@pytest.mark.parametrize('o,v,seed', [(6, 30, 24), (7, 45, 25)])
+def test_mixed_then_fp64_polish(o, v, seed):
+    """bf16x3 cycles to 1e-7, then fp64 cycles (DIIS restarted) to 1e-10: the
+    same energy as the fp64 loop to 1e-10, and the polish takes fewer cycles
+    than the fp64 loop needs from the start."""
+    from aces2.ccsd import CCSD
+    print('case polish', o, v, seed, flush=True)
+    blk, eri = _system(o, v, seed, True)
+    cc = CCSD(o, v, blk)
+    h64 = cc.run(maxcyc=80, conv=10, order=6, start='mp1')
+    hmx = cc.run(maxcyc=80, conv=7, order=6, start='mp1', prec='bf16x3', polish_conv=10)
+    assert h64['converged'] and hmx['converged']
+    assert hmx['precision'][-1] == 'fp64' and hmx['precision'][0] == 'bf16x3'
+    assert abs(hmx['final_energy'] - h64['final_energy']) < 1e-9
+    assert hmx['precision'].count('fp64') < h64['cycles']
+    cc.close()
