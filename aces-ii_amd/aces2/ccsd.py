@@ -12,7 +12,9 @@ import ctypes
 
 from . import _lib
 
-PREC = {'fp64': 0, 'fp32': 1, 'bf16x3': 2}
+# 'refine': fp64 cycle whose eight large products run as bf16x3 products of
+# the operands' differences to an fp64 anchor evaluation (C1 engine only)
+PREC = {'fp64': 0, 'fp32': 1, 'bf16x3': 2, 'refine': 3}
 
 
 class NotConverged(RuntimeError):
@@ -108,6 +110,10 @@ class CCSD:
     def reset_diis(self):
         _lib.call(self._p + 'reset_diis', self.h)
 
+    def reset_anchor(self):
+        """The next 'refine' cycle records a new fp64 anchor."""
+        _lib.call('aces_ccsd_reset_anchor', self.h)
+
     def cycle(self, icycle, prec='fp64', tol=1e-7):
         st = (ctypes.c_double * 8)()
         _lib.call(self._p + 'cycle', self.h, icycle, PREC[prec], tol, st)
@@ -115,12 +121,15 @@ class CCSD:
                 'final_energy': st[4], 'ms_increments': st[5], 'ms_cycle': st[6]}
 
     def run(self, maxcyc=50, conv=7, order=6, start='mp1', prec='fp64', log=None, raise_on_fail=True,
-            polish_conv=None):
+            polish_conv=None, polish='refine'):
         """The reference's CC loop; returns the per-cycle history.  With a
-        low-precision ``prec`` and ``polish_conv``, the loop switches to fp64
-        cycles once |RESMITER| < 10^-conv (DIIS restarted, as its low-precision
-        error vectors would hold the extrapolation at their accuracy) and runs to
-        10^-polish_conv; ``hist['precision']`` records each cycle's form."""
+        low-precision ``prec`` and ``polish_conv``, the loop switches to
+        ``polish`` cycles once |RESMITER| < 10^-conv (DIIS restarted, as its
+        low-precision error vectors would hold the extrapolation at their
+        accuracy) and runs to 10^-polish_conv; ``polish`` is 'fp64' or 'refine'
+        (fp64 accuracy from bf16x3 products of the differences to the amplitudes
+        of the switch, anchored by the first polish cycle's fp64 evaluation).
+        ``hist['precision']`` records each cycle's form."""
         torch = self.torch
         self.set_diis_order(order)
         self.reset_diis()
@@ -145,9 +154,11 @@ class CCSD:
             hist['precision'].append(phase)
             if log:
                 log(icycle, st)
-            if st['converged'] and phase != 'fp64' and polish_conv is not None:
-                phase, tol, ic0 = 'fp64', 10.0 ** (-polish_conv), icycle
+            if st['converged'] and phase == prec and prec in ('fp32', 'bf16x3') and polish_conv is not None:
+                phase, tol, ic0 = polish, 10.0 ** (-polish_conv), icycle
                 self.reset_diis()
+                if polish == 'refine':
+                    self.reset_anchor()
                 continue
             if st['converged']:
                 hist['converged'] = True
@@ -291,7 +302,7 @@ class SymCCSD(CCSD):
             hist['precision'].append(phase)
             if log:
                 log(icycle, st)
-            if st['converged'] and phase != 'fp64' and polish_conv is not None:
+            if st['converged'] and phase == prec and prec in ('fp32', 'bf16x3') and polish_conv is not None:
                 phase, tol, ic0 = 'fp64', 10.0 ** (-polish_conv), icycle
                 self.reset_diis()
                 continue

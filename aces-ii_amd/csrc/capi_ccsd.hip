@@ -57,6 +57,12 @@ int aces_ccsd_start_mp1(void* h) {
   return eng(h)->start_mp1();
 }
 
+int aces_ccsd_reset_anchor(void* h) {
+  REQUIRE_HANDLE(h);
+  eng(h)->reset_anchor();
+  return ACES_OK;
+}
+
 int aces_ccsd_increments(void* h, int prec, double* inc1, double* inc2) {
   REQUIRE_HANDLE(h);
   return eng(h)->increments((GemmPrec)prec, inc1, inc2);

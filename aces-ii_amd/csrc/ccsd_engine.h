@@ -12,6 +12,12 @@
 
 namespace aces {
 
+// Cycle precision of the engine beyond the GEMM library's three: the eight
+// large products (ladders, W build, ring) as bf16x3 products of the operands'
+// differences to an fp64 anchor evaluation (CcsdEngine::gemm_anchored); the
+// first such cycle after the amplitudes were set records the anchor.
+constexpr int kPrecRefine = 3;
+
 struct CcsdStats {
   double energy_new = 0.0;   // CMPENG(60,2): E(T_new) printed each cycle
   double resmiter = 0.0;     // largest element of the residual (signed)
@@ -45,11 +51,13 @@ class CcsdEngine {
   int jacobi_step();
   int diis_step(int icycle, double tol, CcsdStats& st, bool xn_sym = false);
   // Writable views; writing through T() invalidates the resident forward intermediates.
-  double* T() { fwd_valid_ = false; return t1_; }  // [t1 | t2] current amplitudes
+  double* T() { fwd_valid_ = anc_valid_ = false; return t1_; }  // [t1 | t2] current amplitudes
   double* Tnew() { return t1n_; }    // [t1 | t2] Jacobi update
   double* Inc() { return inc1_; }    // [inc1 | inc2]
   int energy_dev(const double* t1, const double* t2, double* d_out);
   int reset_diis();
+  // Forget the refinement anchor (the next kPrecRefine cycle records a new one).
+  void reset_anchor() { anc_valid_ = false; }
   int set_diis_order(int order);
   int energy_of(const double* t1, const double* t2, double& e);
   // Replace the MO Fock matrix (n x n, device) keeping the integrals.
@@ -86,6 +94,17 @@ class CcsdEngine {
   int o, v, n;
 
  private:
+  struct LowP {
+    void* hi = nullptr;
+    void* lo = nullptr;
+    long ld = 0;
+    long rows = 0;
+  };
+  struct LowPOp {  // a bf16 split operand as the GEMM reads it
+    const void* hi = nullptr;
+    const void* lo = nullptr;
+    long ld = 0;
+  };
   int alloc(double** p, long count);
   int alloc_raw(void** p, long bytes);
   int compute_increments(GemmPrec prec);
@@ -99,6 +118,20 @@ class CcsdEngine {
               int bslot, long ldb, double* C, long ldc, double alpha, double beta, int a_kc = 1, int b_kc = 1,
               int cslot = -1);
   int timed_gemm(const GemmDesc& d, GemmPrec prec, double flops);
+  // Anchored refinement of one product C = beta C + alpha op(A) op(B):
+  //   recording:  C0 = op(A0) op(B0) in fp64, copies of the amplitude-dependent
+  //               operands A0/B0 kept; C = beta C + alpha C0
+  //   refining:   C = beta C + alpha (C0 + op(A) op(B - B0) + op(A - A0) op(B0))
+  //               with the last two products in bf16x3 (one of them when an
+  //               operand is a static integral block, whose difference is zero)
+  // The bf16x3 error of a difference product scales with the difference, so
+  // near convergence the result carries fp64 accuracy at bf16x3 speed.
+  int gemm_anchored(int M, int N, int K, const double* A64, int aslot, long lda, const double* B64, int bslot,
+                    long ldb, double* C, long ldc, double alpha, double beta, int a_kc, int b_kc);
+  int anchor_copy(const double* src, long elems, const double** copy);
+  int split_operand(LowPOp& op, int slot, const double* src, const double* sub, long rows, int K, long ld, int kc);
+  int lp_product(int M, int N, int K, const LowPOp& A, const LowPOp& B, double* C, long ldc, double alpha,
+                 double beta);
   int gemm64(GemmDesc d);  // fp64 GEMM with the engine's split-K workspace (timed as prof_cat_)
   int gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda, const double* B64,
                    int bslot, long ldb, double* C, long ldc, double alpha, double beta, int a_kc, int b_kc,
@@ -153,12 +186,6 @@ class CcsdEngine {
   // low precision operand slots: static (converted once per precision)
   // 0 Vp 1 Vm 2 Lmat 3 Kx 4 OVkl; per call 5 Sp 6 Dp 7 Ut 8 Xt 9 Wt 10 Wv
   // 11 U 12 X 13 tau 14 Wo^T 15 tau^T
-  struct LowP {
-    void* hi = nullptr;
-    void* lo = nullptr;
-    long ld = 0;
-    long rows = 0;
-  };
   // slots 0..4, 16..19 and 26..30 hold static operands (converted once per
   // precision), the others this cycle's operands
   static constexpr int kLpStatic = 5, kLpStatic2 = 16, kLpSlots = 31;
@@ -170,6 +197,22 @@ class CcsdEngine {
   bool lp_keep_[kLpSlots] = {};   // operand fixed for the whole cycle: convert once, use many times
   int lowp_slot(int slot, long rows, long K);
   GemmPrec lp_static_prec_ = GP_F64;
+  // refinement anchor (gemm_anchored): 0 off, 1 recording, 2 refining
+  int anc_mode_ = 0;
+  bool anc_valid_ = false;
+  int anc_site_ = 0;  // product index within one evaluation of the increments
+  struct AncSite {
+    double* C0 = nullptr;
+    long elems = 0;
+    LowP B0;  // split of the anchor's right operand (both operands amplitude-dependent)
+  };
+  std::vector<AncSite> anc_sites_;
+  struct AncSrc {
+    const double* src = nullptr;
+    double* copy = nullptr;
+    long elems = 0;
+  };
+  std::vector<AncSrc> anc_srcs_;
   // DIIS (amplitudes and multipliers)
   DiisState tdiis_, ldiis_;
   double* newvec_ = nullptr;

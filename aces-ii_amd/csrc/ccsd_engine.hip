@@ -224,7 +224,7 @@ int CcsdEngine::set_amplitudes(const double* t1, const double* t2) {
   ACES_HIP(hipMemcpyAsync(t1_, t1, sizeof(double) * ov, hipMemcpyDeviceToDevice, s_));
   ACES_HIP(hipMemcpyAsync(t2_, t2, sizeof(double) * ov * ov, hipMemcpyDeviceToDevice, s_));
   ACES_HIP(hipStreamSynchronize(s_));
-  fwd_valid_ = false;
+  fwd_valid_ = anc_valid_ = false;
   return ACES_OK;
 }
 
@@ -246,7 +246,7 @@ int CcsdEngine::start_mp1() {
   t_sym_ = false;
   ACES_CHECK(jacobi(o, v, eo_, ev_, inc1_, inc2_, t1_, t2_, s_));
   ACES_HIP(hipStreamSynchronize(s_));
-  fwd_valid_ = false;
+  fwd_valid_ = anc_valid_ = false;
   return ACES_OK;
 }
 
@@ -336,7 +336,7 @@ int CcsdEngine::set_fock(const double* fock) {
   ACES_CHECK(P4(1, 1, 1, o, eo_).src(fock_, 1, 0, 0, 0, (long)n + 1).run(s_));
   ACES_CHECK(P4(1, 1, 1, v, ev_).src(fock_ + (long)o * n + o, 1, 0, 0, 0, (long)n + 1).run(s_));
   ACES_HIP(hipStreamSynchronize(s_));
-  fwd_valid_ = false;
+  fwd_valid_ = anc_valid_ = false;
   return ACES_OK;
 }
 
@@ -349,7 +349,129 @@ void CcsdEngine::profile(double* out, int reset) {
 int CcsdEngine::gemm_lp(GemmPrec prec, int M, int N, int K, const double* A64, int aslot, long lda,
                         const double* B64, int bslot, long ldb, double* C, long ldc, double alpha, double beta,
                         int a_kc, int b_kc, int cslot) {
+  // the eight large products (profile categories ladder, W build, ring) of a
+  // refinement cycle; everything else of such a cycle is plain fp64
+  if (anc_mode_ != 0 && prec == GP_F64 && prof_cat_ <= 2)
+    return gemm_anchored(M, N, K, A64, aslot, lda, B64, bslot, ldb, C, ldc, alpha, beta, a_kc, b_kc);
   return gemm_lp_impl(prec, M, N, K, A64, aslot, lda, B64, bslot, ldb, C, ldc, alpha, beta, a_kc, b_kc, cslot);
+}
+
+// Keep (or refresh) the anchor copy of an amplitude-dependent operand buffer.
+// An operand buffer holds the same values at every anchored product that
+// reads it within one evaluation, so one copy per buffer serves them all.
+int CcsdEngine::anchor_copy(const double* src, long elems, const double** copy) {
+  AncSrc* e = nullptr;
+  for (AncSrc& a : anc_srcs_)
+    if (a.src == src) e = &a;
+  if (e == nullptr) {
+    ACES_REQUIRE(anc_mode_ == 1, "refinement: operand without an anchor copy");
+    anc_srcs_.push_back(AncSrc());
+    e = &anc_srcs_.back();
+    e->src = src;
+  }
+  if (e->copy == nullptr || e->elems < elems) {
+    ACES_REQUIRE(anc_mode_ == 1, "refinement: anchor copy smaller than the operand");
+    ACES_CHECK(alloc(&e->copy, elems));
+    e->elems = elems;
+  }
+  if (anc_mode_ == 1)
+    ACES_HIP(hipMemcpyAsync(e->copy, src, sizeof(double) * elems, hipMemcpyDeviceToDevice, s_));
+  *copy = e->copy;
+  return ACES_OK;
+}
+
+// Split src (minus sub, when given) into operand slot `slot`: rows x K,
+// k-contiguous (kc) or stored k-major.
+int CcsdEngine::split_operand(LowPOp& op, int slot, const double* src, const double* sub, long rows, int K, long ld,
+                              int kc) {
+  ACES_CHECK(lowp_slot(slot, rows, K));
+  LowP& L = lp_[slot];
+  if (kc)
+    ACES_CHECK(split_bf16((int)rows, K, src, ld, (unsigned short*)L.hi, (unsigned short*)L.lo, L.ld, s_, sub));
+  else
+    ACES_CHECK(split_bf16_t((int)rows, K, src, ld, (unsigned short*)L.hi, (unsigned short*)L.lo, L.ld, s_, sub));
+  lp_fresh_[slot] = false;  // holds a difference, not the operand
+  op.hi = L.hi;
+  op.lo = L.lo;
+  op.ld = L.ld;
+  return ACES_OK;
+}
+
+int CcsdEngine::lp_product(int M, int N, int K, const LowPOp& A, const LowPOp& B, double* C, long ldc, double alpha,
+                           double beta) {
+  GemmDesc d;
+  d.M = M; d.N = N; d.K = (int)pad32(std::max(K, 1));
+  d.A = A.hi; d.A_lo = A.lo; d.lda = A.ld; d.a_kc = 1;
+  d.B = B.hi; d.B_lo = B.lo; d.ldb = B.ld; d.b_kc = 1;
+  d.C = C; d.ldc = ldc; d.alpha = alpha; d.beta = beta;
+  d.ws = ws_;
+  d.ws_elems = ws_elems_;
+  return timed_gemm(d, GP_BF16X3, 2.0 * M * (double)N * K);
+}
+
+int CcsdEngine::gemm_anchored(int M, int N, int K, const double* A64, int aslot, long lda, const double* B64,
+                              int bslot, long ldb, double* C, long ldc, double alpha, double beta, int a_kc,
+                              int b_kc) {
+  const bool sa = lp_static_slot(aslot), sb = lp_static_slot(bslot);
+  ACES_REQUIRE(aslot >= 0 && aslot < kLpSlots && bslot >= 0 && bslot < kLpSlots, "gemm_anchored: bad operand slot");
+  if ((sa && sb) || ldc != N || M == 0 || N == 0 || K == 0)
+    return gemm_lp_impl(GP_F64, M, N, K, A64, aslot, lda, B64, bslot, ldb, C, ldc, alpha, beta, a_kc, b_kc, -1);
+  const int site = anc_site_++;
+  if (anc_mode_ == 2)
+    ACES_REQUIRE(site < (int)anc_sites_.size() && anc_sites_[site].elems == (long)M * N,
+                 "refinement: products differ from the anchor's");
+  if (site >= (int)anc_sites_.size()) anc_sites_.resize(site + 1);
+  AncSite& S = anc_sites_[site];
+  // extents of the operands as stored: rows x K (kc) or K x rows (k-major)
+  const long ea = a_kc ? (M - 1) * lda + K : (K - 1) * lda + M;
+  const long eb = b_kc ? (N - 1) * ldb + K : (K - 1) * ldb + N;
+  const double *A0 = nullptr, *B0 = nullptr;
+  if (anc_mode_ == 1) {
+    if (S.C0 == nullptr || S.elems != (long)M * N) {
+      ACES_CHECK(alloc(&S.C0, (long)M * N));
+      S.elems = (long)M * N;
+    }
+    ACES_CHECK(gemm_lp_impl(GP_F64, M, N, K, A64, aslot, lda, B64, bslot, ldb, S.C0, N, 1.0, 0.0, a_kc, b_kc, -1));
+  }
+  if (!sa) ACES_CHECK(anchor_copy(A64, ea, &A0));
+  if (!sb) ACES_CHECK(anchor_copy(B64, eb, &B0));
+  if (!sa && !sb && anc_mode_ == 1) {  // the split of B0 stays fixed while the anchor lives
+    const long Kp = pad32(K);
+    if (S.B0.hi == nullptr || S.B0.ld != Kp || S.B0.rows != N) {
+      void* blk;
+      ACES_CHECK(alloc_raw(&blk, (long)N * Kp * 4));
+      S.B0.hi = blk;
+      S.B0.lo = (char*)blk + (long)N * Kp * 2;
+      S.B0.ld = Kp;
+      S.B0.rows = N;
+    }
+    if (b_kc)
+      ACES_CHECK(split_bf16(N, K, B64, ldb, (unsigned short*)S.B0.hi, (unsigned short*)S.B0.lo, Kp, s_));
+    else
+      ACES_CHECK(split_bf16_t(N, K, B64, ldb, (unsigned short*)S.B0.hi, (unsigned short*)S.B0.lo, Kp, s_));
+  }
+  // C = beta C + alpha C0
+  ACES_CHECK(P4(1, 1, M, N, C, beta).src(S.C0, alpha, 0, 0, N, 1).run(s_));
+  if (anc_mode_ == 1) return ACES_OK;
+  ACES_CHECK(lowp_static(GP_BF16X3));
+  LowPOp a, b;
+  if (sa || sb) {  // static integral operand: one product with the other's difference
+    if (sa) {
+      a = LowPOp{lp_[aslot].hi, lp_[aslot].lo, lp_[aslot].ld};
+      ACES_CHECK(split_operand(b, bslot, B64, B0, N, K, ldb, b_kc));
+    } else {
+      ACES_CHECK(split_operand(a, aslot, A64, A0, M, K, lda, a_kc));
+      b = LowPOp{lp_[bslot].hi, lp_[bslot].lo, lp_[bslot].ld};
+    }
+    return lp_product(M, N, K, a, b, C, ldc, alpha, 1.0);
+  }
+  // A B - A0 B0 = A (B - B0) + (A - A0) B0
+  ACES_CHECK(split_operand(a, aslot, A64, nullptr, M, K, lda, a_kc));
+  ACES_CHECK(split_operand(b, bslot, B64, B0, N, K, ldb, b_kc));
+  ACES_CHECK(lp_product(M, N, K, a, b, C, ldc, alpha, 1.0));
+  ACES_CHECK(split_operand(a, aslot, A64, A0, M, K, lda, a_kc));
+  b = LowPOp{S.B0.hi, S.B0.lo, S.B0.ld};
+  return lp_product(M, N, K, a, b, C, ldc, alpha, 1.0);
 }
 
 // Launch one MFMA contraction, timed with hipEvents when profiling is on.
@@ -437,6 +559,20 @@ int CcsdEngine::gemm_lp_impl(GemmPrec prec, int M, int N, int K, const double* A
 
 // ---------------------------------------------------------------------------
 int CcsdEngine::compute_increments(GemmPrec prec) {
+  if ((int)prec == kPrecRefine) {
+    // the fp64 evaluation with the large products anchored (recorded on the
+    // first such evaluation for these amplitudes, refined afterwards)
+    anc_mode_ = anc_valid_ ? 2 : 1;
+    anc_site_ = 0;
+    const int rc = compute_increments(GP_F64);
+    const int mode = anc_mode_;
+    anc_mode_ = 0;
+    ACES_CHECK(rc);
+    if (mode == 1) anc_valid_ = true;
+    fwd_valid_ = (mode == 1);
+    return ACES_OK;
+  }
+  ACES_REQUIRE(prec == GP_F64 || prec == GP_F32 || prec == GP_BF16X3, "ccsd: unknown precision");
   hipStream_t s = s_;
   const long ov = (long)o * v, oo = (long)o * o, vv = (long)v * v;
   const long ov2 = ov * v, o2v = oo * v, vo2 = v * oo, v2o = vv * o;

@@ -652,6 +652,7 @@ int SymEngine::outer0(int A, double* Ad, const double* u, const double* w, doubl
 
 // ---------------------------------------------------------------------------
 int SymEngine::forward(GemmPrec prec) {
+  ACES_REQUIRE(prec == GP_F64 || prec == GP_F32 || prec == GP_BF16X3, "sym: unknown precision");
   // amplitude-dependent tensors change between passes; within a pass a split
   // is reused until the tensor is written again
   for (Tensor& t : T_)

@@ -7,13 +7,14 @@ namespace aces {
 // y[r*ldy + c] = (float) x[r*ldx + c] for c < cols; columns cols..ldy-1 zeroed.
 int convert_f32(int rows, int cols, const double* x, long ldx, float* y, long ldy, hipStream_t s);
 // hi = bf16(x), lo = bf16(x - hi) (round to nearest even); padding columns zeroed.
+// With x0 the split is of the difference x - x0 (x0 laid out like x).
 int split_bf16(int rows, int cols, const double* x, long ldx, unsigned short* hi, unsigned short* lo,
-               long ldy, hipStream_t s);
+               long ldy, hipStream_t s, const double* x0 = nullptr);
 // The same conversions for an operand stored k-major, x[c*ldx + r] = element
 // (r, c): the output is the transposed, k-contiguous y[r*ldy + c].
 int convert_f32_t(int rows, int cols, const double* x, long ldx, float* y, long ldy, hipStream_t s);
 int split_bf16_t(int rows, int cols, const double* x, long ldx, unsigned short* hi, unsigned short* lo, long ldy,
-                 hipStream_t s);
+                 hipStream_t s, const double* x0 = nullptr);
 
 // Generic 4-index permute/combine on a row-major output of dims d[0..3]
 // (up to kP4Src sources):

@@ -24,13 +24,14 @@ __global__ void convert_f32_kernel(int rows, int cols, const double* __restrict_
 }
 
 __global__ void split_bf16_kernel(int rows, int cols, const double* __restrict__ x, long ldx,
-                                  unsigned short* __restrict__ hi, unsigned short* __restrict__ lo, long ldy) {
+                                  unsigned short* __restrict__ hi, unsigned short* __restrict__ lo, long ldy,
+                                  const double* __restrict__ x0) {
   const long total = (long)rows * ldy;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
     const long r = i / ldy, c = i - r * ldy;
     unsigned short h = 0, l = 0;
     if (c < cols) {
-      const double v = x[r * ldx + c];
+      const double v = x0 ? x[r * ldx + c] - x0[r * ldx + c] : x[r * ldx + c];
       h = bf16_rn((float)v);
       const double rem = v - (double)bf16_to_f(h);
       l = bf16_rn((float)rem);
@@ -50,12 +51,12 @@ int convert_f32(int rows, int cols, const double* x, long ldx, float* y, long ld
 }
 
 int split_bf16(int rows, int cols, const double* x, long ldx, unsigned short* hi, unsigned short* lo, long ldy,
-               hipStream_t s) {
+               hipStream_t s, const double* x0) {
   ACES_REQUIRE(ldy >= cols && ldx >= cols, "split_bf16: bad leading dimension");
   const long n = (long)rows * ldy;
   if (n == 0) return ACES_OK;
   hipLaunchKernelGGL(split_bf16_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, rows, cols, x, ldx, hi, lo,
-                     ldy);
+                     ldy, x0);
   ACES_HIP(hipGetLastError());
   return ACES_OK;
 }
@@ -65,7 +66,8 @@ int split_bf16(int rows, int cols, const double* x, long ldx, unsigned short* hi
 // zero).  64 x 64 tiles pass through LDS so that both sides are coalesced.
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void convert_t_kernel(int rows, int cols, const double* __restrict__ x, long ldx,
-                                                        void* __restrict__ y0, void* __restrict__ y1, long ldy) {
+                                                        void* __restrict__ y0, void* __restrict__ y1, long ldy,
+                                                        const double* __restrict__ x0) {
   __shared__ double tile[64][65];
   const long ntr = (rows + 63) / 64;
   const long r0 = (blockIdx.x % ntr) * 64;
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(256) void convert_t_kernel(int rows, int cols, cons
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const long c = c0 + w + 4 * k, r = r0 + l;
-    tile[w + 4 * k][l] = (c < cols && r < rows) ? x[c * ldx + r] : 0.0;
+    tile[w + 4 * k][l] = (c < cols && r < rows) ? (x0 ? x[c * ldx + r] - x0[c * ldx + r] : x[c * ldx + r]) : 0.0;
   }
   __syncthreads();
 #pragma unroll
@@ -94,17 +96,17 @@ __global__ __launch_bounds__(256) void convert_t_kernel(int rows, int cols, cons
 }
 
 static int convert_t_launch(bool split, int rows, int cols, const double* x, long ldx, void* y0, void* y1, long ldy,
-                            hipStream_t s) {
+                            hipStream_t s, const double* x0 = nullptr) {
   ACES_REQUIRE(ldy >= cols && ldx >= rows, "transposing conversion: bad leading dimension");
   if (rows == 0 || ldy == 0) return ACES_OK;
   const long nblk = ((rows + 63) / 64) * ((ldy + 63) / 64);
   ACES_REQUIRE(nblk < (1L << 31), "transposing conversion: too many tiles");
   if (split)
     hipLaunchKernelGGL(convert_t_kernel<true>, dim3((unsigned)nblk), dim3(256), 0, s, rows, cols, x, ldx, y0, y1,
-                       ldy);
+                       ldy, x0);
   else
     hipLaunchKernelGGL(convert_t_kernel<false>, dim3((unsigned)nblk), dim3(256), 0, s, rows, cols, x, ldx, y0, y1,
-                       ldy);
+                       ldy, x0);
   ACES_HIP(hipGetLastError());
   return ACES_OK;
 }
@@ -114,8 +116,8 @@ int convert_f32_t(int rows, int cols, const double* x, long ldx, float* y, long 
 }
 
 int split_bf16_t(int rows, int cols, const double* x, long ldx, unsigned short* hi, unsigned short* lo, long ldy,
-                 hipStream_t s) {
-  return convert_t_launch(true, rows, cols, x, ldx, hi, lo, ldy, s);
+                 hipStream_t s, const double* x0) {
+  return convert_t_launch(true, rows, cols, x, ldx, hi, lo, ldy, s, x0);
 }
 
 // ---------------------------------------------------------------------------

@@ -294,7 +294,8 @@ int aces_getrec(int iflag, const char* name, long* nwords, void* dest) {
 int aces_vcc_set_precision(int prec) {
   AcesContext* c;
   ACES_CHECK(ctx(&c));
-  ACES_REQUIRE(prec >= 0 && prec <= 2, "precision must be 0 (fp64), 1 (fp32) or 2 (bf16x3)");
+  ACES_REQUIRE(prec >= 0 && prec <= 3,
+               "precision must be 0 (fp64), 1 (fp32), 2 (bf16x3) or 3 (bf16x3 refinement of an fp64 anchor)");
   c->prec = prec;
   return ACES_OK;
 }

@@ -238,7 +238,7 @@ def main():
 
     # ---- convergence study (untimed for `value`) ----
     if not args.no_convergence:
-        def run(form, conv, polish_conv=None):
+        def run(form, conv, polish_conv=None, polish='refine'):
             cc.set_diis_order(6)
             cc.reset_diis()
             cc.start_mp1()
@@ -252,13 +252,15 @@ def main():
                 times.append((phase, time.perf_counter() - ta))
                 hist.append((phase, st['energy'], st['resmiter']))
                 if st['converged']:
-                    if phase != 'fp64' and polish_conv is not None:
-                        # iterative refinement: continue with fp64 cycles to the tight
-                        # threshold, DIIS restarted (the low-precision error vectors
-                        # would hold the extrapolation at their accuracy)
-                        phase = 'fp64'
+                    if phase == form and polish_conv is not None:
+                        # iterative refinement: continue with `polish` cycles to the
+                        # tight threshold, DIIS restarted (the low-precision error
+                        # vectors would hold the extrapolation at their accuracy);
+                        # 'refine' anchors its difference products here
+                        phase = polish
                         tol = 10.0 ** (-polish_conv)
                         cc.reset_diis()
+                        cc.reset_anchor()
                         ic0 = icycle
                         continue
                     e_final = st['final_energy']
@@ -267,22 +269,32 @@ def main():
 
         e64, times64, hist64, n64 = run('fp64', 9)
         log('fp64: E=%r in %d cycles' % (e64, n64))
-        emx, timesmx, histmx, nmx = run(args.prec, 7, polish_conv=9) if args.prec != 'fp64' else (e64, times64, hist64, n64)
-        log('%s+polish: E=%r in %d cycles' % (args.prec, emx, nmx))
         med = lambda xs: float(np.median(xs)) if xs else None
         s64 = med([t for p, t in times64[2:] if p == 'fp64'])
-        slo = med([t for p, t in timesmx[2:] if p == args.prec])
         result['convergence'] = {
             'fp64': {'energy': e64, 'cycles': n64, 's_per_cycle_median': s64,
-                     'total_s': sum(t for _, t in times64)},
-            'mixed': {'energy': emx, 'cycles': nmx,
-                      'lowp_cycles': sum(1 for p, _ in timesmx if p != 'fp64'),
-                      'polish_cycles': sum(1 for p, _ in timesmx if p == 'fp64'),
-                      's_per_cycle_median_lowp': slo, 'total_s': sum(t for _, t in timesmx)},
-            'abs_dE': abs(emx - e64) if (emx is not None and e64 is not None) else None,
-            'speedup_per_cycle': (s64 / slo) if (s64 and slo) else None,
-            'speedup_total': (sum(t for _, t in times64) / sum(t for _, t in timesmx)) if timesmx else None,
-        }
+                     'total_s': sum(t for _, t in times64)}}
+        if args.prec != 'fp64':
+            # headline: refinement polish; beside it the plain fp64 polish
+            for key, pol in (('mixed', 'refine'), ('mixed_fp64_polish', 'fp64')):
+                emx, timesmx, histmx, nmx = run(args.prec, 7, polish_conv=9, polish=pol)
+                log('%s+%s polish: E=%r in %d cycles' % (args.prec, pol, emx, nmx))
+                slo = med([t for p, t in timesmx[2:] if p == args.prec])
+                spol = med([t for p, t in timesmx[1:] if p == pol])
+                result['convergence'][key] = {
+                    'energy': emx, 'cycles': nmx, 'polish': pol,
+                    'lowp_cycles': sum(1 for p, _ in timesmx if p == args.prec),
+                    'polish_cycles': sum(1 for p, _ in timesmx if p == pol),
+                    's_per_cycle_median_lowp': slo, 's_per_cycle_median_polish': spol,
+                    'total_s': sum(t for _, t in timesmx),
+                    'abs_dE': abs(emx - e64) if (emx is not None and e64 is not None) else None,
+                    'speedup_total': sum(t for _, t in times64) / sum(t for _, t in timesmx)}
+            m = result['convergence']['mixed']
+            result['convergence'].update({
+                'abs_dE': m['abs_dE'],
+                'speedup_per_cycle': (s64 / m['s_per_cycle_median_lowp'])
+                if (s64 and m['s_per_cycle_median_lowp']) else None,
+                'speedup_total': m['speedup_total']})
         result['fp64_s_per_iter'] = s64
 
     if not args.no_cpu:

@@ -70,6 +70,12 @@ int aces_split_bf16(int rows, int cols, const double* d_x, long ldx, unsigned sh
  *   list233 = the reference's compressed RHF <Ab|Cd> list (rows A<=B, columns
  *   Cd, column major; intprc/intproc.F:250-257).
  * Amplitudes: t1[i][a], t2[i][j][a][b] = T(Ab,Ij) (list 46 storage for C1).
+ * prec: 0 fp64, 1 fp32 and 2 bf16x3 products (fp32 accumulation) for the
+ * ladders, W(mbej) build, ring and read-bound T1 products; 3 refinement: an
+ * fp64 cycle whose ladder, W build and ring products run as bf16x3 products of
+ * the operands' differences to an fp64 anchor evaluation (the first prec-3
+ * cycle after the amplitudes were set records the anchor), so that their error
+ * scales with the distance to the anchor; the reference has fp64 only.
  * stats (host double[7]) from aces_ccsd_cycle: E(T_new) printed by CMPENG,
  * RESMITER, RESMDIIS, converged flag, final energy (valid when converged),
  * ms in the increments, ms in the whole cycle.
@@ -83,6 +89,9 @@ int aces_ccsd_get_amplitudes(void* handle, double* d_t1, double* d_t2);
 int aces_ccsd_start_mp1(void* handle);
 int aces_ccsd_increments(void* handle, int prec, double* d_inc1, double* d_inc2);
 int aces_ccsd_cycle(void* handle, int icycle, int prec, double tol, double* stats);
+/* No counterpart in the reference (fp64 only): drop the prec-3 anchor, so that
+ * the next prec-3 cycle records a new one at the current amplitudes. */
+int aces_ccsd_reset_anchor(void* handle);
 int aces_ccsd_set_diis_order(void* handle, int order);
 int aces_ccsd_reset_diis(void* handle);
 int aces_ccsd_energy(void* handle, const double* d_t1, const double* d_t2, double* e_out);

@@ -98,9 +98,70 @@ def test_mixed_then_fp64_polish(o, v, seed):
     blk, eri = _system(o, v, seed, True)
     cc = CCSD(o, v, blk)
     h64 = cc.run(maxcyc=80, conv=10, order=6, start='mp1')
-    hmx = cc.run(maxcyc=80, conv=7, order=6, start='mp1', prec='bf16x3', polish_conv=10)
+    hmx = cc.run(maxcyc=80, conv=7, order=6, start='mp1', prec='bf16x3', polish_conv=10, polish='fp64')
     assert h64['converged'] and hmx['converged']
     assert hmx['precision'][-1] == 'fp64' and hmx['precision'][0] == 'bf16x3'
     assert abs(hmx['final_energy'] - h64['final_energy']) < 1e-9
     assert hmx['precision'].count('fp64') < h64['cycles']
+    cc.close()
+
+
+@pytest.mark.parametrize('o,v,seed', [(6, 30, 24), (7, 45, 25)])
+def test_refine_polish(o, v, seed):
+    """bf16x3 cycles to 1e-6, then refinement cycles (bf16x3 products of the
+    operands' differences to the fp64 anchor taken at the switch) to 1e-10: the
+    fp64 loop's energy, and at the final amplitudes the refined increments
+    agree with the fp64 ones far inside the plain bf16x3 error."""
+    from aces2.ccsd import CCSD
+    print('case refine', o, v, seed, flush=True)
+    blk, eri = _system(o, v, seed, True)
+    cc = CCSD(o, v, blk)
+    h64 = cc.run(maxcyc=80, conv=10, order=6, start='mp1')
+    hr = cc.run(maxcyc=80, conv=6, order=6, start='mp1', prec='bf16x3', polish_conv=10, polish='refine')
+    assert h64['converged'] and hr['converged']
+    assert hr['precision'][0] == 'bf16x3' and hr['precision'][-1] == 'refine'
+    # both loops stop at |RESMITER| < 1e-10, which leaves the energy within ~1e-10 of its limit
+    assert abs(hr['final_energy'] - h64['final_energy']) < 1e-9
+    assert hr['precision'].count('refine') < h64['cycles']
+    # increments at the converged amplitudes (the anchor is the switch point)
+    i1r, i2r = cc.increments('refine')
+    i1d, i2d = cc.increments('fp64')
+    i1b, i2b = cc.increments('bf16x3')
+    err_r = max(float((i1r - i1d).abs().max()), float((i2r - i2d).abs().max()))
+    err_b = max(float((i1b - i1d).abs().max()), float((i2b - i2d).abs().max()))
+    print('refine err %.3e bf16x3 err %.3e' % (err_r, err_b), flush=True)
+    # the refined error scales with the distance to the anchor (switch at 1e-6):
+    # bf16x3's relative error times that distance, far below the 1e-10 threshold
+    assert err_b > 0 and err_r < 1e-3 * err_b and err_r < 1e-11
+    cc.close()
+
+
+def test_refine_anchor_cycle_is_fp64():
+    """The first refinement evaluation after the amplitudes are set records the
+    anchor and equals the fp64 evaluation; near convergence, a later one at
+    other amplitudes (reached by refinement cycles) has a small fraction of
+    plain bf16x3's error (the difference operands are small)."""
+    from aces2.ccsd import CCSD
+    o, v, seed = 5, 19, 26
+    print('case refine anchor', o, v, seed, flush=True)
+    blk, eri = _system(o, v, seed, True)
+    cc = CCSD(o, v, blk)
+    t1, t2 = _random_t(o, v, seed + 7, scale=0.02)
+    cc.set_amplitudes(torch.tensor(t1), torch.tensor(t2))
+    i1a, i2a = cc.increments('refine')
+    i1d, i2d = cc.increments('fp64')
+    assert float((i2a - i2d).abs().max()) < 1e-13 and float((i1a - i1d).abs().max()) < 1e-13
+    # a new anchor near the solution, then three refinement cycles
+    cc.run(maxcyc=60, conv=4, order=6, start='current')
+    cc.reset_anchor()
+    cc.reset_diis()
+    for ic in range(1, 4):
+        cc.cycle(ic, 'refine', 1e-30)
+    i1r, i2r = cc.increments('refine')
+    i1d, i2d = cc.increments('fp64')
+    i1b, i2b = cc.increments('bf16x3')
+    err_r = float((i2r - i2d).abs().max())
+    err_b = float((i2b - i2d).abs().max())
+    print('refine err %.3e bf16x3 err %.3e' % (err_r, err_b), flush=True)
+    assert err_r < 0.05 * err_b
     cc.close()
