@@ -8,10 +8,12 @@ and are resident in HBM before timing starts.
 
 Headline (``value``): seconds per cycle of the mixed-precision form (ladders,
 W(mbej) build, ring and the read-bound T1-dressing contractions as bf16x3 split
-products on MFMA with fp32 accumulation; the rest fp64).  Outside the timed region the convergence
-study runs both forms from the MP1 start: fp64 to |RESMITER| < 1e-9, mixed to
-1e-7 followed by fp64 polish to 1e-9 (DIIS restarted at the switch), and reports |dE|, the fp64 seconds per
-cycle and the speedup.
+products on MFMA with fp32 accumulation; the rest fp64).  Outside the timed
+region the convergence study runs from the MP1 start: fp64 to |RESMITER| < 1e-9,
+and mixed to 1e-7 followed (DIIS restarted at the switch) by a polish to 1e-9,
+once with refinement cycles (the large products as bf16x3 products of the
+operands' differences to an fp64 anchor taken at the switch) and once with fp64
+cycles; it reports |dE|, the seconds per cycle of each form and the speedups.
 
 With ``--workload lambda`` the step is one cycle of the Lambda equations (the
 xlambda loop body: the engine's reverse pass, Jacobi step, Lambda DIIS) at the
