@@ -21,7 +21,7 @@ converged fp64 amplitudes of o=32, v=256; the same run checks dE/dmu from the
 response density and from the Lambda-weighted increments against central
 finite differences (h = 1e-4) for a Fock perturbation F + mu P and a rank-1
 integral perturbation (pq|rs) + mu x_pq x_rs, at fp64 amplitudes and at the
-amplitudes of the mixed form after its fp64 polish.
+amplitudes of the mixed form after its refinement polish.
 
 With ``--workload sym`` the system is the symmetry-blocked form: D2h (8
 irreps), o=64, v=384 split over the irreps by a seeded multinomial (at least
@@ -383,15 +383,17 @@ def bench_lambda(args):
         if start == 'mp1':
             c.start_mp1()
         tol = 10.0 ** (-(7 if prec != 'fp64' else conv))
-        phase, icycle, ic0, ncyc = prec, 0, 0, {'lowp': 0, 'fp64': 0}
+        phase, icycle, ic0, ncyc = prec, 0, 0, {'lowp': 0, 'refine': 0, 'fp64': 0}
         while icycle < args.maxcyc:
             icycle += 1
             st = c.cycle(icycle - ic0, phase, tol)
-            ncyc['fp64' if phase == 'fp64' else 'lowp'] += 1
+            ncyc[phase if phase in ('fp64', 'refine') else 'lowp'] += 1
             if st['converged']:
-                if phase != 'fp64':
-                    phase, tol = 'fp64', 10.0 ** (-conv)
+                if phase == prec and prec != 'fp64':
+                    # polish with refinement cycles anchored here (DIIS restarted)
+                    phase, tol = 'refine', 10.0 ** (-conv)
                     c.reset_diis()
+                    c.reset_anchor()
                     ic0 = icycle
                     continue
                 return st['final_energy'], ncyc
@@ -476,7 +478,7 @@ def bench_lambda(args):
     cc.set_fock(F0)
     fd_f = (e_f[0] - e_f[1]) / (2 * hh)
     fd_f2 = (e_f[2] - e_f[3]) / (4 * hh)
-    # mixed form: bf16x3 to 1e-7, fp64 polish (DIIS restarted) to 1e-10, Lambda in fp64 at those amplitudes
+    # mixed form: bf16x3 to 1e-7, refinement polish (DIIS restarted) to 1e-10, Lambda in fp64 at those amplitudes
     Em, ncm = converge(cc, prec=args.prec if args.prec != 'fp64' else 'bf16x3')
     Tm = cc.get_amplitudes()
     cc.set_lambda_diis_order(6)
