@@ -39,6 +39,7 @@ SIGNATURES = {
     'aces_ccsd_set_diis_order': (c_int, [c_vp, c_int]),
     'aces_ccsd_reset_diis': (c_int, [c_vp]),
     'aces_ccsd_reset_anchor': (c_int, [c_vp]),
+    'aces_ccsd_set_lambda_precision': (c_int, [c_vp, c_int]),
     'aces_ccsd_energy': (c_int, [c_vp, c_vp, c_vp, c_dp]),
     'aces_ccsd_device_bytes': (c_long, [c_vp]),
     'aces_ccsd_set_profiling': (c_int, [c_vp, c_int]),

@@ -93,9 +93,9 @@ class CCSD:
         _lib.call(self._p + 'set_profiling', self.h, 1 if on else 0)
 
     def profile(self, reset=True):
-        out = (ctypes.c_double * 15)()
+        out = (ctypes.c_double * 18)()
         _lib.call(self._p + 'profile', self.h, out, 1 if reset else 0)
-        names = ('ladder', 'wbuild', 'ring', 'other', 'adjoint')
+        names = ('ladder', 'wbuild', 'ring', 'other', 'adjoint', 'adjoint_lp')
         return {nm: {'ms': out[3 * c], 'flops': out[3 * c + 1], 'launches': int(out[3 * c + 2])}
                 for c, nm in enumerate(names)}
 
@@ -188,6 +188,10 @@ class CCSD:
     def lambda_start(self):
         _lib.call(self._p + 'lambda_start', self.h)
 
+    def set_lambda_precision(self, prec):
+        """'fp64' or 'refine' Lambda cycles (C1 engine)."""
+        _lib.call('aces_ccsd_set_lambda_precision', self.h, PREC[prec])
+
     def set_lambda_diis_order(self, order):
         _lib.call(self._p + 'set_lambda_diis_order', self.h, order)
 
@@ -218,10 +222,14 @@ class CCSD:
         _lib.call(self._p + 'density', self.h, d.data_ptr())
         return d
 
-    def solve_lambda(self, maxcyc=50, conv=7, order=6, log=None, raise_on_fail=True):
-        """xlambda's loop (``lambda/vlamcc.F:599-830``) at the engine's amplitudes."""
+    def solve_lambda(self, maxcyc=50, conv=7, order=6, log=None, raise_on_fail=True, prec='fp64'):
+        """xlambda's loop (``lambda/vlamcc.F:599-830``) at the engine's amplitudes.
+        ``prec='refine'`` (C1 engine): residuals from an fp64 anchor plus the
+        linear part on the difference with its large products in bf16x3."""
         self.lambda_start()
         self.set_lambda_diis_order(order)
+        if prec != 'fp64' or self._p == 'aces_ccsd_':
+            self.set_lambda_precision(prec)
         self.lambda_start()
         hist = {'resmiter': [], 'resmdiis': [], 'ms_cycle': [], 'converged': False}
         tol = 10.0 ** (-conv)

@@ -12,8 +12,11 @@ int ladder_unpack_adjoint(int o, int v, const double* g2, const double* gYb, dou
 // Adjoint of pack_tau: gtau[i][j][c][d] += contributions of gSp, gDp.
 int pack_tau_adjoint(int o, int v, const double* gSp, long lds, const double* gDp, long ldd, double* gtau,
                      hipStream_t s);
+// X[p][j] *= sc (cols = 0) or X[j][p] *= sc (cols = 1) for the diagonal pairs
+// p = (c, c), c < v, of a packed a <= b index; j < n.
+int scale_diag_pairs(int v, double* X, long ld, long n, int cols, double sc, hipStream_t s);
 // Seeds of the reverse pass for L = E + l.(inc - D*T):
-//   g1 = 2 f_ia - D_ia l1,  g2 = -D_ijab l2.
+//   g1 = 2 f_ia - D_ia l1,  g2 = -D_ijab l2  (fov == nullptr: without the energy's 2 f_ia).
 int adjoint_seed(int o, int v, const double* eo, const double* ev, const double* fov, long ldf, const double* l1,
                  const double* l2, double* g1, double* g2, hipStream_t s);
 // Orbital-energy derivatives of -l.(D*T):  geo[i] += dL/de_i, gev[a] += dL/de_a.

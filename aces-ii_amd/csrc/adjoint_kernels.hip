@@ -114,7 +114,7 @@ __global__ void adjoint_seed_kernel(int o, int v, const double* __restrict__ eo,
        idx += (long)gridDim.x * blockDim.x) {
     if (idx < n1) {
       const int i = (int)(idx / v), a = (int)(idx - (idx / v) * v);
-      g1[idx] = 2.0 * fov[(long)i * ldf + a] - (eo[i] - ev[a]) * l1[idx];
+      g1[idx] = (fov ? 2.0 * fov[(long)i * ldf + a] : 0.0) - (eo[i] - ev[a]) * l1[idx];
     } else {
       const long r = idx - n1;
       const int b = (int)(r % v);
@@ -124,6 +124,26 @@ __global__ void adjoint_seed_kernel(int o, int v, const double* __restrict__ eo,
       g2[r] = -(eo[i] + eo[j] - ev[a] - ev[b]) * l2[r];
     }
   }
+}
+
+// X[p][j] *= s (rows) or X[j][p] *= s (cols) for the diagonal pairs p = (c, c)
+// of a packed a <= b index, c < v, j < n.
+__global__ void scale_diag_pairs_kernel(int v, double* __restrict__ X, long ld, long n, int cols, double sc) {
+  const long total = (long)v * n;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx / n);
+    const long j = idx - (long)c * n;
+    const long p = pk_le(c, c);
+    X[cols ? j * ld + p : p * ld + j] *= sc;
+  }
+}
+
+int scale_diag_pairs(int v, double* X, long ld, long n, int cols, double sc, hipStream_t s) {
+  const long total = (long)v * n;
+  if (total == 0) return ACES_OK;
+  hipLaunchKernelGGL(scale_diag_pairs_kernel, dim3(grid1(total)), dim3(256), 0, s, v, X, ld, n, cols, sc);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
 }
 
 int adjoint_seed(int o, int v, const double* eo, const double* ev, const double* fov, long ldf, const double* l1,

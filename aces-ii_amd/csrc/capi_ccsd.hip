@@ -144,6 +144,11 @@ int aces_ccsd_lambda_cycle(void* h, int icycle, double tol, double* stats) {
   return rc;
 }
 
+int aces_ccsd_set_lambda_precision(void* h, int prec) {
+  REQUIRE_HANDLE(h);
+  return eng(h)->set_lambda_precision(prec);
+}
+
 int aces_ccsd_set_lambda_diis_order(void* h, int order) {
   REQUIRE_HANDLE(h);
   return eng(h)->set_lambda_diis_order(order);

@@ -31,6 +31,7 @@ namespace aces {
 
 using eu::gd;
 using eu::P4;
+using eu::pad32;
 
 int CcsdEngine::adjoint_alloc() {
   if (adj_ready_) return ACES_OK;
@@ -76,7 +77,50 @@ int CcsdEngine::adjoint_alloc() {
 // yet symmetrised) and, when density is set, gfo_/gfv_/gfov_/geo_/gev_ hold
 // dL/d(f_oo, f_vv, f_ov, e_o, e_v).  Operation numbers refer to the forward
 // order in compute_increments().
-int CcsdEngine::compute_adjoint(const double* lam, bool density) {
+// The fixed right operands of the mixed reverse products, split once per
+// forward state: U^T, Wt^T, X^T, Wv^T (rows = the product's output columns).
+int CcsdEngine::lambda_lowp_operands() {
+  if (lam_lp_valid_) return ACES_OK;
+  ACES_CHECK(lowp_static(GP_BF16X3));
+  const long ov = (long)o * v, Kp = pad32(ov);
+  // {source, rows (the product's N), source leading dimension}; all have K = ov
+  const long ldm = std::max<long>(nvm_, 1);
+  struct S { const double* src; long rows; long ld; };
+  const S tab[6] = {{U_, ov, ov}, {Wt_, ov, ov}, {X_, ov, ov}, {Wv_, ov, ov},
+                    {Vp_ + nvp_ * nvp_, nvp_, nvp_}, {Vm_ + nvm_ * ldm, nvm_, ldm}};
+  for (int q = 0; q < 6; ++q) {
+    LowP& L = lam_lp_[q];
+    if (tab[q].rows == 0) continue;
+    if (L.hi == nullptr) {
+      void* blk;
+      ACES_CHECK(alloc_raw(&blk, tab[q].rows * Kp * 4));
+      L.hi = blk;
+      L.lo = (char*)blk + tab[q].rows * Kp * 2;
+      L.ld = Kp;
+      L.rows = tab[q].rows;
+    }
+    ACES_CHECK(split_bf16_t((int)tab[q].rows, (int)ov, tab[q].src, tab[q].ld, (unsigned short*)L.hi,
+                            (unsigned short*)L.lo, Kp, s_));
+  }
+  lam_lp_valid_ = true;
+  return ACES_OK;
+}
+
+int CcsdEngine::adj_gemm_lp(int M, int N, int K, const double* A, long lda, int akc, int aslot, const LowPOp& B,
+                            double* C, long ldc, double alpha, double beta) {
+  LowPOp a;
+  ACES_CHECK(split_operand(a, aslot, A, nullptr, M, K, lda, akc));
+  const int cat = prof_cat_;
+  prof_cat_ = 5;
+  const int rc = lp_product(M, N, K, a, B, C, ldc, alpha, beta);
+  prof_cat_ = cat;
+  return rc;
+}
+
+int CcsdEngine::compute_adjoint(const double* lam, bool density, GemmPrec prec, bool lin) {
+  ACES_REQUIRE(prec == GP_F64 || (prec == GP_BF16X3 && !density), "adjoint: bf16x3 only without the density");
+  const bool lp = (prec == GP_BF16X3);
+  if (lp) ACES_CHECK(lambda_lowp_operands());
   hipStream_t s = s_;
   const long ov = (long)o * v, oo = (long)o * o, vv = (long)v * v;
   const long ov2 = ov * v, o2v = oo * v, vo2 = v * oo, v2o = vv * o, o3 = oo * o;
@@ -91,9 +135,12 @@ int CcsdEngine::compute_adjoint(const double* lam, bool density) {
   const double* fov = fock_ + o;
 
   prof_cat_ = 4;
-  // seeds: energy (2 f_ia t_ia + L_ijab tau_ijab) and -D*T
-  ACES_CHECK(adjoint_seed(o, v, eo_, ev_, fov, n, l1, l2, gt1, gt2, s));
-  ACES_HIP(hipMemcpyAsync(gtau_, Lijab_, sizeof(double) * ovvo, hipMemcpyDeviceToDevice, s));
+  // seeds: energy (2 f_ia t_ia + L_ijab tau_ijab) and -D*T (lin: -D*T only)
+  ACES_CHECK(adjoint_seed(o, v, eo_, ev_, lin ? nullptr : fov, n, l1, l2, gt1, gt2, s));
+  if (lin)
+    ACES_HIP(hipMemsetAsync(gtau_, 0, sizeof(double) * ovvo, s));
+  else
+    ACES_HIP(hipMemcpyAsync(gtau_, Lijab_, sizeof(double) * ovvo, hipMemcpyDeviceToDevice, s));
   if (density) {
     ACES_HIP(hipMemsetAsync(gfo_, 0, sizeof(double) * oo, s));
     ACES_HIP(hipMemsetAsync(gfv_, 0, sizeof(double) * vv, s));
@@ -196,10 +243,32 @@ int CcsdEngine::compute_adjoint(const double* lam, bool density) {
 
   // ---- (10)-(9) particle-particle ladder ----
   ACES_CHECK(ladder_unpack_adjoint(o, v, l2, gYb_, gZp_, nop_, gZm_, std::max<long>(nom_, 1), s));
-  ACES_CHECK(gemm64(gd((int)nop_, (int)nvp_, (int)(nvp_ + ov), gZp_, nop_, 0, Vp_, nvp_, 0, gSp_, nvp_, 1.0, 0.0)));
-  if (nom_ > 0 && nvm_ > 0)
+  if (lp) {
+    // gS = gZ^T V in bf16x3: over V's pair rows with V's k-contiguous split
+    // read as V^T, over its T1-dressing rows with their transposed split.  V-'s square block is
+    // symmetric; V+'s is G W with G symmetric and W = diag(1/2 on the diagonal
+    // pairs (c,c), else 1), so V+[k][n] = V+[n][k] w_n / w_k: gZ's
+    // diagonal-pair rows are doubled (nothing else reads them) and the
+    // product's diagonal-pair columns halved.
+    ACES_CHECK(scale_diag_pairs(v, gZp_, nop_, nop_, 0, 2.0, s));
+    ACES_CHECK(adj_gemm_lp((int)nop_, (int)nvp_, (int)nvp_, gZp_, nop_, 0, 5, LowPOp{lp_[0].hi, lp_[0].lo, lp_[0].ld},
+                           gSp_, nvp_, 1.0, 0.0));
+    ACES_CHECK(scale_diag_pairs(v, gSp_, nvp_, nop_, 1, 0.5, s));
+    ACES_CHECK(adj_gemm_lp((int)nop_, (int)nvp_, (int)ov, gZp_ + nvp_ * nop_, nop_, 0, 31,
+                           LowPOp{lam_lp_[4].hi, lam_lp_[4].lo, lam_lp_[4].ld}, gSp_, nvp_, 1.0, 1.0));
+    if (nom_ > 0 && nvm_ > 0) {
+      ACES_CHECK(adj_gemm_lp((int)nom_, (int)nvm_, (int)nvm_, gZm_, nom_, 0, 6,
+                             LowPOp{lp_[1].hi, lp_[1].lo, lp_[1].ld}, gDp_, nvm_, 1.0, 0.0));
+      ACES_CHECK(adj_gemm_lp((int)nom_, (int)nvm_, (int)ov, gZm_ + nvm_ * nom_, nom_, 0, 32,
+                             LowPOp{lam_lp_[5].hi, lam_lp_[5].lo, lam_lp_[5].ld}, gDp_, nvm_, 1.0, 1.0));
+    }
+  } else {
     ACES_CHECK(
-        gemm64(gd((int)nom_, (int)nvm_, (int)(nvm_ + ov), gZm_, nom_, 0, Vm_, nvm_, 0, gDp_, nvm_, 1.0, 0.0)));
+        gemm64(gd((int)nop_, (int)nvp_, (int)(nvp_ + ov), gZp_, nop_, 0, Vp_, nvp_, 0, gSp_, nvp_, 1.0, 0.0)));
+    if (nom_ > 0 && nvm_ > 0)
+      ACES_CHECK(
+          gemm64(gd((int)nom_, (int)nvm_, (int)(nvm_ + ov), gZm_, nom_, 0, Vm_, nvm_, 0, gDp_, nvm_, 1.0, 0.0)));
+  }
   ACES_CHECK(pack_tau_adjoint(o, v, gSp_, nvp_, gDp_, std::max<long>(nvm_, 1), gtau_, s));
 
   // ---- (8) ring assembly: RS = (R1 - R2)/2 ; inc2 += RS[iajb] + RS[jbia] - R2[ibja] - R2[jaib] ----
@@ -210,13 +279,23 @@ int CcsdEngine::compute_adjoint(const double* lam, bool density) {
                  .src(l2, -1.0, vv, v, ov2, 1)
                  .run(s));
   // ---- (7) rings: R1 = Wt U^T (gR1 = gRS/2), R2 = Wv X^T ----
-  ACES_CHECK(gemm64(gd(OV, OV, OV, gRS_, ov, 1, U_, ov, 0, gWt_, ov, 0.5, 0.0)));
-  ACES_CHECK(gemm64(gd(OV, OV, OV, gRS_, ov, 0, Wt_, ov, 0, gU_, ov, 0.5, 1.0)));
-  ACES_CHECK(gemm64(gd(OV, OV, OV, gR2_, ov, 1, X_, ov, 0, gWv_, ov, 1.0, 0.0)));
-  ACES_CHECK(gemm64(gd(OV, OV, OV, gR2_, ov, 0, Wv_, ov, 0, gX_, ov, 1.0, 0.0)));
-  // ---- (6) W build: Wt += Ut L^T / 2, Wv -= Xt K^T / 2  (L, K symmetric) ----
-  ACES_CHECK(gemm64(gd(OV, OV, OV, gWt_, ov, 1, Lmat_, ov, 1, gUt_, ov, 0.5, 0.0)));
-  ACES_CHECK(gemm64(gd(OV, OV, OV, gWv_, ov, 1, Kx_, ov, 1, gXt_, ov, -0.5, 0.0)));
+  if (lp) {
+    auto op = [](const LowP& L) { return LowPOp{L.hi, L.lo, L.ld}; };
+    ACES_CHECK(adj_gemm_lp(OV, OV, OV, gRS_, ov, 1, 9, op(lam_lp_[0]), gWt_, ov, 0.5, 0.0));
+    ACES_CHECK(adj_gemm_lp(OV, OV, OV, gRS_, ov, 0, 10, op(lam_lp_[1]), gU_, ov, 0.5, 1.0));
+    ACES_CHECK(adj_gemm_lp(OV, OV, OV, gR2_, ov, 1, 11, op(lam_lp_[2]), gWv_, ov, 1.0, 0.0));
+    ACES_CHECK(adj_gemm_lp(OV, OV, OV, gR2_, ov, 0, 12, op(lam_lp_[3]), gX_, ov, 1.0, 0.0));
+    ACES_CHECK(adj_gemm_lp(OV, OV, OV, gWt_, ov, 1, 7, op(lp_[2]), gUt_, ov, 0.5, 0.0));
+    ACES_CHECK(adj_gemm_lp(OV, OV, OV, gWv_, ov, 1, 8, op(lp_[3]), gXt_, ov, -0.5, 0.0));
+  } else {
+    ACES_CHECK(gemm64(gd(OV, OV, OV, gRS_, ov, 1, U_, ov, 0, gWt_, ov, 0.5, 0.0)));
+    ACES_CHECK(gemm64(gd(OV, OV, OV, gRS_, ov, 0, Wt_, ov, 0, gU_, ov, 0.5, 1.0)));
+    ACES_CHECK(gemm64(gd(OV, OV, OV, gR2_, ov, 1, X_, ov, 0, gWv_, ov, 1.0, 0.0)));
+    ACES_CHECK(gemm64(gd(OV, OV, OV, gR2_, ov, 0, Wv_, ov, 0, gX_, ov, 1.0, 0.0)));
+    // ---- (6) W build: Wt += Ut L^T / 2, Wv -= Xt K^T / 2  (L, K symmetric) ----
+    ACES_CHECK(gemm64(gd(OV, OV, OV, gWt_, ov, 1, Lmat_, ov, 1, gUt_, ov, 0.5, 0.0)));
+    ACES_CHECK(gemm64(gd(OV, OV, OV, gWv_, ov, 1, Kx_, ov, 1, gXt_, ov, -0.5, 0.0)));
+  }
   // ---- (5) Wt = W0t + 2 P1[kcai] - P2[kaci] - Q12[kcia] ; Wv = Wv0 + P2[kaci] - Q2[kcia] ----
   ACES_CHECK(P4(o, v, v, o, gP1_, 1.0).src(gWt_, 2.0, v, 1, ov, ov2).run(s));
   ACES_CHECK(P4(o, v, v, o, gP2_).src(gWt_, -1.0, v, ov, 1, ov2).src(gWv_, 1.0, v, ov, 1, ov2).run(s));
@@ -292,7 +371,22 @@ int CcsdEngine::lambda_start() {
   ACES_HIP(hipMemsetAsync(Lam_, 0, sizeof(double) * ov, s_));
   ACES_HIP(hipMemcpyAsync(Lam_ + ov, t2_, sizeof(double) * ov * ov, hipMemcpyDeviceToDevice, s_));
   ldiis_.R.assign((size_t)ldiis_.order * ldiis_.order, 0.0);
+  lam_anc_valid_ = false;
   ACES_HIP(hipStreamSynchronize(s_));
+  return ACES_OK;
+}
+
+int CcsdEngine::set_lambda_precision(int prec) {
+  ACES_REQUIRE(prec == GP_F64 || prec == kPrecRefine, "lambda precision must be 0 (fp64) or 3 (refinement)");
+  ACES_CHECK(adjoint_alloc());
+  if (prec == kPrecRefine && lamA_ == nullptr) {
+    const long nt = (long)o * v + (long)o * v * o * v;
+    ACES_CHECK(alloc(&lamA_, nt));
+    ACES_CHECK(alloc(&gTA_, nt));
+    ACES_CHECK(alloc(&dlam_, nt));
+  }
+  lam_prec_ = prec;
+  lam_anc_valid_ = false;
   return ACES_OK;
 }
 
@@ -311,11 +405,30 @@ int CcsdEngine::lambda_cycle(int icycle, double tol, CcsdStats& st) {
   ACES_HIP(hipEventCreate(&e1));
   ACES_HIP(hipEventRecord(e0, s_));
   ACES_CHECK(lambda_from_aces(o, v, Lam_, lam_, s_));
-  ACES_CHECK(compute_adjoint(lam_, false));
+  const long nt = ov + ov * ov;
+  const bool anchor = (lam_prec_ == kPrecRefine && !lam_anc_valid_);
+  if (lam_prec_ == kPrecRefine && lam_anc_valid_) {
+    // r(l) = r(l_a) + J^T (l - l_a)
+    ACES_CHECK(sub(nt, lam_, lamA_, dlam_, s_));
+    ACES_CHECK(compute_adjoint(dlam_, false, GP_BF16X3, true));
+    ACES_CHECK(P4(1, 1, 1, nt, gT_, 1.0).src(gTA_, 1.0, 0, 0, 0, 1).run(s_));
+  } else {
+    ACES_CHECK(compute_adjoint(lam_, false));
+    if (anchor) {
+      ACES_HIP(hipMemcpyAsync(lamA_, lam_, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
+      ACES_HIP(hipMemcpyAsync(gTA_, gT_, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
+    }
+  }
   ACES_CHECK(symmetrize_t2(o, v, gT_ + ov, s_));
   ACES_CHECK(lambda_jacobi(o, v, eo_, ev_, Lam_, gT_, Lamn_, s_));
   ACES_HIP(hipEventRecord(e1, s_));
   ACES_CHECK(diis_core(ldiis_, Lam_, Lamn_, icycle, tol, true, st));
+  if (anchor) {
+    lam_anc_valid_ = true;
+    lam_anc_res_ = std::fabs(st.resmiter);
+  } else if (lam_prec_ == kPrecRefine && std::fabs(st.resmiter) < 1e-3 * lam_anc_res_) {
+    lam_anc_valid_ = false;  // the next cycle anchors at the current multipliers
+  }
   float ms = 0.f;
   ACES_HIP(hipEventSynchronize(e1));
   ACES_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -348,6 +461,7 @@ int CcsdEngine::set_lambda(const double* l1, const double* l2, int aces_normalis
   ACES_HIP(hipMemcpyAsync(dst, l1, sizeof(double) * ov, hipMemcpyDeviceToDevice, s_));
   ACES_HIP(hipMemcpyAsync(dst + ov, l2, sizeof(double) * ov * ov, hipMemcpyDeviceToDevice, s_));
   if (!aces_normalisation) ACES_CHECK(lambda_to_aces(o, v, lam_, Lam_, s_));
+  lam_anc_valid_ = false;
   ACES_HIP(hipStreamSynchronize(s_));
   return ACES_OK;
 }

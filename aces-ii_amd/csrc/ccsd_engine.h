@@ -64,7 +64,8 @@ class CcsdEngine {
   int set_fock(const double* fock);
   // Per-launch timing of the MFMA contractions (hipEvents on the engine's stream).
   // out[3*c + 0..2] = {ms, algorithmic flops, launches} for c = 0 ladder, 1 W build,
-  // 2 ring, 3 the other fp64 contractions of the increments, 4 the reverse pass.
+  // 2 ring, 3 the other fp64 contractions of the increments, 4 the reverse pass,
+  // 5 the reverse pass's bf16x3 products (mixed Lambda cycles).
   void set_profiling(bool on) { profile_ = on; }
   void profile(double* out, int reset);
   long device_bytes() const { return bytes_; }
@@ -82,6 +83,13 @@ class CcsdEngine {
   int lambda_start();
   int lambda_cycle(int icycle, double tol, CcsdStats& st);
   int set_lambda_diis_order(int order);
+  // Lambda cycle precision: GP_F64 (0), or kPrecRefine (3): the residual as
+  // r(l) = r(l_a) + J^T (l - l_a) with r(l_a) an fp64 reverse pass at an anchor
+  // l_a and the linear part (the reverse pass without the energy seeds) on the
+  // difference with its eight large products in bf16x3; the anchor moves to
+  // the current multipliers each time |RESMITER| has fallen 1e3-fold below
+  // the anchor cycle's.
+  int set_lambda_precision(int prec);
   // Multipliers in/out: spatial l1/l2 (device) or the reference's lists
   // (L1 = Lambda1 of list 190, L2 = Lambda(Ij,Ab) of list 146).
   int get_lambda(double* l1, double* l2, int aces_normalisation);
@@ -151,12 +159,19 @@ class CcsdEngine {
                 bool x_sym = false, bool xn_sym = false);
   int diis_resize(DiisState& D, int order);
   int adjoint_alloc();
-  int compute_adjoint(const double* lam, bool density);
+  // lin: the linear part only (no energy seeds); prec GP_BF16X3 runs the ring,
+  // W-build and ladder products in bf16x3
+  int compute_adjoint(const double* lam, bool density, GemmPrec prec = GP_F64, bool lin = false);
+  // reverse-pass product C = alpha op(A) B^T + beta C in bf16x3: A (fp64,
+  // k-contiguous when akc) split into operand slot aslot, B a fixed split
+  int adj_gemm_lp(int M, int N, int K, const double* A, long lda, int akc, int aslot, const LowPOp& B, double* C,
+                  long ldc, double alpha, double beta);
+  int lambda_lowp_operands();
 
   hipStream_t s_;
   long bytes_ = 0;
   bool profile_ = false;
-  static constexpr int kProfCats = 5;
+  static constexpr int kProfCats = 6;
   double prof_[3 * kProfCats] = {};
   int prof_cat_ = 0;
   std::vector<void*> allocs_;
@@ -185,12 +200,12 @@ class CcsdEngine {
   double *scal_ = nullptr; // device scalars
   // low precision operand slots: static (converted once per precision)
   // 0 Vp 1 Vm 2 Lmat 3 Kx 4 OVkl; per call 5 Sp 6 Dp 7 Ut 8 Xt 9 Wt 10 Wv
-  // 11 U 12 X 13 tau 14 Wo^T 15 tau^T
+  // 11 U 12 X 13 tau 14 Wo^T 15 tau^T; 31, 32 the ladder adjoints' T1-dressing rows
   // slots 0..4, 16..19 and 26..30 hold static operands (converted once per
   // precision), the others this cycle's operands
-  static constexpr int kLpStatic = 5, kLpStatic2 = 16, kLpSlots = 31;
+  static constexpr int kLpStatic = 5, kLpStatic2 = 16, kLpSlots = 33;
   static constexpr bool lp_static_slot(int slot) {
-    return slot < kLpStatic || (slot >= kLpStatic2 && slot < 20) || slot >= 26;
+    return slot < kLpStatic || (slot >= kLpStatic2 && slot < 20) || (slot >= 26 && slot < 31);
   }
   LowP lp_[kLpSlots];
   bool lp_fresh_[kLpSlots] = {};  // slot already holds this cycle's operand (bf16x3)
@@ -228,6 +243,15 @@ class CcsdEngine {
   double *gS3_, *gtauK_;
   double *gFovI_, *gFooI_, *gFvvI_, *gLoo_, *gLvv_, *gFT_, *gFovT_, *gGk_;
   double *gfo_, *gfv_, *gfov_, *geo_, *gev_;
+  // mixed Lambda cycles (set_lambda_precision)
+  int lam_prec_ = 0;
+  bool lam_anc_valid_ = false;
+  double lam_anc_res_ = 0.0;
+  double *lamA_ = nullptr, *gTA_ = nullptr, *dlam_ = nullptr;
+  // splits fixed while T is: U^T, Wt^T, X^T, Wv^T, and the transposed
+  // T1-dressing rows of V+ and V-
+  LowP lam_lp_[6];
+  bool lam_lp_valid_ = false;
 };
 
 }  // namespace aces

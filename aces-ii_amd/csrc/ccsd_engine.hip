@@ -26,8 +26,7 @@ namespace {
 using eu::P4;
 using eu::gd;
 
-// low-precision operand rows are padded (with zeros) to whole 32-wide k tiles
-inline long pad32(long x) { return (x + 31) / 32 * 32; }
+using eu::pad32;
 
 }  // namespace
 
@@ -573,6 +572,8 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
     return ACES_OK;
   }
   ACES_REQUIRE(prec == GP_F64 || prec == GP_F32 || prec == GP_BF16X3, "ccsd: unknown precision");
+  // the forward intermediates change: the Lambda splits and anchor with them
+  lam_lp_valid_ = lam_anc_valid_ = false;
   hipStream_t s = s_;
   const long ov = (long)o * v, oo = (long)o * o, vv = (long)v * v;
   const long ov2 = ov * v, o2v = oo * v, vo2 = v * oo, v2o = vv * o;

@@ -44,6 +44,9 @@ struct P4 {
   }
 };
 
+// low-precision operand rows are padded (with zeros) to whole 32-wide k tiles
+inline long pad32(long x) { return (x + 31) / 32 * 32; }
+
 inline GemmDesc gd(int M, int N, int K, const double* A, long lda, int akc, const double* B, long ldb, int bkc, double* C,
             long ldc, double alpha, double beta) {
   GemmDesc d;

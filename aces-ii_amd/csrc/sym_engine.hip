@@ -21,8 +21,7 @@ using dpd::Src;
 
 namespace {
 
-// low-precision operand rows are padded (with zeros) to whole 32-wide k tiles
-inline long pad32(long x) { return (x + 31) / 32 * 32; }
+using eu::pad32;
 
 constexpr int kDotBlocks = 512;
 constexpr int kMaxDots = 32;

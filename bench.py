@@ -438,11 +438,58 @@ def bench_lambda(args):
     lam_cycles = icycle
     adj = prof['adjoint']
     achieved = adj['flops'] / (adj['ms'] * 1e-3)
-    roofline = {'bound': 'mfma', 'kernel': 'gemm16_kernel<double> / gemm_f64_dl_kernel (reverse-pass contractions)',
-                'achieved': achieved / 1e12, 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
-                'frac': achieved / FP64_PEAK, 'traffic': None, 'avg_launch_ms': adj['ms'] / adj['launches'],
-                'launches_per_cycle': adj['launches'] / nprof,
-                'gemm_share_of_cycle': (adj['ms'] / nprof) / (s_per_cycle * 1e3)}
+    fp64_roofline = {'bound': 'mfma',
+                     'kernel': 'gemm16_kernel<double> / gemm_f64_dl_kernel (reverse-pass contractions)',
+                     'achieved': achieved / 1e12, 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
+                     'frac': achieved / FP64_PEAK, 'avg_launch_ms': adj['ms'] / adj['launches'],
+                     'launches_per_cycle': adj['launches'] / nprof,
+                     'gemm_share_of_cycle': (adj['ms'] / nprof) / (s_per_cycle * 1e3)}
+    log('fp64 roofline', json.dumps(fp64_roofline))
+
+    # ---- whole Lambda solves: fp64, and mixed (fp64 anchors, the linear part
+    # on the difference to the anchor with its ring/W-build/ladder products in
+    # bf16x3); the mixed solve's seconds per cycle (anchors included) is `value`
+    solves = {}
+    for form in ('fp64', 'refine'):
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        h = cc.solve_lambda(maxcyc=args.maxcyc, conv=conv, order=6, prec=form)
+        torch.cuda.synchronize()
+        solves[form] = {'cycles': h['cycles'], 'total_s': time.perf_counter() - ta}
+        solves[form]['s_per_cycle'] = solves[form]['total_s'] / h['cycles']
+        log('lambda solve %s: %d cycles, %.2f s' % (form, h['cycles'], solves[form]['total_s']))
+    solves['speedup_total'] = solves['fp64']['total_s'] / solves['refine']['total_s']
+    # timed region of the mixed form: K cycles after W warmup cycles from the
+    # Lambda start (the first warmup cycle is the fp64 anchor)
+    s_fp64 = s_per_cycle
+    if args.prec != 'fp64':
+        cc.set_lambda_precision('refine')
+        cc.lambda_start()
+        icycle = 0
+        for _ in range(args.warmup):
+            icycle += 1
+            cc.lambda_cycle(icycle, tol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            icycle += 1
+            cc.lambda_cycle(icycle, tol)
+        torch.cuda.synchronize()
+        s_per_cycle = (time.perf_counter() - t0) / args.steps
+        cc.set_lambda_precision('fp64')
+        log('lambda mixed: %.4f s/cycle (fp64 %.4f)' % (s_per_cycle, s_fp64))
+    cc.set_profiling(True)
+    cc.profile(reset=True)
+    hp = cc.solve_lambda(maxcyc=args.maxcyc, conv=conv, order=6, prec='refine')
+    pm = cc.profile(reset=True)
+    cc.set_profiling(False)
+    lp = pm['adjoint_lp']
+    achieved = 3.0 * lp['flops'] / (lp['ms'] * 1e-3)
+    roofline = {'bound': 'mfma', 'kernel': 'gemm_bf16x3_kernel (reverse-pass products of the mixed cycles)',
+                'achieved': achieved / 1e12, 'peak': BF16_PEAK / 1e12, 'unit': 'TFLOP/s',
+                'frac': achieved / BF16_PEAK, 'traffic': None, 'avg_launch_ms': lp['ms'] / lp['launches'],
+                'launches_per_cycle': lp['launches'] / hp['cycles'],
+                'fp64_anchor_ms_per_cycle': pm['adjoint']['ms'] / hp['cycles']}
     log('roofline', json.dumps(roofline))
 
     # ---- dE/dmu checks ----
@@ -482,7 +529,7 @@ def bench_lambda(args):
     Em, ncm = converge(cc, prec=args.prec if args.prec != 'fp64' else 'bf16x3')
     Tm = cc.get_amplitudes()
     cc.set_lambda_diis_order(6)
-    lm = cc.solve_lambda(maxcyc=args.maxcyc, conv=conv, order=6)
+    lm = cc.solve_lambda(maxcyc=args.maxcyc, conv=conv, order=6, prec='refine')
     an_fm, an_gm = analytic(cc, *Tm)
     cc.close()
     del cc
@@ -527,11 +574,15 @@ def bench_lambda(args):
         'higher_is_better': False,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'f64',
+        'dtype': 'bf16x3/fp32-acc + f64' if args.prec != 'fp64' else 'f64',
         'data': 'synthetic (DF integrals, seed %d, sigma=0.05*sqrt(95/(o*v)))' % args.seed,
         'config': {'workload': 'closed-shell CCSD Lambda cycle (reverse pass + Jacobi + DIIS), RHF C1, o=%d v=%d '
-                               'n_aux=%d' % (o, v, 4 * n), 'o': o, 'v': v, 'diis': 6},
-        'roofline': roofline,
+                               'n_aux=%d' % (o, v, 4 * n), 'o': o, 'v': v, 'diis': 6,
+                   'form': 'mixed (fp64 anchor + bf16x3 linear part)' if args.prec != 'fp64' else 'fp64'},
+        'roofline': roofline if args.prec != 'fp64' else dict(fp64_roofline, traffic=None),
+        'fp64_roofline': fp64_roofline,
+        'fp64_s_per_iter': s_fp64,
+        'lambda_solves': solves,
         'fd_checks': fd,
     }
     if not args.no_cpu:

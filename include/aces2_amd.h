@@ -102,7 +102,9 @@ int aces_ccsd_set_fock(void* handle, const double* d_fock);
 /* Per-launch hipEvent timing of the MFMA contractions (on the engine's stream):
  * out[3c+0] ms, out[3c+1] algorithmic flops (2mnk), out[3c+2] launches, for
  * c = 0 particle ladder, 1 W(mbej) build, 2 ring, 3 the other fp64
- * contractions of the increments, 4 the reverse pass (out holds 15 values).
+ * contractions of the increments, 4 the reverse pass (fp64), 5 the reverse
+ * pass's bf16x3 products (mixed Lambda cycles) (out holds 18 values; the
+ * irrep-blocked engine's aces_sym_profile fills the first 15).
  * No reference counterpart
  * (the reference has only whole-program timers, acescore/aces_fin.F:29-55). */
 int aces_ccsd_set_profiling(void* handle, int on);
@@ -139,6 +141,12 @@ int aces_ccsd_adjoint(void* handle, const double* d_l1, const double* d_l2, doub
 int aces_ccsd_lambda_start(void* handle);
 int aces_ccsd_lambda_cycle(void* handle, int icycle, double tol, double* stats);
 int aces_ccsd_set_lambda_diis_order(void* handle, int order);
+/* No counterpart in the reference (fp64 only): 0 fp64 Lambda cycles, 3 mixed
+ * cycles r(l) = r(l_a) + J^T (l - l_a) with r(l_a) an fp64 reverse pass at an
+ * anchor l_a and the linear part on the difference with its ring, W-build and
+ * ladder products in bf16x3; the anchor moves to the current multipliers each
+ * time |RESMITER| has fallen 1e3-fold below the anchor cycle's. */
+int aces_ccsd_set_lambda_precision(void* handle, int prec);
 int aces_ccsd_get_lambda(void* handle, double* d_l1, double* d_l2, int aces_norm);
 int aces_ccsd_set_lambda(void* handle, const double* d_l1, const double* d_l2, int aces_norm);
 int aces_ccsd_density(void* handle, double* d_dens);

@@ -80,6 +80,32 @@ def test_lambda_solve_and_density_vs_oracle(nonhf):
     np.testing.assert_allclose(D, Dref, atol=1e-10)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize('o,v,seed,nonhf', [(6, 30, 12, True), (7, 45, 13, False)])
+def test_lambda_refine_matches_fp64(o, v, seed, nonhf):
+    """Mixed Lambda cycles (fp64 anchors, the linear part on the difference to
+    the anchor with its large products in bf16x3) converge to the fp64
+    multipliers and density."""
+    from aces2.ccsd import CCSD
+    print('case lambda refine o=%d v=%d seed=%d nonhf=%s' % (o, v, seed, nonhf), flush=True)
+    f, eri, blk = _blocks(o, v, seed, nonhf)
+    cc = CCSD(o, v, blk)
+    cc.run(maxcyc=200, conv=11, order=6)
+    h64 = cc.solve_lambda(maxcyc=200, conv=10, order=6)
+    a1, a2 = cc.get_lambda()
+    Da = cc.density()
+    hmx = cc.solve_lambda(maxcyc=200, conv=10, order=6, prec='refine')
+    b1, b2 = cc.get_lambda()
+    Db = cc.density()
+    cc.close()
+    assert h64['converged'] and hmx['converged']
+    assert hmx['cycles'] <= h64['cycles'] + 4
+    sc = max(float(a2.abs().max()), 1.0)
+    assert float((a1 - b1).abs().max()) < 1e-8 * sc
+    assert float((a2 - b2).abs().max()) < 1e-8 * sc
+    assert float((Da - Db).abs().max()) < 1e-8
+
+
 class _EnergyOfFock:
     """E_CCSD(F) at fixed integrals as a torch.autograd.Function: forward runs the
     GPU amplitude equations, backward multiplies by the response density from
