@@ -82,15 +82,20 @@ int CcsdEngine::adjoint_alloc() {
 int CcsdEngine::lambda_lowp_operands() {
   if (lam_lp_valid_) return ACES_OK;
   ACES_CHECK(lowp_static(GP_BF16X3));
-  const long ov = (long)o * v, Kp = pad32(ov);
-  // {source, rows (the product's N), source leading dimension}; all have K = ov
+  const long ov = (long)o * v, oo = (long)o * o, vv = (long)v * v;
   const long ldm = std::max<long>(nvm_, 1);
-  struct S { const double* src; long rows; long ld; };
-  const S tab[6] = {{U_, ov, ov}, {Wt_, ov, ov}, {X_, ov, ov}, {Wv_, ov, ov},
-                    {Vp_ + nvp_ * nvp_, nvp_, nvp_}, {Vm_ + nvm_ * ldm, nvm_, ldm}};
-  for (int q = 0; q < 6; ++q) {
+  // {source, rows, K, source leading dimension, k-contiguous source}
+  struct S { const double* src; long rows, K, ld; bool kc; };
+  const S tab[kLamLp] = {{U_, ov, ov, ov, false},    {Wt_, ov, ov, ov, false},
+                         {X_, ov, ov, ov, false},    {Wv_, ov, ov, ov, false},
+                         {Vp_ + nvp_ * nvp_, nvp_, ov, nvp_, false},
+                         {Vm_ + nvm_ * ldm, nvm_, ov, ldm, false},
+                         {Wo_, oo, oo, oo, true},    {tau_, oo, vv, vv, true},
+                         {OVkl_, vv, oo, vv, false}};
+  for (int q = 0; q < kLamLp; ++q) {
     LowP& L = lam_lp_[q];
     if (tab[q].rows == 0) continue;
+    const long Kp = pad32(tab[q].K);
     if (L.hi == nullptr) {
       void* blk;
       ACES_CHECK(alloc_raw(&blk, tab[q].rows * Kp * 4));
@@ -99,8 +104,12 @@ int CcsdEngine::lambda_lowp_operands() {
       L.ld = Kp;
       L.rows = tab[q].rows;
     }
-    ACES_CHECK(split_bf16_t((int)tab[q].rows, (int)ov, tab[q].src, tab[q].ld, (unsigned short*)L.hi,
+    if (tab[q].kc)
+      ACES_CHECK(split_bf16((int)tab[q].rows, (int)tab[q].K, tab[q].src, tab[q].ld, (unsigned short*)L.hi,
                             (unsigned short*)L.lo, Kp, s_));
+    else
+      ACES_CHECK(split_bf16_t((int)tab[q].rows, (int)tab[q].K, tab[q].src, tab[q].ld, (unsigned short*)L.hi,
+                              (unsigned short*)L.lo, Kp, s_));
   }
   lam_lp_valid_ = true;
   return ACES_OK;
@@ -230,9 +239,23 @@ int CcsdEngine::compute_adjoint(const double* lam, bool density, GemmPrec prec, 
   ACES_CHECK(gemv(OV, OV, Lmat_, ov, gFovI_, gt1, 1.0, 1.0, s));
 
   // ---- (12) hole-hole ladder ----
-  ACES_CHECK(gemm64(gd((int)oo, (int)vv, (int)oo, Wo_, oo, 1, l2, vv, 0, gtau_, vv, 1.0, 1.0)));
-  ACES_CHECK(gemm64(gd((int)oo, (int)oo, (int)vv, tau_, vv, 1, l2, vv, 1, gWo_, oo, 1.0, 0.0)));
-  ACES_CHECK(gemm64(gd((int)oo, (int)vv, (int)oo, gWo_, oo, 0, OVkl_, vv, 0, gtau_, vv, 1.0, 1.0)));
+  if (lp) {
+    auto op = [](const LowP& L) { return LowPOp{L.hi, L.lo, L.ld}; };
+    LowPOp b;
+    const int cat = prof_cat_;
+    prof_cat_ = 5;
+    ACES_CHECK(split_operand(b, 15, l2, nullptr, vv, (int)oo, vv, 0));
+    ACES_CHECK(lp_product((int)oo, (int)vv, (int)oo, op(lam_lp_[6]), b, gtau_, vv, 1.0, 1.0));
+    ACES_CHECK(split_operand(b, 13, l2, nullptr, oo, (int)vv, vv, 1));
+    ACES_CHECK(lp_product((int)oo, (int)oo, (int)vv, op(lam_lp_[7]), b, gWo_, oo, 1.0, 0.0));
+    ACES_CHECK(split_operand(b, 14, gWo_, nullptr, oo, (int)oo, oo, 0));
+    ACES_CHECK(lp_product((int)oo, (int)vv, (int)oo, b, op(lam_lp_[8]), gtau_, vv, 1.0, 1.0));
+    prof_cat_ = cat;
+  } else {
+    ACES_CHECK(gemm64(gd((int)oo, (int)vv, (int)oo, Wo_, oo, 1, l2, vv, 0, gtau_, vv, 1.0, 1.0)));
+    ACES_CHECK(gemm64(gd((int)oo, (int)oo, (int)vv, tau_, vv, 1, l2, vv, 1, gWo_, oo, 1.0, 0.0)));
+    ACES_CHECK(gemm64(gd((int)oo, (int)vv, (int)oo, gWo_, oo, 0, OVkl_, vv, 0, gtau_, vv, 1.0, 1.0)));
+  }
   ACES_CHECK(P4(o, o, o, o, gQo_).src(gWo_, 1.0, o3, oo, o, 1).src(gWo_, 1.0, oo, o3, 1, o).run(s));
   ACES_CHECK(gemm64(gd(o, v, (int)o3, gQo_, o, 0, ovooQ_, v, 0, gt1, v, 1.0, 1.0)));
 
@@ -245,11 +268,11 @@ int CcsdEngine::compute_adjoint(const double* lam, bool density, GemmPrec prec, 
   ACES_CHECK(ladder_unpack_adjoint(o, v, l2, gYb_, gZp_, nop_, gZm_, std::max<long>(nom_, 1), s));
   if (lp) {
     // gS = gZ^T V in bf16x3: over V's pair rows with V's k-contiguous split
-    // read as V^T, over its T1-dressing rows with their transposed split.  V-'s square block is
-    // symmetric; V+'s is G W with G symmetric and W = diag(1/2 on the diagonal
-    // pairs (c,c), else 1), so V+[k][n] = V+[n][k] w_n / w_k: gZ's
-    // diagonal-pair rows are doubled (nothing else reads them) and the
-    // product's diagonal-pair columns halved.
+    // read as V^T, over its T1-dressing rows with their transposed split.
+    // V-'s square block is symmetric; V+'s is G W with G symmetric and
+    // W = diag(1/2 on the diagonal pairs (c,c), else 1), so
+    // V+[k][n] = V+[n][k] w_n / w_k: gZ's diagonal-pair rows are doubled
+    // (nothing else reads them) and the product's diagonal-pair columns halved.
     ACES_CHECK(scale_diag_pairs(v, gZp_, nop_, nop_, 0, 2.0, s));
     ACES_CHECK(adj_gemm_lp((int)nop_, (int)nvp_, (int)nvp_, gZp_, nop_, 0, 5, LowPOp{lp_[0].hi, lp_[0].lo, lp_[0].ld},
                            gSp_, nvp_, 1.0, 0.0));

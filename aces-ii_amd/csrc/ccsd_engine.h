@@ -248,9 +248,10 @@ class CcsdEngine {
   bool lam_anc_valid_ = false;
   double lam_anc_res_ = 0.0;
   double *lamA_ = nullptr, *gTA_ = nullptr, *dlam_ = nullptr;
-  // splits fixed while T is: U^T, Wt^T, X^T, Wv^T, and the transposed
-  // T1-dressing rows of V+ and V-
-  LowP lam_lp_[6];
+  // splits fixed while T is: U^T, Wt^T, X^T, Wv^T, the transposed T1-dressing
+  // rows of V+ and V-, Wo, tau, OVkl^T
+  static constexpr int kLamLp = 9;
+  LowP lam_lp_[kLamLp];
   bool lam_lp_valid_ = false;
 };
 
