@@ -641,8 +641,8 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
 
 // ---------------------------------------------------------------------------
 // fp64 kernel with direct-to-LDS operand loads (global_load_lds_dwordx4) into
-// an S-stage LDS ring; K % 16 == 0.  Block (32 FM) x (32 FN) on 2 x 2 waves,
-// each a (16 FM) x (16 FN) tile of v_mfma_f64_16x16x4_f64.
+// an S-stage LDS ring; K % 16 == 0.  Block (16 FM WM) x (16 FN WN) on WM x WN
+// waves, each a (16 FM) x (16 FN) tile of v_mfma_f64_16x16x4_f64.
 // LDS layout per operand: k-contiguous (KC) rows hold 16 doubles (128 bytes,
 // eight 16-byte chunks) with chunk q of row r at position q ^ (r & 7); a
 // row-contiguous operand keeps its own [k][row] order (one k row of the tile
@@ -654,6 +654,7 @@ __global__ __launch_bounds__(128 * WM) void gemm_bf16x3_dl_kernel(GemmGroups gg)
 // ---------------------------------------------------------------------------
 template <bool KC, int ROWS>
 struct DlOperand {
+  static_assert(KC || (ROWS <= 128 && 128 % ROWS == 0), "row-contiguous tiles: one k row per 1 KB or less");
   static constexpr int BK = 16;
   static constexpr int NI = ROWS / 8;  // 1 KB wave instructions per stage
   // global pointer (this lane) and LDS offset (wave-uniform) of instruction G
@@ -680,10 +681,10 @@ struct DlOperand {
   }
 };
 
-template <bool AKC, bool BKC, int FM, int FN, int S>
-__global__ __launch_bounds__(256) void gemm_f64_dl_kernel(GemmGroups gg) {
-  constexpr int BK = 16, NW = 4;
-  constexpr int BM = 32 * FM, BN = 32 * FN;
+template <bool AKC, bool BKC, int FM, int FN, int S, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void gemm_f64_dl_kernel(GemmGroups gg) {
+  constexpr int BK = 16, NW = WM * WN;
+  constexpr int BM = 16 * FM * WM, BN = 16 * FN * WN;
   typedef DlOperand<AKC, BM> OA;
   typedef DlOperand<BKC, BN> OB;
   constexpr int NG = OA::NI + OB::NI, IPW = NG / NW;
@@ -709,8 +710,8 @@ __global__ __launch_bounds__(256) void gemm_f64_dl_kernel(GemmGroups gg) {
   const int t = threadIdx.x;
   const int lane = t & 63;
   const int wave = t >> 6;
-  const int wm = wave >> 1;
-  const int wn = wave & 1;
+  const int wm = wave / WN;
+  const int wn = wave % WN;
 
   const double* gp[IPW];
   int goff[IPW];
@@ -902,20 +903,19 @@ static int bf16_shape(const GemmDesc* d, int ng) {
   return !any ? 2 : (wide ? 8 : 4);
 }
 // fp64 problems with 16-byte aligned rows, K % 16 == 0, K >= 256 and
-// M >= 256 run on the direct-to-LDS kernel (128 x 64 blocks), except products
-// of two k-contiguous operands that are not tall and narrow (M >= 1024,
-// N < 4096), which run as fast on the 64 x 64 register-staged kernel.
+// M >= 256 run on the direct-to-LDS kernel: 128 x 128 blocks of 2 x 4 waves
+// (64 x 32 each), two LDS stages (64 KB, two blocks per CU).  Measured against
+// the 64 x 64 register-staged kernel and other direct-to-LDS shapes (128 x 64
+// on 4 waves with 3 stages, 128 x 128 on 4 or 16 waves, 3 or 4 stages): 70 TF
+// on 15360^3 and 62.5 TF on the 66720 x 1176 x 51360 ladder against 55 and 50.
 static constexpr int kShapeF64Dl = 16;
-static constexpr int kF64DlFM = 4, kF64DlFN = 2, kF64DlStages = 3;
+static constexpr int kF64DlFM = 4, kF64DlFN = 2, kF64DlStages = 2, kF64DlWM = 2, kF64DlWN = 4;
 static bool f64_direct(const GemmDesc* d, int ng) {
   bool any = false;
   for (int g = 0; g < ng; ++g)
     if (d[g].M > 0 && d[g].N > 0) {
       const GemmDesc& e = d[g];
       if (e.K < 256 || e.K % 16 != 0 || e.M < 256) return false;
-      // k-contiguous products: tall narrow outputs only (the others run as fast
-      // on the register-staged kernel)
-      if (e.a_kc && e.b_kc && (e.N >= 4096 || e.M < 1024)) return false;
       if ((uintptr_t)e.A % 16 || (uintptr_t)e.B % 16 || e.lda % 2 || e.ldb % 2 || e.sA % 2 || e.sB % 2) return false;
       // row-contiguous operands are loaded in row pairs
       if ((!e.a_kc && e.M % 2) || (!e.b_kc && e.N % 2)) return false;
@@ -937,12 +937,12 @@ static int bf16_bn(int shape) { return shape == 8 ? 256 : 128; }
 // Output block of each kernel: rows x columns.
 static int block_rows(GemmPrec prec, int shape) {
   if (prec == GP_BF16X3) return bf16_bm(shape);
-  if (prec == GP_F64 && shape == kShapeF64Dl) return 32 * kF64DlFM;
+  if (prec == GP_F64 && shape == kShapeF64Dl) return 16 * kF64DlFM * kF64DlWM;
   return (prec == GP_F32) ? TileCfg<float>::type::BM : TileCfg<double>::type::BM;
 }
 static int block_cols(GemmPrec prec, int shape) {
   if (prec == GP_BF16X3) return bf16_bn(shape);
-  if (prec == GP_F64 && shape == kShapeF64Dl) return 32 * kF64DlFN;
+  if (prec == GP_F64 && shape == kShapeF64Dl) return 16 * kF64DlFN * kF64DlWN;
   return (prec == GP_F32) ? TileCfg<float>::type::BN : TileCfg<double>::type::BN;
 }
 
@@ -1029,8 +1029,8 @@ static int launch_groups(const GemmGroups& gg, GemmPrec prec, bool akc, bool bkc
       case GP_F64:
         if (bf_wm == kShapeF64Dl) {
 #define F64DL(AK, BK2)                                                                                        \
-  hipLaunchKernelGGL((gemm_f64_dl_kernel<AK, BK2, kF64DlFM, kF64DlFN, kF64DlStages>), grid, dim3(256), 0, stream, \
-                     gg)
+  hipLaunchKernelGGL((gemm_f64_dl_kernel<AK, BK2, kF64DlFM, kF64DlFN, kF64DlStages, kF64DlWM, kF64DlWN>), grid,  \
+                     dim3(64 * kF64DlWM * kF64DlWN), 0, stream, gg)
           if (akc && bkc)
             F64DL(true, true);
           else if (akc)
