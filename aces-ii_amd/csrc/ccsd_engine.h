@@ -14,8 +14,10 @@ namespace aces {
 
 // Cycle precision of the engine beyond the GEMM library's three: the eight
 // large products (ladders, W build, ring) as bf16x3 products of the operands'
-// differences to an fp64 anchor evaluation (CcsdEngine::gemm_anchored); the
-// first such cycle after the amplitudes were set records the anchor.
+// differences to an fp64 anchor evaluation (CcsdEngine::gemm_anchored).  The
+// first such cycle after set-up or reset_anchor() records the anchor; a new
+// one is recorded each time |RESMITER| has fallen 1e3-fold below the anchor
+// cycle's (the differences, and with them the bf16x3 error, stay small).
 constexpr int kPrecRefine = 3;
 
 struct CcsdStats {
@@ -51,7 +53,7 @@ class CcsdEngine {
   int jacobi_step();
   int diis_step(int icycle, double tol, CcsdStats& st, bool xn_sym = false);
   // Writable views; writing through T() invalidates the resident forward intermediates.
-  double* T() { fwd_valid_ = anc_valid_ = false; return t1_; }  // [t1 | t2] current amplitudes
+  double* T() { fwd_valid_ = false; return t1_; }  // [t1 | t2] current amplitudes
   double* Tnew() { return t1n_; }    // [t1 | t2] Jacobi update
   double* Inc() { return inc1_; }    // [inc1 | inc2]
   int energy_dev(const double* t1, const double* t2, double* d_out);
@@ -215,6 +217,8 @@ class CcsdEngine {
   // refinement anchor (gemm_anchored): 0 off, 1 recording, 2 refining
   int anc_mode_ = 0;
   bool anc_valid_ = false;
+  int anc_last_ = 0;       // mode of the last evaluation of the increments
+  double anc_res_ = 0.0;   // |RESMITER| of the anchor cycle
   int anc_site_ = 0;  // product index within one evaluation of the increments
   struct AncSite {
     double* C0 = nullptr;

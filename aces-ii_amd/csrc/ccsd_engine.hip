@@ -190,6 +190,7 @@ int CcsdEngine::setup(const double* fock, const double* ovov, const double* oovv
   ACES_CHECK(alloc(&d_coef_, 64));
   ACES_CHECK(set_diis_order(tdiis_.order));
   ACES_HIP(hipStreamSynchronize(s));
+  anc_valid_ = false;  // anchored products hold the previous integrals
   ready_ = true;
   return ACES_OK;
 }
@@ -223,7 +224,7 @@ int CcsdEngine::set_amplitudes(const double* t1, const double* t2) {
   ACES_HIP(hipMemcpyAsync(t1_, t1, sizeof(double) * ov, hipMemcpyDeviceToDevice, s_));
   ACES_HIP(hipMemcpyAsync(t2_, t2, sizeof(double) * ov * ov, hipMemcpyDeviceToDevice, s_));
   ACES_HIP(hipStreamSynchronize(s_));
-  fwd_valid_ = anc_valid_ = false;
+  fwd_valid_ = false;
   return ACES_OK;
 }
 
@@ -245,7 +246,7 @@ int CcsdEngine::start_mp1() {
   t_sym_ = false;
   ACES_CHECK(jacobi(o, v, eo_, ev_, inc1_, inc2_, t1_, t2_, s_));
   ACES_HIP(hipStreamSynchronize(s_));
-  fwd_valid_ = anc_valid_ = false;
+  fwd_valid_ = false;
   return ACES_OK;
 }
 
@@ -568,9 +569,11 @@ int CcsdEngine::compute_increments(GemmPrec prec) {
     anc_mode_ = 0;
     ACES_CHECK(rc);
     if (mode == 1) anc_valid_ = true;
+    anc_last_ = mode;
     fwd_valid_ = (mode == 1);
     return ACES_OK;
   }
+  anc_last_ = 0;
   ACES_REQUIRE(prec == GP_F64 || prec == GP_F32 || prec == GP_BF16X3, "ccsd: unknown precision");
   // the forward intermediates change: the Lambda splits and anchor with them
   lam_lp_valid_ = lam_anc_valid_ = false;
@@ -871,6 +874,13 @@ int CcsdEngine::jacobi_step() { return jacobi(o, v, eo_, ev_, inc1_, inc2_, t1n_
 // hold the next amplitudes unless converged.
 int CcsdEngine::diis_step(int icycle, double tol, CcsdStats& st, bool xn_sym) {
   ACES_CHECK(diis_core(tdiis_, t1_, t1n_, icycle, tol, false, st, t_sym_, xn_sym));
+  // refinement anchor: remember the anchor cycle's residual; move the anchor
+  // once the residual has fallen 1e3-fold below it
+  if (anc_last_ == 1)
+    anc_res_ = std::fabs(st.resmiter);
+  else if (anc_last_ == 2 && std::fabs(st.resmiter) < 1e-3 * anc_res_)
+    anc_valid_ = false;
+  anc_last_ = 0;
   // a DIIS combination of symmetrized vectors (or the symmetrized update
   // itself) is exactly pair-symmetric
   if (!st.converged) t_sym_ = true;

@@ -296,6 +296,7 @@ int aces_vcc_set_precision(int prec) {
   ACES_CHECK(ctx(&c));
   ACES_REQUIRE(prec >= 0 && prec <= 3,
                "precision must be 0 (fp64), 1 (fp32), 2 (bf16x3) or 3 (bf16x3 refinement of an fp64 anchor)");
+  if (prec == kPrecRefine && c->prec != kPrecRefine && c->eng) c->eng->reset_anchor();  // anchor at the switch
   c->prec = prec;
   return ACES_OK;
 }

@@ -73,9 +73,11 @@ int aces_split_bf16(int rows, int cols, const double* d_x, long ldx, unsigned sh
  * prec: 0 fp64, 1 fp32 and 2 bf16x3 products (fp32 accumulation) for the
  * ladders, W(mbej) build, ring and read-bound T1 products; 3 refinement: an
  * fp64 cycle whose ladder, W build and ring products run as bf16x3 products of
- * the operands' differences to an fp64 anchor evaluation (the first prec-3
- * cycle after the amplitudes were set records the anchor), so that their error
- * scales with the distance to the anchor; the reference has fp64 only.
+ * the operands' differences to an fp64 anchor evaluation, so that their error
+ * scales with the distance to the anchor (the first prec-3 cycle after set-up
+ * or aces_ccsd_reset_anchor records the anchor, and a new one is recorded each
+ * time |RESMITER| has fallen 1e3-fold below the anchor cycle's); the reference
+ * has fp64 only.
  * stats (host double[7]) from aces_ccsd_cycle: E(T_new) printed by CMPENG,
  * RESMITER, RESMDIIS, converged flag, final energy (valid when converged),
  * ms in the increments, ms in the whole cycle.

@@ -137,8 +137,8 @@ def test_refine_polish(o, v, seed):
 
 
 def test_refine_anchor_cycle_is_fp64():
-    """The first refinement evaluation after the amplitudes are set records the
-    anchor and equals the fp64 evaluation; near convergence, a later one at
+    """The first refinement evaluation after set-up (or reset_anchor) records
+    the anchor and equals the fp64 evaluation; near convergence, a later one at
     other amplitudes (reached by refinement cycles) has a small fraction of
     plain bf16x3's error (the difference operands are small)."""
     from aces2.ccsd import CCSD

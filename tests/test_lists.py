@@ -132,3 +132,47 @@ def test_xvcc_driver_over_lists(nonhf):
     np.testing.assert_allclose(energies, ref['energy'], rtol=0, atol=1e-10)
     np.testing.assert_allclose(resm, ref['resmiter'], rtol=1e-6, atol=1e-13)
     assert abs(ef.value - ref['final_energy']) < 1e-10
+
+
+@pytest.mark.gpu
+def test_xvcc_driver_mixed_then_refine():
+    """The list driver at precision 2 (bf16x3) to 1e-6, then 3 (refinement of
+    an fp64 anchor) to 1e-10 with DIIS restarted: the fp64 loop's energy."""
+    import ctypes
+    from aces2 import _lib, lists, synth
+    print('case xvcc lists mixed+refine', flush=True)
+    o, v = 6, 30
+    eps, B = synth.draw(o, v, 32, sigma=synth.scaled_sigma(o, v))
+    blk, eri = synth.blocks_numpy(eps, B, o)
+    ref = vcc_loop.run(blk['fock'], rccsd.blocks(eri, o), o, maxcyc=80, conv=10, order=6)
+    lists.init()
+    lists.load_c1_system(blk, o, v)
+    _lib.call('aces_vcc_mp1')
+    _lib.call('aces_vcc_set_diis_order', 6)
+    _lib.call('aces_vcc_set_precision', 2)
+    conv = ctypes.c_int(1)
+    st = (ctypes.c_double * 8)()
+    tol, ic0, phase, n_refine = 1e-6, 0, 2, 0
+    for icycle in range(1, 81):
+        _lib.call('aces_genint')
+        _lib.call('aces_dre3en')
+        _lib.call('aces_e4s')
+        _lib.call('aces_newt2')
+        e = ctypes.c_double()
+        _lib.call('aces_cmpeng', 60, 2, None, ctypes.byref(e))
+        _lib.call('aces_dodiis0', icycle - ic0, tol, ctypes.byref(conv), st)
+        n_refine += phase == 3
+        if conv.value == 0:
+            if phase == 2:
+                phase, tol, ic0 = 3, 1e-10, icycle
+                _lib.call('aces_vcc_set_precision', 3)
+                _lib.call('aces_vcc_set_diis_order', 6)
+                continue
+            break
+        _lib.call('aces_drmove')
+    ef = ctypes.c_double()
+    _lib.call('aces_cmpeng', 43, 0, None, ctypes.byref(ef))
+    _lib.call('aces_vcc_set_precision', 0)
+    lists.fin()
+    assert conv.value == 0 and phase == 3 and n_refine < ref['cycles']
+    assert abs(ef.value - ref['final_energy']) < 1e-9
