@@ -709,21 +709,35 @@ __global__ void absmax_partial_kernel(int o, int v, const double* __restrict__ e
   const long n1 = (long)o * v, vv = (long)v * v, n2 = n1 * n1;
   const long nom = (long)o * (o - 1) / 2, nvm = (long)v * (v - 1) / 2;
   AbsMax m{-1.0, 0x7fffffffffffffffL, 0.0};
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n1 + n2;
-       idx += (long)gridDim.x * blockDim.x) {
-    if (idx < n1) {
-      am_update(m, e[idx], idx);
-    } else {
-      const long r = idx - n1;
-      const long ij = r / vv, ab = r - ij * vv;
-      const int i = (int)(ij / o), j = (int)(ij % o), a = (int)(ab / v), b = (int)(ab % v);
-      // list 46 position of element (A=a,b; I=i,j): a + v*b + v*v*(i + o*j)
-      const long pos46 = n1 + (long)a + (long)v * b + vv * ((long)i + (long)o * j);
-      am_update(m, e[idx], pos46);
-      if (i < j && a < b) {
-        const double x = e[idx] - e[n1 + ((long)j * o + i) * vv + ab];
-        const long pos44 = n1 + n2 + pk_lt(a, b) + nvm * pk_lt(i, j);
-        am_update(m, x, pos44);
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n1; idx += (long)gridDim.x * blockDim.x)
+    am_update(m, e[idx], idx);
+  // t2 rows (i,j) over the blocks, (a,b) over the threads: 32-bit index
+  // arithmetic per element (the candidates and the tie rule are those of a
+  // flat scan, so the result does not depend on the order)
+  const int nij = o * o, nab = v * v;
+  for (int ij = blockIdx.x; ij < nij; ij += gridDim.x) {
+    const int i = ij / o, j = ij - i * o;
+    const double* __restrict__ row = e + n1 + (long)ij * vv;
+    const double* __restrict__ twin = e + n1 + ((long)j * o + i) * vv;
+    // list 46 position of element (A=a,b; I=i,j): a + v*b + v*v*(i + o*j)
+    const long base46 = n1 + vv * ((long)i + (long)o * j);
+    const long base44 = n1 + n2 + nvm * pk_lt(i, j);
+    // four elements per thread in flight: the loads go out before the updates
+    for (int ab0 = threadIdx.x; ab0 < nab; ab0 += 4 * blockDim.x) {
+      double x[4], t[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int ab = ab0 + u * blockDim.x;
+        x[u] = ab < nab ? row[ab] : 0.0;
+        t[u] = (ab < nab && i < j) ? twin[ab] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int ab = ab0 + u * blockDim.x;
+        if (ab >= nab) break;
+        const int a = ab / v, b = ab - a * v;
+        am_update(m, x[u], base46 + a + (long)v * b);
+        if (i < j && a < b) am_update(m, x[u] - t[u], base44 + pk_lt(a, b));
       }
     }
   }
