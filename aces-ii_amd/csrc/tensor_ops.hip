@@ -1,5 +1,7 @@
 // HBM-bound tensor kernels for the CCSD amplitude path (gfx950).
 #include <algorithm>
+#include <cstdint>
+#include <type_traits>
 #include "common.h"
 #include "tensor_ops.h"
 #include "bf16_util.h"
@@ -420,6 +422,49 @@ __global__ void gemv_t_kernel(int K, int N, Mat A, long lda, const double* __res
   }
 }
 
+// gemv_t on the bf16 hi/lo split with four columns per thread (8-byte loads
+// of each half; rows 8-byte aligned): block (64 x 4) covers 256 columns, the
+// same k slices and the same order of additions as gemv_t_kernel.
+__global__ void gemv_t_split4_kernel(int K, int N, const unsigned short* __restrict__ hi,
+                                     const unsigned short* __restrict__ lo, long lda, const double* __restrict__ x,
+                                     double* __restrict__ y, double alpha, double beta, int kchunk,
+                                     double* __restrict__ part) {
+  __shared__ double red[4][256];
+  const int n0 = blockIdx.x * 256 + 4 * threadIdx.x;
+  const long k0 = (long)blockIdx.y * kchunk;
+  const long k1 = min((long)K, k0 + kchunk);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (n0 + 3 < N) {
+    for (long k = k0 + threadIdx.y; k < k1; k += blockDim.y) {
+      const ushort4 h = *reinterpret_cast<const ushort4*>(hi + k * lda + n0);
+      const ushort4 l = *reinterpret_cast<const ushort4*>(lo + k * lda + n0);
+      const double xk = x[k];
+      acc[0] += ((double)bf16_to_f(h.x) + (double)bf16_to_f(l.x)) * xk;
+      acc[1] += ((double)bf16_to_f(h.y) + (double)bf16_to_f(l.y)) * xk;
+      acc[2] += ((double)bf16_to_f(h.z) + (double)bf16_to_f(l.z)) * xk;
+      acc[3] += ((double)bf16_to_f(h.w) + (double)bf16_to_f(l.w)) * xk;
+    }
+  } else {
+    for (int u = 0; u < 4; ++u)
+      if (n0 + u < N)
+        for (long k = k0 + threadIdx.y; k < k1; k += blockDim.y)
+          acc[u] += ((double)bf16_to_f(hi[k * lda + n0 + u]) + (double)bf16_to_f(lo[k * lda + n0 + u])) * x[k];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) red[threadIdx.y][4 * threadIdx.x + u] = acc[u];
+  __syncthreads();
+  const int t = threadIdx.y * 64 + threadIdx.x;
+  const int n = blockIdx.x * 256 + t;
+  if (n < N) {
+    double sum = 0.0;
+    for (int j = 0; j < 4; ++j) sum += red[j][t];
+    if (part)
+      part[(long)blockIdx.y * N + n] = sum;
+    else
+      y[n] = alpha * sum + (beta == 0.0 ? 0.0 : beta * y[n]);
+  }
+}
+
 __global__ void gemv_t_finish_kernel(int N, int nsplit, const double* __restrict__ part, double* __restrict__ y,
                                      double alpha, double beta) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -450,8 +495,17 @@ static int gemv_t_launch(int K, int N, Mat A, long lda, const double* x, double*
     nsplit = std::min<long>(nsplit, 65535);
   }
   const long kchunk = (K + nsplit - 1) / std::max<long>(nsplit, 1);
-  hipLaunchKernelGGL(gemv_t_kernel<Mat>, dim3((unsigned)nbx, (unsigned)nsplit), dim3(256, 4), 0, s, K, N, A, lda,
-                     x, y, alpha, beta, (int)std::max<long>(kchunk, 1), nsplit > 1 ? ws : nullptr);
+  bool launched = false;
+  if constexpr (std::is_same<Mat, GemvSplit>::value) {
+    if (lda % 4 == 0 && (uintptr_t)A.hi % 8 == 0 && (uintptr_t)A.lo % 8 == 0) {
+      hipLaunchKernelGGL(gemv_t_split4_kernel, dim3((unsigned)nbx, (unsigned)nsplit), dim3(64, 4), 0, s, K, N, A.hi,
+                         A.lo, lda, x, y, alpha, beta, (int)std::max<long>(kchunk, 1), nsplit > 1 ? ws : nullptr);
+      launched = true;
+    }
+  }
+  if (!launched)
+    hipLaunchKernelGGL(gemv_t_kernel<Mat>, dim3((unsigned)nbx, (unsigned)nsplit), dim3(256, 4), 0, s, K, N, A, lda,
+                       x, y, alpha, beta, (int)std::max<long>(kchunk, 1), nsplit > 1 ? ws : nullptr);
   ACES_HIP(hipGetLastError());
   if (nsplit > 1) {
     hipLaunchKernelGGL(gemv_t_finish_kernel, dim3((unsigned)nbx), dim3(256), 0, s, N, (int)nsplit, ws, y, alpha,
