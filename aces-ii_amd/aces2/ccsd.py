@@ -13,7 +13,7 @@ import ctypes
 from . import _lib
 
 # 'refine': fp64 cycle whose eight large products run as bf16x3 products of
-# the operands' differences to an fp64 anchor evaluation (C1 engine only)
+# the operands' differences to an fp64 anchor evaluation
 PREC = {'fp64': 0, 'fp32': 1, 'bf16x3': 2, 'refine': 3}
 
 
@@ -112,7 +112,7 @@ class CCSD:
 
     def reset_anchor(self):
         """The next 'refine' cycle records a new fp64 anchor."""
-        _lib.call('aces_ccsd_reset_anchor', self.h)
+        _lib.call(self._p + 'reset_anchor', self.h)
 
     def cycle(self, icycle, prec='fp64', tol=1e-7):
         st = (ctypes.c_double * 8)()
@@ -289,7 +289,7 @@ class SymCCSD(CCSD):
         return _lib.lib().aces_sym_packed_t2(self.h)
 
     def run(self, maxcyc=50, conv=7, order=6, start='mp1', prec='fp64', log=None, raise_on_fail=True,
-            polish_conv=None):
+            polish_conv=None, polish='refine'):
         torch = self.torch
         self.set_diis_order(order)
         self.reset_diis()
@@ -311,8 +311,10 @@ class SymCCSD(CCSD):
             if log:
                 log(icycle, st)
             if st['converged'] and phase == prec and prec in ('fp32', 'bf16x3') and polish_conv is not None:
-                phase, tol, ic0 = 'fp64', 10.0 ** (-polish_conv), icycle
+                phase, tol, ic0 = polish, 10.0 ** (-polish_conv), icycle
                 self.reset_diis()
+                if polish == 'refine':
+                    self.reset_anchor()
                 continue
             if st['converged']:
                 hist['converged'] = True

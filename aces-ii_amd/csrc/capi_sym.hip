@@ -91,6 +91,12 @@ int aces_sym_reset_diis(void* h) {
   return sym(h)->reset_diis();
 }
 
+int aces_sym_reset_anchor(void* h) {
+  REQUIRE_SYM(h);
+  sym(h)->reset_anchor();
+  return ACES_OK;
+}
+
 int aces_sym_energy(void* h, double* e) {
   REQUIRE_SYM(h);
   return sym(h)->energy_current(*e);

@@ -38,6 +38,9 @@ class SymEngine {
   int energy_current(double& e);
   int set_diis_order(int order);
   int reset_diis();
+  // prec kPrecRefine (3) as in CcsdEngine: the large products (cat <= 2) as
+  // bf16x3 products of the operands' differences to an fp64 anchor evaluation
+  void reset_anchor() { anc_valid_ = false; }
   // differentiation form (dense multipliers/densities, spatial normalisation
   // of CcsdEngine::adjoint)
   int adjoint(const double* l1, const double* l2, double* r1, double* r2, double* dens);
@@ -108,7 +111,9 @@ class SymEngine {
             double alpha, double beta, GemmPrec prec, int cat, const LowBuf* la, const LowBuf* lb);
   int gemv_run(int y, double* yd, int A, const double* Ad, const double* xd, int trans, double alpha, double beta);
   int outer0(int A, double* Ad, const double* u, const double* w, double alpha);  // A0 += alpha u w^T
-  int lowp_convert(int t, GemmPrec prec, LowBuf& lb);
+  // sub: split the difference T - sub (sub laid out like the tensor)
+  int lowp_convert(int t, GemmPrec prec, LowBuf& lb, const double* sub = nullptr);
+  int gemm_anchored(size_t k, int mode);
   int run_perm(Op& op, double* out, const double* const* src, const double* X, const double* Y,
                const dpd::Denom* den = nullptr);
   int forward(GemmPrec prec);
@@ -162,6 +167,18 @@ class SymEngine {
   // Lambda state
   double *Lam_ = nullptr, *Lamn_ = nullptr, *lam_ = nullptr, *gT_ = nullptr;
   bool fwd_valid_ = false, lam_ready_ = false;
+  // refinement anchor (forward(kPrecRefine)); per op: C0 = A0 B0 and, when both
+  // operands are amplitude-dependent, the split of B0; per tensor: the anchor
+  // copy and the split of the difference to it
+  bool anc_valid_ = false;
+  int anc_last_ = 0;
+  double anc_res_ = 0.0;
+  long anc_pass_ = 0;
+  std::vector<double*> anc_C0_;
+  std::vector<LowBuf> anc_B0_;
+  std::vector<double*> anc_T_;
+  std::vector<long> anc_Tpass_, anc_Tver_;
+  std::vector<LowBuf> low_diff_;
 };
 
 }  // namespace aces

@@ -476,7 +476,8 @@ void SymEngine::profile(double* out, int reset) {
     for (int i = 0; i < 15; ++i) prof_[i] = 0.0;
 }
 
-int SymEngine::lowp_convert(int t, GemmPrec prec, LowBuf& lb) {
+int SymEngine::lowp_convert(int t, GemmPrec prec, LowBuf& lb, const double* sub) {
+  ACES_REQUIRE(sub == nullptr || prec == GP_BF16X3, "sym: difference splits are bf16x3");
   const Tensor& tt = T_[t];
   if (lb.hi == nullptr) {
     long off = 0;
@@ -498,7 +499,8 @@ int SymEngine::lowp_convert(int t, GemmPrec prec, LowBuf& lb) {
                              (float*)lb.hi + lb.off[g], lb.ld[g], s_));
     else
       ACES_CHECK(split_bf16((int)tt.nrow[g], (int)tt.ncol[g], tt.d + tt.boff[g], tt.ncol[g],
-                            (unsigned short*)lb.hi + lb.off[g], (unsigned short*)lb.lo + lb.off[g], lb.ld[g], s_));
+                            (unsigned short*)lb.hi + lb.off[g], (unsigned short*)lb.lo + lb.off[g], lb.ld[g], s_,
+                            sub ? sub + tt.boff[g] : nullptr));
   }
   lb.prec = prec;
   lb.valid = true;
@@ -650,13 +652,92 @@ int SymEngine::outer0(int A, double* Ad, const double* u, const double* w, doubl
 }
 
 // ---------------------------------------------------------------------------
+// One large product of a refinement pass (see CcsdEngine::gemm_anchored):
+// mode 1 records C0 = A B in fp64 and the anchor copies, mode 2 forms
+// C = beta C + al (C0 + A (B - B0) + (A - A0) B0) with the products in bf16x3.
+int SymEngine::gemm_anchored(size_t k, int mode) {
+  const Op& op = ops_[k];
+  Tensor& tc = T_[op.out];
+  const long nC = tc.boff[nirr];
+  const bool sa = !T_[op.A].grad, sb = !T_[op.B].grad;
+  ACES_REQUIRE(op.a_kc && op.b_kc, "sym refinement: operands must be k-contiguous");
+  if (sa && sb)
+    return ggemm(op.out, tc.d, op.A, T_[op.A].d, 1, op.B, T_[op.B].d, 1, op.al, op.beta, GP_F64, op.cat, nullptr,
+                 nullptr);
+  if (mode == 1) {
+    if (anc_C0_[k] == nullptr) ACES_CHECK(alloc(&anc_C0_[k], std::max<long>(nC, 1)));
+    ACES_CHECK(ggemm(op.out, anc_C0_[k], op.A, T_[op.A].d, 1, op.B, T_[op.B].d, 1, 1.0, 0.0, GP_F64, op.cat, nullptr,
+                     nullptr));
+    for (int t : {op.A, op.B}) {
+      if (!T_[t].grad) continue;
+      // one copy per tensor and pass: a tensor read by two anchored products
+      // must not change between them
+      if (anc_Tpass_[t] == anc_pass_) {
+        ACES_REQUIRE(anc_Tver_[t] == T_[t].ver, "sym refinement: anchored operand rewritten within a pass");
+        continue;
+      }
+      const long nt = T_[t].boff[nirr];
+      if (anc_T_[t] == nullptr) ACES_CHECK(alloc(&anc_T_[t], std::max<long>(nt, 1)));
+      if (nt) ACES_HIP(hipMemcpyAsync(anc_T_[t], T_[t].d, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
+      anc_Tpass_[t] = anc_pass_;
+      anc_Tver_[t] = T_[t].ver;
+    }
+    if (!sa && !sb) ACES_CHECK(lowp_convert(op.B, GP_BF16X3, anc_B0_[k]));
+  } else {
+    ACES_REQUIRE(anc_C0_[k] != nullptr, "sym refinement: no anchor for this product");
+  }
+  // C = beta C + al C0
+  if (nC) ACES_CHECK(eu::P4(1, 1, 1, nC, tc.d, op.beta).src(anc_C0_[k], op.al, 0, 0, 0, 1).run(s_));
+  if (mode == 1) return ACES_OK;
+  auto full = [&](int t) -> int {
+    LowBuf& lb = low_[t];
+    if (!lb.valid || lb.prec != GP_BF16X3 || lb.ver != T_[t].ver) {
+      ACES_CHECK(lowp_convert(t, GP_BF16X3, lb));
+      lb.ver = T_[t].ver;
+    }
+    return ACES_OK;
+  };
+  auto diff = [&](int t) -> int {
+    ACES_REQUIRE(anc_T_[t] != nullptr, "sym refinement: operand without an anchor copy");
+    return lowp_convert(t, GP_BF16X3, low_diff_[t], anc_T_[t]);
+  };
+  if (sa || sb) {  // static integral operand: one product with the other's difference
+    const int dyn = sa ? op.B : op.A, st = sa ? op.A : op.B;
+    ACES_CHECK(full(st));
+    ACES_CHECK(diff(dyn));
+    const LowBuf* la = sa ? &low_[op.A] : &low_diff_[op.A];
+    const LowBuf* lb = sa ? &low_diff_[op.B] : &low_[op.B];
+    return ggemm(op.out, tc.d, op.A, T_[op.A].d, 1, op.B, T_[op.B].d, 1, op.al, 1.0, GP_BF16X3, op.cat, la, lb);
+  }
+  // A B - A0 B0 = A (B - B0) + (A - A0) B0
+  ACES_CHECK(full(op.A));
+  ACES_CHECK(diff(op.B));
+  ACES_CHECK(ggemm(op.out, tc.d, op.A, T_[op.A].d, 1, op.B, T_[op.B].d, 1, op.al, 1.0, GP_BF16X3, op.cat, &low_[op.A],
+                   &low_diff_[op.B]));
+  ACES_CHECK(diff(op.A));
+  return ggemm(op.out, tc.d, op.A, T_[op.A].d, 1, op.B, T_[op.B].d, 1, op.al, 1.0, GP_BF16X3, op.cat,
+               &low_diff_[op.A], &anc_B0_[k]);
+}
+
 int SymEngine::forward(GemmPrec prec) {
-  ACES_REQUIRE(prec == GP_F64 || prec == GP_F32 || prec == GP_BF16X3, "sym: unknown precision");
+  const bool refine = ((int)prec == kPrecRefine);
+  ACES_REQUIRE(refine || prec == GP_F64 || prec == GP_F32 || prec == GP_BF16X3, "sym: unknown precision");
+  const int mode = refine ? (anc_valid_ ? 2 : 1) : 0;
+  if (refine) {
+    ++anc_pass_;
+    anc_C0_.resize(ops_.size(), nullptr);
+    anc_B0_.resize(ops_.size());
+    anc_T_.resize(T_.size(), nullptr);
+    anc_Tpass_.resize(T_.size(), -1);
+    anc_Tver_.resize(T_.size(), -1);
+    low_diff_.resize(T_.size());
+  }
   // amplitude-dependent tensors change between passes; within a pass a split
   // is reused until the tensor is written again
   for (Tensor& t : T_)
     if (t.grad) ++t.ver;
-  for (Op& op : ops_) {
+  for (size_t k = 0; k < ops_.size(); ++k) {
+    Op& op = ops_[k];
     switch (op.kind) {
       case OP_PERM: {
         const double* sp[3] = {nullptr, nullptr, nullptr};
@@ -666,7 +747,9 @@ int SymEngine::forward(GemmPrec prec) {
         break;
       }
       case OP_GEMM: {
-        if (op.lowp && prec != GP_F64) {
+        if (refine && op.lowp && op.cat <= 2) {
+          ACES_CHECK(gemm_anchored(k, mode));
+        } else if (op.lowp && (prec == GP_F32 || prec == GP_BF16X3)) {
           for (int t : {op.A, op.B}) {
             LowBuf& lb = low_[t];
             if (!lb.valid || lb.prec != prec || lb.ver != T_[t].ver) {
@@ -703,7 +786,9 @@ int SymEngine::forward(GemmPrec prec) {
       ++T_[Dp_].ver;
     }
   }
-  fwd_valid_ = (prec == GP_F64);
+  fwd_valid_ = (prec == GP_F64) || mode == 1;
+  if (mode == 1) anc_valid_ = true;
+  anc_last_ = mode;
   return ACES_OK;
 }
 
@@ -1453,6 +1538,12 @@ int SymEngine::cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st) {
   ACES_CHECK(energy_dev(T_new_, T_new_ + n1_, enew));
   st.energy_new = enew;
   ACES_CHECK(diis_core(sol_, errv_, R_, order_, T_cur_, T_new_, icycle, tol, false, st));
+  // refinement anchor: moved once the residual has fallen 1e3-fold below the anchor cycle's
+  if (anc_last_ == 1)
+    anc_res_ = std::fabs(st.resmiter);
+  else if (anc_last_ == 2 && std::fabs(st.resmiter) < 1e-3 * anc_res_)
+    anc_valid_ = false;
+  anc_last_ = 0;
   if (st.converged) {
     double e;
     ACES_CHECK(energy_dev(T_cur_, T_cur_ + n1_, e));

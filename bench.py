@@ -28,7 +28,7 @@ irreps), o=64, v=384 split over the irreps by a seeded multinomial (at least
 2 occupied and 16 virtual orbitals per irrep), integrals from the same DF
 generator with symmetry-forbidden elements zeroed; the step is one CC cycle of
 the irrep-blocked engine (one grouped MFMA launch per contraction term) in the
-mixed form, followed outside the timed region by the fp64 polish to 1e-9 and
+mixed form, followed outside the timed region by the refinement polish to 1e-9 and
 the Lambda equations on the same lists.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ccsd|lambda|sym]
@@ -663,20 +663,22 @@ def bench_sym(args):
                                'ms': prof['other']['ms'] / 2}}
     log('roofline', json.dumps(roofline))
 
-    # convergence: mixed to 1e-7, fp64 polish (DIIS restarted) to 1e-9; fp64 reference run
+    # convergence: mixed to 1e-7, refinement polish (DIIS restarted) to 1e-9; fp64 reference run
     def converge(form, conv_lo, conv_hi):
         cc.set_diis_order(6)
         cc.reset_diis()
         cc.start_mp1()
-        phase, tol, ic, ic0, n = form, 10.0 ** (-conv_lo), 0, 0, {'lowp': 0, 'fp64': 0}
+        phase, tol, ic, ic0, n = form, 10.0 ** (-conv_lo), 0, 0, {'lowp': 0, 'refine': 0, 'fp64': 0}
         while ic < args.maxcyc:
             ic += 1
             st = cc.cycle(ic - ic0, phase, tol)
-            n['fp64' if phase == 'fp64' else 'lowp'] += 1
+            n[phase if phase in ('fp64', 'refine') else 'lowp'] += 1
             if st['converged']:
-                if phase != 'fp64':
-                    phase, tol = 'fp64', 10.0 ** (-conv_hi)
+                if phase == form and form != 'fp64':
+                    # polish with refinement cycles anchored here (DIIS restarted)
+                    phase, tol = 'refine', 10.0 ** (-conv_hi)
                     cc.reset_diis()
+                    cc.reset_anchor()
                     ic0 = ic
                     continue
                 return st['final_energy'], n

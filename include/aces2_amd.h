@@ -175,6 +175,9 @@ int aces_sym_increments(void* handle, int prec, double* d_inc1, double* d_inc2);
 int aces_sym_cycle(void* handle, int icycle, int prec, double tol, double* stats);
 int aces_sym_set_diis_order(void* handle, int order);
 int aces_sym_reset_diis(void* handle);
+/* As aces_ccsd_reset_anchor for the irrep-blocked engine (prec 3 there runs its
+ * ring, W-build and ladder products anchored). No reference counterpart. */
+int aces_sym_reset_anchor(void* handle);
 int aces_sym_energy(void* handle, double* e_out);
 int aces_sym_adjoint(void* handle, const double* d_l1, const double* d_l2, double* d_r1, double* d_r2,
                      double* d_dens);

@@ -147,3 +147,28 @@ def test_lambda_and_density_match_c1():
     np.testing.assert_allclose(s1.cpu().numpy(), c1.cpu().numpy(), atol=1e-10)
     np.testing.assert_allclose(s2.cpu().numpy(), c2.cpu().numpy(), atol=1e-10)
     np.testing.assert_allclose(Ds, Dc, atol=1e-10)
+
+
+def test_refine_polish_sym():
+    """Irrep-blocked engine: bf16x3 cycles to 1e-6, then refinement cycles (the
+    large products as bf16x3 products of the differences to the fp64 anchor of
+    the switch) to 1e-10: the fp64 loop's energy, and refined increments far
+    inside the plain bf16x3 error at the final amplitudes."""
+    from aces2.ccsd import SymCCSD
+    occ, vir = [2, 2, 1, 1], [5, 4, 3, 4]
+    print('case sym refine', flush=True)
+    f, eri, blk, irr = _system(occ, vir, 9)
+    cs = SymCCSD(occ, vir, blk)
+    h64 = cs.run(maxcyc=80, conv=10, order=6, start='mp1')
+    hr = cs.run(maxcyc=80, conv=6, order=6, start='mp1', prec='bf16x3', polish_conv=10, polish='refine')
+    assert h64['converged'] and hr['converged']
+    assert hr['precision'][0] == 'bf16x3' and hr['precision'][-1] == 'refine'
+    assert abs(hr['final_energy'] - h64['final_energy']) < 1e-9
+    i1r, i2r = cs.increments('refine')
+    i1d, i2d = cs.increments('fp64')
+    i1b, i2b = cs.increments('bf16x3')
+    cs.close()
+    err_r = max(float((i1r - i1d).abs().max()), float((i2r - i2d).abs().max()))
+    err_b = max(float((i1b - i1d).abs().max()), float((i2b - i2d).abs().max()))
+    print('sym refine err %.3e bf16x3 err %.3e' % (err_r, err_b), flush=True)
+    assert err_b > 0 and err_r < 1e-3 * err_b
