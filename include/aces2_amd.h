@@ -246,6 +246,8 @@ int aces_getrec(int iflag, const char* name, long* nwords, void* dest);
  * ---------------------------------------------------------------------- */
 int aces_vcc_mp1(void);
 int aces_vcc_set_diis_order(int order);
+/* prec as for aces_ccsd_cycle (0 fp64, 1 fp32, 2 bf16x3, 3 refinement); switching
+ * to 3 records a new refinement anchor at the next GENINT. */
 int aces_vcc_set_precision(int prec);
 int aces_genint(void);
 int aces_dre3en(void);
