@@ -98,9 +98,10 @@ SIGNATURES = {
     'aces_dre3en': (c_int, []),
     'aces_e4s': (c_int, []),
     'aces_newt2': (c_int, []),
-    'aces_cmpeng': (c_int, [c_int, c_int, c_dp, c_dp]),
+    'aces_cmpeng': (c_int, [c_int, c_int, c_dp, c_dp, c_dp]),
     'aces_dodiis0': (c_int, [c_int, c_double, ctypes.POINTER(c_int), c_dp]),
     'aces_drmove': (c_int, []),
+    'aces_drmove_lists': (c_int, [c_int, c_int]),
 }
 
 

@@ -54,6 +54,12 @@ class ListStore {
   // SUMSYM2: list(group) += add (the whole block)
   int sumsym2(const double* add, int group, int list, bool dev);
   int zero(int group, int list);
+  // Copy the block (src_group, src_list) into (dst_group, dst_list), creating or
+  // reshaping the destination to the source's shape; a destination without pair
+  // types takes the source's (the list moves of libr/drmove.f: MVSYM, MVLIST).
+  int move(int src_group, int src_list, int dst_group, int dst_list);
+  // Groups that exist for a list (ascending).
+  std::vector<int> groups(int list) const;
   long device_bytes() const { return bytes_; }
 
  private:

@@ -267,6 +267,34 @@ int ListStore::zero(int group, int list) {
   return ACES_OK;
 }
 
+int ListStore::move(int src_group, int src_list, int dst_group, int dst_list) {
+  Block* sb;
+  ACES_CHECK(block(src_group, src_list, &sb));
+  if (src_group == dst_group && src_list == dst_list) return ACES_OK;
+  const long nr = sb->nrows, nc = sb->ncols;
+  long r = -1, k = -1;
+  if (rows(dst_group, dst_list, &r) != ACES_OK || cols(dst_group, dst_list, &k) != ACES_OK || r != nr || k != nc)
+    ACES_CHECK(updmoi(nc, nr, dst_group, dst_list));
+  List& dl = lists_[dst_list];
+  if (dl.tl == 0 && dl.tr == 0) {
+    dl.tl = lists_[src_list].tl;
+    dl.tr = lists_[src_list].tr;
+  }
+  Block* db;
+  ACES_CHECK(block(dst_group, dst_list, &db));
+  if (nr * nc) ACES_HIP(hipMemcpyAsync(db->d, sb->d, sizeof(double) * nr * nc, hipMemcpyDeviceToDevice, s_));
+  ACES_HIP(hipStreamSynchronize(s_));
+  return ACES_OK;
+}
+
+std::vector<int> ListStore::groups(int list) const {
+  std::vector<int> g;
+  auto it = lists_.find(list);
+  if (it != lists_.end())
+    for (const auto& kv : it->second.groups) g.push_back(kv.first);
+  return g;
+}
+
 // ---------------------------------------------------------------------------
 std::string RecordStore::key(const std::string& name) {
   std::string k = name.substr(0, 8);

@@ -15,10 +15,14 @@
 //   DRE3EN   writes the T2 increment to list 63 (AB) and 61 (AA).
 //   E4S      writes T1_new = T1 increment / D1 to list 90 group 3.
 //   NEWT2    list 63 <- list 63 / D (libr2/newt2.F:233-244), list 61 <- ABTOAA(63).
-//   CMPENG   correlation energy from lists NLIST2+1..+3 and T1 group NLIST1+1.
+//   CMPENG   E(AA), E(AB), the total and the T2-only (linear) energy from lists
+//            NLIST2+1..+3 and T1 group NLIST1+1 (libr/cmpeng.F:99-180).
 //   DODIIS0  DIIS on (lists 44/46/90 g1) vs (61/63/90 g3); extrapolated vector
-//            written to the increment lists as in vcc/dodiis0.F.
-//   DRMOVE   T lists <- increment lists.
+//            written to the increment lists as in vcc/dodiis0.F.  Only xvcc's
+//            list arguments (44, 61, 90, 0, 90, 2, 70) are implemented; any
+//            other set ends the process through the fatal path.
+//   DRMOVE   lists 61/63 -> 44+NLIST/46+NLIST and, with SING1, list 90 group 3
+//            -> list 90+NLIST group 1 (libr/drmove.f).
 //
 // Integral lists read by GENINT (RHF, C1; types from intprc/clmoio.F):
 //   13 <Mn|Ij> (14,14), 16 <Ab|Ij> (13,14), 10 <Ij|Ka> (14,18),
@@ -46,6 +50,13 @@ struct AcesContext {
   std::unique_ptr<CcsdEngine> eng;
   int prec = GP_F64;
   int last_cycle = 0;
+  double* fov = nullptr;    // f(i,a) of the MO Fock matrix (zero for HF), [i][a]
+  double* epart = nullptr;  // CMPENG partial sums: o^2 blocks x 4, then the 5 totals
+  ~AcesContext() {
+    eng.reset();
+    if (fov) (void)hipFree(fov);
+    if (epart) (void)hipFree(epart);
+  }
 };
 
 static AcesContext* g_ctx = nullptr;
@@ -86,6 +97,76 @@ static int abtoaa(int o, int v, const double* t2, double* aa, hipStream_t s) {
   hipLaunchKernelGGL(abtoaa_kernel, dim3((unsigned)g), dim3(256), 0, s, o, v, t2, aa);
   ACES_HIP(hipGetLastError());
   return ACES_OK;
+}
+
+// CMPENG pieces (libr/cmpeng.F:99-180 with libr/tener.F, libr/ftau.f) on the
+// closed-shell lists, one block per (p, q) pair of occupied indices.  At memory
+// index m = ((p o + q) v + r) v + s the T2 list (46 or 63) holds t_pq^rs and
+// list 16 <rs|pq> = (pr|qs); the same-spin amplitudes are t_pq^rs - t_qp^rs
+// (ABTOAA) and <RS||PQ> = <rs|pq> - <sr|pq>.  part[4 (p o + q) + k]:
+//   k = 0  E(AB) = sum tau(Ab,Ij) <Ab|Ij>,  tau = t2 + t1 t1   1  its T2-only part
+//   k = 2  E(AA) = sum_{A<B, I<J} tau(AB,IJ) <AB||IJ> (a quarter of the unrestricted
+//          sum),  tau = tAA + t1(A,I) t1(B,J) - t1(A,J) t1(B,I)  3  its T2-only part
+__global__ void cmpeng_partial_kernel(int o, int v, const double* __restrict__ t1, const double* __restrict__ t2,
+                                      const double* __restrict__ w, double* __restrict__ part) {
+  const int p = blockIdx.x / o, q = blockIdx.x - p * o;
+  const long vv = (long)v * v;
+  const double* tpq = t2 + ((long)p * o + q) * vv;
+  const double* tqp = t2 + ((long)q * o + p) * vv;
+  const double* wpq = w + ((long)p * o + q) * vv;
+  const double* t1p = t1 + (long)p * v;
+  const double* t1q = t1 + (long)q * v;
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  for (long m = threadIdx.x; m < vv; m += blockDim.x) {
+    const int r = (int)(m / v), s = (int)(m - (long)r * v);
+    const double t = tpq[m], wab = wpq[m], wba = wpq[(long)s * v + r];
+    const double taa = t - tqp[m];
+    const double ss = t1p[r] * t1q[s];
+    const double waa = wab - wba;
+    s0 += (t + ss) * wab;
+    s1 += t * wab;
+    s2 += (taa + ss - t1q[r] * t1p[s]) * waa;
+    s3 += taa * waa;
+  }
+  __shared__ double red[4][256];
+  red[0][threadIdx.x] = s0;
+  red[1][threadIdx.x] = s1;
+  red[2][threadIdx.x] = s2;
+  red[3][threadIdx.x] = s3;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) part[4L * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Sums the block partials in block order (deterministic) and the non-HF singles
+// term sum t1(i,a) f(i,a): out[0..3] as the partials (AA ones quartered), out[4]
+// the singles term.
+__global__ void cmpeng_final_kernel(int nblk, int ov, const double* __restrict__ part,
+                                    const double* __restrict__ t1, const double* __restrict__ fov,
+                                    double* __restrict__ out) {
+  __shared__ double red[5][256];
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+    for (int k = 0; k < 4; ++k) acc[k] += part[4L * b + k];
+  for (int x = threadIdx.x; x < ov; x += blockDim.x) acc[4] += t1[x] * fov[x];
+  for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = red[0][0];
+    out[1] = red[1][0];
+    out[2] = 0.25 * red[2][0];
+    out[3] = 0.25 * red[3][0];
+    out[4] = red[4][0];
+  }
 }
 
 static int require_list(AcesContext* c, int list, long rows, long cols, const char* what) {
@@ -163,6 +244,20 @@ static int setup_engine(AcesContext* c) {
     const long st[4] = {v, ov * v, 1, ov};
     const long so[4] = {ov * v, v, ov, 1};  // ovov[i][a][j][b] over output (i, j, a, b)
     ACES_CHECK(perm(oovv, o, o, v, v, L23, 1.0, st, ovov, 1.0, so));
+  }
+  // f(i,a) for the non-HF singles term of CMPENG and its partial-sum scratch
+  {
+    std::vector<double> fo((size_t)ov);
+    for (int i = 0; i < o; ++i)
+      for (int a = 0; a < v; ++a) fo[(size_t)i * v + a] = f[(size_t)i * n + o + a];
+    if (c->fov) (void)hipFree(c->fov);
+    if (c->epart) (void)hipFree(c->epart);
+    c->fov = nullptr;
+    c->epart = nullptr;
+    ACES_HIP(hipMalloc(&c->fov, sizeof(double) * ov));
+    ACES_HIP(hipMalloc(&c->epart, sizeof(double) * (4 * oo + 8)));
+    ACES_HIP(hipMemcpyAsync(c->fov, fo.data(), sizeof(double) * ov, hipMemcpyHostToDevice, s));
+    ACES_HIP(hipStreamSynchronize(s));
   }
   c->eng.reset(new CcsdEngine(o, v, s));
   const int rc = c->eng->setup(d_f, ovov, oovv, ovoo, oooo, ovvv, c->lists->device_ptr(1, 233));
@@ -402,23 +497,39 @@ int aces_newt2(void) {
   return ACES_OK;
 }
 
-// CMPENG: nlist2 = 43 (T lists 44/46) or 60 (increment lists 61/63); T1 group nlist1 + 1.
-int aces_cmpeng(int nlist2, int nlist1, double* ecorr3, double* etot) {
+// CMPENG (libr/cmpeng.F:60-188): nlist2 = 43 (T lists 44/46) or 60 (increment
+// lists 61/63), T1 from list 90 group nlist1 + 1.  Closed shell (IUHF = 0):
+//   ecorr3[0] = E(AA) (+ the non-HF singles term sum t1 f), ecorr3[2] = E(AB),
+//   ecorr3[1] untouched (the reference does not set it for RHF),
+//   *etot = 2 ecorr3[0] + ecorr3[2], *etott2 the same with T2 in place of tau.
+// The AA pieces are evaluated from the AB lists (list 44 = ABTOAA(46) for RHF).
+int aces_cmpeng(int nlist2, int nlist1, double* ecorr3, double* etot, double* etott2) {
   AcesContext* c;
   ACES_CHECK(ctx(&c));
   ACES_REQUIRE(c->eng != nullptr, "CMPENG before GENINT");
+  ACES_REQUIRE(nlist2 == 43 || nlist2 == 60, "CMPENG: NLIST2 must be 43 (lists 44/46) or 60 (lists 61/63)");
+  ACES_REQUIRE(nlist1 == 0 || nlist1 == 2, "CMPENG: NLIST1 must be 0 (list 90 group 1) or 2 (group 3)");
   CcsdEngine& e = *c->eng;
-  double* d_e;
-  ACES_HIP(hipMalloc(&d_e, sizeof(double)));
-  ACES_CHECK(e.energy_dev(c->lists->device_ptr(nlist1 + 1, 90), c->lists->device_ptr(1, nlist2 + 3), d_e));
-  double h = 0.0;
-  ACES_HIP(hipMemcpyAsync(&h, d_e, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  ACES_HIP(hipStreamSynchronize(c->stream));
-  (void)hipFree(d_e);
-  *etot = h;
+  const int o = e.o, v = e.v;
+  const double* t1 = c->lists->device_ptr(nlist1 + 1, 90);
+  const double* t2 = c->lists->device_ptr(1, nlist2 + 3);
+  const double* w = c->lists->device_ptr(1, 16);
+  ACES_REQUIRE(t1 && t2 && w, "CMPENG: lists 90, 16 and the T2 list must exist");
+  hipStream_t s = c->stream;
+  double* out = c->epart + 4L * o * o;
+  hipLaunchKernelGGL(cmpeng_partial_kernel, dim3(o * o), dim3(256), 0, s, o, v, t1, t2, w, c->epart);
+  ACES_HIP(hipGetLastError());
+  hipLaunchKernelGGL(cmpeng_final_kernel, dim3(1), dim3(256), 0, s, o * o, o * v, c->epart, t1, c->fov, out);
+  ACES_HIP(hipGetLastError());
+  double h[5];
+  ACES_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, s));
+  ACES_HIP(hipStreamSynchronize(s));
+  const double eaa = h[2] + h[4];
+  if (etot) *etot = 2.0 * eaa + h[0];
+  if (etott2) *etott2 = 2.0 * (h[3] + h[4]) + h[1];
   if (ecorr3) {
-    ecorr3[0] = ecorr3[1] = 0.0;
-    ecorr3[2] = h;
+    ecorr3[0] = eaa;
+    ecorr3[2] = h[0];
   }
   return ACES_OK;
 }
@@ -464,21 +575,26 @@ int aces_dodiis0(int icycle, double tol, int* iconvg, double* stats) {
   return ACES_OK;
 }
 
-int aces_drmove(void) {
+// DRMOVE (libr/drmove.f): lists 61 -> 44 + nlist and 63 -> 46 + nlist (MVSYM,
+// every irrep), with sing1 list 90 group 3 -> list 90 + nlist group 1 (MVLIST).
+// nlist = 0 moves the increments into the T lists; nlist = 100 fills the Lambda
+// start lists 144/146/190 and leaves the T lists unchanged.
+int aces_drmove_lists(int nlist, int sing1) {
   AcesContext* c;
   ACES_CHECK(ctx(&c));
-  ACES_REQUIRE(c->eng != nullptr, "DRMOVE before GENINT");
-  CcsdEngine& e = *c->eng;
-  const long ov = (long)e.o * e.v;
-  hipStream_t s = c->stream;
-  ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 90), c->lists->device_ptr(3, 90), sizeof(double) * ov,
-                          hipMemcpyDeviceToDevice, s));
-  ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 46), c->lists->device_ptr(1, 63), sizeof(double) * ov * ov,
-                          hipMemcpyDeviceToDevice, s));
-  ACES_CHECK(abtoaa(e.o, e.v, c->lists->device_ptr(1, 46), c->lists->device_ptr(1, 44), s));
-  ACES_HIP(hipStreamSynchronize(s));
+  ACES_REQUIRE(nlist >= 0 && nlist + 90 <= 500, "DRMOVE: NLIST out of range");
+  ListStore& L = *c->lists;
+  const int moves[2][2] = {{61, 44 + nlist}, {63, 46 + nlist}};
+  for (const auto& mv : moves) {
+    const std::vector<int> g = L.groups(mv[0]);
+    ACES_REQUIRE(!g.empty(), "DRMOVE: increment list " + std::to_string(mv[0]) + " does not exist");
+    for (int grp : g) ACES_CHECK(L.move(grp, mv[0], grp, mv[1]));
+  }
+  if (sing1) ACES_CHECK(L.move(3, 90, 1, 90 + nlist));
   return ACES_OK;
 }
+
+int aces_drmove(void) { return aces_drmove_lists(0, 1); }
 
 // MP1 start written by xintprc (intprc/dfrmt2.F) into lists 46/44 and T1 = 0.
 int aces_vcc_mp1(void) {
@@ -516,22 +632,56 @@ void t1int2b_(i8*, i8*, i8*) {}
 void t1int1_(i8*, i8*, i8*, i8*) {}
 void initin_(i8*, i8*, i8*) {}
 void initsn_(i8*, i8*, i8*) {}
-void rnabij_(i8*, i8*, i8*, const char*, size_t) {}
+// RNABIJ (libr/rnabij.f) writes re-sorted copies of <Ab|Ij> ('W'), T2 ('T'), the
+// T2 increment ('R') or L2 ('L') in the ring orders (lists 17-22, 34-39); the
+// engine forms those orders inside its ring products, so these four are no-ops
+// here.  'C' (iterative MBPT from another list set) is not implemented.
+void rnabij_(i8*, i8*, i8*, const char* type, size_t len) {
+  const char t = (type && len > 0) ? type[0] : ' ';
+  if (t != 'W' && t != 'T' && t != 'R' && t != 'L') {
+    set_error(std::string("RNABIJ: re-sort type '") + t + "' is not implemented");
+    fatal(ACES_ERR_ARG, "RNABIJ");
+  }
+}
 void dre3en_(i8*, i8*, i8*, i8*) { fatal(aces_dre3en(), "DRE3EN"); }
+// SE4 is the fourth-order singles energy of finite-order MBPT (METHOD 3/4,
+// vcc/e4seng.F); for the CC methods the reference returns 0 as well.
 void e4s_(i8*, i8*, i8*, double* se4) {
   fatal(aces_e4s(), "E4S");
   if (se4) *se4 = 0.0;
 }
 void newt2_(i8*, i8*, i8*) { fatal(aces_newt2(), "NEWT2"); }
-void cmpeng_(i8*, i8*, i8* nlist2, i8* nlist1, double* ecorr, double* etot, double* etott2, i8*, i8* iprint) {
-  fatal(aces_cmpeng((int)*nlist2, (int)*nlist1, ecorr, etot), "CMPENG");
-  if (etott2) *etott2 = *etot;
-  // output line of libr/cmpeng.F (format 81)
-  if (iprint && *iprint) printf("   The total correlation energy is %15.12f a.u.\n", *etot);
+void cmpeng_(i8*, i8*, i8* nlist2, i8* nlist1, double* ecorr, double* etot, double* etott2, i8* iuhf,
+             i8* iprint) {
+  if (iuhf && *iuhf != 0) {
+    set_error("CMPENG: only closed-shell (IUHF = 0) lists are implemented");
+    fatal(ACES_ERR_ARG, "CMPENG");
+  }
+  fatal(aces_cmpeng((int)*nlist2, (int)*nlist1, ecorr, etot, etott2), "CMPENG");
+  // output lines of libr/cmpeng.F (formats 80 and 81)
+  if (iprint && *iprint) {
+    printf("   The AA contribution to the correlation energy is: %12.7f a.u.\n", ecorr[0]);
+    printf("   The AB contribution to the correlation energy is: %12.7f a.u.\n", ecorr[2]);
+    printf("   The total correlation energy is %15.12f a.u.\n", *etot);
+  }
   fflush(stdout);
 }
-void dodiis0_(double*, i8*, i8*, i8*, i8* icycle, i8* iconvg, i8* icontl, i8*, i8*, i8*, i8*, i8*, i8*, i8*,
-              i8*, const char*, double*, size_t) {
+// DODIIS0 with the argument list of vcc/dodiis0.F:1-4.  xvcc's call is
+// DODIIS0(...,1,ICYCLE,ICONVG,ICONTL,SING1,44,61,90,0,90,2,70,'     ',DAMP)
+// (vcc/vcc.F:1073-1075); that is the set implemented.
+void dodiis0_(double*, i8*, i8* iuhf, i8* irrepx, i8* icycle, i8* iconvg, i8* icontl, i8*, i8* listt20,
+              i8* listt21, i8* listt10, i8* ioff0, i8* listt11, i8* ioff1, i8* lsterr, const char*,
+              double* damp, size_t) {
+  const bool ok = *iuhf == 0 && *irrepx == 1 && *listt20 == 44 && *listt21 == 61 && *listt10 == 90 &&
+                  *ioff0 == 0 && *listt11 == 90 && *ioff1 == 2 && *lsterr == 70;
+  if (!ok) {
+    set_error("DODIIS0: only the closed-shell amplitude lists (44, 61, 90, 0, 90, 2, 70) are implemented");
+    fatal(ACES_ERR_ARG, "DODIIS0");
+  }
+  if (*damp != 1.0) {
+    set_error("DODIIS0: DAMP_PARAMETER other than 1 is not implemented");
+    fatal(ACES_ERR_ARG, "DODIIS0");
+  }
   int conv = 1;
   double st[8];
   fatal(aces_dodiis0((int)*icycle, std::pow(10.0, -(double)*icontl), &conv, st), "DODIIS0");
@@ -543,6 +693,12 @@ void dodiis0_(double*, i8*, i8*, i8*, i8* icycle, i8* iconvg, i8* icontl, i8*, i
   if (conv == 0) printf("  Amplitude equations converged in %5ld iterations.\n", (long)*icycle);
   fflush(stdout);
 }
-void drmove_(i8*, i8*, i8*, i8*, i8*) { fatal(aces_drmove(), "DRMOVE"); }
+void drmove_(i8*, i8*, i8* iuhf, i8* nlist, i8* sing1) {
+  if (*iuhf != 0) {
+    set_error("DRMOVE: only closed-shell (IUHF = 0) lists are implemented");
+    fatal(ACES_ERR_ARG, "DRMOVE");
+  }
+  fatal(aces_drmove_lists((int)*nlist, *sing1 != 0), "DRMOVE");
+}
 
 }  // extern "C"

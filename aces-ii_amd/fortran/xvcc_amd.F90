@@ -2,7 +2,12 @@
 ! linked against the MI355X library.  Compile with -fdefault-integer-8 like the
 ! reference so INTEGER arguments are 8 bytes.
 !
-! Usage: xvcc_amd <input> [cc_conv] [cc_maxcyc] [cc_exporder]
+! Usage: xvcc_amd <input> [cc_conv] [cc_maxcyc] [cc_exporder] [mode]
+! mode 'lambda': after convergence move the amplitudes into the Lambda start lists
+!               with DRMOVE(...,100,SING1) (vcc/vcc.F:1138) and print the sums of
+!               squares of lists 44, 46, 90 and 144, 146, 190;
+! mode 'xlambda_diis': call DODIIS0 with xlambda's list set (lambda/vlamcc.F:812-814),
+!               which the library does not implement (it must end with status 1).
 ! Input (stream, little endian): o, v, nlists (int64); per list: list, typel,
 ! typer, nrows, ncols (int64) and nrows*ncols doubles (column major); then the
 ! n = o + v orbital energies (record SCFEVALA).
@@ -14,7 +19,7 @@ program xvcc_amd
   logical :: sing1, do_hbar
   double precision :: ecorr(3), etot, etott2, edummy, damp
   double precision, allocatable :: buf(:)
-  character(len=256) :: fname, arg
+  character(len=256) :: fname, arg, mode
   integer(c_int), target :: pop(8), vrt(8)
 
   call get_command_argument(1, fname)
@@ -33,6 +38,8 @@ program xvcc_amd
     call get_command_argument(4, arg)
     read(arg, *) iorder
   end if
+  mode = ''
+  if (command_argument_count() >= 5) call get_command_argument(5, mode)
   maxcor = 1
   iuhf = 0
   inttyp = 2
@@ -83,6 +90,10 @@ program xvcc_amd
     call e4s(icore, maxcor, iuhf, edummy)
     call newt2(icore, maxcor, iuhf)
     call cmpeng(icore, maxcor, 60, 2, ecorr, etot, etott2, iuhf, 1)
+    if (trim(mode) == 'xlambda_diis') then
+      call dodiis0(icore, maxcor, iuhf, 1, icycle, iconvg, icontl, sing1, 144, 61, 190, 0, 90, 2, 70, &
+                   '     ', damp)
+    end if
     call dodiis0(icore, maxcor, iuhf, 1, icycle, iconvg, icontl, sing1, 44, 61, 90, 0, 90, 2, 70, &
                  '     ', damp)
     if (iconvg /= 0) then
@@ -91,6 +102,10 @@ program xvcc_amd
     else
       call cmpeng(icore, maxcor, 43, 0, ecorr, etot, etott2, iuhf, 1)
       write(6, '(a)') '  The CC iterations have converged.'
+      if (trim(mode) == 'lambda') then
+        call drmove(icore, maxcor, iuhf, 100, sing1)
+        call print_norms()
+      end if
       exit
     end if
   end do
@@ -99,4 +114,25 @@ program xvcc_amd
     write(6, '(a)') '@VCC: The Coupled-Cluster equations did not converge!'
     stop 1
   end if
+
+contains
+
+  ! Sum of squares of one list (group 1) read back through GETLST.
+  subroutine list_norm(lst, nrow, ncol)
+    integer, intent(in) :: lst, nrow, ncol
+    double precision, allocatable :: w(:)
+    allocate(w(nrow * ncol))
+    call getlst(w, 1, ncol, 1, 1, lst)
+    write(6, '(a,i4,a,es24.16)') '  List ', lst, ' sum of squares ', sum(w * w)
+    deallocate(w)
+  end subroutine list_norm
+
+  subroutine print_norms()
+    call list_norm(44, v * (v - 1) / 2, o * (o - 1) / 2)
+    call list_norm(46, v * v, o * o)
+    call list_norm(90, o * v, 1)
+    call list_norm(144, v * (v - 1) / 2, o * (o - 1) / 2)
+    call list_norm(146, v * v, o * o)
+    call list_norm(190, o * v, 1)
+  end subroutine print_norms
 end program xvcc_amd

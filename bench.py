@@ -64,36 +64,78 @@ def f_exec(o, v):
     return ladder, rings
 
 
-def cpu_baseline(o, v, seed, budget_s=20.0):
-    """Time the oracle's closed-shell increments on the host on a bounded sample.
+def f_ref_classes(o, v):
+    """Reference-equivalent flops (2mnk of the reference's XGEMM shapes, RHF C1)
+    of the three product classes the engine times separately:
+      ladder  DRLAD/LADAB1 particle ladder on the A<=B list (o^2 v^3 (v+1)), the
+              T1INW1 product that the engine folds into its rows (2 o^3 v^3), and
+              the hole ladder + QUAD1 W(mnij) (4 o^4 v^2)
+      wbuild  DWMBEJ, two (ov)^3 products (4 o^3 v^3)
+      ring    DRRNG, two (ov)^3 products (4 o^3 v^3)"""
+    return {'ladder': o * o * v ** 3 * (v + 1) + 2 * o ** 3 * v ** 3 + 4 * o ** 4 * v ** 2,
+            'wbuild': 4 * o ** 3 * v ** 3, 'ring': 4 * o ** 3 * v ** 3}
 
-    The sample is the same workload family at a reduced size (o_s, v_s) chosen
-    to fit the time budget; seconds per cycle are scaled to o=48, v=320 by the
-    ratio of reference-equivalent flops.
-    """
+
+def cpu_baseline(o, v, seed):
+    """The reference algorithm's fp64 work per CC cycle at the full (o, v), timed
+    on the host with numpy/BLAS on a bounded sample of each product class.
+
+    The reference's CPU cycle is its XGEMM calls (BLAS dgemm) plus O(o^2 v^2)
+    list passes.  Each class is timed at its real shape with one dimension
+    sampled, and extrapolated linearly in that dimension (GEMM time is linear in
+    each dimension at these sizes):
+      particle ladder  <Ab|Cd>(A<=B, Cd) x tau(Cd, Ij): 768 of v(v+1)/2 rows
+      (ov)^3 products  DWMBEJ x2, DRRNG x2, T1INW1: 1536 of ov columns, x5
+      hole ladder      o^2 x v^2 x o^2, twice (W(mnij) and the ladder)
+      o^2 v^3 products T1RING, QUAD3, FEACONT, T1INT2B, T2T1AB1 (~20 o^2 v^3 flops):
+                       one (o^2 v) x v x v product, x10
+      list passes      NEWT2, CMPENG, DODIIS0 (6 vectors), SYMETRIZE_T2, DRMOVE:
+                       one a*x+y pass over o^2 v^2, x16
+    Operand values do not change BLAS time; random operands are used."""
     import numpy as np
-    from oracle import rccsd
-    from aces2 import synth
-    threads = os.environ.get('OMP_NUM_THREADS') or str(min(16, os.cpu_count() or 1))
-    best = None
-    for (os_, vs_) in [(8, 48), (12, 72), (16, 96)]:
-        eps, B = synth.draw(os_, vs_, seed, sigma=synth.scaled_sigma(os_, vs_))
-        blk, eri = synth.blocks_numpy(eps, B, os_)
-        g = rccsd.blocks(eri, os_)
-        t1 = np.zeros((os_, vs_))
-        _, t2 = (None, g['ovov'].transpose(0, 2, 1, 3) / rccsd.denominators(blk['fock'], os_)[1])
-        rccsd.increments(t1, t2, blk['fock'], g, os_)
-        t0 = time.perf_counter()
-        rccsd.increments(t1 + 1e-3, t2, blk['fock'], g, os_)
-        dt = time.perf_counter() - t0
-        best = (os_, vs_, dt)
-        if dt * (f_ref(os_ + 4, vs_ + 24) / f_ref(os_, vs_)) > budget_s:
-            break
-    os_, vs_, dt = best
-    scale = f_ref(o, v) / f_ref(os_, vs_)
-    return {'value': dt * scale, 'unit': 's/iter', 'cores': int(threads), 'kind': 'port',
-            'sample': 'oracle.rccsd increments (numpy/BLAS, %s threads) at o=%d, v=%d: %.3f s, scaled x%.1f '
-                      'by reference-equivalent flops to o=%d, v=%d' % (threads, os_, vs_, dt, scale, o, v)}
+    threads = int(os.environ.get('OMP_NUM_THREADS') or min(16, os.cpu_count() or 1))
+    rng = np.random.default_rng(seed)
+    ov, oo, vv, nvp = o * v, o * o, v * v, v * (v + 1) // 2
+
+    def timed(fn, reps=1):
+        fn()
+        best = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best
+
+    parts = {}
+    rows = min(768, nvp)
+    A = rng.standard_normal((rows, vv))
+    T = rng.standard_normal((vv, oo))
+    parts['particle_ladder'] = timed(lambda: A @ T) * nvp / rows
+    Y = rng.standard_normal((oo, vv))
+    parts['hole_ladder'] = 2 * timed(lambda: Y @ T, reps=1)
+    del A, Y
+    cols = min(1536, ov)
+    W = rng.standard_normal((ov, ov))
+    X = rng.standard_normal((ov, cols))
+    parts['ov3_products'] = 5 * timed(lambda: W @ X) * ov / cols
+    del X
+    Z = rng.standard_normal((oo * v, v))
+    Q = rng.standard_normal((v, v))
+    parts['o2v3_products'] = 10 * timed(lambda: Z @ Q)
+    del Z, Q
+    x = W.reshape(-1)[:oo * vv]
+    y = np.empty_like(x)
+    parts['list_passes'] = 16 * timed(lambda: np.multiply(x, 0.5, out=y))
+    del W, T
+    total = sum(parts.values())
+    return {'value': total, 'unit': 's/iter', 'cores': threads, 'kind': 'port',
+            'sample': 'the reference algorithm\'s fp64 products per CC cycle at o=%d, v=%d on the host (numpy/'
+                      'OpenBLAS, %d threads), each class at its real shape with one dimension sampled and '
+                      'extrapolated linearly in it: particle ladder %d of %d rows, (ov)^3 products %d of %d '
+                      'columns (x5), hole ladder in full (x2), one o^2v^3 product (x10), one o^2v^2 list pass '
+                      '(x16); seconds per class %s' % (o, v, threads, rows, nvp, cols, ov,
+                                                       {k: round(t, 3) for k, t in parts.items()})}
 
 
 def main():
@@ -179,14 +221,25 @@ def main():
     nl = sum(p['launches'] for p in prof.values())
     execf = 3.0 if args.prec == 'bf16x3' else 1.0
     peak = {'bf16x3': BF16_PEAK, 'fp32': FP32_PEAK, 'fp64': FP64_PEAK}[args.prec]
-    achieved = execf * fl / (ms * 1e-3)
-    roofline = {'bound': 'mfma', 'kernel': {'bf16x3': 'gemm_bf16x3_kernel', 'fp32': 'gemm16_kernel<float>',
-                                             'fp64': 'gemm16_kernel<double>'}[args.prec],
+    # achieved: reference-equivalent flops of these products per second (their
+    # F_ref share); executed MFMA flops (3 x 2mnk for bf16x3) give mfma_pct_peak
+    fr = f_ref_classes(o, v)
+    fref = nprof * sum(fr.values())
+    achieved = fref / (ms * 1e-3)
+    executed = execf * fl / (ms * 1e-3)
+    roofline = {'bound': 'mfma', 'kernel': {'bf16x3': 'gemm_bf16x3_dl_kernel<4,8,2>', 'fp32': 'gemm16_kernel<float>',
+                                             'fp64': 'gemm_f64_dl_kernel'}[args.prec],
                 'achieved': achieved / 1e12, 'peak': peak / 1e12, 'unit': 'TFLOP/s',
                 'frac': achieved / peak, 'traffic': None,
+                'flops_basis': 'reference-equivalent flops (2mnk of the reference XGEMM shapes) of the particle '
+                               'and hole ladders, W(mbej) build and ring products: %.4g per cycle' % (fref / nprof),
+                'algorithmic_TFLOPs': fl / (ms * 1e-3) / 1e12,
+                'mfma_pct_peak': 100.0 * executed / peak, 'executed_TFLOPs': executed / 1e12,
                 'avg_launch_ms': ms / nl, 'launches_per_cycle': nl / nprof,
                 'gemm_share_of_cycle': (ms / nprof) / (s_per_cycle * 1e3),
-                'per_class': {k: {'TFLOP/s': execf * p['flops'] / max(p['ms'], 1e-9) / 1e9, 'ms': p['ms'] / nprof}
+                'per_class': {k: {'ref_TFLOP/s': nprof * fr[k] / max(p['ms'], 1e-9) / 1e9,
+                                  'executed_TFLOP/s': execf * p['flops'] / max(p['ms'], 1e-9) / 1e9,
+                                  'ms': p['ms'] / nprof}
                               for k, p in prof.items()},
                 'fp64_other': {'TFLOP/s': prof_all['other']['flops'] / max(prof_all['other']['ms'], 1e-9) / 1e9,
                                'ms': prof_all['other']['ms'] / nprof,
@@ -208,9 +261,11 @@ def main():
     fl64 = sum(p['flops'] for p in big.values())
     ms64all = ms64 + p64['other']['ms']
     fl64all = fl64 + p64['other']['flops']
-    fp64_roofline = {'bound': 'mfma', 'kernel': 'gemm16_kernel<double> / gemm_f64_dl_kernel',
+    fp64_roofline = {'bound': 'mfma', 'kernel': 'gemm_f64_dl_kernel',
                      'achieved': fl64 / (ms64 * 1e-3) / 1e12,
                      'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s', 'frac': fl64 / (ms64 * 1e-3) / FP64_PEAK,
+                     'flops_basis': 'executed 2mnk (fp64 MFMA utilisation)',
+                     'ref_equivalent_TFLOPs': sum(fr.values()) / (ms64 * 1e-3) / 1e12,
                      'scope': 'ladder + W build + ring contractions of one fp64 cycle',
                      'all_fp64_gemms': {'TFLOP/s': fl64all / (ms64all * 1e-3) / 1e12,
                                         'frac': fl64all / (ms64all * 1e-3) / FP64_PEAK, 'ms': ms64all},
@@ -484,10 +539,12 @@ def bench_lambda(args):
     pm = cc.profile(reset=True)
     cc.set_profiling(False)
     lp = pm['adjoint_lp']
-    achieved = 3.0 * lp['flops'] / (lp['ms'] * 1e-3)
-    roofline = {'bound': 'mfma', 'kernel': 'gemm_bf16x3_kernel (reverse-pass products of the mixed cycles)',
+    achieved = lp['flops'] / (lp['ms'] * 1e-3)
+    roofline = {'bound': 'mfma', 'kernel': 'gemm_bf16x3_dl_kernel (reverse-pass products of the mixed cycles)',
                 'achieved': achieved / 1e12, 'peak': BF16_PEAK / 1e12, 'unit': 'TFLOP/s',
-                'frac': achieved / BF16_PEAK, 'traffic': None, 'avg_launch_ms': lp['ms'] / lp['launches'],
+                'frac': achieved / BF16_PEAK, 'traffic': None,
+                'flops_basis': 'algorithmic 2mnk of the reverse-pass products',
+                'mfma_pct_peak': 100.0 * 3.0 * achieved / BF16_PEAK, 'avg_launch_ms': lp['ms'] / lp['launches'],
                 'launches_per_cycle': lp['launches'] / hp['cycles'],
                 'fp64_anchor_ms_per_cycle': pm['adjoint']['ms'] / hp['cycles']}
     log('roofline', json.dumps(roofline))
@@ -654,8 +711,10 @@ def bench_sym(args):
     execf = 3.0 if prec == 'bf16x3' else 1.0
     peak = BF16_PEAK if prec == 'bf16x3' else FP32_PEAK
     roofline = {'bound': 'mfma', 'kernel': 'grouped %s GEMM (8 irrep blocks per launch)' % prec,
-                'achieved': execf * fl / (ms * 1e-3) / 1e12, 'peak': peak / 1e12, 'unit': 'TFLOP/s',
-                'frac': execf * fl / (ms * 1e-3) / peak, 'traffic': None, 'avg_launch_ms': ms / max(nl, 1),
+                'achieved': fl / (ms * 1e-3) / 1e12, 'peak': peak / 1e12, 'unit': 'TFLOP/s',
+                'frac': fl / (ms * 1e-3) / peak, 'traffic': None,
+                'flops_basis': 'algorithmic 2mnk of the irrep-block products',
+                'mfma_pct_peak': 100.0 * execf * fl / (ms * 1e-3) / peak, 'avg_launch_ms': ms / max(nl, 1),
                 'launches_per_cycle': nl / 2, 'gemm_share_of_cycle': (ms / 2) / (s_mixed * 1e3),
                 'per_class': {k: {'TFLOP/s': execf * p['flops'] / max(p['ms'], 1e-9) / 1e9, 'ms': p['ms'] / 2}
                               for k, p in heavy.items()},

@@ -234,15 +234,25 @@ int aces_getrec(int iflag, const char* name, long* nwords, void* dest);
  *   aces_dre3en            DRE3EN (vcc/dre3en.F): T2 increment -> lists 63/61
  *   aces_e4s               E4S (vcc/e4s.F): new T1 -> list 90 group 3
  *   aces_newt2             NEWT2 (libr2/newt2.F): list 63 <- 63/D, 61 <- ABTOAA
- *   aces_cmpeng            CMPENG (libr/cmpeng.F) on lists nlist2+1..+3, T1 g nlist1+1
+ *   aces_cmpeng            CMPENG (libr/cmpeng.F:60-188) on lists nlist2+1..+3 and
+ *                          T1 group nlist1+1: ecorr3[0] = E(AA), ecorr3[2] = E(AB)
+ *                          (ecorr3[1] untouched, as for RHF there), *etot =
+ *                          2 E(AA) + E(AB), *etott2 = the same with T2 for tau
+ *                          (any output pointer may be NULL)
  *   aces_dodiis0           DODIIS0 (vcc/dodiis0.F); *iconvg = 0 when converged
- *   aces_drmove            DRMOVE (libr/drmove.f): T lists <- increment lists
+ *   aces_drmove            DRMOVE(..., 0, .TRUE.) (libr/drmove.f): T lists <- increment lists
+ *   aces_drmove_lists      DRMOVE(..., nlist, sing1): lists 61/63 -> 44+nlist/46+nlist,
+ *                          with sing1 also list 90 group 3 -> list 90+nlist group 1
  * Integral input lists (RHF, C1): 13 <Mn|Ij> (14,14), 16 <Ab|Ij> (13,14),
  * 10 <Ij|Ka> (14,18), 30 <Ab|Ci> (13,11), 233 <Ab|Cd> (5,13) and
  * 23 <AJ||BI> (9,9); records SCFEVALA (orbital energies) or FOCKMOA (MO Fock).
  * Fortran spellings with the reference's argument lists: genint_, feacont_,
  * fmicont_, fmecont_, t1int2a_, t1int2b_, t1int1_, initin_, initsn_, rnabij_,
- * dre3en_, e4s_, newt2_, cmpeng_, dodiis0_, drmove_.
+ * dre3en_, e4s_, newt2_, cmpeng_, dodiis0_, drmove_.  Argument values they do
+ * not implement (open-shell IUHF, DODIIS0 list sets other than xvcc's
+ * 44,61,90,0,90,2,70, DAMP_PARAMETER != 1, RNABIJ type 'C') end the process
+ * with status 1 after a message on stderr, the reference's fatal path
+ * (aces_exit(1)); e4s_ returns SE4 = 0, as the reference does for CC methods.
  * ---------------------------------------------------------------------- */
 int aces_vcc_mp1(void);
 int aces_vcc_set_diis_order(int order);
@@ -253,9 +263,10 @@ int aces_genint(void);
 int aces_dre3en(void);
 int aces_e4s(void);
 int aces_newt2(void);
-int aces_cmpeng(int nlist2, int nlist1, double* ecorr3, double* etot);
+int aces_cmpeng(int nlist2, int nlist1, double* ecorr3, double* etot, double* etott2);
 int aces_dodiis0(int icycle, double tol, int* iconvg, double* stats);
 int aces_drmove(void);
+int aces_drmove_lists(int nlist, int sing1);
 
 #ifdef __cplusplus
 }

@@ -136,3 +136,31 @@ def energy(t1, t2, f, g, o):
     e = 2 * np.einsum('ia,ia->', f[:o, o:], t1)
     e += np.einsum('iajb,ijab->', L, tau)
     return float(e)
+
+
+def energy_parts(t1, t2, f, g, o):
+    """CMPENG's spin-case pieces (``libr/cmpeng.F:99-180``, ``libr/tener.F``, ``libr/ftau.f``).
+
+    Returns (E_AA, E_AB, E_lin) with
+      E_AA  = sum_{A<B, I<J} tau(AB,IJ) <AB||IJ> + sum f(I,A) t(I,A)   (printed 'AA')
+      E_AB  = sum tau(Ab,Ij) <Ab|Ij>                                   (printed 'AB')
+      E_lin = ETOTT2, the same total 2 E_AA + E_AB with T2 in place of tau,
+    where tau(AB,IJ) = T(AB,IJ) + t(I,A) t(J,B) - t(J,A) t(I,B), T(AB,IJ) =
+    T(Ab,Ij) - T(Ab,Ji) (ABTOAA) and <AB||IJ> = (AI|BJ) - (AJ|BI).
+    """
+    ovov = g['ovov']
+    iu_o = np.triu_indices(o, 1)
+    v = t1.shape[1]
+    iu_v = np.triu_indices(v, 1)
+    W = ovov.transpose(0, 2, 1, 3)                       # <ab|ij> as [i,j,a,b] = (ia|jb)
+    Waa = W - W.transpose(0, 1, 3, 2)
+    Taa = t2 - t2.transpose(1, 0, 2, 3)
+    ss = np.einsum('ia,jb->ijab', t1, t1)
+    tau_aa = Taa + ss - ss.transpose(1, 0, 2, 3)
+    pick = lambda x: x[iu_o[0], iu_o[1]][:, iu_v[0], iu_v[1]]
+    e_aa_tau = float(np.sum(pick(tau_aa) * pick(Waa)))
+    e_aa_lin = float(np.sum(pick(Taa) * pick(Waa)))
+    e_ab_tau = float(np.sum((t2 + ss) * W))
+    e_ab_lin = float(np.sum(t2 * W))
+    esing = float(np.sum(f[:o, o:] * t1))
+    return e_aa_tau + esing, e_ab_tau, 2.0 * (e_aa_lin + esing) + e_ab_lin

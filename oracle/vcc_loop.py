@@ -12,6 +12,7 @@ Order of one cycle follows ``vcc/vcc.F:798-1437``:
   DRMOVE                         T <- increment lists  (``libr/drmove.f``) unless converged
   on convergence: CMPENG(43,0) on the T of that cycle -> TOTENERG (``vcc/vcc.F:1100``,
   ``vcc/finish.F:140``).
+``hist['energy_aa_ab']`` holds CMPENG's printed (AA, AB) pair of every energy line.
 
 DIIS vector (RHF, C1) = [T1 (list 90, AI order), T2(Ab,Ij) (list 46, Ab rows,
 Ij columns, column major), T2(A<B,I<J) (list 44, packed)] -- ``vcc/diislst.F:20-29``
@@ -141,7 +142,9 @@ def run(f, g, o, maxcyc=50, conv=7, order=6, start='mp1', increments=None, energ
             t1, t2 = mp1_start(f, g, o)
         else:
             t1, t2 = np.zeros((o, v)), np.zeros((o, o, v, v))
-    hist = {'energy': [e_fn(t1, t2)], 'resmiter': [], 'resmdiis': [], 'converged': False}
+    parts = lambda a, b: rccsd.energy_parts(a, b, f, g, o)[:2] if energy is None else (None, None)
+    hist = {'energy': [e_fn(t1, t2)], 'resmiter': [], 'resmdiis': [], 'converged': False,
+            'energy_aa_ab': [parts(t1, t2)]}
     diis = DIIS(order)
     tol = 10.0 ** (-conv)
     for icycle in range(1, maxcyc + 1):
@@ -149,6 +152,7 @@ def run(f, g, o, maxcyc=50, conv=7, order=6, start='mp1', increments=None, energ
         t1n = inc1 / d1
         t2n = inc2 / d2
         hist['energy'].append(e_fn(t1n, t2n))
+        hist['energy_aa_ab'].append(parts(t1n, t2n))
         t2 = symmetrize_t2(t2)
         t2n = symmetrize_t2(t2n)
         told = diis_vector(t1, t2)
@@ -161,6 +165,7 @@ def run(f, g, o, maxcyc=50, conv=7, order=6, start='mp1', increments=None, energ
         if abs(resmiter) < tol:
             hist['converged'] = True
             hist['final_energy'] = e_fn(t1, t2)
+            hist['energy_aa_ab'].append(parts(t1, t2))
             hist['cycles'] = icycle
             break
         if newvec is not None:

@@ -3,7 +3,8 @@
 Run in the development container only (reads /root/reference).  Everything
 stored is data: the O STO-3G basis of test/KEEP/GENBAS, the geometry implied
 by the printed nuclear repulsion energy, the SCF occupation by D2h irrep, and
-the per-cycle CC energies and largest residual elements printed in
+the per-cycle CC energies (total and the AA/AB spin-case lines of CMPENG) and
+largest residual elements printed in
 test/OUTPUT/out.zmat.015a (first geometry, MP1 start, CC_EXPORDER = 5,
 CC_CONV = 7), the xlambda trace that follows (largest residual and DIIS
 residual elements per Lambda cycle, cycle count) and the xdens natural orbital
@@ -30,11 +31,14 @@ def main():
     i0 = next(k for k, s in enumerate(lines) if 'E(SCF)=' in s)
     escf = float(lines[i0].split()[1])
     i1 = next(k for k, s in enumerate(lines) if 'Amplitude equations converged' in s)
-    energies, resid = [], []
+    energies, resid, eaa, eab = [], [], [], []
     for s in lines[i0:i1 + 6]:
         m = re.search(r'The total correlation energy is\s+(\S+) a\.u\.', s)
         if m:
             energies.append(float(m.group(1)))
+        m = re.search(r'The (AA|AB) contribution to the correlation energy is:\s+(\S+) a\.u\.', s)
+        if m:
+            (eaa if m.group(1) == 'AA' else eab).append(float(m.group(2)))
         m = re.search(r'Largest element of residual vector :\s+(\S+)\.$', s)
         if m:
             resid.append(float(m.group(1)))
@@ -72,6 +76,8 @@ def main():
             'cc_exporder': 5, 'cc_conv': 7,
             'mp2_and_cycle_energies': energies[:cycles + 1],
             'final_energy': energies[cycles + 1],
+            'aa_contributions': eaa[:cycles + 2],
+            'ab_contributions': eab[:cycles + 2],
             'resmiter': resid[:cycles],
             'cycles': cycles,
             'lambda_resmiter': lres[:lcycles],

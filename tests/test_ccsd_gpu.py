@@ -165,3 +165,17 @@ def test_refine_anchor_cycle_is_fp64():
     print('refine err %.3e bf16x3 err %.3e' % (err_r, err_b), flush=True)
     assert err_r < 0.05 * err_b
     cc.close()
+
+
+def test_run_raises_not_converged():
+    """Too few cycles: the host loop raises (the reference exits with status 1,
+    vcc/vcc.F:1439-1441) after running exactly maxcyc cycles."""
+    from aces2.ccsd import CCSD, NotConverged
+    print('case not converged', flush=True)
+    blk, eri = _system(4, 13, 27)
+    cc = CCSD(4, 13, blk)
+    with pytest.raises(NotConverged):
+        cc.run(maxcyc=2, conv=10)
+    h = cc.run(maxcyc=2, conv=10, raise_on_fail=False)
+    assert not h['converged'] and len(h['resmiter']) == 2
+    cc.close()

@@ -46,3 +46,59 @@ def test_fortran_host_matches_oracle(tmp_path):
     np.testing.assert_allclose(res, ref['resmiter'], rtol=1e-6, atol=1e-13)
     assert abs(eng[-1] - ref['final_energy']) < 1e-10
     assert 'Amplitude equations converged' in out.stdout
+
+
+def _small_input(tmp_path, seed=42):
+    from aces2 import synth
+    o, v = 4, 13
+    eps, B = synth.draw(o, v, seed, sigma=synth.scaled_sigma(o, v))
+    blk, eri = synth.blocks_numpy(eps, B, o)
+    inp = tmp_path / 'lists.bin'
+    write_input(str(inp), blk, o, v)
+    return str(inp), blk, eri, o, v
+
+
+@pytest.mark.gpu
+def test_fortran_host_nonconvergence_exit(tmp_path):
+    """Too few cycles: the reference's exit path (vcc/vcc.F:1439-1441), status 1."""
+    assert os.path.exists(EXE), 'Fortran driver not built (run __graft_entry__.build())'
+    print('case fortran nonconvergence', flush=True)
+    inp, blk, eri, o, v = _small_input(tmp_path)
+    out = subprocess.run([EXE, inp, '10', '2', '6'], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 1, out.stdout + out.stderr
+    assert '@VCC: The Coupled-Cluster equations did not converge!' in out.stdout
+    assert len(re.findall(r'Largest element of residual vector', out.stdout)) == 2
+
+
+@pytest.mark.gpu
+def test_fortran_host_drmove_lambda_lists(tmp_path):
+    """DRMOVE(...,100,SING1) after convergence fills 144/146/190 from the increment
+    lists and leaves 44/46/90 as they were (the converged T)."""
+    assert os.path.exists(EXE), 'Fortran driver not built (run __graft_entry__.build())'
+    print('case fortran drmove 100', flush=True)
+    inp, blk, eri, o, v = _small_input(tmp_path)
+    ref = vcc_loop.run(blk['fock'], rccsd.blocks(eri, o), o, maxcyc=60, conv=9, order=6)
+    out = subprocess.run([EXE, inp, '9', '60', '6', 'lambda'], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    norms = {int(a): float(b) for a, b in re.findall(r'List\s+(\d+) sum of squares\s+(\S+)', out.stdout)}
+    t1, t2 = ref['t1'], ref['t2']
+    aa = t2 - t2.transpose(1, 0, 2, 3)
+    assert abs(norms[46] - np.sum(t2 * t2)) < 1e-9 * np.sum(t2 * t2)
+    assert abs(norms[44] - 0.25 * np.sum(aa * aa)) < 1e-9 * np.sum(aa * aa)
+    assert abs(norms[90] - np.sum(t1 * t1)) <= 1e-9 * max(np.sum(t1 * t1), 1e-30)
+    # the last increment lists hold the converged cycle's update: close to T, not equal
+    assert abs(norms[146] - norms[46]) < 1e-6 * norms[46]
+    assert abs(norms[144] - norms[44]) < 1e-6 * norms[44]
+    assert abs(norms[190] - norms[90]) <= 1e-6 * max(norms[90], 1e-30) + 1e-20
+
+
+@pytest.mark.gpu
+def test_fortran_host_rejects_xlambda_diis_lists(tmp_path):
+    """DODIIS0 with xlambda's lists (144, 61, 190, ...) is not implemented: the
+    library ends the process through the fatal path (status 1, message on stderr)."""
+    assert os.path.exists(EXE), 'Fortran driver not built (run __graft_entry__.build())'
+    print('case fortran xlambda dodiis0', flush=True)
+    inp, blk, eri, o, v = _small_input(tmp_path)
+    out = subprocess.run([EXE, inp, '9', '60', '6', 'xlambda_diis'], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 1, out.stdout + out.stderr
+    assert '@DODIIS0' in out.stderr and 'only the closed-shell amplitude lists' in out.stderr

@@ -52,6 +52,17 @@ def test_oracle_reproduces_reference_cc_trace(first_geometry):
     assert abs(h['final_energy'] - g['final_energy']) < 1e-11
 
 
+def test_oracle_reproduces_reference_cmpeng_lines(first_geometry):
+    """CMPENG's 'AA' and 'AB' lines of every printed energy (F12.7 in the output)."""
+    g, enuc, escf, F, eri = first_geometry
+    o = FIX['nocc']
+    h = vcc_loop.run(F, rccsd.blocks(eri, o), o, maxcyc=50, conv=g['cc_conv'], order=g['cc_exporder'])
+    aa = [x[0] for x in h['energy_aa_ab']]
+    ab = [x[1] for x in h['energy_aa_ab']]
+    np.testing.assert_allclose(aa, g['aa_contributions'], rtol=0, atol=5.001e-8)
+    np.testing.assert_allclose(ab, g['ab_contributions'], rtol=0, atol=5.001e-8)
+
+
 def test_final_geometry_test_dat():
     """TEST.DAT SCFENEG to 1e-9; TOTENERG to the convergence level of the reference run."""
     fg = FIX['final_geometry']
@@ -108,6 +119,11 @@ def test_fortran_host_reproduces_reference_output(first_geometry, tmp_path):
     np.testing.assert_allclose(res, g['resmiter'], rtol=3e-7, atol=0)
     assert abs(eng[-1] - g['final_energy']) < 1e-11
     assert 'Amplitude equations converged in    12 iterations.' in out.stdout
+    # CMPENG's spin-case lines (libr/cmpeng.F format 80), printed to 7 decimals
+    aa = [float(x) for x in re.findall(r'The AA contribution to the correlation energy is:\s+(\S+) a\.u\.', out.stdout)]
+    ab = [float(x) for x in re.findall(r'The AB contribution to the correlation energy is:\s+(\S+) a\.u\.', out.stdout)]
+    np.testing.assert_allclose(aa, g['aa_contributions'], rtol=0, atol=1.001e-7)
+    np.testing.assert_allclose(ab, g['ab_contributions'], rtol=0, atol=1.001e-7)
 
 
 def _lambda_inputs(first_geometry):

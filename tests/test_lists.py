@@ -108,7 +108,7 @@ def test_xvcc_driver_over_lists(nonhf):
     _lib.call('aces_vcc_mp1')
     _lib.call('aces_vcc_set_diis_order', 6)
     e0 = ctypes.c_double()
-    _lib.call('aces_cmpeng', 43, 0, None, ctypes.byref(e0))
+    _lib.call('aces_cmpeng', 43, 0, None, ctypes.byref(e0), None)
     energies, resm = [e0.value], []
     conv = ctypes.c_int(1)
     st = (ctypes.c_double * 8)()
@@ -118,7 +118,7 @@ def test_xvcc_driver_over_lists(nonhf):
         _lib.call('aces_e4s')
         _lib.call('aces_newt2')
         e = ctypes.c_double()
-        _lib.call('aces_cmpeng', 60, 2, None, ctypes.byref(e))
+        _lib.call('aces_cmpeng', 60, 2, None, ctypes.byref(e), None)
         energies.append(e.value)
         _lib.call('aces_dodiis0', icycle, 1e-9, ctypes.byref(conv), st)
         resm.append(st[1])
@@ -126,7 +126,7 @@ def test_xvcc_driver_over_lists(nonhf):
             break
         _lib.call('aces_drmove')
     ef = ctypes.c_double()
-    _lib.call('aces_cmpeng', 43, 0, None, ctypes.byref(ef))
+    _lib.call('aces_cmpeng', 43, 0, None, ctypes.byref(ef), None)
     lists.fin()
     assert conv.value == 0 and len(resm) == ref['cycles']
     np.testing.assert_allclose(energies, ref['energy'], rtol=0, atol=1e-10)
@@ -159,7 +159,7 @@ def test_xvcc_driver_mixed_then_refine():
         _lib.call('aces_e4s')
         _lib.call('aces_newt2')
         e = ctypes.c_double()
-        _lib.call('aces_cmpeng', 60, 2, None, ctypes.byref(e))
+        _lib.call('aces_cmpeng', 60, 2, None, ctypes.byref(e), None)
         _lib.call('aces_dodiis0', icycle - ic0, tol, ctypes.byref(conv), st)
         n_refine += phase == 3
         if conv.value == 0:
@@ -171,8 +171,81 @@ def test_xvcc_driver_mixed_then_refine():
             break
         _lib.call('aces_drmove')
     ef = ctypes.c_double()
-    _lib.call('aces_cmpeng', 43, 0, None, ctypes.byref(ef))
+    _lib.call('aces_cmpeng', 43, 0, None, ctypes.byref(ef), None)
     _lib.call('aces_vcc_set_precision', 0)
     lists.fin()
     assert conv.value == 0 and phase == 3 and n_refine < ref['cycles']
     assert abs(ef.value - ref['final_energy']) < 1e-9
+
+
+def _lists_system(o, v, seed, nonhf):
+    from aces2 import lists, synth
+    eps, B = synth.draw(o, v, seed, sigma=synth.scaled_sigma(o, v))
+    pert = None
+    if nonhf:
+        rng = np.random.default_rng(seed + 1)
+        P = rng.normal(0, 0.01, (o + v, o + v))
+        pert = P + P.T
+    blk, eri = synth.blocks_numpy(eps, B, o, perturb=pert)
+    lists.init()
+    lists.load_c1_system(blk, o, v, nonhf=nonhf)
+    return blk, eri
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('nonhf', [False, True])
+def test_cmpeng_spin_cases_over_lists(nonhf):
+    """CMPENG over lists 46/90 (and 63/90 group 3): ECORR(1) = E(AA), ECORR(3) =
+    E(AB), ETOT and ETOTT2 as libr/cmpeng.F returns them; ECORR(2) untouched."""
+    import ctypes
+    from aces2 import _lib, lists
+    print('case cmpeng lists nonhf=%s' % nonhf, flush=True)
+    o, v = 5, 19
+    blk, eri = _lists_system(o, v, 33, nonhf)
+    _lib.call('aces_vcc_mp1')
+    rng = np.random.default_rng(9)
+    g = rccsd.blocks(eri, o)
+    for nlist2, nlist1 in ((43, 0), (60, 2)):
+        t1 = rng.normal(0, 0.05, (o, v))
+        t2 = rng.normal(0, 0.05, (o, o, v, v))
+        t2 = 0.5 * (t2 + t2.transpose(1, 0, 3, 2))
+        if nlist2 == 60:
+            lists.inipck(1, 13, 14, 63)
+        lists.putlst(t2.ravel(), 1, o * o, 1, nlist2 + 3)
+        lists.putlst(t1.ravel(), 1, 1, nlist1 + 1, 90)
+        ec = (ctypes.c_double * 3)(7.0, 7.0, 7.0)
+        et, et2 = ctypes.c_double(), ctypes.c_double()
+        _lib.call('aces_cmpeng', nlist2, nlist1, ec, ctypes.byref(et), ctypes.byref(et2))
+        e_aa, e_ab, e_lin = rccsd.energy_parts(t1, t2, blk['fock'], g, o)
+        assert abs(ec[0] - e_aa) < 1e-13 and abs(ec[2] - e_ab) < 1e-13 and ec[1] == 7.0
+        assert abs(et.value - rccsd.energy(t1, t2, blk['fock'], g, o)) < 1e-13
+        assert abs(et2.value - e_lin) < 1e-13
+    lists.fin()
+
+
+@pytest.mark.gpu
+def test_drmove_fills_lambda_start_lists():
+    """DRMOVE(...,100,.TRUE.) (vcc/vcc.F:1138): lists 61/63 -> 144/146 and list 90
+    group 3 -> list 190 group 1; the T lists 44/46/90 are left unchanged."""
+    from aces2 import _lib, lists
+    print('case drmove 100', flush=True)
+    o, v = 4, 11
+    _lists_system(o, v, 34, False)
+    _lib.call('aces_vcc_mp1')
+    _lib.call('aces_vcc_set_diis_order', 6)
+    for name in ('aces_genint', 'aces_dre3en', 'aces_e4s', 'aces_newt2'):
+        _lib.call(name)
+    before = {(lst, g): lists.getlst(1, lists.shape(g, lst)[1], g, lst).copy()
+              for (lst, g) in ((44, 1), (46, 1), (90, 1), (61, 1), (63, 1), (90, 3))}
+    _lib.call('aces_drmove_lists', 100, 1)
+    for key, x in before.items():
+        np.testing.assert_array_equal(lists.getlst(1, lists.shape(key[1], key[0])[1], key[1], key[0]), x)
+    np.testing.assert_array_equal(lists.getlst(1, lists.shape(1, 144)[1], 1, 144), before[(61, 1)])
+    np.testing.assert_array_equal(lists.getlst(1, lists.shape(1, 146)[1], 1, 146), before[(63, 1)])
+    np.testing.assert_array_equal(lists.getlst(1, 1, 1, 190), before[(90, 3)])
+    assert lists.isytyp(146) == lists.isytyp(63) == (13, 14)
+    assert np.abs(before[(63, 1)] - before[(46, 1)]).max() > 0
+    _lib.call('aces_drmove_lists', 0, 0)   # without SING1 the T1 list stays
+    np.testing.assert_array_equal(lists.getlst(1, 1, 1, 90), before[(90, 1)])
+    np.testing.assert_array_equal(lists.getlst(1, o * o, 1, 46), before[(63, 1)])
+    lists.fin()

@@ -72,6 +72,62 @@ def test_closed_shell_matches_spin_orbital(o, v, seed, nonhf):
     assert abs(rccsd.energy(t1, t2, f, g, o) - spinorb.energy(T1, T2, F, G, no)) < 1e-13
 
 
+@pytest.mark.parametrize('o,v,seed,nonhf', [(2, 3, 7, True), (3, 4, 8, False)])
+def test_cmpeng_spin_cases_match_spin_orbital(o, v, seed, nonhf):
+    """CMPENG's AA piece is the spin-orbital energy of the alpha orbitals alone;
+    the AB piece is the rest (E = 2 E_AA + E_AB, E_BB = E_AA); ETOTT2 is that
+    total with the t1 t1 part of tau removed."""
+    f, eri = _ham(o, v, seed, nonhf)
+    F, G, no = spinorb.spin_orbital_hamiltonian(f, eri, o)
+    t1, t2 = _rand_closed_t(o, v, seed + 1)
+    T1, T2 = spinorb.closed_to_so_amplitudes(t1, t2)
+    g = rccsd.blocks(eri, o)
+    e_aa, e_ab, e_lin = rccsd.energy_parts(t1, t2, f, g, o)
+    n = o + v
+    alpha = np.concatenate([np.arange(o), np.arange(2 * o, 2 * o + v)])
+    e_alpha = spinorb.energy(T1[:o, :v], T2[:o, :o, :v, :v], F[np.ix_(alpha, alpha)],
+                             G[np.ix_(alpha, alpha, alpha, alpha)], o)
+    e_tot = spinorb.energy(T1, T2, F, G, no)
+    assert abs(e_aa - e_alpha) < 1e-13
+    assert abs(e_ab - (e_tot - 2 * e_alpha)) < 1e-13
+    assert abs(e_lin - spinorb.energy(T1, T2, F, G, no) + 0.5 * np.einsum('ijab,ia,jb->', G[:no, :no, no:, no:], T1,
+                                                                           T1)) < 1e-13
+    assert n == f.shape[0]
+
+
+@pytest.mark.parametrize('o,v,seed,nonhf', [(1, 3, 1, True), (3, 5, 2, True), (4, 7, 3, False), (5, 9, 4, True)])
+def test_element_sampled_increments_match_rccsd(o, v, seed, nonhf):
+    """The element-sampled oracle (re-associated terms, DF factors) equals
+    rccsd.increments for every (i, a) slice and inc1, arbitrary amplitudes; its
+    energy, MP1 and MP2 helpers equal rccsd's and the textbook closed form."""
+    from aces2 import synth
+    from oracle import slices
+    eps, B = synth.draw(o, v, seed, sigma=0.1)
+    rng = np.random.default_rng(seed)
+    f0 = synth.fock(eps)
+    f = f0
+    if nonhf:
+        P = rng.normal(0, 0.05, f.shape)
+        f = f + P + P.T
+    eri = synth.full_eri(B)
+    g = rccsd.blocks(eri, o)
+    t1, t2 = _rand_closed_t(o, v, seed + 3)
+    r1, r2 = rccsd.increments(t1, t2, f, g, o)
+    S = slices.DFSystem(B, f, o)
+    pairs = [(i, a) for i in range(o) for a in range(v)]
+    i1, sl = slices.increments(S, t1, t2, pairs)
+    scale = np.abs(r2).max()
+    np.testing.assert_allclose(i1, r1, atol=1e-14 * max(1.0, np.abs(r1).max()))
+    for (i, a) in pairs:
+        np.testing.assert_allclose(sl[(i, a)], r2[i, :, a, :], atol=1e-14 * scale)
+    assert abs(slices.energy(S, t1, t2) - rccsd.energy(t1, t2, f, g, o)) < 1e-13
+    S0 = slices.DFSystem(B, f0, o)
+    m1, m2 = vcc_loop.mp1_start(f0, g, o)
+    assert abs(slices.mp2_energy(S0) - rccsd.energy(m1, m2, f0, g, o)) < 1e-13
+    for (i, a) in pairs:
+        np.testing.assert_allclose(slices.mp1(S0, i, a), m2[i, :, a, :], atol=1e-15)
+
+
 @pytest.mark.parametrize('v,seed', [(4, 1), (9, 2)])
 def test_ccsd_equals_fci_for_two_electrons(v, seed):
     from aces2 import synth
