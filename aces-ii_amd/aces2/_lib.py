@@ -26,6 +26,8 @@ SIGNATURES = {
                           c_vp, c_long, c_double, c_double, c_int, c_long, c_long, c_long, c_vp]),
     'aces_gemm_grouped': (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp,
                                    c_int, c_vp, c_vp, c_double, c_double, c_vp, c_long, c_vp]),
+    'aces_gemm_grouped_noalloc': (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_vp,
+                                           c_vp, c_int, c_vp, c_vp, c_double, c_double, c_long, c_vp]),
     'aces_convert_f32': (c_int, [c_int, c_int, c_vp, c_long, c_vp, c_long, c_vp]),
     'aces_split_bf16': (c_int, [c_int, c_int, c_vp, c_long, c_vp, c_vp, c_long, c_vp]),
     'aces_ccsd_create': (c_int, [c_int, c_int, c_vp, ctypes.POINTER(c_vp)]),

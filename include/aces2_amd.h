@@ -52,6 +52,19 @@ int aces_gemm_grouped(int prec, int ngroups, const int* M, const int* N, const i
                       const void* const* d_A_lo, const long* lda, int a_kc, const void* const* d_B,
                       const void* const* d_B_lo, const long* ldb, int b_kc, double* const* d_C, const long* ldc,
                       double alpha, double beta, double* d_ws, long ws_elems, void* stream);
+/* aces_gemm_grouped without a workspace pointer: the split-K partials (up to
+ * ws_elems doubles, at most 4194304 = 32 MiB) go to a static device array of
+ * the library, one per GPU, held until the process ends.  ws_elems above that
+ * maximum is rejected (ACES_ERR_RANGE).  Allocates nothing (no hipMalloc, no
+ * allocating library routine).  Calls that overlap in time on one GPU share
+ * that array and race; calls on one stream are ordered.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (HIP graph
+ * capture, serving loops). */
+int aces_gemm_grouped_noalloc(int prec, int ngroups, const int* M, const int* N, const int* K,
+                              const void* const* d_A, const void* const* d_A_lo, const long* lda, int a_kc,
+                              const void* const* d_B, const void* const* d_B_lo, const long* ldb, int b_kc,
+                              double* const* d_C, const long* ldc, double alpha, double beta, long ws_elems,
+                              void* stream);
 int aces_convert_f32(int rows, int cols, const double* d_x, long ldx, float* d_y, long ldy, void* stream);
 int aces_split_bf16(int rows, int cols, const double* d_x, long ldx, unsigned short* d_hi, unsigned short* d_lo,
                     long ldy, void* stream);
