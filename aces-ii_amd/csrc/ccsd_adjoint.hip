@@ -390,9 +390,13 @@ int CcsdEngine::lambda_start() {
   ACES_CHECK(adjoint_alloc());
   ACES_CHECK(compute_increments(GP_F64));
   const long ov = (long)o * v;
-  // Lambda1 = 0 (vcc/init2.F:45-53), Lambda2 = T2 (vcc/vcc.F:320-325)
-  ACES_HIP(hipMemsetAsync(Lam_, 0, sizeof(double) * ov, s_));
-  ACES_HIP(hipMemcpyAsync(Lam_ + ov, t2_, sizeof(double) * ov * ov, hipMemcpyDeviceToDevice, s_));
+  // Lambda start: the Jacobi update of the converged cycle, Lambda1 = T1_new and
+  // Lambda2 = SYMETRIZE_T2(T2_new).  xvcc leaves that update on the increment
+  // lists (61/63, 90 group 3) and moves it to the Lambda lists 144/146/190 with
+  // INIT2 + DRMOVE(...,100,SING1) after convergence (vcc/vcc.F:1130-1143); xlambda
+  // starts from those lists.
+  ACES_CHECK(jacobi(o, v, eo_, ev_, inc1_, inc2_, Lam_, Lam_ + ov, s_));
+  ACES_CHECK(symmetrize_t2(o, v, Lam_ + ov, s_));
   ldiis_.R.assign((size_t)ldiis_.order * ldiis_.order, 0.0);
   lam_anc_valid_ = false;
   ACES_HIP(hipStreamSynchronize(s_));

@@ -80,7 +80,8 @@ class CcsdEngine {
   // it also returns dL/dfock, the response one-particle density (n x n, symmetric).
   int adjoint(const double* l1, const double* l2, double* r1, double* r2, double* dens);
   // xlambda loop (lambda/vlamcc.F:599-830): lambda_start() forms the forward
-  // intermediates at the current amplitudes and sets Lambda2 = T2, Lambda1 = 0;
+  // intermediates at the current amplitudes and starts from their Jacobi update
+  // (Lambda1 = T1_new, Lambda2 = T2_new, the lists xvcc hands to xlambda);
   // each lambda_cycle() is one Jacobi step plus the Lambda DODIIS0.
   int lambda_start();
   int lambda_cycle(int icycle, double tol, CcsdStats& st);

@@ -1626,9 +1626,12 @@ int SymEngine::lambda_start() {
   ACES_CHECK(diis_grow(lsol_, lerr_, lorder_));
   LR_.assign((size_t)lorder_ * lorder_, 0.0);
   ACES_CHECK(forward(GP_F64));
-  // Lambda1 = 0 (vcc/init2.F:45-53), Lambda2 = T2 (vcc/vcc.F:320-325)
-  ACES_HIP(hipMemsetAsync(Lam_, 0, sizeof(double) * n1_, s_));
-  ACES_HIP(hipMemcpyAsync(Lam_ + n1_, T_cur_ + n1_, sizeof(double) * (nt_ - n1_), hipMemcpyDeviceToDevice, s_));
+  // Lambda start: the Jacobi update of the converged cycle (T1_new, SYMETRIZE_T2
+  // of T2_new), which xvcc moves to the Lambda lists with DRMOVE(...,100,SING1)
+  // after convergence (vcc/vcc.F:1130-1143)
+  ACES_CHECK(denom_apply(t1_, Lam_, 0.0, T_[inc1_].d, 1.0, nullptr, 0.0, 0, -1));
+  ACES_CHECK(denom_apply(t2_, Lam_ + n1_, 0.0, T_[inc2_].d, 1.0, nullptr, 0.0, 0, -1));
+  ACES_CHECK(pair_symmetrize(Lam_ + n1_));
   ACES_HIP(hipStreamSynchronize(s_));
   lam_ready_ = true;
   return ACES_OK;

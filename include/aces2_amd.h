@@ -139,8 +139,11 @@ int aces_ccsd_profile(void* handle, double* out, int reset);
  *   aces_ccsd_adjoint        r = dL/dT at the engine's amplitudes for given l
  *                            (r2 symmetrised) and, if d_dens != NULL, the
  *                            response density dL/dfock (n x n, symmetric).
- *   aces_ccsd_lambda_start   forward intermediates at the current amplitudes,
- *                            L2 = T2, L1 = 0 (vcc/vcc.F:320-325, vcc/init2.F:45-53)
+ *   aces_ccsd_lambda_start   forward intermediates at the current amplitudes;
+ *                            start L1 = T1_new, L2 = T2_new, the Jacobi update
+ *                            at those amplitudes (what xvcc moves to lists
+ *                            190/146 with DRMOVE(...,100,SING1) after
+ *                            convergence, vcc/vcc.F:1130-1143)
  *   aces_ccsd_lambda_cycle   one Jacobi step + Lambda DODIIS0 (converged when
  *                            |RESMITER| or |RESMDIIS| < tol); stats as for
  *                            aces_ccsd_cycle (stats[0], stats[4] unused)

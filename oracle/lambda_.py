@@ -167,11 +167,26 @@ def from_aces(L1, L2):
     return 2.0 * L1, 2.0 * L2 - L2.transpose(1, 0, 2, 3)
 
 
+def lambda_start(f, g, o, t1, t2):
+    """xlambda's starting multipliers in the list normalisation (L1, L2).
+
+    After convergence xvcc creates the Lambda lists (INIT2, ``vcc/init2.F``) and
+    moves the increment lists into them with DRMOVE(...,100,SING1)
+    (``vcc/vcc.F:1130-1143``, ``libr/drmove.f``): the converged cycle's Jacobi
+    update, T1_new = inc1/D1 (list 90 group 3 -> 190) and T2_new = inc2/D2
+    after DODIIS0's SYMETRIZE_T2 (list 63 -> 146), at the converged T.
+    """
+    i1, i2 = rccsd.increments(t1, t2, f, g, o)
+    d1, d2 = rccsd.denominators(f, o)
+    return i1 / d1, symmetrize_pair(i2 / d2)
+
+
 def run_reference(f, g, o, t1, t2, maxcyc=50, conv=7, order=5, log=None):
     """xlambda's CCSD Lambda iteration (``lambda/vlamcc.F:599-830``), restated.
 
-    Start: Lambda2 = converged T2 (``vcc/vcc.F:320-325`` MVSYM 43+ISPIN -> 143+ISPIN),
-    Lambda1 = 0 (``vcc/init2.F:45-53``).  Each cycle forms the increments,
+    Start: ``lambda_start`` (the converged CC cycle's Jacobi update moved to
+    the Lambda lists; a zero Lambda1 instead puts cycles 2.. of the reference's
+    printed trace off by up to 5 %).  Each cycle forms the increments,
     divides by the denominators (NEWT2) and calls the Lambda DODIIS0
     (``lambda/dodiis0.F``): same DIIS vector and subspace as the amplitude
     equations, but converged when |RESMITER| or |RESMDIIS| is below
@@ -181,8 +196,7 @@ def run_reference(f, g, o, t1, t2, maxcyc=50, conv=7, order=5, log=None):
     gradient; DIIS runs on the reference's vectors.
     """
     d1, d2 = rccsd.denominators(f, o)
-    L1 = np.zeros_like(t1)
-    L2 = symmetrize_pair(np.array(t2, dtype=np.float64))
+    L1, L2 = lambda_start(f, g, o, t1, t2)
     diis = _vl.DIIS(order)
     tol = 10.0 ** (-conv)
     hist = {'resmiter': [], 'resmdiis': [], 'converged': False}

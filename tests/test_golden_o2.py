@@ -134,18 +134,22 @@ def _lambda_inputs(first_geometry):
     return g, F, eri, blk, o, h
 
 
+# The printed Lambda residuals are reproduced to a few 1e-6 relative (the CC
+# trace to 3e-7): the Lambda residual is first order in the orbitals, whose
+# remaining difference to the reference's SCF shows there first.
+LAMBDA_RTOL = 5e-6
+
+
 def test_oracle_lambda_trace_against_reference(first_geometry):
-    """xlambda of out.zmat.015a: 13 cycles.  The first residual matches to 1e-6
-    relative; later cycles agree to a few per cent (the reference's Lambda map is
-    not reproduced element by element, see the oracle's header), and the
-    converged multipliers reproduce xdens's natural occupations."""
+    """xlambda of out.zmat.015a: every printed RESMITER and RESMDIIS of the 13
+    cycles, from the start xvcc hands over (the converged cycle's Jacobi update,
+    L1 included: with L1 = 0 instead cycles 2.. are off by up to 5 %)."""
     from oracle import lambda_
     g, F, eri, blk, o, h = _lambda_inputs(first_geometry)
     lh = lambda_.run_reference(F, blk, o, h['t1'], h['t2'], conv=g['cc_conv'], order=g['cc_exporder'])
     assert lh['cycles'] == g['lambda_cycles']
-    assert abs(lh['resmiter'][0] / g['lambda_resmiter'][0] - 1) < 1e-6
-    np.testing.assert_allclose(lh['resmiter'], g['lambda_resmiter'], rtol=0.05)
-    np.testing.assert_allclose(lh['resmdiis'], g['lambda_resmdiis'], rtol=0.05)
+    np.testing.assert_allclose(lh['resmiter'], g['lambda_resmiter'], rtol=LAMBDA_RTOL, atol=0)
+    np.testing.assert_allclose(lh['resmdiis'], g['lambda_resmdiis'], rtol=LAMBDA_RTOL, atol=0)
 
 
 def test_oracle_density_natural_occupations(first_geometry):
@@ -179,7 +183,8 @@ def test_gpu_lambda_and_density_against_reference(first_geometry):
     cc.close()
     assert lh['cycles'] == g['lambda_cycles'] == ref['cycles']
     np.testing.assert_allclose(lh['resmiter'], ref['resmiter'], rtol=1e-6, atol=1e-12)
-    np.testing.assert_allclose(lh['resmiter'], g['lambda_resmiter'], rtol=0.05)
+    np.testing.assert_allclose(lh['resmiter'], g['lambda_resmiter'], rtol=LAMBDA_RTOL, atol=0)
+    np.testing.assert_allclose(lh['resmdiis'], g['lambda_resmdiis'], rtol=LAMBDA_RTOL, atol=0)
     D[np.arange(o), np.arange(o)] += 2.0
     occ = np.linalg.eigvalsh(D)[::-1]
     np.testing.assert_allclose(occ, g['natural_occupations'], atol=1e-5)
