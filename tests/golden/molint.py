@@ -244,25 +244,35 @@ def nuclear_repulsion(atoms):
     return e
 
 
-def reflection_labels(shells, C, S):
-    """D2h label (sx, sy, sz) of each MO of a molecule on the z axis, symmetric under z -> -z.
+def reflection_labels(shells, C, S, axes=(0, 1, 2)):
+    """Characters (+1/-1) of each MO under the reflections through the planes
+    normal to the given axes (x -> -x etc.), for a molecule mapped onto itself
+    by each of them: (sx, sy, sz) for D2h on the z axis, (sx, sy) for a C2v
+    molecule in the yz plane.
 
-    The reflection x -> -x multiplies a Cartesian function by (-1)^lx, y by (-1)^ly;
-    z -> -z also swaps the two centres (+z <-> -z) and multiplies by (-1)^lz.
+    The reflection x -> -x multiplies a Cartesian function by (-1)^lx and maps
+    its centre to the mirror image (which must be an atom of the same basis).
     """
+    # a basis function is (centre, index of its shell among that centre's shells,
+    # Cartesian component); mirror-image atoms carry the same shells in the same order
     funcs = []
+    seen = {}
     for sh in shells:
+        cen = tuple(np.round(sh.center, 10) + 0.0)
+        ordinal = seen.get(cen, 0)
+        seen[cen] = ordinal + 1
         for c in sh.comps:
-            funcs.append((tuple(np.round(sh.center, 10)), c))
+            funcs.append((cen, ordinal, c))
     n = len(funcs)
     index = {f: k for k, f in enumerate(funcs)}
+    assert len(index) == n
     ops = []
-    for axis in range(3):
+    for axis in axes:
         P = np.zeros((n, n))
-        for i, (cen, comp) in enumerate(funcs):
+        for i, (cen, ordinal, comp) in enumerate(funcs):
             img = list(cen)
             img[axis] = -img[axis]
-            P[index[(tuple(np.round(img, 10)), comp)], i] = (-1) ** comp[axis]
+            P[index[(tuple(np.asarray(img) + 0.0), ordinal, comp)], i] = (-1) ** comp[axis]
         ops.append(P)
     labels = []
     for k in range(C.shape[1]):
@@ -363,8 +373,27 @@ def d2h_occupation(shells, S, counts):
     return rule
 
 
-def mo_system(atoms_basis, nocc, occ_counts=None):
-    """(E_nuc, E_scf, MO Fock, MO (pq|rs), eps) for a closed-shell molecule."""
+def irrep_order(labels, o):
+    """Irrep-sorted orbital order for the irrep-blocked engine.
+
+    labels: per MO the characters under the symmetry generators; the irrep index
+    is the bit vector (bit k set when the k-th character is -1), so that the
+    irrep of a product is the XOR of the indices.  Returns (perm, occ, vir):
+    occupied MOs (the first o) sorted by irrep, then the virtual ones, keeping
+    the energy order inside each irrep, and the orbital counts per irrep.
+    """
+    irr = [sum(1 << k for k, c in enumerate(lab) if c < 0) for lab in labels]
+    nirr = 1 << len(labels[0])
+    occ_idx = sorted(range(o), key=lambda k: (irr[k], k))
+    vir_idx = sorted(range(o, len(irr)), key=lambda k: (irr[k], k))
+    occ = [sum(1 for k in range(o) if irr[k] == h) for h in range(nirr)]
+    vir = [sum(1 for k in range(o, len(irr)) if irr[k] == h) for h in range(nirr)]
+    return occ_idx + vir_idx, occ, vir
+
+
+def mo_system(atoms_basis, nocc, occ_counts=None, axes=None):
+    """(E_nuc, E_scf, MO Fock, MO (pq|rs), eps) for a closed-shell molecule; with
+    ``axes`` also the MO symmetry labels under those reflections."""
     shells = build_shells(atoms_basis)
     atoms = [(Z, C) for Z, C, _ in atoms_basis]
     S, T, V = one_electron(shells, atoms)
@@ -375,4 +404,6 @@ def mo_system(atoms_basis, nocc, occ_counts=None):
     enuc = nuclear_repulsion(atoms)
     Fmo = C.T @ F @ C
     eri_mo = np.einsum('pqrs,pi,qj,rk,sl->ijkl', eri, C, C, C, C, optimize=True)
+    if axes is not None:
+        return enuc, e_el + enuc, Fmo, eri_mo, eps, reflection_labels(shells, C, S, axes)
     return enuc, e_el + enuc, Fmo, eri_mo, eps

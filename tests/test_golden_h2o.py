@@ -1,0 +1,172 @@
+"""Golden vectors of the reference's regression tests zmat.013a / zmat.014a (H2O, DZP, RHF-CCSD).
+
+The fixture (tests/golden/h2o_dzp.json, made by tests/golden/make_h2o_fixture.py)
+holds what the reference printed for the first geometry (R = 0.943056 A,
+A = 105.96879 degrees, Cartesian d functions): SCF and MP2 energies, every
+CMPENG line and largest residual / DIIS residual element of the CC run
+(MP1 start, CC_EXPORDER 5, CC_CONV 7), the xlambda trace and the xdens natural
+occupations.  The MO Hamiltonian is rebuilt from the fixture's basis and
+Z-matrix (tests/golden/molint.py).  The molecule has C2v symmetry (4 irreps):
+the irrep-blocked engine runs it with the orbitals sorted by irrep.
+
+The sign of the largest residual element depends on the phases of the MOs,
+which the reference's SCF fixes its own way: magnitudes are compared.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import rccsd, vcc_loop
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, 'golden'))
+sys.path.insert(0, HERE)
+import molint  # noqa: E402
+
+FIX = json.load(open(os.path.join(HERE, 'golden', 'h2o_dzp.json')))
+O = FIX['nocc']
+# the printed Lambda residuals are reproduced to ~1e-4 relative (the CC trace to 1e-7)
+LAMBDA_RTOL = 3e-4
+
+
+@pytest.fixture(scope='module')
+def h2o():
+    z = FIX['zmatrix']
+    R = z['R_angstrom'] / z['bohr_angstrom']
+    A = np.radians(z['A_degree'])
+    bas = {el: [(b['l'], b['exps'], np.array(b['coefs'])) for b in FIX['basis'][el]] for el in ('H', 'O')}
+    # Z-matrix H; O 1 R; H 2 R 1 A: O at the origin, H1 on +z, H2 in the yz plane (C2v, plane x = 0)
+    atoms = [(1.0, [0.0, 0.0, R], bas['H']), (8.0, [0.0, 0.0, 0.0], bas['O']),
+             (1.0, [0.0, R * np.sin(A), R * np.cos(A)], bas['H'])]
+    # the molecule is symmetric under x -> -x; the C2 axis bisects the HOH angle, so
+    # rotate the frame to put it on z (the y -> -y reflection then swaps the H atoms)
+    c = np.array([a[1] for a in atoms])
+    axis = (c[0] + c[2]) / 2 - c[1]
+    axis /= np.linalg.norm(axis)
+    ez = axis
+    ey = np.cross(ez, [1.0, 0.0, 0.0])
+    ey /= np.linalg.norm(ey)
+    ex = np.cross(ey, ez)
+    Rm = np.array([ex, ey, ez])
+    atoms = [(Z, list(Rm @ np.asarray(C)), b) for (Z, C, b) in atoms]
+    enuc, escf, F, eri, eps, labels = molint.mo_system(atoms, O, axes=(0, 1))
+    return enuc, escf, F, eri, labels
+
+
+@pytest.fixture(scope='module')
+def cc_run(h2o):
+    enuc, escf, F, eri, labels = h2o
+    g = rccsd.blocks(eri, O)
+    h = vcc_loop.run(F, g, O, maxcyc=50, conv=FIX['cc_conv'], order=FIX['cc_exporder'])
+    return g, h
+
+
+def _check_trace(energies, resmiter, final, cycles=None, resmdiis=None):
+    cc = FIX['cc']
+    if cycles is not None:
+        assert cycles == cc['cycles']
+    np.testing.assert_allclose(energies, cc['mp2_and_cycle_energies'], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(np.abs(resmiter), np.abs(cc['resmiter']), rtol=3e-7, atol=0)
+    if resmdiis is not None:
+        np.testing.assert_allclose(np.abs(resmdiis), np.abs(cc['resmdiis']), rtol=3e-7, atol=0)
+    assert abs(final - cc['final_energy']) < 1e-10
+
+
+def test_scf_and_mp2_match_reference(h2o):
+    enuc, escf, F, eri, labels = h2o
+    assert abs(enuc - FIX['nucrep']) < 1e-9
+    assert abs(escf - FIX['escf']) < 1e-9
+    g = rccsd.blocks(eri, O)
+    t1, t2 = vcc_loop.mp1_start(F, g, O)
+    e_aa, e_ab, _ = rccsd.energy_parts(t1, t2, F, g, O)
+    assert abs(e_aa - FIX['mp2']['E2(AA)']) < 1e-10
+    assert abs(e_ab - FIX['mp2']['E2(AB)']) < 1e-10
+    assert abs(2 * e_aa + e_ab - FIX['mp2']['E2(TOT)']) < 1e-10
+    # C2v: four irreps, the 5 occupied orbitals as the reference's SCF occupied them
+    perm, occ, vir = molint.irrep_order(labels, O)
+    assert sum(occ) == O and sum(vir) == F.shape[0] - O and len(occ) == 4
+
+
+def test_oracle_reproduces_reference_cc_trace(cc_run):
+    g, h = cc_run
+    _check_trace(h['energy'], h['resmiter'], h['final_energy'], h['cycles'], h['resmdiis'])
+    aa = [x[0] for x in h['energy_aa_ab']]
+    ab = [x[1] for x in h['energy_aa_ab']]
+    np.testing.assert_allclose(aa, FIX['cc']['aa_contributions'], rtol=0, atol=5.001e-8)
+    np.testing.assert_allclose(ab, FIX['cc']['ab_contributions'], rtol=0, atol=5.001e-8)
+    assert abs(h['final_energy'] + FIX['escf'] - FIX['ccsd_total_energy']) < 1e-9
+
+
+def test_oracle_lambda_trace_and_density(h2o, cc_run):
+    from oracle import lambda_
+    enuc, escf, F, eri, labels = h2o
+    g, h = cc_run
+    lh = lambda_.run_reference(F, g, O, h['t1'], h['t2'], conv=FIX['cc_conv'], order=FIX['cc_exporder'])
+    assert lh['cycles'] == FIX['lambda_cycles']
+    np.testing.assert_allclose(np.abs(lh['resmiter']), np.abs(FIX['lambda_resmiter']), rtol=LAMBDA_RTOL)
+    np.testing.assert_allclose(np.abs(lh['resmdiis']), np.abs(FIX['lambda_resmdiis']), rtol=LAMBDA_RTOL)
+
+
+def _blocks(F, eri):
+    from aces2 import synth
+    b = dict(rccsd.blocks(eri, O))
+    b['fock'] = F
+    b['list233'] = synth.list233_from_eri(eri, O)
+    return b
+
+
+@pytest.mark.gpu
+def test_gpu_engine_reproduces_reference_cc_trace(h2o):
+    from aces2.ccsd import CCSD
+    print('case golden H2O gpu', flush=True)
+    enuc, escf, F, eri, labels = h2o
+    cc = CCSD(O, F.shape[0] - O, _blocks(F, eri))
+    h = cc.run(maxcyc=50, conv=FIX['cc_conv'], order=FIX['cc_exporder'])
+    cc.close()
+    _check_trace(h['energy'], h['resmiter'], h['final_energy'], h['cycles'], h['resmdiis'])
+
+
+@pytest.mark.gpu
+def test_irrep_blocked_engine_reproduces_reference_cc_trace_c2v(h2o):
+    """The same trace from the irrep-blocked engine on the C2v blocks (4 irreps)."""
+    from aces2.ccsd import SymCCSD
+    print('case golden H2O sym C2v', flush=True)
+    enuc, escf, F, eri, labels = h2o
+    perm, occ, vir = molint.irrep_order(labels, O)
+    p = np.array(perm)
+    Fp = F[np.ix_(p, p)]
+    erip = eri[np.ix_(p, p, p, p)]
+    cc = SymCCSD(occ, vir, _blocks(Fp, erip))
+    h = cc.run(maxcyc=50, conv=FIX['cc_conv'], order=FIX['cc_exporder'])
+    lh = cc.solve_lambda(maxcyc=50, conv=FIX['cc_conv'], order=FIX['cc_exporder'])
+    cc.close()
+    _check_trace(h['energy'], h['resmiter'], h['final_energy'], h['cycles'], h['resmdiis'])
+    assert lh['cycles'] == FIX['lambda_cycles']
+    np.testing.assert_allclose(np.abs(lh['resmiter']), np.abs(FIX['lambda_resmiter']), rtol=LAMBDA_RTOL)
+
+
+@pytest.mark.gpu
+def test_fortran_host_reproduces_reference_output(h2o, tmp_path):
+    """vcc.F's call sequence (flang host) over the device lists: the reference's lines."""
+    import re
+    import subprocess
+    from test_fortran_gpu import EXE, write_input
+    print('case golden H2O fortran', flush=True)
+    enuc, escf, F, eri, labels = h2o
+    v = F.shape[0] - O
+    inp = tmp_path / 'h2o.bin'
+    write_input(str(inp), _blocks(F, eri), O, v)
+    out = subprocess.run([EXE, str(inp), str(FIX['cc_conv']), '50', str(FIX['cc_exporder'])], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    eng = [float(x) for x in re.findall(r'The total correlation energy is\s+(\S+) a\.u\.', out.stdout)]
+    res = [float(x.rstrip('.')) for x in re.findall(r'Largest element of residual vector :\s+(\S+)', out.stdout)]
+    _check_trace(eng[:-1], res, eng[-1])
+    aa = [float(x) for x in re.findall(r'The AA contribution to the correlation energy is:\s+(\S+) a\.u\.', out.stdout)]
+    ab = [float(x) for x in re.findall(r'The AB contribution to the correlation energy is:\s+(\S+) a\.u\.', out.stdout)]
+    np.testing.assert_allclose(aa, FIX['cc']['aa_contributions'], rtol=0, atol=1.001e-7)
+    np.testing.assert_allclose(ab, FIX['cc']['ab_contributions'], rtol=0, atol=1.001e-7)
+    assert 'Amplitude equations converged in    10 iterations.' in out.stdout

@@ -22,11 +22,11 @@ import molint  # noqa: E402
 FIX = json.load(open(os.path.join(HERE, 'golden', 'o2_sto3g.json')))
 
 
-def _system(z):
+def _system(z, labels=False):
     basis = [(b['l'], b['exps'], np.array(b['coefs'])) for b in FIX['basis_O']]
     atoms = [(8.0, [0.0, 0.0, z], basis), (8.0, [0.0, 0.0, -z], basis)]
     counts = {tuple(int(x) for x in k.strip('()').split(',')): v for k, v in FIX['occupation_d2h'].items()}
-    return molint.mo_system(atoms, FIX['nocc'], counts)
+    return molint.mo_system(atoms, FIX['nocc'], counts, axes=(0, 1, 2) if labels else None)
 
 
 @pytest.fixture(scope='module')
@@ -124,6 +124,37 @@ def test_fortran_host_reproduces_reference_output(first_geometry, tmp_path):
     ab = [float(x) for x in re.findall(r'The AB contribution to the correlation energy is:\s+(\S+) a\.u\.', out.stdout)]
     np.testing.assert_allclose(aa, g['aa_contributions'], rtol=0, atol=1.001e-7)
     np.testing.assert_allclose(ab, g['ab_contributions'], rtol=0, atol=1.001e-7)
+
+
+@pytest.mark.gpu
+def test_irrep_blocked_engine_reproduces_reference_trace_d2h():
+    """The reference's CC and xlambda traces from the irrep-blocked engine on the
+    D2h blocks (8 irreps, the SCF occupation by irrep of the fixture)."""
+    from aces2 import synth
+    from aces2.ccsd import SymCCSD
+    print('case golden O2 sym D2h', flush=True)
+    g = FIX['first_geometry']
+    enuc, escf, F, eri, eps, labels = _system(g['z_bohr'], labels=True)
+    o = FIX['nocc']
+    perm, occ, vir = molint.irrep_order(labels, o)
+    assert len(occ) == 8 and sum(occ) == o
+    p = np.array(perm)
+    Fp = F[np.ix_(p, p)]
+    erip = eri[np.ix_(p, p, p, p)]
+    blk = dict(rccsd.blocks(erip, o))
+    blk['fock'] = Fp
+    blk['list233'] = synth.list233_from_eri(erip, o)
+    cc = SymCCSD(occ, vir, blk)
+    h = cc.run(maxcyc=50, conv=g['cc_conv'], order=g['cc_exporder'])
+    lh = cc.solve_lambda(maxcyc=50, conv=g['cc_conv'], order=g['cc_exporder'])
+    cc.close()
+    assert h['cycles'] == g['cycles']
+    np.testing.assert_allclose(h['energy'], g['mp2_and_cycle_energies'], rtol=0, atol=3e-11)
+    np.testing.assert_allclose(h['resmiter'], g['resmiter'], rtol=3e-7, atol=0)
+    assert abs(h['final_energy'] - g['final_energy']) < 1e-11
+    assert lh['cycles'] == g['lambda_cycles']
+    np.testing.assert_allclose(lh['resmiter'], g['lambda_resmiter'], rtol=LAMBDA_RTOL, atol=0)
+    np.testing.assert_allclose(lh['resmdiis'], g['lambda_resmdiis'], rtol=LAMBDA_RTOL, atol=0)
 
 
 def _lambda_inputs(first_geometry):
