@@ -169,4 +169,13 @@ int aces_ccsd_density(void* h, double* dens) {
   return eng(h)->density(dens);
 }
 
+int aces_ccsd_triples(void* h, double* out, double* d_per_triple) {
+  REQUIRE_HANDLE(h);
+  if (out == nullptr) {
+    set_error("aces_ccsd_triples: null output");
+    return ACES_ERR_ARG;
+  }
+  return eng(h)->triples(out, d_per_triple);
+}
+
 }  // extern "C"

@@ -67,7 +67,7 @@ class CcsdEngine {
   // Per-launch timing of the MFMA contractions (hipEvents on the engine's stream).
   // out[3*c + 0..2] = {ms, algorithmic flops, launches} for c = 0 ladder, 1 W build,
   // 2 ring, 3 the other fp64 contractions of the increments, 4 the reverse pass,
-  // 5 the reverse pass's bf16x3 products (mixed Lambda cycles).
+  // 5 the reverse pass's bf16x3 products (mixed Lambda cycles), 6 the (T) products.
   void set_profiling(bool on) { profile_ = on; }
   void profile(double* out, int reset);
   long device_bytes() const { return bytes_; }
@@ -99,6 +99,15 @@ class CcsdEngine {
   int set_lambda(const double* l1, const double* l2, int aces_normalisation);
   // Response density at the current multipliers (n x n, device).
   int density(double* dens);
+
+  // ---- non-iterative triples (triples.hip) ----
+  // (T) of CCSD(T) at the current amplitudes (pair-symmetric t2, canonical HF
+  // orbital energies from the Fock diagonal): out[0] E[4]T, out[1] E[5]ST,
+  // out[2] E4TAAA, out[3] E4TBAA (E[4]T = 2 (E4TAAA + E4TBAA)), out[4] ms,
+  // out[5] MFMA flops, out[6] number of occupied triples i >= j >= k.
+  // d_per_triple (device, 3 per triple, optional): each triple's unweighted
+  // sums (E[4]T, E[4]T + E[5]ST, E4TAAA pieces) in the order i, j <= i, k <= j.
+  int triples(double* out, double* d_per_triple);
   double* Lam() { return Lam_; }     // [Lambda1 | Lambda2(AB)] current
   double* Lamnew() { return Lamn_; }
 
@@ -174,7 +183,7 @@ class CcsdEngine {
   hipStream_t s_;
   long bytes_ = 0;
   bool profile_ = false;
-  static constexpr int kProfCats = 6;
+  static constexpr int kProfCats = 7;
   double prof_[3 * kProfCats] = {};
   int prof_cat_ = 0;
   std::vector<void*> allocs_;
@@ -258,6 +267,11 @@ class CcsdEngine {
   static constexpr int kLamLp = 9;
   LowP lam_lp_[kLamLp];
   bool lam_lp_valid_ = false;
+  // (T) buffers (allocated on first use): W (v^3), canonical abc tile table,
+  // energy-pass partials, per-triple sums
+  double *trW_ = nullptr, *trPart_ = nullptr, *trOut_ = nullptr;
+  int* trTiles_ = nullptr;
+  long trNblk_ = 0;
 };
 
 }  // namespace aces

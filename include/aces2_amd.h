@@ -118,12 +118,21 @@ int aces_ccsd_set_fock(void* handle, const double* d_fock);
  * out[3c+0] ms, out[3c+1] algorithmic flops (2mnk), out[3c+2] launches, for
  * c = 0 particle ladder, 1 W(mbej) build, 2 ring, 3 the other fp64
  * contractions of the increments, 4 the reverse pass (fp64), 5 the reverse
- * pass's bf16x3 products (mixed Lambda cycles) (out holds 18 values; the
- * irrep-blocked engine's aces_sym_profile fills the first 15).
+ * pass's bf16x3 products (mixed Lambda cycles), 6 the (T) products (out holds
+ * 21 values; the irrep-blocked engine's aces_sym_profile fills the first 15).
  * No reference counterpart
  * (the reference has only whole-program timers, acescore/aces_fin.F:29-55). */
 int aces_ccsd_set_profiling(void* handle, int on);
 int aces_ccsd_profile(void* handle, double* out, int reset);
+/* Non-iterative triples of CCSD(T) at the engine's current amplitudes (TRPS,
+ * trp/trps.F:4, called from vcc/vcc.F:1199-1221; closed-shell HF reference,
+ * pair-symmetric T2 as every CC iterate is).  out (host double[8]): E[4]T,
+ * E[5]ST, E4TAAA, E4TBAA (E[4]T = 2 (E4TAAA + E4TBAA)), ms, MFMA flops,
+ * number of occupied triples i >= j >= k; CCSD(T) = E_CCSD + E[4]T + E[5]ST.
+ * d_per_triple (device, optional, 3 doubles per triple in the order i,
+ * j <= i, k <= j): each triple's unweighted E[4]T, E[4]T + E[5]ST and E4TAAA
+ * sums. */
+int aces_ccsd_triples(void* handle, double* out, double* d_per_triple);
 
 /* ------------------------------------------------------------------------
  * Differentiation form: Lambda equations and response density (reverse pass

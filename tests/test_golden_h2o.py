@@ -170,3 +170,44 @@ def test_fortran_host_reproduces_reference_output(h2o, tmp_path):
     np.testing.assert_allclose(aa, FIX['cc']['aa_contributions'], rtol=0, atol=1.001e-7)
     np.testing.assert_allclose(ab, FIX['cc']['ab_contributions'], rtol=0, atol=1.001e-7)
     assert 'Amplitude equations converged in    10 iterations.' in out.stdout
+
+
+def test_oracle_triples_match_reference(h2o, cc_run):
+    """E4T, E4TAAA, E4TBAA and E5ST of out.zmat.014a's TRPS at the converged amplitudes."""
+    from oracle import triples
+    enuc, escf, F, eri, labels = h2o
+    g, h = cc_run
+    r = triples.rccsd_triples(h['t1'], h['t2'], F, g, O)
+    ref = FIX['ccsd_t_run']['triples']
+    for key in ('e4t', 'e5st', 'e4t_aaa', 'e4t_baa'):
+        assert abs(r[key] - ref[key]) < 1e-11, (key, r[key], ref[key])
+    assert abs(FIX['ccsd_total_energy'] + r['e4t'] + r['e5st'] - ref['ccsd_t']) < 1e-9
+
+
+def test_spin_orbital_triples_match_reference(h2o, cc_run):
+    """The spin-orbital (T) brute force on the same amplitudes (10 occupied, 42
+    virtual spin orbitals) gives the reference's pieces too."""
+    from oracle import spinorb, triples
+    enuc, escf, F, eri, labels = h2o
+    g, h = cc_run
+    Fs, G, no = spinorb.spin_orbital_hamiltonian(F, eri, O)
+    T1, T2 = spinorb.closed_to_so_amplitudes(h['t1'], h['t2'])
+    e4, e5, eaaa = triples.spinorb_triples(T1, T2, Fs, G, no)
+    ref = FIX['ccsd_t_run']['triples']
+    assert abs(e4 - ref['e4t']) < 1e-11 and abs(e5 - ref['e5st']) < 1e-11 and abs(eaaa - ref['e4t_aaa']) < 1e-11
+
+
+@pytest.mark.gpu
+def test_gpu_ccsd_t_matches_reference(h2o):
+    """CCSD on the GPU engine, then its (T): the reference's CCSD(T) energy and pieces."""
+    from aces2.ccsd import CCSD
+    print('case golden H2O (T) gpu', flush=True)
+    enuc, escf, F, eri, labels = h2o
+    cc = CCSD(O, F.shape[0] - O, _blocks(F, eri))
+    h = cc.run(maxcyc=50, conv=FIX['cc_conv'], order=FIX['cc_exporder'])
+    r = cc.triples()
+    cc.close()
+    ref = FIX['ccsd_t_run']['triples']
+    for key in ('e4t', 'e5st', 'e4t_aaa', 'e4t_baa'):
+        assert abs(r[key] - ref[key]) < 1e-11, (key, r[key], ref[key])
+    assert abs(FIX['escf'] + h['final_energy'] + r['e4t'] + r['e5st'] - ref['ccsd_t']) < 1e-9

@@ -128,6 +128,29 @@ def test_element_sampled_increments_match_rccsd(o, v, seed, nonhf):
         np.testing.assert_allclose(slices.mp1(S0, i, a), m2[i, :, a, :], atol=1e-15)
 
 
+@pytest.mark.parametrize('o,v,seed', [(1, 4, 1), (2, 3, 2), (3, 4, 3), (2, 6, 4)])
+def test_closed_shell_triples_match_spin_orbital(o, v, seed):
+    """The closed-shell (T) (one occupied triple i >= j >= k at a time, weights
+    6/3/1) equals the spin-orbital brute force: E[4]T, E[5]ST and the same-spin
+    piece, arbitrary (pair-symmetric) amplitudes."""
+    from oracle import triples
+    f, eri = _ham(o, v, seed, nonhf=False)
+    F, G, no = spinorb.spin_orbital_hamiltonian(f, eri, o)
+    t1, t2 = _rand_closed_t(o, v, seed + 1)
+    T1, T2 = spinorb.closed_to_so_amplitudes(t1, t2)
+    e4, e5, eaaa = triples.spinorb_triples(T1, T2, F, G, no)
+    r = triples.rccsd_triples(t1, t2, f, rccsd.blocks(eri, o), o)
+    if o == 1:
+        # two electrons have no triples: the closed-shell sums cancel to rounding
+        assert abs(e4) == 0.0 and abs(r['e4t']) < 1e-14 and abs(r['e5st']) < 1e-14
+        return
+    scale = abs(e4)
+    assert abs(r['e4t'] - e4) < 1e-13 * scale
+    assert abs(r['e5st'] - e5) < 1e-13 * scale
+    assert abs(r['e4t_aaa'] - eaaa) < 1e-13 * scale
+    assert abs(2 * (r['e4t_aaa'] + r['e4t_baa']) - r['e4t']) < 1e-14 * scale
+
+
 @pytest.mark.parametrize('v,seed', [(4, 1), (9, 2)])
 def test_ccsd_equals_fci_for_two_electrons(v, seed):
     from aces2 import synth

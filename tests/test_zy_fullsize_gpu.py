@@ -6,7 +6,8 @@ it with size-independent properties and with the element-sampled oracle
 
   o=32, v=256   fp64 CCSD: MP1 / closed-form MP2 from T = 0, the reference's
                 20-cycle run from the MP1 start, fp64 increments against the
-                oracle's inc1 and inc2[i, :, a, :] slices, bf16x3 and refinement
+                oracle's inc1 and inc2[i, :, a, :] slices, (T) of sampled
+                occupied triples against the oracle's, bf16x3 and refinement
                 increments against fp64
   o=48, v=320   mixed (bf16x3, refinement polish) vs fp64 energies to 1e-9
                 (|dE| <= 1e-7 after the polish), increments at the solution
@@ -68,6 +69,32 @@ def _check_slices(S, t1, t2, i1, i2, pairs, rtol, what):
     return r1, sl
 
 
+def _check_triples(S, B, t1, t2, per, sample):
+    """Per-triple (T) sums from the GPU against the oracle's for sampled occupied
+    triples i >= j >= k (the (xp|qd) integrals built from the DF factors for the
+    triple's occupied indices only)."""
+    from oracle import triples
+    o, v = S.o, S.v
+    P = S.naux
+    ovvv = {}
+    for trip in sample:
+        for x in trip:
+            if x not in ovvv:
+                ovvv[x] = (S.Bov[:, x, :].T @ S.Bvv.reshape(P, v * v)).reshape(v, v, v)
+    g = {'ovvv': ovvv, 'ovoo': S.ovoo, 'ovov': S.ovov}
+    eo = np.diag(S.f)[:o]
+    ev = np.diag(S.f)[o:]
+    refs = {trip: np.array(triples.triple_energy(t1, t2, g, eo, ev, *trip)) for trip in sample}
+    # a triple i = j = k has no same-spin or mixed triples: its sums are rounding
+    # noise of the six cancelling orderings, checked against the sample's scale
+    scale = max(np.abs(r).max() for r in refs.values())
+    for (i, j, k), ref in refs.items():
+        t = i * (i + 1) * (i + 2) // 6 + j * (j + 1) // 2 + k
+        err = np.abs(per[t] - ref).max() / (scale if i == j == k else np.abs(ref).max())
+        print('(T) triple %s: rel err %.1e' % ((i, j, k), err), flush=True)
+        assert err < 1e-11
+
+
 def test_fullsize_o32_v256_fp64():
     o, v, seed = 32, 256, 20262
     print('case full size o=%d v=%d fp64' % (o, v), flush=True)
@@ -102,6 +129,12 @@ def test_fullsize_o32_v256_fp64():
     e_ref = slices.energy(S, t1h, t2h)
     assert abs(e_gpu - e_ref) < 1e-12 * abs(e_ref)
     _check_slices(S, t1h, t2h, i1h, i2h, pairs, 1e-11, 'fp64 o=32 v=256')
+    # (T) at the 20-cycle amplitudes: sampled occupied triples against the oracle
+    r = cc.triples(per_triple=True)
+    per = r['per_triple'].cpu().numpy()
+    print('(T): E[4]T %.12f E[5]ST %.12f, %d triples in %.0f ms (%.1f TF)' %
+          (r['e4t'], r['e5st'], r['ntriples'], r['ms'], r['flops'] / r['ms'] / 1e9), flush=True)
+    _check_triples(S, B, t1h, t2h, per, [(0, 0, 0), (o - 1, o - 1, o - 2), (o - 1, o // 2, 3), (o - 1, o - 2, o - 3)])
     # bf16x3 increments: fp32-accumulated split products, relative error ~1e-6
     b1, b2 = cc.increments('bf16x3')
     eb = float((b2 - i2).abs().max()) / float(i2.abs().max())
