@@ -47,7 +47,7 @@ SIGNATURES = {
     'aces_ccsd_set_profiling': (c_int, [c_vp, c_int]),
     'aces_ccsd_profile': (c_int, [c_vp, c_dp, c_int]),
     'aces_ccsd_set_fock': (c_int, [c_vp, c_vp]),
-    'aces_ccsd_triples': (c_int, [c_vp, c_dp, c_vp]),
+    'aces_ccsd_triples': (c_int, [c_vp, c_int, c_dp, c_vp]),
     'aces_ccsd_adjoint': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     'aces_ccsd_lambda_start': (c_int, [c_vp]),
     'aces_ccsd_lambda_cycle': (c_int, [c_vp, c_int, c_double, c_dp]),

@@ -99,17 +99,18 @@ class CCSD:
         return {nm: {'ms': out[3 * c], 'flops': out[3 * c + 1], 'launches': int(out[3 * c + 2])}
                 for c, nm in enumerate(names)}
 
-    def triples(self, per_triple=False):
+    def triples(self, per_triple=False, prec='fp64'):
         """(T) of CCSD(T) at the current amplitudes (the reference's TRPS):
         dict with e4t, e5st (E(T) = e4t + e5st), e4t_aaa, e4t_baa, ms, flops,
         ntriples; with ``per_triple`` also the (ntriples, 3) device array of each
-        occupied triple's unweighted sums."""
+        occupied triple's unweighted sums.  ``prec``: 'fp64' or 'bf16x3' (the
+        W products as split bf16 products)."""
         torch = self.torch
         out = (ctypes.c_double * 8)()
         o = self.o
         nt = o * (o + 1) * (o + 2) // 6
         per = torch.empty((nt, 3), device='cuda', dtype=torch.float64) if per_triple else None
-        _lib.call(self._p + 'triples', self.h, out, per.data_ptr() if per is not None else None)
+        _lib.call(self._p + 'triples', self.h, PREC[prec], out, per.data_ptr() if per is not None else None)
         r = {'e4t': out[0], 'e5st': out[1], 'et': out[0] + out[1], 'e4t_aaa': out[2], 'e4t_baa': out[3],
              'ms': out[4], 'flops': out[5], 'ntriples': int(out[6])}
         if per_triple:

@@ -169,13 +169,17 @@ int aces_ccsd_density(void* h, double* dens) {
   return eng(h)->density(dens);
 }
 
-int aces_ccsd_triples(void* h, double* out, double* d_per_triple) {
+int aces_ccsd_triples(void* h, int prec, double* out, double* d_per_triple) {
   REQUIRE_HANDLE(h);
   if (out == nullptr) {
     set_error("aces_ccsd_triples: null output");
     return ACES_ERR_ARG;
   }
-  return eng(h)->triples(out, d_per_triple);
+  if (prec != GP_F64 && prec != GP_BF16X3) {
+    set_error("aces_ccsd_triples: prec must be 0 (fp64) or 2 (bf16x3)");
+    return ACES_ERR_ARG;
+  }
+  return eng(h)->triples((GemmPrec)prec, out, d_per_triple);
 }
 
 }  // extern "C"

@@ -107,7 +107,8 @@ class CcsdEngine {
   // out[5] MFMA flops, out[6] number of occupied triples i >= j >= k.
   // d_per_triple (device, 3 per triple, optional): each triple's unweighted
   // sums (E[4]T, E[4]T + E[5]ST, E4TAAA pieces) in the order i, j <= i, k <= j.
-  int triples(double* out, double* d_per_triple);
+  // prec: GP_F64 or GP_BF16X3 (the W products; everything else fp64).
+  int triples(GemmPrec prec, double* out, double* d_per_triple);
   double* Lam() { return Lam_; }     // [Lambda1 | Lambda2(AB)] current
   double* Lamnew() { return Lamn_; }
 

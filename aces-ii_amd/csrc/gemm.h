@@ -48,6 +48,10 @@ struct GemmDesc {
   unsigned short* C_hi = nullptr;
   unsigned short* C_lo = nullptr;
   long ldc_lp = 0;
+  // bf16x3 block shape request: 0 = chosen from the problem size; 8 = the
+  // 256 x 256 direct-to-LDS kernel (K % 32 == 0) whenever M, N >= 256 and the
+  // launch (tiles x batch) has at least 256 blocks.
+  int tile_hint = 0;
 };
 
 int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream);

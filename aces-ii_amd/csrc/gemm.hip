@@ -892,6 +892,9 @@ static bool bf16_direct(const GemmDesc* d, int ng) {
 // (direct-to-LDS kernel; every problem at least 1024 x 1024 and with at least
 // 256 such tiles or a split-K workspace to fill the chip); else 4 = 256 x 128.
 static int bf16_shape(const GemmDesc* d, int ng) {
+  if (ng == 1 && d[0].tile_hint == 8 && bf16_direct(d, 1) && d[0].M >= 256 && d[0].N >= 256 &&
+      (long)((d[0].M + 255) / 256) * ((d[0].N + 255) / 256) * d[0].batch >= 256)
+    return 8;
   bool any = false, wide = bf16_direct(d, ng);
   for (int g = 0; g < ng; ++g)
     if (d[g].M > 0 && d[g].N > 0) {

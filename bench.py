@@ -31,7 +31,13 @@ the irrep-blocked engine (one grouped MFMA launch per contraction term) in the
 mixed form, followed outside the timed region by the refinement polish to 1e-9 and
 the Lambda equations on the same lists.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ccsd|lambda|sym]
+With ``--workload triples`` the step is one evaluation of the non-iterative
+triples correction (T) at o=32, v=256 on the amplitudes of a 20-cycle fp64
+CCSD run from the MP1 start (the o=32, v=256 configuration's run): the W
+products in fp64, and outside the timed region the same evaluation with bf16x3
+W products and its deviation from fp64.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ccsd|lambda|sym|triples]
 """
 import argparse
 import json
@@ -143,7 +149,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--workload', default='ccsd', choices=['ccsd', 'lambda', 'sym'])
+    ap.add_argument('--workload', default='ccsd', choices=['ccsd', 'lambda', 'sym', 'triples'])
     ap.add_argument('--o', type=int, default=None)
     ap.add_argument('--v', type=int, default=None)
     ap.add_argument('--seed', type=int, default=20260)
@@ -165,6 +171,8 @@ def main():
         return bench_lambda(args)
     if args.workload == 'sym':
         return bench_sym(args)
+    if args.workload == 'triples':
+        return bench_triples(args)
     import numpy as np
     import torch
     from aces2 import synth
@@ -787,6 +795,135 @@ def bench_sym(args):
     }
     print(json.dumps(result), flush=True)
 
+
+
+def cpu_baseline_triples(o, v, eps, B, t1, t2, nsample=2):
+    """The oracle's per-triple (T) work (oracle.triples.triple_energy, numpy/BLAS)
+    timed on the host for a sample of occupied triples with three distinct
+    indices at the full (o, v), scaled to all o(o+1)(o+2)/6 triples."""
+    import numpy as np
+    from oracle import triples
+    threads = int(os.environ.get('OMP_NUM_THREADS') or min(16, os.cpu_count() or 1))
+    naux = B.shape[0]
+    Bov = np.ascontiguousarray(B[:, :o, o:])
+    Bvv = np.ascontiguousarray(B[:, o:, o:]).reshape(naux, v * v)
+    Boo = np.ascontiguousarray(B[:, :o, :o])
+    ovoo = (Bov.reshape(naux, o * v).T @ Boo.reshape(naux, o * o)).reshape(o, v, o, o)
+    ovov = (Bov.reshape(naux, o * v).T @ Bov.reshape(naux, o * v)).reshape(o, v, o, v)
+    eo, ev = eps[:o], eps[o:]
+    sample = [(o - 1, o // 2, 0), (o - 2, o // 3, 1)][:nsample]
+    ovvv = {}
+    for trip in sample:
+        for x in trip:
+            if x not in ovvv:
+                ovvv[x] = (Bov[:, x, :].T @ Bvv).reshape(v, v, v)
+    g = {'ovvv': ovvv, 'ovoo': ovoo, 'ovov': ovov}
+    triples.triple_energy(t1, t2, g, eo, ev, *sample[0])
+    t0 = time.perf_counter()
+    for trip in sample:
+        triples.triple_energy(t1, t2, g, eo, ev, *trip)
+    per = (time.perf_counter() - t0) / len(sample)
+    ntrip = o * (o + 1) * (o + 2) // 6
+    return {'value': per * ntrip, 'unit': 's/eval', 'cores': threads, 'kind': 'port',
+            'sample': 'oracle.triples.triple_energy (numpy/OpenBLAS, %d threads; W of one occupied triple from '
+                      'six v^3 particle and six hole products, then the energy sums) for %d triples with distinct '
+                      'indices at o=%d, v=%d: %.3f s per triple, x%d triples' % (threads, len(sample), o, v, per,
+                                                                                ntrip)}
+
+
+def bench_triples(args):
+    """(T) of CCSD(T) at o=32, v=256 after the 20-cycle fp64 CCSD run."""
+    import numpy as np
+    import torch
+    from aces2 import synth
+    from aces2.ccsd import CCSD
+
+    o = args.o or 32
+    v = args.v or 256
+    log = lambda *a: print('[bench]', *a, file=sys.stderr, flush=True)
+    t_setup = time.perf_counter()
+    eps, B = synth.draw(o, v, args.seed, sigma=synth.scaled_sigma(o, v))
+    blocks = synth.blocks_device(eps, B, o)
+    cc = CCSD(o, v, blocks)
+    del blocks
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    log('engine set up: %.1f GB device, %.1f s' % (cc.device_bytes / 1e9, time.perf_counter() - t_setup))
+    h = cc.run(maxcyc=20, conv=14, order=6, start='mp1', raise_on_fail=False)
+    e_cc = h['energy'][-1]
+    log('CCSD: 20 fp64 cycles, E = %.12f' % e_cc)
+    steps, warmup = max(1, args.steps), max(0, args.warmup)
+    for _ in range(warmup):
+        cc.triples(prec='fp64')
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = cc.triples(prec='fp64')
+    torch.cuda.synchronize()
+    s_eval = (time.perf_counter() - t0) / steps
+    log('(T) fp64: %.3f s per evaluation, E[4]T %.12f E[5]ST %.12f' % (s_eval, r['e4t'], r['e5st']))
+    # per-launch timing of the W products (hipEvents on the engine's stream)
+    cc.set_profiling(True)
+    cc.profile(reset=True)
+    cc.triples(prec='fp64')
+    p = cc.profile(reset=True)['triples']
+    rb = cc.triples(prec='bf16x3')
+    pb = cc.profile(reset=True)['triples']
+    cc.set_profiling(False)
+    t0 = time.perf_counter()
+    rb = cc.triples(prec='bf16x3')
+    s_bf = time.perf_counter() - t0
+    ntrip = r['ntriples']
+    algo = ntrip * 6 * 2.0 * v ** 3 * (v + o)
+    kp64 = (v + o + 15) // 16 * 16
+    exec64 = ntrip * 6 * 2.0 * v ** 3 * kp64
+    achieved = algo / (p['ms'] * 1e-3)
+    roofline = {'bound': 'mfma', 'kernel': 'gemm_f64_dl_kernel',
+                'achieved': achieved / 1e12, 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
+                'frac': achieved / FP64_PEAK, 'traffic': None,
+                'flops_basis': 'algorithmic 2 v^3 (v + o) per permutation, 6 per occupied triple: %.4g per (T)' % algo,
+                'executed_TFLOPs': exec64 / (p['ms'] * 1e-3) / 1e12,
+                'avg_launch_ms': p['ms'] / max(p['launches'], 1), 'launches_per_eval': p['launches'],
+                'gemm_share_of_eval': p['ms'] * 1e-3 / s_eval}
+    tr = measured_traffic('triples_o%d_v%d_fp64' % (o, v))
+    if tr is not None:
+        roofline['traffic'] = tr['bytes_per_launch']
+        roofline['traffic_detail'] = tr
+    log('roofline', json.dumps(roofline))
+    dE = max(abs(rb[k] - r[k]) for k in ('e4t', 'e5st', 'e4t_aaa', 'e4t_baa'))
+    result = {
+        'metric': '(T) s per evaluation at o=32,v=256 (fp64 W products)',
+        'value': s_eval,
+        'unit': 's/eval',
+        'n_gpus': 1,
+        'steps': steps,
+        'warmup': warmup,
+        'ms_per_step': s_eval * 1e3,
+        'higher_is_better': False,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic (DF integrals, seed %d, sigma=0.05*sqrt(95/(o*v))), amplitudes of 20 fp64 CCSD cycles'
+                % args.seed,
+        'config': {'workload': 'closed-shell (T) after CCSD, RHF C1, o=%d v=%d' % (o, v), 'o': o, 'v': v,
+                   'triples': ntrip},
+        'energies': {'ccsd_corr': e_cc, 'e4t': r['e4t'], 'e5st': r['e5st'], 'e4t_aaa': r['e4t_aaa'],
+                     'e4t_baa': r['e4t_baa']},
+        'algorithmic_TFLOPs_eval': algo / s_eval / 1e12,
+        'roofline': roofline,
+        'bf16x3': {'s_per_eval': s_bf, 'abs_dE_vs_fp64': dE, 'rel_dE_vs_fp64': dE / abs(r['e4t']),
+                   'algorithmic_TFLOPs_eval': algo / s_bf / 1e12,
+                   'W_products_ms': pb['ms'], 'W_products_executed_TFLOPs':
+                       3.0 * ntrip * 6 * 2.0 * v ** 3 * ((v + o + 31) // 32 * 32) / (pb['ms'] * 1e-3) / 1e12},
+    }
+    if not args.no_cpu:
+        t1, t2 = cc.get_amplitudes()
+        t1h, t2h = t1.cpu().numpy(), t2.cpu().numpy()
+        cc.close()
+        result['cpu_baseline'] = cpu_baseline_triples(o, v, eps, B, t1h, t2h)
+    else:
+        result['cpu_baseline'] = None
+    print(json.dumps(result), flush=True)
 
 if __name__ == '__main__':
     main()

@@ -131,8 +131,11 @@ int aces_ccsd_profile(void* handle, double* out, int reset);
  * number of occupied triples i >= j >= k; CCSD(T) = E_CCSD + E[4]T + E[5]ST.
  * d_per_triple (device, optional, 3 doubles per triple in the order i,
  * j <= i, k <= j): each triple's unweighted E[4]T, E[4]T + E[5]ST and E4TAAA
- * sums. */
-int aces_ccsd_triples(void* handle, double* out, double* d_per_triple);
+ * sums.  prec: 0 = fp64 W products (the reference's arithmetic), 2 = bf16x3
+ * split products with fp32 accumulation (relative error of the triples energy
+ * ~1e-6); the energy pass is fp64 either way.  Uses o v^2 (v + o) + o^2 v (v + o)
+ * elements (8 bytes fp64, 4 bytes bf16x3) of device memory during the call. */
+int aces_ccsd_triples(void* handle, int prec, double* out, double* d_per_triple);
 
 /* ------------------------------------------------------------------------
  * Differentiation form: Lambda equations and response density (reverse pass

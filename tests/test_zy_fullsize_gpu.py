@@ -135,6 +135,11 @@ def test_fullsize_o32_v256_fp64():
     print('(T): E[4]T %.12f E[5]ST %.12f, %d triples in %.0f ms (%.1f TF)' %
           (r['e4t'], r['e5st'], r['ntriples'], r['ms'], r['flops'] / r['ms'] / 1e9), flush=True)
     _check_triples(S, B, t1h, t2h, per, [(0, 0, 0), (o - 1, o - 1, o - 2), (o - 1, o // 2, 3), (o - 1, o - 2, o - 3)])
+    rb = cc.triples(prec='bf16x3')
+    eb = max(abs(rb[k] - r[k]) for k in ('e4t', 'e5st', 'e4t_aaa', 'e4t_baa')) / abs(r['e4t'])
+    print('(T) bf16x3: %.0f ms (%.1f TF algorithmic), rel err vs fp64 %.1e' %
+          (rb['ms'], rb['flops'] / rb['ms'] / 1e9, eb), flush=True)
+    assert eb < 2e-5
     # bf16x3 increments: fp32-accumulated split products, relative error ~1e-6
     b1, b2 = cc.increments('bf16x3')
     eb = float((b2 - i2).abs().max()) / float(i2.abs().max())
