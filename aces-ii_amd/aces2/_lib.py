@@ -92,6 +92,7 @@ SIGNATURES = {
     'aces_putall': (c_int, [c_vp, c_long, c_int, c_int, c_int]),
     'aces_sumsym2': (c_int, [c_vp, c_int, c_int, c_int]),
     'aces_list_device_ptr': (c_vp, [c_int, c_int]),
+    'aces_list_c1': (c_int, [c_int, c_int, c_int, c_int, c_vp, c_int]),
     'aces_putrec': (c_int, [c_char_p, c_long, c_vp]),
     'aces_getrec': (c_int, [c_int, c_char_p, ctypes.POINTER(c_long), c_vp]),
     'aces_vcc_mp1': (c_int, []),
@@ -112,11 +113,25 @@ class AcesError(RuntimeError):
     pass
 
 
+def _init_torch_runtime():
+    """PyTorch-ROCm ships its own HIP runtime; when both it and the system one
+    that the library links serve one process, the GPU must be opened through
+    torch's first (the other order leaves torch without devices), so torch's
+    runtime is initialised before the library's first call."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise AcesError('libaces2_amd.so is not built (%s); run __graft_entry__.build()' % LIB_PATH)
+        _init_torch_runtime()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -15,6 +15,11 @@
 
 namespace aces {
 
+// Pair type `type` (1..25): orbital spaces of its left (p) and right (q) member,
+// each 0 = occupied alpha, 1 = virtual alpha, 2 = occupied beta, 3 = virtual
+// beta, and its packing (0 full, 1 p < q, 2 p <= q).  False for an unknown type.
+bool pair_type(int type, int* space_p, int* space_q, int* pack);
+
 struct DpdSym {
   int nirrep = 1;
   int pop[2][8] = {{0}};  // occupied per irrep, spin 0 = alpha, 1 = beta

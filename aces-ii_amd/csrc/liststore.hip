@@ -28,6 +28,15 @@ const PairType kTypes[26] = {
     {'O', 'V', 1, 0, FULL}};
 }  // namespace
 
+bool pair_type(int type, int* space_p, int* space_q, int* pack) {
+  if (type < 1 || type > 25) return false;
+  const PairType& t = kTypes[type];
+  *space_p = 2 * t.sl + (t.l == 'V' ? 1 : 0);
+  *space_q = 2 * t.sr + (t.r == 'V' ? 1 : 0);
+  *pack = (t.pk == FULL) ? 0 : (t.pk == PACK ? 1 : 2);
+  return true;
+}
+
 static inline int space_count(const DpdSym& s, char sp, int spin, int irrep) {
   return sp == 'O' ? s.pop[spin][irrep] : s.vrt[spin][irrep];
 }

@@ -35,6 +35,12 @@ class SymEngine {
   int start_mp1();
   int increments(GemmPrec prec, double* inc1, double* inc2);  // dense outputs (may be null)
   int cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st);
+  // DODIIS0 of the list driver (vcc/dodiis0.F) on T and T_new given as dense
+  // arrays: st.energy_new = E(T_new); unless converged the next amplitudes
+  // (the extrapolated vector) are the engine's current ones (get_amplitudes),
+  // converged: the current ones are T, pair-symmetrized, and st.energy_final E(T).
+  int diis_dense(int icycle, double tol, const double* t1, const double* t2, const double* t1n, const double* t2n,
+                 CcsdStats& st);
   int energy_current(double& e);
   int set_diis_order(int order);
   int reset_diis();

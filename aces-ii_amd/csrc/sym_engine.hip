@@ -1564,6 +1564,45 @@ int SymEngine::cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st) {
   return ACES_OK;
 }
 
+int SymEngine::diis_dense(int icycle, double tol, const double* t1, const double* t2, const double* t1n,
+                          const double* t2n, CcsdStats& st) {
+  ACES_REQUIRE(ready_, "sym: not set up");
+  ACES_REQUIRE(icycle >= 1, "sym DODIIS0: icycle starts at 1");
+  const long s1[2] = {v, 1};
+  const long s2[4] = {(long)o * v * v, (long)v * v, v, 1};
+  ACES_CHECK(pack_dense(t1_, t1, s1, 0));
+  ACES_CHECK(pack_dense(t2_, t2, s2, 0));
+  // T_new through the t1/t2 tensors' layouts into T_new_
+  {
+    double* keep1 = T_[t1_].d;
+    double* keep2 = T_[t2_].d;
+    T_[t1_].d = T_new_;
+    T_[t2_].d = T_new_ + n1_;
+    int rc = pack_dense(t1_, t1n, s1, 0);
+    if (rc == ACES_OK) rc = pack_dense(t2_, t2n, s2, 0);
+    T_[t1_].d = keep1;
+    T_[t2_].d = keep2;
+    ACES_CHECK(rc);
+  }
+  double enew;
+  ACES_CHECK(energy_dev(T_new_, T_new_ + n1_, enew));
+  st.energy_new = enew;
+  ACES_CHECK(diis_core(sol_, errv_, R_, order_, T_cur_, T_new_, icycle, tol, false, st));
+  if (anc_last_ == 1)
+    anc_res_ = std::fabs(st.resmiter);
+  else if (anc_last_ == 2 && std::fabs(st.resmiter) < 1e-3 * anc_res_)
+    anc_valid_ = false;
+  anc_last_ = 0;
+  if (st.converged) {
+    double e;
+    ACES_CHECK(energy_dev(T_cur_, T_cur_ + n1_, e));
+    st.energy_final = e;
+  }
+  fwd_valid_ = false;
+  ACES_HIP(hipStreamSynchronize(s_));
+  return ACES_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Differentiation form
 int SymEngine::adjoint(const double* l1, const double* l2, double* r1, double* r2, double* dens) {

@@ -38,22 +38,41 @@
 #include "ccsd_engine.h"
 #include "ccsd_kernels.h"
 #include "common.h"
+#include "dpd_lists.h"
 #include "liststore.h"
+#include "sym_engine.h"
 #include "tensor_ops.h"
 
 namespace aces {
+
+// Dense amplitude-space buffers of the irrep-blocked path (nirrep > 1): the
+// symmetry-packed amplitude lists are moved through C1 order, in which the
+// irrep-blocked engine takes and returns its amplitudes.
+struct SymBuffers {
+  double* base = nullptr;
+  double *t1 = nullptr, *t2 = nullptr, *t1n = nullptr, *t2n = nullptr, *inc1 = nullptr, *inc2 = nullptr;
+  double *x1 = nullptr, *x2 = nullptr, *aa = nullptr, *l16 = nullptr, *eo = nullptr, *ev = nullptr;
+  ~SymBuffers() {
+    if (base) (void)hipFree(base);
+  }
+};
 
 struct AcesContext {
   hipStream_t stream = nullptr;
   std::unique_ptr<ListStore> lists;
   RecordStore recs;
-  std::unique_ptr<CcsdEngine> eng;
+  std::unique_ptr<CcsdEngine> eng;   // C1 lists (nirrep = 1)
+  std::unique_ptr<SymEngine> seng;   // symmetry-packed lists (nirrep > 1)
+  SymBuffers sb;
+  int o = 0, v = 0;
   int prec = GP_F64;
   int last_cycle = 0;
   double* fov = nullptr;    // f(i,a) of the MO Fock matrix (zero for HF), [i][a]
   double* epart = nullptr;  // CMPENG partial sums: o^2 blocks x 4, then the 5 totals
+  bool ready() const { return eng != nullptr || seng != nullptr; }
   ~AcesContext() {
     eng.reset();
+    seng.reset();
     if (fov) (void)hipFree(fov);
     if (epart) (void)hipFree(epart);
   }
@@ -181,10 +200,87 @@ static int require_list(AcesContext* c, int list, long rows, long cols, const ch
   return ACES_OK;
 }
 
+// Dense integral blocks (the engines' set-up inputs) from the integral lists in
+// C1 order (device pointers; see the list contract at the top of this file).
+struct DenseBlocks {
+  double *f = nullptr, *ovov = nullptr, *oovv = nullptr, *ovoo = nullptr, *oooo = nullptr, *ovvv = nullptr;
+  ~DenseBlocks() {
+    for (double* p : {f, ovov, oovv, ovoo, oooo, ovvv})
+      if (p) (void)hipFree(p);
+  }
+};
+
+static int dense_blocks(int o, int v, const double* L16, const double* L13, const double* L10, const double* L30,
+                        const double* L23, hipStream_t s, DenseBlocks& b) {
+  const long oo = (long)o * o, vv = (long)v * v, ov = (long)o * v;
+  ACES_HIP(hipMalloc(&b.ovov, sizeof(double) * ov * ov));
+  ACES_HIP(hipMalloc(&b.oovv, sizeof(double) * ov * ov));
+  ACES_HIP(hipMalloc(&b.ovoo, sizeof(double) * ov * oo));
+  ACES_HIP(hipMalloc(&b.oooo, sizeof(double) * oo * oo));
+  ACES_HIP(hipMalloc(&b.ovvv, sizeof(double) * ov * vv));
+  auto perm = [&](double* out, long d0, long d1, long d2, long d3, const double* a, double aa, const long* sa,
+                  const double* bb, double ab, const long* sbb) {
+    Permute4 q;
+    q.d[0] = d0; q.d[1] = d1; q.d[2] = d2; q.d[3] = d3;
+    q.out = out;
+    q.in[0] = a; q.alpha[0] = aa;
+    for (int k = 0; k < 4; ++k) q.st[0][k] = sa[k];
+    if (bb) {
+      q.in[1] = bb; q.alpha[1] = ab;
+      for (int k = 0; k < 4; ++k) q.st[1][k] = sbb[k];
+    }
+    return permute4(q, s);
+  };
+  // ovov[i][a][j][b] = <ab|ij>, list 16 element a + v b + v^2 (i + o j)
+  { const long st[4] = {vv, 1, vv * o, v}; ACES_CHECK(perm(b.ovov, o, v, o, v, L16, 1.0, st, nullptr, 0, nullptr)); }
+  // oooo[m][i][n][j] = <mn|ij>, list 13 element m + o n + o^2 (i + o j)
+  { const long st[4] = {1, oo, o, oo * o}; ACES_CHECK(perm(b.oooo, o, o, o, o, L13, 1.0, st, nullptr, 0, nullptr)); }
+  // ovoo[j][a][i][k] = <ij|ka>, list 10 element i + o j + o^2 (k + o a)
+  { const long st[4] = {o, oo * o, 1, oo}; ACES_CHECK(perm(b.ovoo, o, v, o, o, L10, 1.0, st, nullptr, 0, nullptr)); }
+  // ovvv[i][b][a][c] = <ab|ci>, list 30 element a + v b + v^2 (c + v i)
+  { const long st[4] = {vv * v, v, 1, vv}; ACES_CHECK(perm(b.ovvv, o, v, v, v, L30, 1.0, st, nullptr, 0, nullptr)); }
+  // oovv[i][j][a][b] = (ij|ab) = <AJ||BI>(AI,BJ) + (ai|bj); list 23 element a + v i + ov (b + v j)
+  {
+    const long st[4] = {v, ov * v, 1, ov};
+    const long so[4] = {ov * v, v, ov, 1};  // ovov[i][a][j][b] over output (i, j, a, b)
+    ACES_CHECK(perm(b.oovv, o, o, v, v, L23, 1.0, st, b.ovov, 1.0, so));
+  }
+  return ACES_OK;
+}
+
+// The MO Fock matrix (host, n x n, orbitals in list order: occupied then
+// virtual, irrep-sorted in each) from record FOCKMOA, else diag(SCFEVALA); and
+// f(i,a) plus the CMPENG scratch on the device.
+static int fock_from_records(AcesContext* c, int o, int v, std::vector<double>& f) {
+  const int n = o + v;
+  f.assign((size_t)n * n, 0.0);
+  long len = (long)n * n;
+  if (c->recs.has("FOCKMOA")) {
+    ACES_CHECK(c->recs.getrec(1, "FOCKMOA", &len, f.data()));
+  } else {
+    std::vector<double> e(n);
+    long ne = n;
+    ACES_CHECK(c->recs.getrec(1, "SCFEVALA", &ne, e.data()));
+    for (int p = 0; p < n; ++p) f[(size_t)p * n + p] = e[p];
+  }
+  const long ov = (long)o * v, oo = (long)o * o;
+  std::vector<double> fo((size_t)ov);
+  for (int i = 0; i < o; ++i)
+    for (int a = 0; a < v; ++a) fo[(size_t)i * v + a] = f[(size_t)i * n + o + a];
+  if (c->fov) (void)hipFree(c->fov);
+  if (c->epart) (void)hipFree(c->epart);
+  c->fov = nullptr;
+  c->epart = nullptr;
+  ACES_HIP(hipMalloc(&c->fov, sizeof(double) * ov));
+  ACES_HIP(hipMalloc(&c->epart, sizeof(double) * (4 * oo + 8)));
+  ACES_HIP(hipMemcpyAsync(c->fov, fo.data(), sizeof(double) * ov, hipMemcpyHostToDevice, c->stream));
+  ACES_HIP(hipStreamSynchronize(c->stream));
+  return ACES_OK;
+}
+
 // Build the engine from the integral lists (C1 RHF).
-static int setup_engine(AcesContext* c) {
+static int setup_engine_c1(AcesContext* c) {
   const DpdSym& sy = c->lists->sym();
-  ACES_REQUIRE(sy.nirrep == 1, "the list-fed CCSD engine takes C1 lists (nirrep = 1)");
   const int o = sy.pop[0][0], v = sy.vrt[0][0], n = o + v;
   ACES_REQUIRE(o >= 1 && v >= 1, "need occupied and virtual orbitals");
   const long oo = (long)o * o, vv = (long)v * v, ov = (long)o * v;
@@ -195,86 +291,114 @@ static int setup_engine(AcesContext* c) {
   ACES_CHECK(require_list(c, 233, (long)v * (v + 1) / 2, vv, "<Ab|Cd>"));
   ACES_CHECK(require_list(c, 23, ov, ov, "<AJ||BI>"));
   hipStream_t s = c->stream;
-  std::vector<double> f((size_t)n * n, 0.0);
-  long len = (long)n * n;
-  if (c->recs.has("FOCKMOA")) {
-    ACES_CHECK(c->recs.getrec(1, "FOCKMOA", &len, f.data()));
-  } else {
-    std::vector<double> e(n);
-    long ne = n;
-    ACES_CHECK(c->recs.getrec(1, "SCFEVALA", &ne, e.data()));
-    for (int p = 0; p < n; ++p) f[(size_t)p * n + p] = e[p];
-  }
-  double *d_f, *ovov, *oovv, *ovoo, *oooo, *ovvv;
-  ACES_HIP(hipMalloc(&d_f, sizeof(double) * n * n));
-  ACES_HIP(hipMalloc(&ovov, sizeof(double) * ov * ov));
-  ACES_HIP(hipMalloc(&oovv, sizeof(double) * ov * ov));
-  ACES_HIP(hipMalloc(&ovoo, sizeof(double) * ov * oo));
-  ACES_HIP(hipMalloc(&oooo, sizeof(double) * oo * oo));
-  ACES_HIP(hipMalloc(&ovvv, sizeof(double) * ov * vv));
-  ACES_HIP(hipMemcpyAsync(d_f, f.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, s));
-  auto perm = [&](double* out, long d0, long d1, long d2, long d3, const double* a, double aa, const long* sa,
-                  const double* b, double ab, const long* sb) {
-    Permute4 q;
-    q.d[0] = d0; q.d[1] = d1; q.d[2] = d2; q.d[3] = d3;
-    q.out = out;
-    q.in[0] = a; q.alpha[0] = aa;
-    for (int k = 0; k < 4; ++k) q.st[0][k] = sa[k];
-    if (b) {
-      q.in[1] = b; q.alpha[1] = ab;
-      for (int k = 0; k < 4; ++k) q.st[1][k] = sb[k];
-    }
-    return permute4(q, s);
-  };
-  const double* L16 = c->lists->device_ptr(1, 16);
-  const double* L13 = c->lists->device_ptr(1, 13);
-  const double* L10 = c->lists->device_ptr(1, 10);
-  const double* L30 = c->lists->device_ptr(1, 30);
-  const double* L23 = c->lists->device_ptr(1, 23);
-  // ovov[i][a][j][b] = <ab|ij>, list 16 element a + v b + v^2 (i + o j)
-  { const long st[4] = {vv, 1, vv * o, v}; ACES_CHECK(perm(ovov, o, v, o, v, L16, 1.0, st, nullptr, 0, nullptr)); }
-  // oooo[m][i][n][j] = <mn|ij>, list 13 element m + o n + o^2 (i + o j)
-  { const long st[4] = {1, oo, o, oo * o}; ACES_CHECK(perm(oooo, o, o, o, o, L13, 1.0, st, nullptr, 0, nullptr)); }
-  // ovoo[j][a][i][k] = <ij|ka>, list 10 element i + o j + o^2 (k + o a)
-  { const long st[4] = {o, oo * o, 1, oo}; ACES_CHECK(perm(ovoo, o, v, o, o, L10, 1.0, st, nullptr, 0, nullptr)); }
-  // ovvv[i][b][a][c] = <ab|ci>, list 30 element a + v b + v^2 (c + v i)
-  { const long st[4] = {vv * v, v, 1, vv}; ACES_CHECK(perm(ovvv, o, v, v, v, L30, 1.0, st, nullptr, 0, nullptr)); }
-  // oovv[i][j][a][b] = (ij|ab) = <AJ||BI>(AI,BJ) + (ai|bj); list 23 element a + v i + ov (b + v j)
-  {
-    const long st[4] = {v, ov * v, 1, ov};
-    const long so[4] = {ov * v, v, ov, 1};  // ovov[i][a][j][b] over output (i, j, a, b)
-    ACES_CHECK(perm(oovv, o, o, v, v, L23, 1.0, st, ovov, 1.0, so));
-  }
-  // f(i,a) for the non-HF singles term of CMPENG and its partial-sum scratch
-  {
-    std::vector<double> fo((size_t)ov);
-    for (int i = 0; i < o; ++i)
-      for (int a = 0; a < v; ++a) fo[(size_t)i * v + a] = f[(size_t)i * n + o + a];
-    if (c->fov) (void)hipFree(c->fov);
-    if (c->epart) (void)hipFree(c->epart);
-    c->fov = nullptr;
-    c->epart = nullptr;
-    ACES_HIP(hipMalloc(&c->fov, sizeof(double) * ov));
-    ACES_HIP(hipMalloc(&c->epart, sizeof(double) * (4 * oo + 8)));
-    ACES_HIP(hipMemcpyAsync(c->fov, fo.data(), sizeof(double) * ov, hipMemcpyHostToDevice, s));
-    ACES_HIP(hipStreamSynchronize(s));
-  }
+  std::vector<double> f;
+  ACES_CHECK(fock_from_records(c, o, v, f));
+  DenseBlocks b;
+  ACES_HIP(hipMalloc(&b.f, sizeof(double) * n * n));
+  ACES_HIP(hipMemcpyAsync(b.f, f.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, s));
+  ACES_CHECK(dense_blocks(o, v, c->lists->device_ptr(1, 16), c->lists->device_ptr(1, 13), c->lists->device_ptr(1, 10),
+                          c->lists->device_ptr(1, 30), c->lists->device_ptr(1, 23), s, b));
   c->eng.reset(new CcsdEngine(o, v, s));
-  const int rc = c->eng->setup(d_f, ovov, oovv, ovoo, oooo, ovvv, c->lists->device_ptr(1, 233));
-  (void)hipFree(d_f);
-  (void)hipFree(ovov);
-  (void)hipFree(oovv);
-  (void)hipFree(ovoo);
-  (void)hipFree(oooo);
-  (void)hipFree(ovvv);
-  if (rc != ACES_OK) c->eng.reset();
-  return rc;
+  const int rc = c->eng->setup(b.f, b.ovov, b.oovv, b.ovoo, b.oooo, b.ovvv, c->lists->device_ptr(1, 233));
+  if (rc != ACES_OK) {
+    c->eng.reset();
+    return rc;
+  }
+  c->o = o;
+  c->v = v;
+  return ACES_OK;
+}
+
+// Build the irrep-blocked engine from symmetry-packed integral lists (nirrep
+// 2, 4 or 8; pair types as intprc/clmoio.F writes them for RHF): each list goes
+// through C1 order into the same dense blocks as the C1 path.
+static int setup_engine_sym(AcesContext* c) {
+  const DpdSym& sy = c->lists->sym();
+  int occ[8], vir[8], o = 0, v = 0;
+  for (int h = 0; h < sy.nirrep; ++h) {
+    occ[h] = sy.pop[0][h];
+    vir[h] = sy.vrt[0][h];
+    ACES_REQUIRE(sy.pop[1][h] == occ[h] && sy.vrt[1][h] == vir[h], "the list-fed CCSD engine takes RHF lists");
+    o += occ[h];
+    v += vir[h];
+  }
+  ACES_REQUIRE(o >= 1 && v >= 1, "need occupied and virtual orbitals");
+  const int n = o + v;
+  const long ov = (long)o * v;
+  hipStream_t s = c->stream;
+  struct In { int list, tl, tr; const char* what; };
+  const In ins[6] = {{13, 14, 14, "<Mn|Ij>"}, {16, 13, 14, "<Ab|Ij>"}, {10, 14, 18, "<Ij|Ka>"},
+                     {30, 13, 11, "<Ab|Ci>"}, {23, 9, 9, "<AJ||BI>"}, {233, 5, 13, "<Ab|Cd>"}};
+  double* c1[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  struct Free {
+    double** p;
+    ~Free() {
+      for (int k = 0; k < 6; ++k)
+        if (p[k]) (void)hipFree(p[k]);
+    }
+  } release{c1};
+  for (int k = 0; k < 6; ++k) {
+    int tl = 0, tr = 0;
+    ACES_REQUIRE(c->lists->isytyp(ins[k].list, &tl, &tr) == ACES_OK && tl == ins[k].tl && tr == ins[k].tr,
+                 std::string("list ") + std::to_string(ins[k].list) + " (" + ins[k].what + ") missing or not of pair types (" +
+                     std::to_string(ins[k].tl) + "," + std::to_string(ins[k].tr) + ")");
+    const long n1 = c1_pairs(sy, tl) * c1_pairs(sy, tr);
+    ACES_HIP(hipMalloc(&c1[k], sizeof(double) * n1));
+    ACES_CHECK(dpd_to_c1(*c->lists, ins[k].list, tl, tr, 1, c1[k], s));
+  }
+  std::vector<double> f;
+  ACES_CHECK(fock_from_records(c, o, v, f));
+  DenseBlocks b;
+  ACES_HIP(hipMalloc(&b.f, sizeof(double) * n * n));
+  ACES_HIP(hipMemcpyAsync(b.f, f.data(), sizeof(double) * n * n, hipMemcpyHostToDevice, s));
+  ACES_CHECK(dense_blocks(o, v, c1[1], c1[0], c1[2], c1[3], c1[4], s, b));
+  // amplitude-space buffers; list 16 kept in C1 order for CMPENG
+  SymBuffers& B = c->sb;
+  const long n2 = ov * ov, naa = (long)o * (o - 1) / 2 * ((long)v * (v - 1) / 2);
+  const long total = 5 * n2 + 4 * ov + naa + o + v;
+  if (B.base) (void)hipFree(B.base);
+  B.base = nullptr;
+  ACES_HIP(hipMalloc(&B.base, sizeof(double) * total));
+  double* p = B.base;
+  for (double** q : {&B.t2, &B.t2n, &B.inc2, &B.x2, &B.l16}) {
+    *q = p;
+    p += n2;
+  }
+  for (double** q : {&B.t1, &B.t1n, &B.inc1, &B.x1}) {
+    *q = p;
+    p += ov;
+  }
+  B.aa = p;
+  p += naa;
+  B.eo = p;
+  p += o;
+  B.ev = p;
+  ACES_HIP(hipMemcpyAsync(B.l16, c1[1], sizeof(double) * n2, hipMemcpyDeviceToDevice, s));
+  {
+    std::vector<double> e(n);
+    for (int q = 0; q < n; ++q) e[q] = f[(size_t)q * n + q];
+    ACES_HIP(hipMemcpyAsync(B.eo, e.data(), sizeof(double) * o, hipMemcpyHostToDevice, s));
+    ACES_HIP(hipMemcpyAsync(B.ev, e.data() + o, sizeof(double) * v, hipMemcpyHostToDevice, s));
+  }
+  c->seng.reset(new SymEngine(sy.nirrep, occ, vir, s));
+  const int rc = c->seng->setup(b.f, b.ovov, b.oovv, b.ovoo, b.oooo, b.ovvv, c1[5]);
+  ACES_HIP(hipStreamSynchronize(s));
+  if (rc != ACES_OK) {
+    c->seng.reset();
+    return rc;
+  }
+  c->o = o;
+  c->v = v;
+  return ACES_OK;
+}
+
+static int setup_engine(AcesContext* c) {
+  return c->lists->sym().nirrep == 1 ? setup_engine_c1(c) : setup_engine_sym(c);
 }
 
 static int ensure_amp_lists(AcesContext* c) {
-  const int o = c->eng->o, v = c->eng->v;
-  const long nom = (long)o * (o - 1) / 2, nvm = (long)v * (v - 1) / 2;
-  const long vv = (long)v * v, oo = (long)o * o, ov = (long)o * v;
+  // T1 vector: the (a,i) pairs of irrep 1 (pair type 9), o v elements for C1
+  const long nt1 = c->lists->sym().irpdpd(0, 9);
   long r;
   // list 46 / 63: (13,14) AB; 44 / 61: (1,3) AA; 90 groups 1..4: T1 and increments
   if (c->lists->rows(1, 46, &r) != ACES_OK) ACES_CHECK(c->lists->inipck(1, 13, 14, 46));
@@ -282,8 +406,25 @@ static int ensure_amp_lists(AcesContext* c) {
   if (c->lists->rows(1, 44, &r) != ACES_OK) ACES_CHECK(c->lists->inipck(1, 1, 3, 44));
   if (c->lists->rows(1, 61, &r) != ACES_OK) ACES_CHECK(c->lists->inipck(1, 1, 3, 61));
   for (int g = 1; g <= 4; ++g)
-    if (c->lists->rows(g, 90, &r) != ACES_OK) ACES_CHECK(c->lists->updmoi(1, ov, g, 90));
-  (void)nom; (void)nvm; (void)vv; (void)oo;
+    if (c->lists->rows(g, 90, &r) != ACES_OK) ACES_CHECK(c->lists->updmoi(1, nt1, g, 90));
+  return ACES_OK;
+}
+
+// Irrep-blocked path: amplitude lists <-> the dense buffers.
+// T2 AB list (13,14) in C1 order is t2[j][i][b][a] = t2[i][j][a][b] (pair symmetry).
+static int sym_get_t(AcesContext* c, int g90, int l2, double* t1, double* t2) {
+  if (t1) ACES_CHECK(t1_to_dense(*c->lists, g90, 90, t1, c->stream));
+  if (t2) ACES_CHECK(dpd_to_c1(*c->lists, l2, 13, 14, 1, t2, c->stream));
+  return ACES_OK;
+}
+
+// T2 (dense AB) -> AB list l2 and its same-spin list laa = ABTOAA (l2 < 0: AA only)
+static int sym_put_t2(AcesContext* c, int l2, int laa, const double* t2) {
+  if (l2 > 0) ACES_CHECK(c1_to_dpd(*c->lists, l2, 13, 14, 1, t2, c->stream));
+  if (laa > 0) {
+    ACES_CHECK(abtoaa(c->o, c->v, t2, c->sb.aa, c->stream));
+    ACES_CHECK(c1_to_dpd(*c->lists, laa, 1, 3, 1, c->sb.aa, c->stream));
+  }
   return ACES_OK;
 }
 
@@ -372,6 +513,17 @@ int aces_sumsym2(const double* add, int group, int list, int on_device) {
   ACES_CHECK(ctx(&c));
   return c->lists->sumsym2(add, group, list, on_device != 0);
 }
+int aces_list_c1(int list, int typel, int typer, int irrepx, double* d_c1, int to_c1) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  ACES_REQUIRE(d_c1 != nullptr, "aces_list_c1: null buffer");
+  const int rc = to_c1 ? dpd_to_c1(*c->lists, list, typel, typer, irrepx, d_c1, c->stream)
+                       : c1_to_dpd(*c->lists, list, typel, typer, irrepx, d_c1, c->stream);
+  ACES_CHECK(rc);
+  ACES_HIP(hipStreamSynchronize(c->stream));
+  return ACES_OK;
+}
+
 double* aces_list_device_ptr(int group, int list) { return g_ctx ? g_ctx->lists->device_ptr(group, list) : nullptr; }
 
 // ---- record interface (C names) ----
@@ -391,7 +543,10 @@ int aces_vcc_set_precision(int prec) {
   ACES_CHECK(ctx(&c));
   ACES_REQUIRE(prec >= 0 && prec <= 3,
                "precision must be 0 (fp64), 1 (fp32), 2 (bf16x3) or 3 (bf16x3 refinement of an fp64 anchor)");
-  if (prec == kPrecRefine && c->prec != kPrecRefine && c->eng) c->eng->reset_anchor();  // anchor at the switch
+  if (prec == kPrecRefine && c->prec != kPrecRefine) {  // anchor at the switch
+    if (c->eng) c->eng->reset_anchor();
+    if (c->seng) c->seng->reset_anchor();
+  }
   c->prec = prec;
   return ACES_OK;
 }
@@ -442,22 +597,33 @@ void putrec_(i8* /*iflag*/, const char* /*archive*/, const char* name, i8* len, 
 }
 
 // ---- xvcc driver entry points ----
+// C1 lists run the C1 engine on the lists' own memory; symmetry-packed lists
+// run the irrep-blocked engine, the amplitude lists moving through C1 order.
 int aces_genint(void) {
   AcesContext* c;
   ACES_CHECK(ctx(&c));
-  if (!c->eng) ACES_CHECK(setup_engine(c));
+  if (!c->ready()) ACES_CHECK(setup_engine(c));
   ACES_CHECK(ensure_amp_lists(c));
+  if (c->seng) {
+    SymBuffers& B = c->sb;
+    ACES_CHECK(sym_get_t(c, 1, 46, B.t1, B.t2));
+    ACES_CHECK(c->seng->set_amplitudes(B.t1, B.t2));
+    return c->seng->increments((GemmPrec)c->prec, B.inc1, B.inc2);
+  }
   CcsdEngine& e = *c->eng;
-  const long ov = (long)e.o * e.v;
   ACES_CHECK(e.set_amplitudes(c->lists->device_ptr(1, 90), c->lists->device_ptr(1, 46)));
-  (void)ov;
   return e.increments((GemmPrec)c->prec, nullptr, nullptr);
 }
 
 int aces_dre3en(void) {
   AcesContext* c;
   ACES_CHECK(ctx(&c));
-  ACES_REQUIRE(c->eng != nullptr, "DRE3EN before GENINT");
+  ACES_REQUIRE(c->ready(), "DRE3EN before GENINT");
+  if (c->seng) {
+    ACES_CHECK(sym_put_t2(c, 63, 61, c->sb.inc2));
+    ACES_HIP(hipStreamSynchronize(c->stream));
+    return ACES_OK;
+  }
   CcsdEngine& e = *c->eng;
   const long ov = (long)e.o * e.v;
   ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 63), e.Inc() + ov, sizeof(double) * ov * ov,
@@ -470,7 +636,14 @@ int aces_dre3en(void) {
 int aces_e4s(void) {
   AcesContext* c;
   ACES_CHECK(ctx(&c));
-  ACES_REQUIRE(c->eng != nullptr, "E4S before GENINT");
+  ACES_REQUIRE(c->ready(), "E4S before GENINT");
+  if (c->seng) {
+    SymBuffers& B = c->sb;
+    ACES_CHECK(jacobi(c->o, c->v, B.eo, B.ev, B.inc1, B.inc2, B.t1n, B.t2n, c->stream));
+    ACES_CHECK(dense_to_t1(*c->lists, 3, 90, B.t1n, c->stream));
+    ACES_HIP(hipStreamSynchronize(c->stream));
+    return ACES_OK;
+  }
   CcsdEngine& e = *c->eng;
   ACES_CHECK(e.jacobi_step());
   const long ov = (long)e.o * e.v;
@@ -483,7 +656,16 @@ int aces_e4s(void) {
 int aces_newt2(void) {
   AcesContext* c;
   ACES_CHECK(ctx(&c));
-  ACES_REQUIRE(c->eng != nullptr, "NEWT2 before GENINT");
+  ACES_REQUIRE(c->ready(), "NEWT2 before GENINT");
+  if (c->seng) {
+    SymBuffers& B = c->sb;
+    // the increment on list 63 (possibly modified by the caller) divided by D
+    ACES_CHECK(dpd_to_c1(*c->lists, 63, 13, 14, 1, B.inc2, c->stream));
+    ACES_CHECK(jacobi(c->o, c->v, B.eo, B.ev, B.inc1, B.inc2, B.t1n, B.t2n, c->stream));
+    ACES_CHECK(sym_put_t2(c, 63, 61, B.t2n));
+    ACES_HIP(hipStreamSynchronize(c->stream));
+    return ACES_OK;
+  }
   CcsdEngine& e = *c->eng;
   const long ov = (long)e.o * e.v;
   // the increment on list 63 (possibly modified by the caller) divided by D
@@ -502,18 +684,27 @@ int aces_newt2(void) {
 //   ecorr3[0] = E(AA) (+ the non-HF singles term sum t1 f), ecorr3[2] = E(AB),
 //   ecorr3[1] untouched (the reference does not set it for RHF),
 //   *etot = 2 ecorr3[0] + ecorr3[2], *etott2 the same with T2 in place of tau.
-// The AA pieces are evaluated from the AB lists (list 44 = ABTOAA(46) for RHF).
+// The AA pieces are evaluated from the AB lists (list 44 = ABTOAA(46) for RHF);
+// symmetry-packed lists are read through C1 order.
 int aces_cmpeng(int nlist2, int nlist1, double* ecorr3, double* etot, double* etott2) {
   AcesContext* c;
   ACES_CHECK(ctx(&c));
-  ACES_REQUIRE(c->eng != nullptr, "CMPENG before GENINT");
+  ACES_REQUIRE(c->ready(), "CMPENG before GENINT");
   ACES_REQUIRE(nlist2 == 43 || nlist2 == 60, "CMPENG: NLIST2 must be 43 (lists 44/46) or 60 (lists 61/63)");
   ACES_REQUIRE(nlist1 == 0 || nlist1 == 2, "CMPENG: NLIST1 must be 0 (list 90 group 1) or 2 (group 3)");
-  CcsdEngine& e = *c->eng;
-  const int o = e.o, v = e.v;
-  const double* t1 = c->lists->device_ptr(nlist1 + 1, 90);
-  const double* t2 = c->lists->device_ptr(1, nlist2 + 3);
-  const double* w = c->lists->device_ptr(1, 16);
+  const int o = c->o, v = c->v;
+  const double *t1, *t2, *w;
+  if (c->seng) {
+    SymBuffers& B = c->sb;
+    ACES_CHECK(sym_get_t(c, nlist1 + 1, nlist2 + 3, B.x1, B.x2));
+    t1 = B.x1;
+    t2 = B.x2;
+    w = B.l16;
+  } else {
+    t1 = c->lists->device_ptr(nlist1 + 1, 90);
+    t2 = c->lists->device_ptr(1, nlist2 + 3);
+    w = c->lists->device_ptr(1, 16);
+  }
   ACES_REQUIRE(t1 && t2 && w, "CMPENG: lists 90, 16 and the T2 list must exist");
   hipStream_t s = c->stream;
   double* out = c->epart + 4L * o * o;
@@ -538,20 +729,46 @@ int aces_cmpeng(int nlist2, int nlist1, double* ecorr3, double* etot, double* et
 int aces_dodiis0(int icycle, double tol, int* iconvg, double* stats) {
   AcesContext* c;
   ACES_CHECK(ctx(&c));
-  ACES_REQUIRE(c->eng != nullptr, "DODIIS0 before GENINT");
-  CcsdEngine& e = *c->eng;
-  const long ov = (long)e.o * e.v;
+  ACES_REQUIRE(c->ready(), "DODIIS0 before GENINT");
   hipStream_t s = c->stream;
-  ACES_HIP(hipMemcpyAsync(e.T(), c->lists->device_ptr(1, 90), sizeof(double) * ov, hipMemcpyDeviceToDevice, s));
-  ACES_HIP(hipMemcpyAsync(e.T() + ov, c->lists->device_ptr(1, 46), sizeof(double) * ov * ov,
-                          hipMemcpyDeviceToDevice, s));
-  ACES_HIP(hipMemcpyAsync(e.Tnew(), c->lists->device_ptr(3, 90), sizeof(double) * ov, hipMemcpyDeviceToDevice,
-                          s));
-  ACES_HIP(hipMemcpyAsync(e.Tnew() + ov, c->lists->device_ptr(1, 63), sizeof(double) * ov * ov,
-                          hipMemcpyDeviceToDevice, s));
   CcsdStats st;
-  ACES_CHECK(e.energy_dev(e.Tnew(), e.Tnew() + ov, nullptr));
-  ACES_CHECK(e.diis_step(icycle, tol, st));
+  if (c->seng) {
+    SymBuffers& B = c->sb;
+    ACES_CHECK(sym_get_t(c, 1, 46, B.t1, B.t2));
+    ACES_CHECK(sym_get_t(c, 3, 63, B.t1n, B.t2n));
+    ACES_CHECK(c->seng->diis_dense(icycle, tol, B.t1, B.t2, B.t1n, B.t2n, st));
+    ACES_CHECK(c->seng->get_amplitudes(B.x1, B.x2));
+    if (!st.converged) {
+      // extrapolated (or Jacobi) vector -> increment lists
+      ACES_CHECK(dense_to_t1(*c->lists, 3, 90, B.x1, s));
+      ACES_CHECK(sym_put_t2(c, 63, 61, B.x2));
+    } else {
+      ACES_CHECK(sym_put_t2(c, 46, -1, B.x2));
+    }
+  } else {
+    CcsdEngine& e = *c->eng;
+    const long ov = (long)e.o * e.v;
+    ACES_HIP(hipMemcpyAsync(e.T(), c->lists->device_ptr(1, 90), sizeof(double) * ov, hipMemcpyDeviceToDevice, s));
+    ACES_HIP(hipMemcpyAsync(e.T() + ov, c->lists->device_ptr(1, 46), sizeof(double) * ov * ov,
+                            hipMemcpyDeviceToDevice, s));
+    ACES_HIP(hipMemcpyAsync(e.Tnew(), c->lists->device_ptr(3, 90), sizeof(double) * ov, hipMemcpyDeviceToDevice,
+                            s));
+    ACES_HIP(hipMemcpyAsync(e.Tnew() + ov, c->lists->device_ptr(1, 63), sizeof(double) * ov * ov,
+                            hipMemcpyDeviceToDevice, s));
+    ACES_CHECK(e.energy_dev(e.Tnew(), e.Tnew() + ov, nullptr));
+    ACES_CHECK(e.diis_step(icycle, tol, st));
+    if (!st.converged) {
+      // extrapolated (or Jacobi) vector -> increment lists (vcc/dodiis0.F "WRITE IT OVER CURRENT INCREMENT LISTS")
+      ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(3, 90), e.T(), sizeof(double) * ov, hipMemcpyDeviceToDevice, s));
+      ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 63), e.T() + ov, sizeof(double) * ov * ov,
+                              hipMemcpyDeviceToDevice, s));
+      ACES_CHECK(abtoaa(e.o, e.v, c->lists->device_ptr(1, 63), c->lists->device_ptr(1, 61), s));
+    } else {
+      // DODIIS0 symmetrised list 46 in place (libr2/symetrize_t2.F)
+      ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 46), e.T() + ov, sizeof(double) * ov * ov,
+                              hipMemcpyDeviceToDevice, s));
+    }
+  }
   *iconvg = st.converged ? 0 : 1;
   if (stats) {
     stats[0] = st.energy_new;
@@ -559,17 +776,6 @@ int aces_dodiis0(int icycle, double tol, int* iconvg, double* stats) {
     stats[2] = st.resmdiis;
     stats[3] = st.converged;
     stats[4] = st.energy_final;
-  }
-  if (!st.converged) {
-    // extrapolated (or Jacobi) vector -> increment lists (vcc/dodiis0.F "WRITE IT OVER CURRENT INCREMENT LISTS")
-    ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(3, 90), e.T(), sizeof(double) * ov, hipMemcpyDeviceToDevice, s));
-    ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 63), e.T() + ov, sizeof(double) * ov * ov,
-                            hipMemcpyDeviceToDevice, s));
-    ACES_CHECK(abtoaa(e.o, e.v, c->lists->device_ptr(1, 63), c->lists->device_ptr(1, 61), s));
-  } else {
-    // DODIIS0 symmetrised list 46 in place (libr2/symetrize_t2.F)
-    ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 46), e.T() + ov, sizeof(double) * ov * ov,
-                            hipMemcpyDeviceToDevice, s));
   }
   ACES_HIP(hipStreamSynchronize(s));
   return ACES_OK;
@@ -600,8 +806,17 @@ int aces_drmove(void) { return aces_drmove_lists(0, 1); }
 int aces_vcc_mp1(void) {
   AcesContext* c;
   ACES_CHECK(ctx(&c));
-  if (!c->eng) ACES_CHECK(setup_engine(c));
+  if (!c->ready()) ACES_CHECK(setup_engine(c));
   ACES_CHECK(ensure_amp_lists(c));
+  if (c->seng) {
+    SymBuffers& B = c->sb;
+    ACES_CHECK(c->seng->start_mp1());
+    ACES_CHECK(c->seng->get_amplitudes(B.t1, B.t2));
+    ACES_CHECK(dense_to_t1(*c->lists, 1, 90, B.t1, c->stream));
+    ACES_CHECK(sym_put_t2(c, 46, 44, B.t2));
+    ACES_HIP(hipStreamSynchronize(c->stream));
+    return ACES_OK;
+  }
   CcsdEngine& e = *c->eng;
   const long ov = (long)e.o * e.v;
   ACES_CHECK(e.start_mp1());
@@ -617,7 +832,11 @@ int aces_vcc_mp1(void) {
 int aces_vcc_set_diis_order(int order) {
   AcesContext* c;
   ACES_CHECK(ctx(&c));
-  ACES_REQUIRE(c->eng != nullptr, "DIISLST before GENINT/MP1");
+  ACES_REQUIRE(c->ready(), "DIISLST before GENINT/MP1");
+  if (c->seng) {
+    ACES_CHECK(c->seng->set_diis_order(order));
+    return c->seng->reset_diis();
+  }
   ACES_CHECK(c->eng->set_diis_order(order));
   return c->eng->reset_diis();
 }

@@ -251,6 +251,14 @@ int aces_getall(double* dest, long length, int irrepx, int list, int on_device);
 int aces_putall(const double* src, long length, int irrepx, int list, int on_device);
 int aces_sumsym2(const double* add, int group, int list, int on_device);
 double* aces_list_device_ptr(int group, int list);
+/* A totally symmetric (irrepx 1) or other list of pair types (typel, typer) in
+ * C1 order: to_c1 = 1 fills d_c1 (device, c1 rows x c1 columns, column major,
+ * symmetry-forbidden elements zero) from the list's irrep groups; to_c1 = 0
+ * writes the groups from d_c1.  The C1 order is the list's nirrep = 1 layout
+ * over the irrep-sorted orbitals.  No reference counterpart (the reference
+ * reads its lists by irrep, acescore/aces_com_symloc.F); the bridge the
+ * irrep-blocked driver path uses, exposed for hosts that hold C1-ordered data. */
+int aces_list_c1(int list, int typel, int typer, int irrepx, double* d_c1, int to_c1);
 int aces_putrec(const char* name, long nwords, const void* src);
 int aces_getrec(int iflag, const char* name, long* nwords, void* dest);
 
@@ -271,9 +279,14 @@ int aces_getrec(int iflag, const char* name, long* nwords, void* dest);
  *   aces_drmove            DRMOVE(..., 0, .TRUE.) (libr/drmove.f): T lists <- increment lists
  *   aces_drmove_lists      DRMOVE(..., nlist, sing1): lists 61/63 -> 44+nlist/46+nlist,
  *                          with sing1 also list 90 group 3 -> list 90+nlist group 1
- * Integral input lists (RHF, C1): 13 <Mn|Ij> (14,14), 16 <Ab|Ij> (13,14),
+ * Integral input lists (RHF): 13 <Mn|Ij> (14,14), 16 <Ab|Ij> (13,14),
  * 10 <Ij|Ka> (14,18), 30 <Ab|Ci> (13,11), 233 <Ab|Cd> (5,13) and
- * 23 <AJ||BI> (9,9); records SCFEVALA (orbital energies) or FOCKMOA (MO Fock).
+ * 23 <AJ||BI> (9,9); records SCFEVALA (orbital energies) or FOCKMOA (MO Fock),
+ * orbitals in list order (occupied then virtual, each irrep-sorted).  With one
+ * irrep the C1 engine runs on the lists' own memory; with 2, 4 or 8 irreps the
+ * lists are symmetry packed (one group per irrep, DPD order) and the
+ * irrep-blocked engine runs them (amplitude lists 44/46/61/63/90 in the same
+ * packing, T1 = the irrep-1 (a,i) vector of pair type 9).
  * Fortran spellings with the reference's argument lists: genint_, feacont_,
  * fmicont_, fmecont_, t1int2a_, t1int2b_, t1int1_, initin_, initsn_, rnabij_,
  * dre3en_, e4s_, newt2_, cmpeng_, dodiis0_, drmove_.  Argument values they do

@@ -25,6 +25,24 @@ def write_input(path, blk, o, v):
         np.diag(blk['fock']).astype(np.float64).tofile(f)
 
 
+def write_input_sym(path, blk, occ, vir):
+    """Symmetry-packed lists (one group per irrep) for the Fortran host; blk in
+    irrep-sorted orbital order with symmetry-forbidden elements zero."""
+    from aces2 import lists
+    L = lists.sym_integral_lists(blk, occ, vir)
+    n = len(occ)
+    with open(path, 'wb') as f:
+        np.array([-n] + list(occ) + list(vir) + [len(L)], dtype=np.int64).tofile(f)
+        for lst, (tl, tr, groups) in L.items():
+            np.array([lst, tl, tr, len(groups)], dtype=np.int64).tofile(f)
+            for g, data in enumerate(groups, start=1):
+                nc, nr = data.shape
+                np.array([g, nr, nc], dtype=np.int64).tofile(f)
+                if data.size:
+                    np.ascontiguousarray(data, dtype=np.float64).ravel().tofile(f)
+        np.diag(blk['fock']).astype(np.float64).tofile(f)
+
+
 @pytest.mark.gpu
 def test_fortran_host_matches_oracle(tmp_path):
     from aces2 import synth
@@ -102,3 +120,30 @@ def test_fortran_host_rejects_xlambda_diis_lists(tmp_path):
     out = subprocess.run([EXE, inp, '9', '60', '6', 'xlambda_diis'], capture_output=True, text=True, timeout=120)
     assert out.returncode == 1, out.stdout + out.stderr
     assert '@DODIIS0' in out.stderr and 'only the closed-shell amplitude lists' in out.stderr
+
+
+@pytest.mark.gpu
+def test_fortran_host_symmetry_packed_lists(tmp_path):
+    """The host over symmetry-packed lists (D2h irrep counts, synthetic symmetric
+    integrals): the irrep-blocked engine behind the same entry points gives the
+    oracle's loop."""
+    from aces2 import synth
+    print('case fortran xvcc symmetry packed', flush=True)
+    occ, vir = [1, 1, 0, 1, 1, 1, 0, 1], [2, 3, 2, 2, 3, 2, 2, 3]
+    eps, B, irr = synth.sym_draw(occ, vir, 44)
+    o = sum(occ)
+    eri = synth.full_eri(B)
+    blk = dict(rccsd.blocks(eri, o))
+    blk['fock'] = np.diag(eps)
+    blk['list233'] = synth.list233_from_eri(eri, o)
+    ref = vcc_loop.run(blk['fock'], rccsd.blocks(eri, o), o, maxcyc=60, conv=9, order=6)
+    inp = tmp_path / 'lists_sym.bin'
+    write_input_sym(str(inp), blk, occ, vir)
+    out = subprocess.run([EXE, str(inp), '9', '60', '6'], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    eng = [float(x) for x in re.findall(r'The total correlation energy is\s+(\S+) a\.u\.', out.stdout)]
+    res = [float(x.rstrip('.')) for x in re.findall(r'Largest element of residual vector :\s+(\S+)', out.stdout)]
+    assert len(res) == ref['cycles']
+    np.testing.assert_allclose(eng[:-1], ref['energy'], rtol=0, atol=2e-12 + 1e-11)
+    np.testing.assert_allclose(res, ref['resmiter'], rtol=1e-6, atol=1e-13)
+    assert abs(eng[-1] - ref['final_energy']) < 1e-10

@@ -172,6 +172,33 @@ def test_fortran_host_reproduces_reference_output(h2o, tmp_path):
     assert 'Amplitude equations converged in    10 iterations.' in out.stdout
 
 
+@pytest.mark.gpu
+def test_fortran_host_symmetry_packed_c2v_lists(h2o, tmp_path):
+    """The host over the C2v symmetry-packed lists (4 irreps, orbitals irrep-sorted):
+    the irrep-blocked engine behind the list entry points prints the reference's lines."""
+    import re
+    import subprocess
+    from test_fortran_gpu import EXE, write_input_sym
+    print('case golden H2O fortran C2v lists', flush=True)
+    enuc, escf, F, eri, labels = h2o
+    perm, occ, vir = molint.irrep_order(labels, O)
+    assert len(occ) == 4
+    p = np.array(perm)
+    inp = tmp_path / 'h2o_c2v.bin'
+    write_input_sym(str(inp), _blocks(F[np.ix_(p, p)], eri[np.ix_(p, p, p, p)]), occ, vir)
+    out = subprocess.run([EXE, str(inp), str(FIX['cc_conv']), '50', str(FIX['cc_exporder'])], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    eng = [float(x) for x in re.findall(r'The total correlation energy is\s+(\S+) a\.u\.', out.stdout)]
+    res = [float(x.rstrip('.')) for x in re.findall(r'Largest element of residual vector :\s+(\S+)', out.stdout)]
+    _check_trace(eng[:-1], res, eng[-1])
+    aa = [float(x) for x in re.findall(r'The AA contribution to the correlation energy is:\s+(\S+) a\.u\.', out.stdout)]
+    ab = [float(x) for x in re.findall(r'The AB contribution to the correlation energy is:\s+(\S+) a\.u\.', out.stdout)]
+    np.testing.assert_allclose(aa, FIX['cc']['aa_contributions'], rtol=0, atol=1.001e-7)
+    np.testing.assert_allclose(ab, FIX['cc']['ab_contributions'], rtol=0, atol=1.001e-7)
+    assert 'Amplitude equations converged in    10 iterations.' in out.stdout
+
+
 def test_oracle_triples_match_reference(h2o, cc_run):
     """E4T, E4TAAA, E4TBAA and E5ST of out.zmat.014a's TRPS at the converged amplitudes."""
     from oracle import triples

@@ -157,6 +157,43 @@ def test_irrep_blocked_engine_reproduces_reference_trace_d2h():
     np.testing.assert_allclose(lh['resmdiis'], g['lambda_resmdiis'], rtol=LAMBDA_RTOL, atol=0)
 
 
+@pytest.mark.gpu
+def test_fortran_host_symmetry_packed_d2h_lists(tmp_path):
+    """vcc.F's call sequence (flang host) over the D2h symmetry-packed lists of the
+    fixture: the irrep-blocked engine behind the list entry points prints the
+    reference's CC trace, its AA/AB lines included."""
+    import re
+    import subprocess
+    from aces2 import synth
+    from test_fortran_gpu import EXE, write_input_sym
+    print('case golden O2 fortran D2h lists', flush=True)
+    g = FIX['first_geometry']
+    enuc, escf, F, eri, eps, labels = _system(g['z_bohr'], labels=True)
+    o = FIX['nocc']
+    perm, occ, vir = molint.irrep_order(labels, o)
+    assert len(occ) == 8
+    p = np.array(perm)
+    erip = eri[np.ix_(p, p, p, p)]
+    blk = dict(rccsd.blocks(erip, o))
+    blk['fock'] = F[np.ix_(p, p)]
+    blk['list233'] = synth.list233_from_eri(erip, o)
+    inp = tmp_path / 'o2_d2h.bin'
+    write_input_sym(str(inp), blk, occ, vir)
+    out = subprocess.run([EXE, str(inp), str(g['cc_conv']), '50', str(g['cc_exporder'])], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    eng = [float(x) for x in re.findall(r'The total correlation energy is\s+(\S+) a\.u\.', out.stdout)]
+    res = [float(x.rstrip('.')) for x in re.findall(r'Largest element of residual vector :\s+(\S+)', out.stdout)]
+    np.testing.assert_allclose(eng[:-1], g['mp2_and_cycle_energies'], rtol=0, atol=3e-11)
+    np.testing.assert_allclose(res, g['resmiter'], rtol=3e-7, atol=0)
+    assert abs(eng[-1] - g['final_energy']) < 1e-11
+    assert 'Amplitude equations converged in    12 iterations.' in out.stdout
+    aa = [float(x) for x in re.findall(r'The AA contribution to the correlation energy is:\s+(\S+) a\.u\.', out.stdout)]
+    ab = [float(x) for x in re.findall(r'The AB contribution to the correlation energy is:\s+(\S+) a\.u\.', out.stdout)]
+    np.testing.assert_allclose(aa, g['aa_contributions'], rtol=0, atol=1.001e-7)
+    np.testing.assert_allclose(ab, g['ab_contributions'], rtol=0, atol=1.001e-7)
+
+
 def _lambda_inputs(first_geometry):
     g, enuc, escf, F, eri = first_geometry
     o = FIX['nocc']

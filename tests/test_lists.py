@@ -249,3 +249,127 @@ def test_drmove_fills_lambda_start_lists():
     np.testing.assert_array_equal(lists.getlst(1, 1, 1, 90), before[(90, 1)])
     np.testing.assert_array_equal(lists.getlst(1, o * o, 1, 46), before[(63, 1)])
     lists.fin()
+
+
+def test_dpd_writer_sizes_and_subblocks_match_library():
+    """The host writer's pair enumeration (aces2.lists.dpd_pairs) against the
+    library's IRPDPD / ISYMOFF: group sizes, and each sub-block (irrep of q)
+    starting where ISYMOFF says, p fastest inside it."""
+    from aces2 import lists
+    lists.init()
+    for pop, vrt in [([2, 1, 0, 1], [3, 2, 4, 1]), ([1, 1, 1, 1, 0, 2, 1, 1], [2, 3, 1, 2, 2, 1, 3, 2]), ([2, 3], [4, 1])]:
+        lists.set_symmetry(pop, vrt)
+        irr_o = np.repeat(np.arange(len(pop)), pop)
+        irr_v = np.repeat(np.arange(len(vrt)), vrt)
+        for typ in range(1, 26):
+            sp, sq, pack = lists.PAIR_TYPES[typ]
+            irr_p = irr_o if sp == 'O' else irr_v
+            irr_q = irr_o if sq == 'O' else irr_v
+            for g in range(len(pop)):
+                p, q = lists.dpd_pairs(pop, vrt, typ, g)
+                assert len(p) == lists.irpdpd(g + 1, typ), (typ, g)
+                if len(p) == 0:
+                    continue
+                assert np.all(irr_p[p] ^ irr_q[q] == g)
+                if pack == 1:
+                    assert np.all(p < q)
+                elif pack == 2:
+                    assert np.all(p <= q)
+                for h in sorted(set(irr_q[q].tolist())):
+                    first = int(np.argmax(irr_q[q] == h))
+                    assert first + 1 == lists.isymoff(h + 1, g + 1, typ), (typ, g, h)
+                # inside a sub-block: column (q) major, p fastest
+                key = q * 10 ** 6 + p
+                assert np.all(np.diff(key) > 0)
+    lists.fin()
+
+
+def _sym_system(occ, vir, seed, nonhf=False):
+    from aces2 import synth
+    eps, B, irr = synth.sym_draw(occ, vir, seed)
+    o = sum(occ)
+    eri = synth.full_eri(B)
+    f = np.diag(eps)
+    if nonhf:
+        rng = np.random.default_rng(seed + 7)
+        P = rng.normal(0, 0.01, f.shape)
+        f = f + (P + P.T) * ((irr[:, None] ^ irr[None, :]) == 0)
+    blk = dict(rccsd.blocks(eri, o))
+    blk['fock'] = f
+    blk['list233'] = synth.list233_from_eri(eri, o)
+    return blk, eri, irr
+
+
+@pytest.mark.gpu
+def test_symmetry_packed_lists_through_c1_order():
+    """Device DPD <-> C1 conversion (aces_list_c1) against the host writer:
+    every integral list of a D2h system back in C1 order equals the C1 list with
+    the symmetry-forbidden elements zero, and C1 -> groups reproduces the groups."""
+    import torch
+    from aces2 import _lib, lists
+    print('case dpd lists c1 round trip', flush=True)
+    occ, vir = [1, 2, 0, 1, 1, 1, 1, 1], [2, 3, 2, 1, 3, 2, 2, 3]
+    blk, eri, irr = _sym_system(occ, vir, 7)
+    o, v = sum(occ), sum(vir)
+    lists.init()
+    lists.load_sym_system(blk, occ, vir)
+    for lst, (tl, tr, data) in lists.c1_integral_lists(blk, o).items():
+        rows, cols = lists.c1_pairs(tl, o, v), lists.c1_pairs(tr, o, v)
+        got = lists.list_to_c1(lst, tl, tr, rows, cols)
+        np.testing.assert_array_equal(got.ravel(), data.ravel())   # forbidden elements are zero in blk
+        # C1 -> groups: write a scaled copy back and read the groups
+        dev = torch.as_tensor(2.0 * data.ravel(), device='cuda')
+        _lib.call('aces_list_c1', lst, tl, tr, 1, dev.data_ptr(), 0)
+        want = lists.dpd_groups_from_c1(2.0 * data.ravel(), occ, vir, tl, tr)
+        for g, blkg in enumerate(want, start=1):
+            nr, nc = lists.shape(g, lst)
+            if nr * nc:
+                np.testing.assert_array_equal(lists.getlst(1, nc, g, lst), blkg)
+    lists.fin()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('nonhf', [False, True])
+def test_xvcc_driver_over_symmetry_packed_lists(nonhf):
+    """vcc.F's call sequence over symmetry-packed lists (D2h counts): the
+    irrep-blocked engine behind the list entry points reproduces the oracle
+    loop, CMPENG's spin cases included."""
+    import ctypes
+    from aces2 import _lib, lists
+    print('case xvcc symmetry-packed lists nonhf=%s' % nonhf, flush=True)
+    occ, vir = [1, 1, 0, 1, 1, 1, 0, 1], [2, 3, 2, 2, 3, 2, 2, 3]
+    blk, eri, irr = _sym_system(occ, vir, 21, nonhf)
+    o = sum(occ)
+    g = rccsd.blocks(eri, o)
+    ref = vcc_loop.run(blk['fock'], g, o, maxcyc=60, conv=9, order=6)
+    lists.init()
+    lists.load_sym_system(blk, occ, vir, nonhf=nonhf)
+    _lib.call('aces_vcc_mp1')
+    _lib.call('aces_vcc_set_diis_order', 6)
+    e0 = ctypes.c_double()
+    _lib.call('aces_cmpeng', 43, 0, None, ctypes.byref(e0), None)
+    energies, resm, spin = [e0.value], [], []
+    conv = ctypes.c_int(1)
+    st = (ctypes.c_double * 8)()
+    for icycle in range(1, 61):
+        for name in ('aces_genint', 'aces_dre3en', 'aces_e4s', 'aces_newt2'):
+            _lib.call(name)
+        e = ctypes.c_double()
+        ec = (ctypes.c_double * 3)()
+        _lib.call('aces_cmpeng', 60, 2, ec, ctypes.byref(e), None)
+        energies.append(e.value)
+        spin.append((ec[0], ec[2]))
+        _lib.call('aces_dodiis0', icycle, 1e-9, ctypes.byref(conv), st)
+        resm.append(st[1])
+        if conv.value == 0:
+            break
+        _lib.call('aces_drmove')
+    ef = ctypes.c_double()
+    _lib.call('aces_cmpeng', 43, 0, None, ctypes.byref(ef), None)
+    lists.fin()
+    assert conv.value == 0 and len(resm) == ref['cycles']
+    np.testing.assert_allclose(energies, ref['energy'], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(resm, ref['resmiter'], rtol=1e-6, atol=1e-13)
+    assert abs(ef.value - ref['final_energy']) < 1e-10
+    # the spin cases of the last cycle's T_new against the oracle's split
+    assert all(abs(2 * aa + ab - e) < 1e-12 for (aa, ab), e in zip(spin, energies[1:]))
