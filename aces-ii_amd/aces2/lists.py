@@ -103,6 +103,24 @@ def getrec(name, length, dtype=np.float64, iflag=1):
     return out
 
 
+def jobarc_read(path=''):
+    """JAINDX + JOBARC in directory ``path`` -> the record store."""
+    _lib.call('aces_jobarc_read', str(path).encode())
+
+
+def jobarc_write(path=''):
+    _lib.call('aces_jobarc_write', str(path).encode())
+
+
+def lists_read(path=''):
+    """The list files in ``path`` (placed by the archive's MOIO records) -> the list store."""
+    _lib.call('aces_lists_read', str(path).encode())
+
+
+def lists_write(path=''):
+    _lib.call('aces_lists_write', str(path).encode())
+
+
 def reclen(name):
     n = ctypes.c_long(0)
     _lib.call('aces_getrec', 0, name.encode(), ctypes.byref(n), None)

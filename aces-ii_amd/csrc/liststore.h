@@ -44,6 +44,11 @@ class ListStore {
   // left irrep r xor irrepx), records ISYTYP(list) = (typel, typer).
   int inipck(int irrepx, int typel, int typer, int list);
   int isytyp(int list, int* tl, int* tr) const;
+  // Record ISYTYP(list) without creating groups (lists read from MOINTS files).
+  void set_types(int list, int tl, int tr) {
+    lists_[list].tl = tl;
+    lists_[list].tr = tr;
+  }
 
   int rows(int group, int list, long* n) const;
   int cols(int group, int list, long* n) const;
@@ -83,18 +88,33 @@ class ListStore {
   long bytes_ = 0;
 };
 
+// The job archive (JOBARC records) as ACES II keeps it: logical records in
+// slot order, each a name of at most 8 characters and a length in 8-byte words.
 class RecordStore {
  public:
   // GETREC semantics (acescore/getrec.F:5-20): iflag > 0 record must exist;
   // iflag == 0 returns the length (-1 if absent) in *len; iflag < 0 zero fills
-  // a missing record.  Lengths are in 8-byte words.
+  // a missing record.  A shorter read than the stored length is allowed.
   int getrec(int iflag, const std::string& name, long* len, void* dest);
+  // PUTREC semantics (acescore/putrec.F): a new name takes the next slot; an
+  // existing record keeps its slot and stored length when the new length is
+  // not larger (the first len words are replaced), and moves to a new slot
+  // when it is (the old slot stays as the lost record '. LOST .').
   int putrec(const std::string& name, long len, const void* src);
-  bool has(const std::string& name) const { return recs_.count(key(name)) != 0; }
+  bool has(const std::string& name) const { return index_.count(key(name)) != 0; }
   static std::string key(const std::string& name);
 
+  struct Slot {
+    std::string name;  // key form (no trailing blanks); ". LOST ." for a lost record
+    std::vector<long> data;
+  };
+  const std::vector<Slot>& slots() const { return slots_; }
+  // Replace the whole archive (a JOBARC read).
+  void assign(std::vector<Slot> slots);
+
  private:
-  std::map<std::string, std::vector<long>> recs_;
+  std::vector<Slot> slots_;
+  std::map<std::string, size_t> index_;
 };
 
 }  // namespace aces

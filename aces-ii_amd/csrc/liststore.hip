@@ -312,12 +312,12 @@ std::string RecordStore::key(const std::string& name) {
 }
 
 int RecordStore::getrec(int iflag, const std::string& name, long* len, void* dest) {
-  auto it = recs_.find(key(name));
+  auto it = index_.find(key(name));
   if (iflag == 0) {
-    *len = (it == recs_.end()) ? -1 : (long)it->second.size();
+    *len = (it == index_.end()) ? -1 : (long)slots_[it->second].data.size();
     return ACES_OK;
   }
-  if (it == recs_.end()) {
+  if (it == index_.end()) {
     if (iflag > 0) {
       set_error("GETREC: record " + key(name) + " not found");
       return ACES_ERR_RANGE;
@@ -325,20 +325,40 @@ int RecordStore::getrec(int iflag, const std::string& name, long* len, void* des
     if (*len > 0) std::memset(dest, 0, sizeof(long) * (*len));
     return ACES_OK;
   }
-  if (*len > (long)it->second.size()) {
+  const std::vector<long>& d = slots_[it->second].data;
+  if (*len > (long)d.size()) {
     set_error("GETREC: record " + key(name) + " is shorter than requested");
     return ACES_ERR_RANGE;
   }
-  std::memcpy(dest, it->second.data(), sizeof(long) * (*len));
+  if (*len > 0) std::memcpy(dest, d.data(), sizeof(long) * (*len));
   return ACES_OK;
 }
 
 int RecordStore::putrec(const std::string& name, long len, const void* src) {
   ACES_REQUIRE(len >= 0, "PUTREC: negative length");
-  std::vector<long>& r = recs_[key(name)];
-  r.assign((size_t)len, 0);
-  if (len) std::memcpy(r.data(), src, sizeof(long) * len);
+  const std::string k = key(name);
+  ACES_REQUIRE(!k.empty() && k != "OPENSLOT" && k != ". LOST .", "PUTREC: invalid record name '" + k + "'");
+  auto it = index_.find(k);
+  if (it != index_.end() && len > (long)slots_[it->second].data.size()) {
+    slots_[it->second].name = ". LOST .";
+    index_.erase(it);
+    it = index_.end();
+  }
+  if (it == index_.end()) {
+    ACES_REQUIRE(slots_.size() < 1000, "PUTREC: maximum number of JOBARC records exceeded");
+    slots_.push_back(Slot{k, std::vector<long>((size_t)len, 0)});
+    it = index_.emplace(k, slots_.size() - 1).first;
+  }
+  std::vector<long>& d = slots_[it->second].data;
+  if (len) std::memcpy(d.data(), src, sizeof(long) * len);
   return ACES_OK;
+}
+
+void RecordStore::assign(std::vector<Slot> slots) {
+  slots_ = std::move(slots);
+  index_.clear();
+  for (size_t i = 0; i < slots_.size(); ++i)
+    if (slots_[i].name != ". LOST ." && !index_.count(slots_[i].name)) index_.emplace(slots_[i].name, i);
 }
 
 }  // namespace aces

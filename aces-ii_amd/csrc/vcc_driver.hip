@@ -35,6 +35,7 @@
 #include <memory>
 #include <string>
 #include <vector>
+#include "aces_files.h"
 #include "ccsd_engine.h"
 #include "ccsd_kernels.h"
 #include "common.h"
@@ -527,6 +528,34 @@ int aces_list_c1(int list, int typel, int typer, int irrepx, double* d_c1, int t
 double* aces_list_device_ptr(int group, int list) { return g_ctx ? g_ctx->lists->device_ptr(group, list) : nullptr; }
 
 // ---- record interface (C names) ----
+// ---- the job archive and list files on disk (ACES_JA_INIT / ACES_JA_FIN,
+// ACES_IO_INIT / ACES_IO_FIN: acescore/aces_ja_init.F, aces_ja_fin.F,
+// aces_io_init.F, aces_io_fin.F) ----
+int aces_jobarc_read(const char* dir) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  return jobarc_read(c->recs, dir ? dir : "");
+}
+
+int aces_jobarc_write(const char* dir) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  return jobarc_write(c->recs, dir ? dir : "");
+}
+
+int aces_lists_read(const char* dir) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  ACES_REQUIRE(!c->ready(), "aces_lists_read: lists are read before the engine is built (GENINT)");
+  return lists_read(*c->lists, c->recs, dir ? dir : "");
+}
+
+int aces_lists_write(const char* dir) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  return lists_write(*c->lists, c->recs, dir ? dir : "");
+}
+
 int aces_putrec(const char* name, long nwords, const void* src) {
   AcesContext* c;
   ACES_CHECK(ctx(&c));

@@ -60,6 +60,22 @@ module aces2_amd
       import :: c_int
       integer(c_int), value :: prec
     end function
+    integer(c_int) function aces_jobarc_read(dir) bind(C, name='aces_jobarc_read')
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: dir(*)
+    end function
+    integer(c_int) function aces_jobarc_write(dir) bind(C, name='aces_jobarc_write')
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: dir(*)
+    end function
+    integer(c_int) function aces_lists_read(dir) bind(C, name='aces_lists_read')
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: dir(*)
+    end function
+    integer(c_int) function aces_lists_write(dir) bind(C, name='aces_lists_write')
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: dir(*)
+    end function
     type(c_ptr) function aces_last_error() bind(C, name='aces_last_error')
       import :: c_ptr
     end function

@@ -3,6 +3,13 @@
 ! reference so INTEGER arguments are 8 bytes.
 !
 ! Usage: xvcc_amd <input> [cc_conv] [cc_maxcyc] [cc_exporder] [mode]
+! <input> a file in the stream format below, or "jobdir:<dir>" for a job
+!         directory holding JAINDX, JOBARC and the list files (MOINTS, MOABCD,
+!         ...) as the reference's xintprc leaves them: the archive and lists are
+!         read (ACES_JA_INIT / ACES_IO_INIT), SCFEVALA comes from the archive, and
+!         on convergence TOTENERG = SCFENEG + the CC correlation energy is
+!         stored (vcc/finish.F:140) and the archive and lists written back
+!         (ACES_IO_FIN / ACES_JA_FIN).
 ! mode 'lambda': after convergence move the amplitudes into the Lambda start lists
 !               with DRMOVE(...,100,SING1) (vcc/vcc.F:1138) and print the sums of
 !               squares of lists 44, 46, 90 and 144, 146, 190;
@@ -21,9 +28,9 @@ program xvcc_amd
   integer :: o, v, nl, il, lst, tl, tr, nr, nc, n, nirrep, ng, ig, g, h
   integer :: popi(8), vrti(8)
   logical :: sing1, do_hbar
-  double precision :: ecorr(3), etot, etott2, edummy, damp
+  double precision :: ecorr(3), etot, etott2, edummy, damp, escf, etotal
   double precision, allocatable :: buf(:)
-  character(len=256) :: fname, arg, mode
+  character(len=256) :: fname, arg, mode, jobdir
   integer(c_int), target :: pop(8), vrt(8)
 
   call get_command_argument(1, fname)
@@ -52,6 +59,19 @@ program xvcc_amd
   damp = 1.0d0
 
   call aces_check(aces_init(c_null_ptr), 'aces_init')
+  jobdir = ''
+  if (len_trim(fname) > 7) then
+    if (fname(1:7) == 'jobdir:') jobdir = fname(8:)
+  end if
+  if (len_trim(jobdir) > 0) then
+    call aces_check(aces_jobarc_read(trim(jobdir)//c_null_char), 'ACES_JA_INIT')
+    call aces_check(aces_lists_read(trim(jobdir)//c_null_char), 'ACES_IO_INIT')
+    call getrec(1, 'JOBARC', 'COMPNIRR', 1, nirrep)
+    call getrec(1, 'JOBARC', 'SYMPOPOA', nirrep, popi)
+    call getrec(1, 'JOBARC', 'SYMPOPVA', nirrep, vrti)
+    o = sum(popi(1:nirrep))
+    v = sum(vrti(1:nirrep))
+  else
   open(10, file=trim(fname), access='stream', form='unformatted', status='old')
   read(10) o
   pop = 0
@@ -99,6 +119,7 @@ program xvcc_amd
   close(10)
   call putrec(20, 'JOBARC', 'SCFEVALA', n, buf)
   deallocate(buf)
+  end if
 
   ! MP1 start (xintprc) and DIIS lists (DIISLST)
   call aces_check(aces_vcc_mp1(), 'MP1')
@@ -139,6 +160,14 @@ program xvcc_amd
       exit
     end if
   end do
+  if (len_trim(jobdir) > 0 .and. iconvg == 0) then
+    escf = 0.0d0
+    call getrec(-1, 'JOBARC', 'SCFENEG ', 1, escf)
+    etotal = etot + escf
+    call putrec(1, 'JOBARC', 'TOTENERG', 1, etotal)
+    call aces_check(aces_lists_write(trim(jobdir)//c_null_char), 'ACES_IO_FIN')
+    call aces_check(aces_jobarc_write(trim(jobdir)//c_null_char), 'ACES_JA_FIN')
+  end if
   call aces_check(aces_fin(), 'aces_fin')
   if (iconvg /= 0) then
     write(6, '(a)') '@VCC: The Coupled-Cluster equations did not converge!'

@@ -259,6 +259,26 @@ double* aces_list_device_ptr(int group, int list);
  * reads its lists by irrep, acescore/aces_com_symloc.F); the bridge the
  * irrep-blocked driver path uses, exposed for hosts that hold C1-ordered data. */
 int aces_list_c1(int list, int typel, int typer, int irrepx, double* d_c1, int to_c1);
+/* The job archive and list files on disk, ILP64 layout (8-byte integers, one
+ * double per integer word), in directory dir (NULL or "": the working
+ * directory, where the reference keeps them):
+ *   aces_jobarc_read   JAINDX + JOBARC -> the record store (ACES_JA_INIT,
+ *                      acescore/aces_ja_init.F: the logical records, in slot order)
+ *   aces_jobarc_write  the record store -> JOBARC + JAINDX (ACES_JA_FIN,
+ *                      acescore/aces_ja_fin.F; record placement of acescore/putrec.F)
+ *   aces_lists_read    MOINTS/GAMLAM/MOABCD/DERINT/DERGAM -> the device list store,
+ *                      placed by the MOIOVEC/MOIOWRD/MOIOSIZ/MOIODIS/MOIOFIL records,
+ *                      pair types from ISYMTYP, physical records of IFLAGS(37)
+ *                      integers, symmetry from COMPNIRR/SYMPOP* (ACES_IO_INIT,
+ *                      acescore/aces_io_init.F, getlst.F); read the archive first
+ *   aces_lists_write   every list -> the list files, placed as
+ *                      acescore/aces_list_touch.F places them, and the MOIO*,
+ *                      ISYMTYP, TOTRECMO, TOTWRDMO, COMPNIRR, SYMPOP* records
+ *                      (ACES_IO_FIN, acescore/aces_io_fin.F); write the archive after */
+int aces_jobarc_read(const char* dir);
+int aces_jobarc_write(const char* dir);
+int aces_lists_read(const char* dir);
+int aces_lists_write(const char* dir);
 int aces_putrec(const char* name, long nwords, const void* src);
 int aces_getrec(int iflag, const char* name, long* nwords, void* dest);
 

@@ -199,6 +199,43 @@ def test_fortran_host_symmetry_packed_c2v_lists(h2o, tmp_path):
     assert 'Amplitude equations converged in    10 iterations.' in out.stdout
 
 
+@pytest.mark.gpu
+def test_fortran_host_job_directory(h2o, tmp_path):
+    """The host on a job directory (JAINDX, JOBARC and the list files, here with
+    the C2v symmetry-packed lists): it reads the archive and lists, prints the
+    reference's trace, and stores TOTENERG = SCFENEG + E(CCSD) in the archive
+    it writes back (vcc/finish.F:140) -- the reference's total CCSD energy."""
+    import re
+    import subprocess
+    from aces2 import lists
+    from oracle import jobarc
+    from test_fortran_gpu import EXE
+    print('case golden H2O fortran job directory', flush=True)
+    enuc, escf, F, eri, labels = h2o
+    perm, occ, vir = molint.irrep_order(labels, O)
+    p = np.array(perm)
+    job = tmp_path / 'job'
+    job.mkdir()
+    lists.init()
+    lists.load_sym_system(_blocks(F[np.ix_(p, p)], eri[np.ix_(p, p, p, p)]), occ, vir)
+    lists.putrec('SCFENEG', np.array([FIX['escf']]))
+    lists.lists_write(str(job))
+    lists.jobarc_write(str(job))
+    lists.fin()
+    out = subprocess.run([EXE, 'jobdir:' + str(job), str(FIX['cc_conv']), '50', str(FIX['cc_exporder'])],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    eng = [float(x) for x in re.findall(r'The total correlation energy is\s+(\S+) a\.u\.', out.stdout)]
+    res = [float(x.rstrip('.')) for x in re.findall(r'Largest element of residual vector :\s+(\S+)', out.stdout)]
+    _check_trace(eng[:-1], res, eng[-1])
+    arch = jobarc.Archive.read(str(job))
+    tot = float(arch.getrec('TOTENERG').view(np.float64)[0])
+    assert abs(tot - FIX['ccsd_total_energy']) < 1e-9
+    # the amplitude lists went back to MOINTS with the rest
+    moio = arch.getrec('MOIOVEC').reshape(500, 10)
+    assert moio[46 - 1, 0] > 0 and moio[90 - 1, 0] > 0 and moio[233 - 1, 0] > 0
+
+
 def test_oracle_triples_match_reference(h2o, cc_run):
     """E4T, E4TAAA, E4TBAA and E5ST of out.zmat.014a's TRPS at the converged amplitudes."""
     from oracle import triples
