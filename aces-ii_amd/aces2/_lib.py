@@ -93,6 +93,7 @@ SIGNATURES = {
     'aces_sumsym2': (c_int, [c_vp, c_int, c_int, c_int]),
     'aces_list_device_ptr': (c_vp, [c_int, c_int]),
     'aces_list_c1': (c_int, [c_int, c_int, c_int, c_int, c_vp, c_int]),
+    'aces_intprc_df': (c_int, [c_vp, c_int, c_vp]),
     'aces_jobarc_read': (c_int, [c_char_p]),
     'aces_jobarc_write': (c_int, [c_char_p]),
     'aces_lists_read': (c_int, [c_char_p]),

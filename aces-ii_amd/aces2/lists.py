@@ -103,6 +103,18 @@ def getrec(name, length, dtype=np.float64, iflag=1):
     return out
 
 
+def intprc_df(B, eps, occ, vir):
+    """Symmetry (irrep counts occ, vir) and the integral/denominator lists from
+    factors B (naux, n, n) and orbital energies eps, built on the GPU
+    (aces_intprc_df)."""
+    import torch
+    set_symmetry(list(occ), list(vir))
+    Bd = torch.as_tensor(np.ascontiguousarray(B, dtype=np.float64), device='cuda')
+    e = np.ascontiguousarray(eps, dtype=np.float64)
+    _lib.call('aces_intprc_df', Bd.data_ptr(), Bd.shape[0], e.ctypes.data)
+    del Bd
+
+
 def jobarc_read(path=''):
     """JAINDX + JOBARC in directory ``path`` -> the record store."""
     _lib.call('aces_jobarc_read', str(path).encode())

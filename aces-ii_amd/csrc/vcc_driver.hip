@@ -40,6 +40,7 @@
 #include "ccsd_kernels.h"
 #include "common.h"
 #include "dpd_lists.h"
+#include "intprc_df.h"
 #include "liststore.h"
 #include "sym_engine.h"
 #include "tensor_ops.h"
@@ -554,6 +555,13 @@ int aces_lists_write(const char* dir) {
   AcesContext* c;
   ACES_CHECK(ctx(&c));
   return lists_write(*c->lists, c->recs, dir ? dir : "");
+}
+
+int aces_intprc_df(const double* d_B, int naux, const double* eps) {
+  AcesContext* c;
+  ACES_CHECK(ctx(&c));
+  ACES_REQUIRE(!c->ready(), "aces_intprc_df: the lists are written before the engine is built (GENINT)");
+  return intprc_df(*c->lists, c->recs, d_B, naux, eps, c->stream);
 }
 
 int aces_putrec(const char* name, long nwords, const void* src) {

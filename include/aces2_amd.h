@@ -275,6 +275,18 @@ int aces_list_c1(int list, int typel, int typer, int irrepx, double* d_c1, int t
  *                      acescore/aces_list_touch.F places them, and the MOIO*,
  *                      ISYMTYP, TOTRECMO, TOTWRDMO, COMPNIRR, SYMPOP* records
  *                      (ACES_IO_FIN, acescore/aces_io_fin.F); write the archive after */
+/* The lists xintprc writes for the CC path, from factorised integrals on the
+ * GPU (intprc/intproc.F; list 233 at intprc/intproc.F:240-266; denominators
+ * intprc/dijab.F:178-185, 300-375): d_B (device) holds naux x n x n factors with
+ * (pq|rs) = sum_P B[P][p][q] B[P][r][s], each B[P] symmetric (density fitting or
+ * Cholesky over real orbitals), orbitals in list order (occupied then virtual,
+ * irrep-sorted in each, symmetry-forbidden elements zero); eps (host) the n
+ * orbital energies.  Under the symmetry set by aces_set_symmetry (closed shell,
+ * 1, 2, 4 or 8 irreps) it writes lists 13, 16, 10, 30, 23 and 233 (C1 or
+ * symmetry packed), the reciprocal denominators 1/D of lists 50 (AB), 48 (AA)
+ * and 48 group 9 (T1), and record SCFEVALA.  Every integral block is an fp64
+ * MFMA product over P. */
+int aces_intprc_df(const double* d_B, int naux, const double* eps);
 int aces_jobarc_read(const char* dir);
 int aces_jobarc_write(const char* dir);
 int aces_lists_read(const char* dir);

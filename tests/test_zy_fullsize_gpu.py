@@ -311,3 +311,45 @@ def test_fullsize_d2h_o64_v384():
     S = slices.DFSystem(B, np.diag(eps), o)
     del B
     _check_slices(S, t1h, t2h, i1h, i2h, [(0, 0), (o - 1, v - 1), (o // 2, v // 2)], 1e-11, 'fp64 D2h o=64 v=384')
+
+
+def test_fullsize_intprc_lists_o48_v320():
+    """The GPU list producer at the o=48, v=320 size (n_aux = 1472): list 233
+    (42 GB) and the other integral lists against (pq|rs) from the factors on
+    the host for sampled columns, the denominators, and the time it takes."""
+    import time
+    from aces2 import lists, synth
+    o, v, seed = 48, 320, 20260
+    print('case full size intprc o=%d v=%d' % (o, v), flush=True)
+    eps, B = synth.draw(o, v, seed, sigma=synth.scaled_sigma(o, v))
+    lists.init()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lists.intprc_df(B, eps, [o], [v])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print('intprc lists 13/16/10/30/23/233 + denominators: %.2f s' % dt, flush=True)
+    Bvv = B[:, o:, o:]
+    Bov = B[:, :o, o:]
+    nvp = v * (v + 1) // 2
+    assert lists.shape(1, 233) == (nvp, v * v)
+    pa, pb = synth.pair_le(v)
+    for (c, d) in [(0, 0), (v - 1, v - 1), (v // 3, 7), (5, v - 2)]:
+        col = lists.getlst(c + v * d + 1, 1, 1, 233).ravel()
+        M = Bvv[:, :, c].T @ Bvv[:, :, d]            # (ac|bd) as M[a][b]
+        ref = M[pa, pb]
+        assert np.abs(col - ref).max() < 1e-12 * np.abs(ref).max(), (c, d)
+    # list 16 <ab|ij> = (ai|bj): column (i, j) at i + o j
+    for (i, j) in [(0, 0), (o - 1, 3), (o // 2, o - 1)]:
+        col = lists.getlst(i + o * j + 1, 1, 1, 16).ravel().reshape(v, v)    # [b][a]
+        ref = (Bov[:, i, :].T @ Bov[:, j, :]).T                             # [b][a] = (ai|bj)
+        assert np.abs(col - ref).max() < 1e-12 * np.abs(ref).max(), (i, j)
+    # list 30 <ab|ci> = (ib|ac): column (c, i) at c + v i
+    for (c, i) in [(0, 0), (v - 1, o - 1), (v // 2, 5)]:
+        col = lists.getlst(c + v * i + 1, 1, 1, 30).ravel().reshape(v, v)    # [b][a]
+        ref = Bov[:, i, :].T @ Bvv[:, :, c]                               # [b][a] = (ib|ac)
+        assert np.abs(col - ref).max() < 1e-12 * np.abs(ref).max(), (c, i)
+    d1 = lists.getlst(1, 1, 9, 48).ravel()
+    assert np.abs(d1 - 1.0 / (eps[:o][:, None] - eps[o:][None, :]).ravel()).max() < 1e-15 * np.abs(d1).max()
+    lists.fin()
+    _free()
