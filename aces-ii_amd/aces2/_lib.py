@@ -48,6 +48,7 @@ SIGNATURES = {
     'aces_ccsd_profile': (c_int, [c_vp, c_dp, c_int]),
     'aces_ccsd_set_fock': (c_int, [c_vp, c_vp]),
     'aces_ccsd_triples': (c_int, [c_vp, c_int, c_dp, c_vp]),
+    'aces_ccsd_eom': (c_int, [c_vp, c_vp, c_int, c_int, c_double, c_int, c_dp, c_dp, c_vp, c_vp]),
     'aces_ccsd_adjoint': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     'aces_ccsd_lambda_start': (c_int, [c_vp]),
     'aces_ccsd_lambda_cycle': (c_int, [c_vp, c_int, c_double, c_dp]),
@@ -94,6 +95,8 @@ SIGNATURES = {
     'aces_list_device_ptr': (c_vp, [c_int, c_int]),
     'aces_list_c1': (c_int, [c_int, c_int, c_int, c_int, c_vp, c_int]),
     'aces_intprc_df': (c_int, [c_vp, c_int, c_vp]),
+    'aces_small_eig': (c_int, [c_int, c_vp, c_vp, c_vp]),
+    'aces_small_eigvec': (c_int, [c_int, c_vp, c_double, c_vp]),
     'aces_jobarc_read': (c_int, [c_char_p]),
     'aces_jobarc_write': (c_int, [c_char_p]),
     'aces_lists_read': (c_int, [c_char_p]),

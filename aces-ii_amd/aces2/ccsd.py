@@ -117,6 +117,30 @@ class CCSD:
             r['per_triple'] = per
         return r
 
+    def eom(self, nroots=1, irrep=0, orb_irrep=None, tol=1e-7, maxit=100, vectors=False):
+        """EOM-CCSD singlet excitation energies at the current (converged)
+        amplitudes: dict with omega (hartree), resid, iters and, with
+        ``vectors``, the left eigenvectors (device, (nroots, o v + o^2 v^2)).
+        ``orb_irrep``: orbital irreps (occupied then virtual) to restrict to the
+        excitations of ``irrep``; None: no symmetry."""
+        torch = self.torch
+        om = (ctypes.c_double * nroots)()
+        rs = (ctypes.c_double * nroots)()
+        it = ctypes.c_int()
+        irr = None
+        if orb_irrep is not None:
+            irr = (ctypes.c_int * len(orb_irrep))(*[int(x) for x in orb_irrep])
+        vec = None
+        if vectors:
+            nt = self.o * self.v + (self.o * self.v) ** 2
+            vec = torch.empty((nroots, nt), device='cuda', dtype=torch.float64)
+        _lib.call(self._p + 'eom', self.h, irr, int(irrep), int(nroots), float(tol), int(maxit), om, rs,
+                  ctypes.byref(it), vec.data_ptr() if vec is not None else None)
+        out = {'omega': list(om), 'resid': list(rs), 'iters': it.value}
+        if vectors:
+            out['vectors'] = vec
+        return out
+
     def set_fock(self, fock):
         torch = self.torch
         f = torch.as_tensor(fock, device='cuda', dtype=torch.float64).contiguous()

@@ -169,6 +169,16 @@ int aces_ccsd_density(void* h, double* dens) {
   return eng(h)->density(dens);
 }
 
+int aces_ccsd_eom(void* h, const int* orb_irrep, int irrep, int nroots, double tol, int maxit, double* omega,
+                  double* resid, int* iters, double* d_vecs) {
+  REQUIRE_HANDLE(h);
+  if (omega == nullptr) {
+    set_error("aces_ccsd_eom: null output");
+    return ACES_ERR_ARG;
+  }
+  return eng(h)->eom(orb_irrep, irrep, nroots, tol, maxit, omega, resid, iters, d_vecs);
+}
+
 int aces_ccsd_triples(void* h, int prec, double* out, double* d_per_triple) {
   REQUIRE_HANDLE(h);
   if (out == nullptr) {

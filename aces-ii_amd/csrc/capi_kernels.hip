@@ -1,6 +1,7 @@
 // C-ABI entry points for the raw kernels (GEMM engine and operand conversion).
 #include "common.h"
 #include "gemm.h"
+#include "small_eig.h"
 #include "tensor_ops.h"
 
 using namespace aces;
@@ -47,6 +48,22 @@ int aces_convert_f32(int rows, int cols, const double* x, long ldx, float* y, lo
 int aces_split_bf16(int rows, int cols, const double* x, long ldx, unsigned short* hi, unsigned short* lo,
                     long ldy, void* stream) {
   return split_bf16(rows, cols, x, ldx, hi, lo, ldy, (hipStream_t)stream);
+}
+
+int aces_small_eig(int n, const double* A, double* wr, double* wi) {
+  if (n < 1 || !A || !wr || !wi) {
+    set_error("aces_small_eig: bad arguments");
+    return ACES_ERR_ARG;
+  }
+  return eig_nonsym(n, A, wr, wi);
+}
+
+int aces_small_eigvec(int n, const double* A, double w, double* x) {
+  if (n < 1 || !A || !x) {
+    set_error("aces_small_eigvec: bad arguments");
+    return ACES_ERR_ARG;
+  }
+  return eigvec_real(n, A, w, x);
 }
 
 }  // extern "C"

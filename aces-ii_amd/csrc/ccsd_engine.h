@@ -100,6 +100,17 @@ class CcsdEngine {
   // Response density at the current multipliers (n x n, device).
   int density(double* dens);
 
+  // ---- EOM-CCSD excitation energies (ccsd_eom.hip) ----
+  // The nroots lowest singlet excitation energies at the current (converged)
+  // amplitudes: eigenvalues of the CC Jacobian, from its left eigenvectors
+  // (products J^T l by the reverse pass).  orb_irrep (host, n entries,
+  // occupied then virtual; nullptr: no symmetry) selects the excitations of
+  // irrep `target` (XOR of the orbital irreps).  omega/resid (host, nroots):
+  // eigenvalues and residual norms; iters: Davidson iterations; d_vecs (device,
+  // optional, nroots x (o v + o^2 v^2)): the left eigenvectors [l1 | l2].
+  int eom(const int* orb_irrep, int target, int nroots, double tol, int maxit, double* omega, double* resid,
+          int* iters, double* d_vecs);
+
   // ---- non-iterative triples (triples.hip) ----
   // (T) of CCSD(T) at the current amplitudes (pair-symmetric t2, canonical HF
   // orbital energies from the Fock diagonal): out[0] E[4]T, out[1] E[5]ST,

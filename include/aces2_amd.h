@@ -136,6 +136,19 @@ int aces_ccsd_profile(void* handle, double* out, int reset);
  * ~1e-6); the energy pass is fp64 either way.  Uses o v^2 (v + o) + o^2 v (v + o)
  * elements (8 bytes fp64, 4 bytes bf16x3) of device memory during the call. */
 int aces_ccsd_triples(void* handle, int prec, double* out, double* d_per_triple);
+/* EOM-CCSD excitation energies (the xvee EOMEE singlet roots of a closed-shell
+ * CCSD, vee/; Hbar enters matrix free): the nroots lowest eigenvalues of the CC
+ * Jacobian at the converged amplitudes, Hbar - E_CC on the singles and doubles
+ * space, by a Davidson iteration on its left eigenvectors whose products J^T l
+ * are the engine's reverse pass.  orb_irrep (host, n ints: occupied then
+ * virtual; NULL = no symmetry) with irrep (0-based, XOR product of orbital
+ * irreps) selects one symmetry block, as ESTATE_SYM does.  omega (host): the
+ * excitation energies (hartree); resid (host, optional): residual norms; iters
+ * (optional): iterations; d_vecs (device, optional, nroots x (o v + o^2 v^2)):
+ * the left eigenvectors [l1 | l2].  Converges to |residual| < tol (maxit
+ * iterations at most; check resid).  Overwrites the engine's reverse-pass scratch. */
+int aces_ccsd_eom(void* handle, const int* orb_irrep, int irrep, int nroots, double tol, int maxit,
+                  double* omega, double* resid, int* iters, double* d_vecs);
 
 /* ------------------------------------------------------------------------
  * Differentiation form: Lambda equations and response density (reverse pass

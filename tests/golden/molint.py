@@ -407,3 +407,26 @@ def mo_system(atoms_basis, nocc, occ_counts=None, axes=None):
     if axes is not None:
         return enuc, e_el + enuc, Fmo, eri_mo, eps, reflection_labels(shells, C, S, axes)
     return enuc, e_el + enuc, Fmo, eri_mo, eps
+
+
+def water_atoms(basis, r_bohr, angle_rad):
+    """Z-matrix H; O 1 R; H 2 R 1 A in a C2v frame: C2 axis along z (bisecting
+    the HOH angle), molecular plane x = 0.  ``basis``: {'H': shells, 'O': shells}
+    as returned by read_genbas_entry."""
+    R, A = r_bohr, angle_rad
+    atoms = [(1.0, [0.0, 0.0, R], basis['H']), (8.0, [0.0, 0.0, 0.0], basis['O']),
+             (1.0, [0.0, R * np.sin(A), R * np.cos(A)], basis['H'])]
+    c = np.array([a[1] for a in atoms])
+    ez = (c[0] + c[2]) / 2 - c[1]
+    ez /= np.linalg.norm(ez)
+    ey = np.cross(ez, [1.0, 0.0, 0.0])
+    ey /= np.linalg.norm(ey)
+    ex = np.cross(ey, ez)
+    Rm = np.array([ex, ey, ez])
+    return [(Z, list(Rm @ np.asarray(C)), b) for (Z, C, b) in atoms]
+
+
+def irrep_indices(labels):
+    """Per MO the irrep index (bit k set when odd under generator k): the
+    irrep of a product is the XOR of the indices."""
+    return [sum(1 << k for k, c in enumerate(lab) if c < 0) for lab in labels]

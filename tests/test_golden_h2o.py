@@ -38,20 +38,7 @@ def h2o():
     R = z['R_angstrom'] / z['bohr_angstrom']
     A = np.radians(z['A_degree'])
     bas = {el: [(b['l'], b['exps'], np.array(b['coefs'])) for b in FIX['basis'][el]] for el in ('H', 'O')}
-    # Z-matrix H; O 1 R; H 2 R 1 A: O at the origin, H1 on +z, H2 in the yz plane (C2v, plane x = 0)
-    atoms = [(1.0, [0.0, 0.0, R], bas['H']), (8.0, [0.0, 0.0, 0.0], bas['O']),
-             (1.0, [0.0, R * np.sin(A), R * np.cos(A)], bas['H'])]
-    # the molecule is symmetric under x -> -x; the C2 axis bisects the HOH angle, so
-    # rotate the frame to put it on z (the y -> -y reflection then swaps the H atoms)
-    c = np.array([a[1] for a in atoms])
-    axis = (c[0] + c[2]) / 2 - c[1]
-    axis /= np.linalg.norm(axis)
-    ez = axis
-    ey = np.cross(ez, [1.0, 0.0, 0.0])
-    ey /= np.linalg.norm(ey)
-    ex = np.cross(ey, ez)
-    Rm = np.array([ex, ey, ez])
-    atoms = [(Z, list(Rm @ np.asarray(C)), b) for (Z, C, b) in atoms]
+    atoms = molint.water_atoms(bas, R, A)
     enuc, escf, F, eri, eps, labels = molint.mo_system(atoms, O, axes=(0, 1))
     return enuc, escf, F, eri, labels
 
