@@ -175,14 +175,22 @@ struct Box {
   int d[4];
   int g[4];
   long first;   // first element chunk (prefix over boxes)
-  long tfirst;  // first LDS tile of the tiled walk (prefix over boxes)
+  long tfirst;  // first tile of the tiled walk (prefix over boxes)
+  // Tiled walk: this box's tile extents along output indices 3, tq and rb
+  // (te), the number of tiles along each (tn), and the reciprocals that
+  // divide an element number by te[0] and te[1] (umulhi magic, 0 for 1).
+  int te[3], tn[3];
+  unsigned mg[2];
   long ob, os[4];
   long sb[3], ss[3][4];
   long xb, xs[4], yb, ys[4];  // xb < 0: product term vanishes in this box
 };
 
 constexpr long kChunk = 4096;  // elements per workgroup in box walks
-constexpr int kTile3 = 64, kTileQ = 32;  // tiled walk: output tile (index 3) x (index tq)
+// Tiled walk: a tile spans at most kTileSpan along index 3 and along tq, a
+// thread of its 256 takes at most kTileRows output elements, and the tile's
+// LDS image (pitch tq-extent | 1) fits kTileLds doubles.
+constexpr int kTileSpan = 64, kTileRows = 8, kTileLds = 2304;
 
 // Box of workgroup `chunk` (binary search over Box::first).
 __device__ inline int box_of_chunk(const Box* boxes, int nbox, long chunk) {
@@ -210,6 +218,10 @@ struct Plan {
   int tq = -1;
   int tr = 0;
   long ttotal = 0;
+  // box of every tile (device, stored after the boxes), and whether a tile's
+  // element offsets relative to its first element need 64 bits
+  const int* tbox = nullptr;
+  bool wide = false;
 };
 
 // Source description for plan building: layout and, for each source

@@ -133,6 +133,40 @@ int source_fast_index(const Plan& p, int rank, int s) {
   return 3;
 }
 
+// Reciprocal for the tiled kernel's divisions: floor(e / d) = umulhi(e, m)
+// with m = ceil(2^32 / d), exact for e * d < 2^32 (here e, d <= 4096).
+static unsigned div_magic(int d) {
+  return d <= 1 ? 0u : (unsigned)(((1ull << 32) + (unsigned long long)d - 1) / (unsigned long long)d);
+}
+
+// Tile shape of one box.  The kernel walks a tile as rows along index 3:
+// 256 / te[0] rows at a time, at most kTileRows rows per thread, so a tile
+// holds te[1] * te[2] <= kTileRows * (256 / te[0]) rows.  Indices 3 and q
+// are cut into balanced pieces of at most kTileSpan (a box of 48 x 9 is one
+// 48 x 9 piece, not a 64 x 32 tile that it fills to a fifth), then as many
+// values of index rb join as the row budget and the LDS image (pitch
+// te[1] | 1) allow.
+static void box_tiles(Box& b, int q, int rb) {
+  const int d3 = b.d[3], dq = b.d[q], dr = b.d[rb];
+  const int n3 = (d3 + kTileSpan - 1) / kTileSpan, t3 = (d3 + n3 - 1) / n3;
+  const int rows = kTileRows * (256 / t3);
+  const int qmax = std::min(kTileSpan, rows);
+  const int nq = (dq + qmax - 1) / qmax, tq = (dq + nq - 1) / nq;
+  const int pitch = tq | 1;
+  int tr = std::max(1, std::min(dr, rows / tq));
+  while (tr > 1 && (long)tr * t3 * pitch > kTileLds) --tr;
+  const int nr = (dr + tr - 1) / tr;
+  tr = (dr + nr - 1) / nr;
+  b.te[0] = t3;
+  b.te[1] = tq;
+  b.te[2] = tr;
+  b.tn[0] = n3;
+  b.tn[1] = nq;
+  b.tn[2] = nr;
+  b.mg[0] = div_magic(t3);
+  b.mg[1] = div_magic(tq);
+}
+
 static void plan_tiles(Plan& p, int rank) {
   p.tq = -1;
   p.tr = 0;
@@ -174,11 +208,24 @@ static void plan_tiles(Plan& p, int rank) {
   long t = 0;
   for (Box& b : p.host) {
     b.tfirst = t;
-    t += (long)((b.d[3] + kTile3 - 1) / kTile3) * ((b.d[q] + kTileQ - 1) / kTileQ) * b.d[ra] * b.d[rb];
+    box_tiles(b, q, rb);
+    t += (long)b.tn[0] * b.tn[1] * b.tn[2] * b.d[ra];
   }
   p.tq = q;
   p.tr = mask;
   p.ttotal = t;
+  // 32-bit in-tile offsets when every stride walk of a tile stays below 2^31
+  p.wide = false;
+  auto span = [&](const long* st, const Box& b) {
+    return (long)(b.te[0] - 1) * std::labs(st[3]) + (long)(b.te[1] - 1) * std::labs(st[q]) +
+           (long)(b.te[2] - 1) * std::labs(st[rb]);
+  };
+  for (const Box& b : p.host) {
+    long m = span(b.os, b);
+    for (int s = 0; s < p.nsrc; ++s) m = std::max(m, span(b.ss[s], b));
+    if (p.prod) m = std::max(m, std::max(span(b.xs, b), span(b.ys, b)));
+    if (m >= (1L << 31)) p.wide = true;
+  }
 }
 
 int build_boxes(Plan& p, const Layout& out, const std::vector<Src>& srcs, const Src* X, const Src* Y) {
@@ -254,8 +301,20 @@ int build_plan(Plan& p, const Layout& out, const std::vector<Src>& srcs, const S
   ACES_CHECK(build_boxes(p, out, srcs, X, Y));
   free_plan(p);
   if (p.nbox > 0) {
-    ACES_HIP(hipMalloc((void**)&p.dev, sizeof(Box) * p.nbox));
+    const long nt = (p.tq >= 0) ? p.ttotal : 0;
+    ACES_REQUIRE(nt < (1L << 31), "dpd plan: too many tiles");
+    ACES_HIP(hipMalloc((void**)&p.dev, sizeof(Box) * p.nbox + sizeof(int) * nt));
     ACES_HIP(hipMemcpy(p.dev, p.host.data(), sizeof(Box) * p.nbox, hipMemcpyHostToDevice));
+    if (nt > 0) {
+      std::vector<int> tb((size_t)nt);
+      for (int k = 0; k < p.nbox; ++k) {
+        const long end = (k + 1 < p.nbox) ? p.host[k + 1].tfirst : nt;
+        for (long t = p.host[k].tfirst; t < end; ++t) tb[(size_t)t] = k;
+      }
+      int* dt = reinterpret_cast<int*>(p.dev + p.nbox);
+      ACES_HIP(hipMemcpy(dt, tb.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+      p.tbox = dt;
+    }
   }
   return ACES_OK;
 }
@@ -263,6 +322,7 @@ int build_plan(Plan& p, const Layout& out, const std::vector<Src>& srcs, const S
 void free_plan(Plan& p) {
   if (p.dev) (void)hipFree(p.dev);
   p.dev = nullptr;
+  p.tbox = nullptr;
 }
 
 struct ApplyArgs {
@@ -314,88 +374,119 @@ __global__ __launch_bounds__(256) void apply_plan_kernel(ApplyArgs a) {
   }
 }
 
-// Tiled walk: one workgroup per (box, 64 x 32 tile of output indices 3 and tq,
-// one value of each remaining index).  Sources in the mask tr are read along
-// tq (their unit-stride index) and turned round through LDS; the others, the
-// denominator and the product term follow the output order.
-__device__ inline int box_of_tile(const Box* boxes, int nbox, long tile) {
-  int lo = 0, hi = nbox - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (boxes[mid].tfirst <= tile)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
+// Tiled walk: one workgroup per tile of a box (Box::te values of output
+// indices 3, tq and rb, one value of index ra).  Sources in the mask tr are
+// read along tq (their unit-stride index, q fastest, then 3, then rb) into an
+// LDS image of the tile and read back in the output's order; the others, the
+// denominator and the product term follow the output order (3 fastest).
+__device__ inline int div_by(int e, int d, unsigned m) { return d == 1 ? e : (int)__umulhi((unsigned)e, m); }
 
-__global__ __launch_bounds__(256) void apply_plan_tiled_kernel(ApplyArgs a, int tq, int tr) {
-  __shared__ double tile[kTile3][kTileQ + 1];
-  __shared__ int sb;
-  if (threadIdx.x == 0) sb = box_of_tile(a.boxes, a.nbox, blockIdx.x);
-  __syncthreads();
-  const Box& b = a.boxes[sb];
+template <typename Off>
+__global__ __launch_bounds__(256) void apply_plan_tiled_kernel(ApplyArgs a, int tq, int tr, const int* tbox) {
+  __shared__ double tile[kTileLds];
+  const Box& b = a.boxes[tbox[blockIdx.x]];
   const int ra = (tq == 0) ? 1 : 0, rb = (tq == 2) ? 1 : 2;
-  const int d3 = b.d[3], dq = b.d[tq];
-  const long n3 = (d3 + kTile3 - 1) / kTile3, nq = (dq + kTileQ - 1) / kTileQ;
-  long t = blockIdx.x - b.tfirst;
-  const int t3 = (int)(t % n3);
-  t /= n3;
-  const int tqi = (int)(t % nq);
-  t /= nq;
-  const int ib = (int)(t % b.d[rb]);
-  const int ia = (int)(t / b.d[rb]);
-  const int i3b = t3 * kTile3, iqb = tqi * kTileQ;
+  const int T3 = b.te[0], TQ = b.te[1], TR = b.te[2];
+  const unsigned m3 = b.mg[0], mq = b.mg[1];
+  int t = (int)(blockIdx.x - b.tfirst);
+  const int t3 = t % b.tn[0];
+  t /= b.tn[0];
+  const int tqi = t % b.tn[1];
+  t /= b.tn[1];
+  const int tri = t % b.tn[2];
+  const int ia = t / b.tn[2];
+  const int i3b = t3 * T3, iqb = tqi * TQ, irb = tri * TR;
+  const int e3 = min(T3, b.d[3] - i3b), eq = min(TQ, b.d[tq] - iqb), er = min(TR, b.d[rb] - irb);
+  const int P = TQ | 1;
   const int tid = threadIdx.x;
-  const int c3 = tid & 63, r0 = tid >> 6;  // output walk: rows iqb + r0 + 4k, column i3b + c3
-  const int cq = tid & 31, s0 = tid >> 5;  // LDS reads: column iqb + cq, rows i3b + s0 + 8k
-  const bool ok3 = i3b + c3 < d3;
-  double acc[8];
+  // output walk: thread (c3, j0) takes rows j0, j0 + RP, ... (row j = q + TQ r)
+  const int RP = 256 / T3;
+  const int j0 = div_by(tid, T3, m3);
+  const int c3 = tid - j0 * T3;
+  const bool act = j0 < RP && c3 < e3;
+  const int R = TQ * TR;
+  double acc[kTileRows];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+  for (int k = 0; k < kTileRows; ++k) acc[k] = 0.0;
   for (int s = 0; s < a.nsrc; ++s) {
     const long* st = b.ss[s];
     const double al = a.alpha[s];
-    const double* base = a.src[s] + b.sb[s] + ia * st[ra] + ib * st[rb];
+    const Off s3 = (Off)st[3], sq = (Off)st[tq], sr = (Off)st[rb];
+    const double* base = a.src[s] + b.sb[s] + ia * st[ra] + (long)i3b * s3 + (long)iqb * sq + (long)irb * sr;
     if ((tr >> s) & 1) {
-      const bool okq = iqb + cq < dq;
-      const double* src = base + (long)(iqb + cq) * st[tq] + (long)(i3b + s0) * st[3];
+      // read along q: thread (q, jq) takes rows jq, jq + RPq, ... (row = c3 + T3 r)
+      const int RPq = 256 / TQ;
+      const int jq = div_by(tid, TQ, mq);
+      const int qq = tid - jq * TQ;
       __syncthreads();  // the previous reader of the tile is done
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        tile[s0 + 8 * k][cq] = (okq && i3b + s0 + 8 * k < d3) ? src[(long)(8 * k) * st[3]] : 0.0;
+      if (jq < RPq && qq < eq) {
+#pragma unroll 4
+        for (int j = jq; j < T3 * TR; j += RPq) {
+          const int r = div_by(j, T3, m3);
+          const int cc = j - r * T3;
+          tile[j * P + qq] = (cc < e3 && r < er) ? base[qq * sq + cc * s3 + r * sr] : 0.0;
+        }
+      }
       __syncthreads();
 #pragma unroll
-      for (int k = 0; k < 8; ++k) acc[k] += al * tile[c3][r0 + 4 * k];
+      for (int k = 0; k < kTileRows; ++k) {
+        const int j = j0 + RP * k;
+        if (act && j < R) {
+          const int r = div_by(j, TQ, mq);
+          const int q = j - r * TQ;
+          acc[k] += al * tile[(r * T3 + c3) * P + q];
+        }
+      }
     } else {
-      const double* src = base + (long)(iqb + r0) * st[tq] + (long)(i3b + c3) * st[3];
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (ok3 && iqb + r0 + 4 * k < dq) acc[k] += al * src[(long)(4 * k) * st[tq]];
+      for (int k = 0; k < kTileRows; ++k) {
+        const int j = j0 + RP * k;
+        if (act && j < R) {
+          const int r = div_by(j, TQ, mq);
+          const int q = j - r * TQ;
+          if (q < eq && r < er) acc[k] += al * base[c3 * s3 + q * sq + r * sr];
+        }
+      }
     }
   }
+  const Off os3 = (Off)b.os[3], osq = (Off)b.os[tq], osr = (Off)b.os[rb];
+  double* obase = a.out + b.ob + ia * b.os[ra] + (long)i3b * os3 + (long)iqb * osq + (long)irb * osr;
+  const bool prod = a.X != nullptr && b.xb >= 0;
+  const double* xbase = a.X;
+  const double* ybase = a.Y;
+  Off xs3 = 0, xsq = 0, xsr = 0, ys3 = 0, ysq = 0, ysr = 0;
+  if (prod) {
+    xbase += b.xb + ia * b.xs[ra] + i3b * b.xs[3] + iqb * b.xs[tq] + irb * b.xs[rb];
+    ybase += b.yb + ia * b.ys[ra] + i3b * b.ys[3] + iqb * b.ys[tq] + irb * b.ys[rb];
+    xs3 = (Off)b.xs[3], xsq = (Off)b.xs[tq], xsr = (Off)b.xs[rb];
+    ys3 = (Off)b.ys[3], ysq = (Off)b.ys[tq], ysr = (Off)b.ys[rb];
+  }
+  // denominator: the parts of indices 3 and ra are fixed for this thread
+  double Dfix = 0.0;
+  if (a.den.pow != 0 && act) {
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int iq = iqb + r0 + 4 * k;
-    if (!ok3 || iq >= dq) continue;
-    int l[4];
-    l[ra] = ia;
-    l[rb] = ib;
-    l[tq] = iq;
-    l[3] = i3b + c3;
+    for (int m = 0; m < 4; ++m)
+      if (a.den.sign[m] != 0 && (m == 3 || m == ra))
+        Dfix += a.den.sign[m] * a.den.e[m][a.den.off[m][b.g[m]] + (m == 3 ? i3b + c3 : ia)];
+  }
+#pragma unroll
+  for (int k = 0; k < kTileRows; ++k) {
+    const int j = j0 + RP * k;
+    if (!act || j >= R) continue;
+    const int r = div_by(j, TQ, mq);
+    const int q = j - r * TQ;
+    if (q >= eq || r >= er) continue;
     double v = acc[k];
     if (a.den.pow != 0) {
-      double D = 0.0;
+      double D = Dfix;
 #pragma unroll
-      for (int m = 0; m < 4; ++m)
-        if (a.den.sign[m] != 0) D += a.den.sign[m] * a.den.e[m][a.den.off[m][b.g[m]] + l[m]];
+      for (int m = 0; m < 3; ++m)
+        if (a.den.sign[m] != 0 && m != ra)
+          D += a.den.sign[m] * a.den.e[m][a.den.off[m][b.g[m]] + (m == tq ? iqb + q : irb + r)];
       v = (a.den.pow > 0) ? v * D : v / D;
     }
-    if (a.X != nullptr && b.xb >= 0)
-      v += a.gamma * a.X[b.xb + l[0] * b.xs[0] + l[1] * b.xs[1] + l[2] * b.xs[2] + l[3] * b.xs[3]] *
-           a.Y[b.yb + l[0] * b.ys[0] + l[1] * b.ys[1] + l[2] * b.ys[2] + l[3] * b.ys[3]];
-    double* o = a.out + b.ob + l[0] * b.os[0] + l[1] * b.os[1] + l[2] * b.os[2] + l[3] * b.os[3];
+    if (prod) v += a.gamma * xbase[c3 * xs3 + q * xsq + r * xsr] * ybase[c3 * ys3 + q * ysq + r * ysr];
+    double* o = obase + c3 * os3 + q * osq + r * osr;
     *o = (a.beta == 0.0) ? v : a.beta * (*o) + v;
   }
 }
@@ -419,8 +510,13 @@ int apply_plan(const Plan& p, double* out, double beta, const double* const* src
   a.gamma = gamma;
   if (den) a.den = *den;
   if (p.tq >= 0) {
-    ACES_REQUIRE(p.ttotal < (1L << 31), "dpd apply: too many tiles");
-    hipLaunchKernelGGL(apply_plan_tiled_kernel, dim3((unsigned)p.ttotal), dim3(256), 0, s, a, p.tq, p.tr);
+    ACES_REQUIRE(p.ttotal < (1L << 31) && p.tbox != nullptr, "dpd apply: tile table missing or too many tiles");
+    if (p.wide)
+      hipLaunchKernelGGL(apply_plan_tiled_kernel<long>, dim3((unsigned)p.ttotal), dim3(256), 0, s, a, p.tq, p.tr,
+                         p.tbox);
+    else
+      hipLaunchKernelGGL(apply_plan_tiled_kernel<int>, dim3((unsigned)p.ttotal), dim3(256), 0, s, a, p.tq, p.tr,
+                         p.tbox);
     ACES_HIP(hipGetLastError());
     return ACES_OK;
   }
