@@ -150,6 +150,8 @@ class SymEngine {
   std::vector<LowBuf> low_;
   dpd::LadderDesc lad_;
   dpd::LadderDesc* d_lad_ = nullptr;
+  dpd::UnpackTile* d_unpack_tiles_ = nullptr;  // tiles of the ladder unpack into inc2
+  long n_unpack_tiles_ = 0;
   dpd::Layout* d_layouts_ = nullptr;  // device copies: 0 tau/t2 (O,O|V,V), 1 Yb
   double *eo_ = nullptr, *ev_ = nullptr, *geo_ = nullptr, *gev_ = nullptr;
   double *scal_ = nullptr, *ws_ = nullptr, *scr_ = nullptr, *scr2_ = nullptr;

@@ -27,9 +27,11 @@ constexpr int kDotBlocks = 512;
 constexpr int kMaxDots = 32;
 
 // part[k*kDotBlocks + b] = block b's share of sum_i xs[k][i] * y[i]
+// (256 threads; wave sums by shuffles, then the four wave sums in order).
+// Only constant indices into acc: a runtime index would move it to scratch.
 __global__ __launch_bounds__(256) void mdot_kernel(long n, int m, const double* const* __restrict__ xs,
                                                    const double* __restrict__ y, double* __restrict__ part) {
-  __shared__ double sh[256];
+  __shared__ double sh[4][kMaxDots];
   double acc[kMaxDots];
 #pragma unroll
   for (int k = 0; k < kMaxDots; ++k) acc[k] = 0.0;
@@ -39,16 +41,19 @@ __global__ __launch_bounds__(256) void mdot_kernel(long n, int m, const double* 
     for (int k = 0; k < kMaxDots; ++k)
       if (k < m) acc[k] += xs[k][i] * yi;
   }
-  for (int k = 0; k < m; ++k) {
-    sh[threadIdx.x] = acc[k];
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-      __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kMaxDots; ++k) {
+    if (k < m) {
+      double v = acc[k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0) sh[wv][k] = v;
     }
-    if (threadIdx.x == 0) part[(long)k * kDotBlocks + blockIdx.x] = sh[0];
-    __syncthreads();
   }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < m) part[(long)t * kDotBlocks + blockIdx.x] = ((sh[0][t] + sh[1][t]) + sh[2][t]) + sh[3][t];
 }
 
 __global__ void mdot_finish_kernel(int nb, const double* __restrict__ part, double* __restrict__ out, double scale,
@@ -492,16 +497,26 @@ int SymEngine::lowp_convert(int t, GemmPrec prec, LowBuf& lb, const double* sub)
     lb.hi = p;
     lb.lo = (char*)p + off * 2;
   }
+  // every block of the tensor in one launch
+  SplitBlocks sb;
+  sb.f32 = (prec == GP_F32);
   for (int g = 0; g < nirr; ++g) {
     if (tt.nrow[g] == 0 || tt.ncol[g] == 0) continue;
-    if (prec == GP_F32)
-      ACES_CHECK(convert_f32((int)tt.nrow[g], (int)tt.ncol[g], tt.d + tt.boff[g], tt.ncol[g],
-                             (float*)lb.hi + lb.off[g], lb.ld[g], s_));
-    else
-      ACES_CHECK(split_bf16((int)tt.nrow[g], (int)tt.ncol[g], tt.d + tt.boff[g], tt.ncol[g],
-                            (unsigned short*)lb.hi + lb.off[g], (unsigned short*)lb.lo + lb.off[g], lb.ld[g], s_,
-                            sub ? sub + tt.boff[g] : nullptr));
+    const int k = sb.n++;
+    sb.rows[k] = (int)tt.nrow[g];
+    sb.cols[k] = (int)tt.ncol[g];
+    sb.ldy[k] = lb.ld[g];
+    sb.x[k] = tt.d + tt.boff[g];
+    sb.x0[k] = sub ? sub + tt.boff[g] : nullptr;
+    if (sb.f32) {
+      sb.hi[k] = (float*)lb.hi + lb.off[g];
+      sb.lo[k] = nullptr;
+    } else {
+      sb.hi[k] = (unsigned short*)lb.hi + lb.off[g];
+      sb.lo[k] = (unsigned short*)lb.lo + lb.off[g];
+    }
   }
+  ACES_CHECK(split_blocks(sb, s_));
   lb.prec = prec;
   lb.valid = true;
   return ACES_OK;
@@ -775,8 +790,8 @@ int SymEngine::forward(GemmPrec prec) {
         ACES_CHECK(dpd::pack_tau_sym(d_lad_, lad_, d_layouts_, T_[tau_].d, T_[Sp_].d, T_[Dp_].d, s_));
         break;
       case OP_UNPACK:
-        ACES_CHECK(dpd::unpack_ladder_sym(d_lad_, op.fwd, op.hplan, T_[Zp_].d, T_[Zm_].d, T_[inc2_].d,
-                                          T_[Yb_].d, s_));
+        ACES_CHECK(dpd::unpack_ladder_sym(d_lad_, op.fwd, d_unpack_tiles_, n_unpack_tiles_, op.hplan, T_[Zp_].d,
+                                          T_[Zm_].d, T_[inc2_].d, T_[Yb_].d, s_));
         ++T_[Yb_].ver;
         break;
     }
@@ -1196,6 +1211,15 @@ int SymEngine::setup(const double* fock, const double* ovov_dense, const double*
     op.out = inc2_;
     ACES_CHECK(dpd::build_plan(op.fwd, LOOVV, {}, nullptr, nullptr));
     ACES_CHECK(dpd::build_plan(op.hplan, LVOOO3, {}, nullptr, nullptr));
+    std::vector<dpd::UnpackTile> tiles;
+    dpd::unpack_tiles(op.fwd, tiles);
+    n_unpack_tiles_ = (long)tiles.size();
+    double* p;
+    ACES_CHECK(alloc(&p, ((long)(tiles.size() * sizeof(dpd::UnpackTile)) + 7) / 8));
+    d_unpack_tiles_ = (dpd::UnpackTile*)p;
+    if (!tiles.empty())
+      ACES_HIP(hipMemcpy(d_unpack_tiles_, tiles.data(), tiles.size() * sizeof(dpd::UnpackTile),
+                         hipMemcpyHostToDevice));
     ops_.push_back(op);
     unpack_op_ = (int)ops_.size() - 1;
     T_[Yb_].grad = true;

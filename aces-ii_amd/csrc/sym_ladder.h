@@ -8,6 +8,7 @@
 // The (k,b) compound follows the row order of an (O,V|..) layout.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <vector>
 #include "dpd.h"
 
 namespace aces {
@@ -33,9 +34,21 @@ int build_ladder_sym(const LadderDesc* d_desc, const LadderDesc& h, const double
 // Sp/Dp from tau (layout tauL, device copy).
 int pack_tau_sym(const LadderDesc* d_desc, const LadderDesc& h, const Layout* d_tauL, const double* tau, double* Sp,
                  double* Dp, hipStream_t s);
-// inc2 += unpacked Z (walk of the inc2 plan); Yb = (k,b) rows (walk of the Yb plan).
-int unpack_ladder_sym(const LadderDesc* d_desc, const Plan& inc2_plan, const Plan& yb_plan, const double* Zp,
-                      const double* Zm, double* inc2, double* Yb, hipStream_t s);
+// Tiles of the inc2 side of the unpack: one per (box of the inc2 plan, value
+// a, range of b, range of the box's occupied pairs in Z's column order).
+struct UnpackTile {
+  int box;
+  short la, lb0, tb, tq;  // a; first b and b extent; pair extent
+  int q0;                 // first pair (column offset inside the box's pair block)
+};
+constexpr int kUnpackLds = 2048;  // doubles per LDS image (Zp and Zm parts each)
+void unpack_tiles(const Plan& inc2_plan, std::vector<UnpackTile>& tiles);
+// inc2 += unpacked Z (tiles of the inc2 plan: Z rows read along their
+// occupied pairs into LDS, inc2 written along b); Yb = (k,b) rows (walk of the
+// Yb plan).
+int unpack_ladder_sym(const LadderDesc* d_desc, const Plan& inc2_plan, const UnpackTile* tiles, long ntiles,
+                      const Plan& yb_plan, const double* Zp, const double* Zm, double* inc2, double* Yb,
+                      hipStream_t s);
 // Adjoints: gZp/gZm from g_inc2 and gYb; gtau += pack adjoint (walk of the tau plan).
 int unpack_ladder_sym_adjoint(const LadderDesc* d_desc, const LadderDesc& h, const Layout* d_inc2L,
                               const Layout* d_ybL, const double* g2, const double* gYb, double* gZp, double* gZm,

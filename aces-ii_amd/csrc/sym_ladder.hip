@@ -217,11 +217,127 @@ __global__ __launch_bounds__(256) void unpack_sym_kernel(const LadderDesc* __res
   }
 }
 
-int unpack_ladder_sym(const LadderDesc* d_desc, const Plan& inc2_plan, const Plan& yb_plan, const double* Zp,
-                      const double* Zm, double* inc2, double* Yb, hipStream_t s) {
-  if (inc2_plan.total > 0)
-    hipLaunchKernelGGL(unpack_sym_kernel, dim3((unsigned)inc2_plan.total), dim3(256), 0, s, d_desc, inc2_plan.dev,
-                       inc2_plan.nbox, 0, Zp, Zm, inc2);
+// Occupied pairs of an inc2 box (irreps gi, gj) in Z's column order: gi < gj
+// the rectangle (li, lj), li slowest; gi > gj the rectangle (lj, li), lj
+// slowest; gi == gj the triangle lp <= lq, lq slowest, each pair standing for
+// the outputs (lp, lq) and (lq, lp).
+__host__ __device__ inline int box_pairs(int gi, int gj, int oi, int oj) {
+  return gi == gj ? oi * (oi + 1) / 2 : oi * oj;
+}
+
+__device__ inline void tri_decode(int q, int& lp, int& lq) {
+  int r = (int)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+  while (r * (r + 1) / 2 > q) --r;
+  while ((r + 1) * (r + 2) / 2 <= q) ++r;
+  lq = r;
+  lp = q - r * (r + 1) / 2;
+}
+
+void unpack_tiles(const Plan& p, std::vector<UnpackTile>& tiles) {
+  tiles.clear();
+  for (int k = 0; k < p.nbox; ++k) {
+    const Box& b = p.host[k];
+    const int nq = box_pairs(b.g[0], b.g[1], b.d[0], b.d[1]);
+    const int vb = b.d[3];
+    const int nbc = (vb + 63) / 64, tb = (vb + nbc - 1) / nbc;
+    const int qmax = std::max(1, kUnpackLds / tb);
+    const int nqc = (nq + qmax - 1) / qmax, tq = (nq + nqc - 1) / nqc;
+    for (int la = 0; la < b.d[2]; ++la)
+      for (int lb0 = 0; lb0 < vb; lb0 += tb)
+        for (int q0 = 0; q0 < nq; q0 += tq) {
+          UnpackTile t;
+          t.box = k;
+          t.la = (short)la;
+          t.lb0 = (short)lb0;
+          t.tb = (short)tb;
+          t.tq = (short)tq;
+          t.q0 = q0;
+          tiles.push_back(t);
+        }
+  }
+}
+
+__global__ __launch_bounds__(256) void unpack_sym_tiled_kernel(const LadderDesc* __restrict__ dp, const Box* boxes,
+                                                               const UnpackTile* __restrict__ tiles,
+                                                               const double* __restrict__ Zp,
+                                                               const double* __restrict__ Zm,
+                                                               double* __restrict__ out) {
+  __shared__ double tp[kUnpackLds], tm[kUnpackLds];
+  const UnpackTile tl = tiles[blockIdx.x];
+  const LadderDesc& d = *dp;
+  const Box& bx = boxes[tl.box];
+  const int gi = bx.g[0], gj = bx.g[1], ga = bx.g[2], gb = bx.g[3];
+  const int oi = bx.d[0], oj = bx.d[1];
+  const int G = ga ^ gb;
+  const int la = tl.la, lb0 = tl.lb0, q0 = tl.q0;
+  const int nq = box_pairs(gi, gj, oi, oj);
+  const int eq = min((int)tl.tq, nq - q0), eb = min((int)tl.tb, bx.d[3] - lb0);
+  const long nOP = d.OP.n[G], nOM = d.OM.n[G];
+  const int glo = gi < gj ? gi : gj;
+  const long pij0 = d.OP.sub[G][glo] + q0, mij0 = d.OM.sub[G][glo] + q0;
+  const long A = d.V.off[ga] + la;
+  // Z rows of this tile (one per b) read along the box's pairs into LDS
+  for (int idx = threadIdx.x; idx < eb * eq; idx += blockDim.x) {
+    const int row = idx / eq, qq = idx - row * eq;
+    const int lb = lb0 + row;
+    int g0 = ga, l0 = la, g1 = gb, l1 = lb;
+    order2(d.V, g0, l0, g1, l1);
+    const long pab = pair_index(d.VP, d.V, g0, l0, g1, l1);
+    tp[idx] = Zp[d.zp_off[G] + pab * nOP + pij0 + qq];
+    double m = 0.0;
+    if (A != d.V.off[gb] + lb) {
+      long mij = -1;
+      if (gi != gj) {
+        mij = mij0 + qq;
+      } else {
+        int lp, lq;
+        tri_decode(q0 + qq, lp, lq);
+        if (lp != lq) mij = d.OM.sub[0][gi] + (long)lq * (lq - 1) / 2 + lp;
+      }
+      if (mij >= 0) m = Zm[d.zm_off[G] + pair_index(d.VM, d.V, g0, l0, g1, l1) * nOM + mij];
+    }
+    tm[idx] = m;
+  }
+  __syncthreads();
+  // inc2 += Zp + sign Zm, written along b (a pair of the triangle writes twice)
+  const int nout = (gi == gj) ? 2 * eq : eq;
+  const long o2 = la * bx.os[2];
+  for (int idx = threadIdx.x; idx < nout * eb; idx += blockDim.x) {
+    const int oq = idx / eb, row = idx - oq * eb;
+    int qq, li, lj;
+    if (gi == gj) {
+      qq = oq >> 1;
+      int lp, lq;
+      tri_decode(q0 + qq, lp, lq);
+      if ((oq & 1) && lp == lq) continue;
+      li = (oq & 1) ? lq : lp;
+      lj = (oq & 1) ? lp : lq;
+    } else {
+      qq = oq;
+      const int q = q0 + qq;
+      if (gi < gj) {
+        li = q / oj;
+        lj = q - li * oj;
+      } else {
+        lj = q / oi;
+        li = q - lj * oi;
+      }
+    }
+    const int lb = lb0 + row;
+    const long I = d.O.off[gi] + li, J = d.O.off[gj] + lj, B = d.V.off[gb] + lb;
+    double z = tp[row * eq + qq];
+    if (I != J && A != B) z += (((I < J) == (A < B)) ? 1.0 : -1.0) * tm[row * eq + qq];
+    out[bx.ob + li * bx.os[0] + lj * bx.os[1] + o2 + lb * bx.os[3]] += z;
+  }
+}
+
+int unpack_ladder_sym(const LadderDesc* d_desc, const Plan& inc2_plan, const UnpackTile* tiles, long ntiles,
+                      const Plan& yb_plan, const double* Zp, const double* Zm, double* inc2, double* Yb,
+                      hipStream_t s) {
+  ACES_REQUIRE(ntiles < (1L << 31), "unpack: too many tiles");
+  if (ntiles > 0)
+    hipLaunchKernelGGL(unpack_sym_tiled_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, d_desc, inc2_plan.dev,
+                       tiles, Zp, Zm, inc2);
   if (yb_plan.total > 0)
     hipLaunchKernelGGL(unpack_sym_kernel, dim3((unsigned)yb_plan.total), dim3(256), 0, s, d_desc, yb_plan.dev,
                        yb_plan.nbox, 1, Zp, Zm, Yb);

@@ -12,6 +12,21 @@ int split_bf16(int rows, int cols, const double* x, long ldx, unsigned short* hi
                long ldy, hipStream_t s, const double* x0 = nullptr);
 // The same conversions for an operand stored k-major, x[c*ldx + r] = element
 // (r, c): the output is the transposed, k-contiguous y[r*ldy + c].
+// The blocks of one irrep-blocked tensor converted in one launch: block g is
+// rows[g] x cols[g] at x[g] (leading dimension cols[g]), written with
+// leading dimension ldy[g] (columns past cols[g] zero) as bf16 hi/lo
+// (split, x - x0 when x0[g] is given) or as fp32 (hi holds the floats).
+struct SplitBlocks {
+  int n = 0;
+  bool f32 = false;
+  int rows[8], cols[8];
+  long ldy[8];
+  const double* x[8];
+  const double* x0[8];
+  void* hi[8];
+  void* lo[8];
+};
+int split_blocks(const SplitBlocks& b, hipStream_t s);
 int convert_f32_t(int rows, int cols, const double* x, long ldx, float* y, long ldy, hipStream_t s);
 int split_bf16_t(int rows, int cols, const double* x, long ldx, unsigned short* hi, unsigned short* lo, long ldy,
                  hipStream_t s, const double* x0 = nullptr);

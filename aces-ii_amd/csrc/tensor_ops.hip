@@ -43,6 +43,73 @@ __global__ void split_bf16_kernel(int rows, int cols, const double* __restrict__
   }
 }
 
+// One block row per tensor block (blockIdx.y); the block's description is
+// picked with constant indices so that the argument block stays in the
+// kernarg segment (a runtime index into it, or a loop of selects, copies it
+// to scratch).
+struct SplitBlock {
+  int rows, cols;
+  long ldy;
+  const double* x;
+  const double* x0;
+  void* hi;
+  void* lo;
+};
+
+__device__ inline SplitBlock split_block_of(const SplitBlocks& b, int g) {
+  switch (g) {
+#define ACES_SPLIT_CASE(k) \
+  case k:                  \
+    return SplitBlock{b.rows[k], b.cols[k], b.ldy[k], b.x[k], b.x0[k], b.hi[k], b.lo[k]};
+    ACES_SPLIT_CASE(1)
+    ACES_SPLIT_CASE(2)
+    ACES_SPLIT_CASE(3)
+    ACES_SPLIT_CASE(4)
+    ACES_SPLIT_CASE(5)
+    ACES_SPLIT_CASE(6)
+    ACES_SPLIT_CASE(7)
+#undef ACES_SPLIT_CASE
+    default:
+      return SplitBlock{b.rows[0], b.cols[0], b.ldy[0], b.x[0], b.x0[0], b.hi[0], b.lo[0]};
+  }
+}
+
+__global__ void split_blocks_kernel(SplitBlocks bs) {
+  const SplitBlock b = split_block_of(bs, (int)blockIdx.y);
+  const long total = (long)b.rows * b.ldy;
+  const long ldy = b.ldy;
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < total; k += (long)gridDim.x * blockDim.x) {
+    const long r = k / ldy, c = k - r * ldy;
+    double v = 0.0;
+    if (c < b.cols) {
+      v = b.x[r * b.cols + c];
+      if (b.x0) v -= b.x0[r * b.cols + c];
+    }
+    if (bs.f32) {
+      static_cast<float*>(b.hi)[k] = (float)v;
+    } else {
+      const unsigned short h = bf16_rn((float)v);
+      static_cast<unsigned short*>(b.hi)[k] = h;
+      static_cast<unsigned short*>(b.lo)[k] = bf16_rn((float)(v - (double)bf16_to_f(h)));
+    }
+  }
+}
+
+int split_blocks(const SplitBlocks& b, hipStream_t s) {
+  ACES_REQUIRE(b.n >= 0 && b.n <= 8, "split_blocks: at most 8 blocks");
+  long most = 0;
+  for (int g = 0; g < b.n; ++g) {
+    ACES_REQUIRE(b.ldy[g] >= b.cols[g] && b.rows[g] >= 0 && b.cols[g] >= 0, "split_blocks: bad block shape");
+    most = std::max(most, (long)b.rows[g] * b.ldy[g]);
+  }
+  if (most == 0) return ACES_OK;
+  // blocks per tensor block: enough for the largest to keep every CU busy
+  const unsigned gx = grid_for(most, 256, std::max<long>(1, 4096 / b.n));
+  hipLaunchKernelGGL(split_blocks_kernel, dim3(gx, (unsigned)b.n), dim3(256), 0, s, b);
+  ACES_HIP(hipGetLastError());
+  return ACES_OK;
+}
+
 int convert_f32(int rows, int cols, const double* x, long ldx, float* y, long ldy, hipStream_t s) {
   ACES_REQUIRE(ldy >= cols && ldx >= cols, "convert_f32: bad leading dimension");
   const long n = (long)rows * ldy;
