@@ -158,10 +158,12 @@ def test_lowp_grouped(prec):
 
 
 @pytest.mark.parametrize('M,N,K,ws', [(1500, 300, 512, False), (4100, 4200, 96, False), (1100, 2100, 1024, True),
-                                      (1030, 1040, 2080, True)])
+                                      (1030, 1040, 2080, True), (1100, 1176, 1024, True), (1100, 1200, 1024, True),
+                                      (1100, 1230, 1024, True), (16400, 1128, 96, False)])
 def test_bf16x3_block_shapes(M, N, K, ws):
     """The bf16x3 block shapes (128x128, 256x128, 256x256; direct-to-LDS ring when
-    K % 32 == 0) on ragged M/N, with and without split-K."""
+    K % 32 == 0) on ragged M/N, with and without split-K; last column tiles of
+    256x256 blocks with 104, 152, 176 and 206 valid columns."""
     from aces2 import _lib
     print('case bf16x3 shape', M, N, K, ws, flush=True)
     g = torch.Generator(device='cuda').manual_seed(M + N + K)
