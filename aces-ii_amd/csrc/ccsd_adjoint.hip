@@ -388,15 +388,22 @@ static int lambda_to_aces(int o, int v, const double* lam, double* L, hipStream_
 int CcsdEngine::lambda_start() {
   ACES_REQUIRE(ready_, "ccsd: not set up");
   ACES_CHECK(adjoint_alloc());
+  const bool lists = tn_lists_;
   ACES_CHECK(compute_increments(GP_F64));
   const long ov = (long)o * v;
-  // Lambda start: the Jacobi update of the converged cycle, Lambda1 = T1_new and
-  // Lambda2 = SYMETRIZE_T2(T2_new).  xvcc leaves that update on the increment
-  // lists (61/63, 90 group 3) and moves it to the Lambda lists 144/146/190 with
-  // INIT2 + DRMOVE(...,100,SING1) after convergence (vcc/vcc.F:1130-1143); xlambda
-  // starts from those lists.
-  ACES_CHECK(jacobi(o, v, eo_, ev_, inc1_, inc2_, Lam_, Lam_ + ov, s_));
-  ACES_CHECK(symmetrize_t2(o, v, Lam_ + ov, s_));
+  // Lambda start: what xvcc leaves on the increment lists (61/63, 90 group 3)
+  // after the converged cycle and moves to the Lambda lists 144/146/190 with
+  // INIT2 + DRMOVE(...,100,SING1) (vcc/vcc.F:1130-1143): the converged cycle's
+  // DIIS extrapolant (vcc/dodiis0.F:181-233), kept in t1n_/t2n_ by diis_step.
+  // Without a converged cycle at these amplitudes: their Jacobi update,
+  // Lambda1 = T1_new and Lambda2 = SYMETRIZE_T2(T2_new) (the lists' content
+  // when the run converges before the subspace holds two vectors).
+  if (lists) {
+    ACES_HIP(hipMemcpyAsync(Lam_, t1n_, sizeof(double) * (ov + ov * ov), hipMemcpyDeviceToDevice, s_));
+  } else {
+    ACES_CHECK(jacobi(o, v, eo_, ev_, inc1_, inc2_, Lam_, Lam_ + ov, s_));
+    ACES_CHECK(symmetrize_t2(o, v, Lam_ + ov, s_));
+  }
   ldiis_.R.assign((size_t)ldiis_.order * ldiis_.order, 0.0);
   lam_anc_valid_ = false;
   ACES_HIP(hipStreamSynchronize(s_));

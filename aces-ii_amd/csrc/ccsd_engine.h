@@ -52,9 +52,12 @@ class CcsdEngine {
   // Pieces of one cycle, used by the list-level driver entry points.
   int jacobi_step();
   int diis_step(int icycle, double tol, CcsdStats& st, bool xn_sym = false);
-  // Writable views; writing through T() invalidates the resident forward intermediates.
-  double* T() { fwd_valid_ = false; return t1_; }  // [t1 | t2] current amplitudes
-  double* Tnew() { return t1n_; }    // [t1 | t2] Jacobi update
+  // Writable views; writing through T() invalidates the resident forward
+  // intermediates, writing through either the Lambda start kept in Tnew.
+  double* T() { fwd_valid_ = false; tn_lists_ = false; return t1_; }  // [t1 | t2] current amplitudes
+  double* Tnew() { tn_lists_ = false; return t1n_; }  // [t1 | t2] Jacobi update
+  const double* T_ro() const { return t1_; }
+  const double* Tnew_ro() const { return t1n_; }
   double* Inc() { return inc1_; }    // [inc1 | inc2]
   int energy_dev(const double* t1, const double* t2, double* d_out);
   int reset_diis();
@@ -219,6 +222,11 @@ class CcsdEngine {
   double *red_ = nullptr;  // reduction scratch
   double* jpart_ = nullptr;  // energy partials of the fused Jacobi pass
   bool t_sym_ = false;       // t2_ is known to be exactly pair-symmetric (set by DIIS)
+  // t1n_/t2n_ hold what the reference leaves on the increment lists after a
+  // converged cycle at the current amplitudes (the DIIS extrapolant, or the
+  // symmetrized Jacobi update before the subspace has two vectors): the
+  // Lambda start (set by diis_step on convergence, cleared by any change)
+  bool tn_lists_ = false;
   double *ws_ = nullptr;   // split-K workspace
   long ws_elems_ = 0;
   double *scal_ = nullptr; // device scalars

@@ -221,6 +221,7 @@ int CcsdEngine::set_amplitudes(const double* t1, const double* t2) {
   ACES_REQUIRE(ready_, "ccsd: not set up");
   const long ov = (long)o * v;
   t_sym_ = false;
+  tn_lists_ = false;
   ACES_HIP(hipMemcpyAsync(t1_, t1, sizeof(double) * ov, hipMemcpyDeviceToDevice, s_));
   ACES_HIP(hipMemcpyAsync(t2_, t2, sizeof(double) * ov * ov, hipMemcpyDeviceToDevice, s_));
   ACES_HIP(hipStreamSynchronize(s_));
@@ -244,6 +245,7 @@ int CcsdEngine::start_mp1() {
   ACES_HIP(hipMemsetAsync(inc1_, 0, sizeof(double) * ov, s_));
   ACES_CHECK(P4(o, o, v, v, inc2_).src(ovov_, 1, ov2, v, ov, 1).run(s_));
   t_sym_ = false;
+  tn_lists_ = false;
   ACES_CHECK(jacobi(o, v, eo_, ev_, inc1_, inc2_, t1_, t2_, s_));
   ACES_HIP(hipStreamSynchronize(s_));
   fwd_valid_ = false;
@@ -847,6 +849,7 @@ int CcsdEngine::cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st) {
   ACES_HIP(hipEventCreate(&e1));
   ACES_HIP(hipEventCreate(&e2));
   ACES_HIP(hipEventRecord(e0, s_));
+  tn_lists_ = false;
   ACES_CHECK(compute_increments(prec));
   ACES_HIP(hipEventRecord(e1, s_));
   // Jacobi update, its energy (CMPENG before SYMETRIZE) and SYMETRIZE_T2 of
@@ -867,13 +870,18 @@ int CcsdEngine::cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st) {
 }
 
 // NEWT2 / E4S: T_new = increment / D.
-int CcsdEngine::jacobi_step() { return jacobi(o, v, eo_, ev_, inc1_, inc2_, t1n_, t2n_, s_); }
+int CcsdEngine::jacobi_step() {
+  tn_lists_ = false;
+  return jacobi(o, v, eo_, ev_, inc1_, inc2_, t1n_, t2n_, s_);
+}
 
 // DODIIS0 (vcc/dodiis0.F) on T (t1_/t2_) and T_new (t1n_/t2n_); scal_[0] holds
 // the energy of T_new computed by the caller (CMPENG(60,2)).  On return t1_/t2_
-// hold the next amplitudes unless converged.
+// hold the next amplitudes unless converged; converged, t1n_/t2n_ hold the
+// increment lists' content, the Lambda start.
 int CcsdEngine::diis_step(int icycle, double tol, CcsdStats& st, bool xn_sym) {
   ACES_CHECK(diis_core(tdiis_, t1_, t1n_, icycle, tol, false, st, t_sym_, xn_sym));
+  tn_lists_ = st.converged != 0;
   // refinement anchor: remember the anchor cycle's residual; move the anchor
   // once the residual has fallen 1e3-fold below it
   if (anc_last_ == 1)
@@ -966,7 +974,13 @@ int CcsdEngine::diis_core(DiisState& D, double* x, double* xn, int icycle, doubl
     st.converged = (std::fabs(st.resmiter) < tol || std::fabs(st.resmdiis) < tol) ? 1 : 0;
   else
     st.converged = std::fabs(st.resmiter) < tol ? 1 : 0;
-  if (st.converged) return ACES_OK;
+  // Both DODIIS0s extrapolate in the converged cycle too and write the result
+  // over the increment lists (vcc/dodiis0.F:181-233, lambda/dodiis0.F:131-170).
+  // Converged amplitudes are not moved (vcc/vcc.F:1077-1087): the extrapolant
+  // goes to xn, from where xvcc hands it to xlambda as the Lambda start
+  // (DRMOVE(...,100), vcc/vcc.F:1130-1143).  xlambda moves the increment lists
+  // to the multipliers unconditionally (lambda/vlamcc.F:811-815): x.
+  double* dst = (st.converged && !lambda_rule) ? xn : x;
   if (ndimr >= 2) {
     std::vector<const double*> sv;
     std::vector<double> sc;
@@ -978,8 +992,8 @@ int CcsdEngine::diis_core(DiisState& D, double* x, double* xn, int icycle, doubl
     }
     ACES_HIP(hipMemcpyAsync(d_ptrs_, sv.data(), sizeof(double*) * sv.size(), hipMemcpyHostToDevice, s_));
     ACES_HIP(hipMemcpyAsync(d_coef_, sc.data(), sizeof(double) * sc.size(), hipMemcpyHostToDevice, s_));
-    ACES_CHECK(lincomb(nt, (int)sv.size(), d_ptrs_, d_coef_, x, s_));
-  } else {
+    ACES_CHECK(lincomb(nt, (int)sv.size(), d_ptrs_, d_coef_, dst, s_));
+  } else if (dst != xn) {
     ACES_HIP(hipMemcpyAsync(x, xn, sizeof(double) * nt, hipMemcpyDeviceToDevice, s_));
   }
   return ACES_OK;

@@ -38,10 +38,14 @@ class SymEngine {
   // DODIIS0 of the list driver (vcc/dodiis0.F) on T and T_new given as dense
   // arrays: st.energy_new = E(T_new); unless converged the next amplitudes
   // (the extrapolated vector) are the engine's current ones (get_amplitudes),
-  // converged: the current ones are T, pair-symmetrized, and st.energy_final E(T).
+  // converged: the current ones are T, pair-symmetrized, st.energy_final E(T),
+  // and get_increment_lists returns the extrapolant DODIIS0 writes over the
+  // increment lists in that cycle too (the Lambda start).
   int diis_dense(int icycle, double tol, const double* t1, const double* t2, const double* t1n, const double* t2n,
                  CcsdStats& st);
   int energy_current(double& e);
+  // The content of the increment lists after the last DODIIS0 (dense).
+  int get_increment_lists(double* t1, double* t2);
   int set_diis_order(int order);
   int reset_diis();
   // prec kPrecRefine (3) as in CcsdEngine: the large products (cat <= 2) as
@@ -175,6 +179,9 @@ class SymEngine {
   // Lambda state
   double *Lam_ = nullptr, *Lamn_ = nullptr, *lam_ = nullptr, *gT_ = nullptr;
   bool fwd_valid_ = false, lam_ready_ = false;
+  // T_new_ holds the increment lists' content after a converged cycle at the
+  // current amplitudes (the Lambda start; set on convergence, cleared by any change)
+  bool tn_lists_ = false;
   // refinement anchor (forward(kPrecRefine)); per op: C0 = A0 B0 and, when both
   // operands are amplitude-dependent, the split of B0; per tensor: the anchor
   // copy and the split of the difference to it

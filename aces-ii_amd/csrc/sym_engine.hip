@@ -1372,6 +1372,22 @@ int SymEngine::set_amplitudes(const double* t1, const double* t2) {
   ACES_CHECK(pack_dense(t1_, t1, s1, 0));
   ACES_CHECK(pack_dense(t2_, t2, s2, 0));
   fwd_valid_ = false;
+  tn_lists_ = false;
+  return ACES_OK;
+}
+
+int SymEngine::get_increment_lists(double* t1, double* t2) {
+  ACES_REQUIRE(ready_, "sym: not set up");
+  const long s1[2] = {v, 1};
+  const long s2[4] = {(long)o * v * v, (long)v * v, v, 1};
+  if (t1) {
+    ACES_HIP(hipMemsetAsync(t1, 0, sizeof(double) * o * v, s_));
+    ACES_CHECK(unpack_dense(t1_, T_new_, t1, s1, 0));
+  }
+  if (t2) {
+    ACES_HIP(hipMemsetAsync(t2, 0, sizeof(double) * (long)o * o * v * v, s_));
+    ACES_CHECK(unpack_dense(t2_, T_new_ + n1_, t2, s2, 0));
+  }
   return ACES_OK;
 }
 
@@ -1393,6 +1409,7 @@ int SymEngine::get_amplitudes(double* t1, double* t2) {
 // MP1 (intprc/dfrmt2.F): T2 = <ij|ab>/D, T1 = 0.
 int SymEngine::start_mp1() {
   ACES_REQUIRE(ready_, "sym: not set up");
+  tn_lists_ = false;
   ACES_HIP(hipMemsetAsync(T_cur_, 0, sizeof(double) * n1_, s_));
   // <ij|ab> = (ia|jb) = (O,O|V,V) gather of ovov; Lijab's source is ovov, so use
   // the first recorded inc2 permutation's source 0 (ovov, map 0 2 1 3).
@@ -1526,7 +1543,12 @@ int SymEngine::diis_core(std::vector<double*>& sol, std::vector<double*>& err, s
     st.converged = (std::fabs(st.resmiter) < tol || std::fabs(st.resmdiis) < tol) ? 1 : 0;
   else
     st.converged = std::fabs(st.resmiter) < tol ? 1 : 0;
-  if (st.converged) return ACES_OK;
+  // Both DODIIS0s extrapolate in the converged cycle too (vcc/dodiis0.F:181-233,
+  // lambda/dodiis0.F:131-170); converged amplitudes stay (vcc/vcc.F:1077-1087)
+  // and the extrapolant goes to xn, the increment lists xvcc hands to xlambda
+  // (vcc/vcc.F:1130-1143); multipliers take it unconditionally
+  // (lambda/vlamcc.F:811-815).
+  double* dst = (st.converged && !lambda_rule) ? xn : x;
   if (ndimr >= 2) {
     std::vector<const double*> sv;
     std::vector<double> sc;
@@ -1538,8 +1560,8 @@ int SymEngine::diis_core(std::vector<double*>& sol, std::vector<double*>& err, s
     }
     ACES_HIP(hipMemcpyAsync(dptr_, sv.data(), sizeof(double*) * sv.size(), hipMemcpyHostToDevice, s));
     ACES_HIP(hipMemcpyAsync(coef_, sc.data(), sizeof(double) * sc.size(), hipMemcpyHostToDevice, s));
-    ACES_CHECK(lincomb(nt_, (int)sv.size(), dptr_, coef_, x, s));
-  } else {
+    ACES_CHECK(lincomb(nt_, (int)sv.size(), dptr_, coef_, dst, s));
+  } else if (dst != xn) {
     ACES_HIP(hipMemcpyAsync(x, xn, sizeof(double) * nt_, hipMemcpyDeviceToDevice, s));
   }
   return ACES_OK;
@@ -1553,6 +1575,7 @@ int SymEngine::cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st) {
   ACES_HIP(hipEventCreate(&e1));
   ACES_HIP(hipEventCreate(&e2));
   ACES_HIP(hipEventRecord(e0, s_));
+  tn_lists_ = false;
   ACES_CHECK(forward(prec));
   ACES_HIP(hipEventRecord(e1, s_));
   // NEWT2 / E4S
@@ -1562,6 +1585,7 @@ int SymEngine::cycle(int icycle, GemmPrec prec, double tol, CcsdStats& st) {
   ACES_CHECK(energy_dev(T_new_, T_new_ + n1_, enew));
   st.energy_new = enew;
   ACES_CHECK(diis_core(sol_, errv_, R_, order_, T_cur_, T_new_, icycle, tol, false, st));
+  tn_lists_ = st.converged != 0;
   // refinement anchor: moved once the residual has fallen 1e3-fold below the anchor cycle's
   if (anc_last_ == 1)
     anc_res_ = std::fabs(st.resmiter);
@@ -1596,6 +1620,7 @@ int SymEngine::diis_dense(int icycle, double tol, const double* t1, const double
   const long s2[4] = {(long)o * v * v, (long)v * v, v, 1};
   ACES_CHECK(pack_dense(t1_, t1, s1, 0));
   ACES_CHECK(pack_dense(t2_, t2, s2, 0));
+  tn_lists_ = false;
   // T_new through the t1/t2 tensors' layouts into T_new_
   {
     double* keep1 = T_[t1_].d;
@@ -1612,6 +1637,7 @@ int SymEngine::diis_dense(int icycle, double tol, const double* t1, const double
   ACES_CHECK(energy_dev(T_new_, T_new_ + n1_, enew));
   st.energy_new = enew;
   ACES_CHECK(diis_core(sol_, errv_, R_, order_, T_cur_, T_new_, icycle, tol, false, st));
+  tn_lists_ = st.converged != 0;
   if (anc_last_ == 1)
     anc_res_ = std::fabs(st.resmiter);
   else if (anc_last_ == 2 && std::fabs(st.resmiter) < 1e-3 * anc_res_)
@@ -1688,13 +1714,20 @@ int SymEngine::lambda_start() {
   }
   ACES_CHECK(diis_grow(lsol_, lerr_, lorder_));
   LR_.assign((size_t)lorder_ * lorder_, 0.0);
+  const bool lists = tn_lists_;
   ACES_CHECK(forward(GP_F64));
-  // Lambda start: the Jacobi update of the converged cycle (T1_new, SYMETRIZE_T2
-  // of T2_new), which xvcc moves to the Lambda lists with DRMOVE(...,100,SING1)
-  // after convergence (vcc/vcc.F:1130-1143)
-  ACES_CHECK(denom_apply(t1_, Lam_, 0.0, T_[inc1_].d, 1.0, nullptr, 0.0, 0, -1));
-  ACES_CHECK(denom_apply(t2_, Lam_ + n1_, 0.0, T_[inc2_].d, 1.0, nullptr, 0.0, 0, -1));
-  ACES_CHECK(pair_symmetrize(Lam_ + n1_));
+  // Lambda start: the increment lists xvcc moves to the Lambda lists with
+  // DRMOVE(...,100,SING1) after convergence (vcc/vcc.F:1130-1143), i.e. the
+  // converged cycle's DIIS extrapolant kept in T_new_ (vcc/dodiis0.F:181-233);
+  // without a converged cycle at these amplitudes their Jacobi update (T1_new,
+  // SYMETRIZE_T2 of T2_new)
+  if (lists) {
+    ACES_HIP(hipMemcpyAsync(Lam_, T_new_, sizeof(double) * nt_, hipMemcpyDeviceToDevice, s_));
+  } else {
+    ACES_CHECK(denom_apply(t1_, Lam_, 0.0, T_[inc1_].d, 1.0, nullptr, 0.0, 0, -1));
+    ACES_CHECK(denom_apply(t2_, Lam_ + n1_, 0.0, T_[inc2_].d, 1.0, nullptr, 0.0, 0, -1));
+    ACES_CHECK(pair_symmetrize(Lam_ + n1_));
+  }
   ACES_HIP(hipStreamSynchronize(s_));
   lam_ready_ = true;
   return ACES_OK;

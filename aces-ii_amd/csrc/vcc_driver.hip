@@ -780,7 +780,12 @@ int aces_dodiis0(int icycle, double tol, int* iconvg, double* stats) {
       ACES_CHECK(dense_to_t1(*c->lists, 3, 90, B.x1, s));
       ACES_CHECK(sym_put_t2(c, 63, 61, B.x2));
     } else {
+      // T stays (symmetrised in place); the extrapolant still goes over the
+      // increment lists (vcc/dodiis0.F:181-233), the Lambda start DRMOVE(100) moves
       ACES_CHECK(sym_put_t2(c, 46, -1, B.x2));
+      ACES_CHECK(c->seng->get_increment_lists(B.x1, B.x2));
+      ACES_CHECK(dense_to_t1(*c->lists, 3, 90, B.x1, s));
+      ACES_CHECK(sym_put_t2(c, 63, 61, B.x2));
     }
   } else {
     CcsdEngine& e = *c->eng;
@@ -794,15 +799,17 @@ int aces_dodiis0(int icycle, double tol, int* iconvg, double* stats) {
                             hipMemcpyDeviceToDevice, s));
     ACES_CHECK(e.energy_dev(e.Tnew(), e.Tnew() + ov, nullptr));
     ACES_CHECK(e.diis_step(icycle, tol, st));
-    if (!st.converged) {
-      // extrapolated (or Jacobi) vector -> increment lists (vcc/dodiis0.F "WRITE IT OVER CURRENT INCREMENT LISTS")
-      ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(3, 90), e.T(), sizeof(double) * ov, hipMemcpyDeviceToDevice, s));
-      ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 63), e.T() + ov, sizeof(double) * ov * ov,
-                              hipMemcpyDeviceToDevice, s));
-      ACES_CHECK(abtoaa(e.o, e.v, c->lists->device_ptr(1, 63), c->lists->device_ptr(1, 61), s));
-    } else {
+    // extrapolated (or Jacobi) vector -> increment lists (vcc/dodiis0.F "WRITE IT
+    // OVER CURRENT INCREMENT LISTS", in the converged cycle too: there it is the
+    // Lambda start that DRMOVE(...,100) moves, and the engine keeps it in Tnew)
+    const double* nx = st.converged ? e.Tnew_ro() : e.T_ro();
+    ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(3, 90), nx, sizeof(double) * ov, hipMemcpyDeviceToDevice, s));
+    ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 63), nx + ov, sizeof(double) * ov * ov,
+                            hipMemcpyDeviceToDevice, s));
+    ACES_CHECK(abtoaa(e.o, e.v, c->lists->device_ptr(1, 63), c->lists->device_ptr(1, 61), s));
+    if (st.converged) {
       // DODIIS0 symmetrised list 46 in place (libr2/symetrize_t2.F)
-      ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 46), e.T() + ov, sizeof(double) * ov * ov,
+      ACES_HIP(hipMemcpyAsync(c->lists->device_ptr(1, 46), e.T_ro() + ov, sizeof(double) * ov * ov,
                               hipMemcpyDeviceToDevice, s));
     }
   }

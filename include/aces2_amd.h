@@ -165,10 +165,13 @@ int aces_ccsd_eom(void* handle, const int* orb_irrep, int irrep, int nroots, dou
  *                            (r2 symmetrised) and, if d_dens != NULL, the
  *                            response density dL/dfock (n x n, symmetric).
  *   aces_ccsd_lambda_start   forward intermediates at the current amplitudes;
- *                            start L1 = T1_new, L2 = T2_new, the Jacobi update
- *                            at those amplitudes (what xvcc moves to lists
- *                            190/146 with DRMOVE(...,100,SING1) after
- *                            convergence, vcc/vcc.F:1130-1143)
+ *                            start (L1, L2) = what xvcc leaves on the increment
+ *                            lists and moves to lists 190/146 with
+ *                            DRMOVE(...,100,SING1) after convergence
+ *                            (vcc/vcc.F:1130-1143): the converged cycle's DIIS
+ *                            extrapolant (vcc/dodiis0.F:181-233) when the last
+ *                            cycle at these amplitudes converged, else their
+ *                            Jacobi update T1_new, SYMETRIZE_T2(T2_new)
  *   aces_ccsd_lambda_cycle   one Jacobi step + Lambda DODIIS0 (converged when
  *                            |RESMITER| or |RESMDIIS| < tol); stats as for
  *                            aces_ccsd_cycle (stats[0], stats[4] unused)

@@ -168,35 +168,43 @@ def from_aces(L1, L2):
 
 
 def lambda_start(f, g, o, t1, t2):
-    """xlambda's starting multipliers in the list normalisation (L1, L2).
+    """The Jacobi update at (t1, t2) in the list normalisation (L1, L2).
 
     After convergence xvcc creates the Lambda lists (INIT2, ``vcc/init2.F``) and
     moves the increment lists into them with DRMOVE(...,100,SING1)
-    (``vcc/vcc.F:1130-1143``, ``libr/drmove.f``): the converged cycle's Jacobi
-    update, T1_new = inc1/D1 (list 90 group 3 -> 190) and T2_new = inc2/D2
-    after DODIIS0's SYMETRIZE_T2 (list 63 -> 146), at the converged T.
+    (``vcc/vcc.F:1130-1143``, ``libr/drmove.f``).  Those lists hold the
+    converged cycle's DIIS extrapolant (``vcc/dodiis0.F:181-233``), which
+    ``vcc_loop.run`` records as ``lambda_start``; they hold this Jacobi update,
+    T1_new = inc1/D1 and SYMETRIZE_T2(inc2/D2), only when the run converged in
+    its first cycle (no extrapolation yet).
     """
     i1, i2 = rccsd.increments(t1, t2, f, g, o)
     d1, d2 = rccsd.denominators(f, o)
     return i1 / d1, symmetrize_pair(i2 / d2)
 
 
-def run_reference(f, g, o, t1, t2, maxcyc=50, conv=7, order=5, log=None):
+def run_reference(f, g, o, t1, t2, maxcyc=50, conv=7, order=5, log=None, start=None):
     """xlambda's CCSD Lambda iteration (``lambda/vlamcc.F:599-830``), restated.
 
-    Start: ``lambda_start`` (the converged CC cycle's Jacobi update moved to
-    the Lambda lists; a zero Lambda1 instead puts cycles 2.. of the reference's
-    printed trace off by up to 5 %).  Each cycle forms the increments,
+    Start: ``start`` = (L1, L2), what xvcc leaves on the increment lists and
+    moves to the Lambda lists (the converged cycle's DIIS extrapolant,
+    ``vcc_loop.run(...)['lambda_start']``); None takes the Jacobi update at
+    (t1, t2) (``lambda_start``).  With the Jacobi update instead of the
+    extrapolant the printed trace is off by up to 1e-4 relative, with a zero
+    Lambda1 by up to 5 %.  Each cycle forms the increments,
     divides by the denominators (NEWT2) and calls the Lambda DODIIS0
     (``lambda/dodiis0.F``): same DIIS vector and subspace as the amplitude
     equations, but converged when |RESMITER| or |RESMDIIS| is below
-    10**-conv.  The Jacobi step in the reference's normalisation equals
+    10**-conv.  DODIIS0 extrapolates in the converged cycle too and xlambda
+    moves the result to the Lambda lists unconditionally (DRMOVE after
+    DODIIS0, ``lambda/vlamcc.F:811-815``, ``lambda/dodiis0.F:131-170``), so the
+    multipliers returned are that extrapolant.  The Jacobi step in the reference's normalisation equals
     l <- l + (dL/dT)/D in the spatial one (the map ``to_aces`` is linear and
     commutes with the denominators), so the increments are the Lagrangian
     gradient; DIIS runs on the reference's vectors.
     """
     d1, d2 = rccsd.denominators(f, o)
-    L1, L2 = lambda_start(f, g, o, t1, t2)
+    L1, L2 = lambda_start(f, g, o, t1, t2) if start is None else (start[0].copy(), start[1].copy())
     diis = _vl.DIIS(order)
     tol = 10.0 ** (-conv)
     hist = {'resmiter': [], 'resmdiis': [], 'converged': False}
@@ -214,14 +222,15 @@ def run_reference(f, g, o, t1, t2, maxcyc=50, conv=7, order=5, log=None):
         hist['resmdiis'].append(resmdiis)
         if log:
             log(icycle, resmiter, resmdiis)
-        if abs(resmiter) < tol or abs(resmdiis) < tol:
-            hist['converged'] = True
-            hist['cycles'] = icycle
-            break
+        done = abs(resmiter) < tol or abs(resmdiis) < tol
         if newvec is not None:
             L1, L2 = _vl.from_diis_vector(newvec, o, t1.shape[1])
         else:
             L1, L2 = N1, N2
+        if done:
+            hist['converged'] = True
+            hist['cycles'] = icycle
+            break
     hist['L1'], hist['L2'] = L1, L2
     hist['l1'], hist['l2'] = from_aces(L1, L2)
     return hist

@@ -167,6 +167,13 @@ def run(f, g, o, maxcyc=50, conv=7, order=6, start='mp1', increments=None, energ
             hist['final_energy'] = e_fn(t1, t2)
             hist['energy_aa_ab'].append(parts(t1, t2))
             hist['cycles'] = icycle
+            # DODIIS0 extrapolates in the converged cycle too and writes the
+            # extrapolant over the increment lists (``vcc/dodiis0.F:181-233``,
+            # "SKIP ONLY IN THE FIRST ITERATION"); the amplitudes are not moved
+            # (``vcc/vcc.F:1077-1087``).  That content of the increment lists is
+            # what DRMOVE(...,100,SING1) hands to xlambda (``vcc/vcc.F:1130-1143``).
+            hist['lambda_start'] = (from_diis_vector(newvec, o, v) if newvec is not None
+                                    else (t1n.copy(), t2n.copy()))
             break
         if newvec is not None:
             t1, t2 = from_diis_vector(newvec, o, v)

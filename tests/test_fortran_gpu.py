@@ -91,7 +91,8 @@ def test_fortran_host_nonconvergence_exit(tmp_path):
 @pytest.mark.gpu
 def test_fortran_host_drmove_lambda_lists(tmp_path):
     """DRMOVE(...,100,SING1) after convergence fills 144/146/190 from the increment
-    lists and leaves 44/46/90 as they were (the converged T)."""
+    lists (the converged cycle's DIIS extrapolant) and leaves 44/46/90 as they
+    were (the converged T)."""
     assert os.path.exists(EXE), 'Fortran driver not built (run __graft_entry__.build())'
     print('case fortran drmove 100', flush=True)
     inp, blk, eri, o, v = _small_input(tmp_path)
@@ -104,10 +105,14 @@ def test_fortran_host_drmove_lambda_lists(tmp_path):
     assert abs(norms[46] - np.sum(t2 * t2)) < 1e-9 * np.sum(t2 * t2)
     assert abs(norms[44] - 0.25 * np.sum(aa * aa)) < 1e-9 * np.sum(aa * aa)
     assert abs(norms[90] - np.sum(t1 * t1)) <= 1e-9 * max(np.sum(t1 * t1), 1e-30)
-    # the last increment lists hold the converged cycle's update: close to T, not equal
-    assert abs(norms[146] - norms[46]) < 1e-6 * norms[46]
-    assert abs(norms[144] - norms[44]) < 1e-6 * norms[44]
-    assert abs(norms[190] - norms[90]) <= 1e-6 * max(norms[90], 1e-30) + 1e-20
+    # the increment lists hold the converged cycle's DIIS extrapolant
+    # (vcc/dodiis0.F extrapolates in that cycle too), which DRMOVE(100) moves
+    L1, L2 = ref['lambda_start']
+    laa = L2 - L2.transpose(1, 0, 2, 3)
+    assert abs(norms[146] - np.sum(L2 * L2)) < 1e-9 * np.sum(L2 * L2)
+    assert abs(norms[144] - 0.25 * np.sum(laa * laa)) < 1e-9 * np.sum(laa * laa)
+    assert abs(norms[190] - np.sum(L1 * L1)) <= 1e-9 * max(np.sum(L1 * L1), 1e-30) + 1e-20
+    assert abs(norms[146] - norms[46]) > 0
 
 
 @pytest.mark.gpu

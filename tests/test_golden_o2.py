@@ -202,19 +202,23 @@ def _lambda_inputs(first_geometry):
     return g, F, eri, blk, o, h
 
 
-# The printed Lambda residuals are reproduced to a few 1e-6 relative (the CC
-# trace to 3e-7): the Lambda residual is first order in the orbitals, whose
-# remaining difference to the reference's SCF shows there first.
-LAMBDA_RTOL = 5e-6
+# The printed Lambda residuals are reproduced as tightly as the CC trace (3e-7
+# relative; measured <= 4e-8) from the start xvcc hands over: the converged
+# cycle's DIIS extrapolant, which DODIIS0 writes over the increment lists in
+# that cycle too (vcc/dodiis0.F:181-233).  From the raw Jacobi update instead
+# the trace is off by up to 3.5e-6.
+LAMBDA_RTOL = 3e-7
 
 
 def test_oracle_lambda_trace_against_reference(first_geometry):
     """xlambda of out.zmat.015a: every printed RESMITER and RESMDIIS of the 13
-    cycles, from the start xvcc hands over (the converged cycle's Jacobi update,
-    L1 included: with L1 = 0 instead cycles 2.. are off by up to 5 %)."""
+    cycles, from the start xvcc hands over (the converged cycle's DIIS
+    extrapolant, L1 included: with L1 = 0 instead cycles 2.. are off by up to
+    5 %)."""
     from oracle import lambda_
     g, F, eri, blk, o, h = _lambda_inputs(first_geometry)
-    lh = lambda_.run_reference(F, blk, o, h['t1'], h['t2'], conv=g['cc_conv'], order=g['cc_exporder'])
+    lh = lambda_.run_reference(F, blk, o, h['t1'], h['t2'], conv=g['cc_conv'], order=g['cc_exporder'],
+                               start=h['lambda_start'])
     assert lh['cycles'] == g['lambda_cycles']
     np.testing.assert_allclose(lh['resmiter'], g['lambda_resmiter'], rtol=LAMBDA_RTOL, atol=0)
     np.testing.assert_allclose(lh['resmdiis'], g['lambda_resmdiis'], rtol=LAMBDA_RTOL, atol=0)
@@ -240,7 +244,8 @@ def test_gpu_lambda_and_density_against_reference(first_geometry):
     from oracle import lambda_
     print('case golden O2 lambda gpu', flush=True)
     g, F, eri, blk, o, h = _lambda_inputs(first_geometry)
-    ref = lambda_.run_reference(F, blk, o, h['t1'], h['t2'], conv=g['cc_conv'], order=g['cc_exporder'])
+    ref = lambda_.run_reference(F, blk, o, h['t1'], h['t2'], conv=g['cc_conv'], order=g['cc_exporder'],
+                                start=h['lambda_start'])
     b = dict(blk)
     b['fock'] = F
     b['list233'] = synth.list233_from_eri(eri, o)
