@@ -70,6 +70,7 @@ SIGNATURES = {
     'aces_sym_energy': (c_int, [c_vp, c_dp]),
     'aces_sym_adjoint': (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     'aces_sym_lambda_start': (c_int, [c_vp]),
+    'aces_sym_set_lambda_precision': (c_int, [c_vp, c_int]),
     'aces_sym_lambda_cycle': (c_int, [c_vp, c_int, c_double, c_dp]),
     'aces_sym_set_lambda_diis_order': (c_int, [c_vp, c_int]),
     'aces_sym_get_lambda': (c_int, [c_vp, c_vp, c_vp, c_int]),

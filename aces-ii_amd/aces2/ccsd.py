@@ -231,8 +231,8 @@ class CCSD:
         _lib.call(self._p + 'lambda_start', self.h)
 
     def set_lambda_precision(self, prec):
-        """'fp64' or 'refine' Lambda cycles (C1 engine)."""
-        _lib.call('aces_ccsd_set_lambda_precision', self.h, PREC[prec])
+        """'fp64' or 'refine' (mixed) Lambda cycles."""
+        _lib.call(self._p + 'set_lambda_precision', self.h, PREC[prec])
 
     def set_lambda_diis_order(self, order):
         _lib.call(self._p + 'set_lambda_diis_order', self.h, order)
@@ -266,12 +266,11 @@ class CCSD:
 
     def solve_lambda(self, maxcyc=50, conv=7, order=6, log=None, raise_on_fail=True, prec='fp64'):
         """xlambda's loop (``lambda/vlamcc.F:599-830``) at the engine's amplitudes.
-        ``prec='refine'`` (C1 engine): residuals from an fp64 anchor plus the
-        linear part on the difference with its large products in bf16x3."""
+        ``prec='refine'``: residuals from an fp64 anchor plus the linear part
+        on the difference with its large products in bf16x3."""
         self.lambda_start()
         self.set_lambda_diis_order(order)
-        if prec != 'fp64' or self._p == 'aces_ccsd_':
-            self.set_lambda_precision(prec)
+        self.set_lambda_precision(prec)
         self.lambda_start()
         hist = {'resmiter': [], 'resmdiis': [], 'ms_cycle': [], 'converged': False}
         tol = 10.0 ** (-conv)

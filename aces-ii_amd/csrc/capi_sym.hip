@@ -112,6 +112,11 @@ int aces_sym_lambda_start(void* h) {
   return sym(h)->lambda_start();
 }
 
+int aces_sym_set_lambda_precision(void* h, int prec) {
+  REQUIRE_SYM(h);
+  return sym(h)->set_lambda_precision(prec);
+}
+
 int aces_sym_lambda_cycle(void* h, int icycle, double tol, double* stats) {
   REQUIRE_SYM(h);
   CcsdStats st;

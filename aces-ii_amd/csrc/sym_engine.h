@@ -56,6 +56,10 @@ class SymEngine {
   int adjoint(const double* l1, const double* l2, double* r1, double* r2, double* dens);
   int lambda_start();
   int lambda_cycle(int icycle, double tol, CcsdStats& st);
+  // prec 0 (fp64) or kPrecRefine (3): residual = fp64 reverse pass at an anchor
+  // plus the linear reverse pass on the difference to it, whose large products
+  // (ladder, W build, ring) run as bf16x3 products (CcsdEngine's mixed cycles)
+  int set_lambda_precision(int prec);
   int set_lambda_diis_order(int order);
   int get_lambda(double* l1, double* l2, int aces_normalisation);
   int density(double* dens);
@@ -127,7 +131,12 @@ class SymEngine {
   int run_perm(Op& op, double* out, const double* const* src, const double* X, const double* Y,
                const dpd::Denom* den = nullptr);
   int forward(GemmPrec prec);
-  int backward(const double* lam1, const double* lam2, bool density);
+  // seeds: add dE/dT (false: the linear part J^T lam alone); lowp: the adjoint
+  // products of the low-precision large forward products as bf16x3 products
+  int backward(const double* lam1, const double* lam2, bool density, bool seeds = true, bool lowp = false);
+  // bf16 hi/lo split of tensor t's blocks from data (laid out like t), as
+  // stored ([row][col], trans false) or turned round ([col][row])
+  int lowp_split(int t, const double* data, bool trans, LowBuf& lb);
   int pack_dense(int t, const double* dense, const long* strides, long base_off);
   int unpack_dense(int t, const double* packed, double* dense, const long* strides, long base_off);
   int denom_apply(int t, double* out, double beta, const double* in, double alpha, const double* in2,
@@ -178,6 +187,14 @@ class SymEngine {
   std::vector<double> R_, LR_;
   // Lambda state
   double *Lam_ = nullptr, *Lamn_ = nullptr, *lam_ = nullptr, *gT_ = nullptr;
+  // mixed Lambda cycles: anchor multipliers and residual, difference; splits of
+  // the forward operands turned round (per forward state) and of the output
+  // gradients of the large products (per pass)
+  int lam_prec_ = 0;
+  double *lamA_ = nullptr, *gTA_ = nullptr, *dlam_ = nullptr;
+  bool lam_anc_valid_ = false;
+  double lam_anc_res_ = 0.0;
+  std::vector<LowBuf> lowT_, lowg_, lowgT_;
   bool fwd_valid_ = false, lam_ready_ = false;
   // T_new_ holds the increment lists' content after a converged cycle at the
   // current amplitudes (the Lambda start; set on convergence, cleared by any change)

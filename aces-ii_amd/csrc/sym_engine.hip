@@ -522,6 +522,48 @@ int SymEngine::lowp_convert(int t, GemmPrec prec, LowBuf& lb, const double* sub)
   return ACES_OK;
 }
 
+int SymEngine::lowp_split(int t, const double* data, bool trans, LowBuf& lb) {
+  const Tensor& tt = T_[t];
+  if (lb.hi == nullptr) {
+    long off = 0;
+    for (int g = 0; g < nirr; ++g) {
+      const long rows = trans ? tt.ncol[g] : tt.nrow[g], cols = trans ? tt.nrow[g] : tt.ncol[g];
+      lb.ld[g] = pad32(std::max<long>(cols, 1));
+      lb.off[g] = off;
+      off += rows * lb.ld[g];
+    }
+    lb.off[nirr] = off;
+    double* p;
+    ACES_CHECK(alloc(&p, (off * 4 + 7) / 8 + 8));  // bf16 hi + lo: 4 bytes per element
+    lb.hi = p;
+    lb.lo = (char*)p + off * 2;
+  }
+  if (!trans) {
+    SplitBlocks sb;
+    for (int g = 0; g < nirr; ++g) {
+      if (tt.nrow[g] == 0 || tt.ncol[g] == 0) continue;
+      const int k = sb.n++;
+      sb.rows[k] = (int)tt.nrow[g];
+      sb.cols[k] = (int)tt.ncol[g];
+      sb.ldy[k] = lb.ld[g];
+      sb.x[k] = data + tt.boff[g];
+      sb.x0[k] = nullptr;
+      sb.hi[k] = (unsigned short*)lb.hi + lb.off[g];
+      sb.lo[k] = (unsigned short*)lb.lo + lb.off[g];
+    }
+    ACES_CHECK(split_blocks(sb, s_));
+  } else {
+    for (int g = 0; g < nirr; ++g) {
+      if (tt.nrow[g] == 0 || tt.ncol[g] == 0) continue;
+      ACES_CHECK(split_bf16_t((int)tt.ncol[g], (int)tt.nrow[g], data + tt.boff[g], tt.ncol[g],
+                              (unsigned short*)lb.hi + lb.off[g], (unsigned short*)lb.lo + lb.off[g], lb.ld[g], s_));
+    }
+  }
+  lb.prec = GP_BF16X3;
+  lb.valid = true;
+  return ACES_OK;
+}
+
 int SymEngine::ggemm(int C, const double* Cd, int A, const double* Ad, int a_kc, int B, const double* Bd, int b_kc,
                      double alpha, double beta, GemmPrec prec, int cat, const LowBuf* la, const LowBuf* lb) {
   const Tensor &tc = T_[C], &ta = T_[A], &tb = T_[B];
@@ -576,6 +618,7 @@ int SymEngine::ggemm(int C, const double* Cd, int A, const double* Ad, int a_kc,
     } else {
       const int esz = (prec == GP_F32) ? 4 : 2;
       e.K = (prec == GP_BF16X3) ? (int)la->ld[g] : (int)K;
+      e.a_kc = e.b_kc = 1;  // low-precision buffers are k-contiguous by construction
       e.A = (const char*)la->hi + la->off[g] * esz;
       e.A_lo = (prec == GP_BF16X3) ? (const char*)la->lo + la->off[g] * 2 : nullptr;
       e.lda = la->ld[g];
@@ -809,7 +852,7 @@ int SymEngine::forward(GemmPrec prec) {
 
 // Reverse pass: seeds of L = E + lam.(inc - D*T), then every recorded
 // operation's adjoint in reverse order.  Gradients accumulate in Tensor::g.
-int SymEngine::backward(const double* lam1, const double* lam2, bool density) {
+int SymEngine::backward(const double* lam1, const double* lam2, bool density, bool seeds, bool lowp) {
   hipStream_t s = s_;
   for (Tensor& t : T_)
     if (t.g && t.boff[nirr] > 0) ACES_HIP(hipMemsetAsync(t.g, 0, sizeof(double) * t.boff[nirr], s));
@@ -822,9 +865,25 @@ int SymEngine::backward(const double* lam1, const double* lam2, bool density) {
   ACES_CHECK(denom_apply(t1_, T_[t1_].g, 1.0, lam1, -1.0, nullptr, 0.0, 0, 1));
   ACES_CHECK(denom_apply(t2_, T_[t2_].g, 1.0, lam2, -1.0, nullptr, 0.0, 0, 1));
   // E = 2 f_ia t_ia + L_ijab tau_ijab
-  ACES_CHECK(eu::P4(1, 1, 1, n1, T_[t1_].g, 1.0).src(T_[fov_].d, 2.0, 0, 0, 0, 1).run(s));
-  ACES_CHECK(eu::P4(1, 1, 1, n1, T_[fov_].g, 1.0).src(T_[t1_].d, 2.0, 0, 0, 0, 1).run(s));
-  ACES_CHECK(eu::P4(1, 1, 1, n2, T_[tau_].g, 1.0).src(T_[Lijab_].d, 1.0, 0, 0, 0, 1).run(s));
+  if (seeds) {
+    ACES_CHECK(eu::P4(1, 1, 1, n1, T_[t1_].g, 1.0).src(T_[fov_].d, 2.0, 0, 0, 0, 1).run(s));
+    ACES_CHECK(eu::P4(1, 1, 1, n1, T_[fov_].g, 1.0).src(T_[t1_].d, 2.0, 0, 0, 0, 1).run(s));
+    ACES_CHECK(eu::P4(1, 1, 1, n2, T_[tau_].g, 1.0).src(T_[Lijab_].d, 1.0, 0, 0, 0, 1).run(s));
+  }
+  if (lowp) {
+    lowT_.resize(T_.size());
+    lowg_.resize(T_.size());
+    lowgT_.resize(T_.size());
+  }
+  // a forward operand turned round, split once per forward state
+  auto fixed_t = [&](int t) -> int {
+    LowBuf& lb = lowT_[t];
+    if (!lb.valid || lb.ver != T_[t].ver) {
+      ACES_CHECK(lowp_split(t, T_[t].d, true, lb));
+      lb.ver = T_[t].ver;
+    }
+    return ACES_OK;
+  };
   if (density) {
     // orbital-energy derivatives of -lam.(D*T): rows sums of lam o T over each index
     double* h = scr2_;
@@ -867,6 +926,24 @@ int SymEngine::backward(const double* lam1, const double* lam2, bool density) {
       }
       case OP_GEMM: {
         const int A = op.A, B = op.B, C = op.out;
+        if (lowp && op.lowp && op.cat <= 2) {
+          // large products (ladder, W build, ring; both forward operands
+          // k-contiguous) in bf16x3:
+          //   gA[m][k] += al gC[m][n] B[n][k]   (gC as stored, B turned round)
+          //   gB[n][k] += al gC[m][n] A[m][k]   (gC and A turned round)
+          ACES_REQUIRE(op.a_kc && op.b_kc, "sym: low-precision product with a row-contiguous operand");
+          if (T_[A].grad) {
+            ACES_CHECK(lowp_split(C, to.g, false, lowg_[C]));
+            ACES_CHECK(fixed_t(B));
+            ACES_CHECK(ggemm(A, T_[A].g, C, to.g, 1, B, T_[B].d, 0, op.al, 1.0, GP_BF16X3, 4, &lowg_[C], &lowT_[B]));
+          }
+          if (T_[B].grad) {
+            ACES_CHECK(lowp_split(C, to.g, true, lowgT_[C]));
+            ACES_CHECK(fixed_t(A));
+            ACES_CHECK(ggemm(B, T_[B].g, C, to.g, 0, A, T_[A].d, 0, op.al, 1.0, GP_BF16X3, 4, &lowgT_[C], &lowT_[A]));
+          }
+          break;
+        }
         if (T_[A].grad) {
           if (op.a_kc)
             ACES_CHECK(ggemm(A, T_[A].g, C, to.g, 1, B, T_[B].d, !op.b_kc, op.al, 1.0, GP_F64, 4, nullptr, nullptr));
@@ -1714,6 +1791,7 @@ int SymEngine::lambda_start() {
   }
   ACES_CHECK(diis_grow(lsol_, lerr_, lorder_));
   LR_.assign((size_t)lorder_ * lorder_, 0.0);
+  lam_anc_valid_ = false;
   const bool lists = tn_lists_;
   ACES_CHECK(forward(GP_F64));
   // Lambda start: the increment lists xvcc moves to the Lambda lists with
@@ -1730,6 +1808,19 @@ int SymEngine::lambda_start() {
   }
   ACES_HIP(hipStreamSynchronize(s_));
   lam_ready_ = true;
+  return ACES_OK;
+}
+
+int SymEngine::set_lambda_precision(int prec) {
+  ACES_REQUIRE(prec == GP_F64 || prec == kPrecRefine, "sym lambda precision must be 0 (fp64) or 3 (refinement)");
+  if (prec == kPrecRefine && lamA_ == nullptr) {
+    ACES_CHECK(alloc(&lamA_, nt_));
+    ACES_CHECK(alloc(&gTA_, nt_));
+    ACES_CHECK(alloc(&dlam_, nt_));
+  }
+  if (!lam_) ACES_CHECK(alloc(&lam_, nt_));
+  lam_prec_ = prec;
+  lam_anc_valid_ = false;
   return ACES_OK;
 }
 
@@ -1756,7 +1847,21 @@ int SymEngine::lambda_cycle(int icycle, double tol, CcsdStats& st) {
   ACES_HIP(hipEventCreate(&e1));
   ACES_HIP(hipEventRecord(e0, s_));
   ACES_CHECK(lam_from_aces(Lam_, lam_));
-  ACES_CHECK(backward(lam_, lam_ + n1_, false));
+  const bool refine = (lam_prec_ == kPrecRefine);
+  const bool anchor = refine && !lam_anc_valid_;
+  if (refine && lam_anc_valid_) {
+    // r(l) = r(l_a) + J^T (l - l_a): the linear reverse pass on the difference
+    // with the large adjoint products in bf16x3
+    ACES_CHECK(sub(nt_, lam_, lamA_, dlam_, s_));
+    ACES_CHECK(backward(dlam_, dlam_ + n1_, false, false, true));
+    ACES_CHECK(eu::P4(1, 1, 1, nt_, gT_, 1.0).src(gTA_, 1.0, 0, 0, 0, 1).run(s_));
+  } else {
+    ACES_CHECK(backward(lam_, lam_ + n1_, false));
+    if (anchor) {
+      ACES_HIP(hipMemcpyAsync(lamA_, lam_, sizeof(double) * nt_, hipMemcpyDeviceToDevice, s_));
+      ACES_HIP(hipMemcpyAsync(gTA_, gT_, sizeof(double) * nt_, hipMemcpyDeviceToDevice, s_));
+    }
+  }
   ACES_CHECK(pair_symmetrize(gT_ + n1_));
   // L1n = L1 + g1/(2 D1) ; L2n = L2 + (2 g2 + g2(i<->j)) / (3 D2)
   ACES_HIP(hipMemcpyAsync(Lamn_, Lam_, sizeof(double) * nt_, hipMemcpyDeviceToDevice, s_));
@@ -1764,6 +1869,14 @@ int SymEngine::lambda_cycle(int icycle, double tol, CcsdStats& st) {
   ACES_CHECK(denom_apply(t2_, Lamn_ + n1_, 1.0, gT_ + n1_, 2.0 / 3.0, gT_ + n1_, 1.0 / 3.0, 1, -1));
   ACES_HIP(hipEventRecord(e1, s_));
   ACES_CHECK(diis_core(lsol_, lerr_, LR_, lorder_, Lam_, Lamn_, icycle, tol, true, st));
+  // the anchor moves to the current multipliers once the residual has fallen
+  // 1e3-fold below the anchor cycle's
+  if (anchor) {
+    lam_anc_valid_ = true;
+    lam_anc_res_ = std::fabs(st.resmiter);
+  } else if (refine && std::fabs(st.resmiter) < 1e-3 * lam_anc_res_) {
+    lam_anc_valid_ = false;
+  }
   float ms = 0.f;
   ACES_HIP(hipEventSynchronize(e1));
   ACES_HIP(hipEventElapsedTime(&ms, e0, e1));

@@ -226,6 +226,11 @@ int aces_sym_energy(void* handle, double* e_out);
 int aces_sym_adjoint(void* handle, const double* d_l1, const double* d_l2, double* d_r1, double* d_r2,
                      double* d_dens);
 int aces_sym_lambda_start(void* handle);
+/* Lambda cycle precision of the irrep-blocked engine, as aces_ccsd_set_lambda_precision:
+ * 0 fp64, 3 mixed (fp64 residual at an anchor plus the linear reverse pass on the
+ * difference, its ladder / W-build / ring adjoint products in bf16x3; the anchor moves
+ * after every 1e3-fold residual drop).  No counterpart in the reference (xlambda is fp64). */
+int aces_sym_set_lambda_precision(void* handle, int prec);
 int aces_sym_lambda_cycle(void* handle, int icycle, double tol, double* stats);
 int aces_sym_set_lambda_diis_order(void* handle, int order);
 int aces_sym_get_lambda(void* handle, double* d_l1, double* d_l2, int aces_norm);

@@ -172,3 +172,30 @@ def test_refine_polish_sym():
     err_b = max(float((i1b - i1d).abs().max()), float((i2b - i2d).abs().max()))
     print('sym refine err %.3e bf16x3 err %.3e' % (err_r, err_b), flush=True)
     assert err_b > 0 and err_r < 1e-3 * err_b
+
+
+@pytest.mark.parametrize('occ,vir,seed', [([2, 1, 1, 2], [9, 8, 7, 8], 21), ([2, 1, 1, 2, 1, 2, 1, 1], [6, 5, 7, 4, 5, 6, 4, 5], 22)])
+def test_lambda_refine_sym(occ, vir, seed):
+    """Mixed Lambda cycles on the irrep-blocked engine (fp64 residual at an
+    anchor plus the linear reverse pass on the difference, its ladder, W-build
+    and ring adjoint products in bf16x3 over the irrep blocks) converge to the
+    fp64 multipliers and density."""
+    from aces2.ccsd import SymCCSD
+    print('case sym lambda refine', occ, vir, flush=True)
+    f, eri, blk, irr = _system(occ, vir, seed, nonhf=True)
+    cs = SymCCSD(occ, vir, blk)
+    cs.run(maxcyc=200, conv=11, order=6)
+    h64 = cs.solve_lambda(maxcyc=200, conv=10, order=6)
+    a1, a2 = cs.get_lambda()
+    Da = cs.density()
+    hmx = cs.solve_lambda(maxcyc=200, conv=10, order=6, prec='refine')
+    b1, b2 = cs.get_lambda()
+    Db = cs.density()
+    cs.close()
+    print('fp64 %d cycles, mixed %d cycles' % (h64['cycles'], hmx['cycles']), flush=True)
+    assert h64['converged'] and hmx['converged']
+    assert hmx['cycles'] <= h64['cycles'] + 4
+    sc = max(float(a2.abs().max()), 1.0)
+    assert float((a1 - b1).abs().max()) < 1e-8 * sc
+    assert float((a2 - b2).abs().max()) < 1e-8 * sc
+    assert float((Da - Db).abs().max()) < 1e-8
