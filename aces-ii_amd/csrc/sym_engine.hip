@@ -807,16 +807,26 @@ int SymEngine::forward(GemmPrec prec) {
       case OP_GEMM: {
         if (refine && op.lowp && op.cat <= 2) {
           ACES_CHECK(gemm_anchored(k, mode));
-        } else if (op.lowp && (prec == GP_F32 || prec == GP_BF16X3)) {
-          for (int t : {op.A, op.B}) {
-            LowBuf& lb = low_[t];
+        } else if (op.lowp && (prec == GP_BF16X3 || (prec == GP_F32 && op.a_kc && op.b_kc))) {
+          // k-contiguous operands split as stored, row-contiguous ones turned
+          // round (bf16x3); either split is reused until the tensor is rewritten
+          const LowBuf* lab[2];
+          const int ts[2] = {op.A, op.B}, kc[2] = {op.a_kc, op.b_kc};
+          lowT_.resize(T_.size());
+          for (int q = 0; q < 2; ++q) {
+            const int t = ts[q];
+            LowBuf& lb = kc[q] ? low_[t] : lowT_[t];
             if (!lb.valid || lb.prec != prec || lb.ver != T_[t].ver) {
-              ACES_CHECK(lowp_convert(t, prec, lb));
+              if (kc[q])
+                ACES_CHECK(lowp_convert(t, prec, lb));
+              else
+                ACES_CHECK(lowp_split(t, T_[t].d, true, lb));
               lb.ver = T_[t].ver;
             }
+            lab[q] = &lb;
           }
           ACES_CHECK(ggemm(op.out, T_[op.out].d, op.A, T_[op.A].d, op.a_kc, op.B, T_[op.B].d, op.b_kc, op.al,
-                           op.beta, prec, op.cat, &low_[op.A], &low_[op.B]));
+                           op.beta, prec, op.cat, lab[0], lab[1]));
         } else {
           ACES_CHECK(ggemm(op.out, T_[op.out].d, op.A, T_[op.A].d, op.a_kc, op.B, T_[op.B].d, op.b_kc, op.al,
                            op.beta, GP_F64, op.cat, nullptr, nullptr));
@@ -1310,7 +1320,7 @@ int SymEngine::setup(const double* fock, const double* ovov_dense, const double*
   ACES_CHECK(gemm_op(Qo, ovooQ, 1, t1_, 1, 1, 0, 3, false));
   ACES_CHECK(perm(Wo, 0, {{ooooP, I4}, {Qo, I4}, {Qo, {1, 0, 3, 2}}}, {1, 1, 1}));
   ACES_CHECK(gemm_op(Wo, OVkl, 1, tau_, 1, 1, 1, 3, true));
-  ACES_CHECK(gemm_op(ZH, tau_, 0, Wo, 0, 1, 0, 3, false));
+  ACES_CHECK(gemm_op(ZH, tau_, 0, Wo, 0, 1, 0, 3, true));
   ACES_CHECK(perm(inc2_, 1, {{ZH, {2, 3, 0, 1}}}, {1}));
   // F intermediates
   const int Fov = dy(LOV), tau1 = dy(LOOVV1), Foo = dy(LOO), tauK = dy(LVOOV1), Fvv = dy(LVV);

@@ -758,15 +758,23 @@ def bench_sym(args):
     emx, nmx = converge(prec, 7, 9)
     tmx = time.perf_counter() - ta
     log('fp64 %r (%s) %.1f s; mixed+polish %r (%s) %.1f s' % (e64, n64, t64, emx, nmx, tmx))
-    # Lambda on the same lists at the polished amplitudes
+    # Lambda on the same lists at the polished amplitudes: fp64 cycles, then
+    # mixed cycles (fp64 anchor + bf16x3 linear part) to the same multipliers
     ta = time.perf_counter()
     lh = cc.solve_lambda(maxcyc=args.maxcyc, conv=9, order=6)
     tl = time.perf_counter() - ta
     s_lam = float(np.median(lh['ms_cycle'])) / 1e3
     D = cc.density()
     trace = float(torch.trace(D))
-    log('lambda: %d cycles, %.1f s, median %.4f s/cycle; trace(D_corr) = %.2e' %
+    log('lambda fp64: %d cycles, %.1f s, median %.4f s/cycle; trace(D_corr) = %.2e' %
         (lh['cycles'], tl, s_lam, trace))
+    ta = time.perf_counter()
+    lm = cc.solve_lambda(maxcyc=args.maxcyc, conv=9, order=6, prec='refine')
+    tlm = time.perf_counter() - ta
+    s_lam_mx = float(np.median(lm['ms_cycle'])) / 1e3
+    dD = float((cc.density() - D).abs().max())
+    log('lambda mixed: %d cycles, %.1f s, median %.4f s/cycle; |D_mixed - D_fp64| = %.1e' %
+        (lm['cycles'], tlm, s_lam_mx, dD))
     cc.close()
     result = {
         'metric': 'CCSD s/iter, D2h irrep-blocked o=%d,v=%d (grouped MFMA), mixed vs FP64; Lambda s/iter' % (o, v),
@@ -790,7 +798,9 @@ def bench_sym(args):
                         'mixed_polish': {'energy': emx, 'cycles': nmx, 'total_s': tmx},
                         'abs_dE': abs(emx - e64)},
         'lambda': {'cycles': lh['cycles'], 's_per_cycle_median': s_lam, 'total_s': tl,
-                   'density_trace_corr': trace},
+                   'density_trace_corr': trace,
+                   'mixed': {'cycles': lm['cycles'], 's_per_cycle_median': s_lam_mx, 'total_s': tlm,
+                             'max_abs_density_diff_vs_fp64': dD}},
         'cpu_baseline': None,
     }
     print(json.dumps(result), flush=True)
