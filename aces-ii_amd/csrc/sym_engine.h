@@ -178,6 +178,8 @@ class SymEngine {
   int t1_, t2_, fov_, foo_, fvv_, tau_, inc1_, inc2_, Lijab_, Yb_, Sp_, Dp_, Zp_, Zm_, Vp_, Vm_;
   int pack_op_ = -1, unpack_op_ = -1, tau_op_ = -1, last_op_ = -1;
   dpd::Plan jac1_, jac2_, sym2_, perm2_, absplan_;
+  void* d_pair_tiles_ = nullptr;  // tiles of the in-place pair symmetrization (sym2_)
+  long n_pair_tiles_ = 0;
   dpd::Layout aux_o_, aux_v_, aux_vo_;
   // amplitude state (packed [t1 | t2]) and DIIS
   double *T_cur_ = nullptr, *T_new_ = nullptr, *err_cur_ = nullptr, *newvec_ = nullptr, *eperm_ = nullptr;

@@ -104,6 +104,46 @@ __global__ __launch_bounds__(256) void pair_avg_kernel(const dpd::Box* boxes, in
   }
 }
 
+// The same in 32 x 32 tiles of indices (2, 3) for fixed (0, 1): a block
+// loads a tile and its partner tile (the element map turns (2, 3) round),
+// each along its own unit-stride index, and writes both averaged.  Each
+// element pair belongs to exactly one tile (PairTile list built on the host:
+// the box of the smaller irrep tuple; in a box that is its own partner the
+// pairs l0 < l1, and for l0 = l1 the tiles t2 <= t3, where a diagonal tile is
+// its own partner and both writes store the same values).
+struct PairTile {
+  int box;
+  short l0, l1, t2, t3;
+};
+
+__global__ __launch_bounds__(256) void pair_avg_tiled_kernel(const dpd::Box* boxes, const PairTile* tiles,
+                                                             double* __restrict__ x) {
+  __shared__ double X[32][33], YT[32][33];
+  const PairTile tl = tiles[blockIdx.x];
+  const dpd::Box& b = boxes[tl.box];
+  const long base1 = b.ob + tl.l0 * b.os[0] + tl.l1 * b.os[1];
+  const long base2 = b.sb[0] + tl.l0 * b.ss[0][0] + tl.l1 * b.ss[0][1];
+  const int i2 = tl.t2 * 32, i3 = tl.t3 * 32;
+  const int e2 = min(32, b.d[2] - i2), e3 = min(32, b.d[3] - i3);
+  const int lo = threadIdx.x & 31, hi = threadIdx.x >> 5;
+  // tile: rows l2, columns l3 (read along l3); partner: read along l2
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int r = hi + 8 * k;
+    if (r < e2 && lo < e3) X[r][lo] = x[base1 + (long)(i2 + r) * b.os[2] + (long)(i3 + lo) * b.os[3]];
+    if (r < e3 && lo < e2) YT[r][lo] = x[base2 + (long)(i2 + lo) * b.ss[0][2] + (long)(i3 + r) * b.ss[0][3]];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int r = hi + 8 * k;
+    if (r < e2 && lo < e3)
+      x[base1 + (long)(i2 + r) * b.os[2] + (long)(i3 + lo) * b.os[3]] = 0.5 * (X[r][lo] + YT[lo][r]);
+    if (r < e3 && lo < e2)
+      x[base2 + (long)(i2 + lo) * b.ss[0][2] + (long)(i3 + r) * b.ss[0][3]] = 0.5 * (X[lo][r] + YT[r][lo]);
+  }
+}
+
 struct AbsM {
   double val;
   double a;
@@ -1063,7 +1103,12 @@ int SymEngine::denom_apply(int t, double* out, double beta, const double* in, do
 
 int SymEngine::pair_symmetrize(double* t2) {
   if (sym2_.total == 0) return ACES_OK;
-  hipLaunchKernelGGL(pair_avg_kernel, dim3((unsigned)sym2_.total), dim3(256), 0, s_, sym2_.dev, sym2_.nbox, t2);
+  if (n_pair_tiles_ > 0) {
+    hipLaunchKernelGGL(pair_avg_tiled_kernel, dim3((unsigned)n_pair_tiles_), dim3(256), 0, s_, sym2_.dev,
+                       (const PairTile*)d_pair_tiles_, t2);
+  } else {
+    hipLaunchKernelGGL(pair_avg_kernel, dim3((unsigned)sym2_.total), dim3(256), 0, s_, sym2_.dev, sym2_.nbox, t2);
+  }
   ACES_HIP(hipGetLastError());
   return ACES_OK;
 }
@@ -1401,6 +1446,36 @@ int SymEngine::setup(const double* fock, const double* ovov_dense, const double*
     ACES_CHECK(dpd::build_plan(jac2_, LOOVV, {id4}, nullptr, nullptr));
     ACES_CHECK(dpd::build_plan(perm2_, LOOVV, {id4, sw}, nullptr, nullptr));
     ACES_CHECK(dpd::build_plan(sym2_, LOOVV, {swp}, nullptr, nullptr));
+    {
+      // tiles of the in-place pair symmetrization (each element pair once)
+      std::vector<PairTile> tiles;
+      std::vector<int> index(nirr * nirr * nirr, -1);
+      for (int k = 0; k < sym2_.nbox; ++k) {
+        const dpd::Box& b = sym2_.host[k];
+        index[(b.g[0] * nirr + b.g[1]) * nirr + b.g[2]] = k;
+      }
+      for (int k = 0; k < sym2_.nbox; ++k) {
+        const dpd::Box& b = sym2_.host[k];
+        const int g[4] = {b.g[0], b.g[1], b.g[2], b.g[3]}, gp[4] = {g[1], g[0], g[3], g[2]};
+        const int key = (g[0] * nirr + g[1]) * nirr + g[2], pkey = (gp[0] * nirr + gp[1]) * nirr + gp[2];
+        ACES_REQUIRE(index[pkey] >= 0, "sym: partner box missing");
+        if (pkey < key) continue;  // the partner box's tiles cover these pairs
+        const bool self = (pkey == key);
+        const int n2 = (b.d[2] + 31) / 32, n3 = (b.d[3] + 31) / 32;
+        for (int l0 = 0; l0 < b.d[0]; ++l0)
+          for (int l1 = self ? l0 : 0; l1 < b.d[1]; ++l1)
+            for (int t2 = 0; t2 < n2; ++t2)
+              for (int t3 = (self && l0 == l1) ? t2 : 0; t3 < n3; ++t3)
+                tiles.push_back(PairTile{k, (short)l0, (short)l1, (short)t2, (short)t3});
+      }
+      ACES_REQUIRE(tiles.size() < (1UL << 31), "sym: too many pair tiles");
+      n_pair_tiles_ = (long)tiles.size();
+      double* p;
+      ACES_CHECK(alloc(&p, ((long)(tiles.size() * sizeof(PairTile)) + 7) / 8));
+      d_pair_tiles_ = p;
+      if (!tiles.empty())
+        ACES_HIP(hipMemcpy(d_pair_tiles_, tiles.data(), tiles.size() * sizeof(PairTile), hipMemcpyHostToDevice));
+    }
     ACES_CHECK(dpd::build_plan(absplan_, LOOVV, {sw}, nullptr, nullptr));
     plans_.resize(5);
     Src a;
