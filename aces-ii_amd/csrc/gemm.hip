@@ -810,6 +810,39 @@ __global__ void splitk_reduce_kernel(int M, int N, int splits, int batch, const 
   }
 }
 
+// Many splits over a small output: one wave per output element, lane l
+// summing splits l, l + 64, ... and the lanes combined by a fixed shuffle
+// tree (deterministic), so the partial sums are read 64 at a time instead of
+// one after another.
+__device__ inline void splitk_reduce_wave(long idx, long total, int N, int splits, const double* __restrict__ ws,
+                                          long zC, double* __restrict__ C, long ldc, double beta) {
+  const int lane = threadIdx.x & 63;
+  if (idx >= total) return;
+  double acc = 0.0;
+  for (int z = lane; z < splits; z += 64) acc += ws[z * zC + idx];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane == 0) {
+    const long row = idx / N, col = idx - row * N;
+    double* c = C + row * ldc + col;
+    *c = (beta == 0.0) ? acc : acc + beta * (*c);
+  }
+}
+
+__global__ __launch_bounds__(256) void splitk_reduce_wide_kernel(int M, int N, int splits, int batch,
+                                                                 const double* __restrict__ ws, long zC,
+                                                                 double* __restrict__ C, long ldc, long sC,
+                                                                 double beta) {
+  const long per = (long)M * N;
+  const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= per * batch) return;
+  const long bz = w / per, r = w - bz * per;
+  splitk_reduce_wave(r, per, N, splits, ws + bz * per, zC, C + bz * sC, ldc, beta);
+}
+
+constexpr int kWideSplits = 64;           // splits from which a wave reduces one element
+constexpr long kWideMaxOutputs = 1L << 16;  // outputs up to which that pays
+
 // The split-K reductions of every group of one grouped launch in one launch
 // (blockIdx.y = group; groups without splits have splits = 0).
 struct ReduceGroups {
@@ -824,12 +857,17 @@ struct ReduceGroups {
   } g[kMaxGroups];
 };
 
-__global__ void splitk_reduce_groups_kernel(ReduceGroups rg) {
+__global__ void splitk_reduce_groups_kernel(ReduceGroups rg, int wide) {
   ReduceGroups::G q = rg.g[0];
 #pragma unroll
   for (int k = 1; k < kMaxGroups; ++k)
     if (k == (int)blockIdx.y) q = rg.g[k];
   if (q.splits <= 1) return;
+  if (wide) {
+    const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    splitk_reduce_wave(w, (long)q.M * q.N, q.N, q.splits, q.ws, q.zC, q.C, q.ldc, q.beta);
+    return;
+  }
   const long total = (long)q.M * q.N;
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
     const long row = idx / q.N, col = idx - row * q.N;
@@ -1073,6 +1111,12 @@ static int launch_groups(const GemmGroups& gg, GemmPrec prec, bool akc, bool bkc
 static int reduce_splits(const GemmDesc& d, const GemmArgs& a, hipStream_t stream) {
   if (a.splits <= 1) return ACES_OK;
   const long total = (long)d.M * d.N * d.batch;
+  if (a.splits >= kWideSplits && total <= kWideMaxOutputs) {
+    hipLaunchKernelGGL(splitk_reduce_wide_kernel, dim3((unsigned)((total + 3) / 4)), dim3(256), 0, stream, d.M, d.N,
+                       a.splits, d.batch, a.C, a.zC, d.C, d.ldc, d.sC, d.beta);
+    ACES_HIP(hipGetLastError());
+    return ACES_OK;
+  }
   long g = (total + 255) / 256;
   if (g > (1L << 20)) g = 1L << 20;
   hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)g), dim3(256), 0, stream, d.M, d.N, a.splits, d.batch,
@@ -1235,10 +1279,14 @@ int gemm_grouped(const GemmDesc* d, int ng, GemmPrec prec, hipStream_t stream) {
     most = std::max(most, (long)q.M * q.N);
   }
   if (most > 0) {
-    long gx = (most + 255) / 256;
-    if (gx > 65536) gx = 65536;
+    int minsplit = 1 << 30;
+    for (int g = 0; g < kMaxGroups; ++g)
+      if (rg.g[g].splits > 1) minsplit = std::min(minsplit, rg.g[g].splits);
+    const int wide = (minsplit >= kWideSplits && most <= kWideMaxOutputs) ? 1 : 0;
+    long gx = wide ? (most + 3) / 4 : (most + 255) / 256;
+    if (!wide && gx > 65536) gx = 65536;
     hipLaunchKernelGGL(splitk_reduce_groups_kernel, dim3((unsigned)gx, (unsigned)kMaxGroups), dim3(256), 0, stream,
-                       rg);
+                       rg, wide);
     ACES_HIP(hipGetLastError());
   }
   return ACES_OK;
