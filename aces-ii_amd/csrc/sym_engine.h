@@ -177,12 +177,13 @@ class SymEngine {
   // tensor ids
   int t1_, t2_, fov_, foo_, fvv_, tau_, inc1_, inc2_, Lijab_, Yb_, Sp_, Dp_, Zp_, Zm_, Vp_, Vm_;
   int pack_op_ = -1, unpack_op_ = -1, tau_op_ = -1, last_op_ = -1;
-  dpd::Plan jac1_, jac2_, sym2_, perm2_, absplan_;
+  dpd::Plan jac1_, jac2_, sym2_, perm2_, absplan_, comb_;
+  bool t_sym_ = false;  // T_cur_'s t2 is known to be exactly pair-symmetric (set by DIIS)
   void* d_pair_tiles_ = nullptr;  // tiles of the in-place pair symmetrization (sym2_)
   long n_pair_tiles_ = 0;
   dpd::Layout aux_o_, aux_v_, aux_vo_;
   // amplitude state (packed [t1 | t2]) and DIIS
-  double *T_cur_ = nullptr, *T_new_ = nullptr, *err_cur_ = nullptr, *newvec_ = nullptr, *eperm_ = nullptr;
+  double *T_cur_ = nullptr, *T_new_ = nullptr, *newvec_ = nullptr, *eperm_ = nullptr;
   long nt_ = 0, n1_ = 0;
   int order_ = 6, lorder_ = 6;
   std::vector<double*> sol_, errv_, lsol_, lerr_;

@@ -71,6 +71,17 @@ __global__ void mdot_finish_kernel(int nb, const double* __restrict__ part, doub
   if (threadIdx.x == 0) out[k] = scale * sh[0] + (accumulate ? out[k] : 0.0);
 }
 
+// DODIIS0 bookkeeping in one pass: e = xn - x (the error slot), sol = x (the
+// solution slot).
+__global__ void diis_err_kernel(long n, const double* __restrict__ x, const double* __restrict__ xn,
+                                double* __restrict__ e, double* __restrict__ sol) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double a = x[i];
+    e[i] = xn[i] - a;
+    sol[i] = a;
+  }
+}
+
 __global__ void hadamard_kernel(long n, const double* __restrict__ x, const double* __restrict__ y,
                                 double* __restrict__ z) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -246,7 +257,7 @@ SymEngine::~SymEngine() {
     for (auto& p : op.bwd) dpd::free_plan(p);
     dpd::free_plan(op.hplan);
   }
-  for (Plan* p : {&jac1_, &jac2_, &sym2_, &perm2_, &absplan_}) dpd::free_plan(*p);
+  for (Plan* p : {&jac1_, &jac2_, &sym2_, &perm2_, &absplan_, &comb_}) dpd::free_plan(*p);
   for (Plan& p : plans_) dpd::free_plan(p);
   for (void* p : allocs_) (void)hipFree(p);
 }
@@ -1477,6 +1488,7 @@ int SymEngine::setup(const double* fock, const double* ovov_dense, const double*
         ACES_HIP(hipMemcpy(d_pair_tiles_, tiles.data(), tiles.size() * sizeof(PairTile), hipMemcpyHostToDevice));
     }
     ACES_CHECK(dpd::build_plan(absplan_, LOOVV, {sw}, nullptr, nullptr));
+    ACES_CHECK(dpd::build_plan(comb_, LOOVV, {id4, sw}, nullptr, nullptr));
     plans_.resize(5);
     Src a;
     a.L = &T_[t2_].L;
@@ -1495,7 +1507,6 @@ int SymEngine::setup(const double* fock, const double* ovov_dense, const double*
   ACES_CHECK(alloc(&absp_, (long)((64 + absplan_.total + 1) * sizeof(AbsM) + 7) / 8));
   ACES_CHECK(diis_grow(sol_, errv_, order_));
   R_.assign((size_t)order_ * order_, 0.0);
-  ACES_CHECK(alloc(&err_cur_, nt_));
   ACES_CHECK(alloc(&newvec_, nt_));
   ACES_CHECK(alloc(&eperm_, nt_));
   ACES_HIP(hipStreamSynchronize(s));
@@ -1535,6 +1546,7 @@ int SymEngine::set_amplitudes(const double* t1, const double* t2) {
   ACES_CHECK(pack_dense(t2_, t2, s2, 0));
   fwd_valid_ = false;
   tn_lists_ = false;
+  t_sym_ = false;
   return ACES_OK;
 }
 
@@ -1572,6 +1584,7 @@ int SymEngine::get_amplitudes(double* t1, double* t2) {
 int SymEngine::start_mp1() {
   ACES_REQUIRE(ready_, "sym: not set up");
   tn_lists_ = false;
+  t_sym_ = false;
   ACES_HIP(hipMemsetAsync(T_cur_, 0, sizeof(double) * n1_, s_));
   // <ij|ab> = (ia|jb) = (O,O|V,V) gather of ovov; Lijab's source is ovov, so use
   // the first recorded inc2 permutation's source 0 (ovov, map 0 2 1 3).
@@ -1620,39 +1633,34 @@ int SymEngine::energy_current(double& e) { return energy_dev(T_[t1_].d, T_[t2_].
 int SymEngine::diis_core(std::vector<double*>& sol, std::vector<double*>& err, std::vector<double>& R, int order,
                          double* x, double* xn, int icycle, double tol, bool lambda_rule, CcsdStats& st) {
   hipStream_t s = s_;
-  ACES_CHECK(pair_symmetrize(x + n1_));
+  // (amplitudes left by a DIIS update or a symmetrized Jacobi update are
+  // already pair-symmetric)
+  if (!(lambda_rule ? false : t_sym_)) ACES_CHECK(pair_symmetrize(x + n1_));
   ACES_CHECK(pair_symmetrize(xn + n1_));
   const int ndimr = std::min(icycle, order);
   const int idimr = 1 + (icycle - 1) % order;
-  ACES_CHECK(sub(nt_, xn, x, err_cur_, s));
+  // e = xn - x straight into the error slot, x into the solution slot
+  double* e = err[idimr - 1];
+  hipLaunchKernelGGL(diis_err_kernel, dim3(4096), dim3(256), 0, s, nt_, x, xn, e, sol[idimr - 1]);
+  ACES_HIP(hipGetLastError());
   double resmiter;
-  ACES_CHECK(absmax_vec(err_cur_, resmiter));
-  ACES_HIP(hipMemcpyAsync(sol[idimr - 1], x, sizeof(double) * nt_, hipMemcpyDeviceToDevice, s));
-  // e(i<->j) for the AA part
+  ACES_CHECK(absmax_vec(e, resmiter));
+  // the DIIS dot over [t1 | t2(AB) | t2(AA)] of pair-symmetric vectors is
+  // <x, y> = x1.y1 + 1.5 x2.y2 - 0.5 x2.y2(i<->j): one pass of the slots
+  // against comb = [e1 | 1.5 e2 - 0.5 e2(i<->j)]
   {
-    const double* src[1] = {err_cur_ + n1_};
-    const double one[1] = {1.0};
-    Plan& p = absplan_;
-    ACES_CHECK(dpd::apply_plan(p, eperm_ + n1_, 0.0, src, one, nullptr, nullptr, 0.0, nullptr, s));
+    ACES_HIP(hipMemcpyAsync(eperm_, e, sizeof(double) * n1_, hipMemcpyDeviceToDevice, s));
+    const double* src[2] = {e + n1_, e + n1_};
+    const double al[2] = {1.5, -0.5};
+    ACES_CHECK(dpd::apply_plan(comb_, eperm_ + n1_, 0.0, src, al, nullptr, nullptr, 0.0, nullptr, s));
   }
   std::vector<const double*> xs(ndimr);
-  for (int i = 1; i <= ndimr; ++i) xs[i - 1] = (i == idimr) ? err_cur_ : err[i - 1];
+  for (int i = 1; i <= ndimr; ++i) xs[i - 1] = err[i - 1];
   ACES_HIP(hipMemcpyAsync(dptr_, xs.data(), sizeof(double*) * ndimr, hipMemcpyHostToDevice, s));
   double* out = scal_ + 8;
-  const long n2 = nt_ - n1_;
-  std::vector<const double*> xs2(ndimr);
-  for (int i = 0; i < ndimr; ++i) xs2[i] = xs[i] + n1_;
-  // t1 part
-  hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks), dim3(256), 0, s, n1_, ndimr, (const double* const*)dptr_,
-                     err_cur_, (double*)red_);
+  hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks), dim3(256), 0, s, nt_, ndimr, (const double* const*)dptr_,
+                     eperm_, (double*)red_);
   hipLaunchKernelGGL(mdot_finish_kernel, dim3(ndimr), dim3(256), 0, s, kDotBlocks, (double*)red_, out, 1.0, 0);
-  ACES_HIP(hipMemcpyAsync(dptr_, xs2.data(), sizeof(double*) * ndimr, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks), dim3(256), 0, s, n2, ndimr, (const double* const*)dptr_,
-                     err_cur_ + n1_, (double*)red_);
-  hipLaunchKernelGGL(mdot_finish_kernel, dim3(ndimr), dim3(256), 0, s, kDotBlocks, (double*)red_, out, 1.5, 1);
-  hipLaunchKernelGGL(mdot_kernel, dim3(kDotBlocks), dim3(256), 0, s, n2, ndimr, (const double* const*)dptr_,
-                     eperm_ + n1_, (double*)red_);
-  hipLaunchKernelGGL(mdot_finish_kernel, dim3(ndimr), dim3(256), 0, s, kDotBlocks, (double*)red_, out, -0.5, 1);
   ACES_HIP(hipGetLastError());
   double h[40];
   ACES_HIP(hipMemcpyAsync(h, scal_, sizeof(double) * 40, hipMemcpyDeviceToHost, s));
@@ -1662,7 +1670,6 @@ int SymEngine::diis_core(std::vector<double*>& sol, std::vector<double*>& err, s
     R[(size_t)(i - 1) * order + (idimr - 1)] = h[8 + i - 1];
     R[(size_t)(idimr - 1) * order + (i - 1)] = h[8 + i - 1];
   }
-  ACES_HIP(hipMemcpyAsync(err[idimr - 1], err_cur_, sizeof(double) * nt_, hipMemcpyDeviceToDevice, s));
   const int na = ndimr + 1;
   std::vector<double> A((size_t)na * na, 0.0), b(na, 0.0);
   for (int j = 1; j < na; ++j) {
@@ -1726,6 +1733,9 @@ int SymEngine::diis_core(std::vector<double*>& sol, std::vector<double*>& err, s
   } else if (dst != xn) {
     ACES_HIP(hipMemcpyAsync(x, xn, sizeof(double) * nt_, hipMemcpyDeviceToDevice, s));
   }
+  // a combination of pair-symmetric vectors (or the symmetrized update) is
+  // exactly pair-symmetric
+  if (!lambda_rule) t_sym_ = true;
   return ACES_OK;
 }
 
@@ -1783,6 +1793,7 @@ int SymEngine::diis_dense(int icycle, double tol, const double* t1, const double
   ACES_CHECK(pack_dense(t1_, t1, s1, 0));
   ACES_CHECK(pack_dense(t2_, t2, s2, 0));
   tn_lists_ = false;
+  t_sym_ = false;
   // T_new through the t1/t2 tensors' layouts into T_new_
   {
     double* keep1 = T_[t1_].d;
