@@ -943,8 +943,9 @@ static int bf16_shape(const GemmDesc* d, int ng) {
     }
   return !any ? 2 : (wide ? 8 : 4);
 }
-// fp64 problems with 16-byte aligned rows, K % 16 == 0, K >= 256 and
-// M >= 256 run on the direct-to-LDS kernel: 128 x 128 blocks of 2 x 4 waves
+// fp64 problems with 16-byte aligned rows, K % 16 == 0, M >= 256 and K >= 256
+// (or K >= 32 with N >= 256: 16 % faster on 737280 x 320 x 48, slower on
+// narrow outputs) run on the direct-to-LDS kernel: 128 x 128 blocks of 2 x 4 waves
 // (64 x 32 each), two LDS stages (64 KB, two blocks per CU).  Measured against
 // the 64 x 64 register-staged kernel and other direct-to-LDS shapes (128 x 64
 // on 4 waves with 3 stages, 128 x 128 on 4 or 16 waves, 3 or 4 stages): 70 TF
@@ -956,7 +957,8 @@ static bool f64_direct(const GemmDesc* d, int ng) {
   for (int g = 0; g < ng; ++g)
     if (d[g].M > 0 && d[g].N > 0) {
       const GemmDesc& e = d[g];
-      if (e.K < 256 || e.K % 16 != 0 || e.M < 256) return false;
+      // (short K too when the output is wide: the write-bound rank-o updates)
+      if ((e.K < 256 && !(e.K >= 32 && e.N >= 256)) || e.K % 16 != 0 || e.M < 256) return false;
       if ((uintptr_t)e.A % 16 || (uintptr_t)e.B % 16 || e.lda % 2 || e.ldb % 2 || e.sA % 2 || e.sB % 2) return false;
       // row-contiguous operands are loaded in row pairs
       if ((!e.a_kc && e.M % 2) || (!e.b_kc && e.N % 2)) return false;

@@ -187,7 +187,8 @@ def test_bf16x3_block_shapes(M, N, K, ws):
     assert err < 3e-5, err
 
 
-@pytest.mark.parametrize('M,N,K,ws', [(1300, 200, 512, False), (3000, 1176, 1024, False), (1100, 90, 4096, True)])
+@pytest.mark.parametrize('M,N,K,ws', [(1300, 200, 512, False), (3000, 1176, 1024, False), (1100, 90, 4096, True),
+                                      (2000, 320, 48, False)])
 def test_f64_direct_lds(M, N, K, ws):
     """fp64 problems on the direct-to-LDS kernel (k-contiguous operands, K % 16 == 0,
     narrow outputs), ragged M/N, with and without split-K."""
@@ -206,7 +207,8 @@ def test_f64_direct_lds(M, N, K, ws):
 
 
 @pytest.mark.parametrize('a_kc,b_kc', [(1, 0), (0, 1), (0, 0)])
-@pytest.mark.parametrize('M,N,K,ws', [(1030, 500, 528, False), (300, 90, 4096, True), (1030, 501, 528, False)])
+@pytest.mark.parametrize('M,N,K,ws', [(1030, 500, 528, False), (300, 90, 4096, True), (1030, 501, 528, False),
+                                      (2000, 320, 48, False), (1030, 300, 32, False)])
 def test_f64_direct_lds_layouts(a_kc, b_kc, M, N, K, ws):
     """Row-contiguous operands on the direct-to-LDS fp64 kernel ([k][row] LDS
     tiles, row pairs clamped at the edge); an odd row count takes the
