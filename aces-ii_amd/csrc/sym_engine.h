@@ -137,6 +137,11 @@ class SymEngine {
   // bf16 hi/lo split of tensor t's blocks from data (laid out like t), as
   // stored ([row][col], trans false) or turned round ([col][row])
   int lowp_split(int t, const double* data, bool trans, LowBuf& lb);
+  // fp64 copy of tensor t's blocks with rows padded to a multiple of 32
+  // (zeros): operands of the direct-to-LDS fp64 kernel whatever the block widths
+  int copy64(int t, LowBuf& lb);
+  int padded_f64(int t);
+  bool f64_wide(const Op& op) const;
   int pack_dense(int t, const double* dense, const long* strides, long base_off);
   int unpack_dense(int t, const double* packed, double* dense, const long* strides, long base_off);
   int denom_apply(int t, double* out, double beta, const double* in, double alpha, const double* in2,
@@ -198,6 +203,7 @@ class SymEngine {
   bool lam_anc_valid_ = false;
   double lam_anc_res_ = 0.0;
   std::vector<LowBuf> lowT_, lowg_, lowgT_;
+  std::vector<LowBuf> low64_;  // padded fp64 copies (per tensor version)
   bool fwd_valid_ = false, lam_ready_ = false;
   // T_new_ holds the increment lists' content after a converged cycle at the
   // current amplitudes (the Lambda start; set on convergence, cleared by any change)

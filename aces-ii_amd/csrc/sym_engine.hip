@@ -573,6 +573,67 @@ int SymEngine::lowp_convert(int t, GemmPrec prec, LowBuf& lb, const double* sub)
   return ACES_OK;
 }
 
+int SymEngine::copy64(int t, LowBuf& lb) {
+  const Tensor& tt = T_[t];
+  if (lb.hi == nullptr) {
+    long off = 0;
+    for (int g = 0; g < nirr; ++g) {
+      lb.ld[g] = pad32(std::max<long>(tt.ncol[g], 1));
+      lb.off[g] = off;
+      off += tt.nrow[g] * lb.ld[g];
+    }
+    lb.off[nirr] = off;
+    double* p;
+    ACES_CHECK(alloc(&p, off + 8));
+    lb.hi = p;
+    lb.lo = nullptr;
+  }
+  SplitBlocks sb;
+  sb.f64 = true;
+  for (int g = 0; g < nirr; ++g) {
+    if (tt.nrow[g] == 0 || tt.ncol[g] == 0) continue;
+    const int k = sb.n++;
+    sb.rows[k] = (int)tt.nrow[g];
+    sb.cols[k] = (int)tt.ncol[g];
+    sb.ldy[k] = lb.ld[g];
+    sb.x[k] = tt.d + tt.boff[g];
+    sb.x0[k] = nullptr;
+    sb.hi[k] = (double*)lb.hi + lb.off[g];
+    sb.lo[k] = nullptr;
+  }
+  ACES_CHECK(split_blocks(sb, s_));
+  lb.prec = GP_F64;
+  lb.valid = true;
+  return ACES_OK;
+}
+
+// Products worth padded copies: every nonempty block in the fp64
+// direct-to-LDS kernel's range (at least 256 rows, K at least 256, or at least
+// 32 with 256 columns).
+bool SymEngine::f64_wide(const Op& op) const {
+  bool any = false;
+  for (int g = 0; g < nirr; ++g) {
+    const long M = T_[op.out].nrow[g], N = T_[op.out].ncol[g], K = T_[op.A].ncol[g];
+    if (M == 0 || N == 0 || K == 0) continue;
+    if (M < 256 || K < 32 || (K < 256 && N < 256)) return false;
+    any = true;
+  }
+  return any;
+}
+
+// Large fp64 products run on padded copies of their operands: block widths
+// are arbitrary, the direct-to-LDS kernel needs K % 16 == 0 and even leading
+// dimensions.  A copy is remade when its tensor has been rewritten.
+int SymEngine::padded_f64(int t) {
+  low64_.resize(T_.size());
+  LowBuf& lb = low64_[t];
+  if (!lb.valid || lb.ver != T_[t].ver) {
+    ACES_CHECK(copy64(t, lb));
+    lb.ver = T_[t].ver;
+  }
+  return ACES_OK;
+}
+
 int SymEngine::lowp_split(int t, const double* data, bool trans, LowBuf& lb) {
   const Tensor& tt = T_[t];
   if (lb.hi == nullptr) {
@@ -660,15 +721,15 @@ int SymEngine::ggemm(int C, const double* Cd, int A, const double* Ad, int a_kc,
       continue;
     }
     flops += 2.0 * M * N * K;
-    if (prec == GP_F64 || la == nullptr) {
+    if (la == nullptr) {
       e.K = (int)K;
       e.A = Ad + ta.boff[g];
       e.lda = ta.ncol[g];
       e.B = Bd + tb.boff[g];
       e.ldb = tb.ncol[g];
     } else {
-      const int esz = (prec == GP_F32) ? 4 : 2;
-      e.K = (prec == GP_BF16X3) ? (int)la->ld[g] : (int)K;
+      const int esz = (prec == GP_F64) ? 8 : (prec == GP_F32) ? 4 : 2;
+      e.K = (prec == GP_BF16X3 || prec == GP_F64) ? (int)la->ld[g] : (int)K;
       e.a_kc = e.b_kc = 1;  // low-precision buffers are k-contiguous by construction
       e.A = (const char*)la->hi + la->off[g] * esz;
       e.A_lo = (prec == GP_BF16X3) ? (const char*)la->lo + la->off[g] * 2 : nullptr;
@@ -770,13 +831,17 @@ int SymEngine::gemm_anchored(size_t k, int mode) {
   const long nC = tc.boff[nirr];
   const bool sa = !T_[op.A].grad, sb = !T_[op.B].grad;
   ACES_REQUIRE(op.a_kc && op.b_kc, "sym refinement: operands must be k-contiguous");
+  if ((sa && sb) || mode == 1) {
+    ACES_CHECK(padded_f64(op.A));
+    ACES_CHECK(padded_f64(op.B));
+  }
   if (sa && sb)
-    return ggemm(op.out, tc.d, op.A, T_[op.A].d, 1, op.B, T_[op.B].d, 1, op.al, op.beta, GP_F64, op.cat, nullptr,
-                 nullptr);
+    return ggemm(op.out, tc.d, op.A, T_[op.A].d, 1, op.B, T_[op.B].d, 1, op.al, op.beta, GP_F64, op.cat,
+                 &low64_[op.A], &low64_[op.B]);
   if (mode == 1) {
     if (anc_C0_[k] == nullptr) ACES_CHECK(alloc(&anc_C0_[k], std::max<long>(nC, 1)));
-    ACES_CHECK(ggemm(op.out, anc_C0_[k], op.A, T_[op.A].d, 1, op.B, T_[op.B].d, 1, 1.0, 0.0, GP_F64, op.cat, nullptr,
-                     nullptr));
+    ACES_CHECK(ggemm(op.out, anc_C0_[k], op.A, T_[op.A].d, 1, op.B, T_[op.B].d, 1, 1.0, 0.0, GP_F64, op.cat,
+                     &low64_[op.A], &low64_[op.B]));
     for (int t : {op.A, op.B}) {
       if (!T_[t].grad) continue;
       // one copy per tensor and pass: a tensor read by two anchored products
@@ -878,6 +943,11 @@ int SymEngine::forward(GemmPrec prec) {
           }
           ACES_CHECK(ggemm(op.out, T_[op.out].d, op.A, T_[op.A].d, op.a_kc, op.B, T_[op.B].d, op.b_kc, op.al,
                            op.beta, prec, op.cat, lab[0], lab[1]));
+        } else if (op.a_kc && op.b_kc && f64_wide(op)) {
+          ACES_CHECK(padded_f64(op.A));
+          ACES_CHECK(padded_f64(op.B));
+          ACES_CHECK(ggemm(op.out, T_[op.out].d, op.A, T_[op.A].d, 1, op.B, T_[op.B].d, 1, op.al, op.beta, GP_F64,
+                           op.cat, &low64_[op.A], &low64_[op.B]));
         } else {
           ACES_CHECK(ggemm(op.out, T_[op.out].d, op.A, T_[op.A].d, op.a_kc, op.B, T_[op.B].d, op.b_kc, op.al,
                            op.beta, GP_F64, op.cat, nullptr, nullptr));

@@ -15,10 +15,12 @@ int split_bf16(int rows, int cols, const double* x, long ldx, unsigned short* hi
 // The blocks of one irrep-blocked tensor converted in one launch: block g is
 // rows[g] x cols[g] at x[g] (leading dimension cols[g]), written with
 // leading dimension ldy[g] (columns past cols[g] zero) as bf16 hi/lo
-// (split, x - x0 when x0[g] is given) or as fp32 (hi holds the floats).
+// (split, x - x0 when x0[g] is given), as fp32 (hi holds the floats) or as
+// fp64 (hi holds the padded copy).
 struct SplitBlocks {
   int n = 0;
   bool f32 = false;
+  bool f64 = false;
   int rows[8], cols[8];
   long ldy[8];
   const double* x[8];

@@ -85,7 +85,9 @@ __global__ void split_blocks_kernel(SplitBlocks bs) {
       v = b.x[r * b.cols + c];
       if (b.x0) v -= b.x0[r * b.cols + c];
     }
-    if (bs.f32) {
+    if (bs.f64) {
+      static_cast<double*>(b.hi)[k] = v;
+    } else if (bs.f32) {
       static_cast<float*>(b.hi)[k] = (float)v;
     } else {
       const unsigned short h = bf16_rn((float)v);
