@@ -55,6 +55,11 @@ struct GemmDesc {
 };
 
 int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream);
+// fp64 products that run as operand streams (grouped too): 1 = one output
+// side at most 64 long, the other at least 4096, K at most 64 / ceil(short
+// side / 16); 2 = output sides at most 32 and 64, K at least 4096; 0 =
+// neither; 3 = an empty problem (decides nothing).
+int gemm_skinny_kind(long M, long N, long K);
 // Grouped launch: up to 8 independent problems (same operand orders and
 // precision, batch 1) in one kernel launch, blockIdx.y selecting the problem.
 // Split-K partials of all groups share d[0].ws.

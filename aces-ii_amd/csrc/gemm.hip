@@ -8,6 +8,7 @@
 // walked in groups of 8 row tiles so that co-resident blocks share A and B
 // tiles in the XCD's L2.
 #include <algorithm>
+#include <cstring>
 #include "common.h"
 #include "gemm.h"
 #include "bf16_util.h"
@@ -1015,7 +1016,7 @@ static int prepare(const GemmDesc& d, GemmPrec prec, GemmArgs& a, double* ws, lo
   a.zC = 0;
   a.splits = 1;
   int splits = 1;
-  if (ws != nullptr && d.K >= 1024 && tiles_total < 512) {
+  if (ws != nullptr && d.K >= 1024 && tiles_total > 0 && tiles_total < 512) {
     long sp = 1024 / tiles_total;
     const long kmax = d.K / 512;
     if (sp > kmax) sp = kmax;
@@ -1127,6 +1128,311 @@ static int reduce_splits(const GemmDesc& d, const GemmArgs& a, hipStream_t strea
   return ACES_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Skinny fp64 products: one side of the output at most 64 long (the short
+// side), the other (long) side taken 16 at a time.  Such products read far
+// more operand bytes than they compute on, so they run as HBM streams rather
+// than on 64 x 64 or 128 x 128 tiles that would idle most of their MFMA lanes:
+//   short K (K <= 64 / TM): each wave keeps the short operand's fragments in
+//     registers and walks 16-wide chunks of the long side over the whole K;
+//   long K (both output sides <= 64): each wave takes one 16-wide chunk and
+//     one K range; partial sums go to the split-K workspace and the grouped
+//     reduction adds them in split order (one split writes C itself).
+// S(s, k) = S[s*ss + k*sk], L(l, k) = L[l*ls + k*lk]; output (s, l) at
+// C[s*cs + l*cl].  The side whose index is contiguous in C goes on the MFMA
+// columns (swap = 1: long side on the rows), so that stores run along lanes.
+struct SkinnyArgs {
+  const double* S;
+  long ss, sk;
+  const double* L;
+  long ls, lk;
+  double* C;
+  long cs, cl;
+  int ns, nl, K, nchunks;
+  double alpha, beta;
+  int splits, swap;
+  long kchunk;
+  long zW;  // partial sums of split z at C + z * zW (long K)
+};
+struct SkinnyGroups {
+  int n;
+  SkinnyArgs g[kMaxGroups];
+};
+__device__ inline SkinnyArgs select_skinny(const SkinnyGroups& sg, int gi) {
+  SkinnyArgs p = sg.g[0];
+#pragma unroll
+  for (int k = 1; k < kMaxGroups; ++k)
+    if (k == gi) p = sg.g[k];
+  return p;
+}
+
+constexpr int kSkinnyShort = 64;    // longest short side
+constexpr int kSkinnyLong = 4096;   // shortest long side (short K) / K (long K)
+
+// 16 x 16 output block of one MFMA chain: short-side tile t against the
+// 16-wide long-side chunk; element r of lane (lr, lg) is (s, l) below.
+__device__ inline d4_t skinny_mfma(double sv, double lv, d4_t acc, int swap) {
+  return swap ? __builtin_amdgcn_mfma_f64_16x16x4f64(lv, sv, acc, 0, 0, 0)
+              : __builtin_amdgcn_mfma_f64_16x16x4f64(sv, lv, acc, 0, 0, 0);
+}
+__device__ inline int skinny_s(int t, int lr, int lg, int r, int swap) { return 16 * t + (swap ? lr : lg + 4 * r); }
+__device__ inline int skinny_l(int l0, int lr, int lg, int r, int swap) { return l0 + (swap ? lg + 4 * r : lr); }
+
+template <int TM>
+__global__ __launch_bounds__(256) void skinny_f64_shortk_kernel(SkinnyGroups sg) {
+  constexpr int KSM = 16 / TM;  // MFMA k steps held in registers
+  const SkinnyArgs p = select_skinny(sg, (int)blockIdx.y);
+  if (p.nchunks == 0) return;
+  const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
+  const int ks = (p.K + 3) >> 2;
+  double sf[TM][KSM];
+#pragma unroll
+  for (int t = 0; t < TM; ++t)
+#pragma unroll
+    for (int j = 0; j < KSM; ++j) {
+      const int s = 16 * t + lr, k = 4 * j + lg;
+      sf[t][j] = (s < p.ns && k < p.K) ? p.S[(long)s * p.ss + (long)k * p.sk] : 0.0;
+    }
+  const int nw = (int)gridDim.x * 4;
+  for (int c = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); c < p.nchunks; c += nw) {
+    const int l = c * 16 + lr;
+    double lf[KSM];
+#pragma unroll
+    for (int j = 0; j < KSM; ++j) {
+      const int k = 4 * j + lg;
+      lf[j] = (l < p.nl && k < p.K) ? p.L[(long)l * p.ls + (long)k * p.lk] : 0.0;
+    }
+    d4_t acc[TM];
+#pragma unroll
+    for (int t = 0; t < TM; ++t) acc[t] = d4_t{0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < KSM; ++j)
+      if (j < ks)
+#pragma unroll
+        for (int t = 0; t < TM; ++t) acc[t] = skinny_mfma(sf[t][j], lf[j], acc[t], p.swap);
+#pragma unroll
+    for (int t = 0; t < TM; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int so = skinny_s(t, lr, lg, r, p.swap), lo = skinny_l(c * 16, lr, lg, r, p.swap);
+        if (so < p.ns && lo < p.nl) {
+          double* o = p.C + (long)so * p.cs + (long)lo * p.cl;
+          const double v = p.alpha * acc[t][r];
+          *o = (p.beta == 0.0) ? v : v + p.beta * (*o);
+        }
+      }
+  }
+}
+
+// KC: both operands k-contiguous (sk = lk = 1), the k steps at fixed offsets.
+template <int TM, bool KC>
+__global__ __launch_bounds__(256) void skinny_f64_longk_kernel(SkinnyGroups sg) {
+  constexpr int KSTEP = (TM <= 2) ? 32 : 16;  // k per iteration
+  const SkinnyArgs p = select_skinny(sg, (int)blockIdx.y);
+  const long item = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (item >= (long)p.nchunks * p.splits) return;
+  const int c = (int)(item % p.nchunks), z = (int)(item / p.nchunks);
+  const long k0 = (long)z * p.kchunk;
+  const long k1 = min((long)p.K, k0 + p.kchunk);
+  const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
+  const int l = c * 16 + lr;
+  const bool lin = l < p.nl;
+  bool sok[TM];
+#pragma unroll
+  for (int t = 0; t < TM; ++t) sok[t] = 16 * t + lr < p.ns;
+  const long lk = KC ? 1 : p.lk, sk = KC ? 1 : p.sk;
+  // lane (lr, lg) reads k = kb + 4u + lg of its rows
+  const double* Lp = p.L + (long)(lin ? l : 0) * p.ls + (k0 + lg) * lk;
+  const double* Sp[TM];
+#pragma unroll
+  for (int t = 0; t < TM; ++t) Sp[t] = p.S + (long)(sok[t] ? 16 * t + lr : 0) * p.ss + (k0 + lg) * sk;
+  d4_t acc[TM];
+#pragma unroll
+  for (int t = 0; t < TM; ++t) acc[t] = d4_t{0, 0, 0, 0};
+  const long kend = k1 - k0 - lg;  // this lane's k offsets run below kend
+  for (long kb = 0; kb < k1 - k0; kb += KSTEP) {
+    double lv[KSTEP / 4], sv[TM][KSTEP / 4];
+#pragma unroll
+    for (int u = 0; u < KSTEP / 4; ++u) {
+      const bool ok = kb + 4 * u < kend;
+      lv[u] = (ok && lin) ? Lp[(kb + 4 * u) * lk] : 0.0;
+#pragma unroll
+      for (int t = 0; t < TM; ++t) sv[t][u] = (ok && sok[t]) ? Sp[t][(kb + 4 * u) * sk] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < KSTEP / 4; ++u)
+#pragma unroll
+      for (int t = 0; t < TM; ++t) acc[t] = skinny_mfma(sv[t][u], lv[u], acc[t], p.swap);
+  }
+  // several splits: partial sums (no beta) at C + z * zW; one split: C itself
+  double* W = p.C + (long)z * p.zW;
+  const bool fin = p.splits == 1 && p.beta != 0.0;
+#pragma unroll
+  for (int t = 0; t < TM; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int so = skinny_s(t, lr, lg, r, p.swap), lo = skinny_l(c * 16, lr, lg, r, p.swap);
+      if (so < p.ns && lo < p.nl) {
+        double* o = W + (long)so * p.cs + (long)lo * p.cl;
+        const double v = p.alpha * acc[t][r];
+        *o = fin ? v + p.beta * (*o) : v;
+      }
+    }
+}
+
+int gemm_skinny_kind(long M, long N, long K) {
+  if (M <= 0 || N <= 0 || K <= 0) return 3;  // empty blocks do not decide
+  const long sh = std::min(M, N), lo = std::max(M, N);
+  if (sh > kSkinnyShort) return 0;
+  const int tm = (int)((sh + 15) / 16);
+  if (K <= 4 * (16 / tm) && lo >= kSkinnyLong) return 1;
+  // long K with the short side at most 32: wider short sides re-read it for
+  // every 16-wide chunk of the other, where one 64 x 64 split-K tile reads both once
+  return (sh <= 32 && lo <= kSkinnyShort && K >= kSkinnyLong) ? 2 : 0;
+}
+
+// Skinny path for fp64 groups (batch 1, fp64 output).  *ran = 1 when it ran,
+// 0 when some nonempty group is not skinny (the tiled kernels take it).
+static int gemm_skinny(const GemmDesc* d, int ng, hipStream_t stream, int* ran) {
+  *ran = 0;
+  bool any = false, longk = false, shortk = false;
+  int tm = 1;
+  for (int g = 0; g < ng; ++g) {
+    const GemmDesc& e = d[g];
+    if (e.M == 0 || e.N == 0) continue;
+    const int kind = gemm_skinny_kind(e.M, e.N, e.K);
+    if (e.batch != 1 || e.C_hi != nullptr || kind == 0 || kind == 3) return ACES_OK;
+    const int sh = std::min(e.M, e.N);
+    tm = std::max(tm, (sh + 15) / 16);
+    const bool lk = kind == 2;
+    longk = longk || lk;
+    shortk = shortk || !lk;
+    any = true;
+  }
+  if (!any || (longk && shortk)) return ACES_OK;
+  if (shortk) {
+    for (int g = 0; g < ng; ++g)
+      if (d[g].M > 0 && d[g].N > 0 && d[g].K > 4 * (16 / tm)) return ACES_OK;
+  }
+  if (longk) {
+    // partial sums need the workspace: at least one full output per group
+    long need = 0;
+    for (int g = 0; g < ng; ++g) need += (long)d[g].M * d[g].N;
+    if (d[0].ws == nullptr || d[0].ws_elems < need) return ACES_OK;
+  }
+  SkinnyGroups sg;
+  sg.n = ng;
+  long maxitems = 0, ws_off = 0;
+  ReduceGroups rg;
+  rg.n = ng;
+  long most = 0;
+  int minsplit = 1 << 30;
+  bool kc_all = true;
+  for (int g = 0; g < kMaxGroups; ++g) {
+    SkinnyArgs& a = sg.g[g];
+    std::memset(&a, 0, sizeof(a));
+    rg.g[g].splits = 0;
+    if (g >= ng || d[g].M == 0 || d[g].N == 0) continue;
+    const GemmDesc& e = d[g];
+    const bool tr = e.N < e.M;  // short side on N: S = B, L = A
+    const long Ask = e.a_kc ? 1 : e.lda, Asm = e.a_kc ? e.lda : 1;
+    const long Bsk = e.b_kc ? 1 : e.ldb, Bsn = e.b_kc ? e.ldb : 1;
+    a.S = static_cast<const double*>(tr ? e.B : e.A);
+    a.ss = tr ? Bsn : Asm;
+    a.sk = tr ? Bsk : Ask;
+    a.L = static_cast<const double*>(tr ? e.A : e.B);
+    a.ls = tr ? Asm : Bsn;
+    a.lk = tr ? Ask : Bsk;
+    a.ns = tr ? e.N : e.M;
+    a.nl = tr ? e.M : e.N;
+    a.K = e.K;
+    a.nchunks = (a.nl + 15) / 16;
+    a.alpha = e.alpha;
+    a.beta = e.beta;
+    a.splits = 1;
+    a.kchunk = e.K;
+    a.C = e.C;
+    a.cs = tr ? 1 : e.ldc;
+    a.cl = tr ? e.ldc : 1;
+    a.swap = tr ? 1 : 0;
+    kc_all = kc_all && a.sk == 1 && a.lk == 1;
+    if (longk) {
+      // about 1024 waves per group, whole k steps per split; the groups after
+      // this one keep one output's worth of the workspace each
+      const long per = (long)e.M * e.N;
+      long later = 0;
+      for (int h = g + 1; h < ng; ++h) later += (long)d[h].M * d[h].N;
+      long sp = std::max<long>(1, 1024 / a.nchunks);
+      sp = std::min<long>(sp, (e.K + 31) / 32);
+      const long avail = d[0].ws_elems - ws_off - later;
+      if (sp * per > avail) sp = std::max<long>(1, avail / per);
+      long kc = (e.K + sp - 1) / sp;
+      kc = (kc + 31) / 32 * 32;
+      sp = (e.K + kc - 1) / kc;
+      a.splits = (int)sp;
+      a.kchunk = kc;
+      maxitems = std::max(maxitems, (long)a.nchunks * sp);
+      if (sp == 1) continue;  // the one split writes C itself
+      a.C = d[0].ws + ws_off;
+      a.zW = per;
+      a.cs = tr ? 1 : e.N;
+      a.cl = tr ? e.N : 1;
+      ws_off += sp * per;
+      ReduceGroups::G& q = rg.g[g];
+      q.M = e.M;
+      q.N = e.N;
+      q.splits = (int)sp;
+      q.ws = a.C;
+      q.zC = per;
+      q.C = e.C;
+      q.ldc = e.ldc;
+      q.beta = e.beta;
+      most = std::max(most, per);
+      minsplit = std::min(minsplit, (int)sp);
+    } else {
+      maxitems = std::max(maxitems, (long)a.nchunks);
+    }
+  }
+  for (int g = ng; g < kMaxGroups; ++g) sg.g[g] = sg.g[0];
+  if (longk) {
+    const dim3 grid((unsigned)((maxitems + 3) / 4), (unsigned)ng);
+#define SKL(T)                                                                          \
+  if (kc_all)                                                                            \
+    hipLaunchKernelGGL((skinny_f64_longk_kernel<T, true>), grid, dim3(256), 0, stream, sg); \
+  else                                                                                   \
+    hipLaunchKernelGGL((skinny_f64_longk_kernel<T, false>), grid, dim3(256), 0, stream, sg)
+    if (tm == 1) {
+      SKL(1);
+    } else if (tm == 2) {
+      SKL(2);
+    } else if (tm == 3) {
+      SKL(3);
+    } else {
+      SKL(4);
+    }
+#undef SKL
+    ACES_HIP(hipGetLastError());
+    for (int g = ng; g < kMaxGroups; ++g) rg.g[g].splits = 0;
+    if (most > 0) {
+      const int wide = (minsplit >= kWideSplits && most <= kWideMaxOutputs) ? 1 : 0;
+      long gx = wide ? (most + 3) / 4 : (most + 255) / 256;
+      if (!wide && gx > 65536) gx = 65536;
+      hipLaunchKernelGGL(splitk_reduce_groups_kernel, dim3((unsigned)gx, (unsigned)kMaxGroups), dim3(256), 0,
+                         stream, rg, wide);
+    }
+  } else {
+    // enough waves to fill the chip a few times over, each walking chunks
+    const long blocks = std::min<long>((maxitems + 3) / 4, std::max<long>(1, 4096 / ng));
+    const dim3 grid((unsigned)blocks, (unsigned)ng);
+#define SKS(T) hipLaunchKernelGGL(skinny_f64_shortk_kernel<T>, grid, dim3(256), 0, stream, sg)
+    if (tm == 1) SKS(1); else if (tm == 2) SKS(2); else if (tm == 3) SKS(3); else SKS(4);
+#undef SKS
+  }
+  ACES_HIP(hipGetLastError());
+  *ran = 1;
+  return ACES_OK;
+}
+
 static int gemm_one(const GemmDesc& d, GemmPrec prec, hipStream_t stream);
 
 // Compute units of the current device (one value per process: one GPU).
@@ -1152,6 +1458,11 @@ static int device_cus() {
 int gemm(const GemmDesc& d, GemmPrec prec, hipStream_t stream) {
   ACES_CHECK(check_desc(d, prec));
   if (d.M == 0 || d.N == 0) return ACES_OK;
+  if (prec == GP_F64) {
+    int ran = 0;
+    ACES_CHECK(gemm_skinny(&d, 1, stream, &ran));
+    if (ran) return ACES_OK;
+  }
   if (d.batch == 1 && d.ws != nullptr && d.K >= 1024) {
     const int wm = gemm_shape(&d, 1, prec);
     GemmArgs probe;
@@ -1255,6 +1566,11 @@ int gemm_grouped(const GemmDesc* d, int ng, GemmPrec prec, hipStream_t stream) {
   }
   for (int g = ng; g < kMaxGroups; ++g) gg.g[g] = gg.g[0];
   if (!any) return ACES_OK;
+  if (prec == GP_F64) {
+    int ran = 0;
+    ACES_CHECK(gemm_skinny(d, ng, stream, &ran));
+    if (ran) return ACES_OK;
+  }
   ACES_REQUIRE(maxnb < (1L << 31), "gemm_grouped: grid too large");
   int kz = 0;
   for (int g = 0; g < ng; ++g)

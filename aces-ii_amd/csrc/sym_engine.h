@@ -142,6 +142,7 @@ class SymEngine {
   int copy64(int t, LowBuf& lb);
   int padded_f64(int t);
   bool f64_wide(const Op& op) const;
+  bool skinny(const Op& op) const;
   int pack_dense(int t, const double* dense, const long* strides, long base_off);
   int unpack_dense(int t, const double* packed, double* dense, const long* strides, long base_off);
   int denom_apply(int t, double* out, double beta, const double* in, double alpha, const double* in2,

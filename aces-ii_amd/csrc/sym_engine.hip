@@ -621,6 +621,21 @@ bool SymEngine::f64_wide(const Op& op) const {
   return any;
 }
 
+// Products with one output side at most 64 long and short K stream their
+// operands; in fp64 they take less time than the bf16x3 tiles, which would
+// idle most of their MFMA lanes.  (Long-K skinny products keep bf16x3: there
+// the split operands halve the bytes streamed.)
+bool SymEngine::skinny(const Op& op) const {
+  bool any = false;
+  for (int g = 0; g < nirr; ++g) {
+    const long K = op.a_kc ? T_[op.A].ncol[g] : T_[op.A].nrow[g];
+    const int kind = gemm_skinny_kind(T_[op.out].nrow[g], T_[op.out].ncol[g], K);
+    if (kind != 1 && kind != 3) return false;
+    any = any || kind == 1;
+  }
+  return any;
+}
+
 // Large fp64 products run on padded copies of their operands: block widths
 // are arbitrary, the direct-to-LDS kernel needs K % 16 == 0 and even leading
 // dimensions.  A copy is remade when its tensor has been rewritten.
@@ -923,7 +938,7 @@ int SymEngine::forward(GemmPrec prec) {
       case OP_GEMM: {
         if (refine && op.lowp && op.cat <= 2) {
           ACES_CHECK(gemm_anchored(k, mode));
-        } else if (op.lowp && (prec == GP_BF16X3 || (prec == GP_F32 && op.a_kc && op.b_kc))) {
+        } else if (op.lowp && (prec == GP_BF16X3 || (prec == GP_F32 && op.a_kc && op.b_kc)) && !skinny(op)) {
           // k-contiguous operands split as stored, row-contiguous ones turned
           // round (bf16x3); either split is reused until the tensor is rewritten
           const LowBuf* lab[2];

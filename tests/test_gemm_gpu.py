@@ -265,3 +265,56 @@ def test_sparse_last_round(prec, a_kc, b_kc, M, N, K):
         _grouped(0, [{'M': M, 'N': N, 'K': K, 'A': A, 'B': B, 'C': C}], a_kc, b_kc, 0.75, -0.5, wsb)
         torch.cuda.synchronize()
         assert torch.allclose(C, ref, rtol=1e-12, atol=1e-12 * K ** 0.5)
+
+
+def _skinny_groups(shapes, a_kc, b_kc, seed):
+    g = torch.Generator(device='cuda').manual_seed(seed)
+    groups, refs = [], []
+    for (M, N, K) in shapes:
+        Am = torch.randn(M, K, dtype=torch.float64, device='cuda', generator=g)
+        Bm = torch.randn(N, K, dtype=torch.float64, device='cuda', generator=g)
+        C0 = torch.randn(M, N, dtype=torch.float64, device='cuda', generator=g)
+        A = Am.contiguous() if a_kc else Am.t().contiguous()
+        B = Bm.contiguous() if b_kc else Bm.t().contiguous()
+        if M == 0:
+            A = torch.zeros(1, max(K, 1), dtype=torch.float64, device='cuda')
+        groups.append({'M': M, 'N': N, 'K': K, 'A': A, 'B': B, 'C': C0.clone()})
+        refs.append((Am, Bm, C0))
+    return groups, refs
+
+
+@pytest.mark.parametrize('a_kc,b_kc', [(1, 1), (1, 0), (0, 1), (0, 0)])
+@pytest.mark.parametrize('shapes', [
+    [(20000, 6, 49)], [(6, 20000, 12)], [(5000, 49, 6)], [(49, 5000, 12)], [(4100, 33, 20)], [(4097, 17, 32)],
+    [(20000, 6, 49), (0, 5, 40), (19000, 11, 40), (21000, 12, 46), (4096, 1, 1)],
+    [(6, 6, 50000)], [(6, 49, 20000)], [(49, 49, 9000)], [(64, 64, 4096)], [(1, 33, 4099)], [(30, 64, 5000)],
+    [(64, 20, 4100)],
+    [(6, 6, 50000), (11, 11, 40000), (0, 3, 4096), (5, 57, 30001)]])
+@pytest.mark.parametrize('beta', [0.0, 1.25])
+def test_f64_skinny(a_kc, b_kc, shapes, beta):
+    """Skinny fp64 products (one output side <= 64 with short K, or sides <= 32
+    and <= 64 with long K) on the operand-streaming kernels, grouped with ragged
+    and empty blocks, every operand order, against torch fp64 (49 x 49 and
+    64 x 64 long-K outputs take the tiled split-K kernel)."""
+    print('case f64 skinny', a_kc, b_kc, shapes, beta, flush=True)
+    groups, refs = _skinny_groups(shapes, a_kc, b_kc, 17 + len(shapes) + 2 * a_kc + b_kc)
+    ws = torch.empty(1 << 22, dtype=torch.float64, device='cuda')
+    _grouped(0, groups, a_kc, b_kc, -0.75, beta, ws)
+    torch.cuda.synchronize()
+    for gr, (Am, Bm, C0) in zip(groups, refs):
+        ref = -0.75 * (Am @ Bm.t()) + beta * C0
+        assert torch.allclose(gr['C'], ref, rtol=1e-12, atol=1e-12 * (gr['K'] ** 0.5 + 1))
+
+
+@pytest.mark.parametrize('ws_elems', [0, 40, 1 << 10])
+def test_f64_skinny_small_workspace(ws_elems):
+    """Long-K skinny products with no workspace (tiled kernels), one too small
+    for the partial sums of every group, and one that holds a few splits."""
+    print('case f64 skinny ws', ws_elems, flush=True)
+    groups, refs = _skinny_groups([(6, 6, 50000), (5, 7, 40000)], 1, 1, 3)
+    ws = torch.empty(ws_elems, dtype=torch.float64, device='cuda') if ws_elems else None
+    _grouped(0, groups, 1, 1, 1.0, 0.5, ws)
+    torch.cuda.synchronize()
+    for gr, (Am, Bm, C0) in zip(groups, refs):
+        ref = Am @ Bm.t() + 0.5 * C0
+        assert torch.allclose(gr['C'], ref, rtol=1e-12, atol=1e-12 * (gr['K'] ** 0.5 + 1))
