@@ -621,17 +621,20 @@ bool SymEngine::f64_wide(const Op& op) const {
   return any;
 }
 
-// Products with one output side at most 64 long and short K stream their
-// operands; in fp64 they take less time than the bf16x3 tiles, which would
-// idle most of their MFMA lanes.  (Long-K skinny products keep bf16x3: there
-// the split operands halve the bytes streamed.)
+// Products with one output side at most 64 long and short K, or outputs of
+// at most 16 x 16 with long K, stream their operands; in fp64 they take less
+// time than the bf16x3 tiles, which would idle most of their MFMA lanes.
+// (Wider long-K outputs keep bf16x3: there the split operands halve the bytes
+// streamed and fill more of each tile.)
 bool SymEngine::skinny(const Op& op) const {
   bool any = false;
   for (int g = 0; g < nirr; ++g) {
+    const long M = T_[op.out].nrow[g], N = T_[op.out].ncol[g];
     const long K = op.a_kc ? T_[op.A].ncol[g] : T_[op.A].nrow[g];
-    const int kind = gemm_skinny_kind(T_[op.out].nrow[g], T_[op.out].ncol[g], K);
-    if (kind != 1 && kind != 3) return false;
-    any = any || kind == 1;
+    const int kind = gemm_skinny_kind(M, N, K);
+    if (kind == 3) continue;
+    if (kind != 1 && !(kind == 2 && M <= 16 && N <= 16)) return false;
+    any = true;
   }
   return any;
 }
