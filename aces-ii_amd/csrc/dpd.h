@@ -182,6 +182,9 @@ struct Box {
   int te[3], tn[3];
   unsigned mg[2];
   long ob, os[4];
+  // the same output elements in the bf16 split image of the output (rows of
+  // each irrep block padded to a multiple of 32 columns), Plan::low only
+  long lob, los[4];
   long sb[3], ss[3][4];
   long xb, xs[4], yb, ys[4];  // xb < 0: product term vanishes in this box
 };
@@ -222,6 +225,8 @@ struct Plan {
   // element offsets relative to its first element need 64 bits
   const int* tbox = nullptr;
   bool wide = false;
+  // Box::lob/los hold the split-image map (blocked rank-4 outputs)
+  bool low = false;
 };
 
 // Source description for plan building: layout and, for each source
@@ -256,6 +261,14 @@ struct Denom {
 
 int apply_plan(const Plan& p, double* out, double beta, const double* const* src, const double* alpha,
                const double* X, const double* Y, double gamma, const Denom* den, hipStream_t s);
+// The same, also writing every output value's bf16 split (hi = bf16(v), lo =
+// bf16(v - hi)) to its place in the split image hi/lo: each irrep block G at
+// element offset sum_{g<G} nrow[g] * pad32(ncol[g]), rows padded to
+// pad32(ncol[G]) columns (the pad columns are left as they are).  Tiled plans
+// with Plan::low only.
+int apply_plan_split(const Plan& p, double* out, double beta, const double* const* src, const double* alpha,
+                     const double* X, const double* Y, double gamma, const Denom* den, unsigned short* hi,
+                     unsigned short* lo, hipStream_t s);
 
 // Per-row sums of every block of a layout: out[r] for the rows in block
 // order (for rank-2 (O|O) this is the orbital order).

@@ -3,6 +3,7 @@
 #include <cstring>
 #include "common.h"
 #include "dpd.h"
+#include "bf16_util.h"
 
 namespace aces {
 namespace dpd {
@@ -224,6 +225,7 @@ static void plan_tiles(Plan& p, int rank) {
     long m = span(b.os, b);
     for (int s = 0; s < p.nsrc; ++s) m = std::max(m, span(b.ss[s], b));
     if (p.prod) m = std::max(m, std::max(span(b.xs, b), span(b.ys, b)));
+    if (p.low) m = std::max(m, span(b.los, b));
     if (m >= (1L << 31)) p.wide = true;
   }
 }
@@ -237,6 +239,11 @@ int build_boxes(Plan& p, const Layout& out, const std::vector<Src>& srcs, const 
   p.prod = (X != nullptr);
   for (const Src& s : srcs) ACES_REQUIRE(s.L->rank == out.rank, "dpd plan: source rank differs from output");
   long first = 0;
+  // split image of a blocked rank-4 output: block offsets with padded rows
+  const bool low = out.rank == 4 && !out.dense && out.split == 2;
+  long loff[kMaxIrr + 1] = {0};
+  for (int G = 0; G < nirr; ++G) loff[G + 1] = loff[G] + out.nrow[G] * ((std::max<long>(out.ncol[G], 1) + 31) / 32 * 32);
+  p.low = low;
   auto add_box = [&](const int* g) {
     Box b;
     std::memset(&b, 0, sizeof(b));
@@ -250,6 +257,14 @@ int build_boxes(Plan& p, const Layout& out, const std::vector<Src>& srcs, const 
     long st[4] = {0, 0, 0, 0};
     out.affine(g, b.ob, st);
     for (int k = 0; k < 4; ++k) b.os[k] = st[k];
+    if (low) {
+      int G = 0;
+      for (int k = 0; k < out.split; ++k) G ^= g[k];
+      const long nc = out.ncol[G], ld = (std::max<long>(nc, 1) + 31) / 32 * 32;
+      const long rel = b.ob - out.blk_off[G];
+      b.lob = loff[G] + rel / nc * ld + rel % nc;
+      for (int k = 0; k < 4; ++k) b.los[k] = (k < out.split) ? st[k] / nc * ld : st[k];
+    }
     for (size_t s = 0; s < srcs.size(); ++s) {
       const Src& sr = srcs[s];
       int gs[4];
@@ -337,6 +352,8 @@ struct ApplyArgs {
   const double* Y;
   double gamma;
   Denom den;
+  unsigned short* lhi;  // split image of the output (tiled walk only), or null
+  unsigned short* llo;
 };
 
 __global__ __launch_bounds__(256) void apply_plan_kernel(ApplyArgs a) {
@@ -452,6 +469,16 @@ __global__ __launch_bounds__(256) void apply_plan_tiled_kernel(ApplyArgs a, int 
   const Off os3 = (Off)b.os[3], osq = (Off)b.os[tq], osr = (Off)b.os[rb];
   double* obase = a.out + b.ob + ia * b.os[ra] + (long)i3b * os3 + (long)iqb * osq + (long)irb * osr;
   const bool prod = a.X != nullptr && b.xb >= 0;
+  const bool lsplit = a.lhi != nullptr;
+  unsigned short* lhi = a.lhi;
+  unsigned short* llo = a.llo;
+  Off ls3 = 0, lsq = 0, lsr = 0;
+  if (lsplit) {
+    const long lb = b.lob + ia * b.los[ra] + (long)i3b * b.los[3] + (long)iqb * b.los[tq] + (long)irb * b.los[rb];
+    lhi += lb;
+    llo += lb;
+    ls3 = (Off)b.los[3], lsq = (Off)b.los[tq], lsr = (Off)b.los[rb];
+  }
   const double* xbase = a.X;
   const double* ybase = a.Y;
   Off xs3 = 0, xsq = 0, xsr = 0, ys3 = 0, ysq = 0, ysr = 0;
@@ -487,15 +514,31 @@ __global__ __launch_bounds__(256) void apply_plan_tiled_kernel(ApplyArgs a, int 
     }
     if (prod) v += a.gamma * xbase[c3 * xs3 + q * xsq + r * xsr] * ybase[c3 * ys3 + q * ysq + r * ysr];
     double* o = obase + c3 * os3 + q * osq + r * osr;
-    *o = (a.beta == 0.0) ? v : a.beta * (*o) + v;
+    if (a.beta != 0.0) v += a.beta * (*o);
+    *o = v;
+    if (lsplit) {
+      const Off lo = c3 * ls3 + q * lsq + r * lsr;
+      const unsigned short h = bf16_rn((float)v);
+      lhi[lo] = h;
+      llo[lo] = bf16_rn((float)(v - (double)bf16_to_f(h)));
+    }
   }
 }
 
 int apply_plan(const Plan& p, double* out, double beta, const double* const* src, const double* alpha,
                const double* X, const double* Y, double gamma, const Denom* den, hipStream_t s) {
+  return apply_plan_split(p, out, beta, src, alpha, X, Y, gamma, den, nullptr, nullptr, s);
+}
+
+int apply_plan_split(const Plan& p, double* out, double beta, const double* const* src, const double* alpha,
+                     const double* X, const double* Y, double gamma, const Denom* den, unsigned short* hi,
+                     unsigned short* lo, hipStream_t s) {
+  ACES_REQUIRE(hi == nullptr || (lo != nullptr && p.low && p.tq >= 0), "dpd apply: split image needs a tiled plan");
   if (p.nbox == 0 || p.total == 0) return ACES_OK;
   ApplyArgs a;
   std::memset(&a, 0, sizeof(a));
+  a.lhi = hi;
+  a.llo = lo;
   a.boxes = p.dev;
   a.nbox = p.nbox;
   a.out = out;

@@ -143,6 +143,9 @@ class SymEngine {
   int padded_f64(int t);
   bool f64_wide(const Op& op) const;
   bool skinny(const Op& op) const;
+  int low_alloc(int t, LowBuf& lb);
+  void plan_split_fusion();
+  std::vector<char> fuse_split_;  // per op: the permute also writes its output's bf16x3 split
   int pack_dense(int t, const double* dense, const long* strides, long base_off);
   int unpack_dense(int t, const double* packed, double* dense, const long* strides, long base_off);
   int denom_apply(int t, double* out, double beta, const double* in, double alpha, const double* in2,

@@ -532,22 +532,58 @@ void SymEngine::profile(double* out, int reset) {
     for (int i = 0; i < 15; ++i) prof_[i] = 0.0;
 }
 
+// Split image of tensor t: each irrep block's rows padded to a multiple of
+// 32 columns, 4 bytes per element (fp32, or bf16 hi then bf16 lo); the pad
+// columns start at zero.
+int SymEngine::low_alloc(int t, LowBuf& lb) {
+  if (lb.hi != nullptr) return ACES_OK;
+  const Tensor& tt = T_[t];
+  long off = 0;
+  for (int g = 0; g < nirr; ++g) {
+    lb.ld[g] = pad32(std::max<long>(tt.ncol[g], 1));
+    lb.off[g] = off;
+    off += tt.nrow[g] * lb.ld[g];
+  }
+  lb.off[nirr] = off;
+  double* p;
+  const long words = (off * 4 + 7) / 8 + 8;
+  ACES_CHECK(alloc(&p, words));
+  ACES_HIP(hipMemsetAsync(p, 0, sizeof(double) * words, s_));
+  lb.hi = p;
+  lb.lo = (char*)p + off * 2;
+  return ACES_OK;
+}
+
+// Permutes whose output is, before it is written again, a k-contiguous
+// operand of a bf16x3 product write that operand's split as they go (no
+// separate split pass; reads in between leave the split valid).
+void SymEngine::plan_split_fusion() {
+  fuse_split_.assign(ops_.size(), 0);
+  auto writes = [&](const Op& q, int t) {
+    if (q.out == t) return true;
+    if (q.kind == OP_PACK) return t == Sp_ || t == Dp_;
+    if (q.kind == OP_UNPACK) return t == inc2_ || t == Yb_;
+    return false;
+  };
+  for (size_t k = 0; k < ops_.size(); ++k) {
+    const Op& op = ops_[k];
+    if (op.kind != OP_PERM || op.fwd.tq < 0 || !op.fwd.low || T_[op.out].L.rank != 4) continue;
+    const int t = op.out;
+    for (size_t j = k + 1; j < ops_.size(); ++j) {
+      const Op& q = ops_[j];
+      if (writes(q, t)) break;
+      if (q.kind == OP_GEMM && q.lowp && !skinny(q) && ((q.A == t && q.a_kc) || (q.B == t && q.b_kc))) {
+        fuse_split_[k] = 1;
+        break;
+      }
+    }
+  }
+}
+
 int SymEngine::lowp_convert(int t, GemmPrec prec, LowBuf& lb, const double* sub) {
   ACES_REQUIRE(sub == nullptr || prec == GP_BF16X3, "sym: difference splits are bf16x3");
   const Tensor& tt = T_[t];
-  if (lb.hi == nullptr) {
-    long off = 0;
-    for (int g = 0; g < nirr; ++g) {
-      lb.ld[g] = pad32(std::max<long>(tt.ncol[g], 1));
-      lb.off[g] = off;
-      off += tt.nrow[g] * lb.ld[g];
-    }
-    lb.off[nirr] = off;
-    double* p;
-    ACES_CHECK(alloc(&p, (off * 4 + 7) / 8 + 8));  // 4 bytes per element (fp32, or bf16 hi + lo)
-    lb.hi = p;
-    lb.lo = (char*)p + off * 2;
-  }
+  ACES_CHECK(low_alloc(t, lb));
   // every block of the tensor in one launch
   SplitBlocks sb;
   sb.f32 = (prec == GP_F32);
@@ -928,14 +964,27 @@ int SymEngine::forward(GemmPrec prec) {
   // is reused until the tensor is written again
   for (Tensor& t : T_)
     if (t.grad) ++t.ver;
+  if (fuse_split_.size() != ops_.size()) plan_split_fusion();
+  low_.resize(T_.size());
   for (size_t k = 0; k < ops_.size(); ++k) {
     Op& op = ops_[k];
+    int split_done = -1;  // tensor whose bf16x3 split this op wrote
     switch (op.kind) {
       case OP_PERM: {
         const double* sp[3] = {nullptr, nullptr, nullptr};
         for (int s = 0; s < op.nsrc; ++s) sp[s] = T_[op.src[s]].d;
-        ACES_CHECK(run_perm(op, T_[op.out].d, sp, op.X >= 0 ? T_[op.X].d : nullptr,
-                            op.X >= 0 ? T_[op.Y].d : nullptr));
+        const double* X = op.X >= 0 ? T_[op.X].d : nullptr;
+        const double* Y = op.X >= 0 ? T_[op.Y].d : nullptr;
+        if (prec == GP_BF16X3 && mode == 0 && fuse_split_[k]) {
+          LowBuf& lb = low_[op.out];
+          ACES_CHECK(low_alloc(op.out, lb));
+          ACES_CHECK(dpd::apply_plan_split(op.fwd, T_[op.out].d, op.beta, sp, op.alpha, X, Y, op.gamma, nullptr,
+                                           static_cast<unsigned short*>(lb.hi),
+                                           static_cast<unsigned short*>(lb.lo), s_));
+          split_done = op.out;
+        } else {
+          ACES_CHECK(run_perm(op, T_[op.out].d, sp, X, Y));
+        }
         break;
       }
       case OP_GEMM: {
@@ -988,6 +1037,12 @@ int SymEngine::forward(GemmPrec prec) {
         break;
     }
     if (op.out >= 0) ++T_[op.out].ver;
+    if (split_done >= 0) {
+      LowBuf& lb = low_[split_done];
+      lb.prec = GP_BF16X3;
+      lb.valid = true;
+      lb.ver = T_[split_done].ver;
+    }
     if (op.kind == OP_PACK) {
       ++T_[Sp_].ver;
       ++T_[Dp_].ver;

@@ -100,6 +100,28 @@ def test_lowp_increments(prec, tol):
     assert float((a1 - b1).abs().max()) < tol * sc
 
 
+def test_lowp_increments_tiled():
+    """bf16x3 increments at a size where the permutes run tiled and write the
+    bf16x3 operand splits of the ring products as they go: against fp64, and
+    the same twice in a row."""
+    from aces2.ccsd import SymCCSD
+    occ, vir = [4, 3, 4, 5], [20, 18, 22, 19]
+    print('case sym lowp tiled', flush=True)
+    o, v = sum(occ), sum(vir)
+    f, eri, blk, irr = _system(occ, vir, 31)
+    t1, t2 = _rand_sym(irr, o, v, 32)
+    cs = SymCCSD(occ, vir, blk)
+    cs.set_amplitudes(t1, t2)
+    a1, a2 = cs.increments('fp64')
+    b1, b2 = cs.increments('bf16x3')
+    c1, c2 = cs.increments('bf16x3')
+    cs.close()
+    sc = float(a2.abs().max())
+    assert float((a2 - b2).abs().max()) < 5e-5 * sc
+    assert float((a1 - b1).abs().max()) < 5e-5 * sc
+    assert float((b2 - c2).abs().max()) == 0.0 and float((b1 - c1).abs().max()) == 0.0
+
+
 @pytest.mark.parametrize('occ,vir,seed,nonhf', [CASES[1], CASES[3]])
 def test_adjoint_matches_oracle(occ, vir, seed, nonhf):
     from aces2.ccsd import SymCCSD
