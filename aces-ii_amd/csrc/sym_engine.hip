@@ -1465,12 +1465,15 @@ int SymEngine::setup(const double* fock, const double* ovov_dense, const double*
   ACES_CHECK(gemm_op(P2, kacd, 1, t1_, 1, 1, 0, 3, true));
   ACES_CHECK(gemm_op(Q12, M5c, 1, t1_, 0, 1, 0, 3, false));
   ACES_CHECK(gemm_op(Q2, M5b, 1, t1_, 0, 1, 0, 3, false));
+  // the T2 products first and the permutes accumulating onto them, so that
+  // the last writer of Wt and Wv is a permute (which can write the ring
+  // products' operand splits as it goes)
   const int Wt = dy(LOVOV), Wv = dy(LOVOV);
-  ACES_CHECK(perm(Wt, 0, {{W0t, I4}, {P1, {2, 3, 1, 0}}, {P2, {2, 1, 3, 0}}}, {1, 2, -1}));
+  ACES_CHECK(gemm_op(Wt, Ut, 1, Lmat, 1, 0.5, 0, 1, true));
+  ACES_CHECK(gemm_op(Wv, Xt, 1, Kx, 1, -0.5, 0, 1, true));
+  ACES_CHECK(perm(Wt, 1, {{W0t, I4}, {P1, {2, 3, 1, 0}}, {P2, {2, 1, 3, 0}}}, {1, 2, -1}));
   ACES_CHECK(perm(Wt, 1, {{Q12, {2, 3, 0, 1}}}, {-1}));
-  ACES_CHECK(perm(Wv, 0, {{Wv0, I4}, {P2, {2, 1, 3, 0}}, {Q2, {2, 3, 0, 1}}}, {1, 1, -1}));
-  ACES_CHECK(gemm_op(Wt, Ut, 1, Lmat, 1, 0.5, 1, 1, true));
-  ACES_CHECK(gemm_op(Wv, Xt, 1, Kx, 1, -0.5, 1, 1, true));
+  ACES_CHECK(perm(Wv, 1, {{Wv0, I4}, {P2, {2, 1, 3, 0}}, {Q2, {2, 3, 0, 1}}}, {1, 1, -1}));
   const int R1 = dy(LOVOV), R2 = dy(LOVOV);
   ACES_CHECK(gemm_op(R1, Wt, 1, U, 1, 1, 0, 2, true));
   ACES_CHECK(gemm_op(R2, Wv, 1, X, 1, 1, 0, 2, true));
