@@ -194,6 +194,8 @@ constexpr long kChunk = 4096;  // elements per workgroup in box walks
 // thread of its 256 takes at most kTileRows output elements, and the tile's
 // LDS image (pitch tq-extent | 1) fits kTileLds doubles.
 constexpr int kTileSpan = 64, kTileRows = 8, kTileLds = 2304;
+// turned sources: LDS-image loads per thread issued together
+constexpr int kReadBatch = 8;
 
 // Box of workgroup `chunk` (binary search over Box::first).
 __device__ inline int box_of_chunk(const Box* boxes, int nbox, long chunk) {

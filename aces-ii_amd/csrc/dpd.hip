@@ -430,18 +430,33 @@ __global__ __launch_bounds__(256) void apply_plan_tiled_kernel(ApplyArgs a, int 
     const double al = a.alpha[s];
     const Off s3 = (Off)st[3], sq = (Off)st[tq], sr = (Off)st[rb];
     const double* base = a.src[s] + b.sb[s] + ia * st[ra] + (long)i3b * s3 + (long)iqb * sq + (long)irb * sr;
+    // Loads are branch-free (an element outside the tile reads the tile's
+    // first element, which exists, and is then dropped), so that a thread's
+    // loads are all in flight together.
     if ((tr >> s) & 1) {
       // read along q: thread (q, jq) takes rows jq, jq + RPq, ... (row = c3 + T3 r)
       const int RPq = 256 / TQ;
       const int jq = div_by(tid, TQ, mq);
       const int qq = tid - jq * TQ;
+      const int n = T3 * TR;
       __syncthreads();  // the previous reader of the tile is done
       if (jq < RPq && qq < eq) {
-#pragma unroll 4
-        for (int j = jq; j < T3 * TR; j += RPq) {
-          const int r = div_by(j, T3, m3);
-          const int cc = j - r * T3;
-          tile[j * P + qq] = (cc < e3 && r < er) ? base[qq * sq + cc * s3 + r * sr] : 0.0;
+        for (int jb = jq; jb < n; jb += kReadBatch * RPq) {
+          double v[kReadBatch];
+#pragma unroll
+          for (int u = 0; u < kReadBatch; ++u) {
+            const int j = jb + u * RPq;
+            const int r = div_by(j, T3, m3);
+            const int cc = j - r * T3;
+            const bool ok = j < n && cc < e3 && r < er;
+            v[u] = base[ok ? qq * sq + cc * s3 + r * sr : 0];
+            if (!ok) v[u] = 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < kReadBatch; ++u) {
+            const int j = jb + u * RPq;
+            if (j < n) tile[j * P + qq] = v[u];
+          }
         }
       }
       __syncthreads();
@@ -455,15 +470,18 @@ __global__ __launch_bounds__(256) void apply_plan_tiled_kernel(ApplyArgs a, int 
         }
       }
     } else {
+      double v[kTileRows];
 #pragma unroll
       for (int k = 0; k < kTileRows; ++k) {
         const int j = j0 + RP * k;
-        if (act && j < R) {
-          const int r = div_by(j, TQ, mq);
-          const int q = j - r * TQ;
-          if (q < eq && r < er) acc[k] += al * base[c3 * s3 + q * sq + r * sr];
-        }
+        const int r = div_by(j, TQ, mq);
+        const int q = j - r * TQ;
+        const bool ok = act && j < R && q < eq && r < er;
+        v[k] = base[ok ? c3 * s3 + q * sq + r * sr : 0];
+        if (!ok) v[k] = 0.0;
       }
+#pragma unroll
+      for (int k = 0; k < kTileRows; ++k) acc[k] += al * v[k];
     }
   }
   const Off os3 = (Off)b.os[3], osq = (Off)b.os[tq], osr = (Off)b.os[rb];
@@ -496,6 +514,19 @@ __global__ __launch_bounds__(256) void apply_plan_tiled_kernel(ApplyArgs a, int 
       if (a.den.sign[m] != 0 && (m == 3 || m == ra))
         Dfix += a.den.sign[m] * a.den.e[m][a.den.off[m][b.g[m]] + (m == 3 ? i3b + c3 : ia)];
   }
+  // the old output values and the product terms, loaded together
+  double prev[kTileRows], xy[kTileRows];
+#pragma unroll
+  for (int k = 0; k < kTileRows; ++k) {
+    const int j = j0 + RP * k;
+    const int r = div_by(j, TQ, mq);
+    const int q = j - r * TQ;
+    const bool ok = act && j < R && q < eq && r < er;
+    prev[k] = 0.0;
+    xy[k] = 0.0;
+    if (a.beta != 0.0) prev[k] = obase[ok ? c3 * os3 + q * osq + r * osr : 0];
+    if (prod) xy[k] = xbase[ok ? c3 * xs3 + q * xsq + r * xsr : 0] * ybase[ok ? c3 * ys3 + q * ysq + r * ysr : 0];
+  }
 #pragma unroll
   for (int k = 0; k < kTileRows; ++k) {
     const int j = j0 + RP * k;
@@ -512,9 +543,9 @@ __global__ __launch_bounds__(256) void apply_plan_tiled_kernel(ApplyArgs a, int 
           D += a.den.sign[m] * a.den.e[m][a.den.off[m][b.g[m]] + (m == tq ? iqb + q : irb + r)];
       v = (a.den.pow > 0) ? v * D : v / D;
     }
-    if (prod) v += a.gamma * xbase[c3 * xs3 + q * xsq + r * xsr] * ybase[c3 * ys3 + q * ysq + r * ysr];
+    if (prod) v += a.gamma * xy[k];
     double* o = obase + c3 * os3 + q * osq + r * osr;
-    if (a.beta != 0.0) v += a.beta * (*o);
+    if (a.beta != 0.0) v += a.beta * prev[k];
     *o = v;
     if (lsplit) {
       const Off lo = c3 * ls3 + q * lsq + r * lsr;
