@@ -270,48 +270,71 @@ __global__ __launch_bounds__(256) void permute4_tiled_kernel(Permute4Tile p) {
     const long ia = b / p.d[p.rb];
     const long i3b = t3 * kPT3, iqb = tq * kPTQ;
     const bool ok3 = i3b + c3 < d3;
+    // Loads are branch-free: an element outside the array reads the tile's
+    // first element (which exists) and is dropped, so that a thread's loads
+    // are in flight together.
+    bool ok[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) ok[k] = ok3 && iqb + r0 + 4 * k < dq;
     double acc[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) acc[k] = 0.0;
     for (int s = 0; s < p.nd; ++s) {
       const long* st = p.st[s];
-      const double* src = p.in[s] + ia * st[p.ra] + ib * st[p.rb] + (iqb + r0) * st[p.q] + (i3b + c3) * st[3];
-      const long step = 4 * st[p.q];
+      const double* src = p.in[s] + ia * st[p.ra] + ib * st[p.rb] + iqb * st[p.q] + i3b * st[3];
+      const long off = r0 * st[p.q] + c3 * st[3], step = 4 * st[p.q];
       const double al = p.alpha[s];
+      double v[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (ok3 && iqb + r0 + 4 * k < dq) acc[k] += al * src[k * step];
+      for (int k = 0; k < 8; ++k) v[k] = src[ok[k] ? off + k * step : 0];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] += ok[k] ? al * v[k] : 0.0;
     }
     if (p.X) {
-      const double* xp = p.X + ia * p.xs[p.ra] + ib * p.xs[p.rb] + (iqb + r0) * p.xs[p.q] + (i3b + c3) * p.xs[3];
-      const double* yp = p.Y + ia * p.ys[p.ra] + ib * p.ys[p.rb] + (iqb + r0) * p.ys[p.q] + (i3b + c3) * p.ys[3];
+      const double* xp = p.X + ia * p.xs[p.ra] + ib * p.xs[p.rb] + iqb * p.xs[p.q] + i3b * p.xs[3];
+      const double* yp = p.Y + ia * p.ys[p.ra] + ib * p.ys[p.rb] + iqb * p.ys[p.q] + i3b * p.ys[3];
+      const long xoff = r0 * p.xs[p.q] + c3 * p.xs[3], yoff = r0 * p.ys[p.q] + c3 * p.ys[3];
       const long xstep = 4 * p.xs[p.q], ystep = 4 * p.ys[p.q];
+      double xv[8], yv[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (ok3 && iqb + r0 + 4 * k < dq) acc[k] += p.gamma * xp[k * xstep] * yp[k * ystep];
+      for (int k = 0; k < 8; ++k) {
+        xv[k] = xp[ok[k] ? xoff + k * xstep : 0];
+        yv[k] = yp[ok[k] ? yoff + k * ystep : 0];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] += ok[k] ? p.gamma * xv[k] * yv[k] : 0.0;
     }
     const bool okq = iqb + cq < dq;
     for (int s = p.nd; s < p.nd + p.nt; ++s) {
       const long* st = p.st[s];
-      const double* src = p.in[s] + ia * st[p.ra] + ib * st[p.rb] + (iqb + cq) + (i3b + s0) * st[3];
-      const long step = 8 * st[3];
+      const double* src = p.in[s] + ia * st[p.ra] + ib * st[p.rb] + iqb + i3b * st[3];
+      const long off = cq + s0 * st[3], step = 8 * st[3];
+      double v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const bool okt = okq && i3b + s0 + 8 * k < d3;
+        v[k] = src[okt ? off + k * step : 0];
+        if (!okt) v[k] = 0.0;
+      }
       __syncthreads();  // the previous reader of the tile is done
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
-        tile[s0 + 8 * k][cq] = (okq && i3b + s0 + 8 * k < d3) ? src[k * step] : 0.0;
+      for (int k = 0; k < 8; ++k) tile[s0 + 8 * k][cq] = v[k];
       __syncthreads();
       const double al = p.alpha[s];
 #pragma unroll
       for (int k = 0; k < 8; ++k) acc[k] += al * tile[c3][r0 + 4 * k];
     }
-    double* o = p.out + ia * p.os[p.ra] + ib * p.os[p.rb] + (iqb + r0) * p.os[p.q] + i3b + c3;
-    const long ostep = 4 * p.os[p.q];
+    double* o0 = p.out + ia * p.os[p.ra] + ib * p.os[p.rb] + iqb * p.os[p.q] + i3b;
+    const long ooff = r0 * p.os[p.q] + c3, ostep = 4 * p.os[p.q];
+    double prev[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) prev[k] = (p.beta != 0.0) ? o0[ok[k] ? ooff + k * ostep : 0] : 0.0;
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-      if (ok3 && iqb + r0 + 4 * k < dq) {
+      if (ok[k]) {
         double v = acc[k];
-        if (p.beta != 0.0) v += p.beta * o[k * ostep];
-        o[k * ostep] = v;
+        if (p.beta != 0.0) v += p.beta * prev[k];
+        o0[ooff + k * ostep] = v;
       }
   }
 }
