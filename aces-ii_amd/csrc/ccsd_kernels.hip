@@ -200,17 +200,25 @@ __global__ __launch_bounds__(256) void unpack_ladder_t2_kernel(int o, int v, con
     const long p = c0 + cj;
     int i = 0, j = 0;
     if (p < nop) unpack_le(p, i, j);
+    // branch-free loads (outside the ranges: element 0, dropped), all in
+    // flight together
+    // (Zm has elements only for o > 1 and v > 1)
+    const bool hm = o > 1 && v > 1;
+    double xp[kUpB / 8], xm[kUpB / 8];
 #pragma unroll
     for (int k = 0; k < kUpB / 8; ++k) {
       const int b = b0 + rb + 8 * k;
-      double xp = 0.0, xm = 0.0;
-      if (p < nop && b < v) {
-        const int lo = min(a, b), hi = max(a, b);
-        xp = Zp[pk_le(lo, hi) * ldzp + p];
-        if (i < j && a != b) xm = Zm[pk_lt(lo, hi) * ldzm + pk_lt(i, j)];
-      }
-      zp[rb + 8 * k][cj] = xp;
-      zm[rb + 8 * k][cj] = xm;
+      const bool okp = p < nop && b < v, okm = okp && i < j && a != b;
+      const int lo = min(a, b), hi = max(a, b);
+      xp[k] = Zp[okp ? pk_le(lo, hi) * ldzp + p : 0];
+      xm[k] = hm ? Zm[okm ? pk_lt(lo, hi) * ldzm + pk_lt(i, j) : 0] : 0.0;
+      if (!okp) xp[k] = 0.0;
+      if (!okm) xm[k] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kUpB / 8; ++k) {
+      zp[rb + 8 * k][cj] = xp[k];
+      zm[rb + 8 * k][cj] = xm[k];
     }
   }
   __syncthreads();
@@ -291,14 +299,22 @@ __global__ __launch_bounds__(256) void ring_operands_kernel(int o, int v, const 
   const long vv = (long)v * v, ov = (long)o * v;
   const double* T = t2 + ((long)i * o + j) * vv;  // T[a][b] = t_ijab
   const int tid = threadIdx.x;
+  // Loads are branch-free (an element outside the block reads T[a0][b0],
+  // which exists, and is dropped), so that a thread's loads are in flight
+  // together.
   {
     // t_ijba for a in [a0, a0+32), b in [b0, b0+64): read along a (T[b][a])
     const int ca = tid & 31, rb = tid >> 5;
+    double w[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int b = b0 + rb + 8 * k, a = a0 + ca;
-      tr[rb + 8 * k][ca] = (a < v && b < v) ? T[(long)b * v + a] : 0.0;
+      const bool ok = a < v && b < v;
+      w[k] = T[ok ? (long)b * v + a : (long)b0 * v + a0];
+      if (!ok) w[k] = 0.0;
     }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) tr[rb + 8 * k][ca] = w[k];
   }
   __syncthreads();
   const int cb = tid & 63, ra = tid >> 6;
@@ -307,12 +323,20 @@ __global__ __launch_bounds__(256) void ring_operands_kernel(int o, int v, const 
   const double tib = t1[(long)i * v + b];
   const double tjb = t1[(long)j * v + b];
   const long trow = ((long)i * o + j) * v;  // row of t2 element (i, j, a = 0)
+  double dv[8], tia[8], tja[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int a = min(a0 + ra + 4 * k, v - 1);
+    dv[k] = T[(long)a * v + b];
+    tia[k] = t1[(long)i * v + a];
+    tja[k] = t1[(long)j * v + a];
+  }
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int a = a0 + ra + 4 * k;
     if (a >= v) break;
-    const double d = T[(long)a * v + b];
-    const double tv = d + t1[(long)i * v + a] * tjb;
+    const double d = dv[k];
+    const double tv = d + tia[k] * tjb;
     if (r.tau) r.tau[(trow + a) * v + b] = tv;
 #pragma unroll
     for (int q = 0; q < 3; ++q)
@@ -329,7 +353,7 @@ __global__ __launch_bounds__(256) void ring_operands_kernel(int o, int v, const 
     }
     const double x = tr[cb][ra + 4 * k];
     const double u = 2.0 * d - x;
-    const double p = 2.0 * t1[(long)j * v + a] * tib;
+    const double p = 2.0 * tja[k] * tib;
     const double val[4] = {u, x, u - p, x + p};
     const long row = (long)i * v + a, col = (long)j * v + b;
 #pragma unroll

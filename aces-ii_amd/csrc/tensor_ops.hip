@@ -144,11 +144,22 @@ __global__ __launch_bounds__(256) void convert_t_kernel(int rows, int cols, cons
   const long r0 = (blockIdx.x % ntr) * 64;
   const long c0 = (blockIdx.x / ntr) * 64;
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // branch-free loads (outside the matrix: element (r0, min(c0, cols - 1)),
+  // dropped), all in flight together; tiles wholly in the pad columns read
+  // nothing
+  double v[16];
+  const bool any = c0 < cols;
+  const long fb = (any ? c0 : 0) * ldx + r0;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const long c = c0 + w + 4 * k, r = r0 + l;
-    tile[w + 4 * k][l] = (c < cols && r < rows) ? (x0 ? x[c * ldx + r] - x0[c * ldx + r] : x[c * ldx + r]) : 0.0;
+    const bool ok = c < cols && r < rows;
+    const long i = ok ? c * ldx + r : fb;
+    v[k] = any ? (x0 ? x[i] - x0[i] : x[i]) : 0.0;
+    if (!ok) v[k] = 0.0;
   }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) tile[w + 4 * k][l] = v[k];
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
