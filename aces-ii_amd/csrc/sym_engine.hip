@@ -137,12 +137,24 @@ __global__ __launch_bounds__(256) void pair_avg_tiled_kernel(const dpd::Box* box
   const int i2 = tl.t2 * 32, i3 = tl.t3 * 32;
   const int e2 = min(32, b.d[2] - i2), e3 = min(32, b.d[3] - i3);
   const int lo = threadIdx.x & 31, hi = threadIdx.x >> 5;
-  // tile: rows l2, columns l3 (read along l3); partner: read along l2
+  // tile: rows l2, columns l3 (read along l3); partner: read along l2.
+  // Branch-free loads (outside the tile: the tile's first element, dropped),
+  // all in flight together.
+  const long o1 = base1 + (long)i2 * b.os[2] + (long)i3 * b.os[3];
+  const long o2 = base2 + (long)i2 * b.ss[0][2] + (long)i3 * b.ss[0][3];
+  double xv[4], yv[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int r = hi + 8 * k;
-    if (r < e2 && lo < e3) X[r][lo] = x[base1 + (long)(i2 + r) * b.os[2] + (long)(i3 + lo) * b.os[3]];
-    if (r < e3 && lo < e2) YT[r][lo] = x[base2 + (long)(i2 + lo) * b.ss[0][2] + (long)(i3 + r) * b.ss[0][3]];
+    const bool ok1 = r < e2 && lo < e3, ok2 = r < e3 && lo < e2;
+    xv[k] = x[ok1 ? o1 + (long)r * b.os[2] + (long)lo * b.os[3] : o1];
+    yv[k] = x[ok2 ? o2 + (long)lo * b.ss[0][2] + (long)r * b.ss[0][3] : o2];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int r = hi + 8 * k;
+    if (r < e2 && lo < e3) X[r][lo] = xv[k];
+    if (r < e3 && lo < e2) YT[r][lo] = yv[k];
   }
   __syncthreads();
 #pragma unroll

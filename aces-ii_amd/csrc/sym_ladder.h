@@ -42,6 +42,7 @@ struct UnpackTile {
   int q0;                 // first pair (column offset inside the box's pair block)
 };
 constexpr int kUnpackLds = 2048;  // doubles per LDS image (Zp and Zm parts each)
+constexpr int kUnpackBatch = 4;  // loads of a thread issued together in the tiled unpack
 void unpack_tiles(const Plan& inc2_plan, std::vector<UnpackTile>& tiles);
 // inc2 += unpacked Z (tiles of the inc2 plan: Z rows read along their
 // occupied pairs into LDS, inc2 written along b); Yb = (k,b) rows (walk of the

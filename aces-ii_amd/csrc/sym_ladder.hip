@@ -276,16 +276,24 @@ __global__ __launch_bounds__(256) void unpack_sym_tiled_kernel(const LadderDesc*
   const int glo = gi < gj ? gi : gj;
   const long pij0 = d.OP.sub[G][glo] + q0, mij0 = d.OM.sub[G][glo] + q0;
   const long A = d.V.off[ga] + la;
-  // Z rows of this tile (one per b) read along the box's pairs into LDS
-  for (int idx = threadIdx.x; idx < eb * eq; idx += blockDim.x) {
-    const int row = idx / eq, qq = idx - row * eq;
-    const int lb = lb0 + row;
-    int g0 = ga, l0 = la, g1 = gb, l1 = lb;
-    order2(d.V, g0, l0, g1, l1);
-    const long pab = pair_index(d.VP, d.V, g0, l0, g1, l1);
-    tp[idx] = Zp[d.zp_off[G] + pab * nOP + pij0 + qq];
-    double m = 0.0;
-    if (A != d.V.off[gb] + lb) {
+  // Z rows of this tile (one per b) read along the box's pairs into LDS.
+  // Loads are branch-free in batches of kUnpackBatch (an element outside the
+  // tile or without a Z- term reads element 0 of Zp / Zm and is dropped), so
+  // that a thread's loads are in flight together.
+  const int nrd = eb * eq;
+  const bool zm_any = d.zm_off[d.nirr] > 0;
+  for (int base = threadIdx.x; base < nrd; base += kUnpackBatch * blockDim.x) {
+    double pv[kUnpackBatch], mv[kUnpackBatch];
+#pragma unroll
+    for (int u = 0; u < kUnpackBatch; ++u) {
+      const int idx = base + u * blockDim.x;
+      const bool ok = idx < nrd;
+      const int id = ok ? idx : 0;
+      const int row = id / eq, qq = id - row * eq;
+      const int lb = lb0 + row;
+      int g0 = ga, l0 = la, g1 = gb, l1 = lb;
+      order2(d.V, g0, l0, g1, l1);
+      const long pab = pair_index(d.VP, d.V, g0, l0, g1, l1);
       long mij = -1;
       if (gi != gj) {
         mij = mij0 + qq;
@@ -294,40 +302,70 @@ __global__ __launch_bounds__(256) void unpack_sym_tiled_kernel(const LadderDesc*
         tri_decode(q0 + qq, lp, lq);
         if (lp != lq) mij = d.OM.sub[0][gi] + (long)lq * (lq - 1) / 2 + lp;
       }
-      if (mij >= 0) m = Zm[d.zm_off[G] + pair_index(d.VM, d.V, g0, l0, g1, l1) * nOM + mij];
+      const bool okm = ok && zm_any && A != d.V.off[gb] + lb && mij >= 0;
+      pv[u] = Zp[d.zp_off[G] + pab * nOP + pij0 + qq];
+      mv[u] = zm_any ? Zm[okm ? d.zm_off[G] + pair_index(d.VM, d.V, g0, l0, g1, l1) * nOM + mij : 0] : 0.0;
+      if (!okm) mv[u] = 0.0;
     }
-    tm[idx] = m;
+#pragma unroll
+    for (int u = 0; u < kUnpackBatch; ++u) {
+      const int idx = base + u * blockDim.x;
+      if (idx < nrd) {
+        tp[idx] = pv[u];
+        tm[idx] = mv[u];
+      }
+    }
   }
   __syncthreads();
   // inc2 += Zp + sign Zm, written along b (a pair of the triangle writes twice)
   const int nout = (gi == gj) ? 2 * eq : eq;
   const long o2 = la * bx.os[2];
-  for (int idx = threadIdx.x; idx < nout * eb; idx += blockDim.x) {
-    const int oq = idx / eb, row = idx - oq * eb;
-    int qq, li, lj;
-    if (gi == gj) {
-      qq = oq >> 1;
-      int lp, lq;
-      tri_decode(q0 + qq, lp, lq);
-      if ((oq & 1) && lp == lq) continue;
-      li = (oq & 1) ? lq : lp;
-      lj = (oq & 1) ? lp : lq;
-    } else {
-      qq = oq;
-      const int q = q0 + qq;
-      if (gi < gj) {
-        li = q / oj;
-        lj = q - li * oj;
+  // (the old values of a batch are loaded together, then updated; the
+  // duplicate of a diagonal pair and elements past the tile read element
+  // (0, 0, la, lb0) of the box and write nothing)
+  const int nwr = nout * eb;
+  for (int base = threadIdx.x; base < nwr; base += kUnpackBatch * blockDim.x) {
+    long off[kUnpackBatch];
+    double z[kUnpackBatch], old[kUnpackBatch];
+    bool wr[kUnpackBatch];
+#pragma unroll
+    for (int u = 0; u < kUnpackBatch; ++u) {
+      const int idx = base + u * blockDim.x;
+      const bool ok = idx < nwr;
+      const int id = ok ? idx : 0;
+      const int oq = id / eb, row = id - oq * eb;
+      int qq, li, lj;
+      bool dup = false;
+      if (gi == gj) {
+        qq = oq >> 1;
+        int lp, lq;
+        tri_decode(q0 + qq, lp, lq);
+        dup = (oq & 1) && lp == lq;
+        li = (oq & 1) ? lq : lp;
+        lj = (oq & 1) ? lp : lq;
       } else {
-        lj = q / oi;
-        li = q - lj * oi;
+        qq = oq;
+        const int q = q0 + qq;
+        if (gi < gj) {
+          li = q / oj;
+          lj = q - li * oj;
+        } else {
+          lj = q / oi;
+          li = q - lj * oi;
+        }
       }
+      const int lb = lb0 + row;
+      const long I = d.O.off[gi] + li, J = d.O.off[gj] + lj, B = d.V.off[gb] + lb;
+      double zz = tp[row * eq + qq];
+      if (I != J && A != B) zz += (((I < J) == (A < B)) ? 1.0 : -1.0) * tm[row * eq + qq];
+      wr[u] = ok && !dup;
+      off[u] = wr[u] ? bx.ob + li * bx.os[0] + lj * bx.os[1] + o2 + lb * bx.os[3] : bx.ob + o2 + lb0 * bx.os[3];
+      z[u] = zz;
+      old[u] = out[off[u]];
     }
-    const int lb = lb0 + row;
-    const long I = d.O.off[gi] + li, J = d.O.off[gj] + lj, B = d.V.off[gb] + lb;
-    double z = tp[row * eq + qq];
-    if (I != J && A != B) z += (((I < J) == (A < B)) ? 1.0 : -1.0) * tm[row * eq + qq];
-    out[bx.ob + li * bx.os[0] + lj * bx.os[1] + o2 + lb * bx.os[3]] += z;
+#pragma unroll
+    for (int u = 0; u < kUnpackBatch; ++u)
+      if (wr[u]) out[off[u]] = old[u] + z[u];
   }
 }
 
