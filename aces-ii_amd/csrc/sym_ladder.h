@@ -43,6 +43,7 @@ struct UnpackTile {
 };
 constexpr int kUnpackLds = 2048;  // doubles per LDS image (Zp and Zm parts each)
 constexpr int kUnpackBatch = 4;  // loads of a thread issued together in the tiled unpack
+constexpr int kPackBatch = 4;    // elements of a thread per pass of the ladder-operand packing
 void unpack_tiles(const Plan& inc2_plan, std::vector<UnpackTile>& tiles);
 // inc2 += unpacked Z (tiles of the inc2 plan: Z rows read along their
 // occupied pairs into LDS, inc2 written along b); Yb = (k,b) rows (walk of the

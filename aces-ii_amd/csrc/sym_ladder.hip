@@ -134,23 +134,45 @@ __global__ void pack_tau_sym_kernel(const LadderDesc* __restrict__ dp, const Lay
                                     double* __restrict__ Dp) {
   const LadderDesc& d = *dp;
   const long totS = d.sp_off[d.nirr], tot = totS + d.dp_off[d.nirr];
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < tot; idx += (long)gridDim.x * blockDim.x) {
-    const bool sym = idx < totS;
-    const long e = sym ? idx : idx - totS;
-    const int G = block_of(sym ? d.sp_off : d.dp_off, d.nirr, e);
-    const PairSpace& PO = sym ? d.OP : d.OM;
-    const PairSpace& PV = sym ? d.VP : d.VM;
-    const long nc = PV.n[G];
-    const long loc = e - (sym ? d.sp_off[G] : d.dp_off[G]);
-    const long r = loc / nc, cc = loc - r * nc;
-    int gi, li, gj, lj, gc, lc, gd, ld;
-    pair_decode(PO, d.O, G, r, gi, li, gj, lj);
-    pair_decode(PV, d.V, G, cc, gc, lc, gd, ld);
-    const int g1[4] = {gi, gj, gc, gd}, l1[4] = {li, lj, lc, ld};
-    const int g2[4] = {gj, gi, gc, gd}, l2[4] = {lj, li, lc, ld};
-    const double x = tau[tl->offset(g1, l1)];
-    const double y = tau[tl->offset(g2, l2)];
-    (sym ? Sp : Dp)[e] = sym ? 0.5 * (x + y) : 0.5 * (x - y);
+  // kPackBatch elements per thread and pass, their gathers in flight together
+  // (past the end: element 0, not written)
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i0 = blockIdx.x * (long)blockDim.x + threadIdx.x; i0 < tot; i0 += kPackBatch * stride) {
+    long o1[kPackBatch], o2[kPackBatch];
+#pragma unroll
+    for (int u = 0; u < kPackBatch; ++u) {
+      const long idx0 = i0 + u * stride;
+      const long idx = idx0 < tot ? idx0 : 0;
+      const bool sym = idx < totS;
+      const long e = sym ? idx : idx - totS;
+      const int G = block_of(sym ? d.sp_off : d.dp_off, d.nirr, e);
+      const PairSpace& PO = sym ? d.OP : d.OM;
+      const PairSpace& PV = sym ? d.VP : d.VM;
+      const long nc = PV.n[G];
+      const long loc = e - (sym ? d.sp_off[G] : d.dp_off[G]);
+      const long r = loc / nc, cc = loc - r * nc;
+      int gi, li, gj, lj, gc, lc, gd, ld;
+      pair_decode(PO, d.O, G, r, gi, li, gj, lj);
+      pair_decode(PV, d.V, G, cc, gc, lc, gd, ld);
+      const int g1[4] = {gi, gj, gc, gd}, l1[4] = {li, lj, lc, ld};
+      const int g2[4] = {gj, gi, gc, gd}, l2[4] = {lj, li, lc, ld};
+      o1[u] = tl->offset(g1, l1);
+      o2[u] = tl->offset(g2, l2);
+    }
+    double x[kPackBatch], y[kPackBatch];
+#pragma unroll
+    for (int u = 0; u < kPackBatch; ++u) {
+      x[u] = tau[o1[u]];
+      y[u] = tau[o2[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kPackBatch; ++u) {
+      const long idx = i0 + u * stride;
+      if (idx < tot) {
+        const bool sym = idx < totS;
+        (sym ? Sp : Dp)[sym ? idx : idx - totS] = sym ? 0.5 * (x[u] + y[u]) : 0.5 * (x[u] - y[u]);
+      }
+    }
   }
 }
 
@@ -158,8 +180,9 @@ int pack_tau_sym(const LadderDesc* d_desc, const LadderDesc& h, const Layout* d_
                  double* Dp, hipStream_t s) {
   const long n = h.sp_off[h.nirr] + h.dp_off[h.nirr];
   if (n == 0) return ACES_OK;
-  hipLaunchKernelGGL(pack_tau_sym_kernel, dim3((unsigned)std::min<long>((n + 255) / 256, 1L << 20)), dim3(256), 0,
-                     s, d_desc, d_tauL, tau, Sp, Dp);
+  hipLaunchKernelGGL(pack_tau_sym_kernel,
+                     dim3((unsigned)std::min<long>((n + 256 * kPackBatch - 1) / (256 * kPackBatch), 1L << 20)),
+                     dim3(256), 0, s, d_desc, d_tauL, tau, Sp, Dp);
   ACES_HIP(hipGetLastError());
   return ACES_OK;
 }
