@@ -35,11 +35,20 @@ __global__ __launch_bounds__(256) void mdot_kernel(long n, int m, const double* 
   double acc[kMaxDots];
 #pragma unroll
   for (int k = 0; k < kMaxDots; ++k) acc[k] = 0.0;
+  // vectors in groups of four whose loads are branch-free (a vector past m
+  // reads the last one again and is dropped), so that they are in flight
+  // together
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const double yi = y[i];
 #pragma unroll
-    for (int k = 0; k < kMaxDots; ++k)
-      if (k < m) acc[k] += xs[k][i] * yi;
+    for (int k0 = 0; k0 < kMaxDots; k0 += 4) {
+      if (k0 >= m) break;
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = xs[min(k0 + u, m - 1)][i];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[k0 + u] += (k0 + u < m) ? v[u] * yi : 0.0;
+    }
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
