@@ -194,6 +194,9 @@ constexpr long kChunk = 4096;  // elements per workgroup in box walks
 // thread of its 256 takes at most kTileRows output elements, and the tile's
 // LDS image (pitch tq-extent | 1) fits kTileLds doubles.
 constexpr int kTileSpan = 64, kTileRows = 8, kTileLds = 2304;
+// tiled walk when the largest box spans at least this along the output's
+// fastest index (shorter rows take the element walk)
+constexpr int kTileMinFast = 4;
 // turned sources: LDS-image loads per thread issued together
 constexpr int kReadBatch = 8;
 
@@ -246,7 +249,7 @@ int build_plan(Plan& p, const Layout& out, const std::vector<Src>& srcs, const S
 void free_plan(Plan& p);
 // Host part of build_plan (boxes only, nothing uploaded).
 int build_boxes(Plan& p, const Layout& out, const std::vector<Src>& srcs, const Src* X, const Src* Y);
-// For a rank-4 plan whose largest box spans at least 16 along the output's
+// For a rank-4 plan whose largest box spans at least kTileMinFast along the output's
 // fastest index: the index (0..2) along which source s has unit stride and
 // that a tile can span, 3 when the source reads well in the output's order,
 // -1 when the plan cannot be tiled at all.
@@ -274,7 +277,10 @@ int apply_plan_split(const Plan& p, double* out, double beta, const double* cons
 
 // Per-row sums of every block of a layout: out[r] for the rows in block
 // order (for rank-2 (O|O) this is the orbital order).
-int block_rowsum(const Layout& L, const double* x, double* out, double alpha, double beta, hipStream_t s);
+// ws (ws_elems doubles, optional): partial sums of long rows, split over
+// many workgroups and added in a fixed order.
+int block_rowsum(const Layout& L, const double* x, double* out, double alpha, double beta, hipStream_t s,
+                 double* ws = nullptr, long ws_elems = 0);
 // X -= diag(e) on a rank-2 layout with both indices in the same space.
 int sub_diag_blocks(const Layout& L, double* x, const double* e, double alpha, hipStream_t s);
 // e += alpha * diag(X) (the adjoint of sub_diag_blocks).

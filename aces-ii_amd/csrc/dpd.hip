@@ -126,7 +126,7 @@ int source_fast_index(const Plan& p, int rank, int s) {
       big = &b;
     }
   }
-  if (big->d[3] < 16) return -1;
+  if (big->d[3] < kTileMinFast) return -1;
   const long s3 = big->ss[s][3] < 0 ? -big->ss[s][3] : big->ss[s][3];
   if (s3 < 8) return 3;
   for (int kk = 2; kk >= 0; --kk)
@@ -182,7 +182,7 @@ static void plan_tiles(Plan& p, int rank) {
       big = &b;
     }
   }
-  if (big->d[3] < 16) return;
+  if (big->d[3] < kTileMinFast) return;
   int q = -1, mask = 0;
   for (int s = 0; s < p.nsrc; ++s) {
     const long s3 = big->ss[s][3] < 0 ? -big->ss[s][3] : big->ss[s][3];
@@ -641,10 +641,63 @@ __global__ void block_rowsum_kernel(BlockInfo b, const double* __restrict__ x, d
   }
 }
 
-int block_rowsum(const Layout& L, const double* x, double* out, double alpha, double beta, hipStream_t s) {
+// Long rows: block (row, chunk) sums a contiguous piece of its row into
+// part[row * nchunk + chunk]; the pieces are added in chunk order.
+__global__ __launch_bounds__(256) void block_rowsum_part_kernel(BlockInfo b, const double* __restrict__ x,
+                                                                double* __restrict__ part, int nchunk) {
+  __shared__ double sh[4];
+  const long r = blockIdx.x / nchunk;
+  const int ch = (int)(blockIdx.x - r * nchunk);
+  int g = 0;
+  while (r >= b.rstart[g + 1]) ++g;
+  const long nc = b.ncol[g];
+  const double* row = x + b.blk_off[g] + (r - b.rstart[g]) * nc;
+  const long len = (nc + nchunk - 1) / nchunk, c0 = ch * len, c1 = min(nc, c0 + len);
+  double acc = 0.0;
+  for (long c = c0 + threadIdx.x; c < c1; c += 4 * blockDim.x) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long cc = c + u * (long)blockDim.x;
+      v[u] = row[cc < c1 ? cc : c0];
+      if (cc >= c1) v[u] = 0.0;
+    }
+    acc += (v[0] + v[1]) + (v[2] + v[3]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ void block_rowsum_finish_kernel(long rows, int nchunk, const double* __restrict__ part,
+                                           double* __restrict__ out, double alpha, double beta) {
+  const long r = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  double acc = 0.0;
+  for (int c = 0; c < nchunk; ++c) acc += part[r * nchunk + c];
+  out[r] = alpha * acc + (beta == 0.0 ? 0.0 : beta * out[r]);
+}
+
+int block_rowsum(const Layout& L, const double* x, double* out, double alpha, double beta, hipStream_t s,
+                 double* ws, long ws_elems) {
   BlockInfo b = block_info(L);
   const long rows = b.rstart[b.nirr];
   if (rows == 0) return ACES_OK;
+  long widest = 0;
+  for (int g = 0; g < b.nirr; ++g) widest = std::max(widest, b.ncol[g]);
+  // rows of at least 16384 elements: pieces over about 4096 blocks
+  const long nchunk = std::min<long>(std::max<long>(1, 4096 / rows), std::max<long>(1, widest / 4096));
+  if (ws != nullptr && widest >= 16384 && nchunk > 1 && rows * nchunk <= ws_elems) {
+    hipLaunchKernelGGL(block_rowsum_part_kernel, dim3((unsigned)(rows * nchunk)), dim3(256), 0, s, b, x, ws,
+                       (int)nchunk);
+    ACES_HIP(hipGetLastError());
+    hipLaunchKernelGGL(block_rowsum_finish_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, rows,
+                       (int)nchunk, ws, out, alpha, beta);
+    ACES_HIP(hipGetLastError());
+    return ACES_OK;
+  }
   const unsigned grid = (unsigned)std::min<long>((rows + 3) / 4, 65536);
   hipLaunchKernelGGL(block_rowsum_kernel, dim3(grid), dim3(256), 0, s, b, x, out, alpha, beta);
   ACES_HIP(hipGetLastError());

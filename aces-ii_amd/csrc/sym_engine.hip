@@ -1122,12 +1122,13 @@ int SymEngine::backward(const double* lam1, const double* lam2, bool density, bo
     double* q = scr_;
     for (int k = 0; k < 4; ++k) {
       ACES_CHECK(dpd::apply_plan(plans_[k], q, 0.0, src, one, nullptr, nullptr, 0.0, nullptr, s));
-      ACES_CHECK(dpd::block_rowsum(k < 2 ? aux_o_ : aux_v_, q, k < 2 ? geo_ : gev_, k < 2 ? -1.0 : 1.0, 1.0, s));
+      ACES_CHECK(dpd::block_rowsum(k < 2 ? aux_o_ : aux_v_, q, k < 2 ? geo_ : gev_, k < 2 ? -1.0 : 1.0, 1.0, s,
+                                   ws_, ws_elems_));
     }
     const double* src1[1] = {h + n2};
-    ACES_CHECK(dpd::block_rowsum(T_[t1_].L, h + n2, geo_, -1.0, 1.0, s));
+    ACES_CHECK(dpd::block_rowsum(T_[t1_].L, h + n2, geo_, -1.0, 1.0, s, ws_, ws_elems_));
     ACES_CHECK(dpd::apply_plan(plans_[4], q, 0.0, src1, one, nullptr, nullptr, 0.0, nullptr, s));
-    ACES_CHECK(dpd::block_rowsum(aux_vo_, q, gev_, 1.0, 1.0, s));
+    ACES_CHECK(dpd::block_rowsum(aux_vo_, q, gev_, 1.0, 1.0, s, ws_, ws_elems_));
   }
   for (int i = (int)ops_.size() - 1; i >= 0; --i) {
     Op& op = ops_[i];
