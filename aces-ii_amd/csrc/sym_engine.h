@@ -136,7 +136,7 @@ class SymEngine {
   int backward(const double* lam1, const double* lam2, bool density, bool seeds = true, bool lowp = false);
   // bf16 hi/lo split of tensor t's blocks from data (laid out like t), as
   // stored ([row][col], trans false) or turned round ([col][row])
-  int lowp_split(int t, const double* data, bool trans, LowBuf& lb);
+  int lowp_split(int t, const double* data, bool trans, LowBuf& lb, bool f64 = false);
   // fp64 copy of tensor t's blocks with rows padded to a multiple of 32
   // (zeros): operands of the direct-to-LDS fp64 kernel whatever the block widths
   int copy64(int t, LowBuf& lb);
@@ -208,6 +208,7 @@ class SymEngine {
   double lam_anc_res_ = 0.0;
   std::vector<LowBuf> lowT_, lowg_, lowgT_;
   std::vector<LowBuf> low64_;  // padded fp64 copies (per tensor version)
+  std::vector<LowBuf> low64g_, low64gT_, low64T_;  // the same for the reverse pass's large products
   bool fwd_valid_ = false, lam_ready_ = false;
   // T_new_ holds the increment lists' content after a converged cycle at the
   // current amplitudes (the Lambda start; set on convergence, cleared by any change)

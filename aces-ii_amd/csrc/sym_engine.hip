@@ -709,7 +709,7 @@ int SymEngine::padded_f64(int t) {
   return ACES_OK;
 }
 
-int SymEngine::lowp_split(int t, const double* data, bool trans, LowBuf& lb) {
+int SymEngine::lowp_split(int t, const double* data, bool trans, LowBuf& lb, bool f64) {
   const Tensor& tt = T_[t];
   if (lb.hi == nullptr) {
     long off = 0;
@@ -721,9 +721,37 @@ int SymEngine::lowp_split(int t, const double* data, bool trans, LowBuf& lb) {
     }
     lb.off[nirr] = off;
     double* p;
-    ACES_CHECK(alloc(&p, (off * 4 + 7) / 8 + 8));  // bf16 hi + lo: 4 bytes per element
+    // bf16 hi + lo: 4 bytes per element; fp64 copy: 8
+    ACES_CHECK(alloc(&p, f64 ? off + 8 : (off * 4 + 7) / 8 + 8));
     lb.hi = p;
-    lb.lo = (char*)p + off * 2;
+    lb.lo = f64 ? nullptr : (char*)p + off * 2;
+  }
+  if (f64) {
+    if (!trans) {
+      SplitBlocks sb;
+      sb.f64 = true;
+      for (int g = 0; g < nirr; ++g) {
+        if (tt.nrow[g] == 0 || tt.ncol[g] == 0) continue;
+        const int k = sb.n++;
+        sb.rows[k] = (int)tt.nrow[g];
+        sb.cols[k] = (int)tt.ncol[g];
+        sb.ldy[k] = lb.ld[g];
+        sb.x[k] = data + tt.boff[g];
+        sb.x0[k] = nullptr;
+        sb.hi[k] = (double*)lb.hi + lb.off[g];
+        sb.lo[k] = nullptr;
+      }
+      ACES_CHECK(split_blocks(sb, s_));
+    } else {
+      for (int g = 0; g < nirr; ++g) {
+        if (tt.nrow[g] == 0 || tt.ncol[g] == 0) continue;
+        ACES_CHECK(copy_f64_t((int)tt.ncol[g], (int)tt.nrow[g], data + tt.boff[g], tt.ncol[g],
+                              (double*)lb.hi + lb.off[g], lb.ld[g], s_));
+      }
+    }
+    lb.prec = GP_F64;
+    lb.valid = true;
+    return ACES_OK;
   }
   if (!trans) {
     SplitBlocks sb;
@@ -771,6 +799,7 @@ int SymEngine::ggemm(int C, const double* Cd, int A, const double* Ad, int a_kc,
     e.beta = beta;
     e.a_kc = a_kc;
     e.b_kc = b_kc;
+    if (la != nullptr) e.a_kc = e.b_kc = 1;  // split / padded buffers are k-contiguous (empty blocks too)
     e.ws = ws_;
     e.ws_elems = ws_elems_;
     if (M == 0 || N == 0) {
@@ -1167,6 +1196,33 @@ int SymEngine::backward(const double* lam1, const double* lam2, bool density, bo
             ACES_CHECK(lowp_split(C, to.g, true, lowgT_[C]));
             ACES_CHECK(fixed_t(A));
             ACES_CHECK(ggemm(B, T_[B].g, C, to.g, 0, A, T_[A].d, 0, op.al, 1.0, GP_BF16X3, 4, &lowgT_[C], &lowT_[A]));
+          }
+          break;
+        }
+        if (op.lowp && op.cat <= 2 && op.a_kc && op.b_kc) {
+          // the same large products in fp64 on the direct-to-LDS kernel:
+          // padded copies (the forward operands turned round once per
+          // forward state, the output gradient per pass)
+          low64g_.resize(T_.size());
+          low64gT_.resize(T_.size());
+          low64T_.resize(T_.size());
+          auto fixed64_t = [&](int t) -> int {
+            LowBuf& lb = low64T_[t];
+            if (!lb.valid || lb.ver != T_[t].ver) {
+              ACES_CHECK(lowp_split(t, T_[t].d, true, lb, true));
+              lb.ver = T_[t].ver;
+            }
+            return ACES_OK;
+          };
+          if (T_[A].grad) {
+            ACES_CHECK(lowp_split(C, to.g, false, low64g_[C], true));
+            ACES_CHECK(fixed64_t(B));
+            ACES_CHECK(ggemm(A, T_[A].g, C, to.g, 1, B, T_[B].d, 0, op.al, 1.0, GP_F64, 4, &low64g_[C], &low64T_[B]));
+          }
+          if (T_[B].grad) {
+            ACES_CHECK(lowp_split(C, to.g, true, low64gT_[C], true));
+            ACES_CHECK(fixed64_t(A));
+            ACES_CHECK(ggemm(B, T_[B].g, C, to.g, 0, A, T_[A].d, 0, op.al, 1.0, GP_F64, 4, &low64gT_[C], &low64T_[A]));
           }
           break;
         }

@@ -30,6 +30,8 @@ struct SplitBlocks {
 };
 int split_blocks(const SplitBlocks& b, hipStream_t s);
 int convert_f32_t(int rows, int cols, const double* x, long ldx, float* y, long ldy, hipStream_t s);
+// y[r*ldy + c] = x[c*ldx + r] (r < rows, c < cols), zero for cols <= c < ldy
+int copy_f64_t(int rows, int cols, const double* x, long ldx, double* y, long ldy, hipStream_t s);
 int split_bf16_t(int rows, int cols, const double* x, long ldx, unsigned short* hi, unsigned short* lo, long ldy,
                  hipStream_t s, const double* x0 = nullptr);
 

@@ -135,7 +135,8 @@ int split_bf16(int rows, int cols, const double* x, long ldx, unsigned short* hi
 // Transposing conversions for operands stored k-major: x[c*ldx + r] holds
 // element (r, c); y[r*ldy + c] is written for c < ldy (columns cols..ldy-1
 // zero).  64 x 64 tiles pass through LDS so that both sides are coalesced.
-template <bool SPLIT>
+// MODE 0: fp32 y0; 1: bf16 split y0 (hi) / y1 (lo); 2: fp64 y0.
+template <int MODE>
 __global__ __launch_bounds__(256) void convert_t_kernel(int rows, int cols, const double* __restrict__ x, long ldx,
                                                         void* __restrict__ y0, void* __restrict__ y1, long ldy,
                                                         const double* __restrict__ x0) {
@@ -166,10 +167,12 @@ __global__ __launch_bounds__(256) void convert_t_kernel(int rows, int cols, cons
     const long r = r0 + w + 4 * k, c = c0 + l;
     if (r < rows && c < ldy) {
       const double v = tile[l][w + 4 * k];
-      if (SPLIT) {
+      if (MODE == 1) {
         const unsigned short h = bf16_rn((float)v);
         reinterpret_cast<unsigned short*>(y0)[r * ldy + c] = h;
         reinterpret_cast<unsigned short*>(y1)[r * ldy + c] = bf16_rn((float)(v - (double)bf16_to_f(h)));
+      } else if (MODE == 2) {
+        reinterpret_cast<double*>(y0)[r * ldy + c] = v;
       } else {
         reinterpret_cast<float*>(y0)[r * ldy + c] = (float)v;
       }
@@ -177,29 +180,36 @@ __global__ __launch_bounds__(256) void convert_t_kernel(int rows, int cols, cons
   }
 }
 
-static int convert_t_launch(bool split, int rows, int cols, const double* x, long ldx, void* y0, void* y1, long ldy,
+static int convert_t_launch(int mode, int rows, int cols, const double* x, long ldx, void* y0, void* y1, long ldy,
                             hipStream_t s, const double* x0 = nullptr) {
   ACES_REQUIRE(ldy >= cols && ldx >= rows, "transposing conversion: bad leading dimension");
   if (rows == 0 || ldy == 0) return ACES_OK;
   const long nblk = ((rows + 63) / 64) * ((ldy + 63) / 64);
   ACES_REQUIRE(nblk < (1L << 31), "transposing conversion: too many tiles");
-  if (split)
-    hipLaunchKernelGGL(convert_t_kernel<true>, dim3((unsigned)nblk), dim3(256), 0, s, rows, cols, x, ldx, y0, y1,
-                       ldy, x0);
+  if (mode == 1)
+    hipLaunchKernelGGL(convert_t_kernel<1>, dim3((unsigned)nblk), dim3(256), 0, s, rows, cols, x, ldx, y0, y1, ldy,
+                       x0);
+  else if (mode == 2)
+    hipLaunchKernelGGL(convert_t_kernel<2>, dim3((unsigned)nblk), dim3(256), 0, s, rows, cols, x, ldx, y0, y1, ldy,
+                       x0);
   else
-    hipLaunchKernelGGL(convert_t_kernel<false>, dim3((unsigned)nblk), dim3(256), 0, s, rows, cols, x, ldx, y0, y1,
-                       ldy, x0);
+    hipLaunchKernelGGL(convert_t_kernel<0>, dim3((unsigned)nblk), dim3(256), 0, s, rows, cols, x, ldx, y0, y1, ldy,
+                       x0);
   ACES_HIP(hipGetLastError());
   return ACES_OK;
 }
 
 int convert_f32_t(int rows, int cols, const double* x, long ldx, float* y, long ldy, hipStream_t s) {
-  return convert_t_launch(false, rows, cols, x, ldx, y, nullptr, ldy, s);
+  return convert_t_launch(0, rows, cols, x, ldx, y, nullptr, ldy, s);
+}
+
+int copy_f64_t(int rows, int cols, const double* x, long ldx, double* y, long ldy, hipStream_t s) {
+  return convert_t_launch(2, rows, cols, x, ldx, y, nullptr, ldy, s);
 }
 
 int split_bf16_t(int rows, int cols, const double* x, long ldx, unsigned short* hi, unsigned short* lo, long ldy,
                  hipStream_t s, const double* x0) {
-  return convert_t_launch(true, rows, cols, x, ldx, hi, lo, ldy, s, x0);
+  return convert_t_launch(1, rows, cols, x, ldx, hi, lo, ldy, s, x0);
 }
 
 // ---------------------------------------------------------------------------
